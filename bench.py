@@ -1,0 +1,181 @@
+#!/usr/bin/env python3
+"""Primary-ray throughput of the HIP raymarch on one MI355X.
+
+A step renders every view of the workload once (one plain launch per view,
+primary rays generated on the device, outputs normal + t + leaf index written
+to HBM). Inputs (the octree) are resident in HBM before the timed region.
+
+Workloads:
+  c2      (default) 2048^3 octree built on the GPU from a seeded fBm-displaced
+          icosphere (frequency 316 -> 1,997,120 triangles, displacement 5% of
+          the radius, 8 threads' sub-block partition, 1 GiB budget), 8 seeded
+          orbit cameras at 1920x1080 (16,588,800 rays per step).
+  dragon  the reference's 256^3 dragon octree, same 8 cameras at 1920x1080.
+
+Prints one JSON line (rank 0). The CPU baseline is the oracle (a plain-C
+restatement of the reference raymarch) on the host cores, on the same rays.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+METRIC = "primary Grays/s @2048^3 1080p; batched-vs-plain rays/s + p99 latency, 1 GPU"
+HBM_PEAK_GBS = 8000.0
+
+
+def _pkg():
+    from __graft_entry__ import _import_package
+    return _import_package()
+
+
+def make_workload(pkg, name, cache_dir):
+    if name == "dragon":
+        tree = pkg.VoxelOctree(os.path.join(ROOT, "tests", "golden", "dragon.oct"))
+        return tree, {"workload": "dragon256_8x1080p", "octree": "reference dragon 256^3",
+                      "resolution": 256}
+    if name == "c2":
+        ply = os.path.join(cache_dir, "ico316_seed2048.ply")
+        if not os.path.exists(ply):
+            os.makedirs(cache_dir, exist_ok=True)
+            v, f = pkg.scenes.icosphere(316, seed=2048, displacement=0.05)
+            pkg.scenes.write_ply(ply, v, f)
+        t0 = time.time()
+        tree = pkg.VoxelOctree.build_from_ply(ply, 2048, threads=8, memory_budget=1 << 30)
+        build_s = time.time() - t0
+        return tree, {"workload": "c2_icosphere2048_8x1080p", "octree": "GPU-built 2048^3 displaced icosphere",
+                      "triangles": 1997120, "resolution": 2048, "build_seconds": round(build_s, 3)}
+    raise SystemExit(f"unknown workload {name}")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="c2", choices=["c2", "dragon"])
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--views", type=int, default=8)
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cache", default=os.path.join(ROOT, "gpurun_out", "bench_cache"))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (no multi-GPU path); run with --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    pkg = _pkg()
+    if pkg.svo.device_count() < 1 or not torch.cuda.is_available():
+        raise SystemExit("bench.py: no GPU visible")
+    torch.cuda.set_device(0)
+    tree, config = make_workload(pkg, args.workload, args.cache)
+    tree.upload()
+    W, H, V = args.width, args.height, args.views
+    n = W * H
+    angles = pkg.scenes.orbit_angles(1080, V)
+    cams = [tree.orbit_camera(p, y, 1.0) for p, y in angles]
+
+    dev = torch.device("cuda", 0)
+    normal = [torch.empty(n, dtype=torch.int32, device=dev) for _ in range(V)]
+    tbuf = [torch.empty(n, dtype=torch.float32, device=dev) for _ in range(V)]
+    leaf = [torch.empty(n, dtype=torch.int64, device=dev) for _ in range(V)]
+    stream = torch.cuda.current_stream()
+    sptr = stream.cuda_stream
+
+    def step(events=None):
+        for v in range(V):
+            if events is not None:
+                events[v][0].record(stream)
+            tree.render_primary_device(cams[v], W, H, None, normal[v].data_ptr(), tbuf[v].data_ptr(),
+                                       leaf[v].data_ptr(), sptr)
+            if events is not None:
+                events[v][1].record(stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    ev = [[(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(V)]
+          for _ in range(args.steps)]
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    start.record(stream)
+    for k in range(args.steps):
+        step(ev[k])
+    stop.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    total_ms = start.elapsed_time(stop)
+    launch_ms = [a.elapsed_time(b) for row in ev for a, b in row]
+    avg_launch_s = float(np.mean(launch_ms)) / 1e3
+    rays = n * V * args.steps
+    value = rays / (total_ms / 1e3) / 1e9
+
+    # parity + fetch counts: the oracle on the same rays of every view
+    import oracle_py as orc
+    words = tree.words()
+    cpu_rays = 0
+    cpu_time = 0.0
+    counts = np.zeros(3, np.uint64)
+    mismatches = 0
+    for v in range(V):
+        tf = np.frombuffer(cams[v].tform, np.float32)
+        pos = np.frombuffer(cams[v].pos, np.float32)
+        d = orc.primary_dirs(tf, W, H)
+        c0 = time.perf_counter()
+        ref = orc.raymarch(words, pos[None, :], d, 0.0, threads=args.cpu_threads)
+        cpu_time += time.perf_counter() - c0
+        cpu_rays += n
+        counts += ref["counts"]
+        g_leaf = leaf[v].cpu().numpy().view(np.uint64)
+        g_nrm = normal[v].cpu().numpy().view(np.uint32)
+        g_t = tbuf[v].cpu().numpy().view(np.uint32)
+        mismatches += int(np.count_nonzero(g_leaf != ref["leaf"]) + np.count_nonzero(g_nrm != ref["normal"])
+                          + np.count_nonzero(g_t != ref["t"].view(np.uint32)))
+        if args.no_cpu:
+            break
+    reads_per_ray = float(counts.sum()) / cpu_rays
+    b_ray = 4.0 * reads_per_ray + 16.0
+    achieved = b_ray * n / avg_launch_s / 1e9
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "Grays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(total_ms / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": dict(config, views=V, width=W, height=H, rays_per_step=n * V, kernel="plain (one launch per view)"),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "bytes_per_ray": round(b_ray, 2), "avg_launch_ms": round(avg_launch_s * 1e3, 4)},
+        "cpu_baseline": {"value": round(cpu_rays / cpu_time / 1e9, 6), "unit": "Grays/s", "cores": args.cpu_threads,
+                         "kind": "port", "sample": f"{cpu_rays} primary rays ({cpu_rays // n} of the {V} views, "
+                                                   f"full {W}x{H}) through the oracle raymarch"},
+        "parity": {"rays_checked": cpu_rays, "mismatched_fields": mismatches},
+        "wall_s": round(wall, 3),
+    }
+    print(json.dumps(out), flush=True)
+    if mismatches:
+        sys.exit(3)
+
+
+if __name__ == "__main__":
+    main()

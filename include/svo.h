@@ -1,0 +1,166 @@
+/*
+ * svo.h — C ABI of libsvo_hip.so, the MI355X (gfx950) sparse voxel octree
+ * engine: .oct load/save, GPU raymarch, GPU octree construction.
+ *
+ * The reference (tunabrain/sparse-voxel-octrees) has no FFI; its boundary is
+ * the C++ class VoxelOctree (src/VoxelOctree.hpp:37-58) and its build inputs
+ * VoxelData (src/VoxelData.hpp:101-144) and PlyLoader (src/PlyLoader.hpp:96-113).
+ * Each entry point below names the reference member it replaces. A
+ * header-only C++ class with the reference's member names, forwarding to these
+ * functions, is in include/VoxelOctree.hpp.
+ *
+ * Conventions
+ *  - Every function returns SVO_OK (0) or an SVO_ERR_* code and never exits
+ *    the process; svo_last_error() gives a message for the calling thread.
+ *  - Handles are immutable after load/build, so concurrent raymarch calls on
+ *    one handle from many host threads are safe. Build calls are independent.
+ *  - "_device" entry points take device pointers and a hipStream_t (passed as
+ *    void*, NULL = the null stream) and return after enqueueing; the other
+ *    raymarch/render entry points take host pointers and are synchronous.
+ *  - Raymarch outputs per ray: hit (0 miss, 1 leaf hit, 2 level-of-detail hit),
+ *    normal (the leaf word; 0 unless hit == 1), t (entry distance; 0 on a miss),
+ *    leaf (word index of the leaf; SVO_LEAF_NONE unless hit == 1). Any output
+ *    pointer may be NULL to skip that output.
+ *  - Arithmetic is FP32 in the reference's operation order; hit, normal, t and
+ *    leaf are bit-identical to VoxelOctree::raymarch on the same rays.
+ */
+#ifndef SVO_H
+#define SVO_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SVO_LEAF_NONE UINT64_MAX
+
+enum {
+    SVO_OK = 0,
+    SVO_ERR_INVALID = 1,     /* bad argument */
+    SVO_ERR_IO = 2,          /* file could not be opened, read or written */
+    SVO_ERR_FORMAT = 3,      /* corrupt .oct / .voxel / PLY, or malformed octree */
+    SVO_ERR_NOMEM = 4,       /* host or device allocation failed */
+    SVO_ERR_DEVICE = 5,      /* HIP runtime error (no GPU, launch failure, ...) */
+    SVO_ERR_UNSUPPORTED = 6, /* input outside what this build handles (see message) */
+    SVO_ERR_LIMIT = 7        /* request exceeds a stated limit (budget, size) */
+};
+
+typedef struct svo_octree svo_octree;
+
+/* Message for the last error on this thread ("" if none). */
+const char *svo_last_error(void);
+/* Library version, major*10000 + minor*100 + patch. */
+int svo_version(void);
+/* Number of visible HIP devices (0 when none); never fails. */
+int svo_device_count(void);
+
+/* ---- .oct files --------------------------------------------------------- */
+
+/* VoxelOctree::VoxelOctree(const char *path), src/VoxelOctree.cpp:57-90. The
+ * word array is validated (every child reference in range) before use. */
+int svo_load(const char *path, svo_octree **out);
+/* Same, from an in-memory .oct image. */
+int svo_load_memory(const void *data, uint64_t size, svo_octree **out);
+/* VoxelOctree::save, src/VoxelOctree.cpp:92-123: byte-identical output
+ * (LZ4 1.7.1 stream compression in 64 MiB chunks). */
+int svo_save(const svo_octree *h, const char *path);
+/* The .oct image in memory; free *data with svo_free_buffer. */
+int svo_encode(const svo_octree *h, void **data, uint64_t *size);
+/* The .oct writer of VoxelOctree::save applied to any word array (no tree
+ * validation): header + LZ4 stream slices. Free *data with svo_free_buffer. */
+int svo_encode_words(const uint32_t *words, uint64_t count, const float center[3], void **data,
+                     uint64_t *size);
+void svo_free_buffer(void *p);
+
+/* Wrap an existing word array (copied) with its center; validated. */
+int svo_from_words(const uint32_t *words, uint64_t count, const float center[3], svo_octree **out);
+/* VoxelOctree::center, src/VoxelOctree.hpp:55-57. */
+int svo_center(const svo_octree *h, float out[3]);
+int svo_word_count(const svo_octree *h, uint64_t *count);
+/* Copies min(count, word count) words into dst. */
+int svo_copy_words(const svo_octree *h, uint32_t *dst, uint64_t count);
+/* Octree depth: descriptor levels + 1 (log2 of the side length). */
+int svo_depth(const svo_octree *h, int *depth);
+/* Releases host and device memory of the handle (NULL is a no-op). */
+void svo_free(svo_octree *h);
+
+/* Makes the word array resident on the current HIP device (done implicitly
+ * by the first raymarch); returns its device address. */
+int svo_upload(svo_octree *h, const uint32_t **device_words);
+
+/* ---- raymarch ------------------------------------------------------------ */
+
+/* VoxelOctree::raymarch(o, d, rayScale, normal, t), src/VoxelOctree.cpp:207-346,
+ * one ray, synchronous, on the GPU. *hit gets the hit code; normal/t are
+ * written exactly when the reference writes them (normal on a leaf hit; t on
+ * any hit). */
+int svo_raymarch(svo_octree *h, const float o[3], const float d[3], float ray_scale,
+                 uint32_t *normal, float *t, int *hit);
+
+/* n rays, host arrays (o, d: n x 3 floats). ray_scale may be NULL, then
+ * ray_scale0 applies to every ray. Synchronous. */
+int svo_raymarch_batch(svo_octree *h, int64_t n, const float *o, const float *d,
+                       const float *ray_scale, float ray_scale0, uint8_t *hit, uint32_t *normal,
+                       float *t, uint64_t *leaf);
+/* Same with device arrays, enqueued on `stream`. */
+int svo_raymarch_batch_device(svo_octree *h, int64_t n, const float *o, const float *d,
+                              const float *ray_scale, float ray_scale0, uint8_t *hit,
+                              uint32_t *normal, float *t, uint64_t *leaf, void *stream);
+
+/* ---- cameras and primary rays (src/Main.cpp renderBatch) ------------------ */
+
+typedef struct {
+    float tform[16]; /* INV_MODELVIEW, row-major (Mat4 a11..a44), translation included */
+    float pos[3];    /* ray origin: tform*0 + center + 1 (src/Main.cpp:150-152) */
+} svo_camera;
+
+/* Orbit camera of the viewer (src/Main.cpp:212,244-250; MatrixStack.cpp:65-66):
+ * MODEL = rotXYZ(pitch,0,0)*rotXYZ(0,yaw,0), VIEW = translate(0,0,-radius). */
+int svo_orbit_camera(const svo_octree *h, float pitch, float yaw, float radius, svo_camera *out);
+
+/* Primary rays of a width x height image generated on the device: per pixel
+ * dir = normalize_quake(dx*a11 + dy*a12 + zx, ...) with dx = -1 + x*(2/W),
+ * dy = H/W - y*(2/W) (the per-pixel expression of src/Main.cpp:97-113), origin
+ * cam->pos, rayScale 0. Outputs are W*H arrays (row-major, y down). */
+int svo_render_primary(svo_octree *h, const svo_camera *cam, int width, int height, uint8_t *hit,
+                       uint32_t *normal, float *t, uint64_t *leaf);
+int svo_render_primary_device(svo_octree *h, const svo_camera *cam, int width, int height,
+                              uint8_t *hit, uint32_t *normal, float *t, uint64_t *leaf,
+                              void *stream);
+
+/* ---- construction --------------------------------------------------------- */
+
+typedef struct {
+    /* VoxelData memory budget in bytes; 0 = 1 GiB (src/Main.cpp:269). It fixes
+     * the cache-block size (src/VoxelData.cpp:211-262), which shapes the tree. */
+    uint64_t memory_budget;
+    /* Thread count the reference would have used: it fixes the sub-block
+     * partition (src/PlyLoader.cpp:381-405,418) and therefore the output bytes.
+     * 0 = this host's online processor count, as ThreadUtils::idealThreadCount. */
+    int threads;
+    int reserved[7];
+} svo_build_opts;
+
+/* PlyLoader(path) -> VoxelData(loader, resolution, budget) -> VoxelOctree,
+ * src/Main.cpp:322-327, built on the GPU; .oct bytes identical to the
+ * reference's for the same (resolution, budget, threads). */
+int svo_build_from_ply(const char *path, int resolution, const svo_build_opts *opts,
+                       svo_octree **out);
+/* Same with the PLY already parsed: verts n x {x,y,z,nx,ny,nz,r,g,b} as
+ * PlyLoader::readVertices stores them (absent normals 0, absent colours 255),
+ * faces as CSR (face_counts[nfaces], face_idx flat; fans triangulated). */
+int svo_build_from_mesh(const float *verts, int64_t nverts, int has_normals,
+                        const int32_t *face_counts, const int32_t *face_idx, int64_t nfaces,
+                        int resolution, const svo_build_opts *opts, svo_octree **out);
+/* VoxelData(path, budget) -> VoxelOctree (raw .voxel file), src/Main.cpp:315-321. */
+int svo_build_from_voxels(const char *path, const svo_build_opts *opts, svo_octree **out);
+/* Same with the volume in host memory: w*h*d uint32, x fastest. */
+int svo_build_from_grid(const uint32_t *grid, int w, int h, int d, const svo_build_opts *opts,
+                        svo_octree **out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SVO_H */

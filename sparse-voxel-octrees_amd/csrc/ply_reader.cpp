@@ -1,0 +1,231 @@
+// PLY reader for the mesh builder: the subset of PLY the reference accepts
+// (src/third-party/plyfile.c with PlyLoader's property mapping,
+// src/PlyLoader.cpp:81-224):
+//  - ".ply" is appended when the path lacks it (plyfile.c:798-850);
+//  - ascii and binary_little_endian; old type names only (char short int
+//    uchar ushort uint float double, plyfile.c:41-44);
+//  - vertex properties x y z nx ny nz red green blue are read as float,
+//    absent normals = 0, absent colours = 255; other properties/elements skipped;
+//  - face property vertex_indices (a list) is read as int.
+// Big-endian files are rejected: the reference converts before swapping bytes
+// (PlyLoader.cpp:104-120,152), so it has no meaningful output for them.
+#include <cerrno>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "svo_internal.hpp"
+
+namespace svo {
+namespace {
+
+enum PType { P_INVALID = 0, P_CHAR, P_SHORT, P_INT, P_UCHAR, P_USHORT, P_UINT, P_FLOAT, P_DOUBLE };
+
+PType parse_type(const std::string &s) {
+    static const char *names[] = {"invalid", "char", "short", "int", "uchar", "ushort", "uint", "float", "double"};
+    for (int i = 1; i <= 8; ++i)
+        if (s == names[i]) return PType(i);
+    return P_INVALID;
+}
+
+int type_size(PType t) {
+    static const int sz[] = {0, 1, 2, 4, 1, 2, 4, 4, 8};
+    return sz[t];
+}
+
+struct Prop {
+    std::string name;
+    bool list = false;
+    PType type = P_INVALID, count_type = P_INVALID;
+};
+
+struct Elem {
+    std::string name;
+    int64_t count = 0;
+    std::vector<Prop> props;
+};
+
+// Reads one binary scalar as double (the value plyfile.c's get_binary_item
+// yields before store_item converts it to the internal type).
+bool read_bin(const uint8_t *&p, const uint8_t *end, PType t, double &v) {
+    int n = type_size(t);
+    if (end - p < n) return false;
+    switch (t) {
+        case P_CHAR: { int8_t x; std::memcpy(&x, p, 1); v = x; break; }
+        case P_UCHAR: { uint8_t x; std::memcpy(&x, p, 1); v = x; break; }
+        case P_SHORT: { int16_t x; std::memcpy(&x, p, 2); v = x; break; }
+        case P_USHORT: { uint16_t x; std::memcpy(&x, p, 2); v = x; break; }
+        case P_INT: { int32_t x; std::memcpy(&x, p, 4); v = x; break; }
+        case P_UINT: { uint32_t x; std::memcpy(&x, p, 4); v = x; break; }
+        case P_FLOAT: { float x; std::memcpy(&x, p, 4); v = x; break; }
+        case P_DOUBLE: { double x; std::memcpy(&x, p, 8); v = x; break; }
+        default: return false;
+    }
+    p += n;
+    return true;
+}
+
+bool is_int_type(PType t) { return t != P_FLOAT && t != P_DOUBLE; }
+
+}  // namespace
+
+int read_ply(const char *path_in, std::vector<float> &verts, int &has_normals, std::vector<int32_t> &face_counts,
+             std::vector<int32_t> &face_idx) {
+    std::string path(path_in);
+    if (path.size() < 4 || path.compare(path.size() - 4, 4, ".ply") != 0) path += ".ply";
+    FILE *fp = std::fopen(path.c_str(), "rb");
+    if (!fp) return fail(SVO_ERR_IO, "cannot open PLY " + path);
+    std::vector<uint8_t> buf;
+    {
+        std::fseek(fp, 0, SEEK_END);
+        long n = std::ftell(fp);
+        std::fseek(fp, 0, SEEK_SET);
+        if (n < 0) { std::fclose(fp); return fail(SVO_ERR_IO, "cannot read PLY " + path); }
+        buf.resize(size_t(n));
+        if (n && std::fread(buf.data(), 1, size_t(n), fp) != size_t(n)) { std::fclose(fp); return fail(SVO_ERR_IO, "short read " + path); }
+        std::fclose(fp);
+    }
+    // ---- header ----
+    size_t pos = 0;
+    auto next_line = [&](std::string &line) -> bool {
+        if (pos >= buf.size()) return false;
+        size_t e = pos;
+        while (e < buf.size() && buf[e] != '\n') e++;
+        line.assign(reinterpret_cast<const char *>(buf.data()) + pos, e - pos);
+        if (!line.empty() && line.back() == '\r') line.pop_back();
+        pos = e < buf.size() ? e + 1 : e;
+        return true;
+    };
+    auto words = [](const std::string &s) {
+        std::vector<std::string> w;
+        size_t i = 0;
+        while (i < s.size()) {
+            while (i < s.size() && (s[i] == ' ' || s[i] == '\t')) i++;
+            size_t j = i;
+            while (j < s.size() && s[j] != ' ' && s[j] != '\t') j++;
+            if (j > i) w.push_back(s.substr(i, j - i));
+            i = j;
+        }
+        return w;
+    };
+    std::string line;
+    if (!next_line(line) || words(line) != std::vector<std::string>{"ply"}) return fail(SVO_ERR_FORMAT, "not a PLY file");
+    int format = -1;  // 0 ascii, 1 binary LE
+    std::vector<Elem> elems;
+    bool ended = false;
+    while (next_line(line)) {
+        std::vector<std::string> w = words(line);
+        if (w.empty()) continue;
+        if (w[0] == "format" && w.size() >= 2) {
+            if (w[1] == "ascii") format = 0;
+            else if (w[1] == "binary_little_endian") format = 1;
+            else if (w[1] == "binary_big_endian") return fail(SVO_ERR_UNSUPPORTED, "big-endian PLY is not supported");
+            else return fail(SVO_ERR_FORMAT, "unknown PLY format " + w[1]);
+        } else if (w[0] == "element" && w.size() >= 3) {
+            Elem e;
+            e.name = w[1];
+            e.count = std::strtoll(w[2].c_str(), nullptr, 10);
+            if (e.count < 0) return fail(SVO_ERR_FORMAT, "negative element count");
+            elems.push_back(e);
+        } else if (w[0] == "property") {
+            if (elems.empty()) return fail(SVO_ERR_FORMAT, "property before element");
+            Prop p;
+            if (w.size() >= 5 && w[1] == "list") {
+                p.list = true;
+                p.count_type = parse_type(w[2]);
+                p.type = parse_type(w[3]);
+                p.name = w[4];
+                if (!p.count_type || !p.type) return fail(SVO_ERR_FORMAT, "unknown PLY list type in: " + line);
+            } else if (w.size() >= 3) {
+                p.type = parse_type(w[1]);
+                p.name = w[2];
+                if (!p.type) return fail(SVO_ERR_FORMAT, "unknown PLY property type in: " + line);
+            } else {
+                return fail(SVO_ERR_FORMAT, "bad property line: " + line);
+            }
+            elems.back().props.push_back(p);
+        } else if (w[0] == "end_header") {
+            ended = true;
+            break;
+        }
+    }
+    if (!ended || format < 0) return fail(SVO_ERR_FORMAT, "PLY header incomplete");
+    const Elem *ve = nullptr, *fe = nullptr;
+    for (const Elem &e : elems) {
+        if (e.name == "vertex") ve = &e;
+        else if (e.name == "face") fe = &e;
+    }
+    if (!ve || !fe) return fail(SVO_ERR_FORMAT, "PLY file has to have triangles and vertices");
+    static const char *vnames[9] = {"x", "y", "z", "nx", "ny", "nz", "red", "green", "blue"};
+    bool avail[9] = {false};
+    for (const Prop &p : ve->props)
+        for (int t = 0; t < 9; ++t)
+            if (!p.list && p.name == vnames[t]) avail[t] = true;
+    has_normals = avail[3] && avail[4] && avail[5];
+    bool faceIdx = false;
+    for (const Prop &p : fe->props)
+        if (p.list && p.name == "vertex_indices") faceIdx = true;
+    if (!faceIdx) return fail(SVO_ERR_FORMAT, "No face information found");
+
+    const float defaults[9] = {0, 0, 0, 0, 0, 0, 255.0f, 255.0f, 255.0f};
+    verts.assign(size_t(ve->count) * 9, 0.0f);
+    face_counts.clear();
+    face_idx.clear();
+
+    // ---- body ----
+    const uint8_t *p = buf.data() + pos, *end = buf.data() + buf.size();
+    const char *tp = reinterpret_cast<const char *>(p);
+    const char *tend = reinterpret_cast<const char *>(end);
+    auto next_token = [&](std::string &tok) -> bool {
+        while (tp < tend && (*tp == ' ' || *tp == '\t' || *tp == '\n' || *tp == '\r')) tp++;
+        const char *s = tp;
+        while (tp < tend && !(*tp == ' ' || *tp == '\t' || *tp == '\n' || *tp == '\r')) tp++;
+        if (tp == s) return false;
+        tok.assign(s, size_t(tp - s));
+        return true;
+    };
+    // value of one item as plyfile.c sees it (ascii: atoi for integer types, atof otherwise)
+    auto read_item = [&](PType t, double &v) -> bool {
+        if (format == 1) return read_bin(p, end, t, v);
+        std::string tok;
+        if (!next_token(tok)) return false;
+        v = is_int_type(t) ? double(std::atoi(tok.c_str())) : std::atof(tok.c_str());
+        return true;
+    };
+    for (const Elem &e : elems) {
+        const bool isV = &e == ve, isF = &e == fe;
+        for (int64_t i = 0; i < e.count; ++i) {
+            float row[9];
+            if (isV) std::memcpy(row, defaults, sizeof(row));
+            for (const Prop &pr : e.props) {
+                if (pr.list) {
+                    double cv;
+                    if (!read_item(pr.count_type, cv)) return fail(SVO_ERR_FORMAT, "PLY data truncated");
+                    int64_t cnt = int64_t(cv);
+                    if (cnt < 0) return fail(SVO_ERR_FORMAT, "negative list length");
+                    const bool take = isF && pr.name == "vertex_indices";
+                    if (take) face_counts.push_back(int32_t(cnt));
+                    for (int64_t k = 0; k < cnt; ++k) {
+                        double v;
+                        if (!read_item(pr.type, v)) return fail(SVO_ERR_FORMAT, "PLY data truncated");
+                        if (take) face_idx.push_back(int32_t(v));
+                    }
+                } else {
+                    double v;
+                    if (!read_item(pr.type, v)) return fail(SVO_ERR_FORMAT, "PLY data truncated");
+                    if (isV)
+                        for (int t = 0; t < 9; ++t)
+                            if (pr.name == vnames[t]) row[t] = float(v);
+                }
+            }
+            if (isV) std::memcpy(&verts[size_t(i) * 9], row, sizeof(row));
+        }
+    }
+    for (int32_t v : face_idx)
+        if (v < 0 || int64_t(v) >= ve->count) return fail(SVO_ERR_FORMAT, "face references a missing vertex");
+    return SVO_OK;
+}
+
+}  // namespace svo

@@ -1,0 +1,344 @@
+// Octree raymarch on gfx950.
+//
+// Traversal restates VoxelOctree::raymarch (src/VoxelOctree.cpp:207-346) per
+// lane, in FP32 with contraction off and correctly rounded division, so
+// results are bit-identical to the reference on the same rays. Design:
+//  - one lane per ray; a wave64 owns an 8x8 pixel tile (render path) so its
+//    rays walk the same nodes and its node loads coalesce through L2;
+//  - the restart stack {parent, maxT} per scale lives in LDS, laid out
+//    [slot][lane] so a wave's pushes/pops hit 64 different banks;
+//  - node words are fetched with plain 32-bit global loads (the tree is
+//    read-only, so they stay cached in L2 / MALL across rays);
+//  - word indices are 32-bit when the tree has < 2^32 words, else 64-bit.
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+
+#include "svo_internal.hpp"
+
+namespace svo {
+
+#define HIP_TRY(expr)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return fail(SVO_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+namespace {
+
+constexpr int kMaxScale = 23;  // VoxelOctree::MaxScale, src/VoxelOctree.hpp:38
+
+__device__ __forceinline__ float smin(float a, float b) { return (b < a) ? b : a; }  // std::min
+__device__ __forceinline__ float smax(float a, float b) { return (a < b) ? b : a; }  // std::max
+__device__ __forceinline__ uint32_t f2u(float f) { return __float_as_uint(f); }
+__device__ __forceinline__ float u2f(uint32_t u) { return __uint_as_float(u); }
+
+// invSqrt, src/Util.hpp:47-58
+__device__ __forceinline__ float quake_rsqrt(float x) {
+    float half = x * 0.5f;
+    float y = u2f(0x5f3759dfu - (f2u(x) >> 1));
+    return y * (1.5f - half * y * y);
+}
+
+struct StackRef {
+    void *base;  // LDS: [slots][stride] Idx offsets, then [slots][stride] float maxT
+    int stride;
+    int slots;
+    int lane;
+};
+
+// Returns 0 miss, 1 leaf hit, 2 LOD hit, 3 stack overflow (malformed tree).
+template <typename Idx>
+__device__ __forceinline__ int march(const uint32_t *__restrict__ tree, float ox, float oy, float oz,
+                                     float dx, float dy, float dz, float rayScale, StackRef stk,
+                                     uint32_t &normal, float &tOut, Idx &leaf) {
+    Idx *stkOff = reinterpret_cast<Idx *>(stk.base);
+    float *stkT = reinterpret_cast<float *>(stkOff + stk.slots * stk.stride);
+
+    if (fabsf(dx) < 1e-4f) dx = 1e-4f;
+    if (fabsf(dy) < 1e-4f) dy = 1e-4f;
+    if (fabsf(dz) < 1e-4f) dz = 1e-4f;
+
+    const float dTx = 1.0f / -fabsf(dx);
+    const float dTy = 1.0f / -fabsf(dy);
+    const float dTz = 1.0f / -fabsf(dz);
+    float bTx = dTx * ox;
+    float bTy = dTy * oy;
+    float bTz = dTz * oz;
+
+    int octantMask = 7;
+    if (dx > 0.0f) { octantMask ^= 1; bTx = 3.0f * dTx - bTx; }
+    if (dy > 0.0f) { octantMask ^= 2; bTy = 3.0f * dTy - bTy; }
+    if (dz > 0.0f) { octantMask ^= 4; bTz = 3.0f * dTz - bTz; }
+
+    float minT = smax(2.0f * dTx - bTx, smax(2.0f * dTy - bTy, 2.0f * dTz - bTz));
+    float maxT = smin(dTx - bTx, smin(dTy - bTy, dTz - bTz));
+    minT = smax(minT, 0.0f);
+
+    uint32_t current = 0;
+    Idx parent = 0;
+    int idx = 0;
+    float posX = 1.0f, posY = 1.0f, posZ = 1.0f;
+    int scale = kMaxScale - 1;
+    float scaleExp2 = 0.5f;
+
+    if (1.5f * dTx - bTx > minT) { idx ^= 1; posX = 1.5f; }
+    if (1.5f * dTy - bTy > minT) { idx ^= 2; posY = 1.5f; }
+    if (1.5f * dTz - bTz > minT) { idx ^= 4; posZ = 1.5f; }
+
+    while (scale < kMaxScale) {
+        if (current == 0) current = tree[parent];
+
+        const float cornerTX = posX * dTx - bTx;
+        const float cornerTY = posY * dTy - bTy;
+        const float cornerTZ = posZ * dTz - bTz;
+        const float maxTC = smin(cornerTX, smin(cornerTY, cornerTZ));
+
+        const int childShift = idx ^ octantMask;
+        const uint32_t childMasks = current << childShift;
+
+        if ((childMasks & 0x8000u) && minT <= maxT) {
+            if (maxTC * rayScale >= scaleExp2) {
+                tOut = maxTC;
+                return 2;
+            }
+            const float maxTV = smin(maxT, maxTC);
+            const float half = scaleExp2 * 0.5f;
+            const float centerTX = half * dTx + cornerTX;
+            const float centerTY = half * dTy + cornerTY;
+            const float centerTZ = half * dTz + cornerTZ;
+
+            if (minT <= maxTV) {
+                Idx childOffset = Idx(current >> 18);
+                if (current & 0x20000u)
+                    childOffset = Idx((uint64_t(current >> 18) << 32) | uint64_t(tree[parent + 1]));
+                if (!(childMasks & 0x80u)) {
+                    leaf = childOffset + parent +
+                           Idx(__popc((((childMasks >> (8 + childShift)) << childShift) & 127u)));
+                    normal = tree[leaf];
+                    break;
+                }
+                const int slot = (kMaxScale - 1) - scale;
+                if (slot >= stk.slots) return 3;
+                stkOff[slot * stk.stride + stk.lane] = parent;
+                stkT[slot * stk.stride + stk.lane] = maxT;
+
+                const Idx siblingCount = Idx(__popc(childMasks & 127u));
+                parent += childOffset + siblingCount;
+                if (current & 0x10000u) parent += siblingCount;
+
+                idx = 0;
+                scale--;
+                scaleExp2 = half;
+                if (centerTX > minT) { idx ^= 1; posX += scaleExp2; }
+                if (centerTY > minT) { idx ^= 2; posY += scaleExp2; }
+                if (centerTZ > minT) { idx ^= 4; posZ += scaleExp2; }
+                maxT = maxTV;
+                current = 0;
+                continue;
+            }
+        }
+
+        int stepMask = 0;
+        if (cornerTX <= maxTC) { stepMask ^= 1; posX -= scaleExp2; }
+        if (cornerTY <= maxTC) { stepMask ^= 2; posY -= scaleExp2; }
+        if (cornerTZ <= maxTC) { stepMask ^= 4; posZ -= scaleExp2; }
+
+        minT = maxTC;
+        idx ^= stepMask;
+
+        if ((idx & stepMask) != 0) {
+            uint32_t diff = 0;
+            if (stepMask & 1) diff |= f2u(posX) ^ f2u(posX + scaleExp2);
+            if (stepMask & 2) diff |= f2u(posY) ^ f2u(posY + scaleExp2);
+            if (stepMask & 4) diff |= f2u(posZ) ^ f2u(posZ + scaleExp2);
+            scale = int(f2u(float(int(diff))) >> 23) - 127;
+            scaleExp2 = u2f(uint32_t(scale - kMaxScale + 127) << 23);
+            // the reference reads rayStack[scale] here even when scale >= 23 and
+            // then leaves the loop; that read cannot reach the result
+            if (scale >= kMaxScale) break;
+            const int slot = (kMaxScale - 1) - scale;
+            if (slot >= stk.slots) return 3;
+            parent = stkOff[slot * stk.stride + stk.lane];
+            maxT = stkT[slot * stk.stride + stk.lane];
+
+            const uint32_t shX = f2u(posX) >> scale;
+            const uint32_t shY = f2u(posY) >> scale;
+            const uint32_t shZ = f2u(posZ) >> scale;
+            posX = u2f(shX << scale);
+            posY = u2f(shY << scale);
+            posZ = u2f(shZ << scale);
+            idx = int((shX & 1u) | ((shY & 1u) << 1) | ((shZ & 1u) << 2));
+            current = 0;
+        }
+    }
+
+    if (scale >= kMaxScale) return 0;
+    tOut = minT;
+    return 1;
+}
+
+template <typename Idx>
+__device__ __forceinline__ void store_result(int code, uint32_t normal, float t, Idx leaf, int64_t i,
+                                             uint8_t *hit, uint32_t *nrm, float *tt, uint64_t *lf) {
+    if (code == 3) code = 0;
+    if (hit) hit[i] = uint8_t(code);
+    if (nrm) nrm[i] = code == 1 ? normal : 0u;
+    if (tt) tt[i] = code == 0 ? 0.0f : t;
+    if (lf) lf[i] = code == 1 ? uint64_t(leaf) : SVO_LEAF_NONE;
+}
+
+template <typename Idx>
+__global__ void __launch_bounds__(256) rays_kernel(const uint32_t *__restrict__ tree, int64_t n,
+                                                   const float *__restrict__ o, const float *__restrict__ d,
+                                                   const float *__restrict__ rs, float rs0, int slots,
+                                                   uint8_t *hit, uint32_t *nrm, float *tt, uint64_t *lf) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    StackRef stk{lds, int(blockDim.x), slots, int(threadIdx.x)};
+    uint32_t normal = 0;
+    float t = 0.0f;
+    Idx leaf = 0;
+    const float scale = rs ? rs[i] : rs0;
+    int code = march<Idx>(tree, o[3 * i], o[3 * i + 1], o[3 * i + 2], d[3 * i], d[3 * i + 1], d[3 * i + 2],
+                          scale, stk, normal, t, leaf);
+    store_result<Idx>(code, normal, t, leaf, i, hit, nrm, tt, lf);
+}
+
+struct PrimaryParams {
+    float a11, a12, a21, a22, a31, a32;  // tform rotation columns used by ray generation
+    float zx, zy, zz;                    // planeDist * third column
+    float px, py, pz;                    // origin
+    float aspect, scale;                 // H/W and 2/W
+    int width, height, tilesX;
+};
+
+// 8x8 pixel tile per wave64; lanes raster the tile row by row.
+template <typename Idx>
+__global__ void __launch_bounds__(64) primary_kernel(const uint32_t *__restrict__ tree, PrimaryParams p,
+                                                     int slots, uint8_t *hit, uint32_t *nrm, float *tt,
+                                                     uint64_t *lf) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int tile = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int x = (tile % p.tilesX) * 8 + (lane & 7);
+    const int y = (tile / p.tilesX) * 8 + (lane >> 3);
+    if (x >= p.width || y >= p.height) return;
+    // ray generation: src/Main.cpp:97-113 with closed-form dx, dy
+    const float dy = p.aspect - float(y) * p.scale;
+    const float dx = -1.0f + float(x) * p.scale;
+    const float vx = dx * p.a11 + dy * p.a12 + p.zx;
+    const float vy = dx * p.a21 + dy * p.a22 + p.zy;
+    const float vz = dx * p.a31 + dy * p.a32 + p.zz;
+    const float s = quake_rsqrt(vx * vx + vy * vy + vz * vz);
+    StackRef stk{lds, 64, slots, lane};
+    uint32_t normal = 0;
+    float t = 0.0f;
+    Idx leaf = 0;
+    int code = march<Idx>(tree, p.px, p.py, p.pz, vx * s, vy * s, vz * s, 0.0f, stk, normal, t, leaf);
+    store_result<Idx>(code, normal, t, leaf, int64_t(y) * p.width + x, hit, nrm, tt, lf);
+}
+
+int stack_slots(const svo_octree &h) { return h.depth > 2 ? h.depth - 1 : 1; }
+
+}  // namespace
+
+int device_count() {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int ensure_device(svo_octree &h) {
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (h.dwords && h.device == dev) return SVO_OK;
+    release_device(h);
+    size_t bytes = h.words.size() * sizeof(uint32_t);
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) return fail(SVO_ERR_NOMEM, std::string("hipMalloc octree: ") + hipGetErrorString(e));
+    e = hipMemcpy(p, h.words.data(), bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(p);
+        return fail(SVO_ERR_DEVICE, std::string("upload octree: ") + hipGetErrorString(e));
+    }
+    h.dwords = static_cast<uint32_t *>(p);
+    h.device = dev;
+    return SVO_OK;
+}
+
+void release_device(svo_octree &h) {
+    if (h.dwords) {
+        int cur = 0;
+        (void)hipGetDevice(&cur);
+        if (cur != h.device) (void)hipSetDevice(h.device);
+        (void)hipFree(h.dwords);
+        if (cur != h.device) (void)hipSetDevice(cur);
+    }
+    h.dwords = nullptr;
+    h.device = -1;
+}
+
+int raymarch_device(svo_octree &h, int64_t n, const float *o, const float *d, const float *rs, float rs0,
+                    uint8_t *hit, uint32_t *normal, float *t, uint64_t *leaf, void *stream) {
+    if (n < 0) return fail(SVO_ERR_INVALID, "negative ray count");
+    if (n == 0) return SVO_OK;
+    if (!o || !d) return fail(SVO_ERR_INVALID, "null ray arrays");
+    int r = ensure_device(h);
+    if (r) return r;
+    const int threads = 256;
+    const int slots = stack_slots(h);
+    const bool wide = h.words.size() >= (uint64_t(1) << 32);
+    const size_t lds = size_t(slots) * threads * ((wide ? 8 : 4) + sizeof(float));
+    const int64_t blocks = (n + threads - 1) / threads;
+    if (blocks > 0x7FFFFFFF) return fail(SVO_ERR_LIMIT, "too many rays for one launch");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (!wide)
+        hipLaunchKernelGGL(rays_kernel<uint32_t>, dim3(unsigned(blocks)), dim3(threads), lds, s, h.dwords, n, o, d,
+                           rs, rs0, slots, hit, normal, t, leaf);
+    else
+        hipLaunchKernelGGL(rays_kernel<uint64_t>, dim3(unsigned(blocks)), dim3(threads), lds, s, h.dwords, n, o, d,
+                           rs, rs0, slots, hit, normal, t, leaf);
+    HIP_TRY(hipGetLastError());
+    return SVO_OK;
+}
+
+int render_primary_device(svo_octree &h, const svo_camera &cam, int width, int height, uint8_t *hit,
+                          uint32_t *normal, float *t, uint64_t *leaf, void *stream) {
+    if (width <= 0 || height <= 0 || width > 65536 || height > 65536)
+        return fail(SVO_ERR_INVALID, "image size out of range");
+    int r = ensure_device(h);
+    if (r) return r;
+    PrimaryParams p;
+    const float pi = float(3.14159265358979323846);
+    const float planeDist = 1.0f / std::tan(pi / 6.0f);  // src/Main.cpp:158
+    p.a11 = cam.tform[0]; p.a12 = cam.tform[1];
+    p.a21 = cam.tform[4]; p.a22 = cam.tform[5];
+    p.a31 = cam.tform[8]; p.a32 = cam.tform[9];
+    p.zx = planeDist * cam.tform[2];
+    p.zy = planeDist * cam.tform[6];
+    p.zz = planeDist * cam.tform[10];
+    p.px = cam.pos[0]; p.py = cam.pos[1]; p.pz = cam.pos[2];
+    p.aspect = float(height) / float(width);
+    p.scale = 2.0f / float(width);
+    p.width = width;
+    p.height = height;
+    p.tilesX = (width + 7) / 8;
+    const int tilesY = (height + 7) / 8;
+    const int slots = stack_slots(h);
+    const bool wide = h.words.size() >= (uint64_t(1) << 32);
+    const size_t lds = size_t(slots) * 64 * ((wide ? 8 : 4) + sizeof(float));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    dim3 grid(unsigned(p.tilesX * tilesY));
+    if (!wide)
+        hipLaunchKernelGGL(primary_kernel<uint32_t>, grid, dim3(64), lds, s, h.dwords, p, slots, hit, normal, t, leaf);
+    else
+        hipLaunchKernelGGL(primary_kernel<uint64_t>, grid, dim3(64), lds, s, h.dwords, p, slots, hit, normal, t, leaf);
+    HIP_TRY(hipGetLastError());
+    return SVO_OK;
+}
+
+}  // namespace svo

@@ -1,0 +1,368 @@
+// C ABI of libsvo_hip.so (declared in include/svo.h). Argument checking,
+// handle ownership and error reporting live here; the work is in
+// lz4_oct.cpp (files), raymarch.hip (GPU traversal) and build.hip (GPU
+// construction).
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <new>
+
+#include "svo_internal.hpp"
+
+namespace svo {
+
+static thread_local std::string g_error;
+
+int fail(int code, const std::string &msg) {
+    g_error = msg;
+    return code;
+}
+
+void clear_error() { g_error.clear(); }
+
+static int finish_handle(std::unique_ptr<svo_octree> &h, svo_octree **out) {
+    int depth = octree_validate(h->words.data(), h->words.size());
+    if (depth < 0) return fail(SVO_ERR_FORMAT, "malformed octree: a child reference leaves the word array");
+    h->depth = depth;
+    *out = h.release();
+    return SVO_OK;
+}
+
+static bool read_file(const char *path, std::vector<uint8_t> &buf) {
+    FILE *fp = std::fopen(path, "rb");
+    if (!fp) return false;
+    bool ok = std::fseek(fp, 0, SEEK_END) == 0;
+    long n = ok ? std::ftell(fp) : -1;
+    ok = ok && n >= 0 && std::fseek(fp, 0, SEEK_SET) == 0;
+    if (ok) {
+        buf.resize(size_t(n));
+        ok = n == 0 || std::fread(buf.data(), 1, size_t(n), fp) == size_t(n);
+    }
+    std::fclose(fp);
+    return ok;
+}
+
+}  // namespace svo
+
+using namespace svo;
+
+// ---- host<->device staging for the synchronous entry points ----
+
+namespace {
+
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    int alloc(size_t n) {
+        if (n == 0) return SVO_OK;
+        hipError_t e = hipMalloc(&p, n * sizeof(T));
+        return e == hipSuccess ? SVO_OK : fail(SVO_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+};
+
+template <typename T>
+int to_device(DevBuf<T> &b, const T *src, size_t n) {
+    if (!src) return SVO_OK;
+    int r = b.alloc(n);
+    if (r) return r;
+    hipError_t e = hipMemcpy(b.p, src, n * sizeof(T), hipMemcpyHostToDevice);
+    return e == hipSuccess ? SVO_OK : fail(SVO_ERR_DEVICE, std::string("hipMemcpy: ") + hipGetErrorString(e));
+}
+
+template <typename T>
+int out_buf(DevBuf<T> &b, const T *dst, size_t n) {
+    return dst ? b.alloc(n) : SVO_OK;
+}
+
+template <typename T>
+int from_device(T *dst, const DevBuf<T> &b, size_t n) {
+    if (!dst || n == 0) return SVO_OK;
+    hipError_t e = hipMemcpy(dst, b.p, n * sizeof(T), hipMemcpyDeviceToHost);
+    return e == hipSuccess ? SVO_OK : fail(SVO_ERR_DEVICE, std::string("hipMemcpy: ") + hipGetErrorString(e));
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *svo_last_error(void) { return g_error.c_str(); }
+
+int svo_version(void) { return 100; }
+
+int svo_device_count(void) { return device_count(); }
+
+int svo_load_memory(const void *data, uint64_t size, svo_octree **out) {
+    clear_error();
+    if (!data || !out) return fail(SVO_ERR_INVALID, "null argument");
+    *out = nullptr;
+    std::unique_ptr<svo_octree> h(new (std::nothrow) svo_octree());
+    if (!h) return fail(SVO_ERR_NOMEM, "out of host memory");
+    int r = oct_decode(static_cast<const uint8_t *>(data), size, *h);
+    if (r) return r;
+    return finish_handle(h, out);
+}
+
+int svo_load(const char *path, svo_octree **out) {
+    clear_error();
+    if (!path || !out) return fail(SVO_ERR_INVALID, "null argument");
+    *out = nullptr;
+    std::vector<uint8_t> buf;
+    if (!read_file(path, buf)) return fail(SVO_ERR_IO, std::string("cannot read ") + path);
+    return svo_load_memory(buf.data(), buf.size(), out);
+}
+
+int svo_encode(const svo_octree *h, void **data, uint64_t *size) {
+    clear_error();
+    if (!h || !data || !size) return fail(SVO_ERR_INVALID, "null argument");
+    std::vector<uint8_t> buf;
+    oct_encode(*h, buf);
+    void *p = std::malloc(buf.size());
+    if (!p) return fail(SVO_ERR_NOMEM, "out of host memory");
+    std::memcpy(p, buf.data(), buf.size());
+    *data = p;
+    *size = buf.size();
+    return SVO_OK;
+}
+
+int svo_encode_words(const uint32_t *words, uint64_t count, const float center[3], void **data, uint64_t *size) {
+    clear_error();
+    if (!words || !center || !data || !size || count == 0) return fail(SVO_ERR_INVALID, "null or empty argument");
+    svo_octree tmp;
+    try {
+        tmp.words.assign(words, words + count);
+    } catch (...) {
+        return fail(SVO_ERR_NOMEM, "out of host memory");
+    }
+    std::memcpy(tmp.center, center, sizeof(tmp.center));
+    return svo_encode(&tmp, data, size);
+}
+
+void svo_free_buffer(void *p) { std::free(p); }
+
+int svo_save(const svo_octree *h, const char *path) {
+    clear_error();
+    if (!h || !path) return fail(SVO_ERR_INVALID, "null argument");
+    std::vector<uint8_t> buf;
+    oct_encode(*h, buf);
+    FILE *fp = std::fopen(path, "wb");
+    if (!fp) return fail(SVO_ERR_IO, std::string("cannot write ") + path);
+    bool ok = std::fwrite(buf.data(), 1, buf.size(), fp) == buf.size();
+    ok = (std::fclose(fp) == 0) && ok;
+    return ok ? SVO_OK : fail(SVO_ERR_IO, std::string("short write to ") + path);
+}
+
+int svo_from_words(const uint32_t *words, uint64_t count, const float center[3], svo_octree **out) {
+    clear_error();
+    if (!words || !center || !out || count == 0) return fail(SVO_ERR_INVALID, "null or empty argument");
+    *out = nullptr;
+    std::unique_ptr<svo_octree> h(new (std::nothrow) svo_octree());
+    if (!h) return fail(SVO_ERR_NOMEM, "out of host memory");
+    try {
+        h->words.assign(words, words + count);
+    } catch (...) {
+        return fail(SVO_ERR_NOMEM, "out of host memory");
+    }
+    std::memcpy(h->center, center, sizeof(h->center));
+    return finish_handle(h, out);
+}
+
+int svo_center(const svo_octree *h, float out[3]) {
+    if (!h || !out) return fail(SVO_ERR_INVALID, "null argument");
+    std::memcpy(out, h->center, sizeof(h->center));
+    return SVO_OK;
+}
+
+int svo_word_count(const svo_octree *h, uint64_t *count) {
+    if (!h || !count) return fail(SVO_ERR_INVALID, "null argument");
+    *count = h->words.size();
+    return SVO_OK;
+}
+
+int svo_copy_words(const svo_octree *h, uint32_t *dst, uint64_t count) {
+    if (!h || !dst) return fail(SVO_ERR_INVALID, "null argument");
+    uint64_t n = count < h->words.size() ? count : h->words.size();
+    std::memcpy(dst, h->words.data(), n * sizeof(uint32_t));
+    return SVO_OK;
+}
+
+int svo_depth(const svo_octree *h, int *depth) {
+    if (!h || !depth) return fail(SVO_ERR_INVALID, "null argument");
+    *depth = h->depth;
+    return SVO_OK;
+}
+
+void svo_free(svo_octree *h) {
+    if (!h) return;
+    release_device(*h);
+    delete h;
+}
+
+int svo_upload(svo_octree *h, const uint32_t **device_words) {
+    clear_error();
+    if (!h) return fail(SVO_ERR_INVALID, "null handle");
+    int r = ensure_device(*h);
+    if (r) return r;
+    if (device_words) *device_words = h->dwords;
+    return SVO_OK;
+}
+
+// ---- raymarch ----
+
+int svo_raymarch_batch_device(svo_octree *h, int64_t n, const float *o, const float *d, const float *ray_scale,
+                              float ray_scale0, uint8_t *hit, uint32_t *normal, float *t, uint64_t *leaf,
+                              void *stream) {
+    clear_error();
+    if (!h) return fail(SVO_ERR_INVALID, "null handle");
+    return raymarch_device(*h, n, o, d, ray_scale, ray_scale0, hit, normal, t, leaf, stream);
+}
+
+int svo_raymarch_batch(svo_octree *h, int64_t n, const float *o, const float *d, const float *ray_scale,
+                       float ray_scale0, uint8_t *hit, uint32_t *normal, float *t, uint64_t *leaf) {
+    clear_error();
+    if (!h) return fail(SVO_ERR_INVALID, "null handle");
+    if (n < 0) return fail(SVO_ERR_INVALID, "negative ray count");
+    if (n == 0) return SVO_OK;
+    if (!o || !d) return fail(SVO_ERR_INVALID, "null ray arrays");
+    size_t m = size_t(n);
+    DevBuf<float> dO, dD, dS, dT;
+    DevBuf<uint8_t> dH;
+    DevBuf<uint32_t> dN;
+    DevBuf<uint64_t> dL;
+    int r;
+    if ((r = to_device(dO, o, 3 * m)) || (r = to_device(dD, d, 3 * m)) || (r = to_device(dS, ray_scale, m))) return r;
+    if ((r = out_buf(dH, hit, m)) || (r = out_buf(dN, normal, m)) || (r = out_buf(dT, t, m)) || (r = out_buf(dL, leaf, m)))
+        return r;
+    r = raymarch_device(*h, n, dO.p, dD.p, ray_scale ? dS.p : nullptr, ray_scale0, dH.p, dN.p, dT.p, dL.p, nullptr);
+    if (r) return r;
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) return fail(SVO_ERR_DEVICE, std::string("raymarch: ") + hipGetErrorString(e));
+    if ((r = from_device(hit, dH, m)) || (r = from_device(normal, dN, m)) || (r = from_device(t, dT, m)) ||
+        (r = from_device(leaf, dL, m)))
+        return r;
+    return SVO_OK;
+}
+
+int svo_raymarch(svo_octree *h, const float o[3], const float d[3], float ray_scale, uint32_t *normal, float *t,
+                 int *hit) {
+    clear_error();
+    if (!h || !o || !d) return fail(SVO_ERR_INVALID, "null argument");
+    uint8_t code = 0;
+    uint32_t nrm = 0;
+    float tt = 0.0f;
+    int r = svo_raymarch_batch(h, 1, o, d, nullptr, ray_scale, &code, &nrm, &tt, nullptr);
+    if (r) return r;
+    if (hit) *hit = code;
+    if (code == 1 && normal) *normal = nrm;
+    if (code != 0 && t) *t = tt;
+    return SVO_OK;
+}
+
+int svo_orbit_camera(const svo_octree *h, float pitch, float yaw, float radius, svo_camera *out) {
+    clear_error();
+    if (!h || !out) return fail(SVO_ERR_INVALID, "null argument");
+    orbit_camera(pitch, yaw, radius, h->center, *out);
+    return SVO_OK;
+}
+
+int svo_render_primary_device(svo_octree *h, const svo_camera *cam, int width, int height, uint8_t *hit,
+                              uint32_t *normal, float *t, uint64_t *leaf, void *stream) {
+    clear_error();
+    if (!h || !cam) return fail(SVO_ERR_INVALID, "null argument");
+    return render_primary_device(*h, *cam, width, height, hit, normal, t, leaf, stream);
+}
+
+int svo_render_primary(svo_octree *h, const svo_camera *cam, int width, int height, uint8_t *hit,
+                       uint32_t *normal, float *t, uint64_t *leaf) {
+    clear_error();
+    if (!h || !cam) return fail(SVO_ERR_INVALID, "null argument");
+    if (width <= 0 || height <= 0) return fail(SVO_ERR_INVALID, "image size out of range");
+    size_t m = size_t(width) * size_t(height);
+    DevBuf<uint8_t> dH;
+    DevBuf<uint32_t> dN;
+    DevBuf<float> dT;
+    DevBuf<uint64_t> dL;
+    int r;
+    if ((r = out_buf(dH, hit, m)) || (r = out_buf(dN, normal, m)) || (r = out_buf(dT, t, m)) || (r = out_buf(dL, leaf, m)))
+        return r;
+    r = render_primary_device(*h, *cam, width, height, dH.p, dN.p, dT.p, dL.p, nullptr);
+    if (r) return r;
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) return fail(SVO_ERR_DEVICE, std::string("render: ") + hipGetErrorString(e));
+    if ((r = from_device(hit, dH, m)) || (r = from_device(normal, dN, m)) || (r = from_device(t, dT, m)) ||
+        (r = from_device(leaf, dL, m)))
+        return r;
+    return SVO_OK;
+}
+
+// ---- construction -------------------------------------------------------------
+
+int svo_build_from_mesh(const float *verts, int64_t nverts, int has_normals, const int32_t *face_counts,
+                        const int32_t *face_idx, int64_t nfaces, int resolution, const svo_build_opts *opts,
+                        svo_octree **out) {
+    clear_error();
+    if (!verts || !face_counts || !face_idx || !out || nverts <= 0 || nfaces < 0)
+        return fail(SVO_ERR_INVALID, "null or empty mesh");
+    *out = nullptr;
+    std::unique_ptr<svo_octree> h(new (std::nothrow) svo_octree());
+    if (!h) return fail(SVO_ERR_NOMEM, "out of host memory");
+    int r = build_mesh(verts, nverts, has_normals, face_counts, face_idx, nfaces, resolution, opts, *h);
+    if (r) return r;
+    return finish_handle(h, out);
+}
+
+int svo_build_from_ply(const char *path, int resolution, const svo_build_opts *opts, svo_octree **out) {
+    clear_error();
+    if (!path || !out) return fail(SVO_ERR_INVALID, "null argument");
+    *out = nullptr;
+    std::vector<float> verts;
+    std::vector<int32_t> counts, idx;
+    int has_normals = 0;
+    int r = read_ply(path, verts, has_normals, counts, idx);
+    if (r) return r;
+    return svo_build_from_mesh(verts.data(), int64_t(verts.size() / 9), has_normals, counts.data(), idx.data(),
+                               int64_t(counts.size()), resolution, opts, out);
+}
+
+int svo_build_from_grid(const uint32_t *grid, int w, int h, int d, const svo_build_opts *opts, svo_octree **out) {
+    clear_error();
+    if (!grid || !out || w <= 0 || h <= 0 || d <= 0) return fail(SVO_ERR_INVALID, "null or empty volume");
+    *out = nullptr;
+    std::unique_ptr<svo_octree> t(new (std::nothrow) svo_octree());
+    if (!t) return fail(SVO_ERR_NOMEM, "out of host memory");
+    int r = build_grid(grid, w, h, d, opts, *t);
+    if (r) return r;
+    return finish_handle(t, out);
+}
+
+int svo_build_from_voxels(const char *path, const svo_build_opts *opts, svo_octree **out) {
+    clear_error();
+    if (!path || !out) return fail(SVO_ERR_INVALID, "null argument");
+    *out = nullptr;
+    FILE *fp = std::fopen(path, "rb");
+    if (!fp) return fail(SVO_ERR_IO, std::string("cannot read ") + path);
+    int32_t dims[3];
+    if (std::fread(dims, 4, 3, fp) != 3 || dims[0] <= 0 || dims[1] <= 0 || dims[2] <= 0) {
+        std::fclose(fp);
+        return fail(SVO_ERR_FORMAT, ".voxel: bad header");
+    }
+    size_t n = size_t(dims[0]) * size_t(dims[1]) * size_t(dims[2]);
+    std::vector<uint32_t> vol;
+    try {
+        vol.resize(n);
+    } catch (...) {
+        std::fclose(fp);
+        return fail(SVO_ERR_NOMEM, ".voxel: volume does not fit in host memory");
+    }
+    size_t got = std::fread(vol.data(), 4, n, fp);
+    std::fclose(fp);
+    if (got != n) return fail(SVO_ERR_FORMAT, ".voxel: truncated");
+    return svo_build_from_grid(vol.data(), dims[0], dims[1], dims[2], opts, out);
+}
+
+}  // extern "C"

@@ -1,0 +1,72 @@
+// Internal declarations shared by the host sources of libsvo_hip.so.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "svo.h"
+
+struct svo_octree {
+    std::vector<uint32_t> words;  // Laine-Karras word array (src/VoxelOctree.hpp:40-41)
+    float center[3] = {0, 0, 0};  // VoxelOctree::_center
+    int depth = 0;                // descriptor levels + 1
+    int device = -1;              // device holding `dwords`
+    uint32_t *dwords = nullptr;   // device copy of `words`
+};
+
+namespace svo {
+
+// thread-local error message; returns `code` for convenient `return fail(...)`
+int fail(int code, const std::string &msg);
+void clear_error();
+
+// ---- .oct codec (lz4_oct.cpp) ----
+// Appends the LZ4 block encoding of src[pos, pos+len) to `out`, continuing the
+// stream state of earlier calls over the same contiguous buffer `src`.
+class Lz4StreamEncoder {
+public:
+    Lz4StreamEncoder();
+    void encode(const uint8_t *src, uint64_t pos, uint32_t len, std::vector<uint8_t> &out);
+
+private:
+    uint32_t table_[4096];
+    uint64_t base_;      // absolute position that table entries are relative to
+    uint64_t dictSize_;  // bytes of history usable as the match window start
+    uint64_t consumed_;  // absolute position of the next chunk
+};
+
+// Decodes one LZ4 block into out[pos, pos+outLen); matches may reach back
+// into out[0, pos). Returns false on malformed input.
+bool lz4_decode_block(const uint8_t *src, uint64_t srcLen, uint8_t *out, uint64_t pos,
+                      uint64_t outLen);
+
+int oct_decode(const uint8_t *data, uint64_t size, svo_octree &h);
+void oct_encode(const svo_octree &h, std::vector<uint8_t> &out);
+
+// ---- octree structure (octree_host.cpp) ----
+// Validates every child reference of the tree reachable from the root and
+// returns its depth (descriptor levels + 1), or -1 if malformed.
+int octree_validate(const uint32_t *words, uint64_t n);
+
+// ---- camera (camera.cpp) ----
+void orbit_camera(float pitch, float yaw, float radius, const float center[3], svo_camera &cam);
+
+// ---- device (raymarch.hip) ----
+int ensure_device(svo_octree &h);
+void release_device(svo_octree &h);
+int raymarch_device(svo_octree &h, int64_t n, const float *o, const float *d, const float *rs,
+                    float rs0, uint8_t *hit, uint32_t *normal, float *t, uint64_t *leaf, void *stream);
+int render_primary_device(svo_octree &h, const svo_camera &cam, int width, int height, uint8_t *hit,
+                          uint32_t *normal, float *t, uint64_t *leaf, void *stream);
+int device_count();
+
+// ---- construction (build.hip / build_host.cpp) ----
+int build_mesh(const float *verts, int64_t nverts, int has_normals, const int32_t *face_counts,
+               const int32_t *face_idx, int64_t nfaces, int resolution, const svo_build_opts *opts,
+               svo_octree &out);
+int build_grid(const uint32_t *grid, int w, int h, int d, const svo_build_opts *opts, svo_octree &out);
+int read_ply(const char *path, std::vector<float> &verts, int &has_normals,
+             std::vector<int32_t> &face_counts, std::vector<int32_t> &face_idx);
+
+}  // namespace svo

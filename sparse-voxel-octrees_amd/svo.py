@@ -1,0 +1,276 @@
+"""Python binding of libsvo_hip.so (ctypes over the C ABI in include/svo.h).
+
+Mirrors the reference's VoxelOctree interface (src/VoxelOctree.hpp:37-58):
+``VoxelOctree(path)`` loads an .oct, ``save(path)`` writes one,
+``raymarch(o, d, rayScale)`` traces one ray, ``center()`` returns the stored
+center; builders mirror ``VoxelOctree(VoxelData*)`` fed by a PLY mesh or a raw
+volume. All tracing and construction run in the HIP library on the GPU; this
+module only marshals arguments. There is no CPU fallback: if the library is
+missing, importing raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsvo_hip.so")
+LEAF_NONE = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+OK, ERR_INVALID, ERR_IO, ERR_FORMAT, ERR_NOMEM, ERR_DEVICE, ERR_UNSUPPORTED, ERR_LIMIT = range(8)
+
+# every symbol include/svo.h declares
+EXPORTS = [
+    "svo_last_error", "svo_version", "svo_device_count", "svo_load", "svo_load_memory", "svo_save",
+    "svo_encode", "svo_encode_words", "svo_free_buffer", "svo_from_words", "svo_center", "svo_word_count", "svo_copy_words",
+    "svo_depth", "svo_free", "svo_upload", "svo_raymarch", "svo_raymarch_batch", "svo_raymarch_batch_device",
+    "svo_orbit_camera", "svo_render_primary", "svo_render_primary_device", "svo_build_from_ply",
+    "svo_build_from_mesh", "svo_build_from_voxels", "svo_build_from_grid",
+]
+
+
+class SvoError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"svo error {code}: {msg}")
+        self.code = code
+
+
+class Camera(ctypes.Structure):
+    _fields_ = [("tform", ctypes.c_float * 16), ("pos", ctypes.c_float * 3)]
+
+
+class BuildOpts(ctypes.Structure):
+    _fields_ = [("memory_budget", ctypes.c_uint64), ("threads", ctypes.c_int), ("reserved", ctypes.c_int * 7)]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `make -C sparse-voxel-octrees_amd` "
+                          "or __graft_entry__.build()")
+    L = ctypes.CDLL(LIB_PATH)
+    P, I64, U64, F = ctypes.c_void_p, ctypes.c_int64, ctypes.c_uint64, ctypes.c_float
+    L.svo_last_error.restype = ctypes.c_char_p
+    sig = {
+        "svo_load": [ctypes.c_char_p, P], "svo_load_memory": [P, U64, P], "svo_save": [P, ctypes.c_char_p],
+        "svo_encode": [P, P, P], "svo_encode_words": [P, U64, P, P, P], "svo_free_buffer": [P], "svo_from_words": [P, U64, P, P],
+        "svo_center": [P, P], "svo_word_count": [P, P], "svo_copy_words": [P, P, U64], "svo_depth": [P, P],
+        "svo_free": [P], "svo_upload": [P, P], "svo_raymarch": [P, P, P, F, P, P, P],
+        "svo_raymarch_batch": [P, I64, P, P, P, F, P, P, P, P],
+        "svo_raymarch_batch_device": [P, I64, P, P, P, F, P, P, P, P, P],
+        "svo_orbit_camera": [P, F, F, F, P],
+        "svo_render_primary": [P, P, ctypes.c_int, ctypes.c_int, P, P, P, P],
+        "svo_render_primary_device": [P, P, ctypes.c_int, ctypes.c_int, P, P, P, P, P],
+        "svo_build_from_ply": [ctypes.c_char_p, ctypes.c_int, P, P],
+        "svo_build_from_mesh": [P, I64, ctypes.c_int, P, P, I64, ctypes.c_int, P, P],
+        "svo_build_from_voxels": [ctypes.c_char_p, P, P],
+        "svo_build_from_grid": [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P],
+    }
+    for name, args in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = None if name in ("svo_free", "svo_free_buffer") else ctypes.c_int
+    return L
+
+
+lib = _load()
+
+
+def _check(code):
+    if code != OK:
+        raise SvoError(code, lib.svo_last_error().decode(errors="replace"))
+
+
+def _p(a):
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+def encode_words(words, center):
+    """.oct image of an arbitrary word array (VoxelOctree::save's writer)."""
+    w = np.ascontiguousarray(words, dtype=np.uint32)
+    c = np.ascontiguousarray(center, dtype=np.float32)
+    p = ctypes.c_void_p()
+    n = ctypes.c_uint64()
+    _check(lib.svo_encode_words(_p(w), len(w), _p(c), ctypes.byref(p), ctypes.byref(n)))
+    try:
+        return ctypes.string_at(p, n.value)
+    finally:
+        lib.svo_free_buffer(p)
+
+
+def device_count():
+    return lib.svo_device_count()
+
+
+def _opts(memory_budget=0, threads=0):
+    o = BuildOpts()
+    o.memory_budget = int(memory_budget)
+    o.threads = int(threads)
+    return o
+
+
+class VoxelOctree:
+    """Handle over one octree (host words + lazily uploaded device copy)."""
+
+    def __init__(self, path=None, *, _handle=None):
+        self._h = ctypes.c_void_p()
+        if _handle is not None:
+            self._h = _handle
+        elif path is not None:
+            _check(lib.svo_load(os.fsencode(path), ctypes.byref(self._h)))
+        else:
+            raise ValueError("path required")
+
+    # -- constructors mirroring the reference's build inputs --
+    @classmethod
+    def from_bytes(cls, data):
+        h = ctypes.c_void_p()
+        buf = np.frombuffer(data, dtype=np.uint8)
+        _check(lib.svo_load_memory(_p(buf), len(buf), ctypes.byref(h)))
+        return cls(_handle=h)
+
+    @classmethod
+    def from_words(cls, words, center):
+        h = ctypes.c_void_p()
+        w = np.ascontiguousarray(words, dtype=np.uint32)
+        c = np.ascontiguousarray(center, dtype=np.float32)
+        _check(lib.svo_from_words(_p(w), len(w), _p(c), ctypes.byref(h)))
+        return cls(_handle=h)
+
+    @classmethod
+    def build_from_ply(cls, path, resolution, threads=0, memory_budget=0):
+        h = ctypes.c_void_p()
+        o = _opts(memory_budget, threads)
+        _check(lib.svo_build_from_ply(os.fsencode(path), int(resolution), ctypes.byref(o), ctypes.byref(h)))
+        return cls(_handle=h)
+
+    @classmethod
+    def build_from_mesh(cls, verts9, has_normals, face_counts, face_idx, resolution, threads=0, memory_budget=0):
+        h = ctypes.c_void_p()
+        v = np.ascontiguousarray(verts9, dtype=np.float32)
+        c = np.ascontiguousarray(face_counts, dtype=np.int32)
+        i = np.ascontiguousarray(face_idx, dtype=np.int32)
+        o = _opts(memory_budget, threads)
+        _check(lib.svo_build_from_mesh(_p(v), len(v), int(bool(has_normals)), _p(c), _p(i), len(c), int(resolution),
+                                       ctypes.byref(o), ctypes.byref(h)))
+        return cls(_handle=h)
+
+    @classmethod
+    def build_from_grid(cls, vol, memory_budget=0):
+        """vol: uint32 [d, h, w] (x fastest), as the raw .voxel layout."""
+        h = ctypes.c_void_p()
+        v = np.ascontiguousarray(vol, dtype=np.uint32)
+        d_, h_, w_ = v.shape
+        o = _opts(memory_budget, 0)
+        _check(lib.svo_build_from_grid(_p(v), w_, h_, d_, ctypes.byref(o), ctypes.byref(h)))
+        return cls(_handle=h)
+
+    @classmethod
+    def build_from_voxels(cls, path, memory_budget=0):
+        h = ctypes.c_void_p()
+        o = _opts(memory_budget, 0)
+        _check(lib.svo_build_from_voxels(os.fsencode(path), ctypes.byref(o), ctypes.byref(h)))
+        return cls(_handle=h)
+
+    def __del__(self):
+        self.close()
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib.svo_free(self._h)
+            self._h = ctypes.c_void_p()
+
+    @property
+    def handle(self):
+        return self._h
+
+    # -- reference members --
+    def save(self, path):
+        _check(lib.svo_save(self._h, os.fsencode(path)))
+
+    def encode(self):
+        p = ctypes.c_void_p()
+        n = ctypes.c_uint64()
+        _check(lib.svo_encode(self._h, ctypes.byref(p), ctypes.byref(n)))
+        try:
+            return ctypes.string_at(p, n.value)
+        finally:
+            lib.svo_free_buffer(p)
+
+    def center(self):
+        c = np.zeros(3, np.float32)
+        _check(lib.svo_center(self._h, _p(c)))
+        return c
+
+    def words(self):
+        n = ctypes.c_uint64()
+        _check(lib.svo_word_count(self._h, ctypes.byref(n)))
+        w = np.zeros(n.value, np.uint32)
+        _check(lib.svo_copy_words(self._h, _p(w), n.value))
+        return w
+
+    def depth(self):
+        d = ctypes.c_int()
+        _check(lib.svo_depth(self._h, ctypes.byref(d)))
+        return d.value
+
+    def upload(self):
+        p = ctypes.c_void_p()
+        _check(lib.svo_upload(self._h, ctypes.byref(p)))
+        return p.value
+
+    def raymarch(self, o, d, ray_scale=0.0):
+        """VoxelOctree::raymarch: returns (hit code, normal or None, t or None)."""
+        o = np.ascontiguousarray(o, dtype=np.float32)
+        d = np.ascontiguousarray(d, dtype=np.float32)
+        normal = ctypes.c_uint32(0)
+        t = ctypes.c_float(0.0)
+        hit = ctypes.c_int(0)
+        _check(lib.svo_raymarch(self._h, _p(o), _p(d), float(ray_scale), ctypes.byref(normal), ctypes.byref(t),
+                                ctypes.byref(hit)))
+        return hit.value, (normal.value if hit.value == 1 else None), (t.value if hit.value else None)
+
+    def raymarch_batch(self, o, d, ray_scale=0.0):
+        o = np.ascontiguousarray(o, dtype=np.float32).reshape(-1, 3)
+        d = np.ascontiguousarray(d, dtype=np.float32).reshape(-1, 3)
+        if len(o) == 1 and len(d) > 1:
+            o = np.ascontiguousarray(np.broadcast_to(o, d.shape))
+        n = len(d)
+        rs = None
+        rs0 = 0.0
+        if np.ndim(ray_scale) == 0:
+            rs0 = float(ray_scale)
+        else:
+            rs = np.ascontiguousarray(ray_scale, dtype=np.float32)
+        hit = np.zeros(n, np.uint8)
+        normal = np.zeros(n, np.uint32)
+        t = np.zeros(n, np.float32)
+        leaf = np.zeros(n, np.uint64)
+        _check(lib.svo_raymarch_batch(self._h, n, _p(o), _p(d), _p(rs), rs0, _p(hit), _p(normal), _p(t), _p(leaf)))
+        return {"hit": hit, "normal": normal, "t": t, "leaf": leaf}
+
+    def raymarch_batch_device(self, n, o_ptr, d_ptr, rs_ptr=None, rs0=0.0, hit_ptr=None, normal_ptr=None,
+                              t_ptr=None, leaf_ptr=None, stream=None):
+        _check(lib.svo_raymarch_batch_device(self._h, int(n), o_ptr, d_ptr, rs_ptr, float(rs0), hit_ptr, normal_ptr,
+                                             t_ptr, leaf_ptr, stream))
+
+    def orbit_camera(self, pitch, yaw, radius=1.0):
+        cam = Camera()
+        _check(lib.svo_orbit_camera(self._h, float(pitch), float(yaw), float(radius), ctypes.byref(cam)))
+        return cam
+
+    def render_primary(self, cam, width, height):
+        n = width * height
+        hit = np.zeros(n, np.uint8)
+        normal = np.zeros(n, np.uint32)
+        t = np.zeros(n, np.float32)
+        leaf = np.zeros(n, np.uint64)
+        _check(lib.svo_render_primary(self._h, ctypes.byref(cam), int(width), int(height), _p(hit), _p(normal),
+                                      _p(t), _p(leaf)))
+        return {"hit": hit, "normal": normal, "t": t, "leaf": leaf}
+
+    def render_primary_device(self, cam, width, height, hit_ptr=None, normal_ptr=None, t_ptr=None, leaf_ptr=None,
+                              stream=None):
+        _check(lib.svo_render_primary_device(self._h, ctypes.byref(cam), int(width), int(height), hit_ptr, normal_ptr,
+                                             t_ptr, leaf_ptr, stream))

@@ -1,0 +1,98 @@
+"""CPU tests of the product library's host side: the C ABI loads and exports
+every declared symbol, .oct decode/encode is byte-identical to the reference
+(every reference-written .oct fixture re-encodes to its own bytes; a 2-slice
+stream matches the reference encoder's md5), errors are reported, not fatal,
+and the host camera matches the reference's matrices."""
+import ctypes
+import hashlib
+import json
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle_py as orc
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol(svo):
+    header = open(os.path.join(ROOT, "include", "svo.h")).read()
+    declared = set(re.findall(r"\b(svo_[a-z_0-9]+)\s*\(", header))
+    assert declared == set(svo.svo.EXPORTS)
+    lib = ctypes.CDLL(svo.svo.LIB_PATH)
+    for name in declared:
+        assert hasattr(lib, name), name
+
+
+def test_dragon_reencode_matches_reference(svo, golden):
+    lz = json.load(open(os.path.join(golden, "lz4.json")))
+    t = svo.VoxelOctree(os.path.join(golden, "dragon.oct"))
+    data = t.encode()
+    assert len(data) == lz["dragon_resave"]["size"]
+    assert hashlib.md5(data).hexdigest() == lz["dragon_resave"]["md5"]
+    c, w = orc.read_oct(os.path.join(golden, "dragon.oct"))
+    assert np.array_equal(t.words(), w)
+    assert np.array_equal(t.center(), c)
+    assert t.depth() == 9
+
+
+def test_reference_octs_reencode_to_same_bytes(svo, golden, tmp_path):
+    files = [f for f in os.listdir(golden) if f.startswith("build_") and f.endswith(".oct")]
+    assert len(files) >= 10
+    for f in files:
+        raw = open(os.path.join(golden, f), "rb").read()
+        t = svo.VoxelOctree.from_bytes(raw)
+        assert t.encode() == raw, f
+        out = tmp_path / f
+        t.save(str(out))
+        assert out.read_bytes() == raw
+
+
+def _stream_words():
+    n = 17 * (1 << 20) + 12345
+    i = np.arange(n, dtype=np.uint64)
+    w = (i * np.uint64(2654435761)) >> np.uint64(7)
+    w = np.where((i % np.uint64(5)) == 0, w, i // np.uint64(3))
+    w = np.where((i % np.uint64(97)) < 40, np.uint64(0), w)
+    return (w & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+
+
+def test_two_slice_stream_matches_reference_encoder(svo, golden):
+    """A word array spanning two 64 MiB slices: the second slice's matches
+    reach into the first and the encoder state carries across slices."""
+    lz = json.load(open(os.path.join(golden, "lz4.json")))["two_chunk_stream"]
+    words = _stream_words()
+    assert len(words) == lz["words"]
+    enc = svo.svo.encode_words(words, (0.5, 0.25, 0.125))
+    assert len(enc) == lz["size"]
+    assert hashlib.md5(enc).hexdigest() == lz["md5"]
+    c, w = orc.decode_oct_bytes(enc)
+    assert np.array_equal(w, words)
+
+
+def test_load_errors_are_reported(svo, golden, tmp_path):
+    with pytest.raises(svo.SvoError) as e:
+        svo.VoxelOctree(str(tmp_path / "missing.oct"))
+    assert e.value.code == svo.svo.ERR_IO
+    raw = open(os.path.join(golden, "dragon.oct"), "rb").read()
+    for bad in (raw[:10], raw[:30], raw[:-100], raw[:24] + bytes(len(raw) - 24)):
+        with pytest.raises(svo.SvoError) as e:
+            svo.VoxelOctree.from_bytes(bad)
+        assert e.value.code == svo.svo.ERR_FORMAT
+    # a child reference past the end of the array is rejected before any GPU use
+    w = np.array([(0xFF << 8) | (0xFF) | (1 << 18)] + [(100 << 18) | (1 << 8) | 1] * 8, np.uint32)
+    with pytest.raises(svo.SvoError) as e:
+        svo.VoxelOctree.from_words(w, [0.5, 0.5, 0.5])
+    assert e.value.code == svo.svo.ERR_FORMAT
+
+
+def test_host_camera_matches_reference(svo, golden):
+    g = np.load(os.path.join(golden, "camera.npz"))
+    for inp, tf in zip(g["inputs"], g["tform"]):
+        t = svo.VoxelOctree.from_words(np.array([1 << 18], np.uint32), inp[3:6])
+        cam = t.orbit_camera(inp[0], inp[1], inp[2])
+        assert np.array_equal(np.frombuffer(cam.tform, np.float32).view(np.uint32), tf.view(np.uint32))
+        _, pos = orc.orbit_camera(inp[0], inp[1], inp[2], inp[3:6])
+        assert np.array_equal(np.frombuffer(cam.pos, np.float32).view(np.uint32), pos.view(np.uint32))
