@@ -1,15 +1,1249 @@
-// GPU octree construction (placeholder until the builder kernels land).
+// GPU octree construction.
+//
+// Produces the word array the reference builder writes (VoxelOctree(VoxelData*),
+// src/VoxelOctree.cpp:125-265, fed by PlyLoader/VoxelData), for the same
+// (resolution, memory budget, thread count), without its recursion:
+//
+//  mesh source (PlyLoader, src/PlyLoader.cpp)
+//   1. rescale vertices to the unit cube; fan-triangulate; face normals
+//   2. bin triangles into sub-block cells (iterateOverlappingBlocks: SAT of
+//      the triangle against each +1-shifted cell box)          -> (cell, tri)
+//   3. per (cell, tri): voxel rows of the triangle's grid box clipped to the
+//      cell; SAT per voxel with the incrementally stepped centres of
+//      triangleToVolume                                          -> keys
+//      key = morton(x,y,z) << tribits | tri
+//   4. radix sort the keys: each voxel's triangles end up in ascending order
+//   5. per voxel, fold writeTriangleCell over its triangles in that order
+//      (re-quantised running average)                           -> leaf words
+//  file source (VoxelData(path)): non-zero voxels -> morton keys -> sort
+//
+//  both: the reference's occupancy-table behaviour (VoxelData.cpp:87-209):
+//   - voxels on the unscanned last slice of an odd-depth cache block are
+//     never emitted (buildLowLut covers 2*floor(d/2) slices);
+//   - above the cache-block level occupancy comes from the top table, so a
+//     cache block counted non-empty but holding no voxel still becomes an
+//     (empty) node.
+//  then levels bottom-up (parent = key >> 3, child masks), subtree sizes with
+//  the far-pointer rule, offsets top-down, and a scatter of every word into
+//  the depth-first layout (see size_kernel / place_kernel).
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <unistd.h>
+#include <vector>
+
+#include "build_common.hpp"
 #include "svo_internal.hpp"
 
 namespace svo {
 
-int build_mesh(const float *, int64_t, int, const int32_t *, const int32_t *, int64_t, int, const svo_build_opts *,
-               svo_octree &) {
-    return fail(SVO_ERR_UNSUPPORTED, "GPU mesh builder not available in this build");
+#define HIP_TRY(expr)                                                                       \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(SVO_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+namespace {
+
+using namespace dev;
+
+constexpr int kThreads = 256;
+
+inline unsigned grid_for(uint64_t n) { return unsigned((n + kThreads - 1) / kThreads); }
+
+// Every device allocation of one build; freed together. Frees wait for the
+// build stream first, so no kernel still in flight can touch freed memory.
+struct Arena {
+    std::vector<void *> ptrs;
+    hipStream_t stream = nullptr;
+    ~Arena() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (void *p : ptrs)
+            if (p) (void)hipFree(p);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    template <typename T>
+    int alloc(T *&out, uint64_t n) {
+        out = nullptr;
+        void *p = nullptr;
+        hipError_t e = hipMalloc(&p, std::max<uint64_t>(n, 1) * sizeof(T));
+        if (e != hipSuccess) return fail(SVO_ERR_NOMEM, std::string("builder: hipMalloc: ") + hipGetErrorString(e));
+        ptrs.push_back(p);
+        out = static_cast<T *>(p);
+        return SVO_OK;
+    }
+    void release(void *p) {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto &q : ptrs)
+            if (q == p) {
+                (void)hipFree(q);
+                q = nullptr;
+            }
+    }
+};
+
+// Parameters the reference derives on the host (VoxelData::init,
+// PlyLoader::setupBlockProcessing).
+struct Geometry {
+    int dataW, dataH, dataD;  // VoxelData::_dataW/H/D
+    int side;                 // sideLength(): largest power-of-two virtual dim
+    int depth;                // log2(side)
+    int cacheLog;             // _lowLutLevels: cache block edge = 1 << cacheLog
+    int cache;
+    // mesh source only
+    int scale;               // PlyLoader::_sideLength - 2 = resolution - 4
+    int sbW, sbH, sbD;       // sub-block dims
+    int pW, pH, pD;          // partitions per cache block
+    int gW, gH, gD;          // flat cell grid
+    int blocksX, blocksY, blocksZ;  // cache blocks per virtual axis
+};
+
+int round_pow2(int x) {
+    int y = 1;
+    while (y < x) y *= 2;
+    return y;
 }
 
-int build_grid(const uint32_t *, int, int, int, const svo_build_opts *, svo_octree &) {
-    return fail(SVO_ERR_UNSUPPORTED, "GPU grid builder not available in this build");
+int log2i(int v) {
+    int r = 0;
+    while (v >>= 1) r++;
+    return r;
+}
+
+uint64_t grid_cells(int levels) {
+    uint64_t r = 0, s = 1;
+    for (int i = 0; i < levels; ++i) {
+        r += s;
+        s *= 8;
+    }
+    return r;
+}
+
+// VoxelData::init, src/VoxelData.cpp:211-262
+int choose_cache(Geometry &g, uint64_t mem, bool mesh) {
+    int vW = round_pow2(g.dataW), vH = round_pow2(g.dataH), vD = round_pow2(g.dataD);
+    g.side = std::max(vW, std::max(vH, vD));
+    g.depth = log2i(uint32_t(g.side));
+    uint64_t cellCost = 4 + (mesh ? 1 : 0);
+    int best = -1;
+    uint64_t smallest = UINT64_MAX;
+    for (int i = g.depth; i >= 0; --i) {
+        int lowBit = g.depth - i;
+        uint64_t cost = grid_cells(i + 1) + grid_cells(lowBit) + (uint64_t(1) << uint64_t(lowBit * 3)) * cellCost;
+        if (cost < mem) best = g.depth - i;
+        smallest = std::min(smallest, cost);
+    }
+    if (best < 0)
+        return fail(SVO_ERR_LIMIT, "memory budget below the smallest cache-block partition (" +
+                                       std::to_string(smallest) + " bytes)");
+    g.cacheLog = best;
+    g.cache = 1 << best;
+    g.blocksX = std::max(1, vW / g.cache);
+    g.blocksY = std::max(1, vH / g.cache);
+    g.blocksZ = std::max(1, vD / g.cache);
+    return SVO_OK;
+}
+
+// findBestBlockPartition, src/PlyLoader.cpp:381-405
+void best_partition(int &w, int &h, int &d, int threads) {
+    int used = 1;
+    while (used < threads) {
+        int *mx, *md, *mn;
+        if (w > h && w > d) mx = &w; else if (h > d) mx = &h; else mx = &d;
+        int big = std::max(w, std::max(h, d)), small = std::min(w, std::min(h, d));
+        if (w != small && w != big) md = &w; else if (h != small && h != big) md = &h; else md = &d;
+        if (w < h && w < d) mn = &w; else if (h < d) mn = &h; else mn = &d;
+        if ((*mx % 2) == 0) *mx /= 2;
+        else if ((*md % 2) == 0) *md /= 2;
+        else if ((*mn % 2) == 0) *mn /= 2;
+        else break;
+        used *= 2;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// mesh kernels
+// ---------------------------------------------------------------------------
+
+__global__ void rescale_kernel(float *v9, int64_t nv, float lx, float ly, float lz, float factor) {
+    int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= nv) return;
+    float *p = v9 + 9 * i;
+    p[0] = (p[0] - lx) * factor;
+    p[1] = (p[1] - ly) * factor;
+    p[2] = (p[2] - lz) * factor;
+}
+
+// PlyLoader::readTriangles (src/PlyLoader.cpp:183-224) + Triangle ctor (:40-53):
+// fan (e0, e[k-1], e[k]); face normal (v1-v0)x(v2-v0) normalised when the PLY
+// has no vertex normals. Triangle record: 9 pos, 9 normal, 9 colour floats.
+__global__ void triangles_kernel(const float *__restrict__ v9, const int32_t *__restrict__ fcount,
+                                 const int64_t *__restrict__ fstart, const int64_t *__restrict__ tstart,
+                                 const int32_t *__restrict__ fidx, int64_t nfaces, int hasNormals,
+                                 float *__restrict__ pos, float *__restrict__ nrm, float *__restrict__ col) {
+    int64_t f = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (f >= nfaces) return;
+    const int c = fcount[f];
+    if (c < 3) return;
+    const int32_t *e = fidx + fstart[f];
+    int64_t t = tstart[f];
+    const int a = e[0];
+    int b = e[1];
+    for (int k = 2; k < c; ++k, ++t) {
+        const int cc = e[k];
+        const int vv[3] = {a, b, cc};
+        for (int q = 0; q < 3; ++q)
+            for (int s = 0; s < 3; ++s) {
+                pos[9 * t + 3 * q + s] = v9[9 * int64_t(vv[q]) + s];
+                nrm[9 * t + 3 * q + s] = v9[9 * int64_t(vv[q]) + 3 + s];
+                col[9 * t + 3 * q + s] = v9[9 * int64_t(vv[q]) + 6 + s];
+            }
+        if (!hasNormals) {
+            float e1[3], e2[3], cr[3], n[3];
+            for (int s = 0; s < 3; ++s) {
+                e1[s] = v9[9 * int64_t(b) + s] - v9[9 * int64_t(a) + s];
+                e2[s] = v9[9 * int64_t(cc) + s] - v9[9 * int64_t(a) + s];
+            }
+            cross3(e1, e2, cr);
+            normalize3(cr, n);
+            for (int q = 0; q < 3; ++q)
+                for (int s = 0; s < 3; ++s) nrm[9 * t + 3 * q + s] = n[s];
+        }
+        b = cc;
+    }
+}
+
+struct MeshParams {
+    int scale;  // resolution - 4
+    int sb[3];
+    int g[3];
+    int p[3];
+    int data[3];
+    int cache;
+    float hx;     // 1.0f / scale (voxel edge in the unit cube)
+    float hb[3];  // sub-block edge per axis in the unit cube
+};
+
+// PlyLoader::pointToGrid, src/PlyLoader.cpp:226-231 over the triangle's AABB
+// (Triangle ctor, PlyLoader.cpp:40-53)
+__device__ __forceinline__ void tri_grid_box(const float *tv, float scale, int lo[3], int hi[3]) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float l = fmin_ref(tv[k], fmin_ref(tv[3 + k], tv[6 + k]));
+        const float u = fmax_ref(tv[k], fmax_ref(tv[3 + k], tv[6 + k]));
+        lo[k] = cvt_i32(l * scale + 1.0f);
+        hi[k] = cvt_i32(u * scale + 1.0f);
+    }
+}
+
+__device__ __forceinline__ void load_tri(const float *__restrict__ pos, int64_t t, float tv[9]) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) tv[k] = pos[9 * t + k];
+}
+
+// PlyLoader::iterateOverlappingBlocks, src/PlyLoader.cpp:233-270. Count pass
+// (out == nullptr) writes the number of cells per triangle; fill pass writes
+// the flat cell indices at the triangle's offset. Flags bad[0] if a cell index
+// falls outside the cell table (the reference would write out of bounds).
+__global__ void bin_kernel(const float *__restrict__ pos, int64_t T, MeshParams mp, uint32_t *__restrict__ count,
+                           const uint64_t *__restrict__ offset, uint32_t *__restrict__ cells,
+                           uint32_t *__restrict__ tris, uint32_t *__restrict__ bad) {
+    int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= T) return;
+    float tv[9];
+    load_tri(pos, t, tv);
+    int l[3], u[3];
+    tri_grid_box(tv, float(mp.scale), l, u);
+    int lg[3], ug[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        lg[k] = l[k] / mp.sb[k];
+        ug[k] = (u[k] + 1) / mp.sb[k];
+    }
+    const uint64_t ncells = uint64_t(mp.g[0]) * mp.g[1] * mp.g[2];
+    uint64_t at = cells ? offset[t] : 0;
+    uint32_t n = 0;
+    auto emit = [&](int64_t x, int64_t y, int64_t z) {
+        int64_t f = x + int64_t(mp.g[0]) * (y + int64_t(mp.g[1]) * z);
+        if (f < 0 || uint64_t(f) >= ncells) {
+            atomicOr(bad, 1u);
+            return;
+        }
+        if (cells) {
+            cells[at + n] = uint32_t(f);
+            tris[at + n] = uint32_t(t);
+        }
+        n++;
+    };
+    const int maxSide = max(ug[0] - lg[0], max(ug[1] - lg[1], ug[2] - lg[2]));
+    if (maxSide > 0) {
+        const float half[3] = {0.5f * mp.hb[0], 0.5f * mp.hb[1], 0.5f * mp.hb[2]};
+        float ctr[3];
+        ctr[2] = (float(lg[2]) + 0.5f) * mp.hb[2];
+        for (int z = lg[2]; z <= ug[2]; ++z, ctr[2] += mp.hb[2]) {
+            ctr[1] = (float(lg[1]) + 0.5f) * mp.hb[1];
+            for (int y = lg[1]; y <= ug[1]; ++y, ctr[1] += mp.hb[1]) {
+                ctr[0] = (float(lg[0]) + 0.5f) * mp.hb[0];
+                for (int x = lg[0]; x <= ug[0]; ++x, ctr[0] += mp.hb[0])
+                    if (tri_box(ctr, half, tv)) emit(x, y, z);
+            }
+        }
+    } else {
+        emit(lg[0], lg[1], lg[2]);
+    }
+    if (!cells) count[t] = n;
+}
+
+// Region of voxels that cell f covers in processBlock (src/PlyLoader.cpp:439-474,
+// clip in triangleToVolume :343-352): [cell origin, min(cell end, data end)),
+// only if the cell's cache block starts inside the volume.
+__device__ __forceinline__ bool cell_region(uint32_t f, const MeshParams &mp, int lo[3], int hi[3]) {
+    const int gx = int(f % uint32_t(mp.g[0]));
+    const int gy = int((f / uint32_t(mp.g[0])) % uint32_t(mp.g[1]));
+    const int gz = int(f / (uint32_t(mp.g[0]) * uint32_t(mp.g[1])));
+    const int gg[3] = {gx, gy, gz};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int blockOrigin = (gg[k] / mp.p[k]) * mp.cache;
+        if (blockOrigin >= mp.data[k]) return false;
+        lo[k] = gg[k] * mp.sb[k];
+        hi[k] = min((gg[k] + 1) * mp.sb[k], mp.data[k]) - 1;
+    }
+    return true;
+}
+
+// Clipped voxel box of (cell, tri) pair i; false if empty.
+__device__ __forceinline__ bool pair_box(uint32_t f, const float tv[9], const MeshParams &mp, int l[3], int u[3]) {
+    int cl[3], cu[3];
+    if (!cell_region(f, mp, cl, cu)) return false;
+    tri_grid_box(tv, float(mp.scale), l, u);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        l[k] = max(l[k], cl[k]);
+        u[k] = min(u[k], cu[k]);
+        if (l[k] > u[k]) return false;
+    }
+    return true;
+}
+
+__global__ void pair_rows_kernel(const uint32_t *__restrict__ cells, const uint32_t *__restrict__ tris, uint64_t P,
+                                 const float *__restrict__ pos, MeshParams mp, uint32_t *__restrict__ rows) {
+    uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= P) return;
+    float tv[9];
+    load_tri(pos, tris[i], tv);
+    int l[3], u[3];
+    rows[i] = pair_box(cells[i], tv, mp, l, u) ? uint32_t((u[1] - l[1] + 1) * (u[2] - l[2] + 1)) : 0u;
+}
+
+// Cache blocks with a non-empty cell list (PlyLoader::isBlockEmpty,
+// src/PlyLoader.cpp:476-493), indexed by the block's Morton key.
+__global__ void block_flags_kernel(const uint32_t *__restrict__ cells, uint64_t P, MeshParams mp,
+                                   uint32_t *__restrict__ flags) {
+    uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= P) return;
+    const uint32_t f = cells[i];
+    const uint32_t gx = f % uint32_t(mp.g[0]);
+    const uint32_t gy = (f / uint32_t(mp.g[0])) % uint32_t(mp.g[1]);
+    const uint32_t gz = f / (uint32_t(mp.g[0]) * uint32_t(mp.g[1]));
+    const uint32_t b[3] = {gx / uint32_t(mp.p[0]), gy / uint32_t(mp.p[1]), gz / uint32_t(mp.p[2])};
+    for (int k = 0; k < 3; ++k)
+        if (int64_t(b[k]) * mp.cache >= mp.data[k]) return;
+    flags[morton3(b[0], b[1], b[2])] = 1u;
+}
+
+// Row r of the pair found by binary search over the exclusive row offsets.
+__device__ __forceinline__ uint64_t find_pair(const uint64_t *__restrict__ rowStart, uint64_t P, uint64_t r) {
+    uint64_t lo = 0, hi = P;  // rowStart[lo] <= r < rowStart[hi]
+    while (hi - lo > 1) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (rowStart[mid] <= r) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// The voxel loop of PlyLoader::triangleToVolume (src/PlyLoader.cpp:353-373)
+// for one (y, z) row: centres start at (l - 0.5)*hx and step by +hx, exactly as
+// the reference accumulates them. Count pass (keys == nullptr) or emit pass.
+__global__ void row_voxels_kernel(const uint32_t *__restrict__ cells, const uint32_t *__restrict__ tris, uint64_t P,
+                                  const uint64_t *__restrict__ rowStart, uint64_t R, const float *__restrict__ pos,
+                                  MeshParams mp, int tribits, uint32_t *__restrict__ hitCount,
+                                  const uint64_t *__restrict__ hitStart, uint64_t *__restrict__ keys) {
+    uint64_t r = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r >= R) return;
+    const uint64_t i = find_pair(rowStart, P, r);
+    const uint32_t t = tris[i];
+    float tv[9];
+    load_tri(pos, t, tv);
+    int l[3], u[3];
+    pair_box(cells[i], tv, mp, l, u);
+    const uint64_t local = r - rowStart[i];
+    const int ny = u[1] - l[1] + 1;
+    const int y = l[1] + int(local % uint64_t(ny));
+    const int z = l[2] + int(local / uint64_t(ny));
+    const float hx = mp.hx;
+    const float half[3] = {0.5f * hx, 0.5f * hx, 0.5f * hx};
+    float ctr[3];
+    ctr[2] = (float(l[2]) - 0.5f) * hx;
+    for (int k = l[2]; k < z; ++k) ctr[2] += hx;
+    ctr[1] = (float(l[1]) - 0.5f) * hx;
+    for (int k = l[1]; k < y; ++k) ctr[1] += hx;
+    ctr[0] = (float(l[0]) - 0.5f) * hx;
+    uint32_t n = 0;
+    uint64_t at = keys ? hitStart[r] : 0;
+    for (int x = l[0]; x <= u[0]; ++x, ctr[0] += hx) {
+        if (tri_box(ctr, half, tv)) {
+            if (keys) keys[at + n] = (morton3(uint32_t(x), uint32_t(y), uint32_t(z)) << tribits) | uint64_t(t);
+            n++;
+        }
+    }
+    if (!keys) hitCount[r] = n;
+}
+
+__global__ void seg_heads_kernel(const uint64_t *__restrict__ keys, uint64_t K, int shift, uint32_t *__restrict__ head) {
+    uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= K) return;
+    head[i] = (i == 0 || (keys[i] >> shift) != (keys[i - 1] >> shift)) ? 1u : 0u;
+}
+
+__global__ void seg_starts_kernel(const uint32_t *__restrict__ head, const uint64_t *__restrict__ segIdx, uint64_t K,
+                                  uint64_t *__restrict__ segStart) {
+    uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= K) return;
+    if (head[i]) segStart[segIdx[i]] = i;
+}
+
+// PlyLoader::writeTriangleCell, src/PlyLoader.cpp:296-336, folded over one
+// voxel's triangles in ascending index order. The voxel centre each triangle
+// used is rebuilt with the same +hx steps from that triangle's clipped start.
+__global__ void fold_kernel(const uint64_t *__restrict__ keys, const uint64_t *__restrict__ segStart, uint64_t S,
+                            int tribits, const float *__restrict__ pos, const float *__restrict__ nrm,
+                            const float *__restrict__ col, MeshParams mp, uint64_t *__restrict__ vkey,
+                            uint32_t *__restrict__ vword) {
+    uint64_t s = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    const uint64_t b = segStart[s], e = segStart[s + 1];
+    const uint64_t m = keys[b] >> tribits;
+    const int vx[3] = {int(compact3(m)), int(compact3(m >> 1)), int(compact3(m >> 2))};
+    const uint64_t triMask = (uint64_t(1) << tribits) - 1;
+    uint32_t word = 0;
+    uint32_t count = 0;
+    const float hx = mp.hx;
+    for (uint64_t k = b; k < e; ++k) {
+        const int64_t t = int64_t(keys[k] & triMask);
+        float tv[9];
+        load_tri(pos, t, tv);
+        int l[3], u[3];
+        tri_grid_box(tv, float(mp.scale), l, u);
+        float p[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const int start = max(l[a], (vx[a] / mp.sb[a]) * mp.sb[a]);
+            float c = (float(start) - 0.5f) * hx;
+            for (int q = start; q < vx[a]; ++q) c += hx;
+            p[a] = c;
+        }
+        // Triangle::barycentric, src/PlyLoader.cpp:55-64
+        float f1[3], f2[3], f3[3], ea[3], eb[3], cr[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            f1[a] = tv[a] - p[a];
+            f2[a] = tv[3 + a] - p[a];
+            f3[a] = tv[6 + a] - p[a];
+            ea[a] = tv[a] - tv[3 + a];
+            eb[a] = tv[a] - tv[6 + a];
+        }
+        cross3(ea, eb, cr);
+        const float area = length3(cr);
+        cross3(f2, f3, cr);
+        float l1 = length3(cr) / area;
+        cross3(f3, f1, cr);
+        float l2 = length3(cr) / area;
+        const bool inside = l1 >= 0.0f && l2 >= 0.0f && l1 + l2 <= 1.0f;
+        if (!inside) {
+            l1 = fmin_ref(fmax_ref(l1, 0.0f), 1.0f);
+            l2 = fmin_ref(fmax_ref(l2, 0.0f), 1.0f);
+            const float tau = l1 + l2;
+            if (tau > 1.0f) {
+                l1 /= tau;
+                l2 /= tau;
+            }
+        }
+        const float l3 = 1.0f - l1 - l2;
+        float nb[3], n[3], cc[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            nb[a] = nrm[9 * t + a] * l1 + nrm[9 * t + 3 + a] * l2 + nrm[9 * t + 6 + a] * l3;
+            cc[a] = col[9 * t + a] * l1 + col[9 * t + 3 + a] * l2 + col[9 * t + 6 + a] * l3;
+        }
+        normalize3(nb, n);
+        const float shade = (cc[0] * 0.2126f + cc[1] * 0.7152f + cc[2] * 0.0722f) * (1.0f / 256.0f);
+        if (word == 0) {
+            count = 1;
+            word = compress_material(n, shade);
+        } else {
+            const float cnt = float(count);
+            const float curRatio = cnt / (cnt + 1.0f);
+            const float newRatio = 1.0f - curRatio;
+            float cn[3], cs;
+            decompress_material(word, cn, cs);
+            float nn[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) nn[a] = cn[a] * curRatio + n[a] * newRatio;
+            const float ns = cs * curRatio + shade * newRatio;
+            if (nn[0] * nn[0] + nn[1] * nn[1] + nn[2] * nn[2] < 1e-3f) {
+                nn[0] = cn[0];
+                nn[1] = cn[1];
+                nn[2] = cn[2];
+            }
+            word = compress_material(nn, ns);
+            count = min(count + 1u, 255u);
+        }
+    }
+    vkey[s] = m;
+    vword[s] = word;
+}
+
+// ---------------------------------------------------------------------------
+// file-source kernels
+// ---------------------------------------------------------------------------
+
+__global__ void grid_keys_kernel(const uint32_t *__restrict__ grid, int w, int h, int d, uint32_t *__restrict__ flag) {
+    uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= uint64_t(w) * h * d) return;
+    flag[i] = grid[i] != 0 ? 1u : 0u;
+}
+
+__global__ void grid_compact_kernel(const uint32_t *__restrict__ grid, int w, int h, int d,
+                                    const uint32_t *__restrict__ flag, const uint64_t *__restrict__ at,
+                                    uint64_t *__restrict__ key, uint32_t *__restrict__ word) {
+    uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= uint64_t(w) * h * d || !flag[i]) return;
+    const uint32_t x = uint32_t(i % uint64_t(w));
+    const uint32_t y = uint32_t((i / uint64_t(w)) % uint64_t(h));
+    const uint32_t z = uint32_t(i / (uint64_t(w) * h));
+    key[at[i]] = morton3(x, y, z);
+    word[at[i]] = grid[i];
+}
+
+// ---------------------------------------------------------------------------
+// occupancy-table behaviour shared by both sources
+// ---------------------------------------------------------------------------
+
+struct BlockParams {
+    int data[3];
+    int cache;
+    int cacheLog;
+};
+
+// Voxel on the last slice of an odd-depth cache block: never scanned by
+// VoxelData::buildLowLut (src/VoxelData.cpp:163-166), hence never emitted.
+__device__ __forceinline__ bool unscanned(uint32_t z, const BlockParams &bp) {
+    const int z0 = int(z >> bp.cacheLog) << bp.cacheLog;
+    const int bd = min(bp.cache, bp.data[2] - z0);
+    return (bd & 1) && int(z) == z0 + bd - 1;
+}
+
+// keep[i] = voxel i is emitted; flags cache blocks holding any voxel (the
+// file source's top table, VoxelData::buildTopLutBlock :120-145) and, for the
+// mesh source, marks blocks that hold an unscanned voxel.
+__global__ void keep_kernel(const uint64_t *__restrict__ key, const uint32_t *__restrict__ word, uint64_t V,
+                            BlockParams bp, int markBlocks, uint32_t *__restrict__ blockFlags,
+                            uint32_t *__restrict__ hazardBlocks, uint32_t *__restrict__ keep) {
+    uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= V) return;
+    const uint64_t m = key[i];
+    const uint32_t z = compact3(m >> 2);
+    const bool nz = word[i] != 0;
+    const uint64_t blk = m >> (3 * bp.cacheLog);
+    if (nz && markBlocks) blockFlags[blk] = 1u;
+    const bool drop = nz && unscanned(z, bp);
+    if (drop) hazardBlocks[blk] = 1u;
+    keep[i] = (nz && !drop) ? 1u : 0u;
+}
+
+__global__ void compact_kernel(const uint64_t *__restrict__ key, const uint32_t *__restrict__ word, uint64_t V,
+                               const uint32_t *__restrict__ keep, const uint64_t *__restrict__ at,
+                               uint64_t *__restrict__ okey, uint32_t *__restrict__ oword) {
+    uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= V || !keep[i]) return;
+    okey[at[i]] = key[i];
+    oword[at[i]] = word[i];
+}
+
+// ---------------------------------------------------------------------------
+// levels, sizes, placement
+// ---------------------------------------------------------------------------
+
+__global__ void parent_heads_kernel(const uint64_t *__restrict__ child, uint64_t n, uint32_t *__restrict__ head) {
+    uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    head[i] = (i == 0 || (child[i] >> 3) != (child[i - 1] >> 3)) ? 1u : 0u;
+}
+
+// parent key / child start / mask from sorted child keys (no extra parents)
+__global__ void parents_kernel(const uint64_t *__restrict__ child, uint64_t n, const uint32_t *__restrict__ head,
+                               const uint64_t *__restrict__ incl, uint64_t *__restrict__ pkey,
+                               uint64_t *__restrict__ pstart, uint32_t *__restrict__ pmask) {
+    uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t p = incl[i];  // exclusive scan of heads: parent index of child i is p + head - 1
+    const uint64_t pi = p + head[i] - 1;
+    if (head[i]) {
+        pkey[pi] = child[i] >> 3;
+        pstart[pi] = i;
+    }
+    atomicOr(&pmask[pi], 1u << uint32_t(child[i] & 7));
+}
+
+// Level that also takes the cache blocks the occupancy table marks: parents
+// are the flagged cells of a dense Morton-indexed table.
+__global__ void dense_mark_kernel(const uint64_t *__restrict__ child, uint64_t n, uint32_t *__restrict__ dense) {
+    uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    dense[child[i] >> 3] = 1u;
+}
+
+__global__ void dense_compact_kernel(const uint32_t *__restrict__ dense, uint64_t cells,
+                                     const uint64_t *__restrict__ at, uint64_t *__restrict__ pkey,
+                                     uint64_t *__restrict__ slot) {
+    uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= cells) return;
+    if (dense[i]) pkey[at[i]] = i;
+    slot[i] = at[i];
+}
+
+__global__ void dense_children_kernel(const uint64_t *__restrict__ child, uint64_t n, const uint64_t *__restrict__ slot,
+                                      uint64_t *__restrict__ pstart, uint32_t *__restrict__ pmask) {
+    uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t pk = child[i] >> 3;
+    const uint64_t pi = slot[pk];
+    if (i == 0 || (child[i - 1] >> 3) != pk) pstart[pi] = i;
+    atomicOr(&pmask[pi], 1u << uint32_t(child[i] & 7));
+}
+
+// Subtree sizes, bottom-up (VoxelOctree::buildOctree, src/VoxelOctree.cpp:199-265
+// and ChunkedAllocator insertions, ChunkedAllocator.hpp:72-107): a node with n
+// children occupies n*(1+large) words for its child block plus its children's
+// subtrees S. Child i's offset from its own slot, before the node's far words,
+// is n - i + sum_{j<i} S(c_j); the node needs far pointers (large) if any
+// exceeds 0x3FFF, and then every child slot takes a second word.
+__global__ void size_kernel(const uint32_t *__restrict__ mask, const uint64_t *__restrict__ start, uint64_t n,
+                            int leafParents, const uint64_t *__restrict__ childS, uint64_t *__restrict__ S,
+                            uint8_t *__restrict__ large) {
+    uint64_t k = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t c = __popc(mask[k]);
+    if (leafParents || c == 0) {
+        S[k] = c;
+        large[k] = 0;
+        return;
+    }
+    uint64_t acc = 0, maxOff = 0;
+    const uint64_t s0 = start[k];
+    for (uint32_t i = 0; i < c; ++i) {
+        const uint64_t off = uint64_t(c - i) + acc;
+        if (off > maxOff) maxOff = off;
+        acc += childS[s0 + i];
+    }
+    const bool lg = maxOff > 0x3FFF;
+    large[k] = lg ? 1 : 0;
+    S[k] = uint64_t(c) * (lg ? 2 : 1) + acc;
+}
+
+// Placement, top-down: a node at `pos` whose child block starts at `B` puts
+// child i at B + i*(1+large); that child's own block starts after the node's
+// whole child block and the subtrees of children 0..i-1.
+__global__ void place_kernel(const uint32_t *__restrict__ mask, const uint64_t *__restrict__ start, uint64_t n,
+                             const uint64_t *__restrict__ B, const uint8_t *__restrict__ large,
+                             const uint64_t *__restrict__ childS, int childLeaves, uint64_t *__restrict__ childPos,
+                             uint64_t *__restrict__ childB, uint8_t *__restrict__ childFar) {
+    uint64_t k = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t c = __popc(mask[k]);
+    if (c == 0) return;
+    const uint64_t per = large[k] ? 2 : 1;
+    const uint64_t b = B[k], s0 = start[k];
+    uint64_t acc = 0;
+    for (uint32_t i = 0; i < c; ++i) {
+        childPos[s0 + i] = b + i * per;
+        if (!childLeaves) {
+            childB[s0 + i] = b + uint64_t(c) * per + acc;
+            childFar[s0 + i] = large[k];
+            acc += childS[s0 + i];
+        }
+    }
+}
+
+// Descriptor word (and far word) of every node of one level.
+// Positions are checked against the word count W; a violation (which the
+// size/placement arithmetic excludes) is reported instead of written.
+__global__ void write_nodes_kernel(const uint32_t *__restrict__ mask, uint64_t n, const uint64_t *__restrict__ pos,
+                                   const uint64_t *__restrict__ B, const uint8_t *__restrict__ large,
+                                   const uint8_t *__restrict__ far, int leafParents, uint32_t *__restrict__ out,
+                                   uint64_t W, uint32_t *__restrict__ bad) {
+    uint64_t k = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const uint32_t m = mask[k];
+    uint32_t w = (m << 8) | (leafParents ? 0u : m);
+    if (large[k]) w |= 0x10000u;
+    const uint64_t p = pos[k];
+    const uint64_t off = B[k] - p;
+    if (p >= W || (far[k] && p + 1 >= W)) {
+        atomicOr(bad, 1u);
+        return;
+    }
+    if (far[k]) {
+        w |= 0x20000u | uint32_t((off >> 32) << 18);
+        out[p + 1] = uint32_t(off);
+    } else {
+        w |= uint32_t(off << 18);
+    }
+    out[p] = w;
+}
+
+__global__ void write_leaves_kernel(const uint32_t *__restrict__ word, uint64_t n, const uint64_t *__restrict__ pos,
+                                    uint32_t *__restrict__ out, uint64_t W, uint32_t *__restrict__ bad) {
+    uint64_t k = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    if (pos[k] >= W) {
+        atomicOr(bad, 1u);
+        return;
+    }
+    out[pos[k]] = word[k];
+}
+
+__global__ void fill_u32_kernel(uint32_t *p, uint64_t n, uint32_t v) {
+    uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// host helpers
+// ---------------------------------------------------------------------------
+
+struct ToU64 {
+    template <typename T>
+    __host__ __device__ uint64_t operator()(const T &v) const {
+        return uint64_t(v);
+    }
+};
+
+// Exclusive prefix sum accumulated in 64 bits.
+template <typename In>
+int exclusive_scan(Arena &ar, const In *in, uint64_t *out, uint64_t n, hipStream_t s) {
+    if (n == 0) return SVO_OK;
+    hipcub::TransformInputIterator<uint64_t, ToU64, const In *> it(in, ToU64());
+    size_t bytes = 0;
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, it, out, n, s));
+    uint8_t *tmp;
+    int r = ar.alloc(tmp, bytes);
+    if (r) return r;
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(tmp, bytes, it, out, n, s));
+    ar.release(tmp);
+    return SVO_OK;
+}
+
+// total = exclusive[n-1] + last element
+template <typename In>
+int scan_total(const In *in, const uint64_t *ex, uint64_t n, uint64_t &total) {
+    if (n == 0) {
+        total = 0;
+        return SVO_OK;
+    }
+    In last;
+    uint64_t lastEx;
+    HIP_TRY(hipMemcpy(&last, in + n - 1, sizeof(In), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(&lastEx, ex + n - 1, sizeof(uint64_t), hipMemcpyDeviceToHost));
+    total = lastEx + uint64_t(last);
+    return SVO_OK;
+}
+
+int fill32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_for(n)), dim3(kThreads), 0, s, p, n, v);
+    HIP_TRY(hipGetLastError());
+    return SVO_OK;
+}
+
+struct Level {
+    uint64_t n = 0;
+    uint64_t *key = nullptr;
+    uint32_t *mask = nullptr;   // child mask (nodes above the leaves)
+    uint64_t *start = nullptr;  // index of the first child in the level below
+    uint64_t *S = nullptr;      // subtree size (words after the node's slot)
+    uint8_t *large = nullptr;
+    uint64_t *pos = nullptr;    // word index of the node
+    uint64_t *B = nullptr;      // word index of the node's child block
+    uint8_t *far = nullptr;     // the node's slot carries a far word
+    uint32_t *word = nullptr;   // leaves: material word
+};
+
+// Builds the word array from sorted, unique leaf keys/words. blockFlags marks
+// the cache blocks the occupancy table counts as non-empty (nullptr when the
+// whole volume is one cache block).
+int emit_tree(Arena &ar, const Geometry &g, uint64_t *leafKey, uint32_t *leafWord, uint64_t L,
+              const uint32_t *blockFlags, hipStream_t st, svo_octree &out) {
+    const int D = g.depth;
+    std::vector<Level> lv(size_t(D) + 1);
+    lv[D].n = L;
+    lv[D].key = leafKey;
+    lv[D].word = leafWord;
+    const int cacheLevel = D - g.cacheLog;  // nodes of edge `cache`
+    const bool multi = g.cacheLog < D;
+    int r;
+    for (int d = D - 1; d >= 0; --d) {
+        Level &c = lv[d + 1];
+        Level &p = lv[d];
+        if (multi && d == cacheLevel) {
+            const uint64_t cells = uint64_t(1) << (3 * d);
+            uint32_t *dense;
+            uint64_t *at, *slot;
+            if ((r = ar.alloc(dense, cells)) || (r = ar.alloc(at, cells)) || (r = ar.alloc(slot, cells))) return r;
+            HIP_TRY(hipMemcpyAsync(dense, blockFlags, cells * 4, hipMemcpyDeviceToDevice, st));
+            if (c.n) hipLaunchKernelGGL(dense_mark_kernel, dim3(grid_for(c.n)), dim3(kThreads), 0, st, c.key, c.n, dense);
+            if ((r = exclusive_scan(ar, dense, at, cells, st))) return r;
+            if ((r = scan_total(dense, at, cells, p.n))) return r;
+            if ((r = ar.alloc(p.key, p.n)) || (r = ar.alloc(p.start, p.n)) || (r = ar.alloc(p.mask, p.n))) return r;
+            if ((r = fill32(p.mask, p.n, 0, st))) return r;
+            hipLaunchKernelGGL(dense_compact_kernel, dim3(grid_for(cells)), dim3(kThreads), 0, st, dense, cells, at,
+                               p.key, slot);
+            if (c.n)
+                hipLaunchKernelGGL(dense_children_kernel, dim3(grid_for(c.n)), dim3(kThreads), 0, st, c.key, c.n, slot,
+                                   p.start, p.mask);
+        } else {
+            if (c.n == 0) {
+                // empty level below: only the root can remain (an empty tree)
+                p.n = d == 0 ? 1 : 0;
+                if ((r = ar.alloc(p.key, 1)) || (r = ar.alloc(p.start, 1)) || (r = ar.alloc(p.mask, 1))) return r;
+                uint64_t z = 0;
+                HIP_TRY(hipMemcpy(p.key, &z, 8, hipMemcpyHostToDevice));
+                HIP_TRY(hipMemcpy(p.start, &z, 8, hipMemcpyHostToDevice));
+                if ((r = fill32(p.mask, 1, 0, st))) return r;
+                continue;
+            }
+            uint32_t *head;
+            uint64_t *ex;
+            if ((r = ar.alloc(head, c.n)) || (r = ar.alloc(ex, c.n))) return r;
+            hipLaunchKernelGGL(parent_heads_kernel, dim3(grid_for(c.n)), dim3(kThreads), 0, st, c.key, c.n, head);
+            if ((r = exclusive_scan(ar, head, ex, c.n, st))) return r;
+            if ((r = scan_total(head, ex, c.n, p.n))) return r;
+            if ((r = ar.alloc(p.key, p.n)) || (r = ar.alloc(p.start, p.n)) || (r = ar.alloc(p.mask, p.n))) return r;
+            if ((r = fill32(p.mask, p.n, 0, st))) return r;
+            hipLaunchKernelGGL(parents_kernel, dim3(grid_for(c.n)), dim3(kThreads), 0, st, c.key, c.n, head, ex, p.key,
+                               p.start, p.mask);
+            ar.release(head);
+            ar.release(ex);
+        }
+        HIP_TRY(hipGetLastError());
+    }
+    if (lv[0].n != 1) return fail(SVO_ERR_DEVICE, "builder: root level does not hold one node");
+
+    // sizes, bottom-up
+    for (int d = D - 1; d >= 0; --d) {
+        Level &p = lv[d];
+        if ((r = ar.alloc(p.S, p.n)) || (r = ar.alloc(p.large, p.n))) return r;
+        if (p.n)
+            hipLaunchKernelGGL(size_kernel, dim3(grid_for(p.n)), dim3(kThreads), 0, st, p.mask, p.start, p.n,
+                               int(d == D - 1), lv[d + 1].S, p.S, p.large);
+        HIP_TRY(hipGetLastError());
+    }
+    uint64_t rootS = 0;
+    HIP_TRY(hipMemcpy(&rootS, lv[0].S, 8, hipMemcpyDeviceToHost));
+    const uint64_t W = 1 + rootS;
+
+    // placement, top-down
+    {
+        Level &root = lv[0];
+        if ((r = ar.alloc(root.pos, 1)) || (r = ar.alloc(root.B, 1)) || (r = ar.alloc(root.far, 1))) return r;
+        const uint64_t zero = 0, one = 1;
+        const uint8_t nf = 0;
+        HIP_TRY(hipMemcpy(root.pos, &zero, 8, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(root.B, &one, 8, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(root.far, &nf, 1, hipMemcpyHostToDevice));
+    }
+    for (int d = 0; d < D; ++d) {
+        Level &p = lv[d], &c = lv[d + 1];
+        const bool childLeaves = d + 1 == D;
+        if ((r = ar.alloc(c.pos, c.n))) return r;
+        if (!childLeaves && ((r = ar.alloc(c.B, c.n)) || (r = ar.alloc(c.far, c.n)))) return r;
+        if (p.n)
+            hipLaunchKernelGGL(place_kernel, dim3(grid_for(p.n)), dim3(kThreads), 0, st, p.mask, p.start, p.n, p.B,
+                               p.large, childLeaves ? nullptr : c.S, int(childLeaves), c.pos, c.B, c.far);
+        HIP_TRY(hipGetLastError());
+    }
+
+    // scatter
+    uint32_t *words;
+    {
+        void *pw = nullptr;
+        hipError_t e = hipMalloc(&pw, W * sizeof(uint32_t));
+        if (e != hipSuccess) return fail(SVO_ERR_NOMEM, std::string("builder: word array: ") + hipGetErrorString(e));
+        words = static_cast<uint32_t *>(pw);
+    }
+    uint32_t *bad;
+    if ((r = ar.alloc(bad, 1)) || (r = fill32(bad, 1, 0, st))) {
+        (void)hipFree(words);
+        return r;
+    }
+    for (int d = 0; d < D; ++d) {
+        Level &p = lv[d];
+        if (p.n)
+            hipLaunchKernelGGL(write_nodes_kernel, dim3(grid_for(p.n)), dim3(kThreads), 0, st, p.mask, p.n, p.pos, p.B,
+                               p.large, p.far, int(d == D - 1), words, W, bad);
+    }
+    if (L)
+        hipLaunchKernelGGL(write_leaves_kernel, dim3(grid_for(L)), dim3(kThreads), 0, st, leafWord, L, lv[D].pos, words,
+                           W, bad);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    uint32_t badH = 0;
+    if (e == hipSuccess) e = hipMemcpy(&badH, bad, 4, hipMemcpyDeviceToHost);
+    if (e != hipSuccess || badH) {
+        (void)hipFree(words);
+        if (badH) return fail(SVO_ERR_DEVICE, "builder: placement outside the word array (internal error)");
+        return fail(SVO_ERR_DEVICE, std::string("builder: scatter: ") + hipGetErrorString(e));
+    }
+    try {
+        out.words.resize(W);
+    } catch (...) {
+        (void)hipFree(words);
+        return fail(SVO_ERR_NOMEM, "builder: host word array");
+    }
+    e = hipMemcpy(out.words.data(), words, W * sizeof(uint32_t), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+        (void)hipFree(words);
+        return fail(SVO_ERR_DEVICE, std::string("builder: download: ") + hipGetErrorString(e));
+    }
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    out.dwords = words;
+    out.device = dev;
+    out.center[0] = float(g.dataW) * 0.5f / float(g.side);  // VoxelData::getCenter, VoxelData.cpp:297-303
+    out.center[1] = float(g.dataH) * 0.5f / float(g.side);
+    out.center[2] = float(g.dataD) * 0.5f / float(g.side);
+    return SVO_OK;
+}
+
+// Drops unscanned voxels, flags occupied blocks (file source), and compacts
+// to the emitted leaves. For the mesh source an unscanned voxel in any cache
+// block but the last one processed would linger in the reused block buffer
+// and alter later blocks (VoxelData reuses _bufferedData without clearing);
+// that case is refused.
+int finish_leaves(Arena &ar, const Geometry &g, uint64_t *key, uint32_t *word, uint64_t V, bool mesh,
+                  uint32_t *blockFlags, uint64_t nBlocks, hipStream_t st, uint64_t *&lkey, uint32_t *&lword,
+                  uint64_t &L) {
+    int r;
+    uint32_t *keep, *hazard;
+    uint64_t *at;
+    if ((r = ar.alloc(keep, V)) || (r = ar.alloc(at, V)) || (r = ar.alloc(hazard, nBlocks))) return r;
+    if ((r = fill32(hazard, nBlocks, 0, st))) return r;
+    BlockParams bp{{g.dataW, g.dataH, g.dataD}, g.cache, g.cacheLog};
+    if (V)
+        hipLaunchKernelGGL(keep_kernel, dim3(grid_for(V)), dim3(kThreads), 0, st, key, word, V, bp, int(!mesh),
+                           blockFlags, hazard, keep);
+    HIP_TRY(hipGetLastError());
+    if (mesh && nBlocks > 1) {
+        std::vector<uint32_t> hz(nBlocks), fl(nBlocks);
+        HIP_TRY(hipMemcpy(hz.data(), hazard, nBlocks * 4, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(fl.data(), blockFlags, nBlocks * 4, hipMemcpyDeviceToHost));
+        int64_t last = -1;
+        for (uint64_t b = 0; b < nBlocks; ++b)
+            if (fl[b]) last = int64_t(b);
+        for (uint64_t b = 0; b < nBlocks; ++b)
+            if (hz[b] && int64_t(b) != last)
+                return fail(SVO_ERR_UNSUPPORTED,
+                            "mesh voxels fall on the unscanned last slice of an odd-depth cache block that is "
+                            "not the last one built; the reference's result then depends on its block-buffer "
+                            "reuse, which this builder does not reproduce (use a larger memory budget)");
+    }
+    if ((r = exclusive_scan(ar, keep, at, V, st))) return r;
+    if ((r = scan_total(keep, at, V, L))) return r;
+    if ((r = ar.alloc(lkey, L)) || (r = ar.alloc(lword, L))) return r;
+    if (V) hipLaunchKernelGGL(compact_kernel, dim3(grid_for(V)), dim3(kThreads), 0, st, key, word, V, keep, at, lkey, lword);
+    HIP_TRY(hipGetLastError());
+    return SVO_OK;
+}
+
+int host_threads(const svo_build_opts *opts) {
+    if (opts && opts->threads > 0) return opts->threads;
+    long n = sysconf(_SC_NPROCESSORS_ONLN);  // ThreadUtils::idealThreadCount
+    return n > 0 ? int(n) : 4;
+}
+
+uint64_t budget(const svo_build_opts *opts) {
+    return (opts && opts->memory_budget) ? opts->memory_budget : (uint64_t(1) << 30);
+}
+
+int check_blocks(const Geometry &g, uint64_t &nBlocks) {
+    const int levels = g.depth - g.cacheLog;
+    if (3 * levels > 30) return fail(SVO_ERR_LIMIT, "too many cache blocks for the occupancy table (raise the budget)");
+    nBlocks = uint64_t(1) << (3 * levels);
+    return SVO_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// entry points
+// ---------------------------------------------------------------------------
+
+int build_mesh(const float *verts, int64_t nverts, int has_normals, const int32_t *face_counts,
+               const int32_t *face_idx, int64_t nfaces, int resolution, const svo_build_opts *opts, svo_octree &out) {
+    if (resolution < 8 || resolution > 8192) return fail(SVO_ERR_INVALID, "resolution must be in [8, 8192]");
+    // PlyLoader::readVertices bounds and rescaleVertices (src/PlyLoader.cpp:122-181)
+    float lower[3] = {1e30f, 1e30f, 1e30f}, upper[3] = {-1e30f, -1e30f, -1e30f};
+    for (int64_t i = 0; i < nverts; ++i)
+        for (int k = 0; k < 3; ++k) {
+            const float v = verts[9 * i + k];
+            if (!std::isfinite(v)) return fail(SVO_ERR_UNSUPPORTED, "non-finite vertex position");
+            lower[k] = fmin_ref(lower[k], v);
+            upper[k] = fmax_ref(upper[k], v);
+        }
+    const float diff[3] = {upper[0] - lower[0], upper[1] - lower[1], upper[2] - lower[2]};
+    int largest = 2;
+    if (diff[0] > diff[1] && diff[0] > diff[2]) largest = 0;
+    else if (diff[1] > diff[2]) largest = 1;
+    if (!(diff[largest] > 0.0f)) return fail(SVO_ERR_UNSUPPORTED, "degenerate mesh extent");
+    const float factor = 1.0f / diff[largest];
+    float sl[3], su[3];
+    for (int k = 0; k < 3; ++k) {
+        su[k] = upper[k] * factor;
+        sl[k] = lower[k] * factor;
+    }
+    // face table
+    std::vector<int64_t> fstart(size_t(nfaces) + 1), tstart(size_t(nfaces) + 1);
+    int64_t at = 0, T = 0;
+    for (int64_t f = 0; f < nfaces; ++f) {
+        if (face_counts[f] < 0) return fail(SVO_ERR_FORMAT, "negative face size");
+        fstart[f] = at;
+        tstart[f] = T;
+        at += face_counts[f];
+        if (face_counts[f] > 2) T += face_counts[f] - 2;
+    }
+    for (int64_t i = 0; i < at; ++i)
+        if (face_idx[i] < 0 || face_idx[i] >= nverts) return fail(SVO_ERR_FORMAT, "face references a missing vertex");
+    if (T >= (int64_t(1) << 32)) return fail(SVO_ERR_LIMIT, "too many triangles");
+
+    Geometry g{};
+    // PlyLoader::suggestedDimensions, src/PlyLoader.cpp:499-504
+    g.dataW = cvt_i32((su[0] - sl[0]) * float(resolution - 2)) + 2;
+    g.dataH = cvt_i32((su[1] - sl[1]) * float(resolution - 2)) + 2;
+    g.dataD = cvt_i32((su[2] - sl[2]) * float(resolution - 2)) + 2;
+    int r = choose_cache(g, budget(opts), true);
+    if (r) return r;
+    // PlyLoader::setupBlockProcessing, src/PlyLoader.cpp:407-437
+    g.scale = resolution - 4;
+    g.sbW = g.sbH = g.sbD = g.cache;
+    best_partition(g.sbW, g.sbH, g.sbD, host_threads(opts));
+    g.pW = g.cache / g.sbW;
+    g.pH = g.cache / g.sbH;
+    g.pD = g.cache / g.sbD;
+    g.gW = g.pW * (g.dataW + g.cache - 1) / g.cache;
+    g.gH = g.pH * (g.dataH + g.cache - 1) / g.cache;
+    g.gD = g.pD * (g.dataD + g.cache - 1) / g.cache;
+    if (uint64_t(g.gW) * g.gH * g.gD >= (uint64_t(1) << 32)) return fail(SVO_ERR_LIMIT, "sub-block grid too large");
+    int tribits = 1;
+    while ((int64_t(1) << tribits) < std::max<int64_t>(T, 2)) tribits++;
+    if (3 * g.depth + tribits > 64) return fail(SVO_ERR_LIMIT, "voxel key and triangle index exceed 64 bits");
+    uint64_t nBlocks;
+    if ((r = check_blocks(g, nBlocks))) return r;
+
+    MeshParams mp;
+    mp.scale = g.scale;
+    mp.sb[0] = g.sbW; mp.sb[1] = g.sbH; mp.sb[2] = g.sbD;
+    mp.g[0] = g.gW; mp.g[1] = g.gH; mp.g[2] = g.gD;
+    mp.p[0] = g.pW; mp.p[1] = g.pH; mp.p[2] = g.pD;
+    mp.data[0] = g.dataW; mp.data[1] = g.dataH; mp.data[2] = g.dataD;
+    mp.cache = g.cache;
+    mp.hx = 1.0f / float(g.scale);
+    mp.hb[0] = float(g.sbW) / float(g.scale);
+    mp.hb[1] = float(g.sbH) / float(g.scale);
+    mp.hb[2] = float(g.sbD) / float(g.scale);
+
+    Arena ar;
+    HIP_TRY(hipStreamCreate(&ar.stream));
+    hipStream_t st = ar.stream;
+
+    float *dv, *pos, *nrm, *col;
+    int32_t *dfc, *dfi;
+    int64_t *dfs, *dts;
+    if ((r = ar.alloc(dv, uint64_t(nverts) * 9)) || (r = ar.alloc(dfc, nfaces)) || (r = ar.alloc(dfi, at)) ||
+        (r = ar.alloc(dfs, nfaces)) || (r = ar.alloc(dts, nfaces)) || (r = ar.alloc(pos, uint64_t(T) * 9)) ||
+        (r = ar.alloc(nrm, uint64_t(T) * 9)) || (r = ar.alloc(col, uint64_t(T) * 9)))
+        return r;
+    HIP_TRY(hipMemcpyAsync(dv, verts, uint64_t(nverts) * 36, hipMemcpyHostToDevice, st));
+    if (nfaces) {
+        HIP_TRY(hipMemcpyAsync(dfc, face_counts, uint64_t(nfaces) * 4, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(dfs, fstart.data(), uint64_t(nfaces) * 8, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(dts, tstart.data(), uint64_t(nfaces) * 8, hipMemcpyHostToDevice, st));
+    }
+    if (at) HIP_TRY(hipMemcpyAsync(dfi, face_idx, uint64_t(at) * 4, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(rescale_kernel, dim3(grid_for(nverts)), dim3(kThreads), 0, st, dv, nverts, lower[0], lower[1],
+                       lower[2], factor);
+    if (nfaces)
+        hipLaunchKernelGGL(triangles_kernel, dim3(grid_for(nfaces)), dim3(kThreads), 0, st, dv, dfc, dfs, dts, dfi, nfaces,
+                           has_normals, pos, nrm, col);
+    HIP_TRY(hipGetLastError());
+    ar.release(dv);
+    ar.release(dfc);
+    ar.release(dfi);
+    ar.release(dfs);
+    ar.release(dts);
+
+    // binning
+    uint32_t *binCount, *bad;
+    uint64_t *binOff;
+    if ((r = ar.alloc(binCount, T)) || (r = ar.alloc(binOff, T)) || (r = ar.alloc(bad, 1))) return r;
+    if ((r = fill32(bad, 1, 0, st))) return r;
+    if (T)
+        hipLaunchKernelGGL(bin_kernel, dim3(grid_for(T)), dim3(kThreads), 0, st, pos, T, mp, binCount, nullptr, nullptr,
+                           nullptr, bad);
+    HIP_TRY(hipGetLastError());
+    if ((r = exclusive_scan(ar, binCount, binOff, T, st))) return r;
+    uint64_t P;
+    if ((r = scan_total(binCount, binOff, T, P))) return r;
+    uint32_t *cells, *ptri;
+    if ((r = ar.alloc(cells, P)) || (r = ar.alloc(ptri, P))) return r;
+    if (T)
+        hipLaunchKernelGGL(bin_kernel, dim3(grid_for(T)), dim3(kThreads), 0, st, pos, T, mp, binCount, binOff, cells, ptri,
+                           bad);
+    uint32_t badH = 0;
+    HIP_TRY(hipMemcpy(&badH, bad, 4, hipMemcpyDeviceToHost));
+    if (badH) return fail(SVO_ERR_UNSUPPORTED, "a triangle bins outside the sub-block table (reference undefined behaviour)");
+    ar.release(binCount);
+    ar.release(binOff);
+
+    uint32_t *blockFlags;
+    if ((r = ar.alloc(blockFlags, nBlocks)) || (r = fill32(blockFlags, nBlocks, 0, st))) return r;
+    if (P) hipLaunchKernelGGL(block_flags_kernel, dim3(grid_for(P)), dim3(kThreads), 0, st, cells, P, mp, blockFlags);
+
+    // voxel rows
+    uint32_t *rows;
+    uint64_t *rowStart;
+    if ((r = ar.alloc(rows, P)) || (r = ar.alloc(rowStart, P + 1))) return r;
+    if (P) hipLaunchKernelGGL(pair_rows_kernel, dim3(grid_for(P)), dim3(kThreads), 0, st, cells, ptri, P, pos, mp, rows);
+    HIP_TRY(hipGetLastError());
+    if ((r = exclusive_scan(ar, rows, rowStart, P, st))) return r;
+    uint64_t R;
+    if ((r = scan_total(rows, rowStart, P, R))) return r;
+    HIP_TRY(hipMemcpy(rowStart + P, &R, 8, hipMemcpyHostToDevice));
+    ar.release(rows);
+    uint32_t *hitCount;
+    uint64_t *hitStart;
+    if ((r = ar.alloc(hitCount, R)) || (r = ar.alloc(hitStart, R))) return r;
+    if (R)
+        hipLaunchKernelGGL(row_voxels_kernel, dim3(grid_for(R)), dim3(kThreads), 0, st, cells, ptri, P, rowStart, R, pos,
+                           mp, tribits, hitCount, nullptr, nullptr);
+    HIP_TRY(hipGetLastError());
+    if ((r = exclusive_scan(ar, hitCount, hitStart, R, st))) return r;
+    uint64_t K;
+    if ((r = scan_total(hitCount, hitStart, R, K))) return r;
+    uint64_t *keys, *keys2;
+    if ((r = ar.alloc(keys, K)) || (r = ar.alloc(keys2, K))) return r;
+    if (R)
+        hipLaunchKernelGGL(row_voxels_kernel, dim3(grid_for(R)), dim3(kThreads), 0, st, cells, ptri, P, rowStart, R, pos,
+                           mp, tribits, hitCount, hitStart, keys);
+    HIP_TRY(hipGetLastError());
+    ar.release(hitCount);
+    ar.release(hitStart);
+    ar.release(cells);
+    ar.release(ptri);
+    ar.release(rowStart);
+
+    // sort (voxel, triangle)
+    if (K) {
+        size_t bytes = 0;
+        const int endBit = 3 * g.depth + tribits;
+        HIP_TRY(hipcub::DeviceRadixSort::SortKeys(nullptr, bytes, keys, keys2, K, 0, endBit, st));
+        uint8_t *tmp;
+        if ((r = ar.alloc(tmp, bytes))) return r;
+        HIP_TRY(hipcub::DeviceRadixSort::SortKeys(tmp, bytes, keys, keys2, K, 0, endBit, st));
+        ar.release(tmp);
+    }
+    ar.release(keys);
+
+    // per-voxel fold
+    uint32_t *head;
+    uint64_t *segIdx;
+    if ((r = ar.alloc(head, K)) || (r = ar.alloc(segIdx, K))) return r;
+    if (K) hipLaunchKernelGGL(seg_heads_kernel, dim3(grid_for(K)), dim3(kThreads), 0, st, keys2, K, tribits, head);
+    if ((r = exclusive_scan(ar, head, segIdx, K, st))) return r;
+    uint64_t V;
+    if ((r = scan_total(head, segIdx, K, V))) return r;
+    uint64_t *segStart, *vkey;
+    uint32_t *vword;
+    if ((r = ar.alloc(segStart, V + 1)) || (r = ar.alloc(vkey, V)) || (r = ar.alloc(vword, V))) return r;
+    if (K) hipLaunchKernelGGL(seg_starts_kernel, dim3(grid_for(K)), dim3(kThreads), 0, st, head, segIdx, K, segStart);
+    HIP_TRY(hipMemcpy(segStart + V, &K, 8, hipMemcpyHostToDevice));
+    if (V)
+        hipLaunchKernelGGL(fold_kernel, dim3(grid_for(V)), dim3(kThreads), 0, st, keys2, segStart, V, tribits, pos, nrm,
+                           col, mp, vkey, vword);
+    HIP_TRY(hipGetLastError());
+    ar.release(head);
+    ar.release(segIdx);
+    ar.release(segStart);
+    ar.release(keys2);
+    ar.release(pos);
+    ar.release(nrm);
+    ar.release(col);
+
+    uint64_t *lkey, L;
+    uint32_t *lword;
+    if ((r = finish_leaves(ar, g, vkey, vword, V, true, blockFlags, nBlocks, st, lkey, lword, L))) return r;
+    const bool multi = g.cacheLog < g.depth;
+    return emit_tree(ar, g, lkey, lword, L, multi ? blockFlags : nullptr, st, out);
+}
+
+int build_grid(const uint32_t *grid, int w, int h, int d, const svo_build_opts *opts, svo_octree &out) {
+    Geometry g{};
+    g.dataW = w;
+    g.dataH = h;
+    g.dataD = d;
+    int r = choose_cache(g, budget(opts), false);
+    if (r) return r;
+    if (g.depth > 21) return fail(SVO_ERR_LIMIT, "volume too large");
+    uint64_t nBlocks;
+    if ((r = check_blocks(g, nBlocks))) return r;
+    const uint64_t N = uint64_t(w) * h * d;
+    Arena ar;
+    HIP_TRY(hipStreamCreate(&ar.stream));
+    hipStream_t st = ar.stream;
+    uint32_t *dg, *flag;
+    uint64_t *at;
+    if ((r = ar.alloc(dg, N)) || (r = ar.alloc(flag, N)) || (r = ar.alloc(at, N))) return r;
+    HIP_TRY(hipMemcpyAsync(dg, grid, N * 4, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(grid_keys_kernel, dim3(grid_for(N)), dim3(kThreads), 0, st, dg, w, h, d, flag);
+    if ((r = exclusive_scan(ar, flag, at, N, st))) return r;
+    uint64_t V;
+    if ((r = scan_total(flag, at, N, V))) return r;
+    uint64_t *key, *key2;
+    uint32_t *word, *word2;
+    if ((r = ar.alloc(key, V)) || (r = ar.alloc(word, V)) || (r = ar.alloc(key2, V)) || (r = ar.alloc(word2, V))) return r;
+    hipLaunchKernelGGL(grid_compact_kernel, dim3(grid_for(N)), dim3(kThreads), 0, st, dg, w, h, d, flag, at, key, word);
+    HIP_TRY(hipGetLastError());
+    ar.release(dg);
+    ar.release(flag);
+    ar.release(at);
+    if (V) {
+        size_t bytes = 0;
+        HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, key, key2, word, word2, V, 0, 3 * g.depth, st));
+        uint8_t *tmp;
+        if ((r = ar.alloc(tmp, bytes))) return r;
+        HIP_TRY(hipcub::DeviceRadixSort::SortPairs(tmp, bytes, key, key2, word, word2, V, 0, 3 * g.depth, st));
+        ar.release(tmp);
+    }
+    uint32_t *blockFlags;
+    if ((r = ar.alloc(blockFlags, nBlocks)) || (r = fill32(blockFlags, nBlocks, 0, st))) return r;
+    uint64_t *lkey, L;
+    uint32_t *lword;
+    if ((r = finish_leaves(ar, g, key2, word2, V, false, blockFlags, nBlocks, st, lkey, lword, L))) return r;
+    const bool multi = g.cacheLog < g.depth;
+    return emit_tree(ar, g, lkey, lword, L, multi ? blockFlags : nullptr, st, out);
 }
 
 }  // namespace svo

@@ -25,7 +25,10 @@ void clear_error() { g_error.clear(); }
 
 static int finish_handle(std::unique_ptr<svo_octree> &h, svo_octree **out) {
     int depth = octree_validate(h->words.data(), h->words.size());
-    if (depth < 0) return fail(SVO_ERR_FORMAT, "malformed octree: a child reference leaves the word array");
+    if (depth < 0) {
+        release_device(*h);
+        return fail(SVO_ERR_FORMAT, "malformed octree: a child reference leaves the word array");
+    }
     h->depth = depth;
     *out = h.release();
     return SVO_OK;
