@@ -109,3 +109,25 @@ def test_build_errors(svo, tmp_path):
     with pytest.raises(svo.SvoError) as e:
         svo.VoxelOctree.build_from_ply(ply, 64, memory_budget=1000)
     assert e.value.code == svo.svo.ERR_LIMIT
+
+
+def test_c2_full_size_build_matches_reference(svo, golden, tmp_path):
+    """BASELINE config 2 at full size: 1,997,120-triangle displaced icosphere at
+    2048^3, 8 threads' partition, 1 GiB budget; the .oct must hash to the
+    reference builder's output for the same PLY."""
+    import hashlib
+    b = _builds(golden)["c2"]
+    v, f = svo.scenes.icosphere(b["frequency"], seed=b["seed"], displacement=b["displacement"])
+    ply = str(tmp_path / "c2.ply")
+    svo.scenes.write_ply(ply, v, f)
+    assert hashlib.md5(open(ply, "rb").read()).hexdigest() == b["ply_md5"]
+    print("[gpu] C2 full-size build 2048^3", flush=True)
+    t = svo.VoxelOctree.build_from_ply(ply, b["resolution"], threads=b["threads"], memory_budget=b["mem"])
+    w = t.words()
+    assert len(w) == b["words"]
+    st = orc.octree_stats(w)
+    assert st["desc_per_level"] == b["desc_per_level"]
+    assert st["leaves"] == b["leaves"] and st["far_words"] == b["far_words"]
+    data = t.encode()
+    assert len(data) == b["size"]
+    assert hashlib.md5(data).hexdigest() == b["md5"]
