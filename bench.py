@@ -35,6 +35,19 @@ def _pkg():
     return _import_package()
 
 
+def profiled_traffic(workload, width, height):
+    """HBM bytes per primary launch measured by rocprofv3 (FETCH_SIZE and
+    WRITE_SIZE, calibrated on a 1 GiB copy) for this workload, from the
+    committed profile summary; None when no profile covers it."""
+    path = os.path.join(ROOT, "profiles", "raymarch_primary_c2_1080p_pmc.txt")
+    if workload != "c2_icosphere2048_8x1080p" or (width, height) != (1920, 1080) or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.strip().startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":")[1])
+    return None
+
+
 def make_workload(pkg, name, cache_dir):
     if name == "dragon":
         tree = pkg.VoxelOctree(os.path.join(ROOT, "tests", "golden", "dragon.oct"))
@@ -186,7 +199,8 @@ def main():
         "data": "synthetic",
         "config": dict(config, views=V, width=W, height=H, rays_per_step=n * V, kernel="plain (one launch per view)"),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 5),
+                     "traffic": profiled_traffic(config["workload"], W, H),
                      "bytes_per_ray": round(b_ray, 2), "avg_launch_ms": round(avg_launch_s * 1e3, 4)},
         "cpu_baseline": {"value": round(cpu_rays / cpu_time / 1e9, 6), "unit": "Grays/s", "cores": args.cpu_threads,
                          "kind": "port", "sample": f"{cpu_rays} primary rays ({cpu_rays // n} of the {V} views, "
