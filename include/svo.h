@@ -130,6 +130,43 @@ int svo_render_primary_device(svo_octree *h, const svo_camera *cam, int width, i
                               uint8_t *hit, uint32_t *normal, float *t, uint64_t *leaf,
                               void *stream);
 
+/* ---- batched rendering: many views of many octrees in one launch ----------- */
+
+/* One view: an octree, a camera and its device output arrays (width*height
+ * each, row-major; any may be NULL). Rays are the primary rays of
+ * svo_render_primary with the given LOD scale (0 = none). */
+typedef struct {
+    svo_octree *tree;
+    svo_camera camera;
+    int width, height;
+    float ray_scale;
+    uint8_t *hit;
+    uint32_t *normal;
+    float *t;
+    uint64_t *leaf;
+} svo_view;
+
+typedef struct {
+    int waves_per_cu;     /* resident waves per CU for the persistent grid; 0 = 32 (LDS permitting) */
+    int refill_threshold; /* idle lanes of a wave (1..64) that trigger handing out new pixels; 0 = 16 */
+    int reserved[6];
+} svo_batch_opts;
+
+/* A renderer owns the device view table and tile-queue counters of batched
+ * launches. Launches on one renderer are ordered: a call waits on the host
+ * until the previous launch of the same renderer has finished. Use one
+ * renderer per concurrent stream. No counterpart in the reference (its viewer
+ * renders one frame with a CPU thread pool, src/Main.cpp:204-262,342-367). */
+typedef struct svo_renderer svo_renderer;
+int svo_renderer_create(int max_views, svo_renderer **out);
+void svo_renderer_free(svo_renderer *r);
+
+/* Traces every view in one persistent launch over a single queue of 8x8
+ * pixel tiles (Morton order within 64x64-pixel blocks, one queue chunk per
+ * XCD). Per-ray results equal svo_render_primary's. Enqueued on `stream`. */
+int svo_render_batch_device(svo_renderer *r, const svo_view *views, int nviews, const svo_batch_opts *opts,
+                            void *stream);
+
 /* ---- construction --------------------------------------------------------- */
 
 typedef struct {

@@ -14,6 +14,7 @@
 
 #include <cstdio>
 
+#include "march.hpp"
 #include "svo_internal.hpp"
 
 namespace svo {
@@ -27,156 +28,21 @@ namespace svo {
 
 namespace {
 
-constexpr int kMaxScale = 23;  // VoxelOctree::MaxScale, src/VoxelOctree.hpp:38
+using namespace march;
 
-__device__ __forceinline__ float smin(float a, float b) { return (b < a) ? b : a; }  // std::min
-__device__ __forceinline__ float smax(float a, float b) { return (a < b) ? b : a; }  // std::max
-__device__ __forceinline__ uint32_t f2u(float f) { return __float_as_uint(f); }
-__device__ __forceinline__ float u2f(uint32_t u) { return __uint_as_float(u); }
-
-// invSqrt, src/Util.hpp:47-58
-__device__ __forceinline__ float quake_rsqrt(float x) {
-    float half = x * 0.5f;
-    float y = u2f(0x5f3759dfu - (f2u(x) >> 1));
-    return y * (1.5f - half * y * y);
-}
-
-struct StackRef {
-    void *base;  // LDS: [slots][stride] Idx offsets, then [slots][stride] float maxT
-    int stride;
-    int slots;
-    int lane;
-};
-
-// Returns 0 miss, 1 leaf hit, 2 LOD hit, 3 stack overflow (malformed tree).
+// Runs one ray to completion. Returns 0 miss, 1 leaf hit, 2 LOD hit, 3 stack
+// overflow (malformed tree).
 template <typename Idx>
-__device__ __forceinline__ int march(const uint32_t *__restrict__ tree, float ox, float oy, float oz,
-                                     float dx, float dy, float dz, float rayScale, StackRef stk,
-                                     uint32_t &normal, float &tOut, Idx &leaf) {
-    Idx *stkOff = reinterpret_cast<Idx *>(stk.base);
-    float *stkT = reinterpret_cast<float *>(stkOff + stk.slots * stk.stride);
-
-    if (fabsf(dx) < 1e-4f) dx = 1e-4f;
-    if (fabsf(dy) < 1e-4f) dy = 1e-4f;
-    if (fabsf(dz) < 1e-4f) dz = 1e-4f;
-
-    const float dTx = 1.0f / -fabsf(dx);
-    const float dTy = 1.0f / -fabsf(dy);
-    const float dTz = 1.0f / -fabsf(dz);
-    float bTx = dTx * ox;
-    float bTy = dTy * oy;
-    float bTz = dTz * oz;
-
-    int octantMask = 7;
-    if (dx > 0.0f) { octantMask ^= 1; bTx = 3.0f * dTx - bTx; }
-    if (dy > 0.0f) { octantMask ^= 2; bTy = 3.0f * dTy - bTy; }
-    if (dz > 0.0f) { octantMask ^= 4; bTz = 3.0f * dTz - bTz; }
-
-    float minT = smax(2.0f * dTx - bTx, smax(2.0f * dTy - bTy, 2.0f * dTz - bTz));
-    float maxT = smin(dTx - bTx, smin(dTy - bTy, dTz - bTz));
-    minT = smax(minT, 0.0f);
-
-    uint32_t current = 0;
-    Idx parent = 0;
-    int idx = 0;
-    float posX = 1.0f, posY = 1.0f, posZ = 1.0f;
-    int scale = kMaxScale - 1;
-    float scaleExp2 = 0.5f;
-
-    if (1.5f * dTx - bTx > minT) { idx ^= 1; posX = 1.5f; }
-    if (1.5f * dTy - bTy > minT) { idx ^= 2; posY = 1.5f; }
-    if (1.5f * dTz - bTz > minT) { idx ^= 4; posZ = 1.5f; }
-
-    while (scale < kMaxScale) {
-        if (current == 0) current = tree[parent];
-
-        const float cornerTX = posX * dTx - bTx;
-        const float cornerTY = posY * dTy - bTy;
-        const float cornerTZ = posZ * dTz - bTz;
-        const float maxTC = smin(cornerTX, smin(cornerTY, cornerTZ));
-
-        const int childShift = idx ^ octantMask;
-        const uint32_t childMasks = current << childShift;
-
-        if ((childMasks & 0x8000u) && minT <= maxT) {
-            if (maxTC * rayScale >= scaleExp2) {
-                tOut = maxTC;
-                return 2;
-            }
-            const float maxTV = smin(maxT, maxTC);
-            const float half = scaleExp2 * 0.5f;
-            const float centerTX = half * dTx + cornerTX;
-            const float centerTY = half * dTy + cornerTY;
-            const float centerTZ = half * dTz + cornerTZ;
-
-            if (minT <= maxTV) {
-                Idx childOffset = Idx(current >> 18);
-                if (current & 0x20000u)
-                    childOffset = Idx((uint64_t(current >> 18) << 32) | uint64_t(tree[parent + 1]));
-                if (!(childMasks & 0x80u)) {
-                    leaf = childOffset + parent +
-                           Idx(__popc((((childMasks >> (8 + childShift)) << childShift) & 127u)));
-                    normal = tree[leaf];
-                    break;
-                }
-                const int slot = (kMaxScale - 1) - scale;
-                if (slot >= stk.slots) return 3;
-                stkOff[slot * stk.stride + stk.lane] = parent;
-                stkT[slot * stk.stride + stk.lane] = maxT;
-
-                const Idx siblingCount = Idx(__popc(childMasks & 127u));
-                parent += childOffset + siblingCount;
-                if (current & 0x10000u) parent += siblingCount;
-
-                idx = 0;
-                scale--;
-                scaleExp2 = half;
-                if (centerTX > minT) { idx ^= 1; posX += scaleExp2; }
-                if (centerTY > minT) { idx ^= 2; posY += scaleExp2; }
-                if (centerTZ > minT) { idx ^= 4; posZ += scaleExp2; }
-                maxT = maxTV;
-                current = 0;
-                continue;
-            }
-        }
-
-        int stepMask = 0;
-        if (cornerTX <= maxTC) { stepMask ^= 1; posX -= scaleExp2; }
-        if (cornerTY <= maxTC) { stepMask ^= 2; posY -= scaleExp2; }
-        if (cornerTZ <= maxTC) { stepMask ^= 4; posZ -= scaleExp2; }
-
-        minT = maxTC;
-        idx ^= stepMask;
-
-        if ((idx & stepMask) != 0) {
-            uint32_t diff = 0;
-            if (stepMask & 1) diff |= f2u(posX) ^ f2u(posX + scaleExp2);
-            if (stepMask & 2) diff |= f2u(posY) ^ f2u(posY + scaleExp2);
-            if (stepMask & 4) diff |= f2u(posZ) ^ f2u(posZ + scaleExp2);
-            scale = int(f2u(float(int(diff))) >> 23) - 127;
-            scaleExp2 = u2f(uint32_t(scale - kMaxScale + 127) << 23);
-            // the reference reads rayStack[scale] here even when scale >= 23 and
-            // then leaves the loop; that read cannot reach the result
-            if (scale >= kMaxScale) break;
-            const int slot = (kMaxScale - 1) - scale;
-            if (slot >= stk.slots) return 3;
-            parent = stkOff[slot * stk.stride + stk.lane];
-            maxT = stkT[slot * stk.stride + stk.lane];
-
-            const uint32_t shX = f2u(posX) >> scale;
-            const uint32_t shY = f2u(posY) >> scale;
-            const uint32_t shZ = f2u(posZ) >> scale;
-            posX = u2f(shX << scale);
-            posY = u2f(shY << scale);
-            posZ = u2f(shZ << scale);
-            idx = int((shX & 1u) | ((shY & 1u) << 1) | ((shZ & 1u) << 2));
-            current = 0;
-        }
-    }
-
-    if (scale >= kMaxScale) return 0;
-    tOut = minT;
-    return 1;
+__device__ __forceinline__ int march_ray(const uint32_t *__restrict__ tree, float ox, float oy, float oz, float dx,
+                                         float dy, float dz, float rayScale, const Stack<Idx> &stk, uint32_t &normal,
+                                         float &tOut, Idx &leaf) {
+    Ray<Idx> r;
+    ray_init(r, ox, oy, oz, dx, dy, dz, rayScale);
+    int code;
+    do {
+        code = ray_step(r, tree, stk, normal, tOut, leaf);
+    } while (code == kRunning);
+    return code;
 }
 
 template <typename Idx>
@@ -197,13 +63,13 @@ __global__ void __launch_bounds__(256) rays_kernel(const uint32_t *__restrict__ 
     extern __shared__ __align__(16) unsigned char lds[];
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    StackRef stk{lds, int(blockDim.x), slots, int(threadIdx.x)};
+    Stack<Idx> stk(lds, int(blockDim.x), slots, int(threadIdx.x));
     uint32_t normal = 0;
     float t = 0.0f;
     Idx leaf = 0;
     const float scale = rs ? rs[i] : rs0;
-    int code = march<Idx>(tree, o[3 * i], o[3 * i + 1], o[3 * i + 2], d[3 * i], d[3 * i + 1], d[3 * i + 2],
-                          scale, stk, normal, t, leaf);
+    int code = march_ray<Idx>(tree, o[3 * i], o[3 * i + 1], o[3 * i + 2], d[3 * i], d[3 * i + 1], d[3 * i + 2],
+                              scale, stk, normal, t, leaf);
     store_result<Idx>(code, normal, t, leaf, i, hit, nrm, tt, lf);
 }
 
@@ -233,11 +99,11 @@ __global__ void __launch_bounds__(64) primary_kernel(const uint32_t *__restrict_
     const float vy = dx * p.a21 + dy * p.a22 + p.zy;
     const float vz = dx * p.a31 + dy * p.a32 + p.zz;
     const float s = quake_rsqrt(vx * vx + vy * vy + vz * vz);
-    StackRef stk{lds, 64, slots, lane};
+    Stack<Idx> stk(lds, 64, slots, lane);
     uint32_t normal = 0;
     float t = 0.0f;
     Idx leaf = 0;
-    int code = march<Idx>(tree, p.px, p.py, p.pz, vx * s, vy * s, vz * s, 0.0f, stk, normal, t, leaf);
+    int code = march_ray<Idx>(tree, p.px, p.py, p.pz, vx * s, vy * s, vz * s, 0.0f, stk, normal, t, leaf);
     store_result<Idx>(code, normal, t, leaf, int64_t(y) * p.width + x, hit, nrm, tt, lf);
 }
 

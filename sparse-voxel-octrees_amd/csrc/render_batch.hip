@@ -1,0 +1,319 @@
+// Batched persistent renderer: many views of many resident octrees in one
+// launch.
+//
+// Work is a single queue of 8x8-pixel tiles over all views. Tiles of a view
+// are numbered in 8x8-tile blocks (64x64 pixels) with a Morton order inside
+// each block, so neighbouring queue entries trace neighbouring rays through
+// the same octree nodes. The queue is split into 8 contiguous chunks, one per
+// XCD (workgroup i runs on XCD i % 8), so each XCD's L2 mostly sees one region;
+// a wave whose chunk is exhausted steals from the next chunks in turn and
+// exits after all eight are empty, so every wave terminates.
+//
+// Each wave owns one tile at a time and hands its pixels to idle lanes. A
+// lane that finishes a ray becomes idle; once `refill` lanes are idle the
+// wave pulls the next pixels (and, when the tile is used up, the next tile)
+// for them, so SIMD lanes stay busy while the rays of a tile finish at very
+// different depths. Results per ray are those of VoxelOctree::raymarch
+// (src/VoxelOctree.cpp:207-346): the traversal code is march.hpp.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "march.hpp"
+#include "svo_internal.hpp"
+
+namespace svo {
+
+#define HIP_TRY(expr)                                                                       \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(SVO_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+namespace {
+
+using namespace march;
+
+constexpr int kQueues = 8;             // one tile-queue chunk per XCD
+constexpr int kCounterStride = 64;     // 256 B between queue counters
+constexpr uint64_t kNoTile = ~uint64_t(0);
+
+struct ViewDesc {
+    const uint32_t *tree;
+    uint8_t *hit;
+    uint32_t *normal;
+    float *t;
+    uint64_t *leaf;
+    float a11, a12, a21, a22, a31, a32;  // camera rotation columns used by ray generation
+    float zx, zy, zz;                    // planeDist * third column
+    float px, py, pz;                    // ray origin
+    float aspect, scale, rayScale;       // H/W, 2/W, LOD scale
+    int width, height, tilesX, tilesY, blocksX, pad;
+};
+
+__device__ __forceinline__ int view_of(const uint64_t *__restrict__ tileBase, int nviews, uint64_t tile) {
+    int lo = 0, hi = nviews;  // tileBase[lo] <= tile < tileBase[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (tileBase[mid] <= tile) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ uint32_t even_bits(uint32_t m) {  // bits 0,2,4 -> 0,1,2
+    return (m & 1u) | ((m >> 1) & 2u) | ((m >> 2) & 4u);
+}
+
+template <typename Idx>
+__global__ void __launch_bounds__(256) batch_kernel(const ViewDesc *__restrict__ views,
+                                                    const uint64_t *__restrict__ tileBase, int nviews,
+                                                    uint32_t *__restrict__ counters, int slots, int refill) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    Stack<Idx> stk(lds + size_t(wave) * slots * 64 * (sizeof(Idx) + sizeof(float)), 64, slots, lane);
+    const uint64_t total = tileBase[nviews];
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+
+    int queue = blockIdx.x & (kQueues - 1);
+    int tried = 0;
+    bool queueDone = false;
+    int pix = 64;                 // next pixel of the current tile (wave-uniform)
+    int tView = 0, tx = 0, ty = 0;
+
+    bool active = false;
+    Ray<Idx> r;
+    const uint32_t *tree = nullptr;
+    int rView = 0;
+    int64_t rPix = 0;
+
+    for (;;) {
+        // ---- hand pixels to idle lanes ----
+        uint64_t idle = __ballot(!active);
+        if (!queueDone && (__popcll(idle) >= refill || idle == ~0ull)) {
+            while (idle && !queueDone) {
+                if (pix >= 64) {
+                    uint64_t tile = kNoTile;
+                    while (tried < kQueues) {
+                        const uint64_t lo = total * uint64_t(queue) / kQueues;
+                        const uint64_t hi = total * uint64_t(queue + 1) / kQueues;
+                        uint32_t k = 0;
+                        if (lane == 0) k = atomicAdd(&counters[queue * kCounterStride], 1u);
+                        k = __shfl(k, 0);
+                        if (lo + k < hi) {
+                            tile = lo + k;
+                            break;
+                        }
+                        queue = (queue + 1) & (kQueues - 1);
+                        tried++;
+                    }
+                    if (tile == kNoTile) {
+                        queueDone = true;
+                        break;
+                    }
+                    tView = view_of(tileBase, nviews, tile);
+                    const ViewDesc &vd = views[tView];
+                    const uint64_t local = tile - tileBase[tView];
+                    const uint32_t m = uint32_t(local & 63);
+                    const uint64_t block = local >> 6;
+                    tx = int(block % uint64_t(vd.blocksX)) * 8 + int(even_bits(m));
+                    ty = int(block / uint64_t(vd.blocksX)) * 8 + int(even_bits(m >> 1));
+                    pix = (tx < vd.tilesX && ty < vd.tilesY) ? 0 : 64;  // padding tiles hold no pixels
+                    continue;
+                }
+                const int avail = 64 - pix;
+                const int nIdle = __popcll(idle);
+                const int take = nIdle < avail ? nIdle : avail;
+                const int rank = __popcll(idle & below);
+                const bool mine = ((idle >> lane) & 1ull) && rank < take;
+                if (mine) {
+                    const ViewDesc &vd = views[tView];
+                    const int p = pix + rank;
+                    const int x = tx * 8 + (p & 7);
+                    const int y = ty * 8 + (p >> 3);
+                    if (x < vd.width && y < vd.height) {
+                        // ray generation, src/Main.cpp:97-113 with closed-form dx, dy
+                        const float dy = vd.aspect - float(y) * vd.scale;
+                        const float dx = -1.0f + float(x) * vd.scale;
+                        const float vx = dx * vd.a11 + dy * vd.a12 + vd.zx;
+                        const float vy = dx * vd.a21 + dy * vd.a22 + vd.zy;
+                        const float vz = dx * vd.a31 + dy * vd.a32 + vd.zz;
+                        const float s = quake_rsqrt(vx * vx + vy * vy + vz * vz);
+                        ray_init(r, vd.px, vd.py, vd.pz, vx * s, vy * s, vz * s, vd.rayScale);
+                        tree = vd.tree;
+                        rView = tView;
+                        rPix = int64_t(y) * vd.width + x;
+                        active = true;
+                    }
+                }
+                // the first `take` idle lanes are consumed (edge pixels outside the image included)
+                const uint64_t consumed = __ballot(mine);
+                idle &= ~consumed;
+                pix += take;
+            }
+        }
+        if (!__any(active)) {
+            if (queueDone) break;
+            continue;
+        }
+        // ---- walk until enough lanes are idle ----
+        for (;;) {
+            if (active) {
+                uint32_t normal = 0;
+                float t = 0.0f;
+                Idx leaf = 0;
+                int code = ray_step(r, tree, stk, normal, t, leaf);
+                if (code != kRunning) {
+                    if (code == kBad) code = kMiss;
+                    const ViewDesc &vd = views[rView];
+                    if (vd.hit) vd.hit[rPix] = uint8_t(code);
+                    if (vd.normal) vd.normal[rPix] = code == kLeaf ? normal : 0u;
+                    if (vd.t) vd.t[rPix] = code == kMiss ? 0.0f : t;
+                    if (vd.leaf) vd.leaf[rPix] = code == kLeaf ? uint64_t(leaf) : SVO_LEAF_NONE;
+                    active = false;
+                }
+            }
+            const uint64_t idleNow = __ballot(!active);
+            if (idleNow == ~0ull) break;
+            if (!queueDone && __popcll(idleNow) >= refill) break;
+        }
+    }
+}
+
+__global__ void zero_counters_kernel(uint32_t *c) {
+    if (threadIdx.x < kQueues) c[threadIdx.x * kCounterStride] = 0u;
+}
+
+}  // namespace
+
+}  // namespace svo
+
+struct svo_renderer {
+    int maxViews = 0;
+    svo::ViewDesc *dViews = nullptr;
+    uint64_t *dTileBase = nullptr;
+    uint32_t *dCounters = nullptr;
+    svo::ViewDesc *hViews = nullptr;  // pinned staging
+    uint64_t *hTileBase = nullptr;  // pinned staging
+    hipEvent_t done = nullptr;      // last launch finished reading the staging copies
+    int cus = 0;
+    std::mutex mu;
+};
+
+namespace svo {
+
+int renderer_create(int maxViews, svo_renderer **out) {
+    if (maxViews <= 0 || maxViews > (1 << 20)) return fail(SVO_ERR_INVALID, "max_views out of range");
+    svo_renderer *r = new (std::nothrow) svo_renderer();
+    if (!r) return fail(SVO_ERR_NOMEM, "out of host memory");
+    r->maxViews = maxViews;
+    hipError_t e = hipMalloc(&r->dViews, sizeof(ViewDesc) * maxViews);
+    if (e == hipSuccess) e = hipMalloc(&r->dTileBase, sizeof(uint64_t) * (maxViews + 1));
+    if (e == hipSuccess) e = hipMalloc(&r->dCounters, sizeof(uint32_t) * kQueues * kCounterStride);
+    if (e == hipSuccess) e = hipHostMalloc(&r->hViews, sizeof(ViewDesc) * maxViews, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc(&r->hTileBase, sizeof(uint64_t) * (maxViews + 1), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&r->done, hipEventDisableTiming);
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (e == hipSuccess) e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipGetDeviceProperties(&prop, dev);
+    if (e != hipSuccess) {
+        renderer_free(r);
+        return fail(SVO_ERR_DEVICE, std::string("renderer: ") + hipGetErrorString(e));
+    }
+    r->cus = prop.multiProcessorCount;
+    *out = r;
+    return SVO_OK;
+}
+
+void renderer_free(svo_renderer *r) {
+    if (!r) return;
+    if (r->done) {
+        (void)hipEventSynchronize(r->done);
+        (void)hipEventDestroy(r->done);
+    }
+    if (r->dViews) (void)hipFree(r->dViews);
+    if (r->dTileBase) (void)hipFree(r->dTileBase);
+    if (r->dCounters) (void)hipFree(r->dCounters);
+    if (r->hViews) (void)hipHostFree(r->hViews);
+    if (r->hTileBase) (void)hipHostFree(r->hTileBase);
+    delete r;
+}
+
+int render_batch(svo_renderer *rd, const svo_view *views, int nviews, const svo_batch_opts *opts, void *stream) {
+    if (!rd || !views) return fail(SVO_ERR_INVALID, "null argument");
+    if (nviews <= 0 || nviews > rd->maxViews) return fail(SVO_ERR_LIMIT, "view count exceeds the renderer's max_views");
+    std::lock_guard<std::mutex> lock(rd->mu);
+    // the previous launch must be done with the staging buffers before they are rewritten
+    HIP_TRY(hipEventSynchronize(rd->done));
+    int maxDepth = 1;
+    bool wide = false;
+    uint64_t tiles = 0;
+    const float pi = float(3.14159265358979323846);
+    const float planeDist = 1.0f / std::tan(pi / 6.0f);  // src/Main.cpp:158
+    for (int i = 0; i < nviews; ++i) {
+        const svo_view &v = views[i];
+        if (!v.tree) return fail(SVO_ERR_INVALID, "view without an octree");
+        if (v.width <= 0 || v.height <= 0 || v.width > 65536 || v.height > 65536)
+            return fail(SVO_ERR_INVALID, "view size out of range");
+        int r = ensure_device(*v.tree);
+        if (r) return r;
+        maxDepth = std::max(maxDepth, v.tree->depth);
+        wide = wide || v.tree->words.size() >= (uint64_t(1) << 32);
+        ViewDesc &d = rd->hViews[i];
+        d.tree = v.tree->dwords;
+        d.hit = v.hit;
+        d.normal = v.normal;
+        d.t = v.t;
+        d.leaf = v.leaf;
+        const float *tf = v.camera.tform;
+        d.a11 = tf[0]; d.a12 = tf[1]; d.a21 = tf[4]; d.a22 = tf[5]; d.a31 = tf[8]; d.a32 = tf[9];
+        d.zx = planeDist * tf[2];
+        d.zy = planeDist * tf[6];
+        d.zz = planeDist * tf[10];
+        d.px = v.camera.pos[0]; d.py = v.camera.pos[1]; d.pz = v.camera.pos[2];
+        d.aspect = float(v.height) / float(v.width);
+        d.scale = 2.0f / float(v.width);
+        d.rayScale = v.ray_scale;
+        d.width = v.width;
+        d.height = v.height;
+        d.tilesX = (v.width + 7) / 8;
+        d.tilesY = (v.height + 7) / 8;
+        d.blocksX = (d.tilesX + 7) / 8;
+        d.pad = 0;
+        const int blocksY = (d.tilesY + 7) / 8;
+        rd->hTileBase[i] = tiles;
+        tiles += uint64_t(d.blocksX) * uint64_t(blocksY) * 64;
+    }
+    rd->hTileBase[nviews] = tiles;
+    if (tiles >= (uint64_t(kQueues) << 32)) return fail(SVO_ERR_LIMIT, "too many tiles for one launch");
+    const int slots = maxDepth > 2 ? maxDepth - 1 : 1;
+    const size_t ldsPerWave = size_t(slots) * 64 * ((wide ? 8 : 4) + sizeof(float));
+    const size_t ldsPerBlock = ldsPerWave * 4;
+    const size_t ldsPerCU = 160 * 1024;
+    int wavesPerCU = (opts && opts->waves_per_cu > 0) ? opts->waves_per_cu : 32;
+    int blocksPerCU = std::max(1, std::min(wavesPerCU / 4, int(ldsPerCU / ldsPerBlock)));
+    if (ldsPerBlock > 64 * 1024) return fail(SVO_ERR_LIMIT, "octree too deep for the LDS stack");
+    int refill = (opts && opts->refill_threshold > 0) ? std::min(opts->refill_threshold, 64) : 16;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    HIP_TRY(hipMemcpyAsync(rd->dViews, rd->hViews, sizeof(ViewDesc) * nviews, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(rd->dTileBase, rd->hTileBase, sizeof(uint64_t) * (nviews + 1), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, s, rd->dCounters);
+    const unsigned grid = unsigned(rd->cus * blocksPerCU);
+    if (!wide)
+        hipLaunchKernelGGL(batch_kernel<uint32_t>, dim3(grid), dim3(256), ldsPerBlock, s, rd->dViews, rd->dTileBase,
+                           nviews, rd->dCounters, slots, refill);
+    else
+        hipLaunchKernelGGL(batch_kernel<uint64_t>, dim3(grid), dim3(256), ldsPerBlock, s, rd->dViews, rd->dTileBase,
+                           nviews, rd->dCounters, slots, refill);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(rd->done, s));
+    return SVO_OK;
+}
+
+}  // namespace svo

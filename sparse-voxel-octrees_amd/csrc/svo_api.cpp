@@ -303,6 +303,23 @@ int svo_render_primary(svo_octree *h, const svo_camera *cam, int width, int heig
     return SVO_OK;
 }
 
+// ---- batched rendering ----------------------------------------------------------
+
+int svo_renderer_create(int max_views, svo_renderer **out) {
+    clear_error();
+    if (!out) return fail(SVO_ERR_INVALID, "null argument");
+    *out = nullptr;
+    return renderer_create(max_views, out);
+}
+
+void svo_renderer_free(svo_renderer *r) { renderer_free(r); }
+
+int svo_render_batch_device(svo_renderer *r, const svo_view *views, int nviews, const svo_batch_opts *opts,
+                            void *stream) {
+    clear_error();
+    return render_batch(r, views, nviews, opts, stream);
+}
+
 // ---- construction -------------------------------------------------------------
 
 int svo_build_from_mesh(const float *verts, int64_t nverts, int has_normals, const int32_t *face_counts,

@@ -15,6 +15,8 @@ struct svo_octree {
     uint32_t *dwords = nullptr;   // device copy of `words`
 };
 
+struct svo_renderer;  // render_batch.hip
+
 namespace svo {
 
 // thread-local error message; returns `code` for convenient `return fail(...)`
@@ -60,6 +62,11 @@ int raymarch_device(svo_octree &h, int64_t n, const float *o, const float *d, co
 int render_primary_device(svo_octree &h, const svo_camera &cam, int width, int height, uint8_t *hit,
                           uint32_t *normal, float *t, uint64_t *leaf, void *stream);
 int device_count();
+
+// ---- batched persistent renderer (render_batch.hip) ----
+int renderer_create(int maxViews, svo_renderer **out);
+void renderer_free(svo_renderer *r);
+int render_batch(svo_renderer *r, const svo_view *views, int nviews, const svo_batch_opts *opts, void *stream);
 
 // ---- construction (build.hip / build_host.cpp) ----
 int build_mesh(const float *verts, int64_t nverts, int has_normals, const int32_t *face_counts,

@@ -28,6 +28,7 @@ EXPORTS = [
     "svo_depth", "svo_free", "svo_upload", "svo_raymarch", "svo_raymarch_batch", "svo_raymarch_batch_device",
     "svo_orbit_camera", "svo_render_primary", "svo_render_primary_device", "svo_build_from_ply",
     "svo_build_from_mesh", "svo_build_from_voxels", "svo_build_from_grid",
+    "svo_renderer_create", "svo_renderer_free", "svo_render_batch_device",
 ]
 
 
@@ -39,6 +40,16 @@ class SvoError(RuntimeError):
 
 class Camera(ctypes.Structure):
     _fields_ = [("tform", ctypes.c_float * 16), ("pos", ctypes.c_float * 3)]
+
+
+class View(ctypes.Structure):
+    _fields_ = [("tree", ctypes.c_void_p), ("camera", Camera), ("width", ctypes.c_int), ("height", ctypes.c_int),
+                ("ray_scale", ctypes.c_float), ("hit", ctypes.c_void_p), ("normal", ctypes.c_void_p),
+                ("t", ctypes.c_void_p), ("leaf", ctypes.c_void_p)]
+
+
+class BatchOpts(ctypes.Structure):
+    _fields_ = [("waves_per_cu", ctypes.c_int), ("refill_threshold", ctypes.c_int), ("reserved", ctypes.c_int * 6)]
 
 
 class BuildOpts(ctypes.Structure):
@@ -66,11 +77,13 @@ def _load():
         "svo_build_from_mesh": [P, I64, ctypes.c_int, P, P, I64, ctypes.c_int, P, P],
         "svo_build_from_voxels": [ctypes.c_char_p, P, P],
         "svo_build_from_grid": [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P],
+        "svo_renderer_create": [ctypes.c_int, P], "svo_renderer_free": [P],
+        "svo_render_batch_device": [P, P, ctypes.c_int, P, P],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
         fn.argtypes = args
-        fn.restype = None if name in ("svo_free", "svo_free_buffer") else ctypes.c_int
+        fn.restype = None if name in ("svo_free", "svo_free_buffer", "svo_renderer_free") else ctypes.c_int
     return L
 
 
@@ -274,3 +287,38 @@ class VoxelOctree:
                               stream=None):
         _check(lib.svo_render_primary_device(self._h, ctypes.byref(cam), int(width), int(height), hit_ptr, normal_ptr,
                                              t_ptr, leaf_ptr, stream))
+
+
+class Renderer:
+    """Batched persistent renderer (svo_renderer): many views of many octrees
+    per launch. Views reference device output arrays (pointers as ints)."""
+
+    def __init__(self, max_views=256):
+        self._r = ctypes.c_void_p()
+        _check(lib.svo_renderer_create(int(max_views), ctypes.byref(self._r)))
+        self._keep = []
+
+    def __del__(self):
+        self.close()
+
+    def close(self):
+        if getattr(self, "_r", None) and self._r.value:
+            lib.svo_renderer_free(self._r)
+            self._r = ctypes.c_void_p()
+
+    @staticmethod
+    def view(tree, camera, width, height, hit=None, normal=None, t=None, leaf=None, ray_scale=0.0):
+        v = View()
+        v.tree = tree.handle.value
+        v.camera = camera
+        v.width, v.height = int(width), int(height)
+        v.ray_scale = float(ray_scale)
+        v.hit, v.normal, v.t, v.leaf = hit, normal, t, leaf
+        return v
+
+    def render_device(self, views, stream=None, waves_per_cu=0, refill_threshold=0):
+        arr = (View * len(views))(*views)
+        o = BatchOpts()
+        o.waves_per_cu = int(waves_per_cu)
+        o.refill_threshold = int(refill_threshold)
+        _check(lib.svo_render_batch_device(self._r, arr, len(views), ctypes.byref(o), stream))
