@@ -57,6 +57,14 @@ class BuildOpts(ctypes.Structure):
 
 
 def _load():
+    # torch's wheel ships its own libamdhip64.so.7 and only initialises with it;
+    # both libraries carry that soname, so whichever loads first serves the
+    # whole process. Load torch's first when it is installed, so device
+    # buffers and streams from torch and from this library share one runtime.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is missing: build it with `make -C sparse-voxel-octrees_amd` "
                           "or __graft_entry__.build()")
