@@ -74,6 +74,79 @@ def make_workload(pkg, name, cache_dir):
     raise SystemExit(f"unknown workload {name}")
 
 
+def run_mixed(pkg, torch, stream, steps, warmup, waves_per_cu, refill):
+    """64 GPU-built octrees (128^3..2048^3, SDF scenes and
+    displaced meshes) and 256 views (320x240..1920x1080): 256 plain per-view
+    launches vs one batched persistent launch, same rays."""
+    t0 = time.time()
+    trees = [pkg.scenes.build_mixed_octree(pkg, sp) for sp in pkg.scenes.mixed_octrees()]
+    build_s = time.time() - t0
+    for t in trees:
+        t.upload()
+    specs = pkg.scenes.mixed_views()
+    dev = torch.device("cuda", 0)
+    sptr = stream.cuda_stream
+    outs, cams = [], []
+    for v in specs:
+        n = v["width"] * v["height"]
+        outs.append((torch.empty(n, dtype=torch.int32, device=dev), torch.empty(n, dtype=torch.float32, device=dev),
+                     torch.empty(n, dtype=torch.int64, device=dev)))
+        cams.append(trees[v["tree"]].orbit_camera(v["pitch"], v["yaw"], 1.0))
+    rays = sum(v["width"] * v["height"] for v in specs)
+    ren = pkg.svo.Renderer(len(specs))
+    bviews = [ren.view(trees[v["tree"]], cams[i], v["width"], v["height"], None, outs[i][0].data_ptr(),
+                       outs[i][1].data_ptr(), outs[i][2].data_ptr()) for i, v in enumerate(specs)]
+
+    def plain():
+        for i, v in enumerate(specs):
+            trees[v["tree"]].render_primary_device(cams[i], v["width"], v["height"], None, outs[i][0].data_ptr(),
+                                                   outs[i][1].data_ptr(), outs[i][2].data_ptr(), sptr)
+
+    def batched(w=waves_per_cu, r=refill):
+        ren.render_device(bviews, sptr, w, r)
+
+    def timed(fn, k):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(k):
+            fn()
+        b.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / k
+
+    # tile-queue launch parameters: pick the fastest of a small grid on this workload
+    tune = {}
+    if not waves_per_cu and not refill:
+        for w in (16, 24, 32):
+            for r in (8, 16, 32):
+                tune[(w, r)] = timed(lambda: batched(w, r), 1)
+        (waves_per_cu, refill) = min(tune, key=tune.get)
+    plain_ms = timed(plain, steps)
+    batched_ms = timed(lambda: batched(waves_per_cu, refill), steps)
+    # parity: the batched launch reproduces the plain launches bit for bit
+    plain()
+    torch.cuda.synchronize()
+    snap = [(a.clone(), b.clone(), c.clone()) for a, b, c in outs]
+    for a, b, c in outs:
+        a.fill_(-1)
+        b.fill_(-1.0)
+        c.fill_(-1)
+    batched(waves_per_cu, refill)
+    torch.cuda.synchronize()
+    mism = sum(int((a != sa).sum() + (b.view(torch.int32) != sb.view(torch.int32)).sum() + (c != sc).sum())
+               for (a, b, c), (sa, sb, sc) in zip(outs, snap))
+    return {"workload": "c4_64octrees_256views", "rays_per_step": rays, "octree_build_seconds": round(build_s, 2),
+            "plain_grays_s": round(rays / (plain_ms / 1e3) / 1e9, 4), "plain_ms_per_step": round(plain_ms, 3),
+            "batched_grays_s": round(rays / (batched_ms / 1e3) / 1e9, 4), "batched_ms_per_step": round(batched_ms, 3),
+            "batched_vs_plain": round(plain_ms / batched_ms, 3),
+            "launch": {"waves_per_cu": waves_per_cu, "refill": refill,
+                       "tuned_ms": {f"{w}x{r}": round(t, 3) for (w, r), t in tune.items()}},
+            "mismatched_fields_vs_plain": mism}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -90,6 +163,7 @@ def main():
     ap.add_argument("--cache", default=os.path.join(ROOT, "gpurun_out", "bench_cache"))
     ap.add_argument("--waves-per-cu", type=int, default=0, help="batched form: persistent waves per CU (0 = default)")
     ap.add_argument("--refill", type=int, default=0, help="batched form: idle lanes that trigger a refill (0 = default)")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the 64-octree / 256-view batched workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (no multi-GPU path); run with --gpus 1",
@@ -186,6 +260,10 @@ def main():
               flush=True)
         return
 
+    mixed = None
+    if not args.no_mixed:
+        mixed = run_mixed(pkg, torch, stream, max(1, min(args.steps, 5)), 1, args.waves_per_cu, args.refill)
+
     # batched outputs must equal the plain kernel's: snapshot plain, run batched, compare
     step()
     torch.cuda.synchronize()
@@ -247,12 +325,14 @@ def main():
                          "kind": "port", "sample": f"{cpu_rays} primary rays ({cpu_rays // n} of the {V} views, "
                                                    f"full {W}x{H}) through the oracle raymarch"},
         "batched": batched,
+        "mixed_batched": mixed,
         "parity": {"rays_checked": cpu_rays, "mismatched_fields": mismatches,
                    "note": "plain-kernel outputs vs the oracle on every ray; batched outputs vs plain below"},
         "wall_s": round(wall, 3),
     }
     print(json.dumps(out), flush=True)
-    if mismatches or batched_mismatch or config.get("build_oct_md5_matches_reference") is False:
+    if mismatches or batched_mismatch or config.get("build_oct_md5_matches_reference") is False or \
+            (mixed and mixed["mismatched_fields_vs_plain"]):
         sys.exit(3)
 
 

@@ -197,6 +197,30 @@ int svo_build_from_voxels(const char *path, const svo_build_opts *opts, svo_octr
 int svo_build_from_grid(const uint32_t *grid, int w, int h, int d, const svo_build_opts *opts,
                         svo_octree **out);
 
+/* ---- procedural sources ----------------------------------------------------
+ * The reference builds non-mesh scenes only through a raw .voxel file
+ * (VoxelData(path), src/VoxelData.cpp:37-48,177-209). These sources generate
+ * the same kind of volume on the GPU and build it with svo_build_from_grid's
+ * semantics, without a host volume. */
+
+/* Sphere (kind 0: center, radius r1) or torus (kind 1: center, major radius
+ * r1, minor radius r2, unit axis), in the unit cube. */
+typedef struct {
+    int kind;
+    float center[3];
+    float r1, r2;
+    float axis[3];
+} svo_sdf_primitive;
+
+/* Solid-shell voxelization of the union of the primitives on an n^3 grid
+ * (n a power of two): voxel (x,y,z) with centre c = (v + 0.5)/n is occupied
+ * when |sdf(c)| <= 0.5*sqrt(3)/n; its word is compressMaterial(normalised
+ * central-difference gradient (step 1/n), shade), 0 replaced by 1. FP32. */
+int svo_build_from_sdf(const svo_sdf_primitive *prims, int count, int resolution, float shade,
+                       const svo_build_opts *opts, svo_octree **out);
+/* The same voxels as a dense host volume (n^3 uint32, x fastest; n <= 1024). */
+int svo_sdf_volume(const svo_sdf_primitive *prims, int count, int resolution, float shade, uint32_t *grid);
+
 #ifdef __cplusplus
 }
 #endif

@@ -724,6 +724,108 @@ __global__ void fill_u32_kernel(uint32_t *p, uint64_t n, uint32_t v) {
 }
 
 // ---------------------------------------------------------------------------
+// signed-distance scene source (procedural replacement of the raw .voxel
+// route for SDF scenes; occupancy/normal rule of the workload definition)
+// ---------------------------------------------------------------------------
+
+struct SdfPrim {
+    int kind;  // 0 sphere (radius r1), 1 torus (major r1, minor r2, unit axis)
+    float cx, cy, cz, r1, r2, ax, ay, az;
+};
+
+constexpr int kMaxSdfPrims = 64;
+
+struct SdfScene {
+    SdfPrim p[kMaxSdfPrims];
+    int count;
+    int n;         // voxels per axis
+    float thr;     // |sdf| <= thr is occupied: 0.5*sqrt(3)/n
+    float h;       // central-difference step 1/n
+    float shade;
+};
+
+__device__ __forceinline__ float sdf_eval(const SdfScene &sc, float x, float y, float z) {
+    float d = INFINITY;
+    for (int i = 0; i < sc.count; ++i) {
+        const SdfPrim &q = sc.p[i];
+        const float qx = x - q.cx, qy = y - q.cy, qz = z - q.cz;
+        const float qq = qx * qx + qy * qy + qz * qz;
+        float di;
+        if (q.kind == 0) {
+            di = sqrtf(qq) - q.r1;
+        } else {
+            const float hh = qx * q.ax + qy * q.ay + qz * q.az;
+            const float rad = sqrtf(fmaxf(qq - hh * hh, 0.0f)) - q.r1;
+            di = sqrtf(rad * rad + hh * hh) - q.r2;
+        }
+        d = fminf(d, di);
+    }
+    return d;
+}
+
+__device__ __forceinline__ float vox_center(int v, int n) { return (float(v) + 0.5f) / float(n); }
+
+// Bricks of 8^3 voxels whose centre lies farther from the surface than the
+// brick's half diagonal plus the shell width (and a rounding margin) hold no
+// occupied voxel: the union of exact sphere/torus distances is 1-Lipschitz.
+__global__ void sdf_bricks_kernel(SdfScene sc, uint64_t nBricks, int bricksPerAxis, uint32_t *__restrict__ list,
+                                  uint32_t *__restrict__ count) {
+    uint64_t b = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (b >= nBricks) return;
+    const int bx = int(b % uint64_t(bricksPerAxis));
+    const int by = int((b / uint64_t(bricksPerAxis)) % uint64_t(bricksPerAxis));
+    const int bz = int(b / (uint64_t(bricksPerAxis) * bricksPerAxis));
+    const float inv = 1.0f / float(sc.n);
+    const float d = sdf_eval(sc, (bx * 8 + 4) * inv, (by * 8 + 4) * inv, (bz * 8 + 4) * inv);
+    const float reach = (4.0f * 1.7320508f + 2.0f) * inv + sc.thr;
+    if (fabsf(d) <= reach) list[atomicAdd(count, 1u)] = uint32_t(b);
+}
+
+// One wave per candidate brick, 8 voxels per lane. Emits (morton key, word)
+// for occupied voxels; normal = normalised central-difference gradient,
+// packed with compressMaterial (src/Util.hpp:64-84); a zero word becomes 1.
+__global__ void __launch_bounds__(64) sdf_voxels_kernel(SdfScene sc, const uint32_t *__restrict__ list, int bricksPerAxis,
+                                                        uint64_t *__restrict__ key, uint32_t *__restrict__ word,
+                                                        uint32_t *__restrict__ count, uint64_t cap,
+                                                        uint32_t *__restrict__ overflow) {
+    const uint32_t b = list[blockIdx.x];
+    const int bx = int(b % uint32_t(bricksPerAxis));
+    const int by = int((b / uint32_t(bricksPerAxis)) % uint32_t(bricksPerAxis));
+    const int bz = int(b / (uint32_t(bricksPerAxis) * bricksPerAxis));
+    for (int k = 0; k < 8; ++k) {
+        const int local = threadIdx.x + 64 * k;  // 0..511
+        const int x = bx * 8 + (local & 7), y = by * 8 + ((local >> 3) & 7), z = bz * 8 + (local >> 6);
+        const float px = vox_center(x, sc.n), py = vox_center(y, sc.n), pz = vox_center(z, sc.n);
+        const float d = sdf_eval(sc, px, py, pz);
+        if (!(fabsf(d) <= sc.thr)) continue;
+        float g[3];
+        g[0] = sdf_eval(sc, px + sc.h, py, pz) - sdf_eval(sc, px - sc.h, py, pz);
+        g[1] = sdf_eval(sc, px, py + sc.h, pz) - sdf_eval(sc, px, py - sc.h, pz);
+        g[2] = sdf_eval(sc, px, py, pz + sc.h) - sdf_eval(sc, px, py, pz - sc.h);
+        float nrm[3];
+        normalize3(g, nrm);
+        uint32_t w = compress_material(nrm, sc.shade);
+        if (w == 0) w = 1;
+        const uint32_t at = atomicAdd(count, 1u);
+        if (at >= cap) {
+            atomicOr(overflow, 1u);
+            continue;
+        }
+        key[at] = morton3(uint32_t(x), uint32_t(y), uint32_t(z));
+        word[at] = w;
+    }
+}
+
+__global__ void scatter_dense_kernel(const uint64_t *__restrict__ key, const uint32_t *__restrict__ word, uint64_t n,
+                                     int side, uint32_t *__restrict__ grid) {
+    uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t m = key[i];
+    const uint64_t x = compact3(m), y = compact3(m >> 1), z = compact3(m >> 2);
+    grid[x + uint64_t(side) * (y + uint64_t(side) * z)] = word[i];
+}
+
+// ---------------------------------------------------------------------------
 // host helpers
 // ---------------------------------------------------------------------------
 
@@ -1244,6 +1346,113 @@ int build_grid(const uint32_t *grid, int w, int h, int d, const svo_build_opts *
     if ((r = finish_leaves(ar, g, key2, word2, V, false, blockFlags, nBlocks, st, lkey, lword, L))) return r;
     const bool multi = g.cacheLog < g.depth;
     return emit_tree(ar, g, lkey, lword, L, multi ? blockFlags : nullptr, st, out);
+}
+
+// Occupied voxels of an SDF scene as sorted (morton key, word) pairs on the device.
+static int sdf_leaves(Arena &ar, const svo_sdf_primitive *prims, int count, int n, float shade, uint64_t *&key,
+                      uint32_t *&word, uint64_t &V) {
+    if (count < 1 || count > kMaxSdfPrims) return fail(SVO_ERR_INVALID, "1..64 SDF primitives");
+    if (n < 8 || n > 8192 || (n & (n - 1))) return fail(SVO_ERR_INVALID, "SDF resolution must be a power of two in [8, 8192]");
+    SdfScene sc;
+    std::memset(&sc, 0, sizeof(sc));
+    for (int i = 0; i < count; ++i) {
+        const svo_sdf_primitive &p = prims[i];
+        if (p.kind != 0 && p.kind != 1) return fail(SVO_ERR_INVALID, "SDF primitive kind must be 0 (sphere) or 1 (torus)");
+        sc.p[i] = SdfPrim{p.kind, p.center[0], p.center[1], p.center[2], p.r1, p.r2, p.axis[0], p.axis[1], p.axis[2]};
+    }
+    sc.count = count;
+    sc.n = n;
+    sc.thr = 0.5f * std::sqrt(3.0f) / float(n);
+    sc.h = 1.0f / float(n);
+    sc.shade = shade;
+    hipStream_t st = ar.stream;
+    const int bpa = n / 8;
+    const uint64_t nBricks = uint64_t(bpa) * bpa * bpa;
+    uint32_t *list, *cnt, *overflow;
+    int r;
+    if ((r = ar.alloc(list, nBricks)) || (r = ar.alloc(cnt, 2)) || (r = ar.alloc(overflow, 1))) return r;
+    if ((r = fill32(cnt, 2, 0, st)) || (r = fill32(overflow, 1, 0, st))) return r;
+    hipLaunchKernelGGL(sdf_bricks_kernel, dim3(grid_for(nBricks)), dim3(kThreads), 0, st, sc, nBricks, bpa, list, cnt);
+    HIP_TRY(hipGetLastError());
+    uint32_t nb = 0;
+    HIP_TRY(hipMemcpy(&nb, cnt, 4, hipMemcpyDeviceToHost));
+    // shell voxels: at most a few per brick column; start from 64 per brick, grow if needed
+    uint64_t cap = std::max<uint64_t>(uint64_t(nb) * 64, 1024);
+    for (int attempt = 0; attempt < 4; ++attempt) {
+        uint64_t *k;
+        uint32_t *w;
+        if ((r = ar.alloc(k, cap)) || (r = ar.alloc(w, cap))) return r;
+        if ((r = fill32(cnt + 1, 1, 0, st)) || (r = fill32(overflow, 1, 0, st))) return r;
+        if (nb) hipLaunchKernelGGL(sdf_voxels_kernel, dim3(nb), dim3(64), 0, st, sc, list, bpa, k, w, cnt + 1, cap, overflow);
+        HIP_TRY(hipGetLastError());
+        uint32_t got = 0, of = 0;
+        HIP_TRY(hipMemcpy(&got, cnt + 1, 4, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(&of, overflow, 4, hipMemcpyDeviceToHost));
+        if (!of) {
+            V = got;
+            uint64_t *k2;
+            uint32_t *w2;
+            if ((r = ar.alloc(k2, V)) || (r = ar.alloc(w2, V))) return r;
+            if (V) {
+                size_t bytes = 0;
+                HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, k, k2, w, w2, V, 0, 3 * log2i(n), st));
+                uint8_t *tmp;
+                if ((r = ar.alloc(tmp, bytes))) return r;
+                HIP_TRY(hipcub::DeviceRadixSort::SortPairs(tmp, bytes, k, k2, w, w2, V, 0, 3 * log2i(n), st));
+                ar.release(tmp);
+            }
+            ar.release(k);
+            ar.release(w);
+            key = k2;
+            word = w2;
+            return SVO_OK;
+        }
+        ar.release(k);
+        ar.release(w);
+        cap = uint64_t(got) + 1024;
+    }
+    return fail(SVO_ERR_DEVICE, "SDF voxel count did not converge");
+}
+
+int build_sdf(const svo_sdf_primitive *prims, int count, int n, float shade, const svo_build_opts *opts, svo_octree &out) {
+    Geometry g{};
+    g.dataW = g.dataH = g.dataD = n;
+    int r = choose_cache(g, budget(opts), false);
+    if (r) return r;
+    uint64_t nBlocks;
+    if ((r = check_blocks(g, nBlocks))) return r;
+    Arena ar;
+    HIP_TRY(hipStreamCreate(&ar.stream));
+    hipStream_t st = ar.stream;
+    uint64_t *key, V;
+    uint32_t *word;
+    if ((r = sdf_leaves(ar, prims, count, n, shade, key, word, V))) return r;
+    uint32_t *blockFlags;
+    if ((r = ar.alloc(blockFlags, nBlocks)) || (r = fill32(blockFlags, nBlocks, 0, st))) return r;
+    uint64_t *lkey, L;
+    uint32_t *lword;
+    if ((r = finish_leaves(ar, g, key, word, V, false, blockFlags, nBlocks, st, lkey, lword, L))) return r;
+    const bool multi = g.cacheLog < g.depth;
+    return emit_tree(ar, g, lkey, lword, L, multi ? blockFlags : nullptr, st, out);
+}
+
+int sdf_volume(const svo_sdf_primitive *prims, int count, int n, float shade, uint32_t *grid) {
+    if (n > 1024) return fail(SVO_ERR_LIMIT, "dense SDF volumes are limited to 1024^3");
+    Arena ar;
+    HIP_TRY(hipStreamCreate(&ar.stream));
+    hipStream_t st = ar.stream;
+    uint64_t *key, V;
+    uint32_t *word;
+    int r;
+    if ((r = sdf_leaves(ar, prims, count, n, shade, key, word, V))) return r;
+    const uint64_t N = uint64_t(n) * n * n;
+    uint32_t *dg;
+    if ((r = ar.alloc(dg, N)) || (r = fill32(dg, N, 0, st))) return r;
+    if (V) hipLaunchKernelGGL(scatter_dense_kernel, dim3(grid_for(V)), dim3(kThreads), 0, st, key, word, V, n, dg);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(st));
+    HIP_TRY(hipMemcpy(grid, dg, N * 4, hipMemcpyDeviceToHost));
+    return SVO_OK;
 }
 
 }  // namespace svo

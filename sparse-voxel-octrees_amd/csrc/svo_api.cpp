@@ -360,6 +360,24 @@ int svo_build_from_grid(const uint32_t *grid, int w, int h, int d, const svo_bui
     return finish_handle(t, out);
 }
 
+int svo_build_from_sdf(const svo_sdf_primitive *prims, int count, int resolution, float shade,
+                       const svo_build_opts *opts, svo_octree **out) {
+    clear_error();
+    if (!prims || !out) return fail(SVO_ERR_INVALID, "null argument");
+    *out = nullptr;
+    std::unique_ptr<svo_octree> t(new (std::nothrow) svo_octree());
+    if (!t) return fail(SVO_ERR_NOMEM, "out of host memory");
+    int r = build_sdf(prims, count, resolution, shade, opts, *t);
+    if (r) return r;
+    return finish_handle(t, out);
+}
+
+int svo_sdf_volume(const svo_sdf_primitive *prims, int count, int resolution, float shade, uint32_t *grid) {
+    clear_error();
+    if (!prims || !grid) return fail(SVO_ERR_INVALID, "null argument");
+    return sdf_volume(prims, count, resolution, shade, grid);
+}
+
 int svo_build_from_voxels(const char *path, const svo_build_opts *opts, svo_octree **out) {
     clear_error();
     if (!path || !out) return fail(SVO_ERR_INVALID, "null argument");

@@ -73,6 +73,9 @@ int build_mesh(const float *verts, int64_t nverts, int has_normals, const int32_
                const int32_t *face_idx, int64_t nfaces, int resolution, const svo_build_opts *opts,
                svo_octree &out);
 int build_grid(const uint32_t *grid, int w, int h, int d, const svo_build_opts *opts, svo_octree &out);
+int build_sdf(const svo_sdf_primitive *prims, int count, int n, float shade, const svo_build_opts *opts,
+              svo_octree &out);
+int sdf_volume(const svo_sdf_primitive *prims, int count, int n, float shade, uint32_t *grid);
 int read_ply(const char *path, std::vector<float> &verts, int &has_normals,
              std::vector<int32_t> &face_counts, std::vector<int32_t> &face_idx);
 

@@ -209,25 +209,38 @@ def sdf_eval(prims, p):
     return d
 
 
-def sdf_volume(n, seed, shade=0.8, compress=None, chunk=32):
+def sdf_volume(n, seed, shade=0.8, compress=None, brick=8):
     """Solid-shell voxelization of the seeded SDF scene on an n^3 grid.
 
     Occupied: |sdf(c)| <= 0.5*sqrt(3)/n at voxel centres c = (v+0.5)/n.
     Normal: normalised central-difference gradient (h = 1/n), packed with
     `compress(normals (k,3) float32, shade) -> uint32 (k,)`; a zero word is
     bumped to 1 so the voxel stays non-empty. Returns uint32 (n,n,n) indexed
-    [z, y, x] (x fastest in memory, as the .voxel layout)."""
+    [z, y, x] (x fastest in memory, as the .voxel layout).
+
+    Bricks of `brick`^3 voxels whose centre is farther from the surface than
+    the brick's half diagonal plus the shell width are skipped: the union of
+    exact sphere/torus distances is 1-Lipschitz, so they hold no occupied
+    voxel and the result equals a dense evaluation."""
     if compress is None:
         compress = compress_material_np
     prims = sdf_primitives(seed)
     out = np.zeros((n, n, n), dtype=np.uint32)
     thr = 0.5 * math.sqrt(3.0) / n
     h = 1.0 / n
-    xs = (np.arange(n) + 0.5) / n
-    for z0 in range(0, n, chunk):
-        zs = xs[z0:z0 + chunk]
-        Z, Y, X = np.meshgrid(zs, xs, xs, indexing="ij")
-        P = np.stack([X, Y, Z], axis=-1)
+    nb = n // brick
+    bc = (np.arange(nb) * brick + brick / 2.0) / n
+    Z, Y, X = np.meshgrid(bc, bc, bc, indexing="ij")
+    db = sdf_eval(prims, np.stack([X, Y, Z], axis=-1))
+    reach = (brick / 2.0) * math.sqrt(3.0) / n + thr + 2.0 / n
+    near = np.argwhere(np.abs(db) <= reach)  # (k, 3) as [bz, by, bx]
+    if len(near) == 0:
+        return out
+    off = np.stack(np.meshgrid(np.arange(brick), np.arange(brick), np.arange(brick), indexing="ij"), -1).reshape(-1, 3)
+    for s0 in range(0, len(near), 4096):
+        nb_ = near[s0:s0 + 4096]
+        vox = (nb_[:, None, :] * brick + off[None, :, :]).reshape(-1, 3)  # [z, y, x]
+        P = (vox[:, ::-1] + 0.5) / n  # (x, y, z)
         d = sdf_eval(prims, P)
         occ = np.abs(d) <= thr
         if not occ.any():
@@ -238,8 +251,8 @@ def sdf_volume(n, seed, shade=0.8, compress=None, chunk=32):
         g /= np.maximum(np.linalg.norm(g, axis=-1, keepdims=True), 1e-30)
         w = compress(g.astype(np.float32), shade)
         w[w == 0] = 1
-        blk = out[z0:z0 + chunk]
-        blk[occ] = w
+        v = vox[occ]
+        out[v[:, 0], v[:, 1], v[:, 2]] = w
     return out
 
 
@@ -297,3 +310,52 @@ def orbit_angles(seed, count):
     yaw = rng.uniform(0.0, 360.0, count).astype(np.float32)
     pitch = rng.uniform(-60.0, 60.0, count).astype(np.float32)
     return [(float(p), float(y)) for p, y in zip(pitch, yaw)]
+
+
+# ---------------------------------------------------------------------------
+# multi-octree, multi-view workload (batched / served rendering)
+# ---------------------------------------------------------------------------
+
+
+def mixed_octrees(seed=4096, count=64):
+    """64 octree specs: resolution 2^round(U[7,11]); even entries are SDF
+    scenes, odd ones displaced icospheres whose triangle count scales with
+    resolution^2 (frequency 316 at 2048^3 = 1,997,120 triangles)."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for i in range(count):
+        res = 1 << int(np.round(rng.uniform(7.0, 11.0)))
+        s = int(rng.integers(0, 1 << 31))
+        if i % 2 == 0:
+            specs.append({"kind": "sdf", "resolution": res, "seed": s})
+        else:
+            specs.append({"kind": "mesh", "resolution": res, "seed": s,
+                          "frequency": max(1, int(round(316 * res / 2048)))})
+    return specs
+
+
+def mixed_views(seed=8192, count=256, trees=64):
+    """256 view specs: pixel count log-uniform in [320*240, 1920*1080], aspect
+    4:3 or 16:9, seeded orbit camera, octree id U{0..trees-1}."""
+    rng = np.random.default_rng(seed)
+    views = []
+    for _ in range(count):
+        pixels = math.exp(rng.uniform(math.log(320 * 240), math.log(1920 * 1080)))
+        aspect = 4.0 / 3.0 if rng.random() < 0.5 else 16.0 / 9.0
+        w = max(8, int(round(math.sqrt(pixels * aspect))))
+        h = max(8, int(round(w / aspect)))
+        views.append({"tree": int(rng.integers(0, trees)), "width": w, "height": h,
+                      "pitch": float(np.float32(rng.uniform(-60.0, 60.0))),
+                      "yaw": float(np.float32(rng.uniform(0.0, 360.0)))})
+    return views
+
+
+def build_mixed_octree(pkg, spec):
+    """Builds one mixed_octrees() entry on the GPU (SDF source or mesh builder
+    with 8 threads' partition and a 1 GiB budget)."""
+    if spec["kind"] == "sdf":
+        return pkg.VoxelOctree.build_from_sdf(sdf_primitives(spec["seed"]), spec["resolution"])
+    v, f = icosphere(spec["frequency"], seed=spec["seed"] % 100003, displacement=0.05)
+    tab, has_n, counts, idx = mesh_vertex_table(v, f)
+    return pkg.VoxelOctree.build_from_mesh(tab, has_n, counts, idx, spec["resolution"], threads=8,
+                                           memory_budget=1 << 30)

@@ -29,6 +29,7 @@ EXPORTS = [
     "svo_orbit_camera", "svo_render_primary", "svo_render_primary_device", "svo_build_from_ply",
     "svo_build_from_mesh", "svo_build_from_voxels", "svo_build_from_grid",
     "svo_renderer_create", "svo_renderer_free", "svo_render_batch_device",
+    "svo_build_from_sdf", "svo_sdf_volume",
 ]
 
 
@@ -50,6 +51,31 @@ class View(ctypes.Structure):
 
 class BatchOpts(ctypes.Structure):
     _fields_ = [("waves_per_cu", ctypes.c_int), ("refill_threshold", ctypes.c_int), ("reserved", ctypes.c_int * 6)]
+
+
+class SdfPrimitive(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("center", ctypes.c_float * 3), ("r1", ctypes.c_float),
+                ("r2", ctypes.c_float), ("axis", ctypes.c_float * 3)]
+
+
+def sdf_primitives(prims):
+    """scenes.sdf_primitives(...) tuples -> ctypes array of svo_sdf_primitive."""
+    arr = (SdfPrimitive * len(prims))()
+    for i, (kind, c, r1, r2, axis) in enumerate(prims):
+        arr[i].kind = int(kind)
+        arr[i].center[:] = [float(x) for x in c]
+        arr[i].r1, arr[i].r2 = float(r1), float(r2)
+        arr[i].axis[:] = [float(x) for x in axis]
+    return arr
+
+
+def sdf_volume(prims, resolution, shade=0.8):
+    """Dense uint32 [n,n,n] ([z,y,x]) of the GPU SDF source's voxels."""
+    arr = sdf_primitives(prims)
+    n = int(resolution)
+    out = np.zeros((n, n, n), np.uint32)
+    _check(lib.svo_sdf_volume(arr, len(arr), n, float(shade), _p(out)))
+    return out
 
 
 class BuildOpts(ctypes.Structure):
@@ -87,6 +113,8 @@ def _load():
         "svo_build_from_grid": [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P],
         "svo_renderer_create": [ctypes.c_int, P], "svo_renderer_free": [P],
         "svo_render_batch_device": [P, P, ctypes.c_int, P, P],
+        "svo_build_from_sdf": [P, ctypes.c_int, ctypes.c_int, F, P, P],
+        "svo_sdf_volume": [P, ctypes.c_int, ctypes.c_int, F, P],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -185,6 +213,15 @@ class VoxelOctree:
         d_, h_, w_ = v.shape
         o = _opts(memory_budget, 0)
         _check(lib.svo_build_from_grid(_p(v), w_, h_, d_, ctypes.byref(o), ctypes.byref(h)))
+        return cls(_handle=h)
+
+    @classmethod
+    def build_from_sdf(cls, prims, resolution, shade=0.8, memory_budget=0):
+        """GPU signed-distance scene source (prims: scenes.sdf_primitives tuples)."""
+        h = ctypes.c_void_p()
+        arr = sdf_primitives(prims)
+        o = _opts(memory_budget, 0)
+        _check(lib.svo_build_from_sdf(arr, len(arr), int(resolution), float(shade), ctypes.byref(o), ctypes.byref(h)))
         return cls(_handle=h)
 
     @classmethod

@@ -131,3 +131,36 @@ def test_c2_full_size_build_matches_reference(svo, golden, tmp_path):
     data = t.encode()
     assert len(data) == b["size"]
     assert hashlib.md5(data).hexdigest() == b["md5"]
+
+
+@pytest.mark.parametrize("n,seed,mem", [(64, 3, 1 << 30), (256, 4, 1 << 30), (256, 6, 1 << 22)])
+def test_sdf_source_matches_oracle_build(svo, n, seed, mem):
+    """GPU SDF scene source: its octree equals the oracle's file-source build of
+    the very voxels it generates (dense copy from svo_sdf_volume)."""
+    prims = svo.scenes.sdf_primitives(seed)
+    print(f"[gpu] SDF source n={n} seed={seed} mem={mem}", flush=True)
+    vol = svo.svo.sdf_volume(prims, n)
+    assert (vol != 0).sum() > 1000
+    c, words, _ = orc.build_grid(vol, mem)
+    t = svo.VoxelOctree.build_from_sdf(prims, n, memory_budget=mem)
+    assert np.array_equal(t.words(), words)
+    assert np.array_equal(t.center(), c)
+
+
+def test_config1_sdf256_build_and_render(svo):
+    """256^3 octree of a seeded 16-primitive SDF scene (built
+    through the raw-volume route), 1280x720 primary rays from a seeded orbit
+    camera: GPU build == oracle build, GPU render == oracle raymarch."""
+    vol = svo.scenes.sdf_volume(256, seed=101, compress=orc.compress_material_batch)
+    c, words, _ = orc.build_grid(vol)
+    print("[gpu] config 1: SDF 256^3 grid build + 1280x720 render", flush=True)
+    t = svo.VoxelOctree.build_from_grid(vol)
+    assert np.array_equal(t.words(), words)
+    (pitch, yaw), = svo.scenes.orbit_angles(101, 1)
+    cam = t.orbit_camera(pitch, yaw, 1.0)
+    out = t.render_primary(cam, 1280, 720)
+    d = orc.primary_dirs(np.frombuffer(cam.tform, np.float32), 1280, 720)
+    ref = orc.raymarch(words, np.frombuffer(cam.pos, np.float32)[None, :], d, 0.0)
+    for k in ("hit", "normal", "leaf"):
+        assert np.array_equal(out[k], ref[k])
+    assert np.array_equal(out["t"].view(np.uint32), ref["t"].view(np.uint32))
