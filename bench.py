@@ -138,6 +138,7 @@ def run_mixed(pkg, torch, stream, steps, warmup, waves_per_cu, refill):
     torch.cuda.synchronize()
     mism = sum(int((a != sa).sum() + (b.view(torch.int32) != sb.view(torch.int32)).sum() + (c != sc).sum())
                for (a, b, c), (sa, sb, sc) in zip(outs, snap))
+    run_mixed.trees = trees
     return {"workload": "c4_64octrees_256views", "rays_per_step": rays, "octree_build_seconds": round(build_s, 2),
             "plain_grays_s": round(rays / (plain_ms / 1e3) / 1e9, 4), "plain_ms_per_step": round(plain_ms, 3),
             "batched_grays_s": round(rays / (batched_ms / 1e3) / 1e9, 4), "batched_ms_per_step": round(batched_ms, 3),
@@ -145,6 +146,45 @@ def run_mixed(pkg, torch, stream, steps, warmup, waves_per_cu, refill):
             "launch": {"waves_per_cu": waves_per_cu, "refill": refill,
                        "tuned_ms": {f"{w}x{r}": round(t, 3) for (w, r), t in tune.items()}},
             "mismatched_fields_vs_plain": mism}
+
+
+def run_served(pkg, trees, rates, seconds, seed=2000):
+    """Poisson request streams (seeded) over the 64 resident
+    octrees, each request (octree U{0..63}, seeded orbit camera, resolution as
+    the mixed views). For each mean rate, the continuous-batching loop and the
+    one-request-per-launch baseline replay the same schedule; a final burst
+    (every request at t=0) measures saturated throughput."""
+    rng = np.random.default_rng(seed)
+    out = {"rates": []}
+
+    def make(n):
+        reqs = []
+        for v in pkg.scenes.mixed_views(seed=int(rng.integers(1 << 30)), count=n, trees=len(trees)):
+            t = trees[v["tree"]]
+            reqs.append((v["tree"], t.orbit_camera(v["pitch"], v["yaw"], 1.0), v["width"], v["height"]))
+        return reqs
+
+    def stats(lat, mk, reqs):
+        rays = sum(w * h for _, _, w, h in reqs)
+        return {"p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+                "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
+                "grays_s": round(rays / mk / 1e9, 4), "requests": len(reqs)}
+
+    for lam in rates:
+        n = int(max(50, min(5000, lam * seconds)))
+        reqs = make(n)
+        arr = np.cumsum(rng.exponential(1.0 / lam, n))
+        arr -= arr[0]
+        lat_b, mk_b = pkg.svo.serve_replay(trees, reqs, arr, streams=4)
+        lat_p, mk_p = pkg.svo.serve_replay(trees, reqs, arr, streams=1, one_per_launch=True)
+        out["rates"].append({"req_per_s": lam, "batched": stats(lat_b, mk_b, reqs), "one_per_launch": stats(lat_p, mk_p, reqs)})
+    reqs = make(1000)
+    arr = np.zeros(len(reqs))
+    lat_b, mk_b = pkg.svo.serve_replay(trees, reqs, arr, streams=4)
+    lat_p, mk_p = pkg.svo.serve_replay(trees, reqs, arr, streams=1, one_per_launch=True)
+    b, p = stats(lat_b, mk_b, reqs), stats(lat_p, mk_p, reqs)
+    out["burst"] = {"batched": b, "one_per_launch": p, "batched_vs_one_per_launch": round(b["grays_s"] / p["grays_s"], 3)}
+    return out
 
 
 def main():
@@ -164,6 +204,9 @@ def main():
     ap.add_argument("--waves-per-cu", type=int, default=0, help="batched form: persistent waves per CU (0 = default)")
     ap.add_argument("--refill", type=int, default=0, help="batched form: idle lanes that trigger a refill (0 = default)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the 64-octree / 256-view batched workload")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served (request stream) workload")
+    ap.add_argument("--serve-rates", default="100,500,1000,2000", help="mean request rates (req/s) to replay")
+    ap.add_argument("--serve-seconds", type=float, default=2.0, help="schedule length per rate")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (no multi-GPU path); run with --gpus 1",
@@ -263,6 +306,9 @@ def main():
     mixed = None
     if not args.no_mixed:
         mixed = run_mixed(pkg, torch, stream, max(1, min(args.steps, 5)), 1, args.waves_per_cu, args.refill)
+    served = None
+    if not args.no_mixed and not args.no_serve:
+        served = run_served(pkg, run_mixed.trees, [float(x) for x in args.serve_rates.split(",")], args.serve_seconds)
 
     # batched outputs must equal the plain kernel's: snapshot plain, run batched, compare
     step()
@@ -326,6 +372,7 @@ def main():
                                                    f"full {W}x{H}) through the oracle raymarch"},
         "batched": batched,
         "mixed_batched": mixed,
+        "served": served,
         "parity": {"rays_checked": cpu_rays, "mismatched_fields": mismatches,
                    "note": "plain-kernel outputs vs the oracle on every ray; batched outputs vs plain below"},
         "wall_s": round(wall, 3),

@@ -167,6 +167,41 @@ void svo_renderer_free(svo_renderer *r);
 int svo_render_batch_device(svo_renderer *r, const svo_view *views, int nviews, const svo_batch_opts *opts,
                             void *stream);
 
+/* ---- serving: continuous batching of render requests ------------------------ */
+
+/* One render request: primary rays of an orbit/any camera over octree
+ * trees[tree]. Outputs per ray: normal, t, leaf (as svo_render_primary). */
+typedef struct {
+    int tree;
+    svo_camera camera;
+    int width, height;
+} svo_request;
+
+typedef struct {
+    int streams;            /* launch slots, each its own stream (1..16; 0 = 4) */
+    uint64_t output_budget; /* device bytes for in-flight outputs, split evenly
+                               between slots (0 = 1 GiB); a request is admitted
+                               when its 16 B/ray outputs fit its slot's share */
+    int max_batch;          /* requests per launch (0 = 256) */
+    int one_per_launch;     /* baseline: one request per launch, plain kernel */
+    int waves_per_cu;       /* batched launch parameters (svo_batch_opts) */
+    int refill_threshold;
+    int reserved[6];
+} svo_server_opts;
+
+/* Runs the request loop over a known arrival schedule (arrival_s ascending,
+ * seconds from the start) with the octrees resident: requests are admitted
+ * by output footprint and folded into the next launch as slots free up, with
+ * launches of different slots overlapping on the device. latency_s[i] gets
+ * scheduled arrival -> outputs visible in host memory; makespan_s the time
+ * from the first arrival to the last completion. Optional per-request host
+ * destinations (arrays of n pointers, entries may be NULL) receive the
+ * outputs. Synchronous; spins one host thread. */
+int svo_serve_replay(svo_octree *const *trees, int ntrees, const svo_request *requests,
+                     const double *arrival_s, int n, const svo_server_opts *opts, double *latency_s,
+                     double *makespan_s, uint32_t *const *normal_out, float *const *t_out,
+                     uint64_t *const *leaf_out);
+
 /* ---- construction --------------------------------------------------------- */
 
 typedef struct {

@@ -320,6 +320,13 @@ int svo_render_batch_device(svo_renderer *r, const svo_view *views, int nviews, 
     return render_batch(r, views, nviews, opts, stream);
 }
 
+int svo_serve_replay(svo_octree *const *trees, int ntrees, const svo_request *requests, const double *arrival_s, int n,
+                     const svo_server_opts *opts, double *latency_s, double *makespan_s, uint32_t *const *normal_out,
+                     float *const *t_out, uint64_t *const *leaf_out) {
+    clear_error();
+    return serve_replay(trees, ntrees, requests, arrival_s, n, opts, latency_s, makespan_s, normal_out, t_out, leaf_out);
+}
+
 // ---- construction -------------------------------------------------------------
 
 int svo_build_from_mesh(const float *verts, int64_t nverts, int has_normals, const int32_t *face_counts,

@@ -68,7 +68,12 @@ int renderer_create(int maxViews, svo_renderer **out);
 void renderer_free(svo_renderer *r);
 int render_batch(svo_renderer *r, const svo_view *views, int nviews, const svo_batch_opts *opts, void *stream);
 
-// ---- construction (build.hip / build_host.cpp) ----
+// ---- request loop (serve.cpp) ----
+int serve_replay(svo_octree *const *trees, int ntrees, const svo_request *reqs, const double *arrival, int n,
+                 const svo_server_opts *opts, double *latency, double *makespan, uint32_t *const *normalOut,
+                 float *const *tOut, uint64_t *const *leafOut);
+
+// ---- construction (build.hip) ----
 int build_mesh(const float *verts, int64_t nverts, int has_normals, const int32_t *face_counts,
                const int32_t *face_idx, int64_t nfaces, int resolution, const svo_build_opts *opts,
                svo_octree &out);

@@ -29,7 +29,7 @@ EXPORTS = [
     "svo_orbit_camera", "svo_render_primary", "svo_render_primary_device", "svo_build_from_ply",
     "svo_build_from_mesh", "svo_build_from_voxels", "svo_build_from_grid",
     "svo_renderer_create", "svo_renderer_free", "svo_render_batch_device",
-    "svo_build_from_sdf", "svo_sdf_volume",
+    "svo_build_from_sdf", "svo_sdf_volume", "svo_serve_replay",
 ]
 
 
@@ -78,6 +78,43 @@ def sdf_volume(prims, resolution, shade=0.8):
     return out
 
 
+class Request(ctypes.Structure):
+    _fields_ = [("tree", ctypes.c_int), ("camera", Camera), ("width", ctypes.c_int), ("height", ctypes.c_int)]
+
+
+class ServerOpts(ctypes.Structure):
+    _fields_ = [("streams", ctypes.c_int), ("output_budget", ctypes.c_uint64), ("max_batch", ctypes.c_int),
+                ("one_per_launch", ctypes.c_int), ("waves_per_cu", ctypes.c_int), ("refill_threshold", ctypes.c_int),
+                ("reserved", ctypes.c_int * 6)]
+
+
+def serve_replay(trees, requests, arrivals, streams=4, output_budget=1 << 30, max_batch=256, one_per_launch=False,
+                 waves_per_cu=0, refill_threshold=0, outputs=None):
+    """Request loop over a known arrival schedule (see svo_serve_replay).
+    requests: list of (tree index, Camera, width, height). outputs: optional
+    list of (normal, t, leaf) numpy arrays (or None) per request. Returns
+    (latencies seconds ndarray, makespan seconds)."""
+    n = len(requests)
+    tarr = (ctypes.c_void_p * len(trees))(*[t.handle.value for t in trees])
+    rarr = (Request * n)()
+    for i, (ti, cam, w, h) in enumerate(requests):
+        rarr[i].tree, rarr[i].camera, rarr[i].width, rarr[i].height = int(ti), cam, int(w), int(h)
+    arr = np.ascontiguousarray(arrivals, dtype=np.float64)
+    lat = np.zeros(n, np.float64)
+    mk = ctypes.c_double()
+    o = ServerOpts()
+    o.streams, o.output_budget, o.max_batch = int(streams), int(output_budget), int(max_batch)
+    o.one_per_launch, o.waves_per_cu, o.refill_threshold = int(bool(one_per_launch)), int(waves_per_cu), int(refill_threshold)
+    nout = tout = lout = None
+    if outputs is not None:
+        nout = (ctypes.c_void_p * n)(*[None if x is None else x[0].ctypes.data for x in outputs])
+        tout = (ctypes.c_void_p * n)(*[None if x is None else x[1].ctypes.data for x in outputs])
+        lout = (ctypes.c_void_p * n)(*[None if x is None else x[2].ctypes.data for x in outputs])
+    _check(lib.svo_serve_replay(tarr, len(trees), rarr, _p(arr), n, ctypes.byref(o), _p(lat), ctypes.byref(mk),
+                                nout, tout, lout))
+    return lat, mk.value
+
+
 class BuildOpts(ctypes.Structure):
     _fields_ = [("memory_budget", ctypes.c_uint64), ("threads", ctypes.c_int), ("reserved", ctypes.c_int * 7)]
 
@@ -115,6 +152,7 @@ def _load():
         "svo_render_batch_device": [P, P, ctypes.c_int, P, P],
         "svo_build_from_sdf": [P, ctypes.c_int, ctypes.c_int, F, P, P],
         "svo_sdf_volume": [P, ctypes.c_int, ctypes.c_int, F, P],
+        "svo_serve_replay": [P, ctypes.c_int, P, P, ctypes.c_int, P, P, P, P, P, P],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

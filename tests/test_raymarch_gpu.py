@@ -128,3 +128,39 @@ def test_batched_renderer_matches_plain_kernel(svo, trees, golden):
             mine = {"hit": o["hit"].cpu().numpy(), "normal": o["normal"].cpu().numpy().view(np.uint32),
                     "t": o["t"].cpu().numpy(), "leaf": o["leaf"].cpu().numpy().view(np.uint64)}
             _same(mine, ref, f"{w}x{h} rs={rs}")
+
+
+@pytest.mark.parametrize("one_per_launch,streams,budget", [(False, 4, 1 << 28), (False, 2, 1 << 24), (True, 1, 1 << 26)])
+def test_served_results_equal_alone_and_in_the_mix(svo, trees, one_per_launch, streams, budget):
+    """Requests arriving while others run are folded into the next launch;
+    each served result equals the same request rendered alone."""
+    rng = np.random.default_rng(17)
+    names = ["dragon", "sdf128"]
+    tl = [trees[k] for k in names]
+    reqs = []
+    for i in range(48):
+        ti = int(rng.integers(0, 2))
+        w = int(rng.integers(8, 400))
+        h = int(rng.integers(8, 300))
+        cam = tl[ti].orbit_camera(float(rng.uniform(-60, 60)), float(rng.uniform(0, 360)), 1.0)
+        reqs.append((ti, cam, w, h))
+    arrivals = np.cumsum(rng.exponential(1.0 / 3000.0, len(reqs)))
+    outs = [(np.zeros(w * h, np.uint32), np.zeros(w * h, np.float32), np.zeros(w * h, np.uint64))
+            for _, _, w, h in reqs]
+    print(f"[gpu] serve replay one_per_launch={one_per_launch} streams={streams} budget={budget}", flush=True)
+    lat, makespan = svo.svo.serve_replay(tl, reqs, arrivals, streams=streams, output_budget=budget,
+                                         one_per_launch=one_per_launch, outputs=outs)
+    assert np.all(lat > 0) and makespan > 0
+    for (ti, cam, w, h), (nrm, t, leaf) in zip(reqs, outs):
+        ref = tl[ti].render_primary(cam, w, h)
+        assert np.array_equal(nrm, ref["normal"])
+        assert np.array_equal(t.view(np.uint32), ref["t"].view(np.uint32))
+        assert np.array_equal(leaf, ref["leaf"])
+
+
+def test_served_request_larger_than_budget_is_refused(svo, trees):
+    t = trees["dragon"]
+    cam = t.orbit_camera(0.0, 0.0, 1.0)
+    with pytest.raises(svo.SvoError) as e:
+        svo.svo.serve_replay([t], [(0, cam, 1920, 1080)], [0.0], streams=2, output_budget=1 << 20)
+    assert e.value.code == svo.svo.ERR_LIMIT
