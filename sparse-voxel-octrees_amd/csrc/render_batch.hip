@@ -304,7 +304,10 @@ int render_batch(svo_renderer *rd, const svo_view *views, int nviews, const svo_
     HIP_TRY(hipMemcpyAsync(rd->dViews, rd->hViews, sizeof(ViewDesc) * nviews, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(rd->dTileBase, rd->hTileBase, sizeof(uint64_t) * (nviews + 1), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, s, rd->dCounters);
-    const unsigned grid = unsigned(rd->cus * blocksPerCU);
+    // persistent grid: enough blocks to fill the GPU, but no more than four
+    // tiles per wave of work (small batches would otherwise launch idle waves)
+    const uint64_t workBlocks = (tiles + 15) / 16;
+    const unsigned grid = unsigned(std::max<uint64_t>(kQueues, std::min<uint64_t>(uint64_t(rd->cus) * blocksPerCU, workBlocks)));
     if (!wide)
         hipLaunchKernelGGL(batch_kernel<uint32_t>, dim3(grid), dim3(256), ldsPerBlock, s, rd->dViews, rd->dTileBase,
                            nviews, rd->dCounters, slots, refill);
