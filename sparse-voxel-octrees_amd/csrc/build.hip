@@ -418,20 +418,17 @@ __global__ void seg_starts_kernel(const uint32_t *__restrict__ head, const uint6
 }
 
 // PlyLoader::writeTriangleCell, src/PlyLoader.cpp:296-336, folded over one
-// voxel's triangles in ascending index order. The voxel centre each triangle
-// used is rebuilt with the same +hx steps from that triangle's clipped start.
-__global__ void fold_kernel(const uint64_t *__restrict__ keys, const uint64_t *__restrict__ segStart, uint64_t S,
-                            int tribits, const float *__restrict__ pos, const float *__restrict__ nrm,
-                            const float *__restrict__ col, MeshParams mp, uint64_t *__restrict__ vkey,
-                            uint32_t *__restrict__ vword) {
-    uint64_t s = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (s >= S) return;
-    const uint64_t b = segStart[s], e = segStart[s + 1];
+// voxel's triangles in ascending index order, starting from the buffer state
+// (word, count) the voxel's cell holds when its block is processed (0 for a
+// fresh cell). The voxel centre each triangle used is rebuilt with the same
+// +hx steps from that triangle's clipped start.
+__device__ __forceinline__ void fold_voxel(const uint64_t *__restrict__ keys, uint64_t b, uint64_t e, int tribits,
+                                           const float *__restrict__ pos, const float *__restrict__ nrm,
+                                           const float *__restrict__ col, const MeshParams &mp, uint32_t &word,
+                                           uint32_t &count) {
     const uint64_t m = keys[b] >> tribits;
     const int vx[3] = {int(compact3(m)), int(compact3(m >> 1)), int(compact3(m >> 2))};
     const uint64_t triMask = (uint64_t(1) << tribits) - 1;
-    uint32_t word = 0;
-    uint32_t count = 0;
     const float hx = mp.hx;
     for (uint64_t k = b; k < e; ++k) {
         const int64_t t = int64_t(keys[k] & triMask);
@@ -504,8 +501,35 @@ __global__ void fold_kernel(const uint64_t *__restrict__ keys, const uint64_t *_
             count = min(count + 1u, 255u);
         }
     }
-    vkey[s] = m;
+}
+
+__global__ void fold_kernel(const uint64_t *__restrict__ keys, const uint64_t *__restrict__ segStart, uint64_t S,
+                            int tribits, const float *__restrict__ pos, const float *__restrict__ nrm,
+                            const float *__restrict__ col, MeshParams mp, uint64_t *__restrict__ vkey,
+                            uint32_t *__restrict__ vword, uint8_t *__restrict__ vcount) {
+    uint64_t s = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    uint32_t word = 0, count = 0;
+    fold_voxel(keys, segStart[s], segStart[s + 1], tribits, pos, nrm, col, mp, word, count);
+    vkey[s] = keys[segStart[s]] >> tribits;
     vword[s] = word;
+    vcount[s] = uint8_t(count);
+}
+
+// Folds voxel `voxel[i]` again from a carried buffer state (init word/count).
+__global__ void refold_kernel(const uint64_t *__restrict__ keys, const uint64_t *__restrict__ segStart,
+                              const uint64_t *__restrict__ voxel, const uint32_t *__restrict__ initWord,
+                              const uint32_t *__restrict__ initCount, uint64_t n, int tribits,
+                              const float *__restrict__ pos, const float *__restrict__ nrm,
+                              const float *__restrict__ col, MeshParams mp, uint32_t *__restrict__ outWord,
+                              uint32_t *__restrict__ outCount) {
+    uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t s = voxel[i];
+    uint32_t word = initWord[i], count = initCount[i];
+    fold_voxel(keys, segStart[s], segStart[s + 1], tribits, pos, nrm, col, mp, word, count);
+    outWord[i] = word;
+    outCount[i] = count;
 }
 
 // ---------------------------------------------------------------------------
@@ -1038,7 +1062,7 @@ int emit_tree(Arena &ar, const Geometry &g, uint64_t *leafKey, uint32_t *leafWor
 // that case is refused.
 int finish_leaves(Arena &ar, const Geometry &g, uint64_t *key, uint32_t *word, uint64_t V, bool mesh,
                   uint32_t *blockFlags, uint64_t nBlocks, hipStream_t st, uint64_t *&lkey, uint32_t *&lword,
-                  uint64_t &L) {
+                  uint64_t &L, bool *staleCarry = nullptr) {
     int r;
     uint32_t *keep, *hazard;
     uint64_t *at;
@@ -1057,17 +1081,160 @@ int finish_leaves(Arena &ar, const Geometry &g, uint64_t *key, uint32_t *word, u
         for (uint64_t b = 0; b < nBlocks; ++b)
             if (fl[b]) last = int64_t(b);
         for (uint64_t b = 0; b < nBlocks; ++b)
-            if (hz[b] && int64_t(b) != last)
-                return fail(SVO_ERR_UNSUPPORTED,
-                            "mesh voxels fall on the unscanned last slice of an odd-depth cache block that is "
-                            "not the last one built; the reference's result then depends on its block-buffer "
-                            "reuse, which this builder does not reproduce (use a larger memory budget)");
+            if (hz[b] && int64_t(b) != last) {
+                if (!staleCarry) return fail(SVO_ERR_UNSUPPORTED, "unscanned voxels in a non-final cache block");
+                *staleCarry = true;  // the caller replays the block buffer (stale_buffer_leaves)
+                return SVO_OK;
+            }
     }
     if ((r = exclusive_scan(ar, keep, at, V, st))) return r;
     if ((r = scan_total(keep, at, V, L))) return r;
     if ((r = ar.alloc(lkey, L)) || (r = ar.alloc(lword, L))) return r;
     if (V) hipLaunchKernelGGL(compact_kernel, dim3(grid_for(V)), dim3(kThreads), 0, st, key, word, V, keep, at, lkey, lword);
     HIP_TRY(hipGetLastError());
+    return SVO_OK;
+}
+
+// Mesh source, several cache blocks, and an odd-depth block whose last slice
+// holds voxels that is not the last block built: the reference then carries
+// those voxels over. VoxelData reuses one block buffer (_bufferedData) and one
+// count array (PlyLoader::_counts) for every cache block without clearing
+// them (src/VoxelData.cpp:177-180, src/PlyLoader.cpp:439-474) and relies on
+// destructive reads to empty them; the unscanned slice (VoxelData.cpp:163-166)
+// is never read, so its entries stay, at their buffer index, into the next
+// block processed (blocks are processed in depth-first = ascending Morton
+// order). There a carried entry either is the starting state of the voxel
+// written at that index (writeTriangleCell then blends instead of starting
+// fresh) or, if no triangle writes it, appears as a voxel of its own.
+// This replays that block by block on the host (the carried set is a few
+// slices at most) and folds affected voxels again on the GPU from their
+// carried state; it yields the final leaf list.
+int stale_buffer_leaves(Arena &ar, const Geometry &g, const MeshParams &mp, int tribits, const uint64_t *keys,
+                        const uint64_t *segStart, const float *pos, const float *nrm, const float *col,
+                        const uint64_t *dvkey, const uint32_t *dvword, const uint8_t *dvcount, uint64_t V,
+                        const uint32_t *dBlockFlags, uint64_t nBlocks, hipStream_t st, uint64_t *&lkey,
+                        uint32_t *&lword, uint64_t &L) {
+    std::vector<uint64_t> vkey(V);
+    std::vector<uint32_t> vword(V), flags(nBlocks);
+    std::vector<uint8_t> vcnt(V);
+    HIP_TRY(hipStreamSynchronize(st));
+    if (V) {
+        HIP_TRY(hipMemcpy(vkey.data(), dvkey, V * 8, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(vword.data(), dvword, V * 4, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(vcnt.data(), dvcount, V, hipMemcpyDeviceToHost));
+    }
+    HIP_TRY(hipMemcpy(flags.data(), dBlockFlags, nBlocks * 4, hipMemcpyDeviceToHost));
+    struct Entry {
+        uint64_t idx;
+        uint32_t word, count;
+    };
+    const int shift = 3 * g.cacheLog;
+    const int cache = g.cache;
+    std::vector<Entry> carry;
+    std::vector<uint8_t> emit(V, 1);
+    std::vector<std::pair<uint64_t, uint32_t>> extra;
+    bool started = false;
+    int r;
+    for (uint64_t b = 0; b < nBlocks; ++b) {
+        if (!flags[b]) continue;
+        const int ox = int(compact3(b)) * cache, oy = int(compact3(b >> 1)) * cache, oz = int(compact3(b >> 2)) * cache;
+        const int w = std::min(cache, g.dataW - ox), h = std::min(cache, g.dataH - oy), d = std::min(cache, g.dataD - oz);
+        const uint64_t wh = uint64_t(w) * uint64_t(h), whd = wh * uint64_t(d);
+        const int dEven = (d / 2) * 2;
+        const uint64_t lo = uint64_t(std::lower_bound(vkey.begin(), vkey.end(), b << shift) - vkey.begin());
+        const uint64_t hi = uint64_t(std::lower_bound(vkey.begin(), vkey.end(), (b + 1) << shift) - vkey.begin());
+        bool hazard = false;
+        if (d & 1)
+            for (uint64_t v = lo; v < hi && !hazard; ++v)
+                hazard = vword[v] != 0 && int(compact3(vkey[v] >> 2)) == oz + d - 1;
+        if (!started && !hazard) continue;  // nothing carried into or out of this block
+        started = true;
+        std::vector<Entry> next, standalone;
+        std::vector<uint64_t> rqVox;
+        std::vector<uint32_t> rqWord, rqCount;
+        for (const Entry &e : carry) {
+            if (e.idx >= whd) {  // beyond this block's extent: untouched
+                next.push_back(e);
+                continue;
+            }
+            const uint32_t x = uint32_t(ox + int(e.idx % uint64_t(w)));
+            const uint32_t y = uint32_t(oy + int((e.idx / uint64_t(w)) % uint64_t(h)));
+            const uint32_t z = uint32_t(oz + int(e.idx / wh));
+            const uint64_t k = morton3(x, y, z);
+            const uint64_t v = uint64_t(std::lower_bound(vkey.begin() + lo, vkey.begin() + hi, k) - vkey.begin());
+            if (v < hi && vkey[v] == k) {
+                rqVox.push_back(v);
+                rqWord.push_back(e.word);
+                rqCount.push_back(e.count);
+            } else {
+                standalone.push_back(e);
+            }
+        }
+        if (!rqVox.empty()) {
+            const uint64_t n = rqVox.size();
+            uint64_t *dv;
+            uint32_t *dw, *dc, *ow, *oc;
+            if ((r = ar.alloc(dv, n)) || (r = ar.alloc(dw, n)) || (r = ar.alloc(dc, n)) || (r = ar.alloc(ow, n)) ||
+                (r = ar.alloc(oc, n)))
+                return r;
+            HIP_TRY(hipMemcpy(dv, rqVox.data(), n * 8, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(dw, rqWord.data(), n * 4, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(dc, rqCount.data(), n * 4, hipMemcpyHostToDevice));
+            hipLaunchKernelGGL(refold_kernel, dim3(grid_for(n)), dim3(kThreads), 0, st, keys, segStart, dv, dw, dc, n,
+                               tribits, pos, nrm, col, mp, ow, oc);
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipStreamSynchronize(st));
+            std::vector<uint32_t> hw(n), hc(n);
+            HIP_TRY(hipMemcpy(hw.data(), ow, n * 4, hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(hc.data(), oc, n * 4, hipMemcpyDeviceToHost));
+            for (uint64_t i = 0; i < n; ++i) {
+                vword[rqVox[i]] = hw[i];
+                vcnt[rqVox[i]] = uint8_t(hc[i]);
+            }
+            for (void *p : {(void *)dv, (void *)dw, (void *)dc, (void *)ow, (void *)oc}) ar.release(p);
+        }
+        for (uint64_t v = lo; v < hi; ++v) {
+            if (vword[v] == 0) {
+                emit[v] = 0;
+                continue;
+            }
+            const int x = int(compact3(vkey[v])), y = int(compact3(vkey[v] >> 1)), z = int(compact3(vkey[v] >> 2));
+            if (z - oz >= dEven) {  // unscanned: stays in the buffer
+                emit[v] = 0;
+                const uint64_t idx = uint64_t(x - ox) + uint64_t(w) * (uint64_t(y - oy) + uint64_t(h) * uint64_t(z - oz));
+                next.push_back({idx, vword[v], vcnt[v]});
+            }
+        }
+        for (const Entry &e : standalone) {
+            if (int(e.idx / wh) < dEven) {
+                const uint32_t x = uint32_t(ox + int(e.idx % uint64_t(w)));
+                const uint32_t y = uint32_t(oy + int((e.idx / uint64_t(w)) % uint64_t(h)));
+                const uint32_t z = uint32_t(oz + int(e.idx / wh));
+                extra.push_back({morton3(x, y, z), e.word});
+            } else {
+                next.push_back(e);
+            }
+        }
+        carry.swap(next);
+    }
+    std::vector<std::pair<uint64_t, uint32_t>> leaves;
+    leaves.reserve(V + extra.size());
+    for (uint64_t v = 0; v < V; ++v)
+        if (emit[v] && vword[v] != 0) leaves.push_back({vkey[v], vword[v]});
+    leaves.insert(leaves.end(), extra.begin(), extra.end());
+    std::sort(leaves.begin(), leaves.end());
+    L = leaves.size();
+    std::vector<uint64_t> hk(L);
+    std::vector<uint32_t> hw(L);
+    for (uint64_t i = 0; i < L; ++i) {
+        hk[i] = leaves[i].first;
+        hw[i] = leaves[i].second;
+    }
+    if ((r = ar.alloc(lkey, L)) || (r = ar.alloc(lword, L))) return r;
+    if (L) {
+        HIP_TRY(hipMemcpy(lkey, hk.data(), L * 8, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(lword, hw.data(), L * 4, hipMemcpyHostToDevice));
+    }
     return SVO_OK;
 }
 
@@ -1279,24 +1446,32 @@ int build_mesh(const float *verts, int64_t nverts, int has_normals, const int32_
     if ((r = scan_total(head, segIdx, K, V))) return r;
     uint64_t *segStart, *vkey;
     uint32_t *vword;
-    if ((r = ar.alloc(segStart, V + 1)) || (r = ar.alloc(vkey, V)) || (r = ar.alloc(vword, V))) return r;
+    uint8_t *vcount;
+    if ((r = ar.alloc(segStart, V + 1)) || (r = ar.alloc(vkey, V)) || (r = ar.alloc(vword, V)) ||
+        (r = ar.alloc(vcount, V)))
+        return r;
     if (K) hipLaunchKernelGGL(seg_starts_kernel, dim3(grid_for(K)), dim3(kThreads), 0, st, head, segIdx, K, segStart);
     HIP_TRY(hipMemcpy(segStart + V, &K, 8, hipMemcpyHostToDevice));
     if (V)
         hipLaunchKernelGGL(fold_kernel, dim3(grid_for(V)), dim3(kThreads), 0, st, keys2, segStart, V, tribits, pos, nrm,
-                           col, mp, vkey, vword);
+                           col, mp, vkey, vword, vcount);
     HIP_TRY(hipGetLastError());
     ar.release(head);
     ar.release(segIdx);
+
+    uint64_t *lkey, L;
+    uint32_t *lword;
+    bool stale = false;
+    if ((r = finish_leaves(ar, g, vkey, vword, V, true, blockFlags, nBlocks, st, lkey, lword, L, &stale))) return r;
+    if (stale &&
+        (r = stale_buffer_leaves(ar, g, mp, tribits, keys2, segStart, pos, nrm, col, vkey, vword, vcount, V, blockFlags,
+                                 nBlocks, st, lkey, lword, L)))
+        return r;
     ar.release(segStart);
     ar.release(keys2);
     ar.release(pos);
     ar.release(nrm);
     ar.release(col);
-
-    uint64_t *lkey, L;
-    uint32_t *lword;
-    if ((r = finish_leaves(ar, g, vkey, vword, V, true, blockFlags, nBlocks, st, lkey, lword, L))) return r;
     const bool multi = g.cacheLog < g.depth;
     return emit_tree(ar, g, lkey, lword, L, multi ? blockFlags : nullptr, st, out);
 }

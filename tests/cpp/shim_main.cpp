@@ -1,0 +1,63 @@
+// Caller of include/VoxelOctree.hpp written the way the reference's Main.cpp
+// uses VoxelOctree (load, center, save; build from PlyLoader/VoxelData;
+// raymarch). Modes:
+//   shim_main cpu <in.oct> <out.oct>            load + center + save, error path
+//   shim_main gpu <in.oct> <rays.f32> <out.bin> per-ray raymarch of n x 7 floats
+//   shim_main build <in.ply> <res> <threads> <mem> <out.oct>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "VoxelOctree.hpp"
+
+int main(int argc, char **argv) {
+    if (argc < 2) return 1;
+    const std::string mode = argv[1];
+    if (mode == "cpu" && argc == 4) {
+        VoxelOctree tree(argv[2]);
+        Vec3 c = tree.center();
+        std::printf("center %.9g %.9g %.9g\n", c.x, c.y, c.z);
+        tree.save(argv[3]);
+        try {
+            VoxelOctree missing("/nonexistent/file.oct");
+            return 2;
+        } catch (const svo_error &e) {
+            std::printf("error %d %s\n", e.code(), e.what());
+        }
+        return 0;
+    }
+    if (mode == "gpu" && argc == 5) {
+        VoxelOctree tree(argv[2]);
+        FILE *fp = std::fopen(argv[3], "rb");
+        std::vector<float> rays;
+        float buf[7];
+        while (std::fread(buf, 4, 7, fp) == 7) rays.insert(rays.end(), buf, buf + 7);
+        std::fclose(fp);
+        std::vector<uint32_t> out;
+        for (size_t i = 0; i < rays.size() / 7; ++i) {
+            const float *r = &rays[i * 7];
+            uint32_t normal = 0;
+            float t = 0.0f;
+            bool hit = tree.raymarch(Vec3(r[0], r[1], r[2]), Vec3(r[3], r[4], r[5]), r[6], normal, t);
+            uint32_t tb;
+            std::memcpy(&tb, &t, 4);
+            out.push_back(hit ? 1u : 0u);
+            out.push_back(normal);
+            out.push_back(tb);
+        }
+        fp = std::fopen(argv[4], "wb");
+        std::fwrite(out.data(), 4, out.size(), fp);
+        std::fclose(fp);
+        return 0;
+    }
+    if (mode == "build" && argc == 7) {
+        PlyLoader loader(argv[2]);
+        VoxelData data(&loader, std::atoi(argv[3]), size_t(std::strtoull(argv[5], nullptr, 10)));
+        data.setThreads(std::atoi(argv[4]));
+        VoxelOctree tree(&data);
+        tree.save(argv[6]);
+        return 0;
+    }
+    return 1;
+}

@@ -194,6 +194,7 @@ MESH_CASES = [
     ("ico16n", 16, 3, 0.08, 1.0, True, False),
     ("flat12", 12, 4, 0.05, 0.3, False, False),
     ("ico6a", 6, 5, 0.05, 1.0, True, True),
+    ("flat20", 20, 7, 0.05, 0.2, False, False),
 ]
 
 BUILD_CASES = [
@@ -209,6 +210,10 @@ BUILD_CASES = [
     ("flat12", 64, 8, 100000),
     ("flat12", 128, 8, 250000),
     ("ico6a", 64, 8, 1 << 30),
+    ("flat20", 128, 8, 250000),
+    ("flat20", 256, 8, 1500000),
+    ("flat12", 128, 2, 100000),
+    ("flat20", 256, 16, 12000000),
 ]
 
 GRID_CASES = [
