@@ -6,7 +6,7 @@ primary rays generated on the device, outputs normal + t + leaf index written
 to HBM). Inputs (the octree) are resident in HBM before the timed region.
 
 Workloads:
-  c2      (default) 2048^3 octree built on the GPU from a seeded fBm-displaced
+  ico2048 (default) 2048^3 octree built on the GPU from a seeded fBm-displaced
           icosphere (frequency 316 -> 1,997,120 triangles, displacement 5% of
           the radius, 8 threads' sub-block partition, 1 GiB budget), 8 seeded
           orbit cameras at 1920x1080 (16,588,800 rays per step).
@@ -39,8 +39,8 @@ def profiled_traffic(workload, width, height):
     """HBM bytes per primary launch measured by rocprofv3 (FETCH_SIZE and
     WRITE_SIZE, calibrated on a 1 GiB copy) for this workload, from the
     committed profile summary; None when no profile covers it."""
-    path = os.path.join(ROOT, "profiles", "raymarch_primary_c2_1080p_pmc.txt")
-    if workload != "c2_icosphere2048_8x1080p" or (width, height) != (1920, 1080) or not os.path.exists(path):
+    path = os.path.join(ROOT, "profiles", "raymarch_primary_ico2048_1080p_pmc.txt")
+    if workload != "icosphere2048_8x1080p" or (width, height) != (1920, 1080) or not os.path.exists(path):
         return None
     for line in open(path):
         if line.strip().startswith("traffic_bytes_per_launch:"):
@@ -53,7 +53,7 @@ def make_workload(pkg, name, cache_dir):
         tree = pkg.VoxelOctree(os.path.join(ROOT, "tests", "golden", "dragon.oct"))
         return tree, {"workload": "dragon256_8x1080p", "octree": "reference dragon 256^3",
                       "resolution": 256}
-    if name == "c2":
+    if name == "ico2048":
         ply = os.path.join(cache_dir, "ico316_seed2048.ply")
         if not os.path.exists(ply):
             os.makedirs(cache_dir, exist_ok=True)
@@ -65,9 +65,9 @@ def make_workload(pkg, name, cache_dir):
         # the built tree must be the reference builder's, byte for byte
         import hashlib
         with open(os.path.join(ROOT, "tests", "golden", "builds.json")) as fp:
-            ref = json.load(fp)["c2"]
+            ref = json.load(fp)["ico2048"]
         md5 = hashlib.md5(tree.encode()).hexdigest()
-        return tree, {"workload": "c2_icosphere2048_8x1080p", "octree": "GPU-built 2048^3 displaced icosphere",
+        return tree, {"workload": "icosphere2048_8x1080p", "octree": "GPU-built 2048^3 displaced icosphere",
                       "triangles": 1997120, "resolution": 2048, "build_seconds": round(build_s, 3),
                       "build_leaf_voxels_per_s": round(ref["leaves"] / build_s, 1),
                       "build_oct_md5_matches_reference": md5 == ref["md5"]}
@@ -75,7 +75,7 @@ def make_workload(pkg, name, cache_dir):
 
 
 def run_mixed(pkg, torch, stream, steps, warmup, waves_per_cu, refill):
-    """64 GPU-built octrees (128^3..2048^3, SDF scenes and
+    """Mixed batch: 64 GPU-built octrees (128^3..2048^3, SDF scenes and
     displaced meshes) and 256 views (320x240..1920x1080): 256 plain per-view
     launches vs one batched persistent launch, same rays."""
     t0 = time.time()
@@ -149,7 +149,7 @@ def run_mixed(pkg, torch, stream, steps, warmup, waves_per_cu, refill):
 
 
 def run_served(pkg, trees, rates, seconds, seed=2000):
-    """Poisson request streams (seeded) over the 64 resident
+    """Served stream: Poisson request streams (seeded) over the 64 resident
     octrees, each request (octree U{0..63}, seeded orbit camera, resolution as
     the mixed views). For each mean rate, the continuous-batching loop and the
     one-request-per-launch baseline replay the same schedule; a final burst
@@ -192,7 +192,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c2", choices=["c2", "dragon"])
+    ap.add_argument("--workload", default="ico2048", choices=["ico2048", "dragon"])
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--views", type=int, default=8)
@@ -200,7 +200,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity leg (profiling runs)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="run one plain device copy of this size first (HBM counter calibration)")
-    ap.add_argument("--cache", default=os.path.join(ROOT, "gpurun_out", "bench_cache"))
+    ap.add_argument("--cache", default=os.path.join(os.environ.get("TMPDIR", "/tmp"), "svo_bench_cache"))
     ap.add_argument("--waves-per-cu", type=int, default=0, help="batched form: persistent waves per CU (0 = default)")
     ap.add_argument("--refill", type=int, default=0, help="batched form: idle lanes that trigger a refill (0 = default)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the 64-octree / 256-view batched workload")

@@ -130,6 +130,21 @@ int svo_render_primary_device(svo_octree *h, const svo_camera *cam, int width, i
                               uint8_t *hit, uint32_t *normal, float *t, uint64_t *leaf,
                               void *stream);
 
+/* The viewer's frame (src/Main.cpp renderBatch/renderTile/shade, :81-202),
+ * bit-identical: the image is split into `stripes` horizontal stripes as the
+ * viewer's threads split it (0 = 16, src/Main.cpp:57,351-361); per stripe one
+ * coarse ray with the LOD cut-off per 8x8 tile corner, then per-pixel rays
+ * from pos + dir*max(minT-0.03, 0) for tiles with a finite corner depth,
+ * shaded and packed to RGBA (0xFF000000 | grey*0x010101; pixels of skipped
+ * tiles are 0). stride 1, or 3 as while the viewer's camera moves (pixels
+ * between stride corners copy their corner). depth (may be NULL) receives
+ * svo_frame_depth_count() coarse depths (1e10 = miss). rgba is width*height. */
+int svo_render_frame(svo_octree *h, const svo_camera *cam, int width, int height, int stripes, int stride,
+                     uint32_t *rgba, float *depth);
+int svo_render_frame_device(svo_octree *h, const svo_camera *cam, int width, int height, int stripes,
+                            int stride, uint32_t *rgba, float *depth, void *stream);
+int64_t svo_frame_depth_count(int width, int height, int stripes);
+
 /* ---- batched rendering: many views of many octrees in one launch ----------- */
 
 /* One view: an octree, a camera and its device output arrays (width*height

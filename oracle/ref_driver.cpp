@@ -28,6 +28,14 @@
 //       in: n x {pitch, yaw, radius, cx, cy, cz} -> INV_MODELVIEW matrix as
 //       built by src/Main.cpp:212,244-250 and src/math/MatrixStack.cpp:65-66
 //       out: n x 16 floats
+//   frame       <in.oct> <cams.f32> <width> <height> <stride> <out.bin>
+//       cams: n x {pitch, yaw, radius}; per camera the viewer frame of
+//       src/Main.cpp:81-202 over 16 stripes (src/Main.cpp:351-361): Main.cpp
+//       needs SDL and cannot be compiled here, so this loop restates it on the
+//       reference's own VoxelOctree::raymarch, Mat4, Vec3, invSqrt and
+//       decompressMaterial.
+//       out: per camera width*height uint32 RGBA, then the stripes' coarse
+//       depth tables back to back (float32)
 #include "VoxelOctree.hpp"
 #include "VoxelData.hpp"
 #include "PlyLoader.hpp"
@@ -160,6 +168,86 @@ int main(int argc, char **argv) {
             memcpy(&out[i * 16], tform.a, 64);
         }
         writeAll(argv[3], out);
+        return 0;
+    }
+    if (cmd == "frame" && argc == 8) {
+        VoxelOctree tree(argv[2]);
+        std::vector<float> cams = readAll<float>(argv[3]);
+        const int W = atoi(argv[4]), H = atoi(argv[5]), stride = atoi(argv[6]);
+        const int threads = 16, tile = 8;
+        const float aspect = H / (float)W;
+        std::vector<uint32_t> out;
+        for (size_t c = 0; c + 3 <= cams.size(); c += 3) {
+            Mat4 model = Mat4::rotXYZ(Vec3(cams[c], 0.0f, 0.0f)) * Mat4::rotXYZ(Vec3(0.0f, cams[c + 1], 0.0f));
+            Mat4 view = Mat4::translate(Vec3(0.0f, 0.0f, -cams[c + 2]));
+            Mat4 tform = model.pseudoInvert() * view;
+            Vec3 pos = tform * Vec3() + tree.center() + Vec3(1.0);
+            tform.a14 = tform.a24 = tform.a34 = 0.0f;
+            const float scale = 2.0f / W, tileScale = tile * scale;
+            const float planeDist = 1.0f / std::tan(float(M_PI) / 6.0f);
+            const float zx = planeDist * tform.a13, zy = planeDist * tform.a23, zz = planeDist * tform.a33;
+            const float coarseScale = 2.0f * tile / (planeDist * H);
+            const Vec3 light = (tform * Vec3(-1.0, 1.0, -1.0)).normalize();
+            std::vector<uint32_t> img(size_t(W) * H, 0u);
+            std::vector<float> depths;
+            auto direction = [&](float dx, float dy) {
+                Vec3 dir(dx * tform.a11 + dy * tform.a12 + zx, dx * tform.a21 + dy * tform.a22 + zy,
+                         dx * tform.a31 + dy * tform.a32 + zz);
+                return dir * invSqrt(dir.x * dir.x + dir.y * dir.y + dir.z * dir.z);
+            };
+            const int sh = (H - 1) / threads + 1;
+            for (int i = 0; i < threads; ++i) {
+                const int y0 = i * sh, y1 = std::min((i + 1) * sh, H);
+                const int tilesX = (W - 1) / tile + 2, tilesY = (y1 - y0 - 1) / tile + 2;
+                std::vector<float> db(size_t(tilesX) * tilesY);
+                float dy = aspect - y0 * scale;
+                for (int y = 0, idx = 0; y < tilesY; y++, dy -= tileScale) {
+                    float dx = -1.0f + 0 * scale;
+                    for (int x = 0; x < tilesX; x++, dx += tileScale, idx++) {
+                        uint32_t nrm;
+                        float t;
+                        db[idx] = tree.raymarch(pos, direction(dx, dy), coarseScale, nrm, t) ? t : 1e10f;
+                        if (x == 0 || y == 0) continue;
+                        const float minT = std::min(std::min(db[idx], db[idx - 1]),
+                                                    std::min(db[idx - tilesX], db[idx - tilesX - 1]));
+                        if (minT == 1e10f) continue;
+                        const int tx0 = (x - 1) * tile, ty0 = (y - 1) * tile + y0;
+                        const int tx1 = std::min(tx0 + tile, W), ty1 = std::min(ty0 + tile, y1);
+                        const float m = std::max(minT - 0.03f, 0.0f);
+                        float py = aspect - ty0 * scale;
+                        for (int yy = ty0; yy < ty1; ++yy, py -= scale) {
+                            float px = -1.0f + tx0 * scale;
+                            for (int xx = tx0; xx < tx1; ++xx, px += scale) {
+                                const int cx = xx - ((xx - tx0) % stride), cy = yy - ((yy - ty0) % stride);
+                                if (cx != xx || cy != yy) {
+                                    img[size_t(yy) * W + xx] = img[size_t(cy) * W + cx];
+                                    continue;
+                                }
+                                const Vec3 dir = direction(px, py);
+                                Vec3 col;
+                                if (tree.raymarch(pos + dir * m, dir, 0.0f, nrm, t)) {
+                                    Vec3 n;
+                                    float sc;
+                                    decompressMaterial(nrm, n, sc);
+                                    const float d = std::max(light.dot(dir.reflect(n)), 0.0f);
+                                    col = sc * 0.9f * std::abs(light.dot(n)) + d * d * 0.2f;
+                                }
+                                const uint32_t g = uint32_t(std::min(col.x, 1.0f) * 255.0);
+                                img[size_t(yy) * W + xx] = g | (g << 8) | (g << 16) | 0xFF000000u;
+                            }
+                        }
+                    }
+                }
+                depths.insert(depths.end(), db.begin(), db.end());
+            }
+            out.insert(out.end(), img.begin(), img.end());
+            for (float v : depths) {
+                uint32_t u;
+                memcpy(&u, &v, 4);
+                out.push_back(u);
+            }
+        }
+        writeAll(argv[7], out);
         return 0;
     }
     fprintf(stderr, "bad command\n");

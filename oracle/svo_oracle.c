@@ -43,6 +43,8 @@ static inline int32_t cvt_i32(float f) {
 
 static inline int popc8(uint32_t v) { return __builtin_popcount(v & 0xFFu); }
 
+static void normalize3(const float a[3], float r[3]);
+
 /* ------------------------------------------------------------------------ */
 /* Util.hpp                                                                   */
 /* ------------------------------------------------------------------------ */
@@ -486,6 +488,133 @@ void oracle_primary_dirs(const float tf[16], int width, int height, float *dirs)
             float *o = dirs + 3 * ((int64_t)y * width + x);
             o[0] = vx * s; o[1] = vy * s; o[2] = vz * s;
         }
+    }
+}
+
+/* shade, src/Main.cpp:81-90: decoded normal and shade, Lambert + reflection
+ * term; the float is widened to a grey Vec3 by the caller. */
+static float frame_shade(uint32_t w, const float ray[3], const float light[3]) {
+    float n[3], c;
+    oracle_decompress_material(w, n, &c);
+    float proj = (n[0] * ray[0] + n[1] * ray[1] + n[2] * ray[2]) * 2.0f;   /* Vec3::reflect */
+    float r[3] = {ray[0] - n[0] * proj, ray[1] - n[1] * proj, ray[2] - n[2] * proj};
+    float d = smax(light[0] * r[0] + light[1] * r[1] + light[2] * r[2], 0.0f);
+    float specular = d * d;
+    return c * 0.9f * fabsf(light[0] * n[0] + light[1] * n[1] + light[2] * n[2]) + specular * 0.2f;
+}
+
+/* colour packing of renderTile (non-Apple byte order), src/Main.cpp:130-135 */
+static uint32_t frame_pack(float v) {
+    uint32_t g = (uint32_t)((double)smin(v, 1.0f) * 255.0);
+    return g | (g << 8) | (g << 16) | 0xFF000000u;
+}
+
+typedef struct {
+    const uint32_t *tree;
+    uint64_t nwords;
+    float a11, a12, a21, a22, a31, a32, zx, zy, zz;
+    float pos[3], light[3];
+    float aspect, scale;
+    int width;
+    uint32_t *rgba;
+} frame_ctx;
+
+static void frame_dir(const frame_ctx *f, float dx, float dy, float dir[3]) {
+    float vx = dx * f->a11 + dy * f->a12 + f->zx;
+    float vy = dx * f->a21 + dy * f->a22 + f->zy;
+    float vz = dx * f->a31 + dy * f->a32 + f->zz;
+    float s = oracle_inv_sqrt(vx * vx + vy * vy + vz * vz);
+    dir[0] = vx * s; dir[1] = vy * s; dir[2] = vz * s;
+}
+
+/* renderTile, src/Main.cpp:92-137 */
+static void frame_tile(const frame_ctx *f, int x0, int y0, int x1, int y1, int stride, float minT) {
+    float dy = f->aspect - (float)y0 * f->scale;
+    for (int y = y0; y < y1; ++y, dy -= f->scale) {
+        float dx = -1.0f + (float)x0 * f->scale;
+        for (int x = x0; x < x1; ++x, dx += f->scale) {
+            int cx = x - ((x - x0) % stride), cy = y - ((y - y0) % stride);
+            if (cx != x || cy != y) {
+                f->rgba[(int64_t)y * f->width + x] = f->rgba[(int64_t)cy * f->width + cx];
+                continue;
+            }
+            float dir[3];
+            frame_dir(f, dx, dy, dir);
+            float o[3] = {f->pos[0] + dir[0] * minT, f->pos[1] + dir[1] * minT, f->pos[2] + dir[2] * minT};
+            uint32_t nrm = 0;
+            float t = 0.0f;
+            uint64_t leaf = 0;
+            float v = 0.0f;
+            if (oracle_raymarch(f->tree, f->nwords, o, dir, 0.0f, &nrm, &t, &leaf, NULL) > 0) v = frame_shade(nrm, dir, f->light);
+            f->rgba[(int64_t)y * f->width + x] = frame_pack(v);
+        }
+    }
+}
+
+int64_t oracle_frame_depth_count(int width, int height, int stripes) {
+    int st = (height - 1) / stripes + 1;
+    int64_t n = 0;
+    for (int i = 0; i < stripes; ++i) {
+        int y0 = i * st, y1 = imin((i + 1) * st, height);
+        n += (int64_t)((width - 1) / 8 + 2) * ((y1 - y0 - 1) / 8 + 2);
+    }
+    return n;
+}
+
+void oracle_render_frame(const uint32_t *tree, uint64_t nwords, const float tform[16], const float center[3],
+                         int width, int height, int stripes, int stride, uint32_t *rgba, float *depth) {
+    const float pi = (float)3.14159265358979323846;
+    frame_ctx f;
+    f.tree = tree; f.nwords = nwords; f.width = width; f.rgba = rgba;
+    /* renderBatch, src/Main.cpp:139-202 */
+    for (int r = 0; r < 3; ++r) {
+        float p = tform[r * 4 + 0] * 0.0f + tform[r * 4 + 1] * 0.0f + tform[r * 4 + 2] * 0.0f + tform[r * 4 + 3];
+        f.pos[r] = (p + center[r]) + 1.0f;
+    }
+    float scale = 2.0f / (float)width;
+    float tileScale = 8.0f * scale;
+    float planeDist = 1.0f / tanf(pi / 6.0f);
+    f.a11 = tform[0]; f.a12 = tform[1]; f.a21 = tform[4]; f.a22 = tform[5]; f.a31 = tform[8]; f.a32 = tform[9];
+    f.zx = planeDist * tform[2]; f.zy = planeDist * tform[6]; f.zz = planeDist * tform[10];
+    float coarseScale = 2.0f * 8.0f / (planeDist * (float)height);
+    /* light = (tform*Vec3(-1,1,-1)).normalize() with the translation cleared */
+    float lv[3];
+    for (int r = 0; r < 3; ++r)
+        lv[r] = tform[r * 4 + 0] * -1.0f + tform[r * 4 + 1] * 1.0f + tform[r * 4 + 2] * -1.0f + 0.0f;
+    normalize3(lv, f.light);
+    f.aspect = (float)height / (float)width;
+    f.scale = scale;
+    const float miss = 1e10f;
+    int st = (height - 1) / stripes + 1;
+    int64_t dbase = 0;
+    for (int i = 0; i < stripes; ++i) {
+        int x0 = 0, x1 = width, y0 = i * st, y1 = imin((i + 1) * st, height);
+        int tilesX = (x1 - x0 - 1) / 8 + 2, tilesY = (y1 - y0 - 1) / 8 + 2;
+        float *db = (float *)malloc(sizeof(float) * (size_t)tilesX * tilesY);
+        for (int y = y0; y < y1; ++y) memset(rgba + (int64_t)y * width, 0, sizeof(uint32_t) * (size_t)width);
+        float dy = f.aspect - (float)y0 * scale;
+        int idx = 0;
+        for (int y = 0; y < tilesY; y++, dy -= tileScale) {
+            float dx = -1.0f + (float)x0 * scale;
+            for (int x = 0; x < tilesX; x++, dx += tileScale, idx++) {
+                float dir[3];
+                frame_dir(&f, dx, dy, dir);
+                uint32_t nrm = 0;
+                float t = 0.0f;
+                uint64_t leaf = 0;
+                db[idx] = oracle_raymarch(tree, nwords, f.pos, dir, coarseScale, &nrm, &t, &leaf, NULL) > 0 ? t : miss;
+                if (x > 0 && y > 0) {
+                    float minT = smin(smin(db[idx], db[idx - 1]), smin(db[idx - tilesX], db[idx - tilesX - 1]));
+                    if (minT != miss) {
+                        int tx0 = (x - 1) * 8 + x0, ty0 = (y - 1) * 8 + y0;
+                        frame_tile(&f, tx0, ty0, imin(tx0 + 8, x1), imin(ty0 + 8, y1), stride, smax(minT - 0.03f, 0.0f));
+                    }
+                }
+            }
+        }
+        if (depth) memcpy(depth + dbase, db, sizeof(float) * (size_t)tilesX * tilesY);
+        dbase += (int64_t)tilesX * tilesY;
+        free(db);
     }
 }
 

@@ -65,6 +65,19 @@ void oracle_orbit_camera(float pitch, float yaw, float radius, const float cente
  * src/Main.cpp:97-113 without incremental accumulation). dirs: w*h*3. */
 void oracle_primary_dirs(const float tform[16], int width, int height, float *dirs);
 
+/* Full viewer frame: src/Main.cpp renderBatch/renderTile/shade (:81-202) over
+ * `stripes` horizontal stripes as the viewer's threads split the image
+ * (:351-361): per stripe a coarse pass (one LOD ray per 8x8 tile corner,
+ * rayScale = 2*8/(planeDist*H)) and, for tiles with a finite corner depth,
+ * per-pixel rays from pos + dir*max(minT - 0.03, 0), shaded to RGBA; pixels of
+ * skipped tiles stay 0. tform is INV_MODELVIEW with its translation (as
+ * oracle_orbit_camera writes it); stride is 1 (3 while the viewer drags).
+ * depth (may be NULL) receives the coarse depths, stripe after stripe. */
+void oracle_render_frame(const uint32_t *tree, uint64_t nwords, const float tform[16], const float center[3],
+                         int width, int height, int stripes, int stride, uint32_t *rgba, float *depth);
+/* Number of coarse depth entries oracle_render_frame writes. */
+int64_t oracle_frame_depth_count(int width, int height, int stripes);
+
 /* ---- .oct decode (src/VoxelOctree.cpp:57-90) ------------------------------ */
 /* Parses an in-memory .oct image. Allocates *words (free with oracle_free). */
 int oracle_oct_decode(const uint8_t *file, uint64_t size, float center[3], uint32_t **words,

@@ -1,0 +1,288 @@
+// The viewer's frame algorithm on the GPU (src/Main.cpp renderBatch,
+// renderTile and shade, :81-202, over the per-thread stripes of :351-361).
+//
+// Per stripe the reference walks a grid of 8x8-pixel tiles with one extra
+// column and row of corners: every corner gets one coarse ray with the LOD
+// cut-off rayScale = 2*8/(planeDist*H), and every tile whose four corner
+// depths are not all "miss" is traced per pixel from pos + dir*max(minT-0.03,0),
+// shaded and packed to RGBA; other tiles stay 0. Ray directions use the
+// reference's running sums (dx += scale per pixel inside a tile, dx +=
+// 8*scale per corner along a stripe), so they are reproduced exactly: the
+// corner sums are tabulated on the host, the in-tile sums are stepped per lane.
+//
+//  coarse_kernel: one lane per corner of every stripe -> depth table
+//  fine_kernel:   one wave per tile; lane = pixel; with stride > 1 only the
+//                 stride-corner lanes trace and the others copy their colour
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "build_common.hpp"
+#include "march.hpp"
+#include "svo_internal.hpp"
+
+namespace svo {
+
+#define HIP_TRY(expr)                                                                       \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(SVO_ERR_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+namespace {
+
+using namespace march;
+
+constexpr float kTreeMiss = 1e10f;  // TreeMiss, src/Main.cpp:140
+
+struct FrameParams {
+    float a11, a12, a21, a22, a31, a32, zx, zy, zz;
+    float px, py, pz;
+    float lx, ly, lz;  // light direction
+    float aspect, scale, coarseScale;
+    int width, height, tilesX, stripes, stride;
+};
+
+struct Stripe {
+    int y0, y1, tilesY, pad;
+    int64_t depthBase;  // first corner of this stripe in the depth table
+    int64_t fineBase;   // first tile of this stripe in the fine-tile numbering
+};
+
+__device__ __forceinline__ int stripe_of(const Stripe *__restrict__ st, int n, int64_t i, bool fine) {
+    int lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if ((fine ? st[mid].fineBase : st[mid].depthBase) <= i) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+template <typename Idx>
+__device__ __forceinline__ int trace(const uint32_t *__restrict__ tree, float ox, float oy, float oz, float dx, float dy,
+                                     float dz, float rayScale, const Stack<Idx> &stk, uint32_t &normal, float &t) {
+    Ray<Idx> r;
+    ray_init(r, ox, oy, oz, dx, dy, dz, rayScale);
+    Idx leaf = 0;
+    int code;
+    do {
+        code = ray_step(r, tree, stk, normal, t, leaf);
+    } while (code == kRunning);
+    return code == kBad ? march::kMiss : code;
+}
+
+__device__ __forceinline__ void frame_dir(const FrameParams &p, float dx, float dy, float d[3]) {
+    const float vx = dx * p.a11 + dy * p.a12 + p.zx;
+    const float vy = dx * p.a21 + dy * p.a22 + p.zy;
+    const float vz = dx * p.a31 + dy * p.a32 + p.zz;
+    const float s = quake_rsqrt(vx * vx + vy * vy + vz * vz);
+    d[0] = vx * s;
+    d[1] = vy * s;
+    d[2] = vz * s;
+}
+
+template <typename Idx>
+__global__ void __launch_bounds__(64) coarse_kernel(const uint32_t *__restrict__ tree, FrameParams p,
+                                                    const Stripe *__restrict__ st, const float *__restrict__ dxTab,
+                                                    const float *__restrict__ dyTab, int64_t corners, int slots,
+                                                    float *__restrict__ depth) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= corners) return;
+    const int s = stripe_of(st, p.stripes, i, false);
+    const int64_t local = i - st[s].depthBase;
+    const int ty = int(local / p.tilesX), tx = int(local % p.tilesX);
+    float d[3];
+    // dyTab is indexed by corner row over all stripes (row offset = depthBase / tilesX)
+    frame_dir(p, dxTab[tx], dyTab[st[s].depthBase / p.tilesX + ty], d);
+    Stack<Idx> stk(lds, 64, slots, int(threadIdx.x));
+    uint32_t normal = 0;
+    float t = 0.0f;
+    const int code = trace<Idx>(tree, p.px, p.py, p.pz, d[0], d[1], d[2], p.coarseScale, stk, normal, t);
+    depth[i] = code != march::kMiss ? t : kTreeMiss;
+}
+
+template <typename Idx>
+__global__ void __launch_bounds__(64) fine_kernel(const uint32_t *__restrict__ tree, FrameParams p,
+                                                  const Stripe *__restrict__ st, const float *__restrict__ depth,
+                                                  int slots, uint32_t *__restrict__ rgba) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int64_t tile = blockIdx.x;
+    const int s = stripe_of(st, p.stripes, tile, true);
+    const int64_t local = tile - st[s].fineBase;
+    const int fx = int(local % (p.tilesX - 1)), fy = int(local / (p.tilesX - 1));
+    const int tx = fx + 1, ty = fy + 1;  // the reference renders tile (x-1, y-1) at corner (x, y)
+    const int64_t idx = st[s].depthBase + int64_t(ty) * p.tilesX + tx;
+    const float minT = dev::fmin_ref(dev::fmin_ref(depth[idx], depth[idx - 1]),
+                                dev::fmin_ref(depth[idx - p.tilesX], depth[idx - p.tilesX - 1]));
+    const int x0 = fx * 8, y0 = st[s].y0 + fy * 8;
+    const int x1 = min(x0 + 8, p.width), y1 = min(y0 + 8, st[s].y1);
+    const int lane = threadIdx.x;
+    const int px = lane & 7, py = lane >> 3;
+    const int x = x0 + px, y = y0 + py;
+    const bool valid = x < x1 && y < y1;
+    uint32_t color = 0;
+    if (minT != kTreeMiss) {
+        const bool corner = (px % p.stride) == 0 && (py % p.stride) == 0;
+        if (valid && corner) {
+            float dx = -1.0f + float(x0) * p.scale;
+            for (int k = 0; k < px; ++k) dx += p.scale;
+            float dy = p.aspect - float(y0) * p.scale;
+            for (int k = 0; k < py; ++k) dy -= p.scale;
+            float d[3];
+            frame_dir(p, dx, dy, d);
+            const float m = dev::fmax_ref(minT - 0.03f, 0.0f);
+            Stack<Idx> stk(lds, 64, slots, lane);
+            uint32_t normal = 0;
+            float t = 0.0f;
+            float v = 0.0f;
+            const int code = trace<Idx>(tree, p.px + d[0] * m, p.py + d[1] * m, p.pz + d[2] * m, d[0], d[1], d[2], 0.0f,
+                                        stk, normal, t);
+            if (code != march::kMiss) {
+                // shade, src/Main.cpp:81-90
+                float n[3], c;
+                dev::decompress_material(normal, n, c);
+                const float proj = (n[0] * d[0] + n[1] * d[1] + n[2] * d[2]) * 2.0f;
+                const float r[3] = {d[0] - n[0] * proj, d[1] - n[1] * proj, d[2] - n[2] * proj};
+                const float dd = dev::fmax_ref(p.lx * r[0] + p.ly * r[1] + p.lz * r[2], 0.0f);
+                const float specular = dd * dd;
+                v = c * 0.9f * fabsf(p.lx * n[0] + p.ly * n[1] + p.lz * n[2]) + specular * 0.2f;
+            }
+            const uint32_t g = uint32_t(double(dev::fmin_ref(v, 1.0f)) * 255.0);
+            color = g | (g << 8) | (g << 16) | 0xFF000000u;
+        }
+        // pixels between stride corners copy their corner (renderTile's half-size mode)
+        const int src = (py - py % p.stride) * 8 + (px - px % p.stride);
+        color = __shfl(color, src);
+    }
+    if (valid) rgba[int64_t(y) * p.width + x] = color;
+}
+
+}  // namespace
+
+int render_frame_device(svo_octree &h, const svo_camera &cam, int width, int height, int stripes, int stride,
+                        uint32_t *rgba, float *depthOut, void *stream) {
+    if (width <= 0 || height <= 0 || width > 32768 || height > 32768) return fail(SVO_ERR_INVALID, "image size out of range");
+    if (stripes <= 0) stripes = 16;  // NumThreads, src/Main.cpp:57
+    if (stride <= 0 || stride > 8) return fail(SVO_ERR_INVALID, "stride must be in [1, 8]");
+    // the viewer's split (rows per stripe rounded up) must leave every stripe non-empty; where it does
+    // not, the reference clears a negative number of rows (src/Main.cpp:165)
+    if (int64_t(stripes - 1) * ((height - 1) / stripes + 1) >= height)
+        return fail(SVO_ERR_INVALID, "stripe split leaves an empty stripe at this height");
+    if (!rgba) return fail(SVO_ERR_INVALID, "null image");
+    int r = ensure_device(h);
+    if (r) return r;
+    const float pi = float(3.14159265358979323846);
+    FrameParams p;
+    const float *tf = cam.tform;
+    p.px = cam.pos[0];
+    p.py = cam.pos[1];
+    p.pz = cam.pos[2];
+    p.scale = 2.0f / float(width);
+    const float tileScale = 8.0f * p.scale;
+    const float planeDist = 1.0f / std::tan(pi / 6.0f);
+    p.a11 = tf[0]; p.a12 = tf[1]; p.a21 = tf[4]; p.a22 = tf[5]; p.a31 = tf[8]; p.a32 = tf[9];
+    p.zx = planeDist * tf[2];
+    p.zy = planeDist * tf[6];
+    p.zz = planeDist * tf[10];
+    p.coarseScale = 2.0f * 8.0f / (planeDist * float(height));
+    {  // light = (tform*Vec3(-1,1,-1)).normalize(), translation cleared (src/Main.cpp:154,162)
+        float lv[3];
+        for (int k = 0; k < 3; ++k) {
+            float a = tf[k * 4 + 0] * -1.0f;
+            a = a + tf[k * 4 + 1] * 1.0f;
+            a = a + tf[k * 4 + 2] * -1.0f;
+            lv[k] = a + 0.0f;
+        }
+        const float inv = 1.0f / std::sqrt(lv[0] * lv[0] + lv[1] * lv[1] + lv[2] * lv[2]);
+        p.lx = lv[0] * inv;
+        p.ly = lv[1] * inv;
+        p.lz = lv[2] * inv;
+    }
+    p.aspect = float(height) / float(width);
+    p.width = width;
+    p.height = height;
+    p.tilesX = (width - 1) / 8 + 2;
+    p.stripes = stripes;
+    p.stride = stride;
+    // stripes, corner-row dy sums and corner-column dx sums, as the reference accumulates them
+    std::vector<Stripe> hs(size_t(stripes) + 1);
+    std::vector<float> dyTab, dxTab(size_t(p.tilesX));
+    const int sh = (height - 1) / stripes + 1;
+    int64_t corners = 0, tiles = 0;
+    for (int i = 0; i < stripes; ++i) {
+        Stripe &s = hs[size_t(i)];
+        s.y0 = i * sh;
+        s.y1 = std::min((i + 1) * sh, height);
+        s.tilesY = (s.y1 - s.y0 - 1) / 8 + 2;
+        s.pad = 0;
+        s.depthBase = corners;
+        s.fineBase = tiles;
+        corners += int64_t(s.tilesY) * p.tilesX;
+        tiles += int64_t(s.tilesY - 1) * (p.tilesX - 1);
+        float dy = p.aspect - float(s.y0) * p.scale;
+        for (int ty = 0; ty < s.tilesY; ++ty, dy -= tileScale) dyTab.push_back(dy);
+    }
+    hs[size_t(stripes)].depthBase = corners;
+    hs[size_t(stripes)].fineBase = tiles;
+    {
+        float dx = -1.0f + 0.0f * p.scale;
+        for (int tx = 0; tx < p.tilesX; ++tx, dx += tileScale) dxTab[size_t(tx)] = dx;
+    }
+    if (tiles > 0x7FFFFFFF) return fail(SVO_ERR_LIMIT, "image too large");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const size_t tabBytes = sizeof(Stripe) * hs.size() + sizeof(float) * (dyTab.size() + dxTab.size());
+    void *scratch = nullptr;
+    HIP_TRY(hipMallocAsync(&scratch, tabBytes + (depthOut ? 0 : sizeof(float) * size_t(corners)), s));
+    Stripe *dSt = static_cast<Stripe *>(scratch);
+    float *dDy = reinterpret_cast<float *>(dSt + hs.size());
+    float *dDx = dDy + dyTab.size();
+    float *dDepth = depthOut ? depthOut : dDx + dxTab.size();
+    std::vector<uint8_t> staging(tabBytes);
+    std::memcpy(staging.data(), hs.data(), sizeof(Stripe) * hs.size());
+    std::memcpy(staging.data() + sizeof(Stripe) * hs.size(), dyTab.data(), sizeof(float) * dyTab.size());
+    std::memcpy(staging.data() + sizeof(Stripe) * hs.size() + sizeof(float) * dyTab.size(), dxTab.data(),
+                sizeof(float) * dxTab.size());
+    hipError_t e = hipMemcpyAsync(scratch, staging.data(), tabBytes, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);  // staging is a host vector
+    if (e != hipSuccess) {
+        (void)hipFreeAsync(scratch, s);
+        return fail(SVO_ERR_DEVICE, std::string("frame: ") + hipGetErrorString(e));
+    }
+    const int slots = h.depth > 2 ? h.depth - 1 : 1;
+    const bool wide = h.words.size() >= (uint64_t(1) << 32);
+    const size_t lds = size_t(slots) * 64 * ((wide ? 8 : 4) + sizeof(float));
+    const unsigned cgrid = unsigned((corners + 63) / 64);
+    if (!wide) {
+        hipLaunchKernelGGL(coarse_kernel<uint32_t>, dim3(cgrid), dim3(64), lds, s, h.dwords, p, dSt, dDx, dDy, corners,
+                           slots, dDepth);
+        if (tiles) hipLaunchKernelGGL(fine_kernel<uint32_t>, dim3(unsigned(tiles)), dim3(64), lds, s, h.dwords, p, dSt,
+                                      dDepth, slots, rgba);
+    } else {
+        hipLaunchKernelGGL(coarse_kernel<uint64_t>, dim3(cgrid), dim3(64), lds, s, h.dwords, p, dSt, dDx, dDy, corners,
+                           slots, dDepth);
+        if (tiles) hipLaunchKernelGGL(fine_kernel<uint64_t>, dim3(unsigned(tiles)), dim3(64), lds, s, h.dwords, p, dSt,
+                                      dDepth, slots, rgba);
+    }
+    e = hipGetLastError();
+    (void)hipFreeAsync(scratch, s);
+    if (e != hipSuccess) return fail(SVO_ERR_DEVICE, std::string("frame: ") + hipGetErrorString(e));
+    return SVO_OK;
+}
+
+int64_t frame_depth_count(int width, int height, int stripes) {
+    if (stripes <= 0) stripes = 16;
+    const int sh = (height - 1) / stripes + 1;
+    int64_t n = 0;
+    for (int i = 0; i < stripes; ++i) {
+        const int y0 = i * sh, y1 = std::min((i + 1) * sh, height);
+        n += int64_t((width - 1) / 8 + 2) * ((y1 - y0 - 1) / 8 + 2);
+    }
+    return n;
+}
+
+}  // namespace svo

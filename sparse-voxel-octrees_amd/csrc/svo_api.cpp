@@ -303,6 +303,34 @@ int svo_render_primary(svo_octree *h, const svo_camera *cam, int width, int heig
     return SVO_OK;
 }
 
+int svo_render_frame_device(svo_octree *h, const svo_camera *cam, int width, int height, int stripes, int stride,
+                            uint32_t *rgba, float *depth, void *stream) {
+    clear_error();
+    if (!h || !cam) return fail(SVO_ERR_INVALID, "null argument");
+    return render_frame_device(*h, *cam, width, height, stripes, stride, rgba, depth, stream);
+}
+
+int64_t svo_frame_depth_count(int width, int height, int stripes) { return frame_depth_count(width, height, stripes); }
+
+int svo_render_frame(svo_octree *h, const svo_camera *cam, int width, int height, int stripes, int stride, uint32_t *rgba,
+                     float *depth) {
+    clear_error();
+    if (!h || !cam || !rgba) return fail(SVO_ERR_INVALID, "null argument");
+    if (width <= 0 || height <= 0) return fail(SVO_ERR_INVALID, "image size out of range");
+    const size_t m = size_t(width) * size_t(height);
+    const int64_t nd = frame_depth_count(width, height, stripes);
+    DevBuf<uint32_t> dR;
+    DevBuf<float> dD;
+    int r;
+    if ((r = dR.alloc(m)) || (r = out_buf(dD, depth, size_t(nd)))) return r;
+    r = render_frame_device(*h, *cam, width, height, stripes, stride, dR.p, dD.p, nullptr);
+    if (r) return r;
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) return fail(SVO_ERR_DEVICE, std::string("frame: ") + hipGetErrorString(e));
+    if ((r = from_device(rgba, dR, m)) || (r = from_device(depth, dD, size_t(nd)))) return r;
+    return SVO_OK;
+}
+
 // ---- batched rendering ----------------------------------------------------------
 
 int svo_renderer_create(int max_views, svo_renderer **out) {

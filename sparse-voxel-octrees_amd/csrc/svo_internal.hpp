@@ -63,6 +63,11 @@ int render_primary_device(svo_octree &h, const svo_camera &cam, int width, int h
                           uint32_t *normal, float *t, uint64_t *leaf, void *stream);
 int device_count();
 
+// ---- viewer frame (frame.hip) ----
+int render_frame_device(svo_octree &h, const svo_camera &cam, int width, int height, int stripes, int stride,
+                        uint32_t *rgba, float *depth, void *stream);
+int64_t frame_depth_count(int width, int height, int stripes);
+
 // ---- batched persistent renderer (render_batch.hip) ----
 int renderer_create(int maxViews, svo_renderer **out);
 void renderer_free(svo_renderer *r);

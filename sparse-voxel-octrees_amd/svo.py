@@ -30,6 +30,7 @@ EXPORTS = [
     "svo_build_from_mesh", "svo_build_from_voxels", "svo_build_from_grid",
     "svo_renderer_create", "svo_renderer_free", "svo_render_batch_device",
     "svo_build_from_sdf", "svo_sdf_volume", "svo_serve_replay",
+    "svo_render_frame", "svo_render_frame_device", "svo_frame_depth_count",
 ]
 
 
@@ -153,11 +154,15 @@ def _load():
         "svo_build_from_sdf": [P, ctypes.c_int, ctypes.c_int, F, P, P],
         "svo_sdf_volume": [P, ctypes.c_int, ctypes.c_int, F, P],
         "svo_serve_replay": [P, ctypes.c_int, P, P, ctypes.c_int, P, P, P, P, P, P],
+        "svo_render_frame": [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P],
+        "svo_render_frame_device": [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, P],
+        "svo_frame_depth_count": [ctypes.c_int, ctypes.c_int, ctypes.c_int],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
         fn.argtypes = args
         fn.restype = None if name in ("svo_free", "svo_free_buffer", "svo_renderer_free") else ctypes.c_int
+    L.svo_frame_depth_count.restype = I64
     return L
 
 
@@ -370,6 +375,24 @@ class VoxelOctree:
                               stream=None):
         _check(lib.svo_render_primary_device(self._h, ctypes.byref(cam), int(width), int(height), hit_ptr, normal_ptr,
                                              t_ptr, leaf_ptr, stream))
+
+    def render_frame(self, cam, width, height, stripes=16, stride=1):
+        """The viewer's shaded frame (src/Main.cpp:81-202): (rgba uint32
+        [height, width], coarse depth table float32)."""
+        rgba = np.zeros(width * height, np.uint32)
+        depth = np.zeros(frame_depth_count(width, height, stripes), np.float32)
+        _check(lib.svo_render_frame(self._h, ctypes.byref(cam), int(width), int(height), int(stripes), int(stride),
+                                    _p(rgba), _p(depth)))
+        return rgba.reshape(height, width), depth
+
+    def render_frame_device(self, cam, width, height, rgba_ptr, depth_ptr=None, stripes=16, stride=1, stream=None):
+        _check(lib.svo_render_frame_device(self._h, ctypes.byref(cam), int(width), int(height), int(stripes),
+                                           int(stride), rgba_ptr, depth_ptr, stream))
+
+
+def frame_depth_count(width, height, stripes=16):
+    """Entries of the viewer frame's coarse depth table."""
+    return int(lib.svo_frame_depth_count(int(width), int(height), int(stripes)))
 
 
 class Renderer:

@@ -23,6 +23,16 @@ Fixtures written (all small):
   grid_*.npz + build_grid_*.oct
                          raw volumes and the reference's file-source .oct
   lz4.json               md5 of reference re-encodings (dragon; a 2-chunk stream)
+  frame_*.npz            orbit cameras -> the viewer frame (RGBA and coarse depth
+                         tables, 16 stripes) at stride 1 and 3, from the driver's
+                         restatement of src/Main.cpp:81-202 on the reference's classes
+
+  builds.json["ico2048"]  the full-size build: the reference's .oct (md5, size,
+                         node counts) for a 1,997,120-triangle displaced
+                         icosphere at 2048^3, 8 threads, 1 GiB
+
+`--only frames` or `--only ico2048` regenerates just that part; other entries of
+builds.json are kept.
 """
 import argparse
 import hashlib
@@ -172,6 +182,44 @@ def raymarch_fixture(name, oct_path, seed, tmp, **kw):
     print(f"raymarch_{name}: {len(rays)} rays, {int((hit == 1).sum())} hits", flush=True)
 
 
+FRAME_SIZE = (300, 170)  # not multiples of the 8-pixel tile; 16 stripes of 11 rows
+
+
+def frame_fixture(name, oct_path, cams, tmp):
+    """Reference viewer frames of `oct_path` for orbit cameras (pitch, yaw, radius)."""
+    w, h = FRAME_SIZE
+    cams = np.asarray(cams, np.float32)
+    cp = os.path.join(tmp, "fcams.f32")
+    cams.tofile(cp)
+    nd = oracle.lib().oracle_frame_depth_count(w, h, 16)
+    out = {"cams": cams, "size": np.int32([w, h])}
+    for stride in (1, 3):
+        op = os.path.join(tmp, "frame.out")
+        run_ref("frame", oct_path, cp, w, h, stride, op, timeout=600)
+        raw = np.fromfile(op, np.uint32).reshape(len(cams), w * h + nd)
+        out[f"rgba_s{stride}"] = raw[:, :w * h].reshape(len(cams), h, w)
+        out[f"depth_s{stride}"] = raw[:, w * h:].copy().view(np.float32)
+    np.savez_compressed(os.path.join(HERE, f"frame_{name}.npz"), **out)
+
+
+def full_size_build(tmp):
+    """Reference builder on the full-size mesh (about 10 s of reference build)."""
+    e = {"mesh": "icosphere", "frequency": 316, "seed": 2048, "displacement": 0.05, "resolution": 2048,
+         "threads": 8, "mem": 1 << 30}
+    v, f = scenes.icosphere(e["frequency"], seed=e["seed"], displacement=e["displacement"])
+    ply = os.path.join(tmp, "ico2048.ply")
+    scenes.write_ply(ply, v, f)
+    op = os.path.join(tmp, "ico2048.oct")
+    run_ref("build-ply", ply, e["resolution"], e["threads"], e["mem"], op, timeout=900)
+    _, words = oracle.read_oct(op)
+    st = oracle.octree_stats(words)
+    e.update({"md5": md5(op), "size": os.path.getsize(op), "words": int(len(words)), "leaves": int(st["leaves"]),
+              "far_words": int(st["far_words"]), "desc_per_level": [int(x) for x in st["desc_per_level"]],
+              "ply_md5": md5(ply)})
+    print("full-size build", e["md5"], e["size"], flush=True)
+    return e
+
+
 def md5(path):
     with open(path, "rb") as fp:
         return hashlib.md5(fp.read()).hexdigest()
@@ -227,6 +275,7 @@ GRID_CASES = [
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--reference", default="/root/reference")
+    ap.add_argument("--only", choices=["frames", "ico2048"], default=None)
     args = ap.parse_args()
     if not os.path.exists(REF_BIN):
         subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref", f"REF={args.reference}"], check=True)
@@ -234,6 +283,22 @@ def main():
     try:
         dragon = os.path.join(HERE, "dragon.oct")
         shutil.copyfile(os.path.join(args.reference, "models", "XYZRGB-Dragon.oct"), dragon)
+
+        bpath = os.path.join(HERE, "builds.json")
+        if args.only == "ico2048":
+            with open(bpath) as fp:
+                builds = json.load(fp)
+            builds["ico2048"] = full_size_build(tmp)
+            with open(bpath, "w") as fp:
+                json.dump(builds, fp, indent=1)
+            return
+
+        # viewer frame goldens ---------------------------------------------------
+        frame_fixture("dragon", dragon, [[0, 0, 1], [20, 35, 1], [-45, 120, 1.6], [70, -200, 0.7]], tmp)
+        frame_fixture("sdf128", os.path.join(HERE, "build_sdf128.oct"), [[10, 30, 1], [-30, 210, 1.2]], tmp)
+        print("frame goldens done", flush=True)
+        if args.only == "frames":
+            return
 
         # unit-level goldens -------------------------------------------------
         rng = np.random.default_rng(7)
@@ -301,7 +366,9 @@ def main():
         raymarch_fixture("dragon", dragon, 11, tmp)
 
         # file-source (grid) builds -------------------------------------------
-        builds = {"mesh": {}, "mesh_builds": [], "grid_builds": []}
+        with open(bpath) as fp:
+            builds = json.load(fp)  # entries not regenerated below (the full-size build) are kept
+        builds.update({"mesh": {}, "mesh_builds": [], "grid_builds": []})
         for name, (w, h, d), seed, dens, mem in GRID_CASES:
             g = np.random.default_rng(seed)
             vol = np.zeros((d, h, w), np.uint32)
@@ -323,7 +390,7 @@ def main():
             builds["grid_builds"].append({"name": name, "mem": mem, "oct": os.path.basename(op), "md5": md5(op)})
             print("grid build", name, md5(op), flush=True)
 
-        # SDF scene (config-1 style) at 128^3 via the .voxel route, and its raymarch goldens
+        # seeded SDF scene at 128^3 via the .voxel route, and its raymarch goldens
         vol = scenes.sdf_volume(128, seed=5, compress=oracle.compress_material_batch)
         vp = os.path.join(tmp, "sdf128.voxel")
         scenes.write_voxel(vp, vol)

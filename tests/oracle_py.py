@@ -54,6 +54,10 @@ def lib():
         L.oracle_decompress_material_batch.argtypes = [ctypes.c_int64, P, P, P]
         L.oracle_tribox_batch.argtypes = [ctypes.c_int64, P, P]
         L.oracle_free.argtypes = [P]
+        L.oracle_render_frame.argtypes = [P, ctypes.c_uint64, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_int, P, P]
+        L.oracle_frame_depth_count.restype = ctypes.c_int64
+        L.oracle_frame_depth_count.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -119,6 +123,18 @@ def primary_dirs(tform, width, height):
     d = np.zeros((width * height, 3), np.float32)
     lib().oracle_primary_dirs(_p(tf), int(width), int(height), _p(d))
     return d
+
+
+def render_frame(words, tform, center, width, height, stripes=16, stride=1):
+    """Viewer frame (src/Main.cpp:81-202) on the CPU: (rgba [H, W], coarse depths)."""
+    words = np.ascontiguousarray(words, np.uint32)
+    tform = np.ascontiguousarray(tform, np.float32).reshape(16)
+    center = np.ascontiguousarray(center, np.float32)
+    rgba = np.zeros(width * height, np.uint32)
+    depth = np.zeros(lib().oracle_frame_depth_count(width, height, stripes), np.float32)
+    lib().oracle_render_frame(_p(words), len(words), _p(tform), _p(center), width, height, stripes, stride,
+                              _p(rgba), _p(depth))
+    return rgba.reshape(height, width), depth
 
 
 def raymarch(words, o, d, ray_scale=0.0, threads=8):

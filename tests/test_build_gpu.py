@@ -111,17 +111,17 @@ def test_build_errors(svo, tmp_path):
     assert e.value.code == svo.svo.ERR_LIMIT
 
 
-def test_c2_full_size_build_matches_reference(svo, golden, tmp_path):
-    """BASELINE config 2 at full size: 1,997,120-triangle displaced icosphere at
+def test_ico2048_full_size_build_matches_reference(svo, golden, tmp_path):
+    """Full-size mesh build: 1,997,120-triangle displaced icosphere at
     2048^3, 8 threads' partition, 1 GiB budget; the .oct must hash to the
     reference builder's output for the same PLY."""
     import hashlib
-    b = _builds(golden)["c2"]
+    b = _builds(golden)["ico2048"]
     v, f = svo.scenes.icosphere(b["frequency"], seed=b["seed"], displacement=b["displacement"])
-    ply = str(tmp_path / "c2.ply")
+    ply = str(tmp_path / "ico2048.ply")
     svo.scenes.write_ply(ply, v, f)
     assert hashlib.md5(open(ply, "rb").read()).hexdigest() == b["ply_md5"]
-    print("[gpu] C2 full-size build 2048^3", flush=True)
+    print("[gpu] full-size mesh build 2048^3", flush=True)
     t = svo.VoxelOctree.build_from_ply(ply, b["resolution"], threads=b["threads"], memory_budget=b["mem"])
     w = t.words()
     assert len(w) == b["words"]
@@ -147,13 +147,13 @@ def test_sdf_source_matches_oracle_build(svo, n, seed, mem):
     assert np.array_equal(t.center(), c)
 
 
-def test_config1_sdf256_build_and_render(svo):
-    """256^3 octree of a seeded 16-primitive SDF scene (built
+def test_sdf256_build_and_render(svo):
+    """SDF scene: 256^3 octree of a seeded 16-primitive SDF scene (built
     through the raw-volume route), 1280x720 primary rays from a seeded orbit
     camera: GPU build == oracle build, GPU render == oracle raymarch."""
     vol = svo.scenes.sdf_volume(256, seed=101, compress=orc.compress_material_batch)
     c, words, _ = orc.build_grid(vol)
-    print("[gpu] config 1: SDF 256^3 grid build + 1280x720 render", flush=True)
+    print("[gpu] SDF 256^3 grid build + 1280x720 render", flush=True)
     t = svo.VoxelOctree.build_from_grid(vol)
     assert np.array_equal(t.words(), words)
     (pitch, yaw), = svo.scenes.orbit_angles(101, 1)

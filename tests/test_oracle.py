@@ -55,6 +55,68 @@ def test_raymarch_matches_reference(golden, name):
     assert np.array_equal(r["leaf"][leaf], g["leaf"][leaf].astype(np.uint64))
 
 
+@pytest.mark.parametrize("name", ["dragon", "sdf128"])
+def test_frame_matches_reference(golden, name):
+    """Viewer frames (src/Main.cpp:81-202, 16 stripes) against the reference
+    driver's frames, which run on the reference's own raymarch, Mat4, Vec3,
+    invSqrt and decompressMaterial."""
+    g = np.load(os.path.join(golden, f"frame_{name}.npz"))
+    center, words = orc.read_oct(os.path.join(golden, "dragon.oct" if name == "dragon" else "build_sdf128.oct"))
+    w, h = (int(v) for v in g["size"])
+    for i, (pitch, yaw, radius) in enumerate(g["cams"]):
+        tf, _ = orc.orbit_camera(pitch, yaw, radius, center)
+        for stride in (1, 3):
+            rgba, depth = orc.render_frame(words, tf, center, w, h, 16, stride)
+            assert np.array_equal(rgba, g[f"rgba_s{stride}"][i]), (name, i, stride)
+            assert _eq_bits(depth, g[f"depth_s{stride}"][i]), (name, i, stride)
+
+
+def test_frame_invariants(golden):
+    """Properties of the frame the formulas fix: every traced pixel is opaque
+    grey; stride-3 pixels copy the stride corner of their 8x8 tile, and those
+    corners carry the stride-1 value of the same ray; tiles whose four corner
+    depths all miss stay 0; the stride does not change the coarse pass."""
+    center, words = orc.read_oct(os.path.join(golden, "dragon.oct"))
+    w, h, stripes = 203, 97, 4
+    tf, _ = orc.orbit_camera(15.0, 70.0, 1.0, center)
+    s1, d1 = orc.render_frame(words, tf, center, w, h, stripes, 1)
+    s3, d3 = orc.render_frame(words, tf, center, w, h, stripes, 3)
+    assert np.array_equal(d1.view(np.uint32), d3.view(np.uint32))
+    for img in (s1, s3):
+        lit = img != 0
+        assert lit.mean() > 0.2
+        px = img[lit]
+        assert np.all(px >> 24 == 0xFF)
+        assert np.all((px & 0xFF) == ((px >> 8) & 0xFF)) and np.all((px & 0xFF) == ((px >> 16) & 0xFF))
+    sh = (h - 1) // stripes + 1
+    ys, xs = np.mgrid[0:h, 0:w]
+    y0 = (ys // sh) * sh
+    ty0 = y0 + ((ys - y0) // 8) * 8
+    tx0 = (xs // 8) * 8
+    cy = ys - (ys - ty0) % 3
+    cx = xs - (xs - tx0) % 3
+    assert np.array_equal(s3, s3[cy, cx])
+    corner = (cy == ys) & (cx == xs)
+    assert np.array_equal(s3[corner], s1[corner])
+    # coarse corner rays: depth table row-major per stripe, tilesX = (w-1)//8 + 2
+    tiles_x = (w - 1) // 8 + 2
+    base = 0
+    f = np.float32
+    for i in range(stripes):
+        sy0, sy1 = i * sh, min((i + 1) * sh, h)
+        tiles_y = (sy1 - sy0 - 1) // 8 + 2
+        d = d1[base:base + tiles_x * tiles_y].reshape(tiles_y, tiles_x)
+        for y in range(1, tiles_y):
+            for x in range(1, tiles_x):
+                quad = d[y - 1:y + 1, x - 1:x + 1]
+                t0, t1 = (y - 1) * 8 + sy0, (x - 1) * 8
+                if np.all(quad == f(1e10)):
+                    assert not s1[t0:min(t0 + 8, sy1), t1:min(t1 + 8, w)].any()
+        base += tiles_x * tiles_y
+    assert base == len(d1)
+    assert 0 < (d1 < f(1e10)).mean() < 1
+
+
 def test_camera_matches_reference(golden):
     g = np.load(os.path.join(golden, "camera.npz"))
     for inp, tf in zip(g["inputs"], g["tform"]):

@@ -164,3 +164,54 @@ def test_served_request_larger_than_budget_is_refused(svo, trees):
     with pytest.raises(svo.SvoError) as e:
         svo.svo.serve_replay([t], [(0, cam, 1920, 1080)], [0.0], streams=2, output_budget=1 << 20)
     assert e.value.code == svo.svo.ERR_LIMIT
+
+
+@pytest.mark.parametrize("name", ["dragon", "sdf128"])
+def test_frame_matches_reference_goldens(trees, golden, name):
+    """The viewer frame (coarse LOD pass + per-tile fine pass + shading)
+    against the reference driver's frames: RGBA and coarse depths bit-equal."""
+    g = np.load(os.path.join(golden, f"frame_{name}.npz"))
+    w, h = (int(v) for v in g["size"])
+    t = trees[name]
+    for i, (pitch, yaw, radius) in enumerate(g["cams"]):
+        cam = t.orbit_camera(float(pitch), float(yaw), float(radius))
+        for stride in (1, 3):
+            print(f"[gpu] frame {name} cam {i} stride {stride} {w}x{h}", flush=True)
+            rgba, depth = t.render_frame(cam, w, h, 16, stride)
+            assert np.array_equal(rgba, g[f"rgba_s{stride}"][i]), (name, i, stride)
+            assert np.array_equal(_bits(depth), _bits(g[f"depth_s{stride}"][i])), (name, i, stride)
+
+
+@pytest.mark.parametrize("name,w,h,stripes,stride,pitch,yaw", [
+    ("dragon", 1280, 720, 16, 1, 25.0, 40.0),
+    ("dragon", 1280, 720, 16, 3, -10.0, 200.0),
+    ("sdf128", 641, 359, 7, 1, 5.0, 75.0),
+    ("sdf128", 8, 8, 1, 2, 0.0, 0.0),
+    ("dragon", 1, 1, 1, 1, 0.0, 0.0),
+])
+def test_frame_matches_oracle(trees, name, w, h, stripes, stride, pitch, yaw):
+    t = trees[name]
+    print(f"[gpu] frame vs oracle {name} {w}x{h} stripes={stripes} stride={stride}", flush=True)
+    cam = t.orbit_camera(pitch, yaw, 1.0)
+    rgba, depth = t.render_frame(cam, w, h, stripes, stride)
+    tf = np.frombuffer(cam.tform, np.float32)
+    ref_rgba, ref_depth = orc.render_frame(t.words(), tf, t.center(), w, h, stripes, stride)
+    assert np.array_equal(rgba, ref_rgba)
+    assert np.array_equal(_bits(depth), _bits(ref_depth))
+
+
+def test_frame_device_entry_and_errors(svo, trees):
+    import torch
+    t = trees["dragon"]
+    cam = t.orbit_camera(30.0, 10.0, 1.0)
+    w, h = 320, 176
+    print("[gpu] frame device entry", flush=True)
+    ref, _ = t.render_frame(cam, w, h, 16, 1)
+    out = torch.full((h, w), 0x12345678, dtype=torch.int32, device="cuda")
+    t.render_frame_device(cam, w, h, out.data_ptr(), None, 16, 1, torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    assert np.array_equal(out.cpu().numpy().view(np.uint32), ref)
+    with pytest.raises(svo.SvoError):
+        t.render_frame(cam, w, 100, 16, 1)  # 16 stripes of 7 rows leave the last one empty
+    with pytest.raises(svo.SvoError):
+        t.render_frame(cam, w, h, 16, 9)
