@@ -139,13 +139,103 @@ def run_mixed(pkg, torch, stream, steps, warmup, waves_per_cu, refill):
     mism = sum(int((a != sa).sum() + (b.view(torch.int32) != sb.view(torch.int32)).sum() + (c != sc).sum())
                for (a, b, c), (sa, sb, sc) in zip(outs, snap))
     run_mixed.trees = trees
-    return {"workload": "c4_64octrees_256views", "rays_per_step": rays, "octree_build_seconds": round(build_s, 2),
+    return {"workload": "mixed_64octrees_256views", "rays_per_step": rays, "octree_build_seconds": round(build_s, 2),
             "plain_grays_s": round(rays / (plain_ms / 1e3) / 1e9, 4), "plain_ms_per_step": round(plain_ms, 3),
             "batched_grays_s": round(rays / (batched_ms / 1e3) / 1e9, 4), "batched_ms_per_step": round(batched_ms, 3),
             "batched_vs_plain": round(plain_ms / batched_ms, 3),
             "launch": {"waves_per_cu": waves_per_cu, "refill": refill,
                        "tuned_ms": {f"{w}x{r}": round(t, 3) for (w, r), t in tune.items()}},
             "mismatched_fields_vs_plain": mism}
+
+
+def run_terrain(pkg, torch, stream, steps, warmup, cpu_threads, n=8192, seed=8192, width=3840, height=2160):
+    """Large terrain: the 8192^3 exposed shell of a seeded fBm heightfield,
+    built on the GPU column by column (512^3 cache blocks under a 1 GiB
+    budget), then 3840x2160 primary rays plus one hard-shadow ray per hit
+    toward a seeded directional light. A step is one primary pass and one
+    shadow pass; rays counted = primary rays + shadow rays traced."""
+    import oracle_py as orc
+    torch.cuda.synchronize()
+    t0 = time.time()
+    tree = pkg.VoxelOctree.build_from_terrain(n, seed, memory_budget=1 << 30)
+    build_s = time.time() - t0
+    tree.upload()
+    words = tree.words()
+    st = orc.octree_stats(words)
+    pitch, yaw, light = pkg.scenes.terrain_view(seed)
+    cam = tree.orbit_camera(pitch, yaw, 1.0)
+    eps = 0.25 / n
+    m = width * height
+    dev = torch.device("cuda", 0)
+    hit = torch.empty(m, dtype=torch.uint8, device=dev)
+    nrm = torch.empty(m, dtype=torch.int32, device=dev)
+    tt = torch.empty(m, dtype=torch.float32, device=dev)
+    lf = torch.empty(m, dtype=torch.int64, device=dev)
+    sh = torch.empty(m, dtype=torch.uint8, device=dev)
+    sptr = stream.cuda_stream
+
+    def step():
+        tree.render_shadow_device(cam, width, height, light, eps, hit.data_ptr(), tt.data_ptr(), sh.data_ptr(),
+                                  nrm.data_ptr(), lf.data_ptr(), sptr)
+
+    def primary_only():
+        tree.render_primary_device(cam, width, height, hit.data_ptr(), nrm.data_ptr(), tt.data_ptr(), lf.data_ptr(),
+                                   sptr)
+
+    def timed(fn):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(steps):
+            fn()
+        b.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / steps
+
+    prim_ms = timed(primary_only)
+    ms = timed(step)
+    g_hit = hit.cpu().numpy()
+    g_sh = sh.cpu().numpy()
+    shadow_rays = int((g_hit == 1).sum())
+    rays = m + shadow_rays
+    # parity and CPU baseline on a sample of pixels: the oracle's primary and shadow rays
+    rng = np.random.default_rng(7)
+    sample = np.sort(rng.choice(m, 200000, replace=False))
+    tf = np.frombuffer(cam.tform, np.float32)
+    pos = np.frombuffer(cam.pos, np.float32).copy()
+    d = orc.primary_dirs(tf, width, height)[sample]
+    c0 = time.perf_counter()
+    prim = orc.raymarch(words, pos[None, :], d, 0.0, threads=cpu_threads)
+    hits = prim["hit"] == 1
+    back = (prim["t"][hits] - np.float32(eps)).astype(np.float32)
+    o = (pos[None, :] + d[hits] * back[:, None]).astype(np.float32)
+    shr = orc.raymarch(words, o, np.broadcast_to(light, o.shape).copy(), 0.0, threads=cpu_threads)
+    cpu_s = time.perf_counter() - c0
+    ref_sh = np.zeros(len(sample), np.uint8)
+    ref_sh[hits] = (shr["hit"] != 0).astype(np.uint8)
+    g_t = tt.cpu().numpy()[sample]
+    g_n = nrm.cpu().numpy().view(np.uint32)[sample]
+    mism = int(np.count_nonzero(g_hit[sample] != prim["hit"]) + np.count_nonzero(g_n != prim["normal"])
+               + np.count_nonzero(g_t.view(np.uint32) != prim["t"].view(np.uint32))
+               + np.count_nonzero(g_sh[sample] != ref_sh))
+    cpu_rays = len(sample) + int(hits.sum())
+    counts = prim["counts"] + shr["counts"]
+    b_ray = 4.0 * float(counts.sum()) / cpu_rays + 16.0
+    return {"workload": "terrain8192_4k_shadow", "octree": f"GPU-built {n}^3 fBm heightfield shell (seed {seed})",
+            "build_seconds": round(build_s, 3), "leaf_voxels": int(st["leaves"]), "words": int(len(words)),
+            "build_leaf_voxels_per_s": round(st["leaves"] / build_s, 1),
+            "width": width, "height": height, "primary_rays": m, "shadow_rays": shadow_rays,
+            "grays_s": round(rays / (ms / 1e3) / 1e9, 4), "ms_per_step": round(ms, 3),
+            "primary_only_grays_s": round(m / (prim_ms / 1e3) / 1e9, 4), "primary_only_ms": round(prim_ms, 3),
+            "shadow_pass_ms": round(ms - prim_ms, 3),
+            "bytes_per_ray": round(b_ray, 2),
+            "hbm_frac": round(b_ray * rays / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 5),
+            "cpu_baseline": {"value": round(cpu_rays / cpu_s / 1e9, 6), "unit": "Grays/s", "cores": cpu_threads,
+                             "kind": "port", "sample": f"{len(sample)} sampled pixels' primary rays + their "
+                                                       f"{int(hits.sum())} shadow rays through the oracle"},
+            "parity": {"rays_checked": cpu_rays, "mismatched_fields": mism}}
 
 
 def run_served(pkg, trees, rates, seconds, seed=2000):
@@ -207,6 +297,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the served (request stream) workload")
     ap.add_argument("--serve-rates", default="100,500,1000,2000", help="mean request rates (req/s) to replay")
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="schedule length per rate")
+    ap.add_argument("--no-terrain", action="store_true", help="skip the 8192^3 terrain + shadow-ray workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (no multi-GPU path); run with --gpus 1",
@@ -310,6 +401,10 @@ def main():
     if not args.no_mixed and not args.no_serve:
         served = run_served(pkg, run_mixed.trees, [float(x) for x in args.serve_rates.split(",")], args.serve_seconds)
 
+    terrain = None
+    if not args.no_terrain:
+        terrain = run_terrain(pkg, torch, stream, max(1, min(args.steps, 5)), 1, args.cpu_threads)
+
     # batched outputs must equal the plain kernel's: snapshot plain, run batched, compare
     step()
     torch.cuda.synchronize()
@@ -373,13 +468,14 @@ def main():
         "batched": batched,
         "mixed_batched": mixed,
         "served": served,
+        "large_terrain": terrain,
         "parity": {"rays_checked": cpu_rays, "mismatched_fields": mismatches,
                    "note": "plain-kernel outputs vs the oracle on every ray; batched outputs vs plain below"},
         "wall_s": round(wall, 3),
     }
     print(json.dumps(out), flush=True)
     if mismatches or batched_mismatch or config.get("build_oct_md5_matches_reference") is False or \
-            (mixed and mixed["mismatched_fields_vs_plain"]):
+            (mixed and mixed["mismatched_fields_vs_plain"]) or (terrain and terrain["parity"]["mismatched_fields"]):
         sys.exit(3)
 
 

@@ -130,6 +130,19 @@ int svo_render_primary_device(svo_octree *h, const svo_camera *cam, int width, i
                               uint8_t *hit, uint32_t *normal, float *t, uint64_t *leaf,
                               void *stream);
 
+/* Primary rays (as svo_render_primary) plus one hard-shadow ray per primary
+ * leaf hit: origin pos + d*(t - eps) (the hit leaf's entry point pulled back
+ * along the primary ray d), direction `light` (unit vector toward a
+ * directional light), rayScale 0. shadow[i] = 1 when that ray hits a voxel,
+ * 0 when it escapes or pixel i had no primary hit. No reference counterpart:
+ * a second, incoherent ray per pixel for the large-terrain workload, traced by
+ * the same raymarch. The device form needs hit, t and shadow (device arrays). */
+int svo_render_shadow(svo_octree *h, const svo_camera *cam, int width, int height, const float light[3], float eps,
+                      uint8_t *hit, uint32_t *normal, float *t, uint64_t *leaf, uint8_t *shadow);
+int svo_render_shadow_device(svo_octree *h, const svo_camera *cam, int width, int height, const float light[3],
+                             float eps, uint8_t *hit, uint32_t *normal, float *t, uint64_t *leaf, uint8_t *shadow,
+                             void *stream);
+
 /* The viewer's frame (src/Main.cpp renderBatch/renderTile/shade, :81-202),
  * bit-identical: the image is split into `stripes` horizontal stripes as the
  * viewer's threads split it (0 = 16, src/Main.cpp:57,351-361); per stripe one
@@ -270,6 +283,25 @@ int svo_build_from_sdf(const svo_sdf_primitive *prims, int count, int resolution
                        const svo_build_opts *opts, svo_octree **out);
 /* The same voxels as a dense host volume (n^3 uint32, x fastest; n <= 1024). */
 int svo_sdf_volume(const svo_sdf_primitive *prims, int count, int resolution, float shade, uint32_t *grid);
+
+/* Procedural fBm heightfield terrain on an n^3 grid (n a power of two, <= 8192),
+ * the procedural counterpart of a file-source VoxelData
+ * (src/VoxelData.cpp:37-48,177-209), generated column by column on the device
+ * with no dense volume. FP32, each operation separately rounded:
+ *   h(x,z)  = 8 octaves of smoothstep value noise (gain 0.5, lacunarity 2,
+ *             base frequency 4, lattice hash of (cell, seed + octave)) at
+ *             ((x+0.5)/n, (z+0.5)/n), normalised to [0,1)
+ *   H(x,z)  = n/8 + int(h * n/2); solid for y <= H (y is up)
+ *   leaves  = the exposed shell: y from min(H, lowest in-grid 4-neighbour + 1) to H
+ *   word    = compressMaterial(normalize(-gx, 1, -gz), 0.35 + 0.65*h), 0 -> 1,
+ *             g = clamped central difference of h*n/2
+ * The octree is the one the reference builds from the same voxels as a
+ * .voxel file with the given memory budget. */
+int svo_build_from_terrain(int resolution, uint32_t seed, const svo_build_opts *opts, svo_octree **out);
+/* The same voxels as a dense host volume (n^3 uint32, x fastest; n <= 1024). */
+int svo_terrain_volume(int resolution, uint32_t seed, uint32_t *grid);
+/* Per-column H and h (n*n each, x fastest; either may be NULL). */
+int svo_terrain_heights(int resolution, uint32_t seed, int32_t *heights, float *h);
 
 #ifdef __cplusplus
 }

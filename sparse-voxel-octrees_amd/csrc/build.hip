@@ -850,6 +850,105 @@ __global__ void scatter_dense_kernel(const uint64_t *__restrict__ key, const uin
 }
 
 // ---------------------------------------------------------------------------
+// fBm heightfield terrain source (procedural replacement of the raw .voxel
+// route for terrain; every float operation separately rounded, in the order
+// of the definition: lattice hash -> smoothstep bilinear value noise -> 8
+// octaves, gain 0.5, lacunarity 2, base frequency 4 -> integer heights ->
+// exposed shell with the gradient normal)
+// ---------------------------------------------------------------------------
+
+constexpr int kTerrainOctaves = 8;
+
+__device__ __forceinline__ float terrain_lattice(int i, int k, uint32_t s) {
+    uint32_t h = (uint32_t(i) * 0x8DA6B343u) ^ (uint32_t(k) * 0xD8163841u) ^ (s * 0x9E3779B9u);
+    h ^= h >> 15;
+    h *= 0x2C1B3C6Du;
+    h ^= h >> 12;
+    h *= 0x297A2D39u;
+    h ^= h >> 15;
+    return float(h >> 8) * (1.0f / 16777216.0f);
+}
+
+__device__ __forceinline__ float terrain_smooth(float c) { return (c * c) * (3.0f - 2.0f * c); }
+
+// normalised fBm height in [0, 1) of column (x, z)
+__device__ float terrain_h(int x, int z, int n, uint32_t seed) {
+    const float u = (float(x) + 0.5f) / float(n), w = (float(z) + 0.5f) / float(n);
+    float total = 0.0f, norm = 0.0f, amp = 1.0f, freq = 4.0f;
+    for (int o = 0; o < kTerrainOctaves; ++o) {
+        const float pu = u * freq, pw = w * freq;
+        const float fu = floorf(pu), fw = floorf(pw);
+        const float cu = terrain_smooth(pu - fu), cw = terrain_smooth(pw - fw);
+        const int iu = int(fu), iw = int(fw);
+        const uint32_t s = seed + uint32_t(o);
+        const float v00 = terrain_lattice(iu, iw, s), v10 = terrain_lattice(iu + 1, iw, s);
+        const float v01 = terrain_lattice(iu, iw + 1, s), v11 = terrain_lattice(iu + 1, iw + 1, s);
+        const float a0 = v00 + (v10 - v00) * cu;
+        const float a1 = v01 + (v11 - v01) * cu;
+        total = total + amp * (a0 + (a1 - a0) * cw);
+        norm = norm + amp;
+        amp = amp * 0.5f;
+        freq = freq * 2.0f;
+    }
+    return total / norm;
+}
+
+// per column (index z*n + x): h and the integer height n/8 + int(h*(n/2))
+__global__ void terrain_heights_kernel(int n, uint32_t seed, float *__restrict__ hOut, int32_t *__restrict__ H) {
+    const uint64_t c = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= uint64_t(n) * n) return;
+    const int x = int(c % uint64_t(n)), z = int(c / uint64_t(n));
+    const float h = terrain_h(x, z, n, seed);
+    hOut[c] = h;
+    H[c] = n / 8 + int(h * float(n / 2));
+}
+
+// lowest exposed voxel of a column: a voxel is exposed when a 4-neighbour
+// column inside the grid is lower, or it is the top one
+__device__ __forceinline__ int terrain_lo(const int32_t *__restrict__ H, int n, int x, int z) {
+    const int32_t top = H[uint64_t(z) * n + x];
+    int32_t m = INT32_MAX;
+    if (x > 0) m = min(m, H[uint64_t(z) * n + x - 1]);
+    if (x + 1 < n) m = min(m, H[uint64_t(z) * n + x + 1]);
+    if (z > 0) m = min(m, H[uint64_t(z - 1) * n + x]);
+    if (z + 1 < n) m = min(m, H[uint64_t(z + 1) * n + x]);
+    return m == INT32_MAX ? top : min(top, m + 1);
+}
+
+__global__ void terrain_count_kernel(int n, const int32_t *__restrict__ H, uint32_t *__restrict__ count) {
+    const uint64_t c = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= uint64_t(n) * n) return;
+    const int x = int(c % uint64_t(n)), z = int(c / uint64_t(n));
+    count[c] = uint32_t(H[c] - terrain_lo(H, n, x, z) + 1);
+}
+
+// one column per thread: normal from the clamped central difference of
+// hv = h*(n/2), word = compressMaterial(normal, 0.35 + 0.65*h), keys of the
+// exposed voxels at the column's scanned offset
+__global__ void terrain_emit_kernel(int n, const int32_t *__restrict__ H, const float *__restrict__ hh,
+                                    const uint64_t *__restrict__ at, uint64_t *__restrict__ key,
+                                    uint32_t *__restrict__ word) {
+    const uint64_t c = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= uint64_t(n) * n) return;
+    const int x = int(c % uint64_t(n)), z = int(c / uint64_t(n));
+    const float half = float(n / 2);
+    const int xm = max(x - 1, 0), xp = min(x + 1, n - 1), zm = max(z - 1, 0), zp = min(z + 1, n - 1);
+    const float gx = (hh[uint64_t(z) * n + xp] * half - hh[uint64_t(z) * n + xm] * half) * 0.5f;
+    const float gz = (hh[uint64_t(zp) * n + x] * half - hh[uint64_t(zm) * n + x] * half) * 0.5f;
+    const float a[3] = {-gx, 1.0f, -gz};
+    float nrm[3];
+    normalize3(a, nrm);
+    uint32_t w = compress_material(nrm, 0.35f + 0.65f * hh[c]);
+    if (w == 0) w = 1;
+    const int top = H[c], lo = terrain_lo(H, n, x, z);
+    uint64_t o = at[c];
+    for (int y = lo; y <= top; ++y, ++o) {
+        key[o] = morton3(uint32_t(x), uint32_t(y), uint32_t(z));
+        word[o] = w;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // host helpers
 // ---------------------------------------------------------------------------
 
@@ -1587,6 +1686,120 @@ static int sdf_leaves(Arena &ar, const svo_sdf_primitive *prims, int count, int 
         cap = uint64_t(got) + 1024;
     }
     return fail(SVO_ERR_DEVICE, "SDF voxel count did not converge");
+}
+
+static int terrain_check(int n) {
+    if (n < 8 || n > 8192 || (n & (n - 1))) return fail(SVO_ERR_INVALID, "terrain resolution must be a power of two in [8, 8192]");
+    return SVO_OK;
+}
+
+// Per-column heights of the terrain on the device.
+static int terrain_columns(Arena &ar, int n, uint32_t seed, float *&hh, int32_t *&H) {
+    const uint64_t cols = uint64_t(n) * n;
+    int r;
+    if ((r = ar.alloc(hh, cols)) || (r = ar.alloc(H, cols))) return r;
+    hipLaunchKernelGGL(terrain_heights_kernel, dim3(grid_for(cols)), dim3(kThreads), 0, ar.stream, n, seed, hh, H);
+    HIP_TRY(hipGetLastError());
+    return SVO_OK;
+}
+
+// Exposed-shell voxels of the terrain as sorted (morton key, word) pairs.
+static int terrain_leaves(Arena &ar, int n, uint32_t seed, uint64_t *&key, uint32_t *&word, uint64_t &V) {
+    hipStream_t st = ar.stream;
+    const uint64_t cols = uint64_t(n) * n;
+    float *hh;
+    int32_t *H;
+    uint32_t *cnt;
+    uint64_t *at;
+    int r;
+    if ((r = terrain_columns(ar, n, seed, hh, H))) return r;
+    if ((r = ar.alloc(cnt, cols)) || (r = ar.alloc(at, cols))) return r;
+    hipLaunchKernelGGL(terrain_count_kernel, dim3(grid_for(cols)), dim3(kThreads), 0, st, n, H, cnt);
+    HIP_TRY(hipGetLastError());
+    if ((r = exclusive_scan(ar, cnt, at, cols, st)) || (r = scan_total(cnt, at, cols, V))) return r;
+    ar.release(cnt);
+    uint64_t *k;
+    uint32_t *w;
+    if ((r = ar.alloc(k, V)) || (r = ar.alloc(w, V))) return r;
+    hipLaunchKernelGGL(terrain_emit_kernel, dim3(grid_for(cols)), dim3(kThreads), 0, st, n, H, hh, at, k, w);
+    HIP_TRY(hipGetLastError());
+    ar.release(at);
+    ar.release(hh);
+    ar.release(H);
+    uint64_t *k2;
+    uint32_t *w2;
+    if ((r = ar.alloc(k2, V)) || (r = ar.alloc(w2, V))) return r;
+    size_t bytes = 0;
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, k, k2, w, w2, V, 0, 3 * log2i(n), st));
+    uint8_t *tmp;
+    if ((r = ar.alloc(tmp, bytes))) return r;
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(tmp, bytes, k, k2, w, w2, V, 0, 3 * log2i(n), st));
+    ar.release(tmp);
+    ar.release(k);
+    ar.release(w);
+    key = k2;
+    word = w2;
+    return SVO_OK;
+}
+
+int build_terrain(int n, uint32_t seed, const svo_build_opts *opts, svo_octree &out) {
+    int r = terrain_check(n);
+    if (r) return r;
+    Geometry g{};
+    g.dataW = g.dataH = g.dataD = n;
+    if ((r = choose_cache(g, budget(opts), false))) return r;
+    uint64_t nBlocks;
+    if ((r = check_blocks(g, nBlocks))) return r;
+    Arena ar;
+    HIP_TRY(hipStreamCreate(&ar.stream));
+    hipStream_t st = ar.stream;
+    uint64_t *key, V;
+    uint32_t *word;
+    if ((r = terrain_leaves(ar, n, seed, key, word, V))) return r;
+    uint32_t *blockFlags;
+    if ((r = ar.alloc(blockFlags, nBlocks)) || (r = fill32(blockFlags, nBlocks, 0, st))) return r;
+    uint64_t *lkey, L;
+    uint32_t *lword;
+    if ((r = finish_leaves(ar, g, key, word, V, false, blockFlags, nBlocks, st, lkey, lword, L))) return r;
+    ar.release(key);
+    ar.release(word);
+    const bool multi = g.cacheLog < g.depth;
+    return emit_tree(ar, g, lkey, lword, L, multi ? blockFlags : nullptr, st, out);
+}
+
+int terrain_volume(int n, uint32_t seed, uint32_t *grid) {
+    int r = terrain_check(n);
+    if (r) return r;
+    if (n > 1024) return fail(SVO_ERR_LIMIT, "dense terrain volumes are limited to 1024^3");
+    Arena ar;
+    HIP_TRY(hipStreamCreate(&ar.stream));
+    hipStream_t st = ar.stream;
+    uint64_t *key, V;
+    uint32_t *word;
+    if ((r = terrain_leaves(ar, n, seed, key, word, V))) return r;
+    const uint64_t N = uint64_t(n) * n * n;
+    uint32_t *dg;
+    if ((r = ar.alloc(dg, N)) || (r = fill32(dg, N, 0, st))) return r;
+    if (V) hipLaunchKernelGGL(scatter_dense_kernel, dim3(grid_for(V)), dim3(kThreads), 0, st, key, word, V, n, dg);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(st));
+    HIP_TRY(hipMemcpy(grid, dg, N * 4, hipMemcpyDeviceToHost));
+    return SVO_OK;
+}
+
+int terrain_heights(int n, uint32_t seed, int32_t *heights, float *h) {
+    int r = terrain_check(n);
+    if (r) return r;
+    Arena ar;
+    HIP_TRY(hipStreamCreate(&ar.stream));
+    float *hh;
+    int32_t *H;
+    if ((r = terrain_columns(ar, n, seed, hh, H))) return r;
+    HIP_TRY(hipStreamSynchronize(ar.stream));
+    const uint64_t cols = uint64_t(n) * n;
+    if (heights) HIP_TRY(hipMemcpy(heights, H, cols * 4, hipMemcpyDeviceToHost));
+    if (h) HIP_TRY(hipMemcpy(h, hh, cols * 4, hipMemcpyDeviceToHost));
+    return SVO_OK;
 }
 
 int build_sdf(const svo_sdf_primitive *prims, int count, int n, float shade, const svo_build_opts *opts, svo_octree &out) {

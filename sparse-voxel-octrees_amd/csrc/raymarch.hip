@@ -12,6 +12,7 @@
 //  - word indices are 32-bit when the tree has < 2^32 words, else 64-bit.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdio>
 
 #include "march.hpp"
@@ -81,6 +82,19 @@ struct PrimaryParams {
     int width, height, tilesX;
 };
 
+// ray generation: src/Main.cpp:97-113 with closed-form dx, dy
+__device__ __forceinline__ void primary_dir(const PrimaryParams &p, int x, int y, float d[3]) {
+    const float dy = p.aspect - float(y) * p.scale;
+    const float dx = -1.0f + float(x) * p.scale;
+    const float vx = dx * p.a11 + dy * p.a12 + p.zx;
+    const float vy = dx * p.a21 + dy * p.a22 + p.zy;
+    const float vz = dx * p.a31 + dy * p.a32 + p.zz;
+    const float s = quake_rsqrt(vx * vx + vy * vy + vz * vz);
+    d[0] = vx * s;
+    d[1] = vy * s;
+    d[2] = vz * s;
+}
+
 // 8x8 pixel tile per wave64; lanes raster the tile row by row.
 template <typename Idx>
 __global__ void __launch_bounds__(64) primary_kernel(const uint32_t *__restrict__ tree, PrimaryParams p,
@@ -92,19 +106,47 @@ __global__ void __launch_bounds__(64) primary_kernel(const uint32_t *__restrict_
     const int x = (tile % p.tilesX) * 8 + (lane & 7);
     const int y = (tile / p.tilesX) * 8 + (lane >> 3);
     if (x >= p.width || y >= p.height) return;
-    // ray generation: src/Main.cpp:97-113 with closed-form dx, dy
-    const float dy = p.aspect - float(y) * p.scale;
-    const float dx = -1.0f + float(x) * p.scale;
-    const float vx = dx * p.a11 + dy * p.a12 + p.zx;
-    const float vy = dx * p.a21 + dy * p.a22 + p.zy;
-    const float vz = dx * p.a31 + dy * p.a32 + p.zz;
-    const float s = quake_rsqrt(vx * vx + vy * vy + vz * vz);
+    float d[3];
+    primary_dir(p, x, y, d);
     Stack<Idx> stk(lds, 64, slots, lane);
     uint32_t normal = 0;
     float t = 0.0f;
     Idx leaf = 0;
-    int code = march_ray<Idx>(tree, p.px, p.py, p.pz, vx * s, vy * s, vz * s, 0.0f, stk, normal, t, leaf);
+    int code = march_ray<Idx>(tree, p.px, p.py, p.pz, d[0], d[1], d[2], 0.0f, stk, normal, t, leaf);
     store_result<Idx>(code, normal, t, leaf, int64_t(y) * p.width + x, hit, nrm, tt, lf);
+}
+
+// Hard shadows: one ray per primary leaf hit, from o + d*(t - eps) (the hit
+// leaf's entry point pulled back along the primary ray) toward the light, with
+// no LOD cut-off. shadow = 1 when it hits any voxel, 0 when it escapes or the
+// pixel had no primary hit. Same 8x8 tiles as the primary pass, so a wave's
+// shadow rays start from neighbouring surface points.
+template <typename Idx>
+__global__ void __launch_bounds__(64) shadow_kernel(const uint32_t *__restrict__ tree, PrimaryParams p, float lx,
+                                                    float ly, float lz, float eps, int slots,
+                                                    const uint8_t *__restrict__ hit, const float *__restrict__ tt,
+                                                    uint8_t *__restrict__ shadow) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int tile = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int x = (tile % p.tilesX) * 8 + (lane & 7);
+    const int y = (tile / p.tilesX) * 8 + (lane >> 3);
+    if (x >= p.width || y >= p.height) return;
+    const int64_t i = int64_t(y) * p.width + x;
+    if (hit[i] != 1) {
+        shadow[i] = 0;
+        return;
+    }
+    float d[3];
+    primary_dir(p, x, y, d);
+    const float back = tt[i] - eps;
+    Stack<Idx> stk(lds, 64, slots, lane);
+    uint32_t normal = 0;
+    float t = 0.0f;
+    Idx leaf = 0;
+    const int code = march_ray<Idx>(tree, p.px + d[0] * back, p.py + d[1] * back, p.pz + d[2] * back, lx, ly, lz, 0.0f,
+                                    stk, normal, t, leaf);
+    shadow[i] = (code == 1 || code == 2) ? 1 : 0;
 }
 
 int stack_slots(const svo_octree &h) { return h.depth > 2 ? h.depth - 1 : 1; }
@@ -172,12 +214,9 @@ int raymarch_device(svo_octree &h, int64_t n, const float *o, const float *d, co
     return SVO_OK;
 }
 
-int render_primary_device(svo_octree &h, const svo_camera &cam, int width, int height, uint8_t *hit,
-                          uint32_t *normal, float *t, uint64_t *leaf, void *stream) {
-    if (width <= 0 || height <= 0 || width > 65536 || height > 65536)
-        return fail(SVO_ERR_INVALID, "image size out of range");
-    int r = ensure_device(h);
-    if (r) return r;
+namespace {
+
+PrimaryParams primary_params(const svo_camera &cam, int width, int height) {
     PrimaryParams p;
     const float pi = float(3.14159265358979323846);
     const float planeDist = 1.0f / std::tan(pi / 6.0f);  // src/Main.cpp:158
@@ -193,6 +232,18 @@ int render_primary_device(svo_octree &h, const svo_camera &cam, int width, int h
     p.width = width;
     p.height = height;
     p.tilesX = (width + 7) / 8;
+    return p;
+}
+
+}  // namespace
+
+int render_primary_device(svo_octree &h, const svo_camera &cam, int width, int height, uint8_t *hit,
+                          uint32_t *normal, float *t, uint64_t *leaf, void *stream) {
+    if (width <= 0 || height <= 0 || width > 65536 || height > 65536)
+        return fail(SVO_ERR_INVALID, "image size out of range");
+    int r = ensure_device(h);
+    if (r) return r;
+    const PrimaryParams p = primary_params(cam, width, height);
     const int tilesY = (height + 7) / 8;
     const int slots = stack_slots(h);
     const bool wide = h.words.size() >= (uint64_t(1) << 32);
@@ -203,6 +254,30 @@ int render_primary_device(svo_octree &h, const svo_camera &cam, int width, int h
         hipLaunchKernelGGL(primary_kernel<uint32_t>, grid, dim3(64), lds, s, h.dwords, p, slots, hit, normal, t, leaf);
     else
         hipLaunchKernelGGL(primary_kernel<uint64_t>, grid, dim3(64), lds, s, h.dwords, p, slots, hit, normal, t, leaf);
+    HIP_TRY(hipGetLastError());
+    return SVO_OK;
+}
+
+int render_shadow_device(svo_octree &h, const svo_camera &cam, int width, int height, const float light[3], float eps,
+                         uint8_t *hit, uint32_t *normal, float *t, uint64_t *leaf, uint8_t *shadow, void *stream) {
+    if (!hit || !t || !shadow || !light) return fail(SVO_ERR_INVALID, "shadow pass needs hit, t, shadow and light");
+    if (!(std::fabs(light[0]) + std::fabs(light[1]) + std::fabs(light[2]) > 0.0f))
+        return fail(SVO_ERR_INVALID, "light direction must be non-zero");
+    int r = render_primary_device(h, cam, width, height, hit, normal, t, leaf, stream);
+    if (r) return r;
+    const PrimaryParams p = primary_params(cam, width, height);
+    const int tilesY = (height + 7) / 8;
+    const int slots = stack_slots(h);
+    const bool wide = h.words.size() >= (uint64_t(1) << 32);
+    const size_t lds = size_t(slots) * 64 * ((wide ? 8 : 4) + sizeof(float));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    dim3 grid(unsigned(p.tilesX * tilesY));
+    if (!wide)
+        hipLaunchKernelGGL(shadow_kernel<uint32_t>, grid, dim3(64), lds, s, h.dwords, p, light[0], light[1], light[2], eps,
+                           slots, hit, t, shadow);
+    else
+        hipLaunchKernelGGL(shadow_kernel<uint64_t>, grid, dim3(64), lds, s, h.dwords, p, light[0], light[1], light[2], eps,
+                           slots, hit, t, shadow);
     HIP_TRY(hipGetLastError());
     return SVO_OK;
 }

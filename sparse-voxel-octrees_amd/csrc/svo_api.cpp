@@ -303,6 +303,38 @@ int svo_render_primary(svo_octree *h, const svo_camera *cam, int width, int heig
     return SVO_OK;
 }
 
+int svo_render_shadow_device(svo_octree *h, const svo_camera *cam, int width, int height, const float light[3],
+                             float eps, uint8_t *hit, uint32_t *normal, float *t, uint64_t *leaf, uint8_t *shadow,
+                             void *stream) {
+    clear_error();
+    if (!h || !cam) return fail(SVO_ERR_INVALID, "null argument");
+    return render_shadow_device(*h, *cam, width, height, light, eps, hit, normal, t, leaf, shadow, stream);
+}
+
+int svo_render_shadow(svo_octree *h, const svo_camera *cam, int width, int height, const float light[3], float eps,
+                      uint8_t *hit, uint32_t *normal, float *t, uint64_t *leaf, uint8_t *shadow) {
+    clear_error();
+    if (!h || !cam || !light || !shadow) return fail(SVO_ERR_INVALID, "null argument");
+    if (width <= 0 || height <= 0) return fail(SVO_ERR_INVALID, "image size out of range");
+    size_t m = size_t(width) * size_t(height);
+    DevBuf<uint8_t> dH, dS;
+    DevBuf<uint32_t> dN;
+    DevBuf<float> dT;
+    DevBuf<uint64_t> dL;
+    int r;
+    if ((r = dH.alloc(m)) || (r = dS.alloc(m)) || (r = dT.alloc(m)) || (r = out_buf(dN, normal, m)) ||
+        (r = out_buf(dL, leaf, m)))
+        return r;
+    r = render_shadow_device(*h, *cam, width, height, light, eps, dH.p, dN.p, dT.p, dL.p, dS.p, nullptr);
+    if (r) return r;
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) return fail(SVO_ERR_DEVICE, std::string("render: ") + hipGetErrorString(e));
+    if ((r = from_device(hit, dH, m)) || (r = from_device(normal, dN, m)) || (r = from_device(t, dT, m)) ||
+        (r = from_device(leaf, dL, m)) || (r = from_device(shadow, dS, m)))
+        return r;
+    return SVO_OK;
+}
+
 int svo_render_frame_device(svo_octree *h, const svo_camera *cam, int width, int height, int stripes, int stride,
                             uint32_t *rgba, float *depth, void *stream) {
     clear_error();
@@ -411,6 +443,28 @@ int svo_sdf_volume(const svo_sdf_primitive *prims, int count, int resolution, fl
     clear_error();
     if (!prims || !grid) return fail(SVO_ERR_INVALID, "null argument");
     return sdf_volume(prims, count, resolution, shade, grid);
+}
+
+int svo_build_from_terrain(int resolution, uint32_t seed, const svo_build_opts *opts, svo_octree **out) {
+    clear_error();
+    if (!out) return fail(SVO_ERR_INVALID, "null argument");
+    *out = nullptr;
+    std::unique_ptr<svo_octree> t(new (std::nothrow) svo_octree());
+    if (!t) return fail(SVO_ERR_NOMEM, "out of host memory");
+    int r = build_terrain(resolution, seed, opts, *t);
+    if (r) return r;
+    return finish_handle(t, out);
+}
+
+int svo_terrain_volume(int resolution, uint32_t seed, uint32_t *grid) {
+    clear_error();
+    if (!grid) return fail(SVO_ERR_INVALID, "null argument");
+    return terrain_volume(resolution, seed, grid);
+}
+
+int svo_terrain_heights(int resolution, uint32_t seed, int32_t *heights, float *h) {
+    clear_error();
+    return terrain_heights(resolution, seed, heights, h);
 }
 
 int svo_build_from_voxels(const char *path, const svo_build_opts *opts, svo_octree **out) {

@@ -63,6 +63,9 @@ int render_primary_device(svo_octree &h, const svo_camera &cam, int width, int h
                           uint32_t *normal, float *t, uint64_t *leaf, void *stream);
 int device_count();
 
+int render_shadow_device(svo_octree &h, const svo_camera &cam, int width, int height, const float light[3], float eps,
+                         uint8_t *hit, uint32_t *normal, float *t, uint64_t *leaf, uint8_t *shadow, void *stream);
+
 // ---- viewer frame (frame.hip) ----
 int render_frame_device(svo_octree &h, const svo_camera &cam, int width, int height, int stripes, int stride,
                         uint32_t *rgba, float *depth, void *stream);
@@ -86,6 +89,9 @@ int build_grid(const uint32_t *grid, int w, int h, int d, const svo_build_opts *
 int build_sdf(const svo_sdf_primitive *prims, int count, int n, float shade, const svo_build_opts *opts,
               svo_octree &out);
 int sdf_volume(const svo_sdf_primitive *prims, int count, int n, float shade, uint32_t *grid);
+int build_terrain(int n, uint32_t seed, const svo_build_opts *opts, svo_octree &out);
+int terrain_volume(int n, uint32_t seed, uint32_t *grid);
+int terrain_heights(int n, uint32_t seed, int32_t *heights, float *h);
 int read_ply(const char *path, std::vector<float> &verts, int &has_normals,
              std::vector<int32_t> &face_counts, std::vector<int32_t> &face_idx);
 

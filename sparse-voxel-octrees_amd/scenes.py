@@ -257,7 +257,8 @@ def sdf_volume(n, seed, shade=0.8, compress=None, brick=8):
 
 
 def compress_material_np(n, shade):
-    """Vectorised compressMaterial (src/Util.hpp:64-84) for float32 normals."""
+    """Vectorised compressMaterial (src/Util.hpp:64-84) for float32 normals
+    and a scalar or per-normal shade."""
     n = np.asarray(n, dtype=np.float32)
     a = np.abs(n)
     face = np.zeros(len(n), dtype=np.int64)
@@ -276,7 +277,7 @@ def compress_material_np(n, shade):
     f32 = np.float32
     u = ((n1 * f32(0.5) + f32(0.5)) * f32(2047.0)).astype(np.float32)
     v = ((n2 * f32(0.5) + f32(0.5)) * f32(2047.0)).astype(np.float32)
-    c = np.float32(np.float32(shade) * f32(127.0))
+    c = (np.broadcast_to(np.asarray(shade, dtype=f32), (len(n),)) * f32(127.0)).astype(f32)
 
     def trunc(x):
         bad = ~(np.abs(x) < 2147483648.0)
@@ -286,9 +287,158 @@ def compress_material_np(n, shade):
 
     ui = np.minimum(trunc(u), 0x7FF).astype(np.int64) & 0xFFFFFFFF
     vi = np.minimum(trunc(v), 0x7FF).astype(np.int64) & 0xFFFFFFFF
-    ci = min(int(c) if abs(float(c)) < 2147483648.0 else -(1 << 31), 0x7F) & 0xFFFFFFFF
+    ci = np.minimum(trunc(c), 0x7F).astype(np.int64) & 0xFFFFFFFF
     w = (sign.astype(np.int64) << 31) | (face << 29) | ((ui << 18) & 0xFFFFFFFF) | ((vi << 7) & 0xFFFFFFFF) | ci
     return (w & 0xFFFFFFFF).astype(np.uint32)
+
+
+# ---------------------------------------------------------------------------
+# fBm heightfield terrain
+# ---------------------------------------------------------------------------
+#
+# Definition shared bit for bit with the GPU terrain source (float32, every
+# operation separately rounded, in this order):
+#   lattice value  v(i, k, s) = (mix(i, k, s) >> 8) * 2^-24, mix = uint32 hash below
+#   noise(u, w)    bilinear over the cell with smoothstep weights c*c*(3 - 2c)
+#   h(x, z)        sum_{o<8} 0.5^o * noise(u*f_o, w*f_o, seed+o) / sum_o 0.5^o,
+#                  u = (x+0.5)/n, w = (z+0.5)/n, f_o = 4*2^o
+#   height H       n/8 + int(h * (n/2)); voxel (x, y, z) is solid for y <= H(x, z)
+#   exposed shell  y in [min(H, Hmin + 1), H], Hmin = least height of the
+#                  4-neighbour columns inside the grid (outside counts as solid)
+#   normal         normalize(-gx, 1, -gz), g = (hv(+1) - hv(-1)) * 0.5 with
+#                  hv = h * (n/2) and neighbour indices clamped to the grid
+#   word           compressMaterial(normal, 0.35 + 0.65*h), 0 bumped to 1
+TERRAIN_OCTAVES = 8
+TERRAIN_BASE_FREQ = 4.0
+
+
+def _mix32(i, k, s):
+    """uint32 lattice hash (wrapping arithmetic)."""
+    u32 = np.uint32
+    with np.errstate(over="ignore"):
+        h = (i.astype(u32) * u32(0x8DA6B343)) ^ (k.astype(u32) * u32(0xD8163841)) ^ (u32(s) * u32(0x9E3779B9))
+        h ^= h >> u32(15)
+        h *= u32(0x2C1B3C6D)
+        h ^= h >> u32(12)
+        h *= u32(0x297A2D39)
+        h ^= h >> u32(15)
+    return h
+
+
+def _lattice(i, k, s):
+    return (_mix32(i, k, s) >> np.uint32(8)).astype(np.float32) * np.float32(1.0 / 16777216.0)
+
+
+def _smooth(c):
+    f32 = np.float32
+    return (c * c) * (f32(3.0) - f32(2.0) * c)
+
+
+def terrain_h(x, z, n, seed):
+    """Normalised fBm height h in [0, 1) for integer column coordinates."""
+    f32 = np.float32
+    u = (np.asarray(x).astype(f32) + f32(0.5)) / f32(n)
+    w = (np.asarray(z).astype(f32) + f32(0.5)) / f32(n)
+    total = np.zeros(u.shape, f32)
+    norm, amp, freq = f32(0.0), f32(1.0), f32(TERRAIN_BASE_FREQ)
+    for o in range(TERRAIN_OCTAVES):
+        pu, pw = u * freq, w * freq
+        fu, fw = np.floor(pu), np.floor(pw)
+        cu, cw = _smooth(pu - fu), _smooth(pw - fw)
+        iu, iw = fu.astype(np.int64), fw.astype(np.int64)
+        s = (seed + o) & 0xFFFFFFFF
+        v00, v10 = _lattice(iu, iw, s), _lattice(iu + 1, iw, s)
+        v01, v11 = _lattice(iu, iw + 1, s), _lattice(iu + 1, iw + 1, s)
+        a0 = v00 + (v10 - v00) * cu
+        a1 = v01 + (v11 - v01) * cu
+        total = total + amp * (a0 + (a1 - a0) * cw)
+        norm = f32(norm + amp)
+        amp = f32(amp * f32(0.5))
+        freq = f32(freq * f32(2.0))
+    return total / norm
+
+
+def terrain_heights(n, seed, x=None, z=None):
+    """Integer heights H and scaled heights hv = h*(n/2) on columns (x, z)
+    (default: the whole grid, indexed [z, x])."""
+    if x is None:
+        z, x = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    h = terrain_h(x, z, n, seed)
+    hv = h * np.float32(n // 2)
+    return (n // 8 + hv.astype(np.int32)).astype(np.int32), hv, h
+
+
+def terrain_volume(n, seed, compress=None):
+    """Dense uint32 (n,n,n) [z, y, x] exposed-shell volume of the terrain (small n)."""
+    if compress is None:
+        compress = compress_material_np
+    f32 = np.float32
+    H, hv, h = terrain_heights(n, seed)
+    big = np.iinfo(np.int32).max
+    pad = np.pad(H, 1, constant_values=big)
+    hmin = np.minimum(np.minimum(pad[1:-1, :-2], pad[1:-1, 2:]), np.minimum(pad[:-2, 1:-1], pad[2:, 1:-1]))
+    lo = np.minimum(H, np.where(hmin == big, H, hmin + 1))
+    idx = np.arange(n)
+    xm, xp = np.maximum(idx - 1, 0), np.minimum(idx + 1, n - 1)
+    gx = (hv[:, xp] - hv[:, xm]) * f32(0.5)
+    gz = (hv[xp, :] - hv[xm, :]) * f32(0.5)
+    a = np.stack([-gx, np.ones_like(gx), -gz], axis=-1).astype(f32)
+    inv = f32(1.0) / np.sqrt((a[..., 0] * a[..., 0] + a[..., 1] * a[..., 1]) + a[..., 2] * a[..., 2])
+    nrm = a * inv[..., None]
+    shade = f32(0.35) + f32(0.65) * h
+    words = compress(nrm.reshape(-1, 3), shade.reshape(-1)).reshape(n, n)
+    words[words == 0] = 1
+    vol = np.zeros((n, n, n), np.uint32)
+    y = np.arange(n)[None, :, None]
+    occ = (y >= lo[:, None, :]) & (y <= H[:, None, :])
+    vol[occ] = np.broadcast_to(words[:, None, :], vol.shape)[occ]
+    return vol
+
+
+def terrain_view(seed):
+    """Seeded camera above the terrain and a directional light: orbit pitch
+    U[-50,-25] (negative pitch looks down), yaw U[0,360), radius 1; light
+    elevation U[20,60] degrees, azimuth U[0,360). Returns (pitch, yaw, light
+    float32[3] unit vector toward the light)."""
+    rng = np.random.default_rng(seed)
+    pitch = float(np.float32(rng.uniform(-50.0, -25.0)))
+    yaw = float(np.float32(rng.uniform(0.0, 360.0)))
+    el = math.radians(rng.uniform(20.0, 60.0))
+    az = math.radians(rng.uniform(0.0, 360.0))
+    light = np.array([math.cos(el) * math.cos(az), math.sin(el), math.cos(el) * math.sin(az)], np.float64)
+    return pitch, yaw, (light / np.linalg.norm(light)).astype(np.float32)
+
+
+def terrain_volume_words(n, seed, x, z, compress=None):
+    """Leaf words of columns (x, z) (every voxel of a column shares one word)."""
+    if compress is None:
+        compress = compress_material_np
+    f32 = np.float32
+    x, z = np.asarray(x), np.asarray(z)
+    half = f32(n // 2)
+    h = terrain_h(x, z, n, seed)
+    hx = [terrain_h(np.minimum(x + 1, n - 1), z, n, seed), terrain_h(np.maximum(x - 1, 0), z, n, seed)]
+    hz = [terrain_h(x, np.minimum(z + 1, n - 1), n, seed), terrain_h(x, np.maximum(z - 1, 0), n, seed)]
+    gx = (hx[0] * half - hx[1] * half) * f32(0.5)
+    gz = (hz[0] * half - hz[1] * half) * f32(0.5)
+    a = np.stack([-gx, np.ones_like(gx), -gz], axis=-1).astype(f32)
+    inv = f32(1.0) / np.sqrt((a[:, 0] * a[:, 0] + a[:, 1] * a[:, 1]) + a[:, 2] * a[:, 2])
+    w = compress((a * inv[:, None]).astype(f32), f32(0.35) + f32(0.65) * h)
+    w[w == 0] = 1
+    return w
+
+
+def terrain_shell_count(H):
+    """Number of exposed-shell voxels implied by the column heights H [z, x]."""
+    big = np.iinfo(np.int32).max
+    H = np.asarray(H, np.int32)
+    m = np.full(H.shape, big, np.int32)
+    m[:, 1:] = np.minimum(m[:, 1:], H[:, :-1])
+    m[:, :-1] = np.minimum(m[:, :-1], H[:, 1:])
+    m[1:, :] = np.minimum(m[1:, :], H[:-1, :])
+    m[:-1, :] = np.minimum(m[:-1, :], H[1:, :])
+    lo = np.minimum(H, np.where(m == big, H, m + 1))
+    return int((H.astype(np.int64) - lo + 1).sum())
 
 
 def write_voxel(path, vol):

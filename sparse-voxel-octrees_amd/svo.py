@@ -31,6 +31,8 @@ EXPORTS = [
     "svo_renderer_create", "svo_renderer_free", "svo_render_batch_device",
     "svo_build_from_sdf", "svo_sdf_volume", "svo_serve_replay",
     "svo_render_frame", "svo_render_frame_device", "svo_frame_depth_count",
+    "svo_build_from_terrain", "svo_terrain_volume", "svo_terrain_heights",
+    "svo_render_shadow", "svo_render_shadow_device",
 ]
 
 
@@ -77,6 +79,23 @@ def sdf_volume(prims, resolution, shade=0.8):
     out = np.zeros((n, n, n), np.uint32)
     _check(lib.svo_sdf_volume(arr, len(arr), n, float(shade), _p(out)))
     return out
+
+
+def terrain_volume(resolution, seed):
+    """Dense uint32 [n,n,n] ([z,y,x]) of the GPU terrain source's voxels."""
+    n = int(resolution)
+    out = np.zeros((n, n, n), np.uint32)
+    _check(lib.svo_terrain_volume(n, int(seed) & 0xFFFFFFFF, _p(out)))
+    return out
+
+
+def terrain_heights(resolution, seed):
+    """(H int32 [n,n], h float32 [n,n]) per column, indexed [z, x]."""
+    n = int(resolution)
+    H = np.zeros((n, n), np.int32)
+    h = np.zeros((n, n), np.float32)
+    _check(lib.svo_terrain_heights(n, int(seed) & 0xFFFFFFFF, _p(H), _p(h)))
+    return H, h
 
 
 class Request(ctypes.Structure):
@@ -157,6 +176,11 @@ def _load():
         "svo_render_frame": [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P],
         "svo_render_frame_device": [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, P],
         "svo_frame_depth_count": [ctypes.c_int, ctypes.c_int, ctypes.c_int],
+        "svo_build_from_terrain": [ctypes.c_int, ctypes.c_uint32, P, P],
+        "svo_terrain_volume": [ctypes.c_int, ctypes.c_uint32, P],
+        "svo_terrain_heights": [ctypes.c_int, ctypes.c_uint32, P, P],
+        "svo_render_shadow": [P, P, ctypes.c_int, ctypes.c_int, P, F, P, P, P, P, P],
+        "svo_render_shadow_device": [P, P, ctypes.c_int, ctypes.c_int, P, F, P, P, P, P, P, P],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -268,6 +292,14 @@ class VoxelOctree:
         return cls(_handle=h)
 
     @classmethod
+    def build_from_terrain(cls, resolution, seed, memory_budget=0):
+        """GPU fBm heightfield terrain source (exposed shell, y up)."""
+        h = ctypes.c_void_p()
+        o = _opts(memory_budget, 0)
+        _check(lib.svo_build_from_terrain(int(resolution), int(seed) & 0xFFFFFFFF, ctypes.byref(o), ctypes.byref(h)))
+        return cls(_handle=h)
+
+    @classmethod
     def build_from_voxels(cls, path, memory_budget=0):
         h = ctypes.c_void_p()
         o = _opts(memory_budget, 0)
@@ -375,6 +407,23 @@ class VoxelOctree:
                               stream=None):
         _check(lib.svo_render_primary_device(self._h, ctypes.byref(cam), int(width), int(height), hit_ptr, normal_ptr,
                                              t_ptr, leaf_ptr, stream))
+
+    def render_shadow(self, cam, width, height, light, eps):
+        """Primary rays plus one hard-shadow ray per leaf hit toward `light`."""
+        n = width * height
+        out = {"hit": np.zeros(n, np.uint8), "normal": np.zeros(n, np.uint32), "t": np.zeros(n, np.float32),
+               "leaf": np.zeros(n, np.uint64), "shadow": np.zeros(n, np.uint8)}
+        lt = np.ascontiguousarray(light, np.float32)
+        _check(lib.svo_render_shadow(self._h, ctypes.byref(cam), int(width), int(height), _p(lt), float(eps),
+                                     _p(out["hit"]), _p(out["normal"]), _p(out["t"]), _p(out["leaf"]),
+                                     _p(out["shadow"])))
+        return out
+
+    def render_shadow_device(self, cam, width, height, light, eps, hit_ptr, t_ptr, shadow_ptr, normal_ptr=None,
+                             leaf_ptr=None, stream=None):
+        lt = np.ascontiguousarray(light, np.float32)
+        _check(lib.svo_render_shadow_device(self._h, ctypes.byref(cam), int(width), int(height), _p(lt), float(eps),
+                                            hit_ptr, normal_ptr, t_ptr, leaf_ptr, shadow_ptr, stream))
 
     def render_frame(self, cam, width, height, stripes=16, stride=1):
         """The viewer's shaded frame (src/Main.cpp:81-202): (rgba uint32

@@ -31,8 +31,12 @@ Fixtures written (all small):
                          node counts) for a 1,997,120-triangle displaced
                          icosphere at 2048^3, 8 threads, 1 GiB
 
-`--only frames` or `--only ico2048` regenerates just that part; other entries of
-builds.json are kept.
+  build_terrain_*.oct + builds.json["terrain"]
+                         the reference's file-source build of the fBm terrain
+                         volume (scenes.terrain_volume) at 128^3 and 256^3
+
+`--only frames`, `--only ico2048` or `--only terrain` regenerates just that part;
+other entries of builds.json are kept.
 """
 import argparse
 import hashlib
@@ -202,6 +206,24 @@ def frame_fixture(name, oct_path, cams, tmp):
     np.savez_compressed(os.path.join(HERE, f"frame_{name}.npz"), **out)
 
 
+TERRAIN_CASES = [(128, 11, 1 << 30), (256, 12, 1 << 22), (256, 13, 1 << 30)]
+
+
+def terrain_builds(tmp):
+    """Reference file-source builds of terrain volumes (n, seed, budget)."""
+    out = []
+    for n, seed, mem in TERRAIN_CASES:
+        vol = scenes.terrain_volume(n, seed, compress=oracle.compress_material_batch)
+        vp = os.path.join(tmp, "terrain.voxel")
+        scenes.write_voxel(vp, vol)
+        op = os.path.join(HERE, f"build_terrain_n{n}_s{seed}_m{mem}.oct")
+        run_ref("build-voxel", vp, 8, mem, op)
+        out.append({"n": n, "seed": seed, "mem": mem, "oct": os.path.basename(op), "md5": md5(op),
+                    "voxels": int((vol != 0).sum()), "volume_md5": hashlib.md5(vol.tobytes()).hexdigest()})
+        print("terrain build", out[-1], flush=True)
+    return out
+
+
 def full_size_build(tmp):
     """Reference builder on the full-size mesh (about 10 s of reference build)."""
     e = {"mesh": "icosphere", "frequency": 316, "seed": 2048, "displacement": 0.05, "resolution": 2048,
@@ -275,7 +297,7 @@ GRID_CASES = [
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--reference", default="/root/reference")
-    ap.add_argument("--only", choices=["frames", "ico2048"], default=None)
+    ap.add_argument("--only", choices=["frames", "ico2048", "terrain"], default=None)
     args = ap.parse_args()
     if not os.path.exists(REF_BIN):
         subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref", f"REF={args.reference}"], check=True)
@@ -285,10 +307,13 @@ def main():
         shutil.copyfile(os.path.join(args.reference, "models", "XYZRGB-Dragon.oct"), dragon)
 
         bpath = os.path.join(HERE, "builds.json")
-        if args.only == "ico2048":
+        if args.only in ("ico2048", "terrain"):
             with open(bpath) as fp:
                 builds = json.load(fp)
-            builds["ico2048"] = full_size_build(tmp)
+            if args.only == "ico2048":
+                builds["ico2048"] = full_size_build(tmp)
+            else:
+                builds["terrain"] = terrain_builds(tmp)
             with open(bpath, "w") as fp:
                 json.dump(builds, fp, indent=1)
             return
@@ -399,6 +424,8 @@ def main():
         builds["sdf128"] = {"seed": 5, "n": 128, "mem": 1 << 30, "md5": md5(op),
                             "volume_md5": hashlib.md5(vol.tobytes()).hexdigest()}
         raymarch_fixture("sdf128", op, 12, tmp, n_random=4000, cams=3, w=80, h=45)
+
+        builds["terrain"] = terrain_builds(tmp)
 
         # mesh builds ----------------------------------------------------------
         for name, freq, seed, disp, flat, attrs, ascii in MESH_CASES:

@@ -15,6 +15,7 @@ What pins what:
   threads, budget); plus the invariant that the emitted leaves are exactly
   the voxelization when no hazard applies.
 """
+import hashlib
 import json
 import os
 
@@ -115,6 +116,35 @@ def test_frame_invariants(golden):
         base += tiles_x * tiles_y
     assert base == len(d1)
     assert 0 < (d1 < f(1e10)).mean() < 1
+
+
+def test_terrain_builds_match_reference(golden, svo):
+    """fBm terrain volumes (exposed shell) through the oracle's file-source
+    builder equal the reference builder's .oct for the same volume."""
+    for b in _builds(golden)["terrain"]:
+        vol = svo.scenes.terrain_volume(b["n"], b["seed"], compress=orc.compress_material_batch)
+        assert hashlib.md5(vol.tobytes()).hexdigest() == b["volume_md5"]
+        assert int((vol != 0).sum()) == b["voxels"]
+        c, words, _ = orc.build_grid(vol, b["mem"])
+        rc, rwords = orc.read_oct(os.path.join(golden, b["oct"]))
+        assert np.array_equal(words, rwords), b["oct"]
+        assert _eq_bits(c, rc)
+
+
+def test_terrain_shell_invariants(svo):
+    """The exposed shell is what the solid heightfield implies: every solid
+    voxel with an empty 6-neighbour (outside the grid counts as solid) is in
+    the shell, and nothing else is."""
+    n = 64
+    H, _, _ = svo.scenes.terrain_heights(n, 5)
+    vol = svo.scenes.terrain_volume(n, 5)
+    y = np.arange(n)[None, :, None]
+    solid = y <= H[:, None, :]  # [z, y, x]
+    pad = np.pad(solid, 1, constant_values=True)
+    exposed = solid & ~(pad[2:, 1:-1, 1:-1] & pad[:-2, 1:-1, 1:-1] & pad[1:-1, 2:, 1:-1] & pad[1:-1, :-2, 1:-1]
+                        & pad[1:-1, 1:-1, 2:] & pad[1:-1, 1:-1, :-2])
+    assert np.array_equal(vol != 0, exposed)
+    assert 0 < H.min() and H.max() < n
 
 
 def test_camera_matches_reference(golden):
