@@ -73,6 +73,11 @@ int svo_encode(const svo_octree *h, void **data, uint64_t *size);
 int svo_encode_words(const uint32_t *words, uint64_t count, const float center[3], void **data,
                      uint64_t *size);
 void svo_free_buffer(void *p);
+/* The .oct reader of VoxelOctree(const char*) (src/VoxelOctree.cpp:57-90)
+ * without tree validation, for any word array: decodes an in-memory image
+ * (slices in parallel on the host threads) into a malloc'd word array (free
+ * with svo_free_buffer) and its center. */
+int svo_decode_words(const void *data, uint64_t size, uint32_t **words, uint64_t *count, float center[3]);
 
 /* Wrap an existing word array (copied) with its center; validated. */
 int svo_from_words(const uint32_t *words, uint64_t count, const float center[3], svo_octree **out);

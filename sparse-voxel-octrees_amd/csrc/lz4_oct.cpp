@@ -16,7 +16,11 @@
 // table rebased once the stream passes 2 GiB). These parameters fix the
 // output, so they follow the reference; the code is written from the LZ4
 // block format and that list of parameters.
+#include <algorithm>
+#include <atomic>
 #include <cstring>
+#include <memory>
+#include <thread>
 
 #include "svo_internal.hpp"
 
@@ -51,9 +55,13 @@ inline uint64_t common_prefix(const uint8_t *a, const uint8_t *b, uint64_t limit
     return n;
 }
 
-inline void put_length(std::vector<uint8_t> &out, uint64_t extra) {
-    while (extra >= 255) { out.push_back(255); extra -= 255; }
-    out.push_back(uint8_t(extra));
+inline uint8_t *put_length(uint8_t *o, uint64_t extra) {
+    while (extra >= 255) {
+        *o++ = 255;
+        extra -= 255;
+    }
+    *o++ = uint8_t(extra);
+    return o;
 }
 
 }  // namespace
@@ -62,7 +70,9 @@ Lz4StreamEncoder::Lz4StreamEncoder() : base_(0), dictSize_(0), consumed_(0) {
     std::memset(table_, 0, sizeof(table_));
 }
 
-void Lz4StreamEncoder::encode(const uint8_t *src, uint64_t pos, uint32_t len, std::vector<uint8_t> &out) {
+uint64_t Lz4StreamEncoder::bound(uint32_t len) { return uint64_t(len) + len / 255 + 16; }
+
+uint64_t Lz4StreamEncoder::encode(const uint8_t *src, uint64_t pos, uint32_t len, uint8_t *dst) {
     // Stream bookkeeping for a slice that directly follows the previous one.
     if (consumed_ - base_ > kRebaseAt) {
         uint64_t nb = consumed_ - 65536;
@@ -78,20 +88,23 @@ void Lz4StreamEncoder::encode(const uint8_t *src, uint64_t pos, uint32_t len, st
     const uint64_t matchStartLimit = end - kMatchStartMargin;
     const uint64_t matchEndLimit = end - kEndLiterals;
     uint64_t anchor = pos;
+    uint8_t *o = dst;
 
     auto put = [&](uint64_t p) { table_[hash_at(src + p)] = uint32_t(p - base_); };
     auto lookup = [&](uint32_t h) { return base_ + table_[h]; };
 
-    auto emit_literals_token = [&](uint64_t ip) -> size_t {
-        uint64_t lit = ip - anchor;
-        size_t token = out.size();
+    // literal run [anchor, ip) behind a new token; returns the token
+    auto emit_literals = [&](uint64_t ip) -> uint8_t * {
+        const uint64_t lit = ip - anchor;
+        uint8_t *token = o++;
         if (lit >= 15) {
-            out.push_back(0xF0);
-            put_length(out, lit - 15);
+            *token = 0xF0;
+            o = put_length(o, lit - 15);
         } else {
-            out.push_back(uint8_t(lit << 4));
+            *token = uint8_t(lit << 4);
         }
-        out.insert(out.end(), src + anchor, src + ip);
+        std::memcpy(o, src + anchor, lit);
+        o += lit;
         return token;
     };
 
@@ -121,19 +134,19 @@ void Lz4StreamEncoder::encode(const uint8_t *src, uint64_t pos, uint32_t len, st
             // extend backwards
             while (ip > anchor && match > lowLimit && src[ip - 1] == src[match - 1]) { ip--; match--; }
 
-            size_t token = emit_literals_token(ip);
+            uint8_t *token = emit_literals(ip);
             for (;;) {
-                uint16_t off = uint16_t(ip - match);
-                out.push_back(uint8_t(off));
-                out.push_back(uint8_t(off >> 8));
+                const uint16_t off = uint16_t(ip - match);
+                *o++ = uint8_t(off);
+                *o++ = uint8_t(off >> 8);
                 uint64_t ml = common_prefix(src + ip + kMinMatch, src + match + kMinMatch,
                                             matchEndLimit - (ip + kMinMatch));
                 ip += kMinMatch + ml;
                 if (ml >= 15) {
-                    out[token] = uint8_t(out[token] + 15);
-                    put_length(out, ml - 15);
+                    *token = uint8_t(*token + 15);
+                    o = put_length(o, ml - 15);
                 } else {
-                    out[token] = uint8_t(out[token] + ml);
+                    *token = uint8_t(*token + ml);
                 }
                 anchor = ip;
                 if (ip > matchStartLimit) goto last_literals;
@@ -141,8 +154,8 @@ void Lz4StreamEncoder::encode(const uint8_t *src, uint64_t pos, uint32_t len, st
                 match = lookup(hash_at(src + ip));
                 put(ip);
                 if (match + kWindow >= ip && load32(src + match) == load32(src + ip)) {
-                    token = out.size();
-                    out.push_back(0);
+                    token = o++;
+                    *token = 0;
                     continue;  // immediate next match, no literals
                 }
                 break;
@@ -152,65 +165,146 @@ void Lz4StreamEncoder::encode(const uint8_t *src, uint64_t pos, uint32_t len, st
         }
     }
 last_literals : {
-    uint64_t run = end - anchor;
+    const uint64_t run = end - anchor;
     if (run >= 15) {
-        out.push_back(0xF0);
-        put_length(out, run - 15);
+        *o++ = 0xF0;
+        o = put_length(o, run - 15);
     } else {
-        out.push_back(uint8_t(run << 4));
+        *o++ = uint8_t(run << 4);
     }
-    out.insert(out.end(), src + anchor, src + end);
+    std::memcpy(o, src + anchor, run);
+    o += run;
 }
     dictSize_ += len;
     consumed_ = end;
+    return uint64_t(o - dst);
 }
 
-bool lz4_decode_block(const uint8_t *src, uint64_t srcLen, uint8_t *out, uint64_t pos, uint64_t outLen) {
-    uint64_t ip = 0, op = pos;
-    const uint64_t oend = pos + outLen;
-    while (ip < srcLen) {
-        uint32_t token = src[ip++];
+namespace {
+
+// Bytes of a slice whose value may depend on the previous slice's output
+// while that output is not final yet (one bit per byte of the slice).
+struct Taint {
+    std::vector<uint64_t> bits;
+    int64_t last = -1;  // last tainted byte, relative to the slice start
+    explicit Taint(uint64_t n) : bits((n + 63) / 64, 0) {}
+    bool get(uint64_t i) const { return (bits[i >> 6] >> (i & 63)) & 1u; }
+    void set(uint64_t i) {
+        bits[i >> 6] |= uint64_t(1) << (i & 63);
+        last = int64_t(i);
+    }
+};
+
+// Decodes one LZ4 block into out[pos, pos+outLen); matches may reach back
+// into out[0, pos) (the previous slices of the stream). Every bound is
+// checked.
+//  - Wide: copies run 16 bytes at a time where the slice has room, which may
+//    write a few bytes past the current position that later literals or
+//    matches overwrite before anything reads them.
+//  - taint != null: the bytes before pos are not final (another thread is
+//    still writing them). They are not read; what they would feed is written
+//    as 0 and marked tainted, transitively.
+//  - stop >= 0: return after the token that writes byte pos+stop (the caller
+//    re-decodes just the tainted prefix once the dictionary is final).
+bool decode_slice(const uint8_t *src, uint64_t srcLen, uint8_t *out, uint64_t pos, uint64_t outLen, bool wide,
+                  Taint *taint, int64_t stop) {
+    const uint8_t *ip = src;
+    const uint8_t *const iend = src + srcLen;
+    uint8_t *const ostart = out + pos;
+    uint8_t *op = ostart;
+    uint8_t *const oend = op + outLen;
+    while (ip < iend) {
+        if (stop >= 0 && op - ostart > stop) return true;
+        const uint32_t token = *ip++;
         uint64_t lit = token >> 4;
         if (lit == 15) {
             uint8_t b;
             do {
-                if (ip >= srcLen) return false;
-                b = src[ip++];
+                if (ip >= iend) return false;
+                b = *ip++;
                 lit += b;
             } while (b == 255);
         }
-        if (lit > srcLen - ip || lit > oend - op) return false;
-        std::memcpy(out + op, src + ip, lit);
+        if (lit > uint64_t(iend - ip) || lit > uint64_t(oend - op)) return false;
+        if (wide && uint64_t(iend - ip) >= lit + 16 && uint64_t(oend - op) >= lit + 16) {
+            for (uint64_t k = 0; k < lit; k += 16) std::memcpy(op + k, ip + k, 16);
+        } else {
+            std::memcpy(op, ip, lit);
+        }
         ip += lit;
         op += lit;
-        if (op == oend) return ip == srcLen;
-        if (srcLen - ip < 2) return false;
-        uint64_t off = uint64_t(src[ip]) | (uint64_t(src[ip + 1]) << 8);
+        if (op == oend) return ip == iend;
+        if (iend - ip < 2) return false;
+        const uint64_t off = uint64_t(ip[0]) | (uint64_t(ip[1]) << 8);
         ip += 2;
-        if (off == 0 || off > op) return false;
+        if (off == 0 || off > uint64_t(op - out)) return false;
         uint64_t ml = token & 15u;
         if (ml == 15) {
             uint8_t b;
             do {
-                if (ip >= srcLen) return false;
-                b = src[ip++];
+                if (ip >= iend) return false;
+                b = *ip++;
                 ml += b;
             } while (b == 255);
         }
         ml += kMinMatch;
-        if (ml > oend - op) return false;
-        uint8_t *d = out + op;
-        const uint8_t *s = d - off;
-        if (off >= ml) {
-            std::memcpy(d, s, ml);
+        if (ml > uint64_t(oend - op)) return false;
+        const uint8_t *s = op - off;
+        if (taint && (s < ostart || int64_t(s - ostart) <= taint->last)) {
+            // the source may lie in, or depend on, the unfinished dictionary
+            for (uint64_t k = 0; k < ml; ++k) {
+                const uint64_t d = uint64_t(op - ostart) + k;
+                if (s + k < ostart) {
+                    op[k] = 0;
+                    taint->set(d);
+                } else {
+                    op[k] = s[k];
+                    if (taint->get(uint64_t(s + k - ostart))) taint->set(d);
+                }
+            }
+            op += ml;
+            continue;
+        }
+        const bool room = wide && uint64_t(oend - op) >= ml + 16;
+        if (room && off >= 16) {
+            // each chunk reads bytes at least one chunk behind the ones it writes
+            for (uint64_t k = 0; k < ml; k += 16) std::memcpy(op + k, s + k, 16);
+        } else if (room && off >= 8) {
+            for (uint64_t k = 0; k < ml; k += 8) std::memcpy(op + k, s + k, 8);
+        } else if (ml <= 32) {
+            for (uint64_t k = 0; k < ml; ++k) op[k] = s[k];
+        } else if (off >= ml) {
+            std::memcpy(op, s, ml);
         } else {
-            for (uint64_t k = 0; k < ml; ++k) d[k] = s[k];
+            // repeating pattern of period off: copy it with doubling strides,
+            // every source range ending at or before its destination
+            uint64_t done = 0, span = off;
+            while (done < ml) {
+                const uint64_t n = ml - done < span ? ml - done : span;
+                std::memcpy(op + done, op + done - span, n);
+                done += n;
+                if (span <= done) span *= 2;
+            }
         }
         op += ml;
     }
-    return op == oend;
+    return stop >= 0 ? true : op == oend;
 }
 
+struct SliceRef {
+    uint64_t src, size, pos, len;
+};
+
+}  // namespace
+
+bool lz4_decode_block(const uint8_t *src, uint64_t srcLen, uint8_t *out, uint64_t pos, uint64_t outLen) {
+    return decode_slice(src, srcLen, out, pos, outLen, true, nullptr, -1);
+}
+
+// Slices decode in parallel: each slice first decodes without its dictionary
+// (the 64 KiB before it), marking what depends on it; then, in stream order,
+// only the marked prefix of each slice is decoded again against the final
+// dictionary. Octree slices depend on their dictionary for a short prefix only.
 int oct_decode(const uint8_t *data, uint64_t size, svo_octree &h) {
     if (size < 20) return fail(SVO_ERR_FORMAT, ".oct: truncated header");
     uint64_t n;
@@ -220,22 +314,54 @@ int oct_decode(const uint8_t *data, uint64_t size, svo_octree &h) {
     // every slice needs at least its 8-byte size field
     uint64_t slices = (n * 4 + kSlice - 1) / kSlice;
     if (slices * 8 > size - 20) return fail(SVO_ERR_FORMAT, ".oct: truncated (slice headers missing)");
-    try {
-        h.words.assign(n, 0);
-    } catch (...) {
-        return fail(SVO_ERR_NOMEM, ".oct: cannot allocate word array");
-    }
-    uint8_t *out = reinterpret_cast<uint8_t *>(h.words.data());
-    uint64_t bytes = n * 4, p = 20;
+    const uint64_t bytes = n * 4;
+    std::vector<SliceRef> refs;
+    uint64_t p = 20;
     for (uint64_t off = 0; off < bytes; off += kSlice) {
         uint64_t cs;
         if (size - p < 8) return fail(SVO_ERR_FORMAT, ".oct: truncated slice header");
         std::memcpy(&cs, data + p, 8);
         p += 8;
         if (cs > size - p) return fail(SVO_ERR_FORMAT, ".oct: truncated slice");
-        uint64_t os = bytes - off < kSlice ? bytes - off : kSlice;
-        if (!lz4_decode_block(data + p, cs, out, off, os)) return fail(SVO_ERR_FORMAT, ".oct: corrupt LZ4 slice");
+        refs.push_back({p, cs, off, bytes - off < kSlice ? bytes - off : kSlice});
         p += cs;
+    }
+    try {
+        h.words.assign(n, 0);
+    } catch (...) {
+        return fail(SVO_ERR_NOMEM, ".oct: cannot allocate word array");
+    }
+    uint8_t *out = reinterpret_cast<uint8_t *>(h.words.data());
+    const size_t ns = refs.size();
+    std::vector<std::unique_ptr<Taint>> taint(ns);
+    std::vector<char> ok(ns, 0);
+    std::atomic<size_t> next{0};
+    auto worker = [&]() {
+        for (size_t k; (k = next.fetch_add(1)) < ns;) {
+            const SliceRef &r = refs[k];
+            if (k) taint[k].reset(new Taint(r.len));
+            ok[k] = decode_slice(data + r.src, r.size, out, r.pos, r.len, true, taint[k].get(), -1);
+        }
+    };
+    const size_t nt = std::min<size_t>(ns, std::max(1u, std::thread::hardware_concurrency()));
+    if (nt <= 1) {
+        worker();
+    } else {
+        std::vector<std::thread> pool;
+        try {
+            for (size_t t = 0; t < nt; ++t) pool.emplace_back(worker);
+        } catch (...) {
+        }
+        worker();
+        for (std::thread &t : pool) t.join();
+    }
+    for (size_t k = 0; k < ns; ++k)
+        if (!ok[k]) return fail(SVO_ERR_FORMAT, ".oct: corrupt LZ4 slice");
+    for (size_t k = 1; k < ns; ++k) {
+        if (taint[k]->last < 0) continue;
+        const SliceRef &r = refs[k];
+        if (!decode_slice(data + r.src, r.size, out, r.pos, r.len, false, nullptr, taint[k]->last))
+            return fail(SVO_ERR_FORMAT, ".oct: corrupt LZ4 slice");
     }
     return SVO_OK;
 }
@@ -243,7 +369,10 @@ int oct_decode(const uint8_t *data, uint64_t size, svo_octree &h) {
 void oct_encode(const svo_octree &h, std::vector<uint8_t> &out) {
     uint64_t n = h.words.size(), bytes = n * 4;
     out.clear();
-    out.reserve(20 + bytes / 2 + 1024);
+    uint64_t cap = 20;
+    for (uint64_t off = 0; off < bytes; off += kSlice)
+        cap += 8 + Lz4StreamEncoder::bound(uint32_t(bytes - off < kSlice ? bytes - off : kSlice));
+    out.reserve(cap);
     out.resize(20);
     std::memcpy(out.data(), h.center, 12);
     std::memcpy(out.data() + 12, &n, 8);
@@ -252,9 +381,9 @@ void oct_encode(const svo_octree &h, std::vector<uint8_t> &out) {
     for (uint64_t off = 0; off < bytes; off += kSlice) {
         uint32_t len = uint32_t(bytes - off < kSlice ? bytes - off : kSlice);
         size_t at = out.size();
-        out.resize(at + 8);
-        enc.encode(src, off, len, out);
-        uint64_t cs = out.size() - at - 8;
+        out.resize(at + 8 + Lz4StreamEncoder::bound(len));
+        uint64_t cs = enc.encode(src, off, len, out.data() + at + 8);
+        out.resize(at + 8 + cs);
         std::memcpy(out.data() + at, &cs, 8);
     }
 }

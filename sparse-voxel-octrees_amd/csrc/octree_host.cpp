@@ -7,25 +7,39 @@
 // bit 17 = far pointer (offset = bits31..18 << 32 | next word), bit 16 = two
 // words per child slot, bits 15..8 valid mask, bits 7..0 non-leaf mask. The
 // k-th valid child (ascending bit) sits at index + offset + k*(1+large).
+#include <algorithm>
+#include <atomic>
+#include <thread>
 #include <vector>
 
 #include "svo_internal.hpp"
 
 namespace svo {
 
-int octree_validate(const uint32_t *w, uint64_t n) {
-    if (n == 0) return -1;
-    struct Item {
-        uint64_t idx, off;
-        int level;
-    };
+namespace {
+
+struct Item {
+    uint64_t idx, off;
+    int level;
+};
+
+// Checks the subtree below `it` (descriptor at it.idx, child block at
+// it.idx + it.off). Returns the depth it reaches (0 when it has no children
+// of its own) or -1 if a pointer leaves the array or the tree is deeper than
+// the traversal can follow. With `frontier`, nodes at level `split` are
+// handed out instead of descended into.
+int validate_from(const uint32_t *w, uint64_t n, Item root, int split, std::vector<Item> *frontier) {
     std::vector<Item> stack;
-    stack.push_back({0, w[0] >> 18, 0});
-    int depth = 1;
+    stack.push_back(root);
+    int depth = 0;
     while (!stack.empty()) {
         Item it = stack.back();
         stack.pop_back();
         if (it.level >= 23) return -1;  // MaxScale bounds the traversal depth
+        if (frontier && it.level == split) {
+            frontier->push_back(it);
+            continue;
+        }
         uint32_t d = w[it.idx];
         uint32_t valid = (d >> 8) & 0xFFu, nonleaf = d & 0xFFu;
         uint64_t per = 1 + ((d >> 16) & 1u);
@@ -51,6 +65,45 @@ int octree_validate(const uint32_t *w, uint64_t n) {
         }
     }
     return depth;
+}
+
+}  // namespace
+
+// Every child pointer reachable from the root stays inside the array and the
+// depth is bounded; returns the octree depth (levels of descriptors + 1) or -1.
+// Subtrees below level 3 are checked on all host threads.
+int octree_validate(const uint32_t *w, uint64_t n) {
+    if (n == 0) return -1;
+    std::vector<Item> frontier;
+    const int split = 3;
+    int depth = validate_from(w, n, {0, w[0] >> 18, 0}, split, &frontier);
+    if (depth < 0) return -1;
+    depth = std::max(depth, 1);
+    if (frontier.empty()) return depth;
+    std::atomic<size_t> next{0};
+    std::atomic<int> worst{depth};
+    std::atomic<bool> bad{false};
+    auto worker = [&]() {
+        for (size_t i; !bad.load() && (i = next.fetch_add(1)) < frontier.size();) {
+            int d = validate_from(w, n, frontier[i], split, nullptr);
+            if (d < 0) {
+                bad.store(true);
+                return;
+            }
+            int cur = worst.load();
+            while (d > cur && !worst.compare_exchange_weak(cur, d)) {
+            }
+        }
+    };
+    const size_t nt = std::min<size_t>(frontier.size(), std::max(1u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    try {
+        for (size_t t = 1; t < nt; ++t) pool.emplace_back(worker);
+    } catch (...) {
+    }
+    worker();
+    for (std::thread &t : pool) t.join();
+    return bad.load() ? -1 : worst.load();
 }
 
 }  // namespace svo

@@ -148,6 +148,23 @@ int svo_encode_words(const uint32_t *words, uint64_t count, const float center[3
 
 void svo_free_buffer(void *p) { std::free(p); }
 
+int svo_decode_words(const void *data, uint64_t size, uint32_t **words, uint64_t *count, float center[3]) {
+    clear_error();
+    if (!data || !words || !count || !center) return fail(SVO_ERR_INVALID, "null argument");
+    *words = nullptr;
+    *count = 0;
+    svo_octree tmp;
+    int r = oct_decode(static_cast<const uint8_t *>(data), size, tmp);
+    if (r) return r;
+    uint32_t *p = static_cast<uint32_t *>(std::malloc(tmp.words.size() * sizeof(uint32_t)));
+    if (!p) return fail(SVO_ERR_NOMEM, "out of host memory");
+    std::memcpy(p, tmp.words.data(), tmp.words.size() * sizeof(uint32_t));
+    std::memcpy(center, tmp.center, sizeof(tmp.center));
+    *words = p;
+    *count = tmp.words.size();
+    return SVO_OK;
+}
+
 int svo_save(const svo_octree *h, const char *path) {
     clear_error();
     if (!h || !path) return fail(SVO_ERR_INVALID, "null argument");

@@ -29,7 +29,10 @@ void clear_error();
 class Lz4StreamEncoder {
 public:
     Lz4StreamEncoder();
-    void encode(const uint8_t *src, uint64_t pos, uint32_t len, std::vector<uint8_t> &out);
+    // Compresses src[pos, pos+len) (the slice that follows the previous call's)
+    // into dst, which holds at least bound(len) bytes; returns the bytes written.
+    uint64_t encode(const uint8_t *src, uint64_t pos, uint32_t len, uint8_t *dst);
+    static uint64_t bound(uint32_t len);
 
 private:
     uint32_t table_[4096];

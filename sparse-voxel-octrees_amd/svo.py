@@ -32,7 +32,7 @@ EXPORTS = [
     "svo_build_from_sdf", "svo_sdf_volume", "svo_serve_replay",
     "svo_render_frame", "svo_render_frame_device", "svo_frame_depth_count",
     "svo_build_from_terrain", "svo_terrain_volume", "svo_terrain_heights",
-    "svo_render_shadow", "svo_render_shadow_device",
+    "svo_render_shadow", "svo_render_shadow_device", "svo_decode_words",
 ]
 
 
@@ -179,6 +179,7 @@ def _load():
         "svo_build_from_terrain": [ctypes.c_int, ctypes.c_uint32, P, P],
         "svo_terrain_volume": [ctypes.c_int, ctypes.c_uint32, P],
         "svo_terrain_heights": [ctypes.c_int, ctypes.c_uint32, P, P],
+        "svo_decode_words": [P, U64, P, P, P],
         "svo_render_shadow": [P, P, ctypes.c_int, ctypes.c_int, P, F, P, P, P, P, P],
         "svo_render_shadow_device": [P, P, ctypes.c_int, ctypes.c_int, P, F, P, P, P, P, P, P],
     }
@@ -200,6 +201,19 @@ def _check(code):
 
 def _p(a):
     return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+def decode_words(data):
+    """(center, words) of an .oct image, without tree validation."""
+    buf = np.frombuffer(data, dtype=np.uint8)
+    p = ctypes.POINTER(ctypes.c_uint32)()
+    n = ctypes.c_uint64()
+    c = np.zeros(3, np.float32)
+    _check(lib.svo_decode_words(_p(buf), len(buf), ctypes.byref(p), ctypes.byref(n), _p(c)))
+    try:
+        return c, np.ctypeslib.as_array(p, (n.value,)).copy() if n.value else np.zeros(0, np.uint32)
+    finally:
+        lib.svo_free_buffer(p)
 
 
 def encode_words(words, center):
@@ -310,6 +324,8 @@ class VoxelOctree:
         self.close()
 
     def close(self):
+        if lib is None:  # interpreter shutdown
+            return
         if getattr(self, "_h", None) and self._h.value:
             lib.svo_free(self._h)
             self._h = ctypes.c_void_p()
@@ -457,6 +473,8 @@ class Renderer:
         self.close()
 
     def close(self):
+        if lib is None:  # interpreter shutdown
+            return
         if getattr(self, "_r", None) and self._r.value:
             lib.svo_renderer_free(self._r)
             self._r = ctypes.c_void_p()

@@ -70,6 +70,29 @@ def test_two_slice_stream_matches_reference_encoder(svo, golden):
     assert hashlib.md5(enc).hexdigest() == lz["md5"]
     c, w = orc.decode_oct_bytes(enc)
     assert np.array_equal(w, words)
+    # the library's reader decodes the slices in parallel
+    c2, w2 = svo.svo.decode_words(enc)
+    assert np.array_equal(w2, words) and np.array_equal(c2, c)
+
+
+def test_parallel_decode_of_dictionary_chains(svo):
+    """Slices decoded before their dictionary is final: (a) a periodic stream
+    whose every match in slices 1..3 chains back into the previous slice (the
+    whole slice is re-decoded); (b) noise with periodic islands that straddle
+    slice boundaries (short chains)."""
+    n = 3 * (1 << 24) + 777  # four slices
+    i = np.arange(n, dtype=np.uint64)
+    per = ((i % np.uint64(3)) * np.uint64(0x9E3779B1) + np.uint64(12345)) & np.uint64(0xFFFFFFFF)
+    per = per.astype(np.uint32)
+    rng = np.random.default_rng(3)
+    noise = rng.integers(0, 1 << 32, n, dtype=np.uint32)
+    islands = ((i >> np.uint64(10)) % np.uint64(7)) == 0
+    for words in (per, np.where(islands, per, noise).astype(np.uint32)):
+        enc = svo.svo.encode_words(words, (0.5, 0.5, 0.5))
+        c, w = svo.svo.decode_words(enc)
+        assert np.array_equal(w, words)
+        _, wo = orc.decode_oct_bytes(enc)
+        assert np.array_equal(wo, words)
 
 
 def test_load_errors_are_reported(svo, golden, tmp_path):
