@@ -84,6 +84,14 @@ struct Arena {
                 q = nullptr;
             }
     }
+    void clear() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (void *&p : ptrs)
+            if (p) {
+                (void)hipFree(p);
+                p = nullptr;
+            }
+    }
 };
 
 // Parameters the reference derives on the host (VoxelData::init,
@@ -214,6 +222,137 @@ __global__ void triangles_kernel(const float *__restrict__ v9, const int32_t *__
                 for (int s = 0; s < 3; ++s) nrm[9 * t + 3 * q + s] = n[s];
         }
         b = cc;
+    }
+}
+
+// ---- PLY ingest on the device (binary little-endian bodies) ----
+// A vertex record has a fixed size; a face record too once every face has the
+// first face's vertex count (checked here, else the host parser takes over).
+// Values convert as plyfile.c stores them: the file's type -> double -> float
+// for vertex properties, -> int for face indices (src/PlyLoader.cpp:81-224).
+
+struct PlyVertexLayout {
+    int64_t stride;
+    int off[9];
+    int type[9];  // PType; 0 = property absent (default value)
+};
+
+struct PlyFaceLayout {
+    int64_t stride;
+    int countOff, countType;
+    int idxOff, idxType;
+    int c0;  // vertices per face
+};
+
+__device__ __forceinline__ double ply_value(const uint8_t *p, int type) {
+    uint64_t u = 0;
+    const int n = type == P_CHAR || type == P_UCHAR ? 1 : type == P_SHORT || type == P_USHORT ? 2 : type == P_DOUBLE ? 8 : 4;
+    for (int k = 0; k < n; ++k) u |= uint64_t(p[k]) << (8 * k);
+    switch (type) {
+        case P_CHAR: return double(int8_t(uint8_t(u)));
+        case P_UCHAR: return double(uint8_t(u));
+        case P_SHORT: return double(int16_t(uint16_t(u)));
+        case P_USHORT: return double(uint16_t(u));
+        case P_INT: return double(int32_t(uint32_t(u)));
+        case P_UINT: return double(uint32_t(u));
+        case P_FLOAT: return double(__uint_as_float(uint32_t(u)));
+        default: return __longlong_as_double(int64_t(u));
+    }
+}
+
+__global__ void ply_vertices_kernel(const uint8_t *__restrict__ body, int64_t nv, PlyVertexLayout L,
+                                    float *__restrict__ v9) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= nv) return;
+    const uint8_t *rec = body + i * L.stride;
+#pragma unroll
+    for (int t = 0; t < 9; ++t)
+        v9[9 * i + t] = L.type[t] ? float(ply_value(rec + L.off[t], L.type[t])) : (t < 6 ? 0.0f : 255.0f);
+}
+
+// flags: bit 0 a face with another vertex count, bit 1 an index outside [0, nv)
+__global__ void ply_faces_kernel(const uint8_t *__restrict__ body, int64_t nf, PlyFaceLayout L, int64_t nv,
+                                 int32_t *__restrict__ fcount, int64_t *__restrict__ fstart, int64_t *__restrict__ tstart,
+                                 int32_t *__restrict__ fidx, uint32_t *__restrict__ flags) {
+    const int64_t f = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (f >= nf) return;
+    const uint8_t *rec = body + f * L.stride;
+    const double c = ply_value(rec + L.countOff, L.countType);
+    if (!(c == double(L.c0))) {
+        atomicOr(flags, 1u);
+        return;
+    }
+    const int isz = L.idxType == P_CHAR || L.idxType == P_UCHAR ? 1 : L.idxType == P_SHORT || L.idxType == P_USHORT ? 2
+                    : L.idxType == P_DOUBLE ? 8 : 4;
+    for (int k = 0; k < L.c0; ++k) {
+        const double v = ply_value(rec + L.idxOff + k * isz, L.idxType);
+        const int32_t e = (v >= -2147483648.0 && v < 2147483648.0) ? int32_t(v) : INT32_MIN;
+        if (e < 0 || int64_t(e) >= nv) atomicOr(flags, 2u);
+        fidx[f * L.c0 + k] = e;
+    }
+    fcount[f] = L.c0;
+    fstart[f] = f * L.c0;
+    tstart[f] = f * (L.c0 - 2);
+}
+
+// Vertex position bounds as PlyLoader::readVertices scans them: per axis the
+// smallest / largest value, ties going to the earliest vertex (std::min/max
+// keep the first of equal values, which decides the sign of a zero bound);
+// flags bit 2 marks a non-finite position. One (value, index) pair per block
+// and bound; the host folds the blocks.
+struct BoundPart {
+    float v[6];
+    int64_t i[6];
+};
+
+__device__ __forceinline__ bool bound_better(float a, int64_t ia, float b, int64_t ib, bool isMin) {
+    // does (a, ia) replace (b, ib)?
+    return isMin ? (a < b || (!(b < a) && ia < ib)) : (b < a || (!(a < b) && ia < ib));
+}
+
+__global__ void __launch_bounds__(256) ply_bounds_kernel(const float *__restrict__ v9, int64_t nv,
+                                                         BoundPart *__restrict__ parts, uint32_t *__restrict__ flags) {
+    __shared__ float sv[6][256];
+    __shared__ int64_t si[6][256];
+    float bv[6] = {1e30f, 1e30f, 1e30f, -1e30f, -1e30f, -1e30f};
+    int64_t bi[6] = {INT64_MAX, INT64_MAX, INT64_MAX, INT64_MAX, INT64_MAX, INT64_MAX};
+    bool bad = false;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < nv; i += int64_t(gridDim.x) * blockDim.x) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const float x = v9[9 * i + k];
+            if (!isfinite(x)) bad = true;
+            if (bound_better(x, i, bv[k], bi[k], true)) { bv[k] = x; bi[k] = i; }
+            if (bound_better(x, i, bv[3 + k], bi[3 + k], false)) { bv[3 + k] = x; bi[3 + k] = i; }
+        }
+    }
+    if (bad) atomicOr(flags, 4u);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        sv[k][threadIdx.x] = bv[k];
+        si[k][threadIdx.x] = bi[k];
+    }
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (int(threadIdx.x) < w) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) {
+                const float a = sv[k][threadIdx.x + w];
+                const int64_t ia = si[k][threadIdx.x + w];
+                if (bound_better(a, ia, sv[k][threadIdx.x], si[k][threadIdx.x], k < 3)) {
+                    sv[k][threadIdx.x] = a;
+                    si[k][threadIdx.x] = ia;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            parts[blockIdx.x].v[k] = sv[k][0];
+            parts[blockIdx.x].i[k] = si[k][0];
+        }
     }
 }
 
@@ -1360,18 +1499,12 @@ int check_blocks(const Geometry &g, uint64_t &nBlocks) {
 // entry points
 // ---------------------------------------------------------------------------
 
-int build_mesh(const float *verts, int64_t nverts, int has_normals, const int32_t *face_counts,
-               const int32_t *face_idx, int64_t nfaces, int resolution, const svo_build_opts *opts, svo_octree &out) {
-    if (resolution < 8 || resolution > 8192) return fail(SVO_ERR_INVALID, "resolution must be in [8, 8192]");
-    // PlyLoader::readVertices bounds and rescaleVertices (src/PlyLoader.cpp:122-181)
-    float lower[3] = {1e30f, 1e30f, 1e30f}, upper[3] = {-1e30f, -1e30f, -1e30f};
-    for (int64_t i = 0; i < nverts; ++i)
-        for (int k = 0; k < 3; ++k) {
-            const float v = verts[9 * i + k];
-            if (!std::isfinite(v)) return fail(SVO_ERR_UNSUPPORTED, "non-finite vertex position");
-            lower[k] = fmin_ref(lower[k], v);
-            upper[k] = fmax_ref(upper[k], v);
-        }
+// The mesh build once the vertices (9 floats each: position, normal, colour;
+// not yet rescaled), their bounds and the face table (sizes, first index,
+// first triangle, indices) are on the device in `ar`.
+static int mesh_core(Arena &ar, float *dv, int64_t nverts, int has_normals, int32_t *dfc, int64_t *dfs, int64_t *dts,
+                     int32_t *dfi, int64_t nfaces, int64_t T, const float lower[3], const float upper[3],
+                     int resolution, const svo_build_opts *opts, svo_octree &out) {
     const float diff[3] = {upper[0] - lower[0], upper[1] - lower[1], upper[2] - lower[2]};
     int largest = 2;
     if (diff[0] > diff[1] && diff[0] > diff[2]) largest = 0;
@@ -1383,20 +1516,6 @@ int build_mesh(const float *verts, int64_t nverts, int has_normals, const int32_
         su[k] = upper[k] * factor;
         sl[k] = lower[k] * factor;
     }
-    // face table
-    std::vector<int64_t> fstart(size_t(nfaces) + 1), tstart(size_t(nfaces) + 1);
-    int64_t at = 0, T = 0;
-    for (int64_t f = 0; f < nfaces; ++f) {
-        if (face_counts[f] < 0) return fail(SVO_ERR_FORMAT, "negative face size");
-        fstart[f] = at;
-        tstart[f] = T;
-        at += face_counts[f];
-        if (face_counts[f] > 2) T += face_counts[f] - 2;
-    }
-    for (int64_t i = 0; i < at; ++i)
-        if (face_idx[i] < 0 || face_idx[i] >= nverts) return fail(SVO_ERR_FORMAT, "face references a missing vertex");
-    if (T >= (int64_t(1) << 32)) return fail(SVO_ERR_LIMIT, "too many triangles");
-
     Geometry g{};
     // PlyLoader::suggestedDimensions, src/PlyLoader.cpp:499-504
     g.dataW = cvt_i32((su[0] - sl[0]) * float(resolution - 2)) + 2;
@@ -1433,24 +1552,10 @@ int build_mesh(const float *verts, int64_t nverts, int has_normals, const int32_
     mp.hb[1] = float(g.sbH) / float(g.scale);
     mp.hb[2] = float(g.sbD) / float(g.scale);
 
-    Arena ar;
-    HIP_TRY(hipStreamCreate(&ar.stream));
     hipStream_t st = ar.stream;
-
-    float *dv, *pos, *nrm, *col;
-    int32_t *dfc, *dfi;
-    int64_t *dfs, *dts;
-    if ((r = ar.alloc(dv, uint64_t(nverts) * 9)) || (r = ar.alloc(dfc, nfaces)) || (r = ar.alloc(dfi, at)) ||
-        (r = ar.alloc(dfs, nfaces)) || (r = ar.alloc(dts, nfaces)) || (r = ar.alloc(pos, uint64_t(T) * 9)) ||
-        (r = ar.alloc(nrm, uint64_t(T) * 9)) || (r = ar.alloc(col, uint64_t(T) * 9)))
+    float *pos, *nrm, *col;
+    if ((r = ar.alloc(pos, uint64_t(T) * 9)) || (r = ar.alloc(nrm, uint64_t(T) * 9)) || (r = ar.alloc(col, uint64_t(T) * 9)))
         return r;
-    HIP_TRY(hipMemcpyAsync(dv, verts, uint64_t(nverts) * 36, hipMemcpyHostToDevice, st));
-    if (nfaces) {
-        HIP_TRY(hipMemcpyAsync(dfc, face_counts, uint64_t(nfaces) * 4, hipMemcpyHostToDevice, st));
-        HIP_TRY(hipMemcpyAsync(dfs, fstart.data(), uint64_t(nfaces) * 8, hipMemcpyHostToDevice, st));
-        HIP_TRY(hipMemcpyAsync(dts, tstart.data(), uint64_t(nfaces) * 8, hipMemcpyHostToDevice, st));
-    }
-    if (at) HIP_TRY(hipMemcpyAsync(dfi, face_idx, uint64_t(at) * 4, hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(rescale_kernel, dim3(grid_for(nverts)), dim3(kThreads), 0, st, dv, nverts, lower[0], lower[1],
                        lower[2], factor);
     if (nfaces)
@@ -1573,6 +1678,191 @@ int build_mesh(const float *verts, int64_t nverts, int has_normals, const int32_
     ar.release(col);
     const bool multi = g.cacheLog < g.depth;
     return emit_tree(ar, g, lkey, lword, L, multi ? blockFlags : nullptr, st, out);
+}
+
+int build_mesh(const float *verts, int64_t nverts, int has_normals, const int32_t *face_counts,
+               const int32_t *face_idx, int64_t nfaces, int resolution, const svo_build_opts *opts, svo_octree &out) {
+    if (resolution < 8 || resolution > 8192) return fail(SVO_ERR_INVALID, "resolution must be in [8, 8192]");
+    // PlyLoader::readVertices bounds and rescaleVertices (src/PlyLoader.cpp:122-181)
+    float lower[3] = {1e30f, 1e30f, 1e30f}, upper[3] = {-1e30f, -1e30f, -1e30f};
+    for (int64_t i = 0; i < nverts; ++i)
+        for (int k = 0; k < 3; ++k) {
+            const float v = verts[9 * i + k];
+            if (!std::isfinite(v)) return fail(SVO_ERR_UNSUPPORTED, "non-finite vertex position");
+            lower[k] = fmin_ref(lower[k], v);
+            upper[k] = fmax_ref(upper[k], v);
+        }
+    // face table
+    std::vector<int64_t> fstart(size_t(nfaces) + 1), tstart(size_t(nfaces) + 1);
+    int64_t at = 0, T = 0;
+    for (int64_t f = 0; f < nfaces; ++f) {
+        if (face_counts[f] < 0) return fail(SVO_ERR_FORMAT, "negative face size");
+        fstart[f] = at;
+        tstart[f] = T;
+        at += face_counts[f];
+        if (face_counts[f] > 2) T += face_counts[f] - 2;
+    }
+    for (int64_t i = 0; i < at; ++i)
+        if (face_idx[i] < 0 || face_idx[i] >= nverts) return fail(SVO_ERR_FORMAT, "face references a missing vertex");
+    if (T >= (int64_t(1) << 32)) return fail(SVO_ERR_LIMIT, "too many triangles");
+
+    Arena ar;
+    HIP_TRY(hipStreamCreate(&ar.stream));
+    hipStream_t st = ar.stream;
+    float *dv;
+    int32_t *dfc, *dfi;
+    int64_t *dfs, *dts;
+    int r;
+    if ((r = ar.alloc(dv, uint64_t(nverts) * 9)) || (r = ar.alloc(dfc, nfaces)) || (r = ar.alloc(dfi, at)) ||
+        (r = ar.alloc(dfs, nfaces)) || (r = ar.alloc(dts, nfaces)))
+        return r;
+    HIP_TRY(hipMemcpyAsync(dv, verts, uint64_t(nverts) * 36, hipMemcpyHostToDevice, st));
+    if (nfaces) {
+        HIP_TRY(hipMemcpyAsync(dfc, face_counts, uint64_t(nfaces) * 4, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(dfs, fstart.data(), uint64_t(nfaces) * 8, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(dts, tstart.data(), uint64_t(nfaces) * 8, hipMemcpyHostToDevice, st));
+    }
+    if (at) HIP_TRY(hipMemcpyAsync(dfi, face_idx, uint64_t(at) * 4, hipMemcpyHostToDevice, st));
+    // the host vectors must outlive the copies
+    HIP_TRY(hipStreamSynchronize(st));
+    return mesh_core(ar, dv, nverts, has_normals, dfc, dfs, dts, dfi, nfaces, T, lower, upper, resolution, opts, out);
+}
+
+// PLY path: binary little-endian files whose vertex records have a fixed
+// size and whose faces all have the first face's vertex count are decoded on
+// the device straight from the file bytes; other files (ascii, mixed face
+// sizes, list properties on vertices) go through the host parser.
+int build_ply(const char *path, int resolution, const svo_build_opts *opts, svo_octree &out) {
+    if (resolution < 8 || resolution > 8192) return fail(SVO_ERR_INVALID, "resolution must be in [8, 8192]");
+    PlyLayout L;
+    int r = ply_open(path, L);
+    if (r) return r;
+    auto host_path = [&]() {
+        std::vector<float> verts;
+        std::vector<int32_t> counts, idx;
+        int hn = 0;
+        int rr = ply_parse_host(L, verts, hn, counts, idx);
+        if (rr) return rr;
+        return build_mesh(verts.data(), int64_t(verts.size() / 9), hn, counts.data(), idx.data(),
+                          int64_t(counts.size()), resolution, opts, out);
+    };
+    if (!L.binary) return host_path();
+    // element offsets: every element before the face element needs fixed-size records
+    const PlyElem &ve = L.elems[size_t(L.vertex)], &fe = L.elems[size_t(L.face)];
+    if (L.face < L.vertex) return host_path();
+    uint64_t at = L.body, vbody = 0, fbody = 0;
+    PlyVertexLayout vl{};
+    PlyFaceLayout fl{};
+    for (int e = 0; e <= L.face; ++e) {
+        const PlyElem &el = L.elems[size_t(e)];
+        int64_t stride = 0;
+        bool fixed = true;
+        for (const PlyProp &pr : el.props) {
+            if (pr.list) {
+                if (e != L.face || pr.name != "vertex_indices" || fl.idxType) {
+                    fixed = false;
+                    break;
+                }
+                fl.countOff = int(stride);
+                fl.countType = pr.count_type;
+                fl.idxType = pr.type;
+                if (at + uint64_t(stride) + ply_type_size(pr.count_type) > L.buf.size()) return host_path();
+                double c0 = 0;
+                {
+                    const uint8_t *q = L.buf.data() + at + stride;
+                    uint64_t u = 0;
+                    for (int k = 0; k < ply_type_size(pr.count_type); ++k) u |= uint64_t(q[k]) << (8 * k);
+                    switch (pr.count_type) {
+                        case P_CHAR: c0 = int8_t(u); break;
+                        case P_UCHAR: c0 = uint8_t(u); break;
+                        case P_SHORT: c0 = int16_t(u); break;
+                        case P_USHORT: c0 = uint16_t(u); break;
+                        case P_INT: c0 = int32_t(u); break;
+                        case P_UINT: c0 = uint32_t(u); break;
+                        default: return host_path();
+                    }
+                }
+                if (c0 < 3 || c0 > 64) return host_path();
+                fl.c0 = int(c0);
+                fl.idxOff = int(stride + ply_type_size(pr.count_type));
+                stride += ply_type_size(pr.count_type) + int64_t(fl.c0) * ply_type_size(pr.type);
+            } else {
+                if (e == L.vertex)
+                    for (int t = 0; t < 9; ++t)
+                        if (pr.name == kPlyVertexNames[t]) {
+                            vl.off[t] = int(stride);
+                            vl.type[t] = pr.type;
+                        }
+                stride += ply_type_size(pr.type);
+            }
+        }
+        if (!fixed) return host_path();
+        if (e == L.vertex) {
+            vl.stride = stride;
+            vbody = at;
+        }
+        if (e == L.face) {
+            fl.stride = stride;
+            fbody = at;
+        }
+        if (stride && el.count > int64_t((L.buf.size() - at) / uint64_t(stride))) return host_path();
+        at += uint64_t(el.count) * uint64_t(stride);
+    }
+    const int64_t nv = ve.count, nf = fe.count;
+    if (nf == 0 || !fl.idxType || fl.idxType == P_FLOAT || fl.idxType == P_DOUBLE) return host_path();
+    const int64_t T = nf * (fl.c0 - 2), nidx = nf * fl.c0;
+    if (T >= (int64_t(1) << 32)) return fail(SVO_ERR_LIMIT, "too many triangles");
+
+    Arena ar;
+    HIP_TRY(hipStreamCreate(&ar.stream));
+    hipStream_t st = ar.stream;
+    uint8_t *raw;
+    float *dv;
+    int32_t *dfc, *dfi;
+    int64_t *dfs, *dts;
+    uint32_t *flags;
+    BoundPart *parts;
+    const int nparts = int(std::min<int64_t>(std::max<int64_t>((nv + 255) / 256, 1), 1024));
+    const uint64_t span = at - vbody;  // vertex element through face element
+    if ((r = ar.alloc(raw, span)) || (r = ar.alloc(dv, uint64_t(nv) * 9)) || (r = ar.alloc(dfc, nf)) ||
+        (r = ar.alloc(dfi, nidx)) || (r = ar.alloc(dfs, nf)) || (r = ar.alloc(dts, nf)) || (r = ar.alloc(flags, 1)) ||
+        (r = ar.alloc(parts, nparts)))
+        return r;
+    if ((r = fill32(flags, 1, 0, st))) return r;
+    HIP_TRY(hipMemcpyAsync(raw, L.buf.data() + vbody, span, hipMemcpyHostToDevice, st));
+    if (nv) hipLaunchKernelGGL(ply_vertices_kernel, dim3(grid_for(nv)), dim3(kThreads), 0, st, raw, nv, vl, dv);
+    hipLaunchKernelGGL(ply_faces_kernel, dim3(grid_for(nf)), dim3(kThreads), 0, st, raw + (fbody - vbody), nf, fl, nv, dfc,
+                       dfs, dts, dfi, flags);
+    hipLaunchKernelGGL(ply_bounds_kernel, dim3(nparts), dim3(256), 0, st, dv, nv, parts, flags);
+    HIP_TRY(hipGetLastError());
+    uint32_t fh = 0;
+    std::vector<BoundPart> hp(static_cast<size_t>(nparts));
+    HIP_TRY(hipMemcpyAsync(&fh, flags, 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(hp.data(), parts, sizeof(BoundPart) * size_t(nparts), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (fh & 1u) {
+        ar.clear();  // faces of several sizes
+        return host_path();
+    }
+    if (fh & 2u) return fail(SVO_ERR_FORMAT, "face references a missing vertex");
+    if (fh & 4u) return fail(SVO_ERR_UNSUPPORTED, "non-finite vertex position");
+    float lower[3] = {1e30f, 1e30f, 1e30f}, upper[3] = {-1e30f, -1e30f, -1e30f};
+    int64_t li[3] = {INT64_MAX, INT64_MAX, INT64_MAX}, ui[3] = {INT64_MAX, INT64_MAX, INT64_MAX};
+    for (const BoundPart &bp : hp)
+        for (int k = 0; k < 3; ++k) {
+            if (bp.i[k] != INT64_MAX && (bp.v[k] < lower[k] || (!(lower[k] < bp.v[k]) && bp.i[k] < li[k]))) {
+                lower[k] = bp.v[k];
+                li[k] = bp.i[k];
+            }
+            if (bp.i[3 + k] != INT64_MAX && (upper[k] < bp.v[3 + k] || (!(bp.v[3 + k] < upper[k]) && bp.i[3 + k] < ui[k]))) {
+                upper[k] = bp.v[3 + k];
+                ui[k] = bp.i[3 + k];
+            }
+        }
+    ar.release(raw);
+    ar.release(parts);
+    ar.release(flags);
+    return mesh_core(ar, dv, nv, L.has_normals, dfc, dfs, dts, dfi, nf, T, lower, upper, resolution, opts, out);
 }
 
 int build_grid(const uint32_t *grid, int w, int h, int d, const svo_build_opts *opts, svo_octree &out) {

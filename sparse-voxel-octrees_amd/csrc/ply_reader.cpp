@@ -21,8 +21,6 @@
 namespace svo {
 namespace {
 
-enum PType { P_INVALID = 0, P_CHAR, P_SHORT, P_INT, P_UCHAR, P_USHORT, P_UINT, P_FLOAT, P_DOUBLE };
-
 PType parse_type(const std::string &s) {
     static const char *names[] = {"invalid", "char", "short", "int", "uchar", "ushort", "uint", "float", "double"};
     for (int i = 1; i <= 8; ++i)
@@ -30,22 +28,7 @@ PType parse_type(const std::string &s) {
     return P_INVALID;
 }
 
-int type_size(PType t) {
-    static const int sz[] = {0, 1, 2, 4, 1, 2, 4, 4, 8};
-    return sz[t];
-}
-
-struct Prop {
-    std::string name;
-    bool list = false;
-    PType type = P_INVALID, count_type = P_INVALID;
-};
-
-struct Elem {
-    std::string name;
-    int64_t count = 0;
-    std::vector<Prop> props;
-};
+int type_size(PType t) { return ply_type_size(t); }
 
 // Reads one binary scalar as double (the value plyfile.c's get_binary_item
 // yields before store_item converts it to the internal type).
@@ -71,13 +54,12 @@ bool is_int_type(PType t) { return t != P_FLOAT && t != P_DOUBLE; }
 
 }  // namespace
 
-int read_ply(const char *path_in, std::vector<float> &verts, int &has_normals, std::vector<int32_t> &face_counts,
-             std::vector<int32_t> &face_idx) {
+int ply_open(const char *path_in, PlyLayout &L) {
     std::string path(path_in);
     if (path.size() < 4 || path.compare(path.size() - 4, 4, ".ply") != 0) path += ".ply";
     FILE *fp = std::fopen(path.c_str(), "rb");
     if (!fp) return fail(SVO_ERR_IO, "cannot open PLY " + path);
-    std::vector<uint8_t> buf;
+    std::vector<uint8_t> &buf = L.buf;
     {
         std::fseek(fp, 0, SEEK_END);
         long n = std::ftell(fp);
@@ -113,7 +95,8 @@ int read_ply(const char *path_in, std::vector<float> &verts, int &has_normals, s
     std::string line;
     if (!next_line(line) || words(line) != std::vector<std::string>{"ply"}) return fail(SVO_ERR_FORMAT, "not a PLY file");
     int format = -1;  // 0 ascii, 1 binary LE
-    std::vector<Elem> elems;
+    std::vector<PlyElem> &elems = L.elems;
+    elems.clear();
     bool ended = false;
     while (next_line(line)) {
         std::vector<std::string> w = words(line);
@@ -124,14 +107,14 @@ int read_ply(const char *path_in, std::vector<float> &verts, int &has_normals, s
             else if (w[1] == "binary_big_endian") return fail(SVO_ERR_UNSUPPORTED, "big-endian PLY is not supported");
             else return fail(SVO_ERR_FORMAT, "unknown PLY format " + w[1]);
         } else if (w[0] == "element" && w.size() >= 3) {
-            Elem e;
+            PlyElem e;
             e.name = w[1];
             e.count = std::strtoll(w[2].c_str(), nullptr, 10);
             if (e.count < 0) return fail(SVO_ERR_FORMAT, "negative element count");
             elems.push_back(e);
         } else if (w[0] == "property") {
             if (elems.empty()) return fail(SVO_ERR_FORMAT, "property before element");
-            Prop p;
+            PlyProp p;
             if (w.size() >= 5 && w[1] == "list") {
                 p.list = true;
                 p.count_type = parse_type(w[2]);
@@ -152,23 +135,44 @@ int read_ply(const char *path_in, std::vector<float> &verts, int &has_normals, s
         }
     }
     if (!ended || format < 0) return fail(SVO_ERR_FORMAT, "PLY header incomplete");
-    const Elem *ve = nullptr, *fe = nullptr;
-    for (const Elem &e : elems) {
-        if (e.name == "vertex") ve = &e;
-        else if (e.name == "face") fe = &e;
+    L.binary = format == 1;
+    L.body = pos;
+    L.vertex = L.face = -1;
+    for (size_t i = 0; i < elems.size(); ++i) {
+        if (elems[i].name == "vertex") L.vertex = int(i);
+        else if (elems[i].name == "face") L.face = int(i);
     }
-    if (!ve || !fe) return fail(SVO_ERR_FORMAT, "PLY file has to have triangles and vertices");
-    static const char *vnames[9] = {"x", "y", "z", "nx", "ny", "nz", "red", "green", "blue"};
+    if (L.vertex < 0 || L.face < 0) return fail(SVO_ERR_FORMAT, "PLY file has to have triangles and vertices");
+    const PlyElem *ve = &elems[size_t(L.vertex)], *fe = &elems[size_t(L.face)];
     bool avail[9] = {false};
-    for (const Prop &p : ve->props)
+    for (const PlyProp &p : ve->props)
         for (int t = 0; t < 9; ++t)
-            if (!p.list && p.name == vnames[t]) avail[t] = true;
-    has_normals = avail[3] && avail[4] && avail[5];
+            if (!p.list && p.name == kPlyVertexNames[t]) avail[t] = true;
+    L.has_normals = avail[3] && avail[4] && avail[5];
     bool faceIdx = false;
-    for (const Prop &p : fe->props)
+    for (const PlyProp &p : fe->props)
         if (p.list && p.name == "vertex_indices") faceIdx = true;
     if (!faceIdx) return fail(SVO_ERR_FORMAT, "No face information found");
+    return SVO_OK;
+}
 
+int read_ply(const char *path_in, std::vector<float> &verts, int &has_normals, std::vector<int32_t> &face_counts,
+             std::vector<int32_t> &face_idx) {
+    PlyLayout L;
+    int r = ply_open(path_in, L);
+    if (r) return r;
+    return ply_parse_host(L, verts, has_normals, face_counts, face_idx);
+}
+
+int ply_parse_host(const PlyLayout &L, std::vector<float> &verts, int &has_normals, std::vector<int32_t> &face_counts,
+                   std::vector<int32_t> &face_idx) {
+    const std::vector<uint8_t> &buf = L.buf;
+    const std::vector<PlyElem> &elems = L.elems;
+    const PlyElem *ve = &elems[size_t(L.vertex)], *fe = &elems[size_t(L.face)];
+    const int format = L.binary ? 1 : 0;
+    const size_t pos = L.body;
+    const char *const *vnames = kPlyVertexNames;
+    has_normals = L.has_normals;
     const float defaults[9] = {0, 0, 0, 0, 0, 0, 255.0f, 255.0f, 255.0f};
     verts.assign(size_t(ve->count) * 9, 0.0f);
     face_counts.clear();
@@ -194,12 +198,12 @@ int read_ply(const char *path_in, std::vector<float> &verts, int &has_normals, s
         v = is_int_type(t) ? double(std::atoi(tok.c_str())) : std::atof(tok.c_str());
         return true;
     };
-    for (const Elem &e : elems) {
+    for (const PlyElem &e : elems) {
         const bool isV = &e == ve, isF = &e == fe;
         for (int64_t i = 0; i < e.count; ++i) {
             float row[9];
             if (isV) std::memcpy(row, defaults, sizeof(row));
-            for (const Prop &pr : e.props) {
+            for (const PlyProp &pr : e.props) {
                 if (pr.list) {
                     double cv;
                     if (!read_item(pr.count_type, cv)) return fail(SVO_ERR_FORMAT, "PLY data truncated");
@@ -227,5 +231,12 @@ int read_ply(const char *path_in, std::vector<float> &verts, int &has_normals, s
         if (v < 0 || int64_t(v) >= ve->count) return fail(SVO_ERR_FORMAT, "face references a missing vertex");
     return SVO_OK;
 }
+
+int ply_type_size(PType t) {
+    static const int sz[] = {0, 1, 2, 4, 1, 2, 4, 4, 8};
+    return sz[t];
+}
+
+const char *const kPlyVertexNames[9] = {"x", "y", "z", "nx", "ny", "nz", "red", "green", "blue"};
 
 }  // namespace svo

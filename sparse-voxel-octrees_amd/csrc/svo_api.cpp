@@ -424,13 +424,11 @@ int svo_build_from_ply(const char *path, int resolution, const svo_build_opts *o
     clear_error();
     if (!path || !out) return fail(SVO_ERR_INVALID, "null argument");
     *out = nullptr;
-    std::vector<float> verts;
-    std::vector<int32_t> counts, idx;
-    int has_normals = 0;
-    int r = read_ply(path, verts, has_normals, counts, idx);
+    std::unique_ptr<svo_octree> t(new (std::nothrow) svo_octree());
+    if (!t) return fail(SVO_ERR_NOMEM, "out of host memory");
+    int r = build_ply(path, resolution, opts, *t);
     if (r) return r;
-    return svo_build_from_mesh(verts.data(), int64_t(verts.size() / 9), has_normals, counts.data(), idx.data(),
-                               int64_t(counts.size()), resolution, opts, out);
+    return finish_handle(t, out);
 }
 
 int svo_build_from_grid(const uint32_t *grid, int w, int h, int d, const svo_build_opts *opts, svo_octree **out) {

@@ -95,7 +95,35 @@ int sdf_volume(const svo_sdf_primitive *prims, int count, int n, float shade, ui
 int build_terrain(int n, uint32_t seed, const svo_build_opts *opts, svo_octree &out);
 int terrain_volume(int n, uint32_t seed, uint32_t *grid);
 int terrain_heights(int n, uint32_t seed, int32_t *heights, float *h);
+// ---- PLY (ply_reader.cpp) ----
+enum PType { P_INVALID = 0, P_CHAR, P_SHORT, P_INT, P_UCHAR, P_USHORT, P_UINT, P_FLOAT, P_DOUBLE };
+int ply_type_size(PType t);
+extern const char *const kPlyVertexNames[9];  // x y z nx ny nz red green blue
+struct PlyProp {
+    std::string name;
+    bool list = false;
+    PType type = P_INVALID, count_type = P_INVALID;
+};
+struct PlyElem {
+    std::string name;
+    int64_t count = 0;
+    std::vector<PlyProp> props;
+};
+struct PlyLayout {
+    std::vector<uint8_t> buf;  // the whole file
+    std::vector<PlyElem> elems;
+    size_t body = 0;           // first byte after end_header
+    bool binary = false;       // binary_little_endian (else ascii)
+    int vertex = -1, face = -1;
+    int has_normals = 0;
+};
+// Reads the file and parses its header (the subset src/third-party/plyfile.c accepts).
+int ply_open(const char *path, PlyLayout &layout);
+// Parses the body on the host: 9 floats per vertex (defaults 0 / 255), face sizes and indices.
+int ply_parse_host(const PlyLayout &layout, std::vector<float> &verts, int &has_normals,
+                   std::vector<int32_t> &face_counts, std::vector<int32_t> &face_idx);
 int read_ply(const char *path, std::vector<float> &verts, int &has_normals,
              std::vector<int32_t> &face_counts, std::vector<int32_t> &face_idx);
+int build_ply(const char *path, int resolution, const svo_build_opts *opts, svo_octree &out);
 
 }  // namespace svo

@@ -284,3 +284,80 @@ def test_terrain_full_size(svo):
     assert np.array_equal(r["t"][sample].view(np.uint32), prim["t"].view(np.uint32))
     assert np.array_equal(r["shadow"][sample], shadow)
     assert (r["hit"] == 1).mean() > 0.3
+
+
+# ---- PLY ingest: device decode of binary bodies, host parser otherwise ---------
+
+def _write_custom_ply(path, verts, faces, vtypes, count_type="uchar", idx_type="int", extra_vertex=None,
+                      extra_face=None, extra_elem=None):
+    """Binary little-endian PLY with chosen property types. verts: (n, k)
+    columns named by vtypes [(name, type)]; faces: list of index lists."""
+    np_t = {"char": "i1", "uchar": "u1", "short": "<i2", "ushort": "<u2", "int": "<i4", "uint": "<u4",
+            "float": "<f4", "double": "<f8"}
+    hdr = ["ply", "format binary_little_endian 1.0"]
+    if extra_elem:
+        hdr += [f"element {extra_elem[0]} {extra_elem[1]}", "property int tag"]
+    hdr.append(f"element vertex {len(verts)}")
+    vfields = list(vtypes) + ([extra_vertex] if extra_vertex else [])
+    hdr += [f"property {t} {n}" for n, t in vfields]
+    hdr.append(f"element face {len(faces)}")
+    if extra_face:
+        hdr.append(f"property {extra_face[1]} {extra_face[0]}")
+    hdr += [f"property list {count_type} {idx_type} vertex_indices", "end_header"]
+    body = bytearray()
+    if extra_elem:
+        body += np.arange(extra_elem[1], dtype="<i4").tobytes()
+    rec = np.zeros(len(verts), dtype=[(n, np_t[t]) for n, t in vfields])
+    for j, (n, t) in enumerate(vtypes):
+        rec[n] = verts[:, j]
+    body += rec.tobytes()
+    for k, f in enumerate(faces):
+        if extra_face:
+            body += np.array([k], np_t[extra_face[1]]).tobytes()
+        body += np.array([len(f)], np_t[count_type]).tobytes() + np.array(f, np_t[idx_type]).tobytes()
+    with open(path, "wb") as fp:
+        fp.write(("\n".join(hdr) + "\n").encode() + bytes(body))
+
+
+@pytest.mark.parametrize("case", ["types", "quads", "extra"])
+def test_ply_ingest_matches_oracle(svo, tmp_path, case):
+    """Binary PLY variants through svo_build_from_ply against the oracle's own
+    parse + build: other vertex property types and an extra vertex property
+    (device decode), faces of mixed sizes (host parser), an element before the
+    vertices and a face property before the index list (device decode)."""
+    v, f = svo.scenes.icosphere(20, seed=77, displacement=0.05)
+    v = v.astype(np.float64)
+    rng = np.random.default_rng(1)
+    faces = [list(x) for x in f]
+    if case == "types":
+        cols = np.concatenate([v, rng.integers(0, 256, (len(v), 3))], axis=1)
+        vt = [("x", "double"), ("y", "double"), ("z", "double"), ("red", "uchar"), ("green", "uchar"), ("blue", "uchar")]
+        kw = dict(count_type="uchar", idx_type="uint", extra_vertex=("confidence", "float"))
+    elif case == "quads":
+        cols, vt = v, [("x", "float"), ("y", "float"), ("z", "float")]
+        for k in range(0, len(faces) - 1, 7):  # merge some triangle pairs into 4- and 5-gons
+            faces[k] = faces[k] + [faces[k + 1][2]] + ([faces[k + 1][1]] if k % 2 else [])
+        kw = dict(count_type="int", idx_type="int")
+    else:
+        nrm = v / np.linalg.norm(v, axis=1, keepdims=True)
+        cols = np.concatenate([v, nrm], axis=1)
+        vt = [("x", "float"), ("y", "float"), ("z", "float"), ("nx", "float"), ("ny", "float"), ("nz", "float")]
+        kw = dict(count_type="uchar", idx_type="int", extra_face=("flags", "ushort"), extra_elem=("camera", 3))
+    ply = str(tmp_path / f"{case}.ply")
+    _write_custom_ply(ply, cols, faces, vt, **kw)
+    print(f"[gpu] PLY ingest {case}", flush=True)
+    c, words, _ = orc.Mesh.from_ply(ply).build(256, 8, 1 << 30)
+    t = svo.VoxelOctree.build_from_ply(ply, 256, threads=8)
+    assert np.array_equal(t.words(), words)
+    assert np.array_equal(t.center(), c)
+
+
+def test_ply_ingest_rejects_bad_indices(svo, tmp_path):
+    v, f = svo.scenes.icosphere(4, seed=1)
+    faces = [list(x) for x in f]
+    faces[5][1] = len(v) + 3
+    ply = str(tmp_path / "bad.ply")
+    _write_custom_ply(ply, v, faces, [("x", "float"), ("y", "float"), ("z", "float")])
+    with pytest.raises(svo.SvoError) as e:
+        svo.VoxelOctree.build_from_ply(ply, 64)
+    assert e.value.code == svo.svo.ERR_FORMAT
