@@ -238,6 +238,59 @@ def run_terrain(pkg, torch, stream, steps, warmup, cpu_threads, n=8192, seed=819
             "parity": {"rays_checked": cpu_rays, "mismatched_fields": mism}}
 
 
+def run_frames(pkg, torch, stream, tree, cams, width, height, steps, warmup, cpu_check=True):
+    """Viewer frames (src/Main.cpp renderBatch/renderTile/shade): per frame a
+    coarse LOD pass over 8x8-tile corners and a fine pass over tiles that may
+    hit, shaded to RGBA, 16 stripes, stride 1. One step renders every camera's
+    frame once; the rate counts coarse + traced fine rays."""
+    import oracle_py as orc
+    dev = torch.device("cuda", 0)
+    m = width * height
+    nd = pkg.svo.frame_depth_count(width, height, 16)
+    rgba = [torch.empty(m, dtype=torch.int32, device=dev) for _ in cams]
+    depth = [torch.empty(nd, dtype=torch.float32, device=dev) for _ in cams]
+    sptr = stream.cuda_stream
+
+    def step():
+        for k, c in enumerate(cams):
+            tree.render_frame_device(c, width, height, rgba[k].data_ptr(), depth[k].data_ptr(), 16, 1, sptr)
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    for _ in range(steps):
+        step()
+    b.record(stream)
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / steps
+    # rays traced: every coarse corner, plus every pixel of a traced tile (those
+    # carry an opaque colour; pixels of skipped tiles stay 0)
+    traced = 0
+    for k in range(len(cams)):
+        img = rgba[k].cpu().numpy().view(np.uint32)
+        traced += nd + int(np.count_nonzero(img))
+    out = {"workload": f"viewer_frames_{width}x{height}", "frames_per_step": len(cams),
+           "ms_per_frame": round(ms / len(cams), 4), "frames_per_s": round(len(cams) / (ms / 1e3), 1),
+           "rays_per_step": traced, "grays_s": round(traced / (ms / 1e3) / 1e9, 4),
+           "fine_pixels_traced_frac": round((traced - nd * len(cams)) / (m * len(cams)), 4)}
+    if cpu_check:
+        words = tree.words()
+        tf = np.frombuffer(cams[0].tform, np.float32)
+        c0 = time.perf_counter()
+        ref_rgba, ref_depth = orc.render_frame(words, tf, tree.center(), width, height, 16, 1)
+        cpu_s = time.perf_counter() - c0
+        g_rgba = rgba[0].cpu().numpy().view(np.uint32).reshape(height, width)
+        g_depth = depth[0].cpu().numpy().view(np.uint32)
+        out["parity"] = {"frames_checked": 1, "mismatched_pixels": int(np.count_nonzero(g_rgba != ref_rgba)),
+                         "mismatched_depths": int(np.count_nonzero(g_depth != ref_depth.view(np.uint32)))}
+        out["cpu_baseline"] = {"value": round(1.0 / cpu_s, 3), "unit": "frames/s", "cores": 1, "kind": "port",
+                               "sample": f"one {width}x{height} frame through the oracle's frame loop (one thread; "
+                                         f"the reference viewer splits it over 16 threads)"}
+    return out
+
+
 def run_served(pkg, trees, rates, seconds, seed=2000):
     """Served stream: Poisson request streams (seeded) over the 64 resident
     octrees, each request (octree U{0..63}, seeded orbit camera, resolution as
@@ -298,6 +351,9 @@ def main():
     ap.add_argument("--serve-rates", default="100,500,1000,2000", help="mean request rates (req/s) to replay")
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="schedule length per rate")
     ap.add_argument("--no-terrain", action="store_true", help="skip the 8192^3 terrain + shadow-ray workload")
+    ap.add_argument("--no-frames", action="store_true", help="skip the viewer-frame workload")
+    ap.add_argument("--only", choices=["terrain", "frames"], default=None,
+                    help="run just that workload and print its JSON (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (no multi-GPU path); run with --gpus 1",
@@ -310,6 +366,10 @@ def main():
     if pkg.svo.device_count() < 1 or not torch.cuda.is_available():
         raise SystemExit("bench.py: no GPU visible")
     torch.cuda.set_device(0)
+    if args.only == "terrain":
+        print(json.dumps(run_terrain(pkg, torch, torch.cuda.current_stream(), args.steps, args.warmup,
+                                     args.cpu_threads)), flush=True)
+        return
     tree, config = make_workload(pkg, args.workload, args.cache)
     tree.upload()
     W, H, V = args.width, args.height, args.views
@@ -317,6 +377,10 @@ def main():
     angles = pkg.scenes.orbit_angles(1080, V)
     cams = [tree.orbit_camera(p, y, 1.0) for p, y in angles]
 
+    if args.only == "frames":
+        print(json.dumps(run_frames(pkg, torch, torch.cuda.current_stream(), tree, cams, W, H, args.steps,
+                                    args.warmup, cpu_check=not args.no_cpu)), flush=True)
+        return
     dev = torch.device("cuda", 0)
     normal = [torch.empty(n, dtype=torch.int32, device=dev) for _ in range(V)]
     tbuf = [torch.empty(n, dtype=torch.float32, device=dev) for _ in range(V)]
@@ -401,6 +465,9 @@ def main():
     if not args.no_mixed and not args.no_serve:
         served = run_served(pkg, run_mixed.trees, [float(x) for x in args.serve_rates.split(",")], args.serve_seconds)
 
+    frames = None
+    if not args.no_frames:
+        frames = run_frames(pkg, torch, stream, tree, cams, W, H, max(1, min(args.steps, 5)), 1)
     terrain = None
     if not args.no_terrain:
         terrain = run_terrain(pkg, torch, stream, max(1, min(args.steps, 5)), 1, args.cpu_threads)
@@ -469,13 +536,15 @@ def main():
         "mixed_batched": mixed,
         "served": served,
         "large_terrain": terrain,
+        "viewer_frames": frames,
         "parity": {"rays_checked": cpu_rays, "mismatched_fields": mismatches,
                    "note": "plain-kernel outputs vs the oracle on every ray; batched outputs vs plain below"},
         "wall_s": round(wall, 3),
     }
     print(json.dumps(out), flush=True)
     if mismatches or batched_mismatch or config.get("build_oct_md5_matches_reference") is False or \
-            (mixed and mixed["mismatched_fields_vs_plain"]) or (terrain and terrain["parity"]["mismatched_fields"]):
+            (mixed and mixed["mismatched_fields_vs_plain"]) or (terrain and terrain["parity"]["mismatched_fields"]) or \
+            (frames and (frames["parity"]["mismatched_pixels"] or frames["parity"]["mismatched_depths"])):
         sys.exit(3)
 
 
