@@ -158,6 +158,13 @@ public:
         svo_shim::check(svo_render_primary(h_, &cam, width, height, hit, normal, t, leaf), &err_);
     }
 
+    /* The viewer's frame (src/Main.cpp renderBatch/renderTile/shade, :81-202)
+     * as one call: rgba is width*height pixels (0xFF000000 | grey*0x010101,
+     * skipped tiles 0); stride 1, or 3 while the camera moves. */
+    void renderFrame(const svo_camera &cam, int width, int height, uint32_t *rgba, int stride = 1) {
+        svo_shim::check(svo_render_frame(h_, &cam, width, height, 16, stride, rgba, nullptr), &err_);
+    }
+
     svo_octree *handle() const { return h_; }
     bool ok() const { return h_ != nullptr && err_.empty(); }
     const std::string &error() const { return err_; }

@@ -4,6 +4,7 @@
 //   shim_main cpu <in.oct> <out.oct>            load + center + save, error path
 //   shim_main gpu <in.oct> <rays.f32> <out.bin> per-ray raymarch of n x 7 floats
 //   shim_main build <in.ply> <res> <threads> <mem> <out.oct>
+//   shim_main frame <in.oct> <pitch> <yaw> <radius> <w> <h> <stride> <out.rgba>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -57,6 +58,17 @@ int main(int argc, char **argv) {
         data.setThreads(std::atoi(argv[4]));
         VoxelOctree tree(&data);
         tree.save(argv[6]);
+        return 0;
+    }
+    if (mode == "frame" && argc == 10) {
+        VoxelOctree tree(argv[2]);
+        svo_camera cam = tree.orbitCamera(float(std::atof(argv[3])), float(std::atof(argv[4])), float(std::atof(argv[5])));
+        const int w = std::atoi(argv[6]), h = std::atoi(argv[7]);
+        std::vector<uint32_t> pixels(size_t(w) * h);
+        tree.renderFrame(cam, w, h, pixels.data(), std::atoi(argv[8]));
+        FILE *fp = std::fopen(argv[9], "wb");
+        std::fwrite(pixels.data(), 4, pixels.size(), fp);
+        std::fclose(fp);
         return 0;
     }
     return 1;

@@ -52,3 +52,18 @@ def test_frontend_builder_matches_reference(shim_bin, golden, tmp_path):
     subprocess.run([shim_bin, "build", os.path.join(golden, "mesh_ico16n.ply"), str(b["resolution"]), str(b["threads"]),
                     str(b["mem"]), str(tmp_path / "b.oct")], check=True, timeout=120)
     assert (tmp_path / "b.oct").read_bytes() == open(os.path.join(golden, b["oct"]), "rb").read()
+
+
+@pytest.mark.gpu
+def test_frontend_frame_matches_reference(shim_bin, golden, tmp_path):
+    """renderFrame (the viewer's renderBatch/renderTile/shade as one call)
+    reproduces the reference driver's frames."""
+    g = np.load(os.path.join(golden, "frame_dragon.npz"))
+    w, h = (int(v) for v in g["size"])
+    for i, (pitch, yaw, radius) in enumerate(g["cams"][:2]):
+        for stride in (1, 3):
+            print(f"[gpu] C++ front end frame cam {i} stride {stride}", flush=True)
+            out = tmp_path / f"f{i}_{stride}.rgba"
+            subprocess.run([shim_bin, "frame", os.path.join(golden, "dragon.oct"), repr(float(pitch)), repr(float(yaw)),
+                            repr(float(radius)), str(w), str(h), str(stride), str(out)], check=True, timeout=120)
+            assert np.array_equal(np.fromfile(out, np.uint32).reshape(h, w), g[f"rgba_s{stride}"][i])
