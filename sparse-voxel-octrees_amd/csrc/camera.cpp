@@ -99,4 +99,12 @@ void orbit_camera(float pitch, float yaw, float radius, const float center[3], s
     }
 }
 
+bool camera_ok(const svo_camera &cam) {
+    for (float v : cam.tform)
+        if (!(std::fabs(v) <= 1e6f)) return false;
+    for (float v : cam.pos)
+        if (!(std::fabs(v) <= 1e6f)) return false;
+    return true;
+}
+
 }  // namespace svo

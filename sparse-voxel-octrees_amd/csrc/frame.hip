@@ -62,15 +62,15 @@ __device__ __forceinline__ int stripe_of(const Stripe *__restrict__ st, int n, i
     return lo;
 }
 
-template <typename Idx>
+template <bool Lod, typename Idx>
 __device__ __forceinline__ int trace(const uint32_t *__restrict__ tree, float ox, float oy, float oz, float dx, float dy,
                                      float dz, float rayScale, const Stack<Idx> &stk, uint32_t &normal, float &t) {
     Ray<Idx> r;
-    ray_init(r, ox, oy, oz, dx, dy, dz, rayScale);
+    ray_init(r, tree, ox, oy, oz, dx, dy, dz, rayScale);
     Idx leaf = 0;
     int code;
     do {
-        code = ray_step(r, tree, stk, normal, t, leaf);
+        code = ray_step<Lod, true>(r, tree, stk, normal, t, leaf);  // camera rays are finite
     } while (code == kRunning);
     return code == kBad ? march::kMiss : code;
 }
@@ -102,7 +102,7 @@ __global__ void __launch_bounds__(64) coarse_kernel(const uint32_t *__restrict__
     Stack<Idx> stk(lds, 64, slots, int(threadIdx.x));
     uint32_t normal = 0;
     float t = 0.0f;
-    const int code = trace<Idx>(tree, p.px, p.py, p.pz, d[0], d[1], d[2], p.coarseScale, stk, normal, t);
+    const int code = trace<true, Idx>(tree, p.px, p.py, p.pz, d[0], d[1], d[2], p.coarseScale, stk, normal, t);
     depth[i] = code != march::kMiss ? t : kTreeMiss;
 }
 
@@ -140,8 +140,8 @@ __global__ void __launch_bounds__(64) fine_kernel(const uint32_t *__restrict__ t
             uint32_t normal = 0;
             float t = 0.0f;
             float v = 0.0f;
-            const int code = trace<Idx>(tree, p.px + d[0] * m, p.py + d[1] * m, p.pz + d[2] * m, d[0], d[1], d[2], 0.0f,
-                                        stk, normal, t);
+            const int code = trace<false, Idx>(tree, p.px + d[0] * m, p.py + d[1] * m, p.pz + d[2] * m, d[0], d[1], d[2],
+                                               0.0f, stk, normal, t);
             if (code != march::kMiss) {
                 // shade, src/Main.cpp:81-90
                 float n[3], c;
@@ -174,6 +174,7 @@ int render_frame_device(svo_octree &h, const svo_camera &cam, int width, int hei
     if (int64_t(stripes - 1) * ((height - 1) / stripes + 1) >= height)
         return fail(SVO_ERR_INVALID, "stripe split leaves an empty stripe at this height");
     if (!rgba) return fail(SVO_ERR_INVALID, "null image");
+    if (!camera_ok(cam)) return fail(SVO_ERR_INVALID, "camera entries must be finite and within +-1e6");
     int r = ensure_device(h);
     if (r) return r;
     const float pi = float(3.14159265358979323846);

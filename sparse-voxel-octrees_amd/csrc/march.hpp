@@ -28,7 +28,8 @@ __device__ __forceinline__ float quake_rsqrt(float x) {
     return y * (1.5f - half * y * y);
 }
 
-// Per-lane restart stack in LDS: [slot][stride] offsets then [slot][stride] maxT.
+// Per-lane restart stack in LDS, one entry per scale below the root.
+// 64-bit word indices: [slot][stride] indices then [slot][stride] maxT.
 template <typename Idx>
 struct Stack {
     Idx *off;
@@ -40,6 +41,33 @@ struct Stack {
         : off(reinterpret_cast<Idx *>(base)),
           tmax(reinterpret_cast<float *>(reinterpret_cast<Idx *>(base) + size_t(slots_) * stride_)),
           stride(stride_), slots(slots_), lane(lane_) {}
+    __device__ __forceinline__ void push(int slot, Idx parent, float t) const {
+        off[slot * stride + lane] = parent;
+        tmax[slot * stride + lane] = t;
+    }
+    __device__ __forceinline__ void pop(int slot, Idx &parent, float &t) const {
+        parent = off[slot * stride + lane];
+        t = tmax[slot * stride + lane];
+    }
+};
+
+// 32-bit word indices: one 8-byte {index, maxT} entry per [slot][stride], so a
+// push or pop is a single 64-bit LDS access (same 8 bytes per entry).
+template <>
+struct Stack<uint32_t> {
+    uint2 *e;
+    int stride;
+    int slots;
+    __device__ __forceinline__ Stack(void *base, int stride_, int slots_, int lane_)
+        : e(reinterpret_cast<uint2 *>(base) + lane_), stride(stride_), slots(slots_) {}
+    __device__ __forceinline__ void push(int slot, uint32_t parent, float t) const {
+        e[slot * stride] = make_uint2(parent, __float_as_uint(t));
+    }
+    __device__ __forceinline__ void pop(int slot, uint32_t &parent, float &t) const {
+        const uint2 v = e[slot * stride];
+        parent = v.x;
+        t = __uint_as_float(v.y);
+    }
 };
 
 template <typename Idx>
@@ -54,9 +82,11 @@ struct Ray {
 // Outcome of a finished ray.
 enum : int { kRunning = -1, kMiss = 0, kLeaf = 1, kLod = 2, kBad = 3 };
 
+// Set-up before the loop (:216-250). The root descriptor is read here, the
+// read the first loop pass makes (:254).
 template <typename Idx>
-__device__ __forceinline__ void ray_init(Ray<Idx> &r, float ox, float oy, float oz, float dx, float dy, float dz,
-                                         float rayScale) {
+__device__ __forceinline__ void ray_init(Ray<Idx> &r, const uint32_t *__restrict__ tree, float ox, float oy, float oz,
+                                         float dx, float dy, float dz, float rayScale) {
     if (fabsf(dx) < 1e-4f) dx = 1e-4f;
     if (fabsf(dy) < 1e-4f) dy = 1e-4f;
     if (fabsf(dz) < 1e-4f) dz = 1e-4f;
@@ -74,7 +104,7 @@ __device__ __forceinline__ void ray_init(Ray<Idx> &r, float ox, float oy, float 
     r.maxT = smin(r.dTx - r.bTx, smin(r.dTy - r.bTy, r.dTz - r.bTz));
     minT = smax(minT, 0.0f);
     r.minT = minT;
-    r.current = 0;
+    r.current = tree[0];
     r.parent = 0;
     r.idx = 0;
     r.posX = 1.0f;
@@ -89,22 +119,30 @@ __device__ __forceinline__ void ray_init(Ray<Idx> &r, float ox, float oy, float 
 }
 
 // One pass of the loop body. Returns kRunning, or the outcome with tOut /
-// normal / leaf set as the reference sets them.
-template <typename Idx>
+// normal / leaf set as the reference sets them. The descriptor of the node
+// the ray is in (`current`) is read as soon as its index is known (init,
+// push, pop) rather than at the top of the next pass; the value is the same.
+//
+//  Lod = false: the ray's rayScale is 0 (+0 or -0). maxTC is finite, so
+//    maxTC * rayScale >= scaleExp2 (> 0) never holds and the test is dropped.
+//  Finite = true: the ray's origin and direction are finite (|origin| below
+//    2^100). Then every corner t is finite and never -0 (pos*dT is a nonzero
+//    product; a difference of equal nonzero values is +0), so the minimum of
+//    the three corner times does not depend on the order of ties or on NaN
+//    handling and one hardware min3 returns the reference's std::min result.
+template <bool Lod, bool Finite, typename Idx>
 __device__ __forceinline__ int ray_step(Ray<Idx> &r, const uint32_t *__restrict__ tree, const Stack<Idx> &stk,
                                         uint32_t &normal, float &tOut, Idx &leaf) {
-    if (r.current == 0) r.current = tree[r.parent];
-
     const float cornerTX = r.posX * r.dTx - r.bTx;
     const float cornerTY = r.posY * r.dTy - r.bTy;
     const float cornerTZ = r.posZ * r.dTz - r.bTz;
-    const float maxTC = smin(cornerTX, smin(cornerTY, cornerTZ));
+    const float maxTC = Finite ? fminf(cornerTX, fminf(cornerTY, cornerTZ)) : smin(cornerTX, smin(cornerTY, cornerTZ));
 
     const int childShift = r.idx ^ r.octantMask;
     const uint32_t childMasks = r.current << childShift;
 
     if ((childMasks & 0x8000u) && r.minT <= r.maxT) {
-        if (maxTC * r.rayScale >= r.scaleExp2) {
+        if (Lod && maxTC * r.rayScale >= r.scaleExp2) {
             tOut = maxTC;
             return kLod;
         }
@@ -127,12 +165,12 @@ __device__ __forceinline__ int ray_step(Ray<Idx> &r, const uint32_t *__restrict_
             }
             const int slot = (kMaxScale - 1) - r.scale;
             if (slot >= stk.slots) return kBad;
-            stk.off[slot * stk.stride + stk.lane] = r.parent;
-            stk.tmax[slot * stk.stride + stk.lane] = r.maxT;
+            stk.push(slot, r.parent, r.maxT);
 
             const Idx siblingCount = Idx(__popc(childMasks & 127u));
             r.parent += childOffset + siblingCount;
             if (r.current & 0x10000u) r.parent += siblingCount;
+            r.current = tree[r.parent];
 
             r.idx = 0;
             r.scale--;
@@ -141,7 +179,6 @@ __device__ __forceinline__ int ray_step(Ray<Idx> &r, const uint32_t *__restrict_
             if (centerTY > r.minT) { r.idx ^= 2; r.posY += r.scaleExp2; }
             if (centerTZ > r.minT) { r.idx ^= 4; r.posZ += r.scaleExp2; }
             r.maxT = maxTV;
-            r.current = 0;
             return kRunning;
         }
     }
@@ -166,8 +203,8 @@ __device__ __forceinline__ int ray_step(Ray<Idx> &r, const uint32_t *__restrict_
         if (r.scale >= kMaxScale) return kMiss;
         const int slot = (kMaxScale - 1) - r.scale;
         if (slot >= stk.slots) return kBad;
-        r.parent = stk.off[slot * stk.stride + stk.lane];
-        r.maxT = stk.tmax[slot * stk.stride + stk.lane];
+        stk.pop(slot, r.parent, r.maxT);
+        r.current = tree[r.parent];
 
         const uint32_t shX = f2u(r.posX) >> r.scale;
         const uint32_t shY = f2u(r.posY) >> r.scale;
@@ -176,7 +213,6 @@ __device__ __forceinline__ int ray_step(Ray<Idx> &r, const uint32_t *__restrict_
         r.posY = u2f(shY << r.scale);
         r.posZ = u2f(shZ << r.scale);
         r.idx = int((shX & 1u) | ((shY & 1u) << 1) | ((shZ & 1u) << 2));
-        r.current = 0;
     }
     return kRunning;
 }

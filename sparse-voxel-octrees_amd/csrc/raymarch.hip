@@ -33,15 +33,15 @@ using namespace march;
 
 // Runs one ray to completion. Returns 0 miss, 1 leaf hit, 2 LOD hit, 3 stack
 // overflow (malformed tree).
-template <typename Idx>
+template <bool Lod, bool Finite, typename Idx>
 __device__ __forceinline__ int march_ray(const uint32_t *__restrict__ tree, float ox, float oy, float oz, float dx,
                                          float dy, float dz, float rayScale, const Stack<Idx> &stk, uint32_t &normal,
                                          float &tOut, Idx &leaf) {
     Ray<Idx> r;
-    ray_init(r, ox, oy, oz, dx, dy, dz, rayScale);
+    ray_init(r, tree, ox, oy, oz, dx, dy, dz, rayScale);
     int code;
     do {
-        code = ray_step(r, tree, stk, normal, tOut, leaf);
+        code = ray_step<Lod, Finite>(r, tree, stk, normal, tOut, leaf);
     } while (code == kRunning);
     return code;
 }
@@ -69,8 +69,9 @@ __global__ void __launch_bounds__(256) rays_kernel(const uint32_t *__restrict__ 
     float t = 0.0f;
     Idx leaf = 0;
     const float scale = rs ? rs[i] : rs0;
-    int code = march_ray<Idx>(tree, o[3 * i], o[3 * i + 1], o[3 * i + 2], d[3 * i], d[3 * i + 1], d[3 * i + 2],
-                              scale, stk, normal, t, leaf);
+    // arbitrary caller rays: LOD test kept, exact std::min semantics
+    int code = march_ray<true, false, Idx>(tree, o[3 * i], o[3 * i + 1], o[3 * i + 2], d[3 * i], d[3 * i + 1],
+                                           d[3 * i + 2], scale, stk, normal, t, leaf);
     store_result<Idx>(code, normal, t, leaf, i, hit, nrm, tt, lf);
 }
 
@@ -112,7 +113,7 @@ __global__ void __launch_bounds__(64) primary_kernel(const uint32_t *__restrict_
     uint32_t normal = 0;
     float t = 0.0f;
     Idx leaf = 0;
-    int code = march_ray<Idx>(tree, p.px, p.py, p.pz, d[0], d[1], d[2], 0.0f, stk, normal, t, leaf);
+    int code = march_ray<false, true, Idx>(tree, p.px, p.py, p.pz, d[0], d[1], d[2], 0.0f, stk, normal, t, leaf);
     store_result<Idx>(code, normal, t, leaf, int64_t(y) * p.width + x, hit, nrm, tt, lf);
 }
 
@@ -144,8 +145,8 @@ __global__ void __launch_bounds__(64) shadow_kernel(const uint32_t *__restrict__
     uint32_t normal = 0;
     float t = 0.0f;
     Idx leaf = 0;
-    const int code = march_ray<Idx>(tree, p.px + d[0] * back, p.py + d[1] * back, p.pz + d[2] * back, lx, ly, lz, 0.0f,
-                                    stk, normal, t, leaf);
+    const int code = march_ray<false, true, Idx>(tree, p.px + d[0] * back, p.py + d[1] * back, p.pz + d[2] * back, lx,
+                                                 ly, lz, 0.0f, stk, normal, t, leaf);
     shadow[i] = (code == 1 || code == 2) ? 1 : 0;
 }
 
@@ -241,6 +242,7 @@ int render_primary_device(svo_octree &h, const svo_camera &cam, int width, int h
                           uint32_t *normal, float *t, uint64_t *leaf, void *stream) {
     if (width <= 0 || height <= 0 || width > 65536 || height > 65536)
         return fail(SVO_ERR_INVALID, "image size out of range");
+    if (!camera_ok(cam)) return fail(SVO_ERR_INVALID, "camera entries must be finite and within +-1e6");
     int r = ensure_device(h);
     if (r) return r;
     const PrimaryParams p = primary_params(cam, width, height);
@@ -261,6 +263,8 @@ int render_primary_device(svo_octree &h, const svo_camera &cam, int width, int h
 int render_shadow_device(svo_octree &h, const svo_camera &cam, int width, int height, const float light[3], float eps,
                          uint8_t *hit, uint32_t *normal, float *t, uint64_t *leaf, uint8_t *shadow, void *stream) {
     if (!hit || !t || !shadow || !light) return fail(SVO_ERR_INVALID, "shadow pass needs hit, t, shadow and light");
+    for (int k = 0; k < 3; ++k)
+        if (!(std::fabs(light[k]) <= 1e6f)) return fail(SVO_ERR_INVALID, "light direction must be finite");
     if (!(std::fabs(light[0]) + std::fabs(light[1]) + std::fabs(light[2]) > 0.0f))
         return fail(SVO_ERR_INVALID, "light direction must be non-zero");
     int r = render_primary_device(h, cam, width, height, hit, normal, t, leaf, stream);

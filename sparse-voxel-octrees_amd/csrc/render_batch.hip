@@ -69,7 +69,9 @@ __device__ __forceinline__ uint32_t even_bits(uint32_t m) {  // bits 0,2,4 -> 0,
     return (m & 1u) | ((m >> 1) & 2u) | ((m >> 2) & 4u);
 }
 
-template <typename Idx>
+// Lod = false when every view's rayScale is 0 (the LOD test can never fire);
+// camera rays are finite (cameras are checked on the host).
+template <bool Lod, typename Idx>
 __global__ void __launch_bounds__(256) batch_kernel(const ViewDesc *__restrict__ views,
                                                     const uint64_t *__restrict__ tileBase, int nviews,
                                                     uint32_t *__restrict__ counters, int slots, int refill) {
@@ -144,7 +146,7 @@ __global__ void __launch_bounds__(256) batch_kernel(const ViewDesc *__restrict__
                         const float vy = dx * vd.a21 + dy * vd.a22 + vd.zy;
                         const float vz = dx * vd.a31 + dy * vd.a32 + vd.zz;
                         const float s = quake_rsqrt(vx * vx + vy * vy + vz * vz);
-                        ray_init(r, vd.px, vd.py, vd.pz, vx * s, vy * s, vz * s, vd.rayScale);
+                        ray_init(r, vd.tree, vd.px, vd.py, vd.pz, vx * s, vy * s, vz * s, vd.rayScale);
                         tree = vd.tree;
                         rView = tView;
                         rPix = int64_t(y) * vd.width + x;
@@ -167,7 +169,7 @@ __global__ void __launch_bounds__(256) batch_kernel(const ViewDesc *__restrict__
                 uint32_t normal = 0;
                 float t = 0.0f;
                 Idx leaf = 0;
-                int code = ray_step(r, tree, stk, normal, t, leaf);
+                int code = ray_step<Lod, true>(r, tree, stk, normal, t, leaf);
                 if (code != kRunning) {
                     if (code == kBad) code = kMiss;
                     const ViewDesc &vd = views[rView];
@@ -252,7 +254,7 @@ int render_batch(svo_renderer *rd, const svo_view *views, int nviews, const svo_
     // the previous launch must be done with the staging buffers before they are rewritten
     HIP_TRY(hipEventSynchronize(rd->done));
     int maxDepth = 1;
-    bool wide = false;
+    bool wide = false, lod = false;
     uint64_t tiles = 0;
     const float pi = float(3.14159265358979323846);
     const float planeDist = 1.0f / std::tan(pi / 6.0f);  // src/Main.cpp:158
@@ -261,6 +263,9 @@ int render_batch(svo_renderer *rd, const svo_view *views, int nviews, const svo_
         if (!v.tree) return fail(SVO_ERR_INVALID, "view without an octree");
         if (v.width <= 0 || v.height <= 0 || v.width > 65536 || v.height > 65536)
             return fail(SVO_ERR_INVALID, "view size out of range");
+        if (!camera_ok(v.camera)) return fail(SVO_ERR_INVALID, "camera entries must be finite and within +-1e6");
+        if (!(std::fabs(v.ray_scale) <= 1e6f)) return fail(SVO_ERR_INVALID, "ray_scale must be finite");
+        lod = lod || v.ray_scale != 0.0f;
         int r = ensure_device(*v.tree);
         if (r) return r;
         maxDepth = std::max(maxDepth, v.tree->depth);
@@ -308,12 +313,18 @@ int render_batch(svo_renderer *rd, const svo_view *views, int nviews, const svo_
     // tiles per wave of work (small batches would otherwise launch idle waves)
     const uint64_t workBlocks = (tiles + 15) / 16;
     const unsigned grid = unsigned(std::max<uint64_t>(kQueues, std::min<uint64_t>(uint64_t(rd->cus) * blocksPerCU, workBlocks)));
-    if (!wide)
-        hipLaunchKernelGGL(batch_kernel<uint32_t>, dim3(grid), dim3(256), ldsPerBlock, s, rd->dViews, rd->dTileBase,
-                           nviews, rd->dCounters, slots, refill);
+    if (!wide && !lod)
+        hipLaunchKernelGGL((batch_kernel<false, uint32_t>), dim3(grid), dim3(256), ldsPerBlock, s, rd->dViews,
+                           rd->dTileBase, nviews, rd->dCounters, slots, refill);
+    else if (!wide)
+        hipLaunchKernelGGL((batch_kernel<true, uint32_t>), dim3(grid), dim3(256), ldsPerBlock, s, rd->dViews,
+                           rd->dTileBase, nviews, rd->dCounters, slots, refill);
+    else if (!lod)
+        hipLaunchKernelGGL((batch_kernel<false, uint64_t>), dim3(grid), dim3(256), ldsPerBlock, s, rd->dViews,
+                           rd->dTileBase, nviews, rd->dCounters, slots, refill);
     else
-        hipLaunchKernelGGL(batch_kernel<uint64_t>, dim3(grid), dim3(256), ldsPerBlock, s, rd->dViews, rd->dTileBase,
-                           nviews, rd->dCounters, slots, refill);
+        hipLaunchKernelGGL((batch_kernel<true, uint64_t>), dim3(grid), dim3(256), ldsPerBlock, s, rd->dViews,
+                           rd->dTileBase, nviews, rd->dCounters, slots, refill);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(rd->done, s));
     return SVO_OK;

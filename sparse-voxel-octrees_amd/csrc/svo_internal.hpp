@@ -56,6 +56,9 @@ int octree_validate(const uint32_t *words, uint64_t n);
 
 // ---- camera (camera.cpp) ----
 void orbit_camera(float pitch, float yaw, float radius, const float center[3], svo_camera &cam);
+// Every tform and pos entry finite and within +-1e6: the rays the renderers
+// generate from such a camera are finite (the traversal relies on it).
+bool camera_ok(const svo_camera &cam);
 
 // ---- device (raymarch.hip) ----
 int ensure_device(svo_octree &h);
