@@ -117,13 +117,10 @@ def run_mixed(pkg, torch, stream, steps, warmup, waves_per_cu, refill):
         torch.cuda.synchronize()
         return a.elapsed_time(b) / k
 
-    # tile-queue launch parameters: pick the fastest of a small grid on this workload
+    # tile-queue launch shape: the library's autotuner on this workload
     tune = {}
     if not waves_per_cu and not refill:
-        for w in (16, 24, 32):
-            for r in (8, 16, 32):
-                tune[(w, r)] = timed(lambda: batched(w, r), 1)
-        (waves_per_cu, refill) = min(tune, key=tune.get)
+        (waves_per_cu, refill), tune = ren.autotune(bviews, sptr)
     plain_ms = timed(plain, steps)
     batched_ms = timed(lambda: batched(waves_per_cu, refill), steps)
     # parity: the batched launch reproduces the plain launches bit for bit
@@ -143,8 +140,7 @@ def run_mixed(pkg, torch, stream, steps, warmup, waves_per_cu, refill):
             "plain_grays_s": round(rays / (plain_ms / 1e3) / 1e9, 4), "plain_ms_per_step": round(plain_ms, 3),
             "batched_grays_s": round(rays / (batched_ms / 1e3) / 1e9, 4), "batched_ms_per_step": round(batched_ms, 3),
             "batched_vs_plain": round(plain_ms / batched_ms, 3),
-            "launch": {"waves_per_cu": waves_per_cu, "refill": refill,
-                       "tuned_ms": {f"{w}x{r}": round(t, 3) for (w, r), t in tune.items()}},
+            "launch": {"waves_per_cu": waves_per_cu, "refill": refill, "autotune_ms": tune},
             "mismatched_fields_vs_plain": mism}
 
 
@@ -405,6 +401,20 @@ def main():
     def batched_step():
         renderer.render_device(bviews, sptr, args.waves_per_cu, args.refill)
 
+    def time_graph():
+        g = renderer.capture(bviews, sptr, args.waves_per_cu, args.refill)
+        for _ in range(args.warmup):
+            g.launch(sptr)
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(args.steps):
+            g.launch(sptr)
+        b.record(stream)
+        torch.cuda.synchronize()
+        g.close()
+        return a.elapsed_time(b) / args.steps
+
     def time_batched():
         for _ in range(args.warmup):
             batched_step()
@@ -446,9 +456,11 @@ def main():
     rays = n * V * args.steps
     value = rays / (total_ms / 1e3) / 1e9
     batched_ms = time_batched()
+    graph_ms = time_graph()
     batched = {"value": round(n * V / (batched_ms / 1e3) / 1e9, 4), "unit": "Grays/s",
                "ms_per_step": round(batched_ms, 4), "vs_plain": round((total_ms / args.steps) / batched_ms, 3),
-               "form": f"one persistent launch for all {V} views"}
+               "form": f"one persistent launch for all {V} views",
+               "graph_replay": {"value": round(n * V / (graph_ms / 1e3) / 1e9, 4), "ms_per_step": round(graph_ms, 4)}}
 
     if args.no_cpu:
         print(json.dumps({"metric": METRIC, "value": round(value, 4), "unit": "Grays/s", "n_gpus": 1,

@@ -200,6 +200,25 @@ void svo_renderer_free(svo_renderer *r);
 int svo_render_batch_device(svo_renderer *r, const svo_view *views, int nviews, const svo_batch_opts *opts,
                             void *stream);
 
+/* Launch-shape autotuner: times the batched launch of `views` on `stream` for
+ * waves per CU {16, 24, 32} x refill threshold {8, 16, 32} (one warm-up and
+ * two timed launches each; the launches write the views' outputs) and stores
+ * the fastest in *best. times (may be NULL) receives the 9 averages in ms,
+ * waves-major. */
+int svo_renderer_autotune(svo_renderer *r, const svo_view *views, int nviews, void *stream, svo_batch_opts *best,
+                          float *times);
+
+/* One batched launch of fixed views (cameras, sizes, output pointers) captured
+ * in a HIP graph: the graph holds its own view table and queue counters, so
+ * every replay is the queue reset kernel plus the persistent kernel, launched
+ * with one call. Replays of one graph must not overlap (launch them on one
+ * stream); the octrees and output arrays must outlive the graph. */
+typedef struct svo_batch_graph svo_batch_graph;
+int svo_batch_graph_create(svo_renderer *r, const svo_view *views, int nviews, const svo_batch_opts *opts,
+                           void *stream, svo_batch_graph **out);
+int svo_batch_graph_launch(svo_batch_graph *g, void *stream);
+void svo_batch_graph_free(svo_batch_graph *g);
+
 /* ---- serving: continuous batching of render requests ------------------------ */
 
 /* One render request: primary rays of an orbit/any camera over octree

@@ -247,12 +247,22 @@ void renderer_free(svo_renderer *r) {
     delete r;
 }
 
-int render_batch(svo_renderer *rd, const svo_view *views, int nviews, const svo_batch_opts *opts, void *stream) {
-    if (!rd || !views) return fail(SVO_ERR_INVALID, "null argument");
-    if (nviews <= 0 || nviews > rd->maxViews) return fail(SVO_ERR_LIMIT, "view count exceeds the renderer's max_views");
-    std::lock_guard<std::mutex> lock(rd->mu);
-    // the previous launch must be done with the staging buffers before they are rewritten
-    HIP_TRY(hipEventSynchronize(rd->done));
+namespace {
+
+// Everything one batched launch needs besides the device view table.
+struct LaunchPlan {
+    int nviews = 0;
+    uint64_t tiles = 0;
+    int slots = 1;
+    bool wide = false, lod = false;
+    size_t ldsPerBlock = 0;
+    unsigned grid = 0;
+    int refill = 16;
+};
+
+// Validates the views and fills the host view table, the tile bases and the launch shape.
+int plan_views(const svo_view *views, int nviews, const svo_batch_opts *opts, int cus, ViewDesc *hv, uint64_t *htb,
+               LaunchPlan &plan) {
     int maxDepth = 1;
     bool wide = false, lod = false;
     uint64_t tiles = 0;
@@ -270,7 +280,7 @@ int render_batch(svo_renderer *rd, const svo_view *views, int nviews, const svo_
         if (r) return r;
         maxDepth = std::max(maxDepth, v.tree->depth);
         wide = wide || v.tree->words.size() >= (uint64_t(1) << 32);
-        ViewDesc &d = rd->hViews[i];
+        ViewDesc &d = hv[i];
         d.tree = v.tree->dwords;
         d.hit = v.hit;
         d.normal = v.normal;
@@ -292,41 +302,174 @@ int render_batch(svo_renderer *rd, const svo_view *views, int nviews, const svo_
         d.blocksX = (d.tilesX + 7) / 8;
         d.pad = 0;
         const int blocksY = (d.tilesY + 7) / 8;
-        rd->hTileBase[i] = tiles;
+        htb[i] = tiles;
         tiles += uint64_t(d.blocksX) * uint64_t(blocksY) * 64;
     }
-    rd->hTileBase[nviews] = tiles;
+    htb[nviews] = tiles;
     if (tiles >= (uint64_t(kQueues) << 32)) return fail(SVO_ERR_LIMIT, "too many tiles for one launch");
-    const int slots = maxDepth > 2 ? maxDepth - 1 : 1;
-    const size_t ldsPerWave = size_t(slots) * 64 * ((wide ? 8 : 4) + sizeof(float));
-    const size_t ldsPerBlock = ldsPerWave * 4;
+    plan.nviews = nviews;
+    plan.tiles = tiles;
+    plan.slots = maxDepth > 2 ? maxDepth - 1 : 1;
+    plan.wide = wide;
+    plan.lod = lod;
+    const size_t ldsPerWave = size_t(plan.slots) * 64 * ((wide ? 8 : 4) + sizeof(float));
+    plan.ldsPerBlock = ldsPerWave * 4;
     const size_t ldsPerCU = 160 * 1024;
-    int wavesPerCU = (opts && opts->waves_per_cu > 0) ? opts->waves_per_cu : 32;
-    int blocksPerCU = std::max(1, std::min(wavesPerCU / 4, int(ldsPerCU / ldsPerBlock)));
-    if (ldsPerBlock > 64 * 1024) return fail(SVO_ERR_LIMIT, "octree too deep for the LDS stack");
-    int refill = (opts && opts->refill_threshold > 0) ? std::min(opts->refill_threshold, 64) : 16;
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    HIP_TRY(hipMemcpyAsync(rd->dViews, rd->hViews, sizeof(ViewDesc) * nviews, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(rd->dTileBase, rd->hTileBase, sizeof(uint64_t) * (nviews + 1), hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, s, rd->dCounters);
+    const int wavesPerCU = (opts && opts->waves_per_cu > 0) ? opts->waves_per_cu : 32;
+    const int blocksPerCU = std::max(1, std::min(wavesPerCU / 4, int(ldsPerCU / plan.ldsPerBlock)));
+    if (plan.ldsPerBlock > 64 * 1024) return fail(SVO_ERR_LIMIT, "octree too deep for the LDS stack");
+    plan.refill = (opts && opts->refill_threshold > 0) ? std::min(opts->refill_threshold, 64) : 16;
     // persistent grid: enough blocks to fill the GPU, but no more than four
     // tiles per wave of work (small batches would otherwise launch idle waves)
     const uint64_t workBlocks = (tiles + 15) / 16;
-    const unsigned grid = unsigned(std::max<uint64_t>(kQueues, std::min<uint64_t>(uint64_t(rd->cus) * blocksPerCU, workBlocks)));
-    if (!wide && !lod)
-        hipLaunchKernelGGL((batch_kernel<false, uint32_t>), dim3(grid), dim3(256), ldsPerBlock, s, rd->dViews,
-                           rd->dTileBase, nviews, rd->dCounters, slots, refill);
-    else if (!wide)
-        hipLaunchKernelGGL((batch_kernel<true, uint32_t>), dim3(grid), dim3(256), ldsPerBlock, s, rd->dViews,
-                           rd->dTileBase, nviews, rd->dCounters, slots, refill);
-    else if (!lod)
-        hipLaunchKernelGGL((batch_kernel<false, uint64_t>), dim3(grid), dim3(256), ldsPerBlock, s, rd->dViews,
-                           rd->dTileBase, nviews, rd->dCounters, slots, refill);
+    plan.grid = unsigned(std::max<uint64_t>(kQueues, std::min<uint64_t>(uint64_t(cus) * blocksPerCU, workBlocks)));
+    return SVO_OK;
+}
+
+// Queue reset and the persistent kernel (no memset: the counters are cleared
+// by a kernel, so the pair can be captured in a graph).
+void launch_plan(const LaunchPlan &p, const ViewDesc *dViews, const uint64_t *dTileBase, uint32_t *dCounters,
+                 hipStream_t s) {
+    hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, s, dCounters);
+    if (!p.wide && !p.lod)
+        hipLaunchKernelGGL((batch_kernel<false, uint32_t>), dim3(p.grid), dim3(256), p.ldsPerBlock, s, dViews, dTileBase,
+                           p.nviews, dCounters, p.slots, p.refill);
+    else if (!p.wide)
+        hipLaunchKernelGGL((batch_kernel<true, uint32_t>), dim3(p.grid), dim3(256), p.ldsPerBlock, s, dViews, dTileBase,
+                           p.nviews, dCounters, p.slots, p.refill);
+    else if (!p.lod)
+        hipLaunchKernelGGL((batch_kernel<false, uint64_t>), dim3(p.grid), dim3(256), p.ldsPerBlock, s, dViews, dTileBase,
+                           p.nviews, dCounters, p.slots, p.refill);
     else
-        hipLaunchKernelGGL((batch_kernel<true, uint64_t>), dim3(grid), dim3(256), ldsPerBlock, s, rd->dViews,
-                           rd->dTileBase, nviews, rd->dCounters, slots, refill);
+        hipLaunchKernelGGL((batch_kernel<true, uint64_t>), dim3(p.grid), dim3(256), p.ldsPerBlock, s, dViews, dTileBase,
+                           p.nviews, dCounters, p.slots, p.refill);
+}
+
+}  // namespace
+
+int render_batch(svo_renderer *rd, const svo_view *views, int nviews, const svo_batch_opts *opts, void *stream) {
+    if (!rd || !views) return fail(SVO_ERR_INVALID, "null argument");
+    if (nviews <= 0 || nviews > rd->maxViews) return fail(SVO_ERR_LIMIT, "view count exceeds the renderer's max_views");
+    std::lock_guard<std::mutex> lock(rd->mu);
+    // the previous launch must be done with the staging buffers before they are rewritten
+    HIP_TRY(hipEventSynchronize(rd->done));
+    LaunchPlan plan;
+    int r = plan_views(views, nviews, opts, rd->cus, rd->hViews, rd->hTileBase, plan);
+    if (r) return r;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    HIP_TRY(hipMemcpyAsync(rd->dViews, rd->hViews, sizeof(ViewDesc) * nviews, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(rd->dTileBase, rd->hTileBase, sizeof(uint64_t) * (nviews + 1), hipMemcpyHostToDevice, s));
+    launch_plan(plan, rd->dViews, rd->dTileBase, rd->dCounters, s);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(rd->done, s));
+    return SVO_OK;
+}
+
+int renderer_autotune(svo_renderer *rd, const svo_view *views, int nviews, void *stream, svo_batch_opts *best,
+                      float *times) {
+    if (!rd || !views || !best) return fail(SVO_ERR_INVALID, "null argument");
+    static const int kWaves[3] = {16, 24, 32}, kRefill[3] = {8, 16, 32};
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(hipEventCreate(&e0));
+    hipError_t he = hipEventCreate(&e1);
+    if (he != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return fail(SVO_ERR_DEVICE, std::string("autotune: ") + hipGetErrorString(he));
+    }
+    float bestMs = 0.0f;
+    int r = SVO_OK;
+    for (int i = 0; i < 9 && !r; ++i) {
+        svo_batch_opts o{};
+        o.waves_per_cu = kWaves[i / 3];
+        o.refill_threshold = kRefill[i % 3];
+        r = render_batch(rd, views, nviews, &o, stream);  // warm-up
+        if (!r && (he = hipEventRecord(e0, s)) != hipSuccess) r = fail(SVO_ERR_DEVICE, hipGetErrorString(he));
+        for (int k = 0; k < 2 && !r; ++k) r = render_batch(rd, views, nviews, &o, stream);
+        if (!r && (he = hipEventRecord(e1, s)) != hipSuccess) r = fail(SVO_ERR_DEVICE, hipGetErrorString(he));
+        if (!r && (he = hipEventSynchronize(e1)) != hipSuccess) r = fail(SVO_ERR_DEVICE, hipGetErrorString(he));
+        float ms = 0.0f;
+        if (!r && (he = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) r = fail(SVO_ERR_DEVICE, hipGetErrorString(he));
+        if (r) break;
+        ms *= 0.5f;
+        if (times) times[i] = ms;
+        if (i == 0 || ms < bestMs) {
+            bestMs = ms;
+            *best = o;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return r;
+}
+
+}  // namespace svo
+
+// One batched launch captured in a HIP graph: its own view table, tile bases
+// and queue counters, fixed at capture.
+struct svo_batch_graph {
+    svo::ViewDesc *dViews = nullptr;
+    uint64_t *dTileBase = nullptr;
+    uint32_t *dCounters = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace svo {
+
+void batch_graph_free(svo_batch_graph *g) {
+    if (!g) return;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    if (g->dViews) (void)hipFree(g->dViews);
+    if (g->dTileBase) (void)hipFree(g->dTileBase);
+    if (g->dCounters) (void)hipFree(g->dCounters);
+    delete g;
+}
+
+int batch_graph_create(svo_renderer *rd, const svo_view *views, int nviews, const svo_batch_opts *opts, void *stream,
+                       svo_batch_graph **out) {
+    if (!rd || !views || !out) return fail(SVO_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (nviews <= 0) return fail(SVO_ERR_INVALID, "no views");
+    std::vector<ViewDesc> hv(static_cast<size_t>(nviews));
+    std::vector<uint64_t> htb(static_cast<size_t>(nviews) + 1);
+    LaunchPlan plan;
+    int r = plan_views(views, nviews, opts, rd->cus, hv.data(), htb.data(), plan);
+    if (r) return r;
+    svo_batch_graph *g = new (std::nothrow) svo_batch_graph();
+    if (!g) return fail(SVO_ERR_NOMEM, "out of host memory");
+    hipError_t e = hipMalloc(&g->dViews, sizeof(ViewDesc) * nviews);
+    if (e == hipSuccess) e = hipMalloc(&g->dTileBase, sizeof(uint64_t) * (nviews + 1));
+    if (e == hipSuccess) e = hipMalloc(&g->dCounters, sizeof(uint32_t) * kQueues * kCounterStride);
+    if (e == hipSuccess) e = hipMemcpy(g->dViews, hv.data(), sizeof(ViewDesc) * nviews, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(g->dTileBase, htb.data(), sizeof(uint64_t) * (nviews + 1), hipMemcpyHostToDevice);
+    // capture needs a stream other than the null stream
+    hipStream_t s = static_cast<hipStream_t>(stream), own = nullptr;
+    if (e == hipSuccess && !s) {
+        e = hipStreamCreateWithFlags(&own, hipStreamNonBlocking);
+        s = own;
+    }
+    if (e == hipSuccess) e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        launch_plan(plan, g->dViews, g->dTileBase, g->dCounters, s);
+        hipError_t le = hipGetLastError();
+        e = hipStreamEndCapture(s, &g->graph);
+        if (e == hipSuccess) e = le;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    if (own) (void)hipStreamDestroy(own);
+    if (e != hipSuccess) {
+        batch_graph_free(g);
+        return fail(SVO_ERR_DEVICE, std::string("batch graph: ") + hipGetErrorString(e));
+    }
+    *out = g;
+    return SVO_OK;
+}
+
+int batch_graph_launch(svo_batch_graph *g, void *stream) {
+    if (!g) return fail(SVO_ERR_INVALID, "null graph");
+    HIP_TRY(hipGraphLaunch(g->exec, static_cast<hipStream_t>(stream)));
     return SVO_OK;
 }
 

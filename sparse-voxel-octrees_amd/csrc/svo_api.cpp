@@ -397,6 +397,25 @@ int svo_render_batch_device(svo_renderer *r, const svo_view *views, int nviews, 
     return render_batch(r, views, nviews, opts, stream);
 }
 
+int svo_renderer_autotune(svo_renderer *r, const svo_view *views, int nviews, void *stream, svo_batch_opts *best,
+                          float *times) {
+    clear_error();
+    return renderer_autotune(r, views, nviews, stream, best, times);
+}
+
+int svo_batch_graph_create(svo_renderer *r, const svo_view *views, int nviews, const svo_batch_opts *opts,
+                           void *stream, svo_batch_graph **out) {
+    clear_error();
+    return batch_graph_create(r, views, nviews, opts, stream, out);
+}
+
+int svo_batch_graph_launch(svo_batch_graph *g, void *stream) {
+    clear_error();
+    return batch_graph_launch(g, stream);
+}
+
+void svo_batch_graph_free(svo_batch_graph *g) { batch_graph_free(g); }
+
 int svo_serve_replay(svo_octree *const *trees, int ntrees, const svo_request *requests, const double *arrival_s, int n,
                      const svo_server_opts *opts, double *latency_s, double *makespan_s, uint32_t *const *normal_out,
                      float *const *t_out, uint64_t *const *leaf_out) {

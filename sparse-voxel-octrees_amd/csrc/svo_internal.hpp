@@ -15,7 +15,8 @@ struct svo_octree {
     uint32_t *dwords = nullptr;   // device copy of `words`
 };
 
-struct svo_renderer;  // render_batch.hip
+struct svo_renderer;     // render_batch.hip
+struct svo_batch_graph;  // render_batch.hip
 
 namespace svo {
 
@@ -81,6 +82,12 @@ int64_t frame_depth_count(int width, int height, int stripes);
 int renderer_create(int maxViews, svo_renderer **out);
 void renderer_free(svo_renderer *r);
 int render_batch(svo_renderer *r, const svo_view *views, int nviews, const svo_batch_opts *opts, void *stream);
+int renderer_autotune(svo_renderer *r, const svo_view *views, int nviews, void *stream, svo_batch_opts *best,
+                      float *times);
+int batch_graph_create(svo_renderer *r, const svo_view *views, int nviews, const svo_batch_opts *opts, void *stream,
+                       svo_batch_graph **out);
+int batch_graph_launch(svo_batch_graph *g, void *stream);
+void batch_graph_free(svo_batch_graph *g);
 
 // ---- request loop (serve.cpp) ----
 int serve_replay(svo_octree *const *trees, int ntrees, const svo_request *reqs, const double *arrival, int n,

@@ -33,6 +33,7 @@ EXPORTS = [
     "svo_render_frame", "svo_render_frame_device", "svo_frame_depth_count",
     "svo_build_from_terrain", "svo_terrain_volume", "svo_terrain_heights",
     "svo_render_shadow", "svo_render_shadow_device", "svo_decode_words",
+    "svo_renderer_autotune", "svo_batch_graph_create", "svo_batch_graph_launch", "svo_batch_graph_free",
 ]
 
 
@@ -180,13 +181,18 @@ def _load():
         "svo_terrain_volume": [ctypes.c_int, ctypes.c_uint32, P],
         "svo_terrain_heights": [ctypes.c_int, ctypes.c_uint32, P, P],
         "svo_decode_words": [P, U64, P, P, P],
+        "svo_renderer_autotune": [P, P, ctypes.c_int, P, P, P],
+        "svo_batch_graph_create": [P, P, ctypes.c_int, P, P, P],
+        "svo_batch_graph_launch": [P, P],
+        "svo_batch_graph_free": [P],
         "svo_render_shadow": [P, P, ctypes.c_int, ctypes.c_int, P, F, P, P, P, P, P],
         "svo_render_shadow_device": [P, P, ctypes.c_int, ctypes.c_int, P, F, P, P, P, P, P, P],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
         fn.argtypes = args
-        fn.restype = None if name in ("svo_free", "svo_free_buffer", "svo_renderer_free") else ctypes.c_int
+        fn.restype = None if name in ("svo_free", "svo_free_buffer", "svo_renderer_free",
+                                      "svo_batch_graph_free") else ctypes.c_int
     L.svo_frame_depth_count.restype = I64
     return L
 
@@ -495,3 +501,41 @@ class Renderer:
         o.waves_per_cu = int(waves_per_cu)
         o.refill_threshold = int(refill_threshold)
         _check(lib.svo_render_batch_device(self._r, arr, len(views), ctypes.byref(o), stream))
+
+    def autotune(self, views, stream=None):
+        """((waves_per_cu, refill), {"WxR": ms}) of the fastest launch shape."""
+        arr = (View * len(views))(*views)
+        best = BatchOpts()
+        times = (ctypes.c_float * 9)()
+        _check(lib.svo_renderer_autotune(self._r, arr, len(views), stream, ctypes.byref(best), times))
+        grid = {f"{w}x{r}": round(times[i], 3) for i, (w, r) in enumerate((w, r) for w in (16, 24, 32) for r in (8, 16, 32))}
+        return (best.waves_per_cu, best.refill_threshold), grid
+
+    def capture(self, views, stream=None, waves_per_cu=0, refill_threshold=0):
+        """BatchGraph replaying one batched launch of these views."""
+        return BatchGraph(self, views, stream, waves_per_cu, refill_threshold)
+
+
+class BatchGraph:
+    """A batched launch captured in a HIP graph (svo_batch_graph)."""
+
+    def __init__(self, renderer, views, stream=None, waves_per_cu=0, refill_threshold=0):
+        self._g = ctypes.c_void_p()
+        arr = (View * len(views))(*views)
+        o = BatchOpts()
+        o.waves_per_cu = int(waves_per_cu)
+        o.refill_threshold = int(refill_threshold)
+        _check(lib.svo_batch_graph_create(renderer._r, arr, len(views), ctypes.byref(o), stream, ctypes.byref(self._g)))
+
+    def launch(self, stream=None):
+        _check(lib.svo_batch_graph_launch(self._g, stream))
+
+    def __del__(self):
+        self.close()
+
+    def close(self):
+        if lib is None:  # interpreter shutdown
+            return
+        if getattr(self, "_g", None) and self._g.value:
+            lib.svo_batch_graph_free(self._g)
+            self._g = ctypes.c_void_p()

@@ -130,6 +130,79 @@ def test_batched_renderer_matches_plain_kernel(svo, trees, golden):
             _same(mine, ref, f"{w}x{h} rs={rs}")
 
 
+def _batch_setup(svo, trees, ren, dev, specs):
+    import torch
+    outs, views = [], []
+    for name, w, h, p, y, rs in specs:
+        t = trees[name]
+        cam = t.orbit_camera(p, y, 1.0)
+        o = {"hit": torch.full((w * h,), 9, dtype=torch.uint8, device=dev),
+             "normal": torch.zeros(w * h, dtype=torch.int32, device=dev),
+             "t": torch.zeros(w * h, dtype=torch.float32, device=dev),
+             "leaf": torch.zeros(w * h, dtype=torch.int64, device=dev)}
+        outs.append((t, cam, w, h, rs, o))
+        views.append(ren.view(t, cam, w, h, o["hit"].data_ptr(), o["normal"].data_ptr(), o["t"].data_ptr(),
+                              o["leaf"].data_ptr(), ray_scale=rs))
+    return outs, views
+
+
+def _check_batch(outs, refs):
+    for (t, cam, w, h, rs, o), ref in zip(outs, refs):
+        mine = {"hit": o["hit"].cpu().numpy(), "normal": o["normal"].cpu().numpy().view(np.uint32),
+                "t": o["t"].cpu().numpy(), "leaf": o["leaf"].cpu().numpy().view(np.uint64)}
+        _same(mine, ref, f"{w}x{h} rs={rs}")
+
+
+def test_batch_graph_replays_match_eager(svo, trees):
+    """A batched launch captured in a HIP graph, replayed several times with
+    the program's own eager work and synchronisations in between (outputs
+    cleared by eager fills each time): every replay equals the plain kernel."""
+    import torch
+    dev = torch.device("cuda", 0)
+    specs = [("dragon", 320, 180, 20.0, 35.0, 0.0), ("sdf128", 257, 131, -30.0, 200.0, 0.0),
+             ("dragon", 200, 100, -60.0, 90.0, 0.02)]
+    ren = svo.svo.Renderer(8)
+    outs, views = _batch_setup(svo, trees, ren, dev, specs)
+    refs = []
+    for t, cam, w, h, rs, o in outs:
+        if rs == 0.0:
+            refs.append(t.render_primary(cam, w, h))
+        else:
+            d = orc.primary_dirs(np.frombuffer(cam.tform, np.float32), w, h)
+            refs.append(orc.raymarch(t.words(), np.frombuffer(cam.pos, np.float32)[None, :], d, rs))
+    stream = torch.cuda.Stream()
+    print("[gpu] batch graph capture", flush=True)
+    g = ren.capture(views, stream.cuda_stream)
+    for k in range(4):
+        with torch.cuda.stream(stream):
+            for _, _, _, _, _, o in outs:
+                o["hit"].fill_(9)
+                o["normal"].fill_(-1)
+                o["t"].fill_(-1.0)
+                o["leaf"].fill_(-1)
+            # eager work of the program between replays, on the same stream
+            trees["dragon"].render_primary_device(outs[0][1], 16, 16, None, None, None, None, stream.cuda_stream)
+        print(f"[gpu] batch graph replay {k}", flush=True)
+        g.launch(stream.cuda_stream)
+        stream.synchronize()
+        _check_batch(outs, refs)
+    g.close()
+
+
+def test_autotune_picks_a_shape_and_keeps_results(svo, trees):
+    import torch
+    dev = torch.device("cuda", 0)
+    specs = [("dragon", 320, 180, 20.0, 35.0, 0.0), ("sdf128", 640, 360, 45.0, 300.0, 0.0)]
+    ren = svo.svo.Renderer(4)
+    outs, views = _batch_setup(svo, trees, ren, dev, specs)
+    print("[gpu] batch autotune", flush=True)
+    (w, r), grid = ren.autotune(views)
+    torch.cuda.synchronize()
+    assert w in (16, 24, 32) and r in (8, 16, 32) and len(grid) == 9 and all(v > 0 for v in grid.values())
+    assert grid[f"{w}x{r}"] == min(grid.values())
+    _check_batch(outs, [t.render_primary(cam, ww, hh) for t, cam, ww, hh, _, _ in outs])
+
+
 @pytest.mark.parametrize("one_per_launch,streams,budget", [(False, 4, 1 << 28), (False, 2, 1 << 24), (True, 1, 1 << 26)])
 def test_served_results_equal_alone_and_in_the_mix(svo, trees, one_per_launch, streams, budget):
     """Requests arriving while others run are folded into the next launch;
