@@ -72,7 +72,7 @@ __device__ __forceinline__ int trace(const uint32_t *__restrict__ tree, float ox
     do {
         code = ray_step<Lod, true>(r, tree, stk, normal, t, leaf);  // camera rays are finite
     } while (code == kRunning);
-    return code == kBad ? march::kMiss : code;
+    return code;
 }
 
 __device__ __forceinline__ void frame_dir(const FrameParams &p, float dx, float dy, float d[3]) {

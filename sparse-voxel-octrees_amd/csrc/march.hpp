@@ -80,7 +80,7 @@ struct Ray {
 };
 
 // Outcome of a finished ray.
-enum : int { kRunning = -1, kMiss = 0, kLeaf = 1, kLod = 2, kBad = 3 };
+enum : int { kRunning = -1, kMiss = 0, kLeaf = 1, kLod = 2 };
 
 // Set-up before the loop (:216-250). The root descriptor is read here, the
 // read the first loop pass makes (:254).
@@ -163,8 +163,10 @@ __device__ __forceinline__ int ray_step(Ray<Idx> &r, const uint32_t *__restrict_
                 tOut = r.minT;
                 return kLeaf;
             }
+            // slot < stk.slots: a node at level L (slot L) pushes only when it has a
+            // non-leaf child, and octree_validate then counted depth >= L + 2 =
+            // slots + 1 (every loaded or built tree is validated)
             const int slot = (kMaxScale - 1) - r.scale;
-            if (slot >= stk.slots) return kBad;
             stk.push(slot, r.parent, r.maxT);
 
             const Idx siblingCount = Idx(__popc(childMasks & 127u));
@@ -201,8 +203,7 @@ __device__ __forceinline__ int ray_step(Ray<Idx> &r, const uint32_t *__restrict_
         // the reference reads rayStack[scale] even when scale >= 23 and then
         // leaves the loop; that read cannot reach the result
         if (r.scale >= kMaxScale) return kMiss;
-        const int slot = (kMaxScale - 1) - r.scale;
-        if (slot >= stk.slots) return kBad;
+        const int slot = (kMaxScale - 1) - r.scale;  // a slot pushed earlier: the new scale is above the current one
         stk.pop(slot, r.parent, r.maxT);
         r.current = tree[r.parent];
 

@@ -31,8 +31,7 @@ namespace {
 
 using namespace march;
 
-// Runs one ray to completion. Returns 0 miss, 1 leaf hit, 2 LOD hit, 3 stack
-// overflow (malformed tree).
+// Runs one ray to completion. Returns 0 miss, 1 leaf hit, 2 LOD hit.
 template <bool Lod, bool Finite, typename Idx>
 __device__ __forceinline__ int march_ray(const uint32_t *__restrict__ tree, float ox, float oy, float oz, float dx,
                                          float dy, float dz, float rayScale, const Stack<Idx> &stk, uint32_t &normal,
@@ -49,7 +48,6 @@ __device__ __forceinline__ int march_ray(const uint32_t *__restrict__ tree, floa
 template <typename Idx>
 __device__ __forceinline__ void store_result(int code, uint32_t normal, float t, Idx leaf, int64_t i,
                                              uint8_t *hit, uint32_t *nrm, float *tt, uint64_t *lf) {
-    if (code == 3) code = 0;
     if (hit) hit[i] = uint8_t(code);
     if (nrm) nrm[i] = code == 1 ? normal : 0u;
     if (tt) tt[i] = code == 0 ? 0.0f : t;

@@ -171,7 +171,6 @@ __global__ void __launch_bounds__(256) batch_kernel(const ViewDesc *__restrict__
                 Idx leaf = 0;
                 int code = ray_step<Lod, true>(r, tree, stk, normal, t, leaf);
                 if (code != kRunning) {
-                    if (code == kBad) code = kMiss;
                     const ViewDesc &vd = views[rView];
                     if (vd.hit) vd.hit[rPix] = uint8_t(code);
                     if (vd.normal) vd.normal[rPix] = code == kLeaf ? normal : 0u;
