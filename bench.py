@@ -74,6 +74,38 @@ def make_workload(pkg, name, cache_dir):
     raise SystemExit(f"unknown workload {name}")
 
 
+def reference_cpu_rate(tree, cam, width, height, threads, cache_dir):
+    """The reference's own VoxelOctree::raymarch (oracle/_ref/svo_ref, built
+    from the reference sources) on one view's primary rays with `threads`
+    host threads; None when that binary is not present."""
+    import subprocess
+    import oracle_py as orc
+    exe = os.path.join(ROOT, "oracle", "_ref", "svo_ref")
+    if not os.access(exe, os.X_OK):
+        return None
+    os.makedirs(cache_dir, exist_ok=True)
+    oct_path = os.path.join(cache_dir, "bench_tree.oct")
+    rays_path = os.path.join(cache_dir, "bench_rays.f32")
+    out_path = os.path.join(cache_dir, "bench_rays.out")
+    tree.save(oct_path)
+    d = orc.primary_dirs(np.frombuffer(cam.tform, np.float32), width, height)
+    rays = np.zeros((len(d), 7), np.float32)
+    rays[:, 0:3] = np.frombuffer(cam.pos, np.float32)
+    rays[:, 3:6] = d
+    rays.tofile(rays_path)
+    try:
+        r = subprocess.run([exe, "raymarch", oct_path, rays_path, out_path, str(threads)], capture_output=True,
+                           text=True, timeout=600)
+    except (OSError, subprocess.SubprocessError):
+        return None
+    for line in r.stdout.splitlines():
+        if line.startswith("raymarch_seconds"):
+            f = line.split()
+            out = np.fromfile(out_path, np.uint32).reshape(-1, 3)  # hit, normal, t bits (0 where unwritten)
+            return {"rate": int(f[3]) / float(f[1]) / 1e9, "rays": int(f[3]), "threads": int(f[5]), "out": out}
+    return None
+
+
 def run_mixed(pkg, torch, stream, steps, warmup, waves_per_cu, refill):
     """Mixed batch: 64 GPU-built octrees (128^3..2048^3, SDF scenes and
     displaced meshes) and 256 views (320x240..1920x1080): 256 plain per-view
@@ -522,6 +554,23 @@ def main():
     reads_per_ray = float(counts.sum()) / cpu_rays
     b_ray = 4.0 * reads_per_ray + 16.0
     achieved = b_ray * n / avg_launch_s / 1e9
+    port_rate = cpu_rays / cpu_time / 1e9
+    cpu_baseline = {"value": round(port_rate, 6), "unit": "Grays/s", "cores": args.cpu_threads, "kind": "port",
+                    "sample": f"{cpu_rays} primary rays ({cpu_rays // n} of the {V} views, full {W}x{H}) through "
+                              f"the oracle raymarch"}
+    ref = reference_cpu_rate(tree, cams[0], W, H, args.cpu_threads, args.cache)
+    ref_parity = None
+    if ref is not None:
+        # view 0 against the reference itself: normal and t bits of every ray (0 where a miss leaves them unwritten)
+        g_nrm = normal[0].cpu().numpy().view(np.uint32)
+        g_t = tbuf[0].cpu().numpy().view(np.uint32)
+        ref_parity = {"rays": int(len(g_nrm)), "mismatched_fields": int(np.count_nonzero(g_nrm != ref["out"][:, 1])
+                                                                        + np.count_nonzero(g_t != ref["out"][:, 2]))}
+        mismatches += ref_parity["mismatched_fields"]
+        cpu_baseline = {"value": round(ref["rate"], 6), "unit": "Grays/s", "cores": ref["threads"], "kind": "reference",
+                        "sample": f"{ref['rays']} primary rays (view 0, {W}x{H}) through the reference's own "
+                                  f"VoxelOctree::raymarch (oracle/_ref/svo_ref, {ref['threads']} threads, raymarch loop "
+                                  f"only); the oracle port on all {cpu_rays} rays: {port_rate:.6f} Grays/s"}
 
     out = {
         "metric": METRIC,
@@ -541,16 +590,15 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": profiled_traffic(config["workload"], W, H),
                      "bytes_per_ray": round(b_ray, 2), "avg_launch_ms": round(avg_launch_s * 1e3, 4)},
-        "cpu_baseline": {"value": round(cpu_rays / cpu_time / 1e9, 6), "unit": "Grays/s", "cores": args.cpu_threads,
-                         "kind": "port", "sample": f"{cpu_rays} primary rays ({cpu_rays // n} of the {V} views, "
-                                                   f"full {W}x{H}) through the oracle raymarch"},
+        "cpu_baseline": cpu_baseline,
         "batched": batched,
         "mixed_batched": mixed,
         "served": served,
         "large_terrain": terrain,
         "viewer_frames": frames,
-        "parity": {"rays_checked": cpu_rays, "mismatched_fields": mismatches,
-                   "note": "plain-kernel outputs vs the oracle on every ray; batched outputs vs plain below"},
+        "parity": {"rays_checked": cpu_rays, "mismatched_fields": mismatches, "vs_reference_binary": ref_parity,
+                   "note": "plain-kernel outputs vs the oracle on every ray and vs the reference's own raymarch on "
+                           "view 0; batched outputs vs plain below"},
         "wall_s": round(wall, 3),
     }
     print(json.dumps(out), flush=True)

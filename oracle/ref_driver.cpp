@@ -15,7 +15,8 @@
 //       VoxelData(path, mem) -> VoxelOctree -> save (src/Main.cpp:315-321)
 //   resave      <in.oct> <out.oct>
 //       VoxelOctree(path) -> save: exercises the LZ4 1.7.1 stream encoder
-//   raymarch    <in.oct> <rays.f32> <out.bin>
+//   raymarch    <in.oct> <rays.f32> <out.bin> [threads]
+//       (prints the wall time of the raymarch loop alone)
 //       rays: n x {ox,oy,oz,dx,dy,dz,rayScale} float32
 //       out:  n x {uint32 hit, uint32 normal, float32 t}; normal/t are 0 when
 //             the reference leaves them unwritten (miss or LOD hit)
@@ -44,8 +45,12 @@
 #include "thread/ThreadUtils.hpp"
 #include "third-party/tribox3.h"
 
+#include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
+#include <thread>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -106,20 +111,33 @@ int main(int argc, char **argv) {
         tree->save(argv[3]);
         return 0;
     }
-    if (cmd == "raymarch" && argc == 5) {
+    if (cmd == "raymarch" && (argc == 5 || argc == 6)) {
         std::unique_ptr<VoxelOctree> tree(new VoxelOctree(argv[2]));
         std::vector<float> rays = readAll<float>(argv[3]);
         size_t n = rays.size() / 7;
         std::vector<uint32> out(n * 3, 0);
-        for (size_t i = 0; i < n; ++i) {
-            const float *r = &rays[i * 7];
-            uint32 normal = 0;
-            float t = 0.0f;
-            bool hit = tree->raymarch(Vec3(r[0], r[1], r[2]), Vec3(r[3], r[4], r[5]), r[6], normal, t);
-            out[i * 3 + 0] = hit ? 1u : 0u;
-            out[i * 3 + 1] = normal;
-            memcpy(&out[i * 3 + 2], &t, 4);
-        }
+        // the viewer calls raymarch from many threads on one tree (src/Main.cpp:342-367)
+        const int threads = argc == 6 ? std::max(1, atoi(argv[5])) : 1;
+        std::atomic<size_t> next(0);
+        auto work = [&]() {
+            for (size_t b; (b = next.fetch_add(4096)) < n;)
+                for (size_t i = b; i < std::min(n, b + 4096); ++i) {
+                    const float *r = &rays[i * 7];
+                    uint32 normal = 0;
+                    float t = 0.0f;
+                    bool hit = tree->raymarch(Vec3(r[0], r[1], r[2]), Vec3(r[3], r[4], r[5]), r[6], normal, t);
+                    out[i * 3 + 0] = hit ? 1u : 0u;
+                    out[i * 3 + 1] = normal;
+                    memcpy(&out[i * 3 + 2], &t, 4);
+                }
+        };
+        const auto t0 = std::chrono::steady_clock::now();
+        std::vector<std::thread> pool;
+        for (int k = 1; k < threads; ++k) pool.emplace_back(work);
+        work();
+        for (std::thread &th : pool) th.join();
+        const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        printf("raymarch_seconds %.6f rays %zu threads %d\n", sec, n, threads);
         writeAll(argv[4], out);
         return 0;
     }
