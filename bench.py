@@ -74,12 +74,12 @@ def make_workload(pkg, name, cache_dir):
     raise SystemExit(f"unknown workload {name}")
 
 
-def reference_cpu_rate(tree, cam, width, height, threads, cache_dir):
+def reference_rays(tree, rays7, threads, cache_dir):
     """The reference's own VoxelOctree::raymarch (oracle/_ref/svo_ref, built
-    from the reference sources) on one view's primary rays with `threads`
-    host threads; None when that binary is not present."""
+    from the reference sources) on rays (n x {o, d, rayScale}) with `threads`
+    host threads: {"rate", "rays", "threads", "out" (hit, normal, t bits;
+    0 where unwritten)}, or None when that build is not present."""
     import subprocess
-    import oracle_py as orc
     exe = os.path.join(ROOT, "oracle", "_ref", "svo_ref")
     if not os.access(exe, os.X_OK):
         return None
@@ -88,22 +88,28 @@ def reference_cpu_rate(tree, cam, width, height, threads, cache_dir):
     rays_path = os.path.join(cache_dir, "bench_rays.f32")
     out_path = os.path.join(cache_dir, "bench_rays.out")
     tree.save(oct_path)
-    d = orc.primary_dirs(np.frombuffer(cam.tform, np.float32), width, height)
-    rays = np.zeros((len(d), 7), np.float32)
-    rays[:, 0:3] = np.frombuffer(cam.pos, np.float32)
-    rays[:, 3:6] = d
-    rays.tofile(rays_path)
+    np.ascontiguousarray(rays7, np.float32).tofile(rays_path)
     try:
         r = subprocess.run([exe, "raymarch", oct_path, rays_path, out_path, str(threads)], capture_output=True,
-                           text=True, timeout=600)
+                           text=True, timeout=900)
     except (OSError, subprocess.SubprocessError):
         return None
     for line in r.stdout.splitlines():
         if line.startswith("raymarch_seconds"):
             f = line.split()
-            out = np.fromfile(out_path, np.uint32).reshape(-1, 3)  # hit, normal, t bits (0 where unwritten)
+            out = np.fromfile(out_path, np.uint32).reshape(-1, 3)
             return {"rate": int(f[3]) / float(f[1]) / 1e9, "rays": int(f[3]), "threads": int(f[5]), "out": out}
     return None
+
+
+def reference_cpu_rate(tree, cam, width, height, threads, cache_dir):
+    """reference_rays on one view's primary rays."""
+    import oracle_py as orc
+    d = orc.primary_dirs(np.frombuffer(cam.tform, np.float32), width, height)
+    rays = np.zeros((len(d), 7), np.float32)
+    rays[:, 0:3] = np.frombuffer(cam.pos, np.float32)
+    rays[:, 3:6] = d
+    return reference_rays(tree, rays, threads, cache_dir)
 
 
 def run_mixed(pkg, torch, stream, steps, warmup, waves_per_cu, refill):
@@ -176,7 +182,7 @@ def run_mixed(pkg, torch, stream, steps, warmup, waves_per_cu, refill):
             "mismatched_fields_vs_plain": mism}
 
 
-def run_terrain(pkg, torch, stream, steps, warmup, cpu_threads, n=8192, seed=8192, width=3840, height=2160):
+def run_terrain(pkg, torch, stream, steps, warmup, cpu_threads, cache_dir, n=8192, seed=8192, width=3840, height=2160):
     """Large terrain: the 8192^3 exposed shell of a seeded fBm heightfield,
     built on the GPU column by column (512^3 cache blocks under a 1 GiB
     budget), then 3840x2160 primary rays plus one hard-shadow ray per hit
@@ -251,6 +257,27 @@ def run_terrain(pkg, torch, stream, steps, warmup, cpu_threads, n=8192, seed=819
     cpu_rays = len(sample) + int(hits.sum())
     counts = prim["counts"] + shr["counts"]
     b_ray = 4.0 * float(counts.sum()) / cpu_rays + 16.0
+    cpu_baseline = {"value": round(cpu_rays / cpu_s / 1e9, 6), "unit": "Grays/s", "cores": cpu_threads, "kind": "port",
+                    "sample": f"{len(sample)} sampled pixels' primary rays + their {int(hits.sum())} shadow rays "
+                              f"through the oracle"}
+    # the same rays through the reference's own raymarch, when its build is present
+    rays7 = np.zeros((cpu_rays, 7), np.float32)
+    rays7[:len(sample), 0:3] = pos
+    rays7[:len(sample), 3:6] = d
+    rays7[len(sample):, 0:3] = o
+    rays7[len(sample):, 3:6] = light
+    ref = reference_rays(tree, rays7, cpu_threads, cache_dir)
+    ref_parity = None
+    if ref is not None:
+        ro = ref["out"]
+        ref_parity = {"rays": cpu_rays, "mismatched_fields": int(
+            np.count_nonzero(g_n != ro[:len(sample), 1]) + np.count_nonzero(g_t.view(np.uint32) != ro[:len(sample), 2])
+            + np.count_nonzero(g_sh[sample][hits] != (ro[len(sample):, 0] != 0)))}
+        mism += ref_parity["mismatched_fields"]
+        cpu_baseline = {"value": round(ref["rate"], 6), "unit": "Grays/s", "cores": ref["threads"], "kind": "reference",
+                        "sample": f"{len(sample)} sampled pixels' primary rays + their {int(hits.sum())} shadow rays "
+                                  f"through the reference's own VoxelOctree::raymarch (oracle/_ref/svo_ref, "
+                                  f"{ref['threads']} threads); the oracle port: {cpu_rays / cpu_s / 1e9:.6f} Grays/s"}
     return {"workload": "terrain8192_4k_shadow", "octree": f"GPU-built {n}^3 fBm heightfield shell (seed {seed})",
             "build_seconds": round(build_s, 3), "leaf_voxels": int(st["leaves"]), "words": int(len(words)),
             "build_leaf_voxels_per_s": round(st["leaves"] / build_s, 1),
@@ -260,13 +287,12 @@ def run_terrain(pkg, torch, stream, steps, warmup, cpu_threads, n=8192, seed=819
             "shadow_pass_ms": round(ms - prim_ms, 3),
             "bytes_per_ray": round(b_ray, 2),
             "hbm_frac": round(b_ray * rays / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 5),
-            "cpu_baseline": {"value": round(cpu_rays / cpu_s / 1e9, 6), "unit": "Grays/s", "cores": cpu_threads,
-                             "kind": "port", "sample": f"{len(sample)} sampled pixels' primary rays + their "
-                                                       f"{int(hits.sum())} shadow rays through the oracle"},
-            "parity": {"rays_checked": cpu_rays, "mismatched_fields": mism}}
+            "cpu_baseline": cpu_baseline,
+            "parity": {"rays_checked": cpu_rays, "mismatched_fields": mism, "vs_reference_binary": ref_parity}}
 
 
-def run_frames(pkg, torch, stream, tree, cams, width, height, steps, warmup, cpu_check=True):
+def run_frames(pkg, torch, stream, tree, cams, width, height, steps, warmup, cpu_check=True, angles=None,
+               threads=16, cache_dir=None):
     """Viewer frames (src/Main.cpp renderBatch/renderTile/shade): per frame a
     coarse LOD pass over 8x8-tile corners and a fine pass over tiles that may
     hit, shaded to RGBA, 16 stripes, stride 1. One step renders every camera's
@@ -316,7 +342,44 @@ def run_frames(pkg, torch, stream, tree, cams, width, height, steps, warmup, cpu
         out["cpu_baseline"] = {"value": round(1.0 / cpu_s, 3), "unit": "frames/s", "cores": 1, "kind": "port",
                                "sample": f"one {width}x{height} frame through the oracle's frame loop (one thread; "
                                          f"the reference viewer splits it over 16 threads)"}
+        ref = reference_frame(tree, angles[0] if angles else None, width, height, threads, cache_dir)
+        if ref is not None:
+            nd1 = len(ref_depth)
+            out["parity"]["vs_reference_classes"] = {
+                "mismatched_pixels": int(np.count_nonzero(g_rgba.reshape(-1) != ref["out"][:m])),
+                "mismatched_depths": int(np.count_nonzero(g_depth != ref["out"][m:m + nd1]))}
+            out["cpu_baseline"] = {"value": round(1.0 / ref["seconds"], 3), "unit": "frames/s", "cores": threads,
+                                   "kind": "reference",
+                                   "sample": f"one {width}x{height} frame: the viewer's renderBatch/renderTile/shade "
+                                             f"loop on the reference's own VoxelOctree::raymarch, Mat4, Vec3 "
+                                             f"(oracle/_ref/svo_ref frame; src/Main.cpp needs SDL), one thread per "
+                                             f"stripe on {threads} threads; the oracle port on one thread: "
+                                             f"{1.0 / cpu_s:.3f} frames/s"}
     return out
+
+
+def reference_frame(tree, angle, width, height, threads, cache_dir):
+    """oracle/_ref/svo_ref's frame loop (the reference's classes) on one
+    orbit camera (pitch, yaw; radius 1): {"seconds", "out"} or None."""
+    import subprocess
+    exe = os.path.join(ROOT, "oracle", "_ref", "svo_ref")
+    if angle is None or cache_dir is None or not os.access(exe, os.X_OK):
+        return None
+    os.makedirs(cache_dir, exist_ok=True)
+    oct_path = os.path.join(cache_dir, "bench_tree.oct")
+    cam_path = os.path.join(cache_dir, "bench_cam.f32")
+    out_path = os.path.join(cache_dir, "bench_frame.out")
+    tree.save(oct_path)
+    np.float32([[angle[0], angle[1], 1.0]]).tofile(cam_path)
+    try:
+        r = subprocess.run([exe, "frame", oct_path, cam_path, str(width), str(height), "1", out_path, str(threads)],
+                           capture_output=True, text=True, timeout=600)
+    except (OSError, subprocess.SubprocessError):
+        return None
+    for line in r.stdout.splitlines():
+        if line.startswith("frame_seconds"):
+            return {"seconds": float(line.split()[1]), "out": np.fromfile(out_path, np.uint32)}
+    return None
 
 
 def run_served(pkg, trees, rates, seconds, seed=2000):
@@ -396,7 +459,7 @@ def main():
     torch.cuda.set_device(0)
     if args.only == "terrain":
         print(json.dumps(run_terrain(pkg, torch, torch.cuda.current_stream(), args.steps, args.warmup,
-                                     args.cpu_threads)), flush=True)
+                                     args.cpu_threads, args.cache)), flush=True)
         return
     tree, config = make_workload(pkg, args.workload, args.cache)
     tree.upload()
@@ -407,7 +470,8 @@ def main():
 
     if args.only == "frames":
         print(json.dumps(run_frames(pkg, torch, torch.cuda.current_stream(), tree, cams, W, H, args.steps,
-                                    args.warmup, cpu_check=not args.no_cpu)), flush=True)
+                                    args.warmup, cpu_check=not args.no_cpu, angles=angles, threads=args.cpu_threads,
+                                    cache_dir=args.cache)), flush=True)
         return
     dev = torch.device("cuda", 0)
     normal = [torch.empty(n, dtype=torch.int32, device=dev) for _ in range(V)]
@@ -511,10 +575,11 @@ def main():
 
     frames = None
     if not args.no_frames:
-        frames = run_frames(pkg, torch, stream, tree, cams, W, H, max(1, min(args.steps, 5)), 1)
+        frames = run_frames(pkg, torch, stream, tree, cams, W, H, max(1, min(args.steps, 5)), 1, angles=angles,
+                            threads=args.cpu_threads, cache_dir=args.cache)
     terrain = None
     if not args.no_terrain:
-        terrain = run_terrain(pkg, torch, stream, max(1, min(args.steps, 5)), 1, args.cpu_threads)
+        terrain = run_terrain(pkg, torch, stream, max(1, min(args.steps, 5)), 1, args.cpu_threads, args.cache)
 
     # batched outputs must equal the plain kernel's: snapshot plain, run batched, compare
     step()
@@ -604,7 +669,8 @@ def main():
     print(json.dumps(out), flush=True)
     if mismatches or batched_mismatch or config.get("build_oct_md5_matches_reference") is False or \
             (mixed and mixed["mismatched_fields_vs_plain"]) or (terrain and terrain["parity"]["mismatched_fields"]) or \
-            (frames and (frames["parity"]["mismatched_pixels"] or frames["parity"]["mismatched_depths"])):
+            (frames and (frames["parity"]["mismatched_pixels"] or frames["parity"]["mismatched_depths"]
+                         or any(frames["parity"].get("vs_reference_classes", {}).values()))):
         sys.exit(3)
 
 

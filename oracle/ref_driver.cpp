@@ -29,7 +29,7 @@
 //       in: n x {pitch, yaw, radius, cx, cy, cz} -> INV_MODELVIEW matrix as
 //       built by src/Main.cpp:212,244-250 and src/math/MatrixStack.cpp:65-66
 //       out: n x 16 floats
-//   frame       <in.oct> <cams.f32> <width> <height> <stride> <out.bin>
+//   frame       <in.oct> <cams.f32> <width> <height> <stride> <out.bin> [workers]
 //       cams: n x {pitch, yaw, radius}; per camera the viewer frame of
 //       src/Main.cpp:81-202 over 16 stripes (src/Main.cpp:351-361): Main.cpp
 //       needs SDL and cannot be compiled here, so this loop restates it on the
@@ -188,11 +188,13 @@ int main(int argc, char **argv) {
         writeAll(argv[3], out);
         return 0;
     }
-    if (cmd == "frame" && argc == 8) {
+    if (cmd == "frame" && (argc == 8 || argc == 9)) {
         VoxelOctree tree(argv[2]);
         std::vector<float> cams = readAll<float>(argv[3]);
         const int W = atoi(argv[4]), H = atoi(argv[5]), stride = atoi(argv[6]);
         const int threads = 16, tile = 8;
+        const int workers = argc == 9 ? std::max(1, atoi(argv[8])) : 1;  // the viewer runs one thread per stripe
+        double seconds = 0.0;
         const float aspect = H / (float)W;
         std::vector<uint32_t> out;
         for (size_t c = 0; c + 3 <= cams.size(); c += 3) {
@@ -214,10 +216,12 @@ int main(int argc, char **argv) {
                 return dir * invSqrt(dir.x * dir.x + dir.y * dir.y + dir.z * dir.z);
             };
             const int sh = (H - 1) / threads + 1;
-            for (int i = 0; i < threads; ++i) {
+            std::vector<std::vector<float>> stripeDepths(threads);
+            auto stripe = [&](int i) {
                 const int y0 = i * sh, y1 = std::min((i + 1) * sh, H);
                 const int tilesX = (W - 1) / tile + 2, tilesY = (y1 - y0 - 1) / tile + 2;
-                std::vector<float> db(size_t(tilesX) * tilesY);
+                std::vector<float> &db = stripeDepths[size_t(i)];
+                db.assign(size_t(tilesX) * tilesY, 0.0f);
                 float dy = aspect - y0 * scale;
                 for (int y = 0, idx = 0; y < tilesY; y++, dy -= tileScale) {
                     float dx = -1.0f + 0 * scale;
@@ -256,8 +260,18 @@ int main(int argc, char **argv) {
                         }
                     }
                 }
-                depths.insert(depths.end(), db.begin(), db.end());
-            }
+            };
+            std::atomic<int> nextStripe(0);
+            auto worker = [&]() {
+                for (int i; (i = nextStripe.fetch_add(1)) < threads;) stripe(i);
+            };
+            const auto t0 = std::chrono::steady_clock::now();
+            std::vector<std::thread> pool;
+            for (int k = 1; k < workers; ++k) pool.emplace_back(worker);
+            worker();
+            for (std::thread &th : pool) th.join();
+            seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            for (const std::vector<float> &db : stripeDepths) depths.insert(depths.end(), db.begin(), db.end());
             out.insert(out.end(), img.begin(), img.end());
             for (float v : depths) {
                 uint32_t u;
@@ -265,6 +279,7 @@ int main(int argc, char **argv) {
                 out.push_back(u);
             }
         }
+        printf("frame_seconds %.6f frames %zu workers %d\n", seconds, cams.size() / 3, workers);
         writeAll(argv[7], out);
         return 0;
     }
