@@ -131,7 +131,8 @@ int serve_replay(svo_octree *const *trees, int ntrees, const svo_request *reqs, 
         int r = render_primary_device(*v.tree, v.camera, v.width, v.height, nullptr, v.normal, v.t, v.leaf, s.stream);
         if (!r && !single) r = render_batch(s.renderer, &v, 1, &bo, s.stream);
         if (r) return r;
-        hipError_t e = hipMemcpyAsync(s.host, s.dev, footprint(w), hipMemcpyDeviceToHost, s.stream);
+        // one copy of the whole slab maps every page of both slabs for the copy engine up front
+        hipError_t e = hipMemcpyAsync(s.host, s.dev, slab, hipMemcpyDeviceToHost, s.stream);
         if (e == hipSuccess) e = hipStreamSynchronize(s.stream);
         if (e != hipSuccess) return fail(SVO_ERR_DEVICE, std::string("serve: warm-up: ") + hipGetErrorString(e));
     }
