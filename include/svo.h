@@ -163,6 +163,22 @@ int svo_render_frame_device(svo_octree *h, const svo_camera *cam, int width, int
                             int stride, uint32_t *rgba, float *depth, void *stream);
 int64_t svo_frame_depth_count(int width, int height, int stripes);
 
+/* Counters of the GPU build that produced a tree (all zero for a loaded tree),
+ * for the construction roofline: algorithmic bytes B = 36*triangles (vertex
+ * reads) + 24*candidates (each (voxel, triangle) key or source voxel written
+ * and read once) + 4*words (output). device_ms is the wall time from the
+ * first upload to the last scatter kernel (file reading and the download of
+ * the words excluded). */
+typedef struct {
+    uint64_t triangles;
+    uint64_t candidates;
+    uint64_t leaves;
+    uint64_t words;
+    double device_ms;
+    uint64_t reserved[3];
+} svo_build_stats;
+int svo_build_stats_get(const svo_octree *h, svo_build_stats *out);
+
 /* ---- batched rendering: many views of many octrees in one launch ----------- */
 
 /* One view: an octree, a camera and its device output arrays (width*height

@@ -30,6 +30,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <unistd.h>
@@ -55,40 +56,57 @@ constexpr int kThreads = 256;
 
 inline unsigned grid_for(uint64_t n) { return unsigned((n + kThreads - 1) / kThreads); }
 
-// Every device allocation of one build; freed together. Frees wait for the
-// build stream first, so no kernel still in flight can touch freed memory.
+// Every device allocation of one build, from the device's stream-ordered
+// pool: allocations and frees are ordered on the build stream, so a free never
+// waits for the device and no kernel still in flight can see freed memory.
+// The pool keeps what a build frees for its later allocations (release
+// threshold raised) and is trimmed when the build ends.
 struct Arena {
     std::vector<void *> ptrs;
     hipStream_t stream = nullptr;
+    std::chrono::steady_clock::time_point started = std::chrono::steady_clock::now();  // device phase begins
     ~Arena() {
-        if (stream) (void)hipStreamSynchronize(stream);
         for (void *p : ptrs)
-            if (p) (void)hipFree(p);
-        if (stream) (void)hipStreamDestroy(stream);
+            if (p) (void)hipFreeAsync(p, stream);
+        if (stream) {
+            (void)hipStreamSynchronize(stream);
+            (void)hipStreamDestroy(stream);
+        }
+        hipMemPool_t pool;
+        if (default_pool(pool)) (void)hipMemPoolTrimTo(pool, 0);
     }
+    static bool default_pool(hipMemPool_t &pool) {
+        int dev = 0;
+        return hipGetDevice(&dev) == hipSuccess && hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess;
+    }
+    bool tuned = false;
     template <typename T>
     int alloc(T *&out, uint64_t n) {
+        if (!tuned) {
+            hipMemPool_t pool;
+            uint64_t keep = ~uint64_t(0);
+            if (default_pool(pool)) (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+            tuned = true;
+        }
         out = nullptr;
         void *p = nullptr;
-        hipError_t e = hipMalloc(&p, std::max<uint64_t>(n, 1) * sizeof(T));
-        if (e != hipSuccess) return fail(SVO_ERR_NOMEM, std::string("builder: hipMalloc: ") + hipGetErrorString(e));
+        hipError_t e = hipMallocAsync(&p, std::max<uint64_t>(n, 1) * sizeof(T), stream);
+        if (e != hipSuccess) return fail(SVO_ERR_NOMEM, std::string("builder: hipMallocAsync: ") + hipGetErrorString(e));
         ptrs.push_back(p);
         out = static_cast<T *>(p);
         return SVO_OK;
     }
     void release(void *p) {
-        if (stream) (void)hipStreamSynchronize(stream);
         for (auto &q : ptrs)
             if (q == p) {
-                (void)hipFree(q);
+                (void)hipFreeAsync(q, stream);
                 q = nullptr;
             }
     }
     void clear() {
-        if (stream) (void)hipStreamSynchronize(stream);
         for (void *&p : ptrs)
             if (p) {
-                (void)hipFree(p);
+                (void)hipFreeAsync(p, stream);
                 p = nullptr;
             }
     }
@@ -1265,6 +1283,9 @@ int emit_tree(Arena &ar, const Geometry &g, uint64_t *leafKey, uint32_t *leafWor
                            W, bad);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(st);
+    out.build.leaves = L;
+    out.build.words = W;
+    out.build.device_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ar.started).count();
     uint32_t badH = 0;
     if (e == hipSuccess) e = hipMemcpy(&badH, bad, 4, hipMemcpyDeviceToHost);
     if (e != hipSuccess || badH) {
@@ -1616,6 +1637,8 @@ static int mesh_core(Arena &ar, float *dv, int64_t nverts, int has_normals, int3
     if ((r = exclusive_scan(ar, hitCount, hitStart, R, st))) return r;
     uint64_t K;
     if ((r = scan_total(hitCount, hitStart, R, K))) return r;
+    out.build.triangles = uint64_t(T);
+    out.build.candidates = K;
     uint64_t *keys, *keys2;
     if ((r = ar.alloc(keys, K)) || (r = ar.alloc(keys2, K))) return r;
     if (R)
@@ -1887,6 +1910,7 @@ int build_grid(const uint32_t *grid, int w, int h, int d, const svo_build_opts *
     if ((r = exclusive_scan(ar, flag, at, N, st))) return r;
     uint64_t V;
     if ((r = scan_total(flag, at, N, V))) return r;
+    out.build.candidates = V;
     uint64_t *key, *key2;
     uint32_t *word, *word2;
     if ((r = ar.alloc(key, V)) || (r = ar.alloc(word, V)) || (r = ar.alloc(key2, V)) || (r = ar.alloc(word2, V))) return r;
@@ -2046,6 +2070,7 @@ int build_terrain(int n, uint32_t seed, const svo_build_opts *opts, svo_octree &
     uint64_t *key, V;
     uint32_t *word;
     if ((r = terrain_leaves(ar, n, seed, key, word, V))) return r;
+    out.build.candidates = V;
     uint32_t *blockFlags;
     if ((r = ar.alloc(blockFlags, nBlocks)) || (r = fill32(blockFlags, nBlocks, 0, st))) return r;
     uint64_t *lkey, L;
@@ -2105,6 +2130,7 @@ int build_sdf(const svo_sdf_primitive *prims, int count, int n, float shade, con
     uint64_t *key, V;
     uint32_t *word;
     if ((r = sdf_leaves(ar, prims, count, n, shade, key, word, V))) return r;
+    out.build.candidates = V;
     uint32_t *blockFlags;
     if ((r = ar.alloc(blockFlags, nBlocks)) || (r = fill32(blockFlags, nBlocks, 0, st))) return r;
     uint64_t *lkey, L;

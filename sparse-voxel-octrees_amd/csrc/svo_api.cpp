@@ -211,6 +211,13 @@ int svo_copy_words(const svo_octree *h, uint32_t *dst, uint64_t count) {
     return SVO_OK;
 }
 
+int svo_build_stats_get(const svo_octree *h, svo_build_stats *out) {
+    clear_error();
+    if (!h || !out) return fail(SVO_ERR_INVALID, "null argument");
+    *out = h->build;
+    return SVO_OK;
+}
+
 int svo_depth(const svo_octree *h, int *depth) {
     if (!h || !depth) return fail(SVO_ERR_INVALID, "null argument");
     *depth = h->depth;

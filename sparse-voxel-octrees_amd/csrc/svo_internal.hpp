@@ -13,6 +13,7 @@ struct svo_octree {
     int depth = 0;                // descriptor levels + 1
     int device = -1;              // device holding `dwords`
     uint32_t *dwords = nullptr;   // device copy of `words`
+    svo_build_stats build{};      // counters of the GPU build that made this tree (zeros otherwise)
 };
 
 struct svo_renderer;     // render_batch.hip

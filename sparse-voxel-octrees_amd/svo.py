@@ -34,6 +34,7 @@ EXPORTS = [
     "svo_build_from_terrain", "svo_terrain_volume", "svo_terrain_heights",
     "svo_render_shadow", "svo_render_shadow_device", "svo_decode_words",
     "svo_renderer_autotune", "svo_batch_graph_create", "svo_batch_graph_launch", "svo_batch_graph_free",
+    "svo_build_stats_get",
 ]
 
 
@@ -55,6 +56,11 @@ class View(ctypes.Structure):
 
 class BatchOpts(ctypes.Structure):
     _fields_ = [("waves_per_cu", ctypes.c_int), ("refill_threshold", ctypes.c_int), ("reserved", ctypes.c_int * 6)]
+
+
+class BuildStats(ctypes.Structure):
+    _fields_ = [("triangles", ctypes.c_uint64), ("candidates", ctypes.c_uint64), ("leaves", ctypes.c_uint64),
+                ("words", ctypes.c_uint64), ("device_ms", ctypes.c_double), ("reserved", ctypes.c_uint64 * 3)]
 
 
 class SdfPrimitive(ctypes.Structure):
@@ -182,6 +188,7 @@ def _load():
         "svo_terrain_heights": [ctypes.c_int, ctypes.c_uint32, P, P],
         "svo_decode_words": [P, U64, P, P, P],
         "svo_renderer_autotune": [P, P, ctypes.c_int, P, P, P],
+        "svo_build_stats_get": [P, P],
         "svo_batch_graph_create": [P, P, ctypes.c_int, P, P, P],
         "svo_batch_graph_launch": [P, P],
         "svo_batch_graph_free": [P],
@@ -352,6 +359,15 @@ class VoxelOctree:
             return ctypes.string_at(p, n.value)
         finally:
             lib.svo_free_buffer(p)
+
+    def build_stats(self):
+        """Counters of the GPU build that made this tree and its algorithmic
+        bytes 36*triangles + 24*candidates + 4*words."""
+        st = BuildStats()
+        _check(lib.svo_build_stats_get(self._h, ctypes.byref(st)))
+        d = {k: getattr(st, k) for k in ("triangles", "candidates", "leaves", "words", "device_ms")}
+        d["algorithmic_bytes"] = 36 * st.triangles + 24 * st.candidates + 4 * st.words
+        return d
 
     def center(self):
         c = np.zeros(3, np.float32)

@@ -361,3 +361,23 @@ def test_ply_ingest_rejects_bad_indices(svo, tmp_path):
     with pytest.raises(svo.SvoError) as e:
         svo.VoxelOctree.build_from_ply(ply, 64)
     assert e.value.code == svo.svo.ERR_FORMAT
+
+
+def test_build_stats_are_consistent(svo, tmp_path):
+    """The counters the construction roofline is computed from."""
+    v, f = svo.scenes.icosphere(12, seed=3, displacement=0.05)
+    ply = str(tmp_path / "m.ply")
+    svo.scenes.write_ply(ply, v, f)
+    print("[gpu] build stats", flush=True)
+    t = svo.VoxelOctree.build_from_ply(ply, 128, threads=8)
+    st = t.build_stats()
+    words = t.words()
+    kinds = orc.octree_stats(words)
+    assert st["triangles"] == len(f) and st["words"] == len(words) and st["leaves"] == kinds["leaves"]
+    assert st["candidates"] >= st["leaves"] > 0 and st["device_ms"] > 0
+    assert st["algorithmic_bytes"] == 36 * st["triangles"] + 24 * st["candidates"] + 4 * st["words"]
+    g = svo.VoxelOctree.build_from_terrain(64, 5)
+    gs = g.build_stats()
+    assert gs["triangles"] == 0 and gs["candidates"] == gs["leaves"] == orc.octree_stats(g.words())["leaves"]
+    loaded = svo.VoxelOctree.from_words(words, t.center())
+    assert loaded.build_stats()["words"] == 0
