@@ -1316,9 +1316,10 @@ int emit_tree(Arena &ar, const Geometry &g, uint64_t *leafKey, uint32_t *leafWor
 
 // Drops unscanned voxels, flags occupied blocks (file source), and compacts
 // to the emitted leaves. For the mesh source an unscanned voxel in any cache
-// block but the last one processed would linger in the reused block buffer
-// and alter later blocks (VoxelData reuses _bufferedData without clearing);
-// that case is refused.
+// block but the last one processed lingers in the reused block buffer and
+// alters later blocks (VoxelData reuses _bufferedData without clearing): then
+// *staleCarry is set and the caller replays the buffer (stale_buffer_leaves);
+// without staleCarry that case is refused.
 int finish_leaves(Arena &ar, const Geometry &g, uint64_t *key, uint32_t *word, uint64_t V, bool mesh,
                   uint32_t *blockFlags, uint64_t nBlocks, hipStream_t st, uint64_t *&lkey, uint32_t *&lword,
                   uint64_t &L, bool *staleCarry = nullptr) {
