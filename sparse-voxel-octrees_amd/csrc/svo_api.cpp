@@ -279,11 +279,49 @@ int svo_raymarch(svo_octree *h, const float o[3], const float d[3], float ray_sc
                  int *hit) {
     clear_error();
     if (!h || !o || !d) return fail(SVO_ERR_INVALID, "null argument");
-    uint8_t code = 0;
-    uint32_t nrm = 0;
-    float tt = 0.0f;
-    int r = svo_raymarch_batch(h, 1, o, d, nullptr, ray_scale, &code, &nrm, &tt, nullptr);
+    // One ray per call (the reference's per-pixel interface): a small device
+    // block per thread and device, reused by every call, so a call is one
+    // upload, one launch and one download. It is not freed at thread exit
+    // (64 bytes per thread; a thread-exit handler can outlive the runtime).
+    struct Block {
+        int device = -1;
+        uint8_t *p = nullptr;
+    };
+    thread_local Block blk;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess && (!blk.p || blk.device != dev)) {
+        if (blk.p) {  // the previous device's block; calls are synchronous, so nothing still uses it
+            int cur = dev;
+            if (hipSetDevice(blk.device) == hipSuccess) (void)hipFree(blk.p);
+            (void)hipSetDevice(cur);
+            blk.p = nullptr;
+        }
+        void *q = nullptr;
+        e = hipMalloc(&q, 64);
+        if (e == hipSuccess) {
+            blk.p = static_cast<uint8_t *>(q);
+            blk.device = dev;
+        }
+    }
+    if (e != hipSuccess) return fail(SVO_ERR_DEVICE, std::string("raymarch: ") + hipGetErrorString(e));
+    float in[6] = {o[0], o[1], o[2], d[0], d[1], d[2]};
+    e = hipMemcpy(blk.p, in, sizeof(in), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return fail(SVO_ERR_DEVICE, std::string("raymarch: ") + hipGetErrorString(e));
+    const float *dO = reinterpret_cast<const float *>(blk.p), *dD = dO + 3;
+    uint8_t *dHit = blk.p + 32;
+    uint32_t *dNrm = reinterpret_cast<uint32_t *>(blk.p + 36);
+    float *dT = reinterpret_cast<float *>(blk.p + 40);
+    int r = raymarch_device(*h, 1, dO, dD, nullptr, ray_scale, dHit, dNrm, dT, nullptr, nullptr);
     if (r) return r;
+    uint8_t out[12];
+    e = hipMemcpy(out, blk.p + 32, sizeof(out), hipMemcpyDeviceToHost);  // waits for the launch (null stream)
+    if (e != hipSuccess) return fail(SVO_ERR_DEVICE, std::string("raymarch: ") + hipGetErrorString(e));
+    const uint8_t code = out[0];
+    uint32_t nrm;
+    float tt;
+    std::memcpy(&nrm, out + 4, 4);
+    std::memcpy(&tt, out + 8, 4);
     if (hit) *hit = code;
     if (code == 1 && normal) *normal = nrm;
     if (code != 0 && t) *t = tt;
