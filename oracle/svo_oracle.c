@@ -1402,6 +1402,8 @@ int oracle_build_mesh(oracle_mesh *m, int resolution, int threads, uint64_t mem,
 int oracle_build_grid(const uint32_t *grid, int w, int h, int d, uint64_t mem, uint32_t **words,
                       uint64_t *nwords, float center[3], oracle_build_info *info) {
     vctx c;
+    /* a one-voxel volume (octree side 1): the reference builder crashes on it */
+    if (w < 2 && h < 2 && d < 2) return -2;
     memset(&c, 0, sizeof(c));
     c.grid = grid;
     c.dataW = w; c.dataH = h; c.dataD = d;

@@ -153,6 +153,9 @@ uint64_t grid_cells(int levels) {
 int choose_cache(Geometry &g, uint64_t mem, bool mesh) {
     int vW = round_pow2(g.dataW), vH = round_pow2(g.dataH), vD = round_pow2(g.dataD);
     g.side = std::max(vW, std::max(vH, vD));
+    // side 1 (a one-voxel volume) gives an octree of depth 0, on which the
+    // reference builder crashes; there is no output to reproduce
+    if (g.side < 2) return fail(SVO_ERR_UNSUPPORTED, "a one-voxel volume has no octree (the reference builder crashes)");
     g.depth = log2i(uint32_t(g.side));
     uint64_t cellCost = 4 + (mesh ? 1 : 0);
     int best = -1;

@@ -35,7 +35,10 @@ Fixtures written (all small):
                          the reference's file-source build of the fBm terrain
                          volume (scenes.terrain_volume) at 128^3 and 256^3
 
-`--only frames`, `--only ico2048` or `--only terrain` regenerates just that part;
+  grid_edges.npz + build_grid_{empty16,one16,slice537,full4}.oct + builds.json["grid_edges"]
+                         edge volumes through the reference's file-source builder
+
+`--only frames|ico2048|terrain|edges` regenerates just that part;
 other entries of builds.json are kept.
 """
 import argparse
@@ -286,6 +289,38 @@ BUILD_CASES = [
     ("flat20", 256, 16, 12000000),
 ]
 
+# edge volumes: no voxel, one voxel in a corner, a voxel only on the unscanned
+# last slice of an odd depth, every voxel set
+EDGE_GRIDS = [("empty16", (16, 16, 16)), ("one16", (16, 16, 16)), ("slice537", (5, 3, 7)), ("full4", (4, 4, 4))]
+
+
+def edge_builds(tmp):
+    """Reference file-source builds of the edge volumes."""
+    out, vols = [], {}
+    for name, dims in EDGE_GRIDS:
+        vol = edge_volume(name, dims)
+        vols[name] = vol
+        vp = os.path.join(tmp, f"{name}.voxel")
+        scenes.write_voxel(vp, vol)
+        op = os.path.join(HERE, f"build_grid_{name}.oct")
+        run_ref("build-voxel", vp, 4, 1 << 30, op)
+        out.append({"name": name, "dims": list(dims), "oct": os.path.basename(op), "md5": md5(op)})
+    np.savez_compressed(os.path.join(HERE, "grid_edges.npz"), **vols)
+    return out
+
+
+def edge_volume(name, dims):
+    w, h, d = dims
+    vol = np.zeros((d, h, w), np.uint32)
+    if name == "one16":
+        vol[15, 0, 3] = 0x12345
+    elif name == "slice537":
+        vol[6, 2, 4] = 3
+    elif name == "full4":
+        vol[:] = 9
+    return vol
+
+
 GRID_CASES = [
     # name, dims (w,h,d), seed, density, memory budget
     ("g45", (45, 37, 29), 1, 0.02, 30000),
@@ -297,7 +332,7 @@ GRID_CASES = [
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--reference", default="/root/reference")
-    ap.add_argument("--only", choices=["frames", "ico2048", "terrain"], default=None)
+    ap.add_argument("--only", choices=["frames", "ico2048", "terrain", "edges"], default=None)
     args = ap.parse_args()
     if not os.path.exists(REF_BIN):
         subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref", f"REF={args.reference}"], check=True)
@@ -307,13 +342,15 @@ def main():
         shutil.copyfile(os.path.join(args.reference, "models", "XYZRGB-Dragon.oct"), dragon)
 
         bpath = os.path.join(HERE, "builds.json")
-        if args.only in ("ico2048", "terrain"):
+        if args.only in ("ico2048", "terrain", "edges"):
             with open(bpath) as fp:
                 builds = json.load(fp)
             if args.only == "ico2048":
                 builds["ico2048"] = full_size_build(tmp)
-            else:
+            elif args.only == "terrain":
                 builds["terrain"] = terrain_builds(tmp)
+            else:
+                builds["grid_edges"] = edge_builds(tmp)
             with open(bpath, "w") as fp:
                 json.dump(builds, fp, indent=1)
             return
@@ -421,6 +458,7 @@ def main():
         scenes.write_voxel(vp, vol)
         op = os.path.join(HERE, "build_sdf128.oct")
         run_ref("build-voxel", vp, 8, 1 << 30, op)
+        builds["grid_edges"] = edge_builds(tmp)
         builds["sdf128"] = {"seed": 5, "n": 128, "mem": 1 << 30, "md5": md5(op),
                             "volume_md5": hashlib.md5(vol.tobytes()).hexdigest()}
         raymarch_fixture("sdf128", op, 12, tmp, n_random=4000, cams=3, w=80, h=45)

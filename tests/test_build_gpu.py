@@ -49,6 +49,20 @@ def test_grid_builds_match_reference(svo, golden):
         assert t.encode() == raw, b["name"]
 
 
+def test_edge_volumes_match_reference(svo, golden):
+    vols = np.load(os.path.join(golden, "grid_edges.npz"))
+    for b in _builds(golden)["grid_edges"]:
+        print(f"[gpu] edge volume {b['name']}", flush=True)
+        t = svo.VoxelOctree.build_from_grid(vols[b["name"]], memory_budget=1 << 30)
+        assert t.encode() == open(os.path.join(golden, b["oct"]), "rb").read(), b["name"]
+    empty = svo.VoxelOctree.build_from_grid(vols["empty16"])
+    r = empty.render_primary(empty.orbit_camera(20.0, 30.0, 1.0), 64, 48)
+    assert not r["hit"].any()
+    with pytest.raises(svo.SvoError) as e:
+        svo.VoxelOctree.build_from_grid(np.full((1, 1, 1), 7, np.uint32))
+    assert e.value.code == svo.svo.ERR_UNSUPPORTED
+
+
 def test_sdf_scene_build_matches_reference(svo, golden):
     b = _builds(golden)["sdf128"]
     print("[gpu] sdf128 grid build", flush=True)

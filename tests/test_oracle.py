@@ -118,6 +118,20 @@ def test_frame_invariants(golden):
     assert 0 < (d1 < f(1e10)).mean() < 1
 
 
+def test_edge_volumes_match_reference(golden):
+    """No voxel, one voxel, a voxel only on the unscanned last slice of an odd
+    depth, a full volume; a one-voxel volume (on which the reference crashes)
+    is refused."""
+    vols = np.load(os.path.join(golden, "grid_edges.npz"))
+    for b in _builds(golden)["grid_edges"]:
+        c, words, _ = orc.build_grid(vols[b["name"]], 1 << 30)
+        rc, rwords = orc.read_oct(os.path.join(golden, b["oct"]))
+        assert np.array_equal(words, rwords) and _eq_bits(c, rc), b["name"]
+    assert len(orc.read_oct(os.path.join(golden, "build_grid_empty16.oct"))[1]) == 1
+    with pytest.raises(RuntimeError):
+        orc.build_grid(np.full((1, 1, 1), 7, np.uint32))
+
+
 def test_terrain_builds_match_reference(golden, svo):
     """fBm terrain volumes (exposed shell) through the oracle's file-source
     builder equal the reference builder's .oct for the same volume."""
