@@ -100,7 +100,10 @@ int svo_upload(svo_octree *h, const uint32_t **device_words);
 /* VoxelOctree::raymarch(o, d, rayScale, normal, t), src/VoxelOctree.cpp:207-346,
  * one ray, synchronous, on the GPU. *hit gets the hit code; normal/t are
  * written exactly when the reference writes them (normal on a leaf hit; t on
- * any hit). */
+ * any hit). For every raymarch entry point: a ray with a NaN origin or
+ * direction component, or an infinite origin on an axis whose direction is
+ * infinite, is a miss (the reference's loop never ends on it); every other
+ * ray, infinite and huge components included, gets the reference's result. */
 int svo_raymarch(svo_octree *h, const float o[3], const float d[3], float ray_scale,
                  uint32_t *normal, float *t, int *hit);
 
@@ -161,6 +164,8 @@ int svo_render_frame(svo_octree *h, const svo_camera *cam, int width, int height
                      uint32_t *rgba, float *depth);
 int svo_render_frame_device(svo_octree *h, const svo_camera *cam, int width, int height, int stripes,
                             int stride, uint32_t *rgba, float *depth, void *stream);
+/* Coarse depth entries of a frame, or -1 (message in svo_last_error) when the
+ * size or stripe split is refused. */
 int64_t svo_frame_depth_count(int width, int height, int stripes);
 
 /* Counters of the GPU build that produced a tree (all zero for a loaded tree),

@@ -228,6 +228,12 @@ int oracle_raymarch(const uint32_t *tree, uint64_t nwords, const float o[3], con
     if (dy > 0.0f) { octantMask ^= 2; bTy = 3.0f * dTy - bTy; }
     if (dz > 0.0f) { octantMask ^= 4; bTz = 3.0f * dTz - bTz; }
 
+    /* A NaN coefficient (NaN origin or direction, or -0 * inf) makes every
+     * corner time NaN; no axis ever steps and the reference's loop never ends.
+     * Such rays are reported as misses, as the product's generic ray path
+     * reports them. */
+    if (dTx != dTx || dTy != dTy || dTz != dTz || bTx != bTx || bTy != bTy || bTz != bTz) return 0;
+
     float minT = smax(2.0f * dTx - bTx, smax(2.0f * dTy - bTy, 2.0f * dTz - bTz));
     float maxT = smin(dTx - bTx, smin(dTy - bTy, dTz - bTz));
     minT = smax(minT, 0.0f);

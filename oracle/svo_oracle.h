@@ -39,7 +39,9 @@ void oracle_tribox_batch(int64_t n, const float *rec, int32_t *out);
 
 /* ---- raymarch (src/VoxelOctree.cpp:207-346) ------------------------------- */
 /* counts (may be NULL) accumulates {descriptor, far, leaf} word reads. Returns
- * the hit code, or -1 if the walk leaves the word array (malformed tree). */
+ * the hit code, or -1 if the walk leaves the word array (malformed tree). A ray
+ * with a NaN coefficient (NaN component, or an infinite origin on an axis whose
+ * direction is infinite) returns 0: the reference never ends it. */
 int oracle_raymarch(const uint32_t *tree, uint64_t nwords, const float o[3], const float d[3],
                     float rayScale, uint32_t *normal, float *t, uint64_t *leaf, uint64_t counts[3]);
 

@@ -164,18 +164,30 @@ __global__ void __launch_bounds__(64) fine_kernel(const uint32_t *__restrict__ t
 
 }  // namespace
 
-int render_frame_device(svo_octree &h, const svo_camera &cam, int width, int height, int stripes, int stride,
-                        uint32_t *rgba, float *depthOut, void *stream) {
+namespace {
+
+// Image size and stripe split checks shared by the renderer and the depth
+// count; stripes 0 selects the viewer's 16 (NumThreads, src/Main.cpp:57).
+int frame_layout(int width, int height, int &stripes) {
     if (width <= 0 || height <= 0 || width > 32768 || height > 32768) return fail(SVO_ERR_INVALID, "image size out of range");
-    if (stripes <= 0) stripes = 16;  // NumThreads, src/Main.cpp:57
-    if (stride <= 0 || stride > 8) return fail(SVO_ERR_INVALID, "stride must be in [1, 8]");
+    if (stripes <= 0) stripes = 16;
     // the viewer's split (rows per stripe rounded up) must leave every stripe non-empty; where it does
     // not, the reference clears a negative number of rows (src/Main.cpp:165)
     if (int64_t(stripes - 1) * ((height - 1) / stripes + 1) >= height)
         return fail(SVO_ERR_INVALID, "stripe split leaves an empty stripe at this height");
+    return SVO_OK;
+}
+
+}  // namespace
+
+int render_frame_device(svo_octree &h, const svo_camera &cam, int width, int height, int stripes, int stride,
+                        uint32_t *rgba, float *depthOut, void *stream) {
+    int r = frame_layout(width, height, stripes);
+    if (r) return r;
+    if (stride <= 0 || stride > 8) return fail(SVO_ERR_INVALID, "stride must be in [1, 8]");
     if (!rgba) return fail(SVO_ERR_INVALID, "null image");
     if (!camera_ok(cam)) return fail(SVO_ERR_INVALID, "camera entries must be finite and within +-1e6");
-    int r = ensure_device(h);
+    r = ensure_device(h);
     if (r) return r;
     const float pi = float(3.14159265358979323846);
     FrameParams p;
@@ -276,7 +288,7 @@ int render_frame_device(svo_octree &h, const svo_camera &cam, int width, int hei
 }
 
 int64_t frame_depth_count(int width, int height, int stripes) {
-    if (stripes <= 0) stripes = 16;
+    if (frame_layout(width, height, stripes)) return -1;
     const int sh = (height - 1) / stripes + 1;
     int64_t n = 0;
     for (int i = 0; i < stripes; ++i) {

@@ -118,6 +118,18 @@ __device__ __forceinline__ void ray_init(Ray<Idx> &r, const uint32_t *__restrict
     if (1.5f * r.dTz - r.bTz > minT) { r.idx ^= 4; r.posZ = 1.5f; }
 }
 
+// False when a ray coefficient is NaN (a NaN origin or direction component, or
+// an infinite origin on an axis whose direction is infinite: -0 * inf). Every
+// corner time is then NaN, no axis ever steps and the reference's loop does not
+// end. For all other rays some axis steps on every advance (the minimum corner
+// time equals at least one corner time), each node is entered at most once and
+// the walk ends.
+template <typename Idx>
+__device__ __forceinline__ bool ray_defined(const Ray<Idx> &r) {
+    return r.dTx == r.dTx && r.dTy == r.dTy && r.dTz == r.dTz && r.bTx == r.bTx && r.bTy == r.bTy &&
+           r.bTz == r.bTz;
+}
+
 // One pass of the loop body. Returns kRunning, or the outcome with tOut /
 // normal / leaf set as the reference sets them. The descriptor of the node
 // the ray is in (`current`) is read as soon as its index is known (init,

@@ -38,6 +38,7 @@ __device__ __forceinline__ int march_ray(const uint32_t *__restrict__ tree, floa
                                          float &tOut, Idx &leaf) {
     Ray<Idx> r;
     ray_init(r, tree, ox, oy, oz, dx, dy, dz, rayScale);
+    if (!Finite && !ray_defined(r)) return kMiss;  // the reference never ends such a ray
     int code;
     do {
         code = ray_step<Lod, Finite>(r, tree, stk, normal, tOut, leaf);
@@ -265,6 +266,8 @@ int render_shadow_device(svo_octree &h, const svo_camera &cam, int width, int he
         if (!(std::fabs(light[k]) <= 1e6f)) return fail(SVO_ERR_INVALID, "light direction must be finite");
     if (!(std::fabs(light[0]) + std::fabs(light[1]) + std::fabs(light[2]) > 0.0f))
         return fail(SVO_ERR_INVALID, "light direction must be non-zero");
+    // shadow origins o + d*(t - eps) must stay finite (the shadow traversal assumes finite rays)
+    if (!(std::fabs(eps) <= 1e6f)) return fail(SVO_ERR_INVALID, "eps must be finite and within +-1e6");
     int r = render_primary_device(h, cam, width, height, hit, normal, t, leaf, stream);
     if (r) return r;
     const PrimaryParams p = primary_params(cam, width, height);

@@ -9,6 +9,7 @@
 #include <cstring>
 #include <memory>
 #include <new>
+#include <stdexcept>
 
 #include "svo_internal.hpp"
 
@@ -90,6 +91,23 @@ int from_device(T *dst, const DevBuf<T> &b, size_t n) {
     return e == hipSuccess ? SVO_OK : fail(SVO_ERR_DEVICE, std::string("hipMemcpy: ") + hipGetErrorString(e));
 }
 
+// Runs an entry point's body; a C++ exception (host allocation failure) becomes
+// an error code instead of crossing the C boundary.
+template <typename F>
+int guarded(F &&body) {
+    try {
+        return body();
+    } catch (const std::bad_alloc &) {
+        return fail(SVO_ERR_NOMEM, "out of host memory");
+    } catch (const std::length_error &) {
+        return fail(SVO_ERR_NOMEM, "out of host memory");
+    } catch (const std::exception &e) {
+        return fail(SVO_ERR_INVALID, std::string("internal error: ") + e.what());
+    } catch (...) {
+        return fail(SVO_ERR_INVALID, "internal error");
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -101,127 +119,151 @@ int svo_version(void) { return 100; }
 int svo_device_count(void) { return device_count(); }
 
 int svo_load_memory(const void *data, uint64_t size, svo_octree **out) {
-    clear_error();
-    if (!data || !out) return fail(SVO_ERR_INVALID, "null argument");
-    *out = nullptr;
-    std::unique_ptr<svo_octree> h(new (std::nothrow) svo_octree());
-    if (!h) return fail(SVO_ERR_NOMEM, "out of host memory");
-    int r = oct_decode(static_cast<const uint8_t *>(data), size, *h);
-    if (r) return r;
-    return finish_handle(h, out);
+    return guarded([&]() -> int {
+        clear_error();
+        if (!data || !out) return fail(SVO_ERR_INVALID, "null argument");
+        *out = nullptr;
+        std::unique_ptr<svo_octree> h(new (std::nothrow) svo_octree());
+        if (!h) return fail(SVO_ERR_NOMEM, "out of host memory");
+        int r = oct_decode(static_cast<const uint8_t *>(data), size, *h);
+        if (r) return r;
+        return finish_handle(h, out);
+    });
 }
 
 int svo_load(const char *path, svo_octree **out) {
-    clear_error();
-    if (!path || !out) return fail(SVO_ERR_INVALID, "null argument");
-    *out = nullptr;
-    std::vector<uint8_t> buf;
-    if (!read_file(path, buf)) return fail(SVO_ERR_IO, std::string("cannot read ") + path);
-    return svo_load_memory(buf.data(), buf.size(), out);
+    return guarded([&]() -> int {
+        clear_error();
+        if (!path || !out) return fail(SVO_ERR_INVALID, "null argument");
+        *out = nullptr;
+        std::vector<uint8_t> buf;
+        if (!read_file(path, buf)) return fail(SVO_ERR_IO, std::string("cannot read ") + path);
+        return svo_load_memory(buf.data(), buf.size(), out);
+    });
 }
 
 int svo_encode(const svo_octree *h, void **data, uint64_t *size) {
-    clear_error();
-    if (!h || !data || !size) return fail(SVO_ERR_INVALID, "null argument");
-    std::vector<uint8_t> buf;
-    oct_encode(*h, buf);
-    void *p = std::malloc(buf.size());
-    if (!p) return fail(SVO_ERR_NOMEM, "out of host memory");
-    std::memcpy(p, buf.data(), buf.size());
-    *data = p;
-    *size = buf.size();
-    return SVO_OK;
+    return guarded([&]() -> int {
+        clear_error();
+        if (!h || !data || !size) return fail(SVO_ERR_INVALID, "null argument");
+        std::vector<uint8_t> buf;
+        oct_encode(*h, buf);
+        void *p = std::malloc(buf.size());
+        if (!p) return fail(SVO_ERR_NOMEM, "out of host memory");
+        std::memcpy(p, buf.data(), buf.size());
+        *data = p;
+        *size = buf.size();
+        return SVO_OK;
+    });
 }
 
 int svo_encode_words(const uint32_t *words, uint64_t count, const float center[3], void **data, uint64_t *size) {
-    clear_error();
-    if (!words || !center || !data || !size || count == 0) return fail(SVO_ERR_INVALID, "null or empty argument");
-    svo_octree tmp;
-    try {
-        tmp.words.assign(words, words + count);
-    } catch (...) {
-        return fail(SVO_ERR_NOMEM, "out of host memory");
-    }
-    std::memcpy(tmp.center, center, sizeof(tmp.center));
-    return svo_encode(&tmp, data, size);
+    return guarded([&]() -> int {
+        clear_error();
+        if (!words || !center || !data || !size || count == 0) return fail(SVO_ERR_INVALID, "null or empty argument");
+        svo_octree tmp;
+        try {
+            tmp.words.assign(words, words + count);
+        } catch (...) {
+            return fail(SVO_ERR_NOMEM, "out of host memory");
+        }
+        std::memcpy(tmp.center, center, sizeof(tmp.center));
+        return svo_encode(&tmp, data, size);
+    });
 }
 
 void svo_free_buffer(void *p) { std::free(p); }
 
 int svo_decode_words(const void *data, uint64_t size, uint32_t **words, uint64_t *count, float center[3]) {
-    clear_error();
-    if (!data || !words || !count || !center) return fail(SVO_ERR_INVALID, "null argument");
-    *words = nullptr;
-    *count = 0;
-    svo_octree tmp;
-    int r = oct_decode(static_cast<const uint8_t *>(data), size, tmp);
-    if (r) return r;
-    uint32_t *p = static_cast<uint32_t *>(std::malloc(tmp.words.size() * sizeof(uint32_t)));
-    if (!p) return fail(SVO_ERR_NOMEM, "out of host memory");
-    std::memcpy(p, tmp.words.data(), tmp.words.size() * sizeof(uint32_t));
-    std::memcpy(center, tmp.center, sizeof(tmp.center));
-    *words = p;
-    *count = tmp.words.size();
-    return SVO_OK;
+    return guarded([&]() -> int {
+        clear_error();
+        if (!data || !words || !count || !center) return fail(SVO_ERR_INVALID, "null argument");
+        *words = nullptr;
+        *count = 0;
+        svo_octree tmp;
+        int r = oct_decode(static_cast<const uint8_t *>(data), size, tmp);
+        if (r) return r;
+        uint32_t *p = static_cast<uint32_t *>(std::malloc(tmp.words.size() * sizeof(uint32_t)));
+        if (!p) return fail(SVO_ERR_NOMEM, "out of host memory");
+        std::memcpy(p, tmp.words.data(), tmp.words.size() * sizeof(uint32_t));
+        std::memcpy(center, tmp.center, sizeof(tmp.center));
+        *words = p;
+        *count = tmp.words.size();
+        return SVO_OK;
+    });
 }
 
 int svo_save(const svo_octree *h, const char *path) {
-    clear_error();
-    if (!h || !path) return fail(SVO_ERR_INVALID, "null argument");
-    std::vector<uint8_t> buf;
-    oct_encode(*h, buf);
-    FILE *fp = std::fopen(path, "wb");
-    if (!fp) return fail(SVO_ERR_IO, std::string("cannot write ") + path);
-    bool ok = std::fwrite(buf.data(), 1, buf.size(), fp) == buf.size();
-    ok = (std::fclose(fp) == 0) && ok;
-    return ok ? SVO_OK : fail(SVO_ERR_IO, std::string("short write to ") + path);
+    return guarded([&]() -> int {
+        clear_error();
+        if (!h || !path) return fail(SVO_ERR_INVALID, "null argument");
+        std::vector<uint8_t> buf;
+        oct_encode(*h, buf);
+        FILE *fp = std::fopen(path, "wb");
+        if (!fp) return fail(SVO_ERR_IO, std::string("cannot write ") + path);
+        bool ok = std::fwrite(buf.data(), 1, buf.size(), fp) == buf.size();
+        ok = (std::fclose(fp) == 0) && ok;
+        return ok ? SVO_OK : fail(SVO_ERR_IO, std::string("short write to ") + path);
+    });
 }
 
 int svo_from_words(const uint32_t *words, uint64_t count, const float center[3], svo_octree **out) {
-    clear_error();
-    if (!words || !center || !out || count == 0) return fail(SVO_ERR_INVALID, "null or empty argument");
-    *out = nullptr;
-    std::unique_ptr<svo_octree> h(new (std::nothrow) svo_octree());
-    if (!h) return fail(SVO_ERR_NOMEM, "out of host memory");
-    try {
-        h->words.assign(words, words + count);
-    } catch (...) {
-        return fail(SVO_ERR_NOMEM, "out of host memory");
-    }
-    std::memcpy(h->center, center, sizeof(h->center));
-    return finish_handle(h, out);
+    return guarded([&]() -> int {
+        clear_error();
+        if (!words || !center || !out || count == 0) return fail(SVO_ERR_INVALID, "null or empty argument");
+        *out = nullptr;
+        std::unique_ptr<svo_octree> h(new (std::nothrow) svo_octree());
+        if (!h) return fail(SVO_ERR_NOMEM, "out of host memory");
+        try {
+            h->words.assign(words, words + count);
+        } catch (...) {
+            return fail(SVO_ERR_NOMEM, "out of host memory");
+        }
+        std::memcpy(h->center, center, sizeof(h->center));
+        return finish_handle(h, out);
+    });
 }
 
 int svo_center(const svo_octree *h, float out[3]) {
-    if (!h || !out) return fail(SVO_ERR_INVALID, "null argument");
-    std::memcpy(out, h->center, sizeof(h->center));
-    return SVO_OK;
+    return guarded([&]() -> int {
+        if (!h || !out) return fail(SVO_ERR_INVALID, "null argument");
+        std::memcpy(out, h->center, sizeof(h->center));
+        return SVO_OK;
+    });
 }
 
 int svo_word_count(const svo_octree *h, uint64_t *count) {
-    if (!h || !count) return fail(SVO_ERR_INVALID, "null argument");
-    *count = h->words.size();
-    return SVO_OK;
+    return guarded([&]() -> int {
+        if (!h || !count) return fail(SVO_ERR_INVALID, "null argument");
+        *count = h->words.size();
+        return SVO_OK;
+    });
 }
 
 int svo_copy_words(const svo_octree *h, uint32_t *dst, uint64_t count) {
-    if (!h || !dst) return fail(SVO_ERR_INVALID, "null argument");
-    uint64_t n = count < h->words.size() ? count : h->words.size();
-    std::memcpy(dst, h->words.data(), n * sizeof(uint32_t));
-    return SVO_OK;
+    return guarded([&]() -> int {
+        if (!h || !dst) return fail(SVO_ERR_INVALID, "null argument");
+        uint64_t n = count < h->words.size() ? count : h->words.size();
+        std::memcpy(dst, h->words.data(), n * sizeof(uint32_t));
+        return SVO_OK;
+    });
 }
 
 int svo_build_stats_get(const svo_octree *h, svo_build_stats *out) {
-    clear_error();
-    if (!h || !out) return fail(SVO_ERR_INVALID, "null argument");
-    *out = h->build;
-    return SVO_OK;
+    return guarded([&]() -> int {
+        clear_error();
+        if (!h || !out) return fail(SVO_ERR_INVALID, "null argument");
+        *out = h->build;
+        return SVO_OK;
+    });
 }
 
 int svo_depth(const svo_octree *h, int *depth) {
-    if (!h || !depth) return fail(SVO_ERR_INVALID, "null argument");
-    *depth = h->depth;
-    return SVO_OK;
+    return guarded([&]() -> int {
+        if (!h || !depth) return fail(SVO_ERR_INVALID, "null argument");
+        *depth = h->depth;
+        return SVO_OK;
+    });
 }
 
 void svo_free(svo_octree *h) {
@@ -231,12 +273,14 @@ void svo_free(svo_octree *h) {
 }
 
 int svo_upload(svo_octree *h, const uint32_t **device_words) {
-    clear_error();
-    if (!h) return fail(SVO_ERR_INVALID, "null handle");
-    int r = ensure_device(*h);
-    if (r) return r;
-    if (device_words) *device_words = h->dwords;
-    return SVO_OK;
+    return guarded([&]() -> int {
+        clear_error();
+        if (!h) return fail(SVO_ERR_INVALID, "null handle");
+        int r = ensure_device(*h);
+        if (r) return r;
+        if (device_words) *device_words = h->dwords;
+        return SVO_OK;
+    });
 }
 
 // ---- raymarch ----
@@ -244,219 +288,253 @@ int svo_upload(svo_octree *h, const uint32_t **device_words) {
 int svo_raymarch_batch_device(svo_octree *h, int64_t n, const float *o, const float *d, const float *ray_scale,
                               float ray_scale0, uint8_t *hit, uint32_t *normal, float *t, uint64_t *leaf,
                               void *stream) {
-    clear_error();
-    if (!h) return fail(SVO_ERR_INVALID, "null handle");
-    return raymarch_device(*h, n, o, d, ray_scale, ray_scale0, hit, normal, t, leaf, stream);
+    return guarded([&]() -> int {
+        clear_error();
+        if (!h) return fail(SVO_ERR_INVALID, "null handle");
+        return raymarch_device(*h, n, o, d, ray_scale, ray_scale0, hit, normal, t, leaf, stream);
+    });
 }
 
 int svo_raymarch_batch(svo_octree *h, int64_t n, const float *o, const float *d, const float *ray_scale,
                        float ray_scale0, uint8_t *hit, uint32_t *normal, float *t, uint64_t *leaf) {
-    clear_error();
-    if (!h) return fail(SVO_ERR_INVALID, "null handle");
-    if (n < 0) return fail(SVO_ERR_INVALID, "negative ray count");
-    if (n == 0) return SVO_OK;
-    if (!o || !d) return fail(SVO_ERR_INVALID, "null ray arrays");
-    size_t m = size_t(n);
-    DevBuf<float> dO, dD, dS, dT;
-    DevBuf<uint8_t> dH;
-    DevBuf<uint32_t> dN;
-    DevBuf<uint64_t> dL;
-    int r;
-    if ((r = to_device(dO, o, 3 * m)) || (r = to_device(dD, d, 3 * m)) || (r = to_device(dS, ray_scale, m))) return r;
-    if ((r = out_buf(dH, hit, m)) || (r = out_buf(dN, normal, m)) || (r = out_buf(dT, t, m)) || (r = out_buf(dL, leaf, m)))
-        return r;
-    r = raymarch_device(*h, n, dO.p, dD.p, ray_scale ? dS.p : nullptr, ray_scale0, dH.p, dN.p, dT.p, dL.p, nullptr);
-    if (r) return r;
-    hipError_t e = hipDeviceSynchronize();
-    if (e != hipSuccess) return fail(SVO_ERR_DEVICE, std::string("raymarch: ") + hipGetErrorString(e));
-    if ((r = from_device(hit, dH, m)) || (r = from_device(normal, dN, m)) || (r = from_device(t, dT, m)) ||
-        (r = from_device(leaf, dL, m)))
-        return r;
-    return SVO_OK;
+    return guarded([&]() -> int {
+        clear_error();
+        if (!h) return fail(SVO_ERR_INVALID, "null handle");
+        if (n < 0) return fail(SVO_ERR_INVALID, "negative ray count");
+        if (n == 0) return SVO_OK;
+        if (!o || !d) return fail(SVO_ERR_INVALID, "null ray arrays");
+        size_t m = size_t(n);
+        DevBuf<float> dO, dD, dS, dT;
+        DevBuf<uint8_t> dH;
+        DevBuf<uint32_t> dN;
+        DevBuf<uint64_t> dL;
+        int r;
+        if ((r = to_device(dO, o, 3 * m)) || (r = to_device(dD, d, 3 * m)) || (r = to_device(dS, ray_scale, m))) return r;
+        if ((r = out_buf(dH, hit, m)) || (r = out_buf(dN, normal, m)) || (r = out_buf(dT, t, m)) || (r = out_buf(dL, leaf, m)))
+            return r;
+        r = raymarch_device(*h, n, dO.p, dD.p, ray_scale ? dS.p : nullptr, ray_scale0, dH.p, dN.p, dT.p, dL.p, nullptr);
+        if (r) return r;
+        hipError_t e = hipDeviceSynchronize();
+        if (e != hipSuccess) return fail(SVO_ERR_DEVICE, std::string("raymarch: ") + hipGetErrorString(e));
+        if ((r = from_device(hit, dH, m)) || (r = from_device(normal, dN, m)) || (r = from_device(t, dT, m)) ||
+            (r = from_device(leaf, dL, m)))
+            return r;
+        return SVO_OK;
+    });
 }
 
 int svo_raymarch(svo_octree *h, const float o[3], const float d[3], float ray_scale, uint32_t *normal, float *t,
                  int *hit) {
-    clear_error();
-    if (!h || !o || !d) return fail(SVO_ERR_INVALID, "null argument");
-    // One ray per call (the reference's per-pixel interface): a small device
-    // block per thread and device, reused by every call, so a call is one
-    // upload, one launch and one download. It is not freed at thread exit
-    // (64 bytes per thread; a thread-exit handler can outlive the runtime).
-    struct Block {
-        int device = -1;
-        uint8_t *p = nullptr;
-    };
-    thread_local Block blk;
-    int dev = 0;
-    hipError_t e = hipGetDevice(&dev);
-    if (e == hipSuccess && (!blk.p || blk.device != dev)) {
-        if (blk.p) {  // the previous device's block; calls are synchronous, so nothing still uses it
-            int cur = dev;
-            if (hipSetDevice(blk.device) == hipSuccess) (void)hipFree(blk.p);
-            (void)hipSetDevice(cur);
-            blk.p = nullptr;
+    return guarded([&]() -> int {
+        clear_error();
+        if (!h || !o || !d) return fail(SVO_ERR_INVALID, "null argument");
+        // One ray per call (the reference's per-pixel interface): a small device
+        // block per thread and device, reused by every call, so a call is one
+        // upload, one launch and one download. It is not freed at thread exit
+        // (64 bytes per thread; a thread-exit handler can outlive the runtime).
+        struct Block {
+            int device = -1;
+            uint8_t *p = nullptr;
+        };
+        thread_local Block blk;
+        int dev = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess && (!blk.p || blk.device != dev)) {
+            if (blk.p) {  // the previous device's block; calls are synchronous, so nothing still uses it
+                int cur = dev;
+                if (hipSetDevice(blk.device) == hipSuccess) (void)hipFree(blk.p);
+                (void)hipSetDevice(cur);
+                blk.p = nullptr;
+            }
+            void *q = nullptr;
+            e = hipMalloc(&q, 64);
+            if (e == hipSuccess) {
+                blk.p = static_cast<uint8_t *>(q);
+                blk.device = dev;
+            }
         }
-        void *q = nullptr;
-        e = hipMalloc(&q, 64);
-        if (e == hipSuccess) {
-            blk.p = static_cast<uint8_t *>(q);
-            blk.device = dev;
-        }
-    }
-    if (e != hipSuccess) return fail(SVO_ERR_DEVICE, std::string("raymarch: ") + hipGetErrorString(e));
-    float in[6] = {o[0], o[1], o[2], d[0], d[1], d[2]};
-    e = hipMemcpy(blk.p, in, sizeof(in), hipMemcpyHostToDevice);
-    if (e != hipSuccess) return fail(SVO_ERR_DEVICE, std::string("raymarch: ") + hipGetErrorString(e));
-    const float *dO = reinterpret_cast<const float *>(blk.p), *dD = dO + 3;
-    uint8_t *dHit = blk.p + 32;
-    uint32_t *dNrm = reinterpret_cast<uint32_t *>(blk.p + 36);
-    float *dT = reinterpret_cast<float *>(blk.p + 40);
-    int r = raymarch_device(*h, 1, dO, dD, nullptr, ray_scale, dHit, dNrm, dT, nullptr, nullptr);
-    if (r) return r;
-    uint8_t out[12];
-    e = hipMemcpy(out, blk.p + 32, sizeof(out), hipMemcpyDeviceToHost);  // waits for the launch (null stream)
-    if (e != hipSuccess) return fail(SVO_ERR_DEVICE, std::string("raymarch: ") + hipGetErrorString(e));
-    const uint8_t code = out[0];
-    uint32_t nrm;
-    float tt;
-    std::memcpy(&nrm, out + 4, 4);
-    std::memcpy(&tt, out + 8, 4);
-    if (hit) *hit = code;
-    if (code == 1 && normal) *normal = nrm;
-    if (code != 0 && t) *t = tt;
-    return SVO_OK;
+        if (e != hipSuccess) return fail(SVO_ERR_DEVICE, std::string("raymarch: ") + hipGetErrorString(e));
+        float in[6] = {o[0], o[1], o[2], d[0], d[1], d[2]};
+        e = hipMemcpy(blk.p, in, sizeof(in), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return fail(SVO_ERR_DEVICE, std::string("raymarch: ") + hipGetErrorString(e));
+        const float *dO = reinterpret_cast<const float *>(blk.p), *dD = dO + 3;
+        uint8_t *dHit = blk.p + 32;
+        uint32_t *dNrm = reinterpret_cast<uint32_t *>(blk.p + 36);
+        float *dT = reinterpret_cast<float *>(blk.p + 40);
+        int r = raymarch_device(*h, 1, dO, dD, nullptr, ray_scale, dHit, dNrm, dT, nullptr, nullptr);
+        if (r) return r;
+        uint8_t out[12];
+        e = hipMemcpy(out, blk.p + 32, sizeof(out), hipMemcpyDeviceToHost);  // waits for the launch (null stream)
+        if (e != hipSuccess) return fail(SVO_ERR_DEVICE, std::string("raymarch: ") + hipGetErrorString(e));
+        const uint8_t code = out[0];
+        uint32_t nrm;
+        float tt;
+        std::memcpy(&nrm, out + 4, 4);
+        std::memcpy(&tt, out + 8, 4);
+        if (hit) *hit = code;
+        if (code == 1 && normal) *normal = nrm;
+        if (code != 0 && t) *t = tt;
+        return SVO_OK;
+    });
 }
 
 int svo_orbit_camera(const svo_octree *h, float pitch, float yaw, float radius, svo_camera *out) {
-    clear_error();
-    if (!h || !out) return fail(SVO_ERR_INVALID, "null argument");
-    orbit_camera(pitch, yaw, radius, h->center, *out);
-    return SVO_OK;
+    return guarded([&]() -> int {
+        clear_error();
+        if (!h || !out) return fail(SVO_ERR_INVALID, "null argument");
+        orbit_camera(pitch, yaw, radius, h->center, *out);
+        return SVO_OK;
+    });
 }
 
 int svo_render_primary_device(svo_octree *h, const svo_camera *cam, int width, int height, uint8_t *hit,
                               uint32_t *normal, float *t, uint64_t *leaf, void *stream) {
-    clear_error();
-    if (!h || !cam) return fail(SVO_ERR_INVALID, "null argument");
-    return render_primary_device(*h, *cam, width, height, hit, normal, t, leaf, stream);
+    return guarded([&]() -> int {
+        clear_error();
+        if (!h || !cam) return fail(SVO_ERR_INVALID, "null argument");
+        return render_primary_device(*h, *cam, width, height, hit, normal, t, leaf, stream);
+    });
 }
 
 int svo_render_primary(svo_octree *h, const svo_camera *cam, int width, int height, uint8_t *hit,
                        uint32_t *normal, float *t, uint64_t *leaf) {
-    clear_error();
-    if (!h || !cam) return fail(SVO_ERR_INVALID, "null argument");
-    if (width <= 0 || height <= 0) return fail(SVO_ERR_INVALID, "image size out of range");
-    size_t m = size_t(width) * size_t(height);
-    DevBuf<uint8_t> dH;
-    DevBuf<uint32_t> dN;
-    DevBuf<float> dT;
-    DevBuf<uint64_t> dL;
-    int r;
-    if ((r = out_buf(dH, hit, m)) || (r = out_buf(dN, normal, m)) || (r = out_buf(dT, t, m)) || (r = out_buf(dL, leaf, m)))
-        return r;
-    r = render_primary_device(*h, *cam, width, height, dH.p, dN.p, dT.p, dL.p, nullptr);
-    if (r) return r;
-    hipError_t e = hipDeviceSynchronize();
-    if (e != hipSuccess) return fail(SVO_ERR_DEVICE, std::string("render: ") + hipGetErrorString(e));
-    if ((r = from_device(hit, dH, m)) || (r = from_device(normal, dN, m)) || (r = from_device(t, dT, m)) ||
-        (r = from_device(leaf, dL, m)))
-        return r;
-    return SVO_OK;
+    return guarded([&]() -> int {
+        clear_error();
+        if (!h || !cam) return fail(SVO_ERR_INVALID, "null argument");
+        if (width <= 0 || height <= 0) return fail(SVO_ERR_INVALID, "image size out of range");
+        size_t m = size_t(width) * size_t(height);
+        DevBuf<uint8_t> dH;
+        DevBuf<uint32_t> dN;
+        DevBuf<float> dT;
+        DevBuf<uint64_t> dL;
+        int r;
+        if ((r = out_buf(dH, hit, m)) || (r = out_buf(dN, normal, m)) || (r = out_buf(dT, t, m)) || (r = out_buf(dL, leaf, m)))
+            return r;
+        r = render_primary_device(*h, *cam, width, height, dH.p, dN.p, dT.p, dL.p, nullptr);
+        if (r) return r;
+        hipError_t e = hipDeviceSynchronize();
+        if (e != hipSuccess) return fail(SVO_ERR_DEVICE, std::string("render: ") + hipGetErrorString(e));
+        if ((r = from_device(hit, dH, m)) || (r = from_device(normal, dN, m)) || (r = from_device(t, dT, m)) ||
+            (r = from_device(leaf, dL, m)))
+            return r;
+        return SVO_OK;
+    });
 }
 
 int svo_render_shadow_device(svo_octree *h, const svo_camera *cam, int width, int height, const float light[3],
                              float eps, uint8_t *hit, uint32_t *normal, float *t, uint64_t *leaf, uint8_t *shadow,
                              void *stream) {
-    clear_error();
-    if (!h || !cam) return fail(SVO_ERR_INVALID, "null argument");
-    return render_shadow_device(*h, *cam, width, height, light, eps, hit, normal, t, leaf, shadow, stream);
+    return guarded([&]() -> int {
+        clear_error();
+        if (!h || !cam) return fail(SVO_ERR_INVALID, "null argument");
+        return render_shadow_device(*h, *cam, width, height, light, eps, hit, normal, t, leaf, shadow, stream);
+    });
 }
 
 int svo_render_shadow(svo_octree *h, const svo_camera *cam, int width, int height, const float light[3], float eps,
                       uint8_t *hit, uint32_t *normal, float *t, uint64_t *leaf, uint8_t *shadow) {
-    clear_error();
-    if (!h || !cam || !light || !shadow) return fail(SVO_ERR_INVALID, "null argument");
-    if (width <= 0 || height <= 0) return fail(SVO_ERR_INVALID, "image size out of range");
-    size_t m = size_t(width) * size_t(height);
-    DevBuf<uint8_t> dH, dS;
-    DevBuf<uint32_t> dN;
-    DevBuf<float> dT;
-    DevBuf<uint64_t> dL;
-    int r;
-    if ((r = dH.alloc(m)) || (r = dS.alloc(m)) || (r = dT.alloc(m)) || (r = out_buf(dN, normal, m)) ||
-        (r = out_buf(dL, leaf, m)))
-        return r;
-    r = render_shadow_device(*h, *cam, width, height, light, eps, dH.p, dN.p, dT.p, dL.p, dS.p, nullptr);
-    if (r) return r;
-    hipError_t e = hipDeviceSynchronize();
-    if (e != hipSuccess) return fail(SVO_ERR_DEVICE, std::string("render: ") + hipGetErrorString(e));
-    if ((r = from_device(hit, dH, m)) || (r = from_device(normal, dN, m)) || (r = from_device(t, dT, m)) ||
-        (r = from_device(leaf, dL, m)) || (r = from_device(shadow, dS, m)))
-        return r;
-    return SVO_OK;
+    return guarded([&]() -> int {
+        clear_error();
+        if (!h || !cam || !light || !shadow) return fail(SVO_ERR_INVALID, "null argument");
+        if (width <= 0 || height <= 0) return fail(SVO_ERR_INVALID, "image size out of range");
+        size_t m = size_t(width) * size_t(height);
+        DevBuf<uint8_t> dH, dS;
+        DevBuf<uint32_t> dN;
+        DevBuf<float> dT;
+        DevBuf<uint64_t> dL;
+        int r;
+        if ((r = dH.alloc(m)) || (r = dS.alloc(m)) || (r = dT.alloc(m)) || (r = out_buf(dN, normal, m)) ||
+            (r = out_buf(dL, leaf, m)))
+            return r;
+        r = render_shadow_device(*h, *cam, width, height, light, eps, dH.p, dN.p, dT.p, dL.p, dS.p, nullptr);
+        if (r) return r;
+        hipError_t e = hipDeviceSynchronize();
+        if (e != hipSuccess) return fail(SVO_ERR_DEVICE, std::string("render: ") + hipGetErrorString(e));
+        if ((r = from_device(hit, dH, m)) || (r = from_device(normal, dN, m)) || (r = from_device(t, dT, m)) ||
+            (r = from_device(leaf, dL, m)) || (r = from_device(shadow, dS, m)))
+            return r;
+        return SVO_OK;
+    });
 }
 
 int svo_render_frame_device(svo_octree *h, const svo_camera *cam, int width, int height, int stripes, int stride,
                             uint32_t *rgba, float *depth, void *stream) {
-    clear_error();
-    if (!h || !cam) return fail(SVO_ERR_INVALID, "null argument");
-    return render_frame_device(*h, *cam, width, height, stripes, stride, rgba, depth, stream);
+    return guarded([&]() -> int {
+        clear_error();
+        if (!h || !cam) return fail(SVO_ERR_INVALID, "null argument");
+        return render_frame_device(*h, *cam, width, height, stripes, stride, rgba, depth, stream);
+    });
 }
 
-int64_t svo_frame_depth_count(int width, int height, int stripes) { return frame_depth_count(width, height, stripes); }
+int64_t svo_frame_depth_count(int width, int height, int stripes) {
+    clear_error();
+    return frame_depth_count(width, height, stripes);
+}
 
 int svo_render_frame(svo_octree *h, const svo_camera *cam, int width, int height, int stripes, int stride, uint32_t *rgba,
                      float *depth) {
-    clear_error();
-    if (!h || !cam || !rgba) return fail(SVO_ERR_INVALID, "null argument");
-    if (width <= 0 || height <= 0) return fail(SVO_ERR_INVALID, "image size out of range");
-    const size_t m = size_t(width) * size_t(height);
-    const int64_t nd = frame_depth_count(width, height, stripes);
-    DevBuf<uint32_t> dR;
-    DevBuf<float> dD;
-    int r;
-    if ((r = dR.alloc(m)) || (r = out_buf(dD, depth, size_t(nd)))) return r;
-    r = render_frame_device(*h, *cam, width, height, stripes, stride, dR.p, dD.p, nullptr);
-    if (r) return r;
-    hipError_t e = hipDeviceSynchronize();
-    if (e != hipSuccess) return fail(SVO_ERR_DEVICE, std::string("frame: ") + hipGetErrorString(e));
-    if ((r = from_device(rgba, dR, m)) || (r = from_device(depth, dD, size_t(nd)))) return r;
-    return SVO_OK;
+    return guarded([&]() -> int {
+        clear_error();
+        if (!h || !cam || !rgba) return fail(SVO_ERR_INVALID, "null argument");
+        if (width <= 0 || height <= 0) return fail(SVO_ERR_INVALID, "image size out of range");
+        const size_t m = size_t(width) * size_t(height);
+        const int64_t nd = frame_depth_count(width, height, stripes);
+        if (nd < 0) return SVO_ERR_INVALID;  // the size or stripe split is refused (message set)
+        DevBuf<uint32_t> dR;
+        DevBuf<float> dD;
+        int r;
+        if ((r = dR.alloc(m)) || (r = out_buf(dD, depth, size_t(nd)))) return r;
+        r = render_frame_device(*h, *cam, width, height, stripes, stride, dR.p, dD.p, nullptr);
+        if (r) return r;
+        hipError_t e = hipDeviceSynchronize();
+        if (e != hipSuccess) return fail(SVO_ERR_DEVICE, std::string("frame: ") + hipGetErrorString(e));
+        if ((r = from_device(rgba, dR, m)) || (r = from_device(depth, dD, size_t(nd)))) return r;
+        return SVO_OK;
+    });
 }
 
 // ---- batched rendering ----------------------------------------------------------
 
 int svo_renderer_create(int max_views, svo_renderer **out) {
-    clear_error();
-    if (!out) return fail(SVO_ERR_INVALID, "null argument");
-    *out = nullptr;
-    return renderer_create(max_views, out);
+    return guarded([&]() -> int {
+        clear_error();
+        if (!out) return fail(SVO_ERR_INVALID, "null argument");
+        *out = nullptr;
+        return renderer_create(max_views, out);
+    });
 }
 
 void svo_renderer_free(svo_renderer *r) { renderer_free(r); }
 
 int svo_render_batch_device(svo_renderer *r, const svo_view *views, int nviews, const svo_batch_opts *opts,
                             void *stream) {
-    clear_error();
-    return render_batch(r, views, nviews, opts, stream);
+    return guarded([&]() -> int {
+        clear_error();
+        return render_batch(r, views, nviews, opts, stream);
+    });
 }
 
 int svo_renderer_autotune(svo_renderer *r, const svo_view *views, int nviews, void *stream, svo_batch_opts *best,
                           float *times) {
-    clear_error();
-    return renderer_autotune(r, views, nviews, stream, best, times);
+    return guarded([&]() -> int {
+        clear_error();
+        return renderer_autotune(r, views, nviews, stream, best, times);
+    });
 }
 
 int svo_batch_graph_create(svo_renderer *r, const svo_view *views, int nviews, const svo_batch_opts *opts,
                            void *stream, svo_batch_graph **out) {
-    clear_error();
-    return batch_graph_create(r, views, nviews, opts, stream, out);
+    return guarded([&]() -> int {
+        clear_error();
+        return batch_graph_create(r, views, nviews, opts, stream, out);
+    });
 }
 
 int svo_batch_graph_launch(svo_batch_graph *g, void *stream) {
-    clear_error();
-    return batch_graph_launch(g, stream);
+    return guarded([&]() -> int {
+        clear_error();
+        return batch_graph_launch(g, stream);
+    });
 }
 
 void svo_batch_graph_free(svo_batch_graph *g) { batch_graph_free(g); }
@@ -464,8 +542,10 @@ void svo_batch_graph_free(svo_batch_graph *g) { batch_graph_free(g); }
 int svo_serve_replay(svo_octree *const *trees, int ntrees, const svo_request *requests, const double *arrival_s, int n,
                      const svo_server_opts *opts, double *latency_s, double *makespan_s, uint32_t *const *normal_out,
                      float *const *t_out, uint64_t *const *leaf_out) {
-    clear_error();
-    return serve_replay(trees, ntrees, requests, arrival_s, n, opts, latency_s, makespan_s, normal_out, t_out, leaf_out);
+    return guarded([&]() -> int {
+        clear_error();
+        return serve_replay(trees, ntrees, requests, arrival_s, n, opts, latency_s, makespan_s, normal_out, t_out, leaf_out);
+    });
 }
 
 // ---- construction -------------------------------------------------------------
@@ -473,102 +553,120 @@ int svo_serve_replay(svo_octree *const *trees, int ntrees, const svo_request *re
 int svo_build_from_mesh(const float *verts, int64_t nverts, int has_normals, const int32_t *face_counts,
                         const int32_t *face_idx, int64_t nfaces, int resolution, const svo_build_opts *opts,
                         svo_octree **out) {
-    clear_error();
-    if (!verts || !face_counts || !face_idx || !out || nverts <= 0 || nfaces < 0)
-        return fail(SVO_ERR_INVALID, "null or empty mesh");
-    *out = nullptr;
-    std::unique_ptr<svo_octree> h(new (std::nothrow) svo_octree());
-    if (!h) return fail(SVO_ERR_NOMEM, "out of host memory");
-    int r = build_mesh(verts, nverts, has_normals, face_counts, face_idx, nfaces, resolution, opts, *h);
-    if (r) return r;
-    return finish_handle(h, out);
+    return guarded([&]() -> int {
+        clear_error();
+        if (!verts || !face_counts || !face_idx || !out || nverts <= 0 || nfaces < 0)
+            return fail(SVO_ERR_INVALID, "null or empty mesh");
+        *out = nullptr;
+        std::unique_ptr<svo_octree> h(new (std::nothrow) svo_octree());
+        if (!h) return fail(SVO_ERR_NOMEM, "out of host memory");
+        int r = build_mesh(verts, nverts, has_normals, face_counts, face_idx, nfaces, resolution, opts, *h);
+        if (r) return r;
+        return finish_handle(h, out);
+    });
 }
 
 int svo_build_from_ply(const char *path, int resolution, const svo_build_opts *opts, svo_octree **out) {
-    clear_error();
-    if (!path || !out) return fail(SVO_ERR_INVALID, "null argument");
-    *out = nullptr;
-    std::unique_ptr<svo_octree> t(new (std::nothrow) svo_octree());
-    if (!t) return fail(SVO_ERR_NOMEM, "out of host memory");
-    int r = build_ply(path, resolution, opts, *t);
-    if (r) return r;
-    return finish_handle(t, out);
+    return guarded([&]() -> int {
+        clear_error();
+        if (!path || !out) return fail(SVO_ERR_INVALID, "null argument");
+        *out = nullptr;
+        std::unique_ptr<svo_octree> t(new (std::nothrow) svo_octree());
+        if (!t) return fail(SVO_ERR_NOMEM, "out of host memory");
+        int r = build_ply(path, resolution, opts, *t);
+        if (r) return r;
+        return finish_handle(t, out);
+    });
 }
 
 int svo_build_from_grid(const uint32_t *grid, int w, int h, int d, const svo_build_opts *opts, svo_octree **out) {
-    clear_error();
-    if (!grid || !out || w <= 0 || h <= 0 || d <= 0) return fail(SVO_ERR_INVALID, "null or empty volume");
-    *out = nullptr;
-    std::unique_ptr<svo_octree> t(new (std::nothrow) svo_octree());
-    if (!t) return fail(SVO_ERR_NOMEM, "out of host memory");
-    int r = build_grid(grid, w, h, d, opts, *t);
-    if (r) return r;
-    return finish_handle(t, out);
+    return guarded([&]() -> int {
+        clear_error();
+        if (!grid || !out || w <= 0 || h <= 0 || d <= 0) return fail(SVO_ERR_INVALID, "null or empty volume");
+        *out = nullptr;
+        std::unique_ptr<svo_octree> t(new (std::nothrow) svo_octree());
+        if (!t) return fail(SVO_ERR_NOMEM, "out of host memory");
+        int r = build_grid(grid, w, h, d, opts, *t);
+        if (r) return r;
+        return finish_handle(t, out);
+    });
 }
 
 int svo_build_from_sdf(const svo_sdf_primitive *prims, int count, int resolution, float shade,
                        const svo_build_opts *opts, svo_octree **out) {
-    clear_error();
-    if (!prims || !out) return fail(SVO_ERR_INVALID, "null argument");
-    *out = nullptr;
-    std::unique_ptr<svo_octree> t(new (std::nothrow) svo_octree());
-    if (!t) return fail(SVO_ERR_NOMEM, "out of host memory");
-    int r = build_sdf(prims, count, resolution, shade, opts, *t);
-    if (r) return r;
-    return finish_handle(t, out);
+    return guarded([&]() -> int {
+        clear_error();
+        if (!prims || !out) return fail(SVO_ERR_INVALID, "null argument");
+        *out = nullptr;
+        std::unique_ptr<svo_octree> t(new (std::nothrow) svo_octree());
+        if (!t) return fail(SVO_ERR_NOMEM, "out of host memory");
+        int r = build_sdf(prims, count, resolution, shade, opts, *t);
+        if (r) return r;
+        return finish_handle(t, out);
+    });
 }
 
 int svo_sdf_volume(const svo_sdf_primitive *prims, int count, int resolution, float shade, uint32_t *grid) {
-    clear_error();
-    if (!prims || !grid) return fail(SVO_ERR_INVALID, "null argument");
-    return sdf_volume(prims, count, resolution, shade, grid);
+    return guarded([&]() -> int {
+        clear_error();
+        if (!prims || !grid) return fail(SVO_ERR_INVALID, "null argument");
+        return sdf_volume(prims, count, resolution, shade, grid);
+    });
 }
 
 int svo_build_from_terrain(int resolution, uint32_t seed, const svo_build_opts *opts, svo_octree **out) {
-    clear_error();
-    if (!out) return fail(SVO_ERR_INVALID, "null argument");
-    *out = nullptr;
-    std::unique_ptr<svo_octree> t(new (std::nothrow) svo_octree());
-    if (!t) return fail(SVO_ERR_NOMEM, "out of host memory");
-    int r = build_terrain(resolution, seed, opts, *t);
-    if (r) return r;
-    return finish_handle(t, out);
+    return guarded([&]() -> int {
+        clear_error();
+        if (!out) return fail(SVO_ERR_INVALID, "null argument");
+        *out = nullptr;
+        std::unique_ptr<svo_octree> t(new (std::nothrow) svo_octree());
+        if (!t) return fail(SVO_ERR_NOMEM, "out of host memory");
+        int r = build_terrain(resolution, seed, opts, *t);
+        if (r) return r;
+        return finish_handle(t, out);
+    });
 }
 
 int svo_terrain_volume(int resolution, uint32_t seed, uint32_t *grid) {
-    clear_error();
-    if (!grid) return fail(SVO_ERR_INVALID, "null argument");
-    return terrain_volume(resolution, seed, grid);
+    return guarded([&]() -> int {
+        clear_error();
+        if (!grid) return fail(SVO_ERR_INVALID, "null argument");
+        return terrain_volume(resolution, seed, grid);
+    });
 }
 
 int svo_terrain_heights(int resolution, uint32_t seed, int32_t *heights, float *h) {
-    clear_error();
-    return terrain_heights(resolution, seed, heights, h);
+    return guarded([&]() -> int {
+        clear_error();
+        return terrain_heights(resolution, seed, heights, h);
+    });
 }
 
 int svo_build_from_voxels(const char *path, const svo_build_opts *opts, svo_octree **out) {
-    clear_error();
-    if (!path || !out) return fail(SVO_ERR_INVALID, "null argument");
-    *out = nullptr;
-    FILE *fp = std::fopen(path, "rb");
-    if (!fp) return fail(SVO_ERR_IO, std::string("cannot read ") + path);
-    int32_t dims[3];
-    if (std::fread(dims, 4, 3, fp) != 3 || dims[0] <= 0 || dims[1] <= 0 || dims[2] <= 0) {
+    return guarded([&]() -> int {
+        clear_error();
+        if (!path || !out) return fail(SVO_ERR_INVALID, "null argument");
+        *out = nullptr;
+        FILE *fp = std::fopen(path, "rb");
+        if (!fp) return fail(SVO_ERR_IO, std::string("cannot read ") + path);
+        int32_t dims[3];
+        if (std::fread(dims, 4, 3, fp) != 3 || dims[0] <= 0 || dims[1] <= 0 || dims[2] <= 0) {
+            std::fclose(fp);
+            return fail(SVO_ERR_FORMAT, ".voxel: bad header");
+        }
+        size_t n = size_t(dims[0]) * size_t(dims[1]) * size_t(dims[2]);
+        std::vector<uint32_t> vol;
+        try {
+            vol.resize(n);
+        } catch (...) {
+            std::fclose(fp);
+            return fail(SVO_ERR_NOMEM, ".voxel: volume does not fit in host memory");
+        }
+        size_t got = std::fread(vol.data(), 4, n, fp);
         std::fclose(fp);
-        return fail(SVO_ERR_FORMAT, ".voxel: bad header");
-    }
-    size_t n = size_t(dims[0]) * size_t(dims[1]) * size_t(dims[2]);
-    std::vector<uint32_t> vol;
-    try {
-        vol.resize(n);
-    } catch (...) {
-        std::fclose(fp);
-        return fail(SVO_ERR_NOMEM, ".voxel: volume does not fit in host memory");
-    }
-    size_t got = std::fread(vol.data(), 4, n, fp);
-    std::fclose(fp);
-    if (got != n) return fail(SVO_ERR_FORMAT, ".voxel: truncated");
-    return svo_build_from_grid(vol.data(), dims[0], dims[1], dims[2], opts, out);
+        if (got != n) return fail(SVO_ERR_FORMAT, ".voxel: truncated");
+        return svo_build_from_grid(vol.data(), dims[0], dims[1], dims[2], opts, out);
+    });
 }
 
 }  // extern "C"

@@ -478,8 +478,12 @@ class VoxelOctree:
 
 
 def frame_depth_count(width, height, stripes=16):
-    """Entries of the viewer frame's coarse depth table."""
-    return int(lib.svo_frame_depth_count(int(width), int(height), int(stripes)))
+    """Entries of the viewer frame's coarse depth table; raises SvoError for
+    a size or stripe split the renderer refuses."""
+    n = int(lib.svo_frame_depth_count(int(width), int(height), int(stripes)))
+    if n < 0:
+        raise SvoError(ERR_INVALID, lib.svo_last_error().decode(errors="replace"))
+    return n
 
 
 class Renderer:

@@ -38,7 +38,7 @@ Fixtures written (all small):
   grid_edges.npz + build_grid_{empty16,one16,slice537,full4}.oct + builds.json["grid_edges"]
                          edge volumes through the reference's file-source builder
 
-`--only frames|ico2048|terrain|edges` regenerates just that part;
+`--only frames|ico2048|terrain|edges|extremes` regenerates just that part;
 other entries of builds.json are kept.
 """
 import argparse
@@ -176,9 +176,37 @@ def tag_words(words):
     return tagged
 
 
-def raymarch_fixture(name, oct_path, seed, tmp, **kw):
+EXTREME_VALUES = np.float32([0.0, -0.0, 1.0, -1.0, 1.5, 0.3, -0.7, 1e-30, -1e-30, 1e-5, 3e34, -3e34, 1e38, -1e38,
+                             np.inf, -np.inf, 2.0, 1.2, 1.75])
+
+
+def extreme_rays(center, seed, k=20000):
+    """Rays whose components are drawn from signed zeros, tiny, huge and
+    infinite values (no NaN, and no axis with both an infinite origin and an
+    infinite direction: the reference's loop never ends on those), half of them
+    from orbit-camera origins, with rayScale from {0, -0, 1e-3, 0.05, 1e38, inf, nan}."""
+    rng = np.random.default_rng(seed)
+    o = EXTREME_VALUES[rng.integers(0, len(EXTREME_VALUES), (k, 3))]
+    d = EXTREME_VALUES[rng.integers(0, len(EXTREME_VALUES), (k, 3))]
+    cams = [oracle.orbit_camera(p, y, 1.0, center)[1] for p, y in scenes.orbit_angles(seed, 8)]
+    half = k // 2
+    o[:half] = np.asarray(cams, np.float32)[rng.integers(0, len(cams), half)]
+    keep = rng.random((half, 3)) < 0.6
+    d[:half] = np.where(keep, rng.normal(size=(half, 3)).astype(np.float32), d[:half])
+    bad = (np.isinf(o) & np.isinf(d)).any(axis=1)
+    o[bad] = np.float32(1.5)
+    scales = np.float32([0.0, -0.0, 1e-3, 0.05, 1e38, np.inf, np.nan])
+    r = np.zeros((k, 7), np.float32)
+    r[:, 0:3] = o
+    r[:, 3:6] = d
+    r[:, 6] = scales[rng.integers(0, len(scales), k)]
+    return r
+
+
+def raymarch_fixture(name, oct_path, seed, tmp, rays=None, **kw):
     center, words = oracle.read_oct(oct_path)
-    rays = make_rays(center, seed, **kw)
+    if rays is None:
+        rays = make_rays(center, seed, **kw)
     hit, normal, t = ref_raymarch(oct_path, rays, tmp)
     tagged_path = os.path.join(tmp, "tagged.oct")
     literal_oct(tagged_path, center, tag_words(words))
@@ -332,7 +360,7 @@ GRID_CASES = [
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--reference", default="/root/reference")
-    ap.add_argument("--only", choices=["frames", "ico2048", "terrain", "edges"], default=None)
+    ap.add_argument("--only", choices=["frames", "ico2048", "terrain", "edges", "extremes"], default=None)
     args = ap.parse_args()
     if not os.path.exists(REF_BIN):
         subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref", f"REF={args.reference}"], check=True)
@@ -340,6 +368,11 @@ def main():
     try:
         dragon = os.path.join(HERE, "dragon.oct")
         shutil.copyfile(os.path.join(args.reference, "models", "XYZRGB-Dragon.oct"), dragon)
+
+        if args.only == "extremes":
+            center, _ = oracle.read_oct(dragon)
+            raymarch_fixture("extremes", dragon, 0, tmp, rays=extreme_rays(center, 4242))
+            return
 
         bpath = os.path.join(HERE, "builds.json")
         if args.only in ("ico2048", "terrain", "edges"):

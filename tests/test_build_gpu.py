@@ -248,6 +248,10 @@ def test_shadow_rays_match_oracle(svo, n, seed, w, h):
     assert np.array_equal(r["shadow"], shadow)
     hits = r["hit"] == 1
     assert hits.mean() > 0.3 and 0.02 < r["shadow"][hits].mean() < 0.98
+    for bad_eps in (float("nan"), float("inf")):  # would move the shadow origins off to NaN/inf
+        with pytest.raises(svo.SvoError) as e:
+            t.render_shadow(cam, 16, 16, light, bad_eps)
+        assert e.value.code == svo.svo.ERR_INVALID
 
 
 def test_terrain_full_size(svo):

@@ -119,3 +119,13 @@ def test_host_camera_matches_reference(svo, golden):
         assert np.array_equal(np.frombuffer(cam.tform, np.float32).view(np.uint32), tf.view(np.uint32))
         _, pos = orc.orbit_camera(inp[0], inp[1], inp[2], inp[3:6])
         assert np.array_equal(np.frombuffer(cam.pos, np.float32).view(np.uint32), pos.view(np.uint32))
+
+
+def test_frame_depth_count_matches_oracle_and_refuses_empty_stripes(svo):
+    for w, h, stripes in [(1920, 1080, 16), (320, 176, 16), (641, 359, 7), (64, 48, 1), (64, 64, 0)]:
+        assert svo.svo.frame_depth_count(w, h, stripes) == \
+            orc.lib().oracle_frame_depth_count(w, h, stripes if stripes > 0 else 16), (w, h, stripes)
+    for w, h, stripes in [(320, 100, 16), (0, 10, 1), (10, 0, 1), (40000, 8, 1), (8, 8, 9)]:
+        with pytest.raises(svo.SvoError) as e:
+            svo.svo.frame_depth_count(w, h, stripes)
+        assert e.value.code == svo.svo.ERR_INVALID

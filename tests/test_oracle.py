@@ -56,6 +56,35 @@ def test_raymarch_matches_reference(golden, name):
     assert np.array_equal(r["leaf"][leaf], g["leaf"][leaf].astype(np.uint64))
 
 
+def _nan_rays():
+    """Rays the reference never ends: a NaN origin or direction component, or
+    an infinite origin on an axis whose direction is infinite (-0 * inf)."""
+    o = np.float32([[np.nan, 1.5, 1.5], [1.5, 1.5, 1.5], [np.inf, 0.5, 0.5], [0.2, -np.inf, 0.3],
+                    [1.2, 1.3, np.nan]])
+    d = np.float32([[0.0, 0.0, 1.0], [np.nan, 1.0, 0.0], [-np.inf, 0.1, 0.2], [0.1, np.inf, 0.0],
+                    [np.nan, np.nan, np.nan]])
+    return o, d
+
+
+def test_extreme_rays_match_reference(golden):
+    """Signed zeros, tiny, huge and infinite components and rayScales (golden
+    from the reference binary); rays with a NaN coefficient are misses."""
+    g = np.load(os.path.join(golden, "raymarch_extremes.npz"))
+    _, words = orc.read_oct(os.path.join(golden, "dragon.oct"))
+    rays = g["rays"]
+    r = orc.raymarch(words, rays[:, 0:3], rays[:, 3:6], rays[:, 6])
+    assert np.array_equal(r["hit"] != 0, g["hit"] != 0)
+    leaf = r["hit"] == 1
+    assert leaf.sum() > 100
+    assert np.array_equal(r["normal"][leaf], g["normal"][leaf])
+    hit = r["hit"] != 0
+    assert _eq_bits(r["t"][hit], g["t"][hit])
+    assert np.array_equal(r["leaf"][leaf], g["leaf"][leaf].astype(np.uint64))
+    o, d = _nan_rays()
+    r = orc.raymarch(words, o, d, 0.0)
+    assert not r["hit"].any()
+
+
 @pytest.mark.parametrize("name", ["dragon", "sdf128"])
 def test_frame_matches_reference(golden, name):
     """Viewer frames (src/Main.cpp:81-202, 16 stripes) against the reference

@@ -50,6 +50,32 @@ def test_batch_matches_reference_goldens(trees, golden, name):
     _same(r, orc.raymarch(trees[name].words(), rays[:, 0:3], rays[:, 3:6], rays[:, 6]), name)
 
 
+def test_extreme_and_nan_rays(trees, golden, svo):
+    """Extreme components against the reference's outputs; rays whose
+    coefficients are NaN (the reference never ends them) come back as misses
+    from the batch, device and single-ray entries."""
+    g = np.load(os.path.join(golden, "raymarch_extremes.npz"))
+    rays = g["rays"]
+    t = trees["dragon"]
+    print(f"[gpu] extreme rays: {len(rays)}", flush=True)
+    r = t.raymarch_batch(rays[:, 0:3], rays[:, 3:6], rays[:, 6])
+    assert np.array_equal(r["hit"] != 0, g["hit"] != 0)
+    leaf = r["hit"] == 1
+    assert np.array_equal(r["normal"][leaf], g["normal"][leaf])
+    hit = r["hit"] != 0
+    assert np.array_equal(_bits(r["t"][hit]), _bits(g["t"][hit]))
+    assert np.array_equal(r["leaf"][leaf], g["leaf"][leaf].astype(np.uint64))
+    _same(r, orc.raymarch(t.words(), rays[:, 0:3], rays[:, 3:6], rays[:, 6]), "extremes")
+    o = np.float32([[np.nan, 1.5, 1.5], [1.5, 1.5, 1.5], [np.inf, 0.5, 0.5], [1.2, 1.3, np.nan]])
+    d = np.float32([[0.0, 0.0, 1.0], [np.nan, 1.0, 0.0], [-np.inf, 0.1, 0.2], [np.nan, np.nan, np.nan]])
+    print("[gpu] NaN rays", flush=True)
+    r = t.raymarch_batch(o, d, 0.0)
+    assert not r["hit"].any() and not r["normal"].any() and not r["t"].any()
+    assert (r["leaf"] == svo.svo.LEAF_NONE).all()
+    for i in range(len(o)):
+        assert t.raymarch(o[i], d[i], 0.0)[0] == 0
+
+
 @pytest.mark.parametrize("name,w,h,pitch,yaw", [
     ("dragon", 320, 180, 20.0, 35.0),
     ("dragon", 257, 131, -55.0, 301.5),
