@@ -233,7 +233,9 @@ int svo_renderer_autotune(svo_renderer *r, const svo_view *views, int nviews, vo
  * in a HIP graph: the graph holds its own view table and queue counters, so
  * every replay is the queue reset kernel plus the persistent kernel, launched
  * with one call. Replays of one graph must not overlap (launch them on one
- * stream); the octrees and output arrays must outlive the graph. */
+ * stream); the octrees and output arrays must outlive the graph, and the
+ * octrees must stay on the graph's device (using one on another device moves
+ * its word array there). */
 typedef struct svo_batch_graph svo_batch_graph;
 int svo_batch_graph_create(svo_renderer *r, const svo_view *views, int nviews, const svo_batch_opts *opts,
                            void *stream, svo_batch_graph **out);

@@ -62,7 +62,8 @@ __device__ __forceinline__ int stripe_of(const Stripe *__restrict__ st, int n, i
     return lo;
 }
 
-template <bool Lod, typename Idx>
+// One ray to completion; the leaf word (normal) is read only when Normal is set.
+template <bool Lod, bool Normal, typename Idx>
 __device__ __forceinline__ int trace(const uint32_t *__restrict__ tree, float ox, float oy, float oz, float dx, float dy,
                                      float dz, float rayScale, const Stack<Idx> &stk, uint32_t &normal, float &t) {
     Ray<Idx> r;
@@ -70,7 +71,7 @@ __device__ __forceinline__ int trace(const uint32_t *__restrict__ tree, float ox
     Idx leaf = 0;
     int code;
     do {
-        code = ray_step<Lod, true>(r, tree, stk, normal, t, leaf);  // camera rays are finite
+        code = ray_step<Lod, true, Idx, Normal>(r, tree, stk, normal, t, leaf);  // camera rays are finite
     } while (code == kRunning);
     return code;
 }
@@ -102,7 +103,7 @@ __global__ void __launch_bounds__(64) coarse_kernel(const uint32_t *__restrict__
     Stack<Idx> stk(lds, 64, slots, int(threadIdx.x));
     uint32_t normal = 0;
     float t = 0.0f;
-    const int code = trace<true, Idx>(tree, p.px, p.py, p.pz, d[0], d[1], d[2], p.coarseScale, stk, normal, t);
+    const int code = trace<true, false, Idx>(tree, p.px, p.py, p.pz, d[0], d[1], d[2], p.coarseScale, stk, normal, t);
     depth[i] = code != march::kMiss ? t : kTreeMiss;
 }
 
@@ -140,7 +141,7 @@ __global__ void __launch_bounds__(64) fine_kernel(const uint32_t *__restrict__ t
             uint32_t normal = 0;
             float t = 0.0f;
             float v = 0.0f;
-            const int code = trace<false, Idx>(tree, p.px + d[0] * m, p.py + d[1] * m, p.pz + d[2] * m, d[0], d[1], d[2],
+            const int code = trace<false, true, Idx>(tree, p.px + d[0] * m, p.py + d[1] * m, p.pz + d[2] * m, d[0], d[1], d[2],
                                                0.0f, stk, normal, t);
             if (code != march::kMiss) {
                 // shade, src/Main.cpp:81-90

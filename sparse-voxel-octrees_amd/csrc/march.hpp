@@ -137,12 +137,14 @@ __device__ __forceinline__ bool ray_defined(const Ray<Idx> &r) {
 //
 //  Lod = false: the ray's rayScale is 0 (+0 or -0). maxTC is finite, so
 //    maxTC * rayScale >= scaleExp2 (> 0) never holds and the test is dropped.
+//  Normal = false: the caller needs only the hit code and t; a leaf hit
+//    leaves `leaf` and `normal` unset and reads no leaf word.
 //  Finite = true: the ray's origin and direction are finite (|origin| below
 //    2^100). Then every corner t is finite and never -0 (pos*dT is a nonzero
 //    product; a difference of equal nonzero values is +0), so the minimum of
 //    the three corner times does not depend on the order of ties or on NaN
 //    handling and one hardware min3 returns the reference's std::min result.
-template <bool Lod, bool Finite, typename Idx>
+template <bool Lod, bool Finite, typename Idx, bool Normal = true>
 __device__ __forceinline__ int ray_step(Ray<Idx> &r, const uint32_t *__restrict__ tree, const Stack<Idx> &stk,
                                         uint32_t &normal, float &tOut, Idx &leaf) {
     const float cornerTX = r.posX * r.dTx - r.bTx;
@@ -169,9 +171,11 @@ __device__ __forceinline__ int ray_step(Ray<Idx> &r, const uint32_t *__restrict_
             if (r.current & 0x20000u)
                 childOffset = Idx((uint64_t(r.current >> 18) << 32) | uint64_t(tree[r.parent + 1]));
             if (!(childMasks & 0x80u)) {
-                leaf = childOffset + r.parent +
-                       Idx(__popc((((childMasks >> (8 + childShift)) << childShift) & 127u)));
-                normal = tree[leaf];
+                if (Normal) {
+                    leaf = childOffset + r.parent +
+                           Idx(__popc((((childMasks >> (8 + childShift)) << childShift) & 127u)));
+                    normal = tree[leaf];
+                }
                 tOut = r.minT;
                 return kLeaf;
             }

@@ -31,8 +31,9 @@ namespace {
 
 using namespace march;
 
-// Runs one ray to completion. Returns 0 miss, 1 leaf hit, 2 LOD hit.
-template <bool Lod, bool Finite, typename Idx>
+// Runs one ray to completion. Returns 0 miss, 1 leaf hit, 2 LOD hit. With
+// Normal set, a leaf hit also gives the leaf's word index and word.
+template <bool Lod, bool Finite, typename Idx, bool Normal = true>
 __device__ __forceinline__ int march_ray(const uint32_t *__restrict__ tree, float ox, float oy, float oz, float dx,
                                          float dy, float dz, float rayScale, const Stack<Idx> &stk, uint32_t &normal,
                                          float &tOut, Idx &leaf) {
@@ -41,7 +42,7 @@ __device__ __forceinline__ int march_ray(const uint32_t *__restrict__ tree, floa
     if (!Finite && !ray_defined(r)) return kMiss;  // the reference never ends such a ray
     int code;
     do {
-        code = ray_step<Lod, Finite>(r, tree, stk, normal, tOut, leaf);
+        code = ray_step<Lod, Finite, Idx, Normal>(r, tree, stk, normal, tOut, leaf);
     } while (code == kRunning);
     return code;
 }
@@ -144,7 +145,7 @@ __global__ void __launch_bounds__(64) shadow_kernel(const uint32_t *__restrict__
     uint32_t normal = 0;
     float t = 0.0f;
     Idx leaf = 0;
-    const int code = march_ray<false, true, Idx>(tree, p.px + d[0] * back, p.py + d[1] * back, p.pz + d[2] * back, lx,
+    const int code = march_ray<false, true, Idx, false>(tree, p.px + d[0] * back, p.py + d[1] * back, p.pz + d[2] * back, lx,
                                                  ly, lz, 0.0f, stk, normal, t, leaf);
     shadow[i] = (code == 1 || code == 2) ? 1 : 0;
 }
