@@ -33,6 +33,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <unistd.h>
 #include <vector>
 
@@ -72,8 +73,16 @@ struct Arena {
             (void)hipStreamSynchronize(stream);
             (void)hipStreamDestroy(stream);
         }
+        // keep a bounded reserve for the next build (1/64 of the device, at most
+        // 4 GiB): memory newly obtained from the driver costs milliseconds per
+        // large allocation
         hipMemPool_t pool;
-        if (default_pool(pool)) (void)hipMemPoolTrimTo(pool, 0);
+        if (default_pool(pool)) (void)hipMemPoolTrimTo(pool, pool_reserve());
+    }
+    static size_t pool_reserve() {
+        size_t freeB = 0, total = 0;
+        if (hipMemGetInfo(&freeB, &total) != hipSuccess) return 0;
+        return std::min<size_t>(total / 64, size_t(4) << 30);
     }
     static bool default_pool(hipMemPool_t &pool) {
         int dev = 0;
@@ -1168,6 +1177,64 @@ struct Level {
     uint32_t *word = nullptr;   // leaves: material word
 };
 
+// Two pinned chunks shared by every download of the process (never freed: a
+// static destructor may run after the HIP runtime has gone).
+struct Staging {
+    static constexpr uint64_t kChunk = uint64_t(8) << 20;  // bytes
+    std::mutex mu;
+    void *buf[2] = {nullptr, nullptr};
+};
+
+Staging &staging() {
+    static Staging *s = new Staging();
+    return *s;
+}
+
+// Copies W device words into out: the DMA fills one pinned chunk while the host
+// copies the previous one out (a copy straight into pageable memory goes
+// through the runtime's own staging at a few GB/s).
+int download_words(const uint32_t *src, uint64_t W, WordVec &out, hipStream_t st) {
+    try {
+        out.resize(W);
+    } catch (...) {
+        return fail(SVO_ERR_NOMEM, "builder: host word array");
+    }
+    const uint64_t bytes = W * sizeof(uint32_t);
+    if (bytes <= Staging::kChunk) {
+        HIP_TRY(hipMemcpyAsync(out.data(), src, bytes, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        return SVO_OK;
+    }
+    Staging &sg = staging();
+    std::lock_guard<std::mutex> lock(sg.mu);
+    for (void *&b : sg.buf)
+        if (!b) HIP_TRY(hipHostMalloc(&b, Staging::kChunk, hipHostMallocDefault));
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    hipError_t e = hipEventCreateWithFlags(&ev[0], hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ev[1], hipEventDisableTiming);
+    const uint8_t *s8 = reinterpret_cast<const uint8_t *>(src);
+    uint8_t *d8 = reinterpret_cast<uint8_t *>(out.data());
+    const uint64_t n = (bytes + Staging::kChunk - 1) / Staging::kChunk;
+    auto len_of = [&](uint64_t i) { return std::min(Staging::kChunk, bytes - i * Staging::kChunk); };
+    for (uint64_t i = 0; i <= n && e == hipSuccess; ++i) {
+        // chunk i goes into the buffer chunk i-2 used; the host copied that out in the previous pass
+        if (i < n) {
+            e = hipMemcpyAsync(sg.buf[i & 1], s8 + i * Staging::kChunk, len_of(i), hipMemcpyDeviceToHost, st);
+            if (e == hipSuccess) e = hipEventRecord(ev[i & 1], st);
+        }
+        if (i > 0 && e == hipSuccess) {
+            const uint64_t j = i - 1;
+            e = hipEventSynchronize(ev[j & 1]);
+            if (e == hipSuccess) std::memcpy(d8 + j * Staging::kChunk, sg.buf[j & 1], len_of(j));
+        }
+    }
+    if (e != hipSuccess) (void)hipStreamSynchronize(st);  // no copy may still target the staging chunks
+    for (hipEvent_t x : ev)
+        if (x) (void)hipEventDestroy(x);
+    if (e != hipSuccess) return fail(SVO_ERR_DEVICE, std::string("builder: download: ") + hipGetErrorString(e));
+    return SVO_OK;
+}
+
 // Builds the word array from sorted, unique leaf keys/words. blockFlags marks
 // the cache blocks the occupancy table counts as non-empty (nullptr when the
 // whole volume is one cache block).
@@ -1296,16 +1363,9 @@ int emit_tree(Arena &ar, const Geometry &g, uint64_t *leafKey, uint32_t *leafWor
         if (badH) return fail(SVO_ERR_DEVICE, "builder: placement outside the word array (internal error)");
         return fail(SVO_ERR_DEVICE, std::string("builder: scatter: ") + hipGetErrorString(e));
     }
-    try {
-        out.words.resize(W);
-    } catch (...) {
+    if ((r = download_words(words, W, out.words, st))) {
         (void)hipFree(words);
-        return fail(SVO_ERR_NOMEM, "builder: host word array");
-    }
-    e = hipMemcpy(out.words.data(), words, W * sizeof(uint32_t), hipMemcpyDeviceToHost);
-    if (e != hipSuccess) {
-        (void)hipFree(words);
-        return fail(SVO_ERR_DEVICE, std::string("builder: download: ") + hipGetErrorString(e));
+        return r;
     }
     int dev = 0;
     (void)hipGetDevice(&dev);

@@ -7,6 +7,8 @@
 // bit 17 = far pointer (offset = bits31..18 << 32 | next word), bit 16 = two
 // words per child slot, bits 15..8 valid mask, bits 7..0 non-leaf mask. The
 // k-th valid child (ascending bit) sits at index + offset + k*(1+large).
+#include <sys/mman.h>
+
 #include <algorithm>
 #include <atomic>
 #include <thread>
@@ -104,6 +106,24 @@ int octree_validate(const uint32_t *w, uint64_t n) {
     worker();
     for (std::thread &t : pool) t.join();
     return bad.load() ? -1 : worst.load();
+}
+
+// ---- host memory for large word arrays ----
+
+void *map_large(size_t bytes) {
+    void *p = mmap(nullptr, bytes, PROT_READ | PROT_WRITE, MAP_PRIVATE | MAP_ANONYMOUS, -1, 0);
+    if (p == MAP_FAILED) return nullptr;
+#ifdef MADV_HUGEPAGE
+    (void)madvise(p, bytes, MADV_HUGEPAGE);
+#endif
+#ifdef MADV_POPULATE_WRITE
+    (void)madvise(p, bytes, MADV_POPULATE_WRITE);
+#endif
+    return p;
+}
+
+void unmap_large(void *p, size_t bytes) {
+    if (p) (void)munmap(p, bytes);
 }
 
 }  // namespace svo

@@ -2,13 +2,60 @@
 #pragma once
 
 #include <cstdint>
+#include <memory>
+#include <new>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "svo.h"
 
+namespace svo {
+// Large host arrays (word arrays of big octrees): mapped directly, with
+// transparent huge pages requested and the pages populated in one call, so
+// that filling them (a download, a decode) does not take a page fault per
+// 4 KiB. Both requests are advisory; without them the pages fault in as usual.
+void *map_large(size_t bytes);
+void unmap_large(void *p, size_t bytes);
+constexpr size_t kLargeArray = size_t(4) << 20;
+
+// Allocator that leaves default-constructed elements uninitialised, so that
+// sizing a word array before it is overwritten does not first write zeros over
+// it, and that maps large arrays with map_large.
+template <typename T>
+struct NoInitAlloc : std::allocator<T> {
+    using value_type = T;
+    template <typename U>
+    struct rebind {
+        using other = NoInitAlloc<U>;
+    };
+    NoInitAlloc() = default;
+    template <typename U>
+    NoInitAlloc(const NoInitAlloc<U> &) noexcept {}
+    T *allocate(size_t n) {
+        if (n * sizeof(T) < kLargeArray) return std::allocator<T>::allocate(n);
+        void *p = map_large(n * sizeof(T));
+        if (!p) throw std::bad_alloc();
+        return static_cast<T *>(p);
+    }
+    void deallocate(T *p, size_t n) noexcept {
+        if (n * sizeof(T) < kLargeArray) std::allocator<T>::deallocate(p, n);
+        else unmap_large(p, n * sizeof(T));
+    }
+    template <typename U, typename... A>
+    void construct(U *p, A &&...a) {
+        ::new (static_cast<void *>(p)) U(std::forward<A>(a)...);
+    }
+    template <typename U>
+    void construct(U *p) noexcept {
+        ::new (static_cast<void *>(p)) U;
+    }
+};
+using WordVec = std::vector<uint32_t, NoInitAlloc<uint32_t>>;
+}  // namespace svo
+
 struct svo_octree {
-    std::vector<uint32_t> words;  // Laine-Karras word array (src/VoxelOctree.hpp:40-41)
+    svo::WordVec words;           // Laine-Karras word array (src/VoxelOctree.hpp:40-41)
     float center[3] = {0, 0, 0};  // VoxelOctree::_center
     int depth = 0;                // descriptor levels + 1
     int device = -1;              // device holding `dwords`
