@@ -8,8 +8,12 @@
 // shaded and packed to RGBA; other tiles stay 0. Ray directions use the
 // reference's running sums (dx += scale per pixel inside a tile, dx +=
 // 8*scale per corner along a stripe), so they are reproduced exactly: the
-// corner sums are tabulated on the host, the in-tile sums are stepped per lane.
+// corner sums are tabulated by a small kernel that takes the same float steps,
+// the in-tile sums are stepped per lane. Nothing is staged from the host, so
+// consecutive frames queue on the stream without a host round trip.
 //
+//  tables_kernel: stripe table and corner sums (one thread per stripe, one
+//                 for the column sums)
 //  coarse_kernel: one lane per corner of every stripe -> depth table
 //  fine_kernel:   one wave per tile; lane = pixel; with stride > 1 only the
 //                 stride-corner lanes trace and the others copy their colour
@@ -17,8 +21,6 @@
 
 #include <algorithm>
 #include <cmath>
-#include <cstring>
-#include <vector>
 
 #include "build_common.hpp"
 #include "march.hpp"
@@ -105,6 +107,37 @@ __global__ void __launch_bounds__(64) coarse_kernel(const uint32_t *__restrict__
     float t = 0.0f;
     const int code = trace<true, false, Idx>(tree, p.px, p.py, p.pz, d[0], d[1], d[2], p.coarseScale, stk, normal, t);
     depth[i] = code != march::kMiss ? t : kTreeMiss;
+}
+
+// Stripe i covers rows [i*sh, min((i+1)*sh, H)); every stripe but the last is
+// full (the layout check guarantees it), so its bases follow from i. dy per
+// corner row and dx per corner column are the reference's running sums, taken
+// with the same float steps as its loops.
+__global__ void tables_kernel(FrameParams p, int sh, float tileScale, int64_t corners, int64_t tiles,
+                              Stripe *__restrict__ st, float *__restrict__ dyTab, float *__restrict__ dxTab) {
+    const int full = (sh - 1) / 8 + 2;  // corner rows of a full stripe
+    for (int i = int(blockIdx.x * blockDim.x + threadIdx.x); i <= p.stripes; i += int(gridDim.x * blockDim.x)) {
+        if (i == p.stripes) {
+            Stripe e{};
+            e.depthBase = corners;
+            e.fineBase = tiles;
+            st[i] = e;
+            float dx = -1.0f + 0.0f * p.scale;
+            for (int tx = 0; tx < p.tilesX; ++tx, dx += tileScale) dxTab[tx] = dx;
+            continue;
+        }
+        Stripe s;
+        s.y0 = i * sh;
+        s.y1 = min(s.y0 + sh, p.height);
+        s.tilesY = (s.y1 - s.y0 - 1) / 8 + 2;
+        s.pad = 0;
+        s.depthBase = int64_t(i) * full * p.tilesX;
+        s.fineBase = int64_t(i) * (full - 1) * (p.tilesX - 1);
+        st[i] = s;
+        float dy = p.aspect - float(s.y0) * p.scale;
+        float *row = dyTab + int64_t(i) * full;
+        for (int ty = 0; ty < s.tilesY; ++ty, dy -= tileScale) row[ty] = dy;
+    }
 }
 
 template <typename Idx>
@@ -223,50 +256,28 @@ int render_frame_device(svo_octree &h, const svo_camera &cam, int width, int hei
     p.tilesX = (width - 1) / 8 + 2;
     p.stripes = stripes;
     p.stride = stride;
-    // stripes, corner-row dy sums and corner-column dx sums, as the reference accumulates them
-    std::vector<Stripe> hs(size_t(stripes) + 1);
-    std::vector<float> dyTab, dxTab(size_t(p.tilesX));
+    // corner and fine-tile counts (the tables themselves are made on the device)
     const int sh = (height - 1) / stripes + 1;
     int64_t corners = 0, tiles = 0;
     for (int i = 0; i < stripes; ++i) {
-        Stripe &s = hs[size_t(i)];
-        s.y0 = i * sh;
-        s.y1 = std::min((i + 1) * sh, height);
-        s.tilesY = (s.y1 - s.y0 - 1) / 8 + 2;
-        s.pad = 0;
-        s.depthBase = corners;
-        s.fineBase = tiles;
-        corners += int64_t(s.tilesY) * p.tilesX;
-        tiles += int64_t(s.tilesY - 1) * (p.tilesX - 1);
-        float dy = p.aspect - float(s.y0) * p.scale;
-        for (int ty = 0; ty < s.tilesY; ++ty, dy -= tileScale) dyTab.push_back(dy);
-    }
-    hs[size_t(stripes)].depthBase = corners;
-    hs[size_t(stripes)].fineBase = tiles;
-    {
-        float dx = -1.0f + 0.0f * p.scale;
-        for (int tx = 0; tx < p.tilesX; ++tx, dx += tileScale) dxTab[size_t(tx)] = dx;
+        const int y0 = i * sh, y1 = std::min(y0 + sh, height);
+        const int tilesY = (y1 - y0 - 1) / 8 + 2;
+        corners += int64_t(tilesY) * p.tilesX;
+        tiles += int64_t(tilesY - 1) * (p.tilesX - 1);
     }
     if (tiles > 0x7FFFFFFF) return fail(SVO_ERR_LIMIT, "image too large");
     hipStream_t s = static_cast<hipStream_t>(stream);
-    const size_t tabBytes = sizeof(Stripe) * hs.size() + sizeof(float) * (dyTab.size() + dxTab.size());
+    const size_t rows = size_t(corners / p.tilesX);
+    const size_t stripeBytes = sizeof(Stripe) * size_t(stripes + 1);
+    const size_t tabBytes = stripeBytes + sizeof(float) * (rows + size_t(p.tilesX));
     void *scratch = nullptr;
     HIP_TRY(hipMallocAsync(&scratch, tabBytes + (depthOut ? 0 : sizeof(float) * size_t(corners)), s));
     Stripe *dSt = static_cast<Stripe *>(scratch);
-    float *dDy = reinterpret_cast<float *>(dSt + hs.size());
-    float *dDx = dDy + dyTab.size();
-    float *dDepth = depthOut ? depthOut : dDx + dxTab.size();
-    std::vector<uint8_t> staging(tabBytes);
-    std::memcpy(staging.data(), hs.data(), sizeof(Stripe) * hs.size());
-    std::memcpy(staging.data() + sizeof(Stripe) * hs.size(), dyTab.data(), sizeof(float) * dyTab.size());
-    std::memcpy(staging.data() + sizeof(Stripe) * hs.size() + sizeof(float) * dyTab.size(), dxTab.data(),
-                sizeof(float) * dxTab.size());
-    hipError_t e = hipMemcpyAsync(scratch, staging.data(), tabBytes, hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);  // staging is a host vector
-    if (e != hipSuccess) {
-        (void)hipFreeAsync(scratch, s);
-        return fail(SVO_ERR_DEVICE, std::string("frame: ") + hipGetErrorString(e));
-    }
+    float *dDy = reinterpret_cast<float *>(static_cast<uint8_t *>(scratch) + stripeBytes);
+    float *dDx = dDy + rows;
+    float *dDepth = depthOut ? depthOut : dDx + p.tilesX;
+    hipLaunchKernelGGL(tables_kernel, dim3(unsigned((stripes + 1 + 255) / 256)), dim3(256), 0, s, p, sh, tileScale,
+                       corners, tiles, dSt, dDy, dDx);
     const int slots = h.depth > 2 ? h.depth - 1 : 1;
     const bool wide = h.words.size() >= (uint64_t(1) << 32);
     const size_t lds = size_t(slots) * 64 * ((wide ? 8 : 4) + sizeof(float));
@@ -282,7 +293,7 @@ int render_frame_device(svo_octree &h, const svo_camera &cam, int width, int hei
         if (tiles) hipLaunchKernelGGL(fine_kernel<uint64_t>, dim3(unsigned(tiles)), dim3(64), lds, s, h.dwords, p, dSt,
                                       dDepth, slots, rgba);
     }
-    e = hipGetLastError();
+    hipError_t e = hipGetLastError();
     (void)hipFreeAsync(scratch, s);
     if (e != hipSuccess) return fail(SVO_ERR_DEVICE, std::string("frame: ") + hipGetErrorString(e));
     return SVO_OK;
