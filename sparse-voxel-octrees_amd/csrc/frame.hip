@@ -7,13 +7,12 @@
 // depths are not all "miss" is traced per pixel from pos + dir*max(minT-0.03,0),
 // shaded and packed to RGBA; other tiles stay 0. Ray directions use the
 // reference's running sums (dx += scale per pixel inside a tile, dx +=
-// 8*scale per corner along a stripe), so they are reproduced exactly: the
-// corner sums are tabulated by a small kernel that takes the same float steps,
-// the in-tile sums are stepped per lane. Nothing is staged from the host, so
-// consecutive frames queue on the stream without a host round trip.
+// 8*scale per corner along a stripe), so they are reproduced exactly: every
+// lane replays the running sums it needs with the same float steps. Stripe
+// bounds are closed-form (every stripe but the last is full), so nothing is
+// staged from the host and consecutive frames queue on the stream without a
+// host round trip.
 //
-//  tables_kernel: stripe table and corner sums (one thread per stripe, one
-//                 for the column sums)
 //  coarse_kernel: one lane per corner of every stripe -> depth table
 //  fine_kernel:   one wave per tile; lane = pixel; with stride > 1 only the
 //                 stride-corner lanes trace and the others copy their colour
@@ -45,26 +44,23 @@ struct FrameParams {
     float a11, a12, a21, a22, a31, a32, zx, zy, zz;
     float px, py, pz;
     float lx, ly, lz;  // light direction
-    float aspect, scale, coarseScale;
+    float aspect, scale, coarseScale, tileScale;
     int width, height, tilesX, stripes, stride;
+    int sh;    // rows of a full stripe: (H - 1) / stripes + 1
+    int full;  // corner rows of a full stripe: (sh - 1) / 8 + 2
 };
 
-struct Stripe {
-    int y0, y1, tilesY, pad;
-    int64_t depthBase;  // first corner of this stripe in the depth table
-    int64_t fineBase;   // first tile of this stripe in the fine-tile numbering
-};
+// Stripe s covers rows [s*sh, min((s+1)*sh, H)) (src/Main.cpp:351-361). Every
+// stripe but the last is full (the layout check guarantees it), so stripe s
+// starts at corner s*full*tilesX of the depth table and at fine tile
+// s*(full-1)*(tilesX-1).
+__device__ __forceinline__ int stripe_y0(const FrameParams &p, int s) { return s * p.sh; }
+__device__ __forceinline__ int stripe_y1(const FrameParams &p, int s) { return min((s + 1) * p.sh, p.height); }
 
-__device__ __forceinline__ int stripe_of(const Stripe *__restrict__ st, int n, int64_t i, bool fine) {
-    int lo = 0, hi = n;
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if ((fine ? st[mid].fineBase : st[mid].depthBase) <= i) lo = mid; else hi = mid;
-    }
-    return lo;
-}
 
 // One ray to completion; the leaf word (normal) is read only when Normal is set.
+// The frame kernels keep the whole restart stack in LDS (the fine pass measured
+// slower with the root's entry in registers).
 template <bool Lod, bool Normal, typename Idx>
 __device__ __forceinline__ int trace(const uint32_t *__restrict__ tree, float ox, float oy, float oz, float dx, float dy,
                                      float dz, float rayScale, const Stack<Idx> &stk, uint32_t &normal, float &t) {
@@ -73,7 +69,7 @@ __device__ __forceinline__ int trace(const uint32_t *__restrict__ tree, float ox
     Idx leaf = 0;
     int code;
     do {
-        code = ray_step<Lod, true, Idx, Normal>(r, tree, stk, normal, t, leaf);  // camera rays are finite
+        code = ray_step<Lod, true, Idx, Normal, false>(r, tree, stk, normal, t, leaf);  // camera rays are finite
     } while (code == kRunning);
     return code;
 }
@@ -90,18 +86,21 @@ __device__ __forceinline__ void frame_dir(const FrameParams &p, float dx, float 
 
 template <typename Idx>
 __global__ void __launch_bounds__(64) coarse_kernel(const uint32_t *__restrict__ tree, FrameParams p,
-                                                    const Stripe *__restrict__ st, const float *__restrict__ dxTab,
-                                                    const float *__restrict__ dyTab, int64_t corners, int slots,
-                                                    float *__restrict__ depth) {
+                                                    int64_t corners, int slots, float *__restrict__ depth) {
     extern __shared__ __align__(16) unsigned char lds[];
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= corners) return;
-    const int s = stripe_of(st, p.stripes, i, false);
-    const int64_t local = i - st[s].depthBase;
+    const int64_t perStripe = int64_t(p.full) * p.tilesX;
+    const int s = int(min(i / perStripe, int64_t(p.stripes - 1)));
+    const int64_t local = i - s * perStripe;
     const int ty = int(local / p.tilesX), tx = int(local % p.tilesX);
+    // the reference's running corner sums (src/Main.cpp:156-176), step by step
+    float dx = -1.0f + 0.0f * p.scale;
+    for (int k = 0; k < tx; ++k) dx += p.tileScale;
+    float dy = p.aspect - float(stripe_y0(p, s)) * p.scale;
+    for (int k = 0; k < ty; ++k) dy -= p.tileScale;
     float d[3];
-    // dyTab is indexed by corner row over all stripes (row offset = depthBase / tilesX)
-    frame_dir(p, dxTab[tx], dyTab[st[s].depthBase / p.tilesX + ty], d);
+    frame_dir(p, dx, dy, d);
     Stack<Idx> stk(lds, 64, slots, int(threadIdx.x));
     uint32_t normal = 0;
     float t = 0.0f;
@@ -109,52 +108,22 @@ __global__ void __launch_bounds__(64) coarse_kernel(const uint32_t *__restrict__
     depth[i] = code != march::kMiss ? t : kTreeMiss;
 }
 
-// Stripe i covers rows [i*sh, min((i+1)*sh, H)); every stripe but the last is
-// full (the layout check guarantees it), so its bases follow from i. dy per
-// corner row and dx per corner column are the reference's running sums, taken
-// with the same float steps as its loops.
-__global__ void tables_kernel(FrameParams p, int sh, float tileScale, int64_t corners, int64_t tiles,
-                              Stripe *__restrict__ st, float *__restrict__ dyTab, float *__restrict__ dxTab) {
-    const int full = (sh - 1) / 8 + 2;  // corner rows of a full stripe
-    for (int i = int(blockIdx.x * blockDim.x + threadIdx.x); i <= p.stripes; i += int(gridDim.x * blockDim.x)) {
-        if (i == p.stripes) {
-            Stripe e{};
-            e.depthBase = corners;
-            e.fineBase = tiles;
-            st[i] = e;
-            float dx = -1.0f + 0.0f * p.scale;
-            for (int tx = 0; tx < p.tilesX; ++tx, dx += tileScale) dxTab[tx] = dx;
-            continue;
-        }
-        Stripe s;
-        s.y0 = i * sh;
-        s.y1 = min(s.y0 + sh, p.height);
-        s.tilesY = (s.y1 - s.y0 - 1) / 8 + 2;
-        s.pad = 0;
-        s.depthBase = int64_t(i) * full * p.tilesX;
-        s.fineBase = int64_t(i) * (full - 1) * (p.tilesX - 1);
-        st[i] = s;
-        float dy = p.aspect - float(s.y0) * p.scale;
-        float *row = dyTab + int64_t(i) * full;
-        for (int ty = 0; ty < s.tilesY; ++ty, dy -= tileScale) row[ty] = dy;
-    }
-}
-
 template <typename Idx>
 __global__ void __launch_bounds__(64) fine_kernel(const uint32_t *__restrict__ tree, FrameParams p,
-                                                  const Stripe *__restrict__ st, const float *__restrict__ depth,
-                                                  int slots, uint32_t *__restrict__ rgba) {
+                                                  const float *__restrict__ depth, int slots,
+                                                  uint32_t *__restrict__ rgba) {
     extern __shared__ __align__(16) unsigned char lds[];
     const int64_t tile = blockIdx.x;
-    const int s = stripe_of(st, p.stripes, tile, true);
-    const int64_t local = tile - st[s].fineBase;
+    const int64_t perStripe = int64_t(p.full - 1) * (p.tilesX - 1);
+    const int s = int(min(tile / perStripe, int64_t(p.stripes - 1)));
+    const int64_t local = tile - s * perStripe;
     const int fx = int(local % (p.tilesX - 1)), fy = int(local / (p.tilesX - 1));
     const int tx = fx + 1, ty = fy + 1;  // the reference renders tile (x-1, y-1) at corner (x, y)
-    const int64_t idx = st[s].depthBase + int64_t(ty) * p.tilesX + tx;
+    const int64_t idx = int64_t(s) * p.full * p.tilesX + int64_t(ty) * p.tilesX + tx;
     const float minT = dev::fmin_ref(dev::fmin_ref(depth[idx], depth[idx - 1]),
                                 dev::fmin_ref(depth[idx - p.tilesX], depth[idx - p.tilesX - 1]));
-    const int x0 = fx * 8, y0 = st[s].y0 + fy * 8;
-    const int x1 = min(x0 + 8, p.width), y1 = min(y0 + 8, st[s].y1);
+    const int x0 = fx * 8, y0 = stripe_y0(p, s) + fy * 8;
+    const int x1 = min(x0 + 8, p.width), y1 = min(y0 + 8, stripe_y1(p, s));
     const int lane = threadIdx.x;
     const int px = lane & 7, py = lane >> 3;
     const int x = x0 + px, y = y0 + py;
@@ -256,45 +225,39 @@ int render_frame_device(svo_octree &h, const svo_camera &cam, int width, int hei
     p.tilesX = (width - 1) / 8 + 2;
     p.stripes = stripes;
     p.stride = stride;
-    // corner and fine-tile counts (the tables themselves are made on the device)
-    const int sh = (height - 1) / stripes + 1;
+    p.tileScale = tileScale;
+    p.sh = (height - 1) / stripes + 1;
+    p.full = (p.sh - 1) / 8 + 2;
+    // corner and fine-tile counts over the stripes (the last may be shorter)
     int64_t corners = 0, tiles = 0;
     for (int i = 0; i < stripes; ++i) {
-        const int y0 = i * sh, y1 = std::min(y0 + sh, height);
+        const int y0 = i * p.sh, y1 = std::min(y0 + p.sh, height);
         const int tilesY = (y1 - y0 - 1) / 8 + 2;
         corners += int64_t(tilesY) * p.tilesX;
         tiles += int64_t(tilesY - 1) * (p.tilesX - 1);
     }
     if (tiles > 0x7FFFFFFF) return fail(SVO_ERR_LIMIT, "image too large");
     hipStream_t s = static_cast<hipStream_t>(stream);
-    const size_t rows = size_t(corners / p.tilesX);
-    const size_t stripeBytes = sizeof(Stripe) * size_t(stripes + 1);
-    const size_t tabBytes = stripeBytes + sizeof(float) * (rows + size_t(p.tilesX));
     void *scratch = nullptr;
-    HIP_TRY(hipMallocAsync(&scratch, tabBytes + (depthOut ? 0 : sizeof(float) * size_t(corners)), s));
-    Stripe *dSt = static_cast<Stripe *>(scratch);
-    float *dDy = reinterpret_cast<float *>(static_cast<uint8_t *>(scratch) + stripeBytes);
-    float *dDx = dDy + rows;
-    float *dDepth = depthOut ? depthOut : dDx + p.tilesX;
-    hipLaunchKernelGGL(tables_kernel, dim3(unsigned((stripes + 1 + 255) / 256)), dim3(256), 0, s, p, sh, tileScale,
-                       corners, tiles, dSt, dDy, dDx);
-    const int slots = h.depth > 2 ? h.depth - 1 : 1;
+    if (!depthOut) HIP_TRY(hipMallocAsync(&scratch, sizeof(float) * size_t(corners), s));
+    float *dDepth = depthOut ? depthOut : static_cast<float *>(scratch);
+    const int slots = h.depth > 2 ? h.depth - 1 : 1;  // the whole stack in LDS (RootReg = false)
     const bool wide = h.words.size() >= (uint64_t(1) << 32);
     const size_t lds = size_t(slots) * 64 * ((wide ? 8 : 4) + sizeof(float));
     const unsigned cgrid = unsigned((corners + 63) / 64);
     if (!wide) {
-        hipLaunchKernelGGL(coarse_kernel<uint32_t>, dim3(cgrid), dim3(64), lds, s, h.dwords, p, dSt, dDx, dDy, corners,
-                           slots, dDepth);
-        if (tiles) hipLaunchKernelGGL(fine_kernel<uint32_t>, dim3(unsigned(tiles)), dim3(64), lds, s, h.dwords, p, dSt,
-                                      dDepth, slots, rgba);
+        hipLaunchKernelGGL(coarse_kernel<uint32_t>, dim3(cgrid), dim3(64), lds, s, h.dwords, p, corners, slots, dDepth);
+        if (tiles)
+            hipLaunchKernelGGL(fine_kernel<uint32_t>, dim3(unsigned(tiles)), dim3(64), lds, s, h.dwords, p, dDepth,
+                               slots, rgba);
     } else {
-        hipLaunchKernelGGL(coarse_kernel<uint64_t>, dim3(cgrid), dim3(64), lds, s, h.dwords, p, dSt, dDx, dDy, corners,
-                           slots, dDepth);
-        if (tiles) hipLaunchKernelGGL(fine_kernel<uint64_t>, dim3(unsigned(tiles)), dim3(64), lds, s, h.dwords, p, dSt,
-                                      dDepth, slots, rgba);
+        hipLaunchKernelGGL(coarse_kernel<uint64_t>, dim3(cgrid), dim3(64), lds, s, h.dwords, p, corners, slots, dDepth);
+        if (tiles)
+            hipLaunchKernelGGL(fine_kernel<uint64_t>, dim3(unsigned(tiles)), dim3(64), lds, s, h.dwords, p, dDepth,
+                               slots, rgba);
     }
     hipError_t e = hipGetLastError();
-    (void)hipFreeAsync(scratch, s);
+    if (scratch) (void)hipFreeAsync(scratch, s);
     if (e != hipSuccess) return fail(SVO_ERR_DEVICE, std::string("frame: ") + hipGetErrorString(e));
     return SVO_OK;
 }

@@ -28,7 +28,14 @@ __device__ __forceinline__ float quake_rsqrt(float x) {
     return y * (1.5f - half * y * y);
 }
 
-// Per-lane restart stack in LDS, one entry per scale below the root.
+// LDS stack entries for a tree of `depth` (descriptor levels + 1): one per
+// scale a ray can push from, except the root's, which is always (word 0, the
+// ray's initial maxT: nothing changes maxT while the ray is at the root) and
+// is kept in a register. One entry fewer lets the 2048^3 tree's stacks fit 32
+// waves per CU in 160 KiB of LDS.
+__host__ __device__ inline int lds_slots(int depth) { return depth > 2 ? depth - 2 : 1; }
+
+// Per-lane restart stack in LDS, one entry per scale below the root's children.
 // 64-bit word indices: [slot][stride] indices then [slot][stride] maxT.
 template <typename Idx>
 struct Stack {
@@ -73,7 +80,7 @@ struct Stack<uint32_t> {
 template <typename Idx>
 struct Ray {
     float dTx, dTy, dTz, bTx, bTy, bTz;
-    float minT, maxT, posX, posY, posZ, scaleExp2, rayScale;
+    float minT, maxT, maxT0, posX, posY, posZ, scaleExp2, rayScale;  // maxT0: the root's maxT
     uint32_t current;
     Idx parent;
     int octantMask, idx, scale;
@@ -104,6 +111,7 @@ __device__ __forceinline__ void ray_init(Ray<Idx> &r, const uint32_t *__restrict
     r.maxT = smin(r.dTx - r.bTx, smin(r.dTy - r.bTy, r.dTz - r.bTz));
     minT = smax(minT, 0.0f);
     r.minT = minT;
+    r.maxT0 = r.maxT;
     r.current = tree[0];
     r.parent = 0;
     r.idx = 0;
@@ -139,12 +147,15 @@ __device__ __forceinline__ bool ray_defined(const Ray<Idx> &r) {
 //    maxTC * rayScale >= scaleExp2 (> 0) never holds and the test is dropped.
 //  Normal = false: the caller needs only the hit code and t; a leaf hit
 //    leaves `leaf` and `normal` unset and reads no leaf word.
+//  RootReg = true: the root's stack entry is kept in registers and the LDS
+//    stack holds lds_slots(depth) entries; false: every entry is in LDS
+//    (depth - 1 entries), for kernels where that measured faster.
 //  Finite = true: the ray's origin and direction are finite (|origin| below
 //    2^100). Then every corner t is finite and never -0 (pos*dT is a nonzero
 //    product; a difference of equal nonzero values is +0), so the minimum of
 //    the three corner times does not depend on the order of ties or on NaN
 //    handling and one hardware min3 returns the reference's std::min result.
-template <bool Lod, bool Finite, typename Idx, bool Normal = true>
+template <bool Lod, bool Finite, typename Idx, bool Normal = true, bool RootReg = true>
 __device__ __forceinline__ int ray_step(Ray<Idx> &r, const uint32_t *__restrict__ tree, const Stack<Idx> &stk,
                                         uint32_t &normal, float &tOut, Idx &leaf) {
     const float cornerTX = r.posX * r.dTx - r.bTx;
@@ -179,11 +190,13 @@ __device__ __forceinline__ int ray_step(Ray<Idx> &r, const uint32_t *__restrict_
                 tOut = r.minT;
                 return kLeaf;
             }
-            // slot < stk.slots: a node at level L (slot L) pushes only when it has a
-            // non-leaf child, and octree_validate then counted depth >= L + 2 =
-            // slots + 1 (every loaded or built tree is validated)
+            // slot - 1 < stk.slots: a node at level L (slot L) pushes only when it
+            // has a non-leaf child, and octree_validate then counted depth >= L + 2
+            // (every loaded or built tree is validated); the root's entry (slot 0)
+            // is (0, maxT0) and is not stored
             const int slot = (kMaxScale - 1) - r.scale;
-            stk.push(slot, r.parent, r.maxT);
+            if (!RootReg) stk.push(slot, r.parent, r.maxT);
+            else if (slot != 0) stk.push(slot - 1, r.parent, r.maxT);
 
             const Idx siblingCount = Idx(__popc(childMasks & 127u));
             r.parent += childOffset + siblingCount;
@@ -220,7 +233,14 @@ __device__ __forceinline__ int ray_step(Ray<Idx> &r, const uint32_t *__restrict_
         // leaves the loop; that read cannot reach the result
         if (r.scale >= kMaxScale) return kMiss;
         const int slot = (kMaxScale - 1) - r.scale;  // a slot pushed earlier: the new scale is above the current one
-        stk.pop(slot, r.parent, r.maxT);
+        if (!RootReg) {
+            stk.pop(slot, r.parent, r.maxT);
+        } else if (slot != 0) {
+            stk.pop(slot - 1, r.parent, r.maxT);
+        } else {
+            r.parent = 0;
+            r.maxT = r.maxT0;
+        }
         r.current = tree[r.parent];
 
         const uint32_t shX = f2u(r.posX) >> r.scale;

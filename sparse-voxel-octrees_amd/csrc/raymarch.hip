@@ -150,7 +150,7 @@ __global__ void __launch_bounds__(64) shadow_kernel(const uint32_t *__restrict__
     shadow[i] = (code == 1 || code == 2) ? 1 : 0;
 }
 
-int stack_slots(const svo_octree &h) { return h.depth > 2 ? h.depth - 1 : 1; }
+int stack_slots(const svo_octree &h) { return lds_slots(h.depth); }
 
 }  // namespace
 

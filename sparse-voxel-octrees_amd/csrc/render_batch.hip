@@ -308,7 +308,7 @@ int plan_views(const svo_view *views, int nviews, const svo_batch_opts *opts, in
     if (tiles >= (uint64_t(kQueues) << 32)) return fail(SVO_ERR_LIMIT, "too many tiles for one launch");
     plan.nviews = nviews;
     plan.tiles = tiles;
-    plan.slots = maxDepth > 2 ? maxDepth - 1 : 1;
+    plan.slots = lds_slots(maxDepth);
     plan.wide = wide;
     plan.lod = lod;
     const size_t ldsPerWave = size_t(plan.slots) * 64 * ((wide ? 8 : 4) + sizeof(float));
