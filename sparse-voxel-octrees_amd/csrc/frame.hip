@@ -69,7 +69,7 @@ __device__ __forceinline__ int trace(const uint32_t *__restrict__ tree, float ox
     Idx leaf = 0;
     int code;
     do {
-        code = ray_step<Lod, true, Idx, Normal, false>(r, tree, stk, normal, t, leaf);  // camera rays are finite
+        code = ray_step<Lod, true, Idx, Normal, 0>(r, tree, stk, normal, t, leaf);  // camera rays are finite
     } while (code == kRunning);
     return code;
 }
@@ -241,7 +241,7 @@ int render_frame_device(svo_octree &h, const svo_camera &cam, int width, int hei
     void *scratch = nullptr;
     if (!depthOut) HIP_TRY(hipMallocAsync(&scratch, sizeof(float) * size_t(corners), s));
     float *dDepth = depthOut ? depthOut : static_cast<float *>(scratch);
-    const int slots = h.depth > 2 ? h.depth - 1 : 1;  // the whole stack in LDS (RootReg = false)
+    const int slots = lds_slots(h.depth, 0);  // the whole stack in LDS (Reg = 0)
     const bool wide = h.words.size() >= (uint64_t(1) << 32);
     const size_t lds = size_t(slots) * 64 * ((wide ? 8 : 4) + sizeof(float));
     const unsigned cgrid = unsigned((corners + 63) / 64);

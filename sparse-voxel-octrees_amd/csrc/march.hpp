@@ -28,12 +28,16 @@ __device__ __forceinline__ float quake_rsqrt(float x) {
     return y * (1.5f - half * y * y);
 }
 
-// LDS stack entries for a tree of `depth` (descriptor levels + 1): one per
-// scale a ray can push from, except the root's, which is always (word 0, the
-// ray's initial maxT: nothing changes maxT while the ray is at the root) and
-// is kept in a register. One entry fewer lets the 2048^3 tree's stacks fit 32
-// waves per CU in 160 KiB of LDS.
-__host__ __device__ inline int lds_slots(int depth) { return depth > 2 ? depth - 2 : 1; }
+// The restart stack has one entry per scale a ray can push from: depth - 1
+// for a tree of `depth` (descriptor levels + 1). The top `reg` entries live in
+// registers instead of LDS: entry 0, the root's, is always (word 0, the ray's
+// initial maxT: nothing changes maxT while the ray is at the root), entries 1
+// and 2 are register pairs. 160 KiB of LDS holds 32 waves' stacks of 10
+// entries (8 bytes per lane), so trees up to depth 12 keep one entry in
+// registers and deeper ones three (the 8192^3 terrain, depth 14: 26 -> 32
+// waves per CU).
+__host__ __device__ inline int reg_entries(int depth) { return depth > 12 ? 3 : 1; }
+__host__ __device__ inline int lds_slots(int depth, int reg) { return depth - 1 - reg > 0 ? depth - 1 - reg : 1; }
 
 // Per-lane restart stack in LDS, one entry per scale below the root's children.
 // 64-bit word indices: [slot][stride] indices then [slot][stride] maxT.
@@ -81,6 +85,8 @@ template <typename Idx>
 struct Ray {
     float dTx, dTy, dTz, bTx, bTy, bTz;
     float minT, maxT, maxT0, posX, posY, posZ, scaleExp2, rayScale;  // maxT0: the root's maxT
+    Idx parent1, parent2;  // stack entries 1 and 2 when they are held in registers
+    float maxT1, maxT2;
     uint32_t current;
     Idx parent;
     int octantMask, idx, scale;
@@ -147,15 +153,15 @@ __device__ __forceinline__ bool ray_defined(const Ray<Idx> &r) {
 //    maxTC * rayScale >= scaleExp2 (> 0) never holds and the test is dropped.
 //  Normal = false: the caller needs only the hit code and t; a leaf hit
 //    leaves `leaf` and `normal` unset and reads no leaf word.
-//  RootReg = true: the root's stack entry is kept in registers and the LDS
-//    stack holds lds_slots(depth) entries; false: every entry is in LDS
-//    (depth - 1 entries), for kernels where that measured faster.
+//  Reg: how many top stack entries are held in registers (0, 1 or 3); the
+//    LDS stack holds the others, entry e at LDS slot e - Reg. 0 keeps every
+//    entry in LDS, for kernels where that measured faster.
 //  Finite = true: the ray's origin and direction are finite (|origin| below
 //    2^100). Then every corner t is finite and never -0 (pos*dT is a nonzero
 //    product; a difference of equal nonzero values is +0), so the minimum of
 //    the three corner times does not depend on the order of ties or on NaN
 //    handling and one hardware min3 returns the reference's std::min result.
-template <bool Lod, bool Finite, typename Idx, bool Normal = true, bool RootReg = true>
+template <bool Lod, bool Finite, typename Idx, bool Normal = true, int Reg = 1>
 __device__ __forceinline__ int ray_step(Ray<Idx> &r, const uint32_t *__restrict__ tree, const Stack<Idx> &stk,
                                         uint32_t &normal, float &tOut, Idx &leaf) {
     const float cornerTX = r.posX * r.dTx - r.bTx;
@@ -190,13 +196,21 @@ __device__ __forceinline__ int ray_step(Ray<Idx> &r, const uint32_t *__restrict_
                 tOut = r.minT;
                 return kLeaf;
             }
-            // slot - 1 < stk.slots: a node at level L (slot L) pushes only when it
-            // has a non-leaf child, and octree_validate then counted depth >= L + 2
-            // (every loaded or built tree is validated); the root's entry (slot 0)
-            // is (0, maxT0) and is not stored
+            // slot - Reg < stk.slots: a node at level L (slot L) pushes only when
+            // it has a non-leaf child, and octree_validate then counted depth >=
+            // L + 2 (every loaded or built tree is validated); the root's entry
+            // (slot 0) is (0, maxT0) and is not stored
             const int slot = (kMaxScale - 1) - r.scale;
-            if (!RootReg) stk.push(slot, r.parent, r.maxT);
-            else if (slot != 0) stk.push(slot - 1, r.parent, r.maxT);
+            static_assert(Reg == 0 || Reg == 1 || Reg == 3, "register-held stack entries: 0, 1 or 3");
+            if (slot >= Reg) {
+                stk.push(slot - Reg, r.parent, r.maxT);
+            } else if (Reg == 3 && slot == 1) {
+                r.parent1 = r.parent;
+                r.maxT1 = r.maxT;
+            } else if (Reg == 3 && slot == 2) {
+                r.parent2 = r.parent;
+                r.maxT2 = r.maxT;
+            }
 
             const Idx siblingCount = Idx(__popc(childMasks & 127u));
             r.parent += childOffset + siblingCount;
@@ -233,13 +247,17 @@ __device__ __forceinline__ int ray_step(Ray<Idx> &r, const uint32_t *__restrict_
         // leaves the loop; that read cannot reach the result
         if (r.scale >= kMaxScale) return kMiss;
         const int slot = (kMaxScale - 1) - r.scale;  // a slot pushed earlier: the new scale is above the current one
-        if (!RootReg) {
-            stk.pop(slot, r.parent, r.maxT);
-        } else if (slot != 0) {
-            stk.pop(slot - 1, r.parent, r.maxT);
-        } else {
+        if (slot >= Reg) {
+            stk.pop(slot - Reg, r.parent, r.maxT);
+        } else if (slot == 0) {
             r.parent = 0;
             r.maxT = r.maxT0;
+        } else if (Reg == 3 && slot == 1) {
+            r.parent = r.parent1;
+            r.maxT = r.maxT1;
+        } else if (Reg == 3) {
+            r.parent = r.parent2;
+            r.maxT = r.maxT2;
         }
         r.current = tree[r.parent];
 

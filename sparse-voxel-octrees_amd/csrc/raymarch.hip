@@ -33,7 +33,7 @@ using namespace march;
 
 // Runs one ray to completion. Returns 0 miss, 1 leaf hit, 2 LOD hit. With
 // Normal set, a leaf hit also gives the leaf's word index and word.
-template <bool Lod, bool Finite, typename Idx, bool Normal = true>
+template <bool Lod, bool Finite, typename Idx, bool Normal = true, int Reg = 1>
 __device__ __forceinline__ int march_ray(const uint32_t *__restrict__ tree, float ox, float oy, float oz, float dx,
                                          float dy, float dz, float rayScale, const Stack<Idx> &stk, uint32_t &normal,
                                          float &tOut, Idx &leaf) {
@@ -42,7 +42,7 @@ __device__ __forceinline__ int march_ray(const uint32_t *__restrict__ tree, floa
     if (!Finite && !ray_defined(r)) return kMiss;  // the reference never ends such a ray
     int code;
     do {
-        code = ray_step<Lod, Finite, Idx, Normal>(r, tree, stk, normal, tOut, leaf);
+        code = ray_step<Lod, Finite, Idx, Normal, Reg>(r, tree, stk, normal, tOut, leaf);
     } while (code == kRunning);
     return code;
 }
@@ -97,7 +97,7 @@ __device__ __forceinline__ void primary_dir(const PrimaryParams &p, int x, int y
 }
 
 // 8x8 pixel tile per wave64; lanes raster the tile row by row.
-template <typename Idx>
+template <typename Idx, int Reg>
 __global__ void __launch_bounds__(64) primary_kernel(const uint32_t *__restrict__ tree, PrimaryParams p,
                                                      int slots, uint8_t *hit, uint32_t *nrm, float *tt,
                                                      uint64_t *lf) {
@@ -113,7 +113,8 @@ __global__ void __launch_bounds__(64) primary_kernel(const uint32_t *__restrict_
     uint32_t normal = 0;
     float t = 0.0f;
     Idx leaf = 0;
-    int code = march_ray<false, true, Idx>(tree, p.px, p.py, p.pz, d[0], d[1], d[2], 0.0f, stk, normal, t, leaf);
+    int code = march_ray<false, true, Idx, true, Reg>(tree, p.px, p.py, p.pz, d[0], d[1], d[2], 0.0f, stk, normal, t,
+                                                      leaf);
     store_result<Idx>(code, normal, t, leaf, int64_t(y) * p.width + x, hit, nrm, tt, lf);
 }
 
@@ -122,7 +123,7 @@ __global__ void __launch_bounds__(64) primary_kernel(const uint32_t *__restrict_
 // no LOD cut-off. shadow = 1 when it hits any voxel, 0 when it escapes or the
 // pixel had no primary hit. Same 8x8 tiles as the primary pass, so a wave's
 // shadow rays start from neighbouring surface points.
-template <typename Idx>
+template <typename Idx, int Reg>
 __global__ void __launch_bounds__(64) shadow_kernel(const uint32_t *__restrict__ tree, PrimaryParams p, float lx,
                                                     float ly, float lz, float eps, int slots,
                                                     const uint8_t *__restrict__ hit, const float *__restrict__ tt,
@@ -145,12 +146,12 @@ __global__ void __launch_bounds__(64) shadow_kernel(const uint32_t *__restrict__
     uint32_t normal = 0;
     float t = 0.0f;
     Idx leaf = 0;
-    const int code = march_ray<false, true, Idx, false>(tree, p.px + d[0] * back, p.py + d[1] * back, p.pz + d[2] * back, lx,
+    const int code = march_ray<false, true, Idx, false, Reg>(tree, p.px + d[0] * back, p.py + d[1] * back, p.pz + d[2] * back, lx,
                                                  ly, lz, 0.0f, stk, normal, t, leaf);
     shadow[i] = (code == 1 || code == 2) ? 1 : 0;
 }
 
-int stack_slots(const svo_octree &h) { return lds_slots(h.depth); }
+int stack_slots(const svo_octree &h, int reg) { return lds_slots(h.depth, reg); }
 
 }  // namespace
 
@@ -199,7 +200,7 @@ int raymarch_device(svo_octree &h, int64_t n, const float *o, const float *d, co
     int r = ensure_device(h);
     if (r) return r;
     const int threads = 256;
-    const int slots = stack_slots(h);
+    const int slots = stack_slots(h, 1);
     const bool wide = h.words.size() >= (uint64_t(1) << 32);
     const size_t lds = size_t(slots) * threads * ((wide ? 8 : 4) + sizeof(float));
     const int64_t blocks = (n + threads - 1) / threads;
@@ -247,15 +248,20 @@ int render_primary_device(svo_octree &h, const svo_camera &cam, int width, int h
     if (r) return r;
     const PrimaryParams p = primary_params(cam, width, height);
     const int tilesY = (height + 7) / 8;
-    const int slots = stack_slots(h);
+    const int reg = reg_entries(h.depth);
+    const int slots = stack_slots(h, reg);
     const bool wide = h.words.size() >= (uint64_t(1) << 32);
     const size_t lds = size_t(slots) * 64 * ((wide ? 8 : 4) + sizeof(float));
     hipStream_t s = static_cast<hipStream_t>(stream);
     dim3 grid(unsigned(p.tilesX * tilesY));
-    if (!wide)
-        hipLaunchKernelGGL(primary_kernel<uint32_t>, grid, dim3(64), lds, s, h.dwords, p, slots, hit, normal, t, leaf);
+    if (!wide && reg == 1)
+        hipLaunchKernelGGL((primary_kernel<uint32_t, 1>), grid, dim3(64), lds, s, h.dwords, p, slots, hit, normal, t, leaf);
+    else if (!wide)
+        hipLaunchKernelGGL((primary_kernel<uint32_t, 3>), grid, dim3(64), lds, s, h.dwords, p, slots, hit, normal, t, leaf);
+    else if (reg == 1)
+        hipLaunchKernelGGL((primary_kernel<uint64_t, 1>), grid, dim3(64), lds, s, h.dwords, p, slots, hit, normal, t, leaf);
     else
-        hipLaunchKernelGGL(primary_kernel<uint64_t>, grid, dim3(64), lds, s, h.dwords, p, slots, hit, normal, t, leaf);
+        hipLaunchKernelGGL((primary_kernel<uint64_t, 3>), grid, dim3(64), lds, s, h.dwords, p, slots, hit, normal, t, leaf);
     HIP_TRY(hipGetLastError());
     return SVO_OK;
 }
@@ -273,17 +279,24 @@ int render_shadow_device(svo_octree &h, const svo_camera &cam, int width, int he
     if (r) return r;
     const PrimaryParams p = primary_params(cam, width, height);
     const int tilesY = (height + 7) / 8;
-    const int slots = stack_slots(h);
+    const int reg = reg_entries(h.depth);
+    const int slots = stack_slots(h, reg);
     const bool wide = h.words.size() >= (uint64_t(1) << 32);
     const size_t lds = size_t(slots) * 64 * ((wide ? 8 : 4) + sizeof(float));
     hipStream_t s = static_cast<hipStream_t>(stream);
     dim3 grid(unsigned(p.tilesX * tilesY));
-    if (!wide)
-        hipLaunchKernelGGL(shadow_kernel<uint32_t>, grid, dim3(64), lds, s, h.dwords, p, light[0], light[1], light[2], eps,
-                           slots, hit, t, shadow);
+    if (!wide && reg == 1)
+        hipLaunchKernelGGL((shadow_kernel<uint32_t, 1>), grid, dim3(64), lds, s, h.dwords, p, light[0], light[1],
+                           light[2], eps, slots, hit, t, shadow);
+    else if (!wide)
+        hipLaunchKernelGGL((shadow_kernel<uint32_t, 3>), grid, dim3(64), lds, s, h.dwords, p, light[0], light[1],
+                           light[2], eps, slots, hit, t, shadow);
+    else if (reg == 1)
+        hipLaunchKernelGGL((shadow_kernel<uint64_t, 1>), grid, dim3(64), lds, s, h.dwords, p, light[0], light[1],
+                           light[2], eps, slots, hit, t, shadow);
     else
-        hipLaunchKernelGGL(shadow_kernel<uint64_t>, grid, dim3(64), lds, s, h.dwords, p, light[0], light[1], light[2], eps,
-                           slots, hit, t, shadow);
+        hipLaunchKernelGGL((shadow_kernel<uint64_t, 3>), grid, dim3(64), lds, s, h.dwords, p, light[0], light[1],
+                           light[2], eps, slots, hit, t, shadow);
     HIP_TRY(hipGetLastError());
     return SVO_OK;
 }

@@ -70,8 +70,9 @@ __device__ __forceinline__ uint32_t even_bits(uint32_t m) {  // bits 0,2,4 -> 0,
 }
 
 // Lod = false when every view's rayScale is 0 (the LOD test can never fire);
-// camera rays are finite (cameras are checked on the host).
-template <bool Lod, typename Idx>
+// camera rays are finite (cameras are checked on the host). Reg: top stack
+// entries held in registers, for the deepest octree of the batch.
+template <bool Lod, typename Idx, int Reg>
 __global__ void __launch_bounds__(256) batch_kernel(const ViewDesc *__restrict__ views,
                                                     const uint64_t *__restrict__ tileBase, int nviews,
                                                     uint32_t *__restrict__ counters, int slots, int refill) {
@@ -169,7 +170,7 @@ __global__ void __launch_bounds__(256) batch_kernel(const ViewDesc *__restrict__
                 uint32_t normal = 0;
                 float t = 0.0f;
                 Idx leaf = 0;
-                int code = ray_step<Lod, true>(r, tree, stk, normal, t, leaf);
+                int code = ray_step<Lod, true, Idx, true, Reg>(r, tree, stk, normal, t, leaf);
                 if (code != kRunning) {
                     const ViewDesc &vd = views[rView];
                     if (vd.hit) vd.hit[rPix] = uint8_t(code);
@@ -253,6 +254,7 @@ struct LaunchPlan {
     int nviews = 0;
     uint64_t tiles = 0;
     int slots = 1;
+    int reg = 1;  // stack entries held in registers
     bool wide = false, lod = false;
     size_t ldsPerBlock = 0;
     unsigned grid = 0;
@@ -308,7 +310,8 @@ int plan_views(const svo_view *views, int nviews, const svo_batch_opts *opts, in
     if (tiles >= (uint64_t(kQueues) << 32)) return fail(SVO_ERR_LIMIT, "too many tiles for one launch");
     plan.nviews = nviews;
     plan.tiles = tiles;
-    plan.slots = lds_slots(maxDepth);
+    plan.reg = reg_entries(maxDepth);
+    plan.slots = lds_slots(maxDepth, plan.reg);
     plan.wide = wide;
     plan.lod = lod;
     const size_t ldsPerWave = size_t(plan.slots) * 64 * ((wide ? 8 : 4) + sizeof(float));
@@ -327,21 +330,28 @@ int plan_views(const svo_view *views, int nviews, const svo_batch_opts *opts, in
 
 // Queue reset and the persistent kernel (no memset: the counters are cleared
 // by a kernel, so the pair can be captured in a graph).
+template <int Reg>
+void launch_batch(const LaunchPlan &p, const ViewDesc *dViews, const uint64_t *dTileBase, uint32_t *dCounters,
+                  hipStream_t s) {
+    if (!p.wide && !p.lod)
+        hipLaunchKernelGGL((batch_kernel<false, uint32_t, Reg>), dim3(p.grid), dim3(256), p.ldsPerBlock, s, dViews,
+                           dTileBase, p.nviews, dCounters, p.slots, p.refill);
+    else if (!p.wide)
+        hipLaunchKernelGGL((batch_kernel<true, uint32_t, Reg>), dim3(p.grid), dim3(256), p.ldsPerBlock, s, dViews,
+                           dTileBase, p.nviews, dCounters, p.slots, p.refill);
+    else if (!p.lod)
+        hipLaunchKernelGGL((batch_kernel<false, uint64_t, Reg>), dim3(p.grid), dim3(256), p.ldsPerBlock, s, dViews,
+                           dTileBase, p.nviews, dCounters, p.slots, p.refill);
+    else
+        hipLaunchKernelGGL((batch_kernel<true, uint64_t, Reg>), dim3(p.grid), dim3(256), p.ldsPerBlock, s, dViews,
+                           dTileBase, p.nviews, dCounters, p.slots, p.refill);
+}
+
 void launch_plan(const LaunchPlan &p, const ViewDesc *dViews, const uint64_t *dTileBase, uint32_t *dCounters,
                  hipStream_t s) {
     hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, s, dCounters);
-    if (!p.wide && !p.lod)
-        hipLaunchKernelGGL((batch_kernel<false, uint32_t>), dim3(p.grid), dim3(256), p.ldsPerBlock, s, dViews, dTileBase,
-                           p.nviews, dCounters, p.slots, p.refill);
-    else if (!p.wide)
-        hipLaunchKernelGGL((batch_kernel<true, uint32_t>), dim3(p.grid), dim3(256), p.ldsPerBlock, s, dViews, dTileBase,
-                           p.nviews, dCounters, p.slots, p.refill);
-    else if (!p.lod)
-        hipLaunchKernelGGL((batch_kernel<false, uint64_t>), dim3(p.grid), dim3(256), p.ldsPerBlock, s, dViews, dTileBase,
-                           p.nviews, dCounters, p.slots, p.refill);
-    else
-        hipLaunchKernelGGL((batch_kernel<true, uint64_t>), dim3(p.grid), dim3(256), p.ldsPerBlock, s, dViews, dTileBase,
-                           p.nviews, dCounters, p.slots, p.refill);
+    if (p.reg == 3) launch_batch<3>(p, dViews, dTileBase, dCounters, s);
+    else launch_batch<1>(p, dViews, dTileBase, dCounters, s);
 }
 
 }  // namespace
