@@ -164,6 +164,7 @@ int device_count() {
 int ensure_device(svo_octree &h) {
     int dev = 0;
     HIP_TRY(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(h.upload);
     if (h.dwords && h.device == dev) return SVO_OK;
     release_device(h);
     size_t bytes = h.words.size() * sizeof(uint32_t);

@@ -3,6 +3,7 @@
 
 #include <cstdint>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <utility>
@@ -61,6 +62,7 @@ struct svo_octree {
     int device = -1;              // device holding `dwords`
     uint32_t *dwords = nullptr;   // device copy of `words`
     svo_build_stats build{};      // counters of the GPU build that made this tree (zeros otherwise)
+    std::mutex upload;            // serialises the first upload (ensure_device) of concurrent callers
 };
 
 struct svo_renderer;     // render_batch.hip

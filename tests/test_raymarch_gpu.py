@@ -95,6 +95,37 @@ def test_render_primary_matches_oracle(trees, name, w, h, pitch, yaw):
     _same(mine, ref, f"{name} {w}x{h}")
 
 
+def test_concurrent_first_use_of_a_handle(svo, golden):
+    """The reference's viewer raymarches one tree from 16 threads: here 8
+    threads render and trace single rays on a fresh (not yet uploaded) handle
+    at once, and each result equals a sequential one."""
+    import threading
+    t = svo.VoxelOctree(os.path.join(golden, "dragon.oct"))
+    cams = [t.orbit_camera(10.0 * k, 25.0 * k, 1.0) for k in range(8)]
+    rng = np.random.default_rng(9)
+    o = rng.uniform(-0.5, 3.5, (8, 3)).astype(np.float32)
+    d = (rng.uniform(0.9, 2.1, (8, 3)).astype(np.float32) - o).astype(np.float32)
+    renders, rays, errors = [None] * 8, [None] * 8, []
+
+    def work(k):
+        try:
+            renders[k] = t.render_primary(cams[k], 96, 64)
+            rays[k] = t.raymarch(o[k], d[k], 0.0)
+        except Exception as e:  # reported below
+            errors.append(repr(e))
+
+    print("[gpu] 8 threads on a fresh handle", flush=True)
+    threads = [threading.Thread(target=work, args=(k,)) for k in range(8)]
+    for x in threads:
+        x.start()
+    for x in threads:
+        x.join()
+    assert not errors, errors
+    for k in range(8):
+        _same(renders[k], t.render_primary(cams[k], 96, 64), f"camera {k}")
+        assert rays[k] == t.raymarch(o[k], d[k], 0.0)
+
+
 def test_single_ray_entry_matches_batch(trees):
     t = trees["dragon"]
     rng = np.random.default_rng(5)
