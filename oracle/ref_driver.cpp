@@ -10,7 +10,8 @@
 //       PlyLoader -> VoxelData(loader, res, mem) -> VoxelOctree -> save
 //       (src/Main.cpp:322-327, in-memory builder mode)
 //   convert     <in.ply> <resolution> <threads> <mem_bytes> <out.voxel>
-//       PlyLoader::convertToVolume (src/PlyLoader.cpp:506-534)
+//       PlyLoader::convertToVolume (src/PlyLoader.cpp:506-534), its loop restated
+//       on the reference's members (the function itself dies with SIGFPE)
 //   build-voxel <in.voxel> <threads> <mem_bytes> <out.oct>
 //       VoxelData(path, mem) -> VoxelOctree -> save (src/Main.cpp:315-321)
 //   resave      <in.oct> <out.oct>
@@ -92,10 +93,38 @@ int main(int argc, char **argv) {
         return 0;
     }
     if (cmd == "convert" && argc == 7) {
+        // PlyLoader::convertToVolume (src/PlyLoader.cpp:506-534) restated on the
+        // reference's own public members. The function itself never calls
+        // isBlockEmpty, so processBlock's progress line divides by a zero
+        // _numNonZeroBlocks (PlyLoader.cpp:466) and the process dies with SIGFPE
+        // before the first slab is written. Here isBlockEmpty runs once per slab
+        // first (it only feeds that counter), and the slab buffer is zeroed
+        // (the reference's is uninitialised memory, zero when freshly mapped);
+        // the voxels come from the reference's setupBlockProcessing and
+        // processBlock, slab after slab into the same buffer, as there.
         ThreadUtils::startThreads(atoi(argv[4]));
+        const int maxSize = atoi(argv[3]);
         size_t mem = size_t(strtoull(argv[5], 0, 10));
-        std::unique_ptr<PlyLoader> loader(new PlyLoader(argv[2]));
-        loader->convertToVolume(argv[6], atoi(argv[3]), mem);
+        PlyLoader loader(argv[2]);
+        int w, h, d;
+        loader.suggestedDimensions(maxSize, w, h, d);
+        size_t sliceCost = loader.blockMemRequirement(w, h, 1);
+        int sliceZ = std::min((int)(mem / sliceCost), d);
+        if (sliceZ == 0) return 1;
+        FILE *fp = fopen(argv[6], "wb");
+        if (!fp) return 1;
+        std::unique_ptr<uint32[]> data(new uint32[size_t(w) * size_t(h) * size_t(sliceZ)]());
+        loader.setupBlockProcessing(maxSize, w, h, sliceZ, w, h, d);
+        for (int z = 0; z < d; z += sliceZ) loader.isBlockEmpty(0, 0, z);
+        fwrite(&w, 4, 1, fp);
+        fwrite(&h, 4, 1, fp);
+        fwrite(&d, 4, 1, fp);
+        for (int z = 0; z < d; z += sliceZ) {
+            loader.processBlock(data.get(), 0, 0, z, w, h, sliceZ);
+            fwrite(data.get(), sizeof(uint32), size_t(w) * size_t(h) * size_t(std::min(sliceZ, d - z)), fp);
+        }
+        fclose(fp);
+        loader.teardownBlockProcessing();
         return 0;
     }
     if (cmd == "build-voxel" && argc == 6) {

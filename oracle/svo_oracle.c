@@ -1094,13 +1094,12 @@ static void best_partition(int *w, int *h, int *d, int threads) {
 }
 
 /* PlyLoader::setupBlockProcessing + buildBlockLists, PlyLoader.cpp:272-294,407-437 */
-static int setup_blocks(vctx *c, int resolution, int threads) {
-    int bs = (int)c->cache;
-    c->counts = (uint8_t *)calloc((size_t)bs * bs * bs, 1);
+static int setup_blocks_dims(vctx *c, int resolution, int threads, int bw, int bh, int bd) {
+    c->counts = (uint8_t *)calloc((size_t)bw * bh * bd, 1);
     c->side = resolution - 2;
-    c->blkW = c->sbW = bs;
-    c->blkH = c->sbH = bs;
-    c->blkD = c->sbD = bs;
+    c->blkW = c->sbW = bw;
+    c->blkH = c->sbH = bh;
+    c->blkD = c->sbD = bd;
     best_partition(&c->sbW, &c->sbH, &c->sbD, threads);
     c->pW = c->blkW / c->sbW;
     c->pH = c->blkH / c->sbH;
@@ -1118,6 +1117,12 @@ static int setup_blocks(vctx *c, int resolution, int threads) {
     for (uint64_t i = nc; i >= 1; --i) c->blockOffsets[i] = c->blockOffsets[i - 1];
     c->blockOffsets[0] = 0;
     return c->err ? -1 : 0;
+}
+
+/* VoxelData's mesh source: cubic cache blocks of edge `cache` */
+static int setup_blocks(vctx *c, int resolution, int threads) {
+    int bs = (int)c->cache;
+    return setup_blocks_dims(c, resolution, threads, bs, bs, bs);
 }
 
 /* VoxelData::init, src/VoxelData.cpp:211-262 */
@@ -1451,6 +1456,42 @@ int oracle_voxelize_mesh(oracle_mesh *m, int resolution, int threads, uint64_t m
     }
     vctx_free(&c);
     return r;
+}
+
+/* PlyLoader::convertToVolume, PlyLoader.cpp:506-534 */
+int oracle_convert_to_volume(oracle_mesh *m, int maxSize, int threads, uint64_t mem, uint32_t **out,
+                             int dims[3], int *sliceZ) {
+    vctx c;
+    memset(&c, 0, sizeof(c));
+    c.mesh = m;
+    *out = NULL;
+    suggested_dims(m, maxSize, dims);
+    const int w = dims[0], h = dims[1], d = dims[2];
+    c.dataW = w; c.dataH = h; c.dataD = d;
+    /* blockMemRequirement(w, h, 1): one count byte per voxel of a slice */
+    const uint64_t slices = mem / ((uint64_t)w * (uint64_t)h);
+    const int sz = slices < (uint64_t)d ? (int)slices : d;
+    *sliceZ = sz;
+    if (sz == 0) return -2; /* the reference asserts "Not enough memory available for single slice" */
+    const size_t slab = (size_t)w * h * sz, plane = (size_t)w * h;
+    int r = setup_blocks_dims(&c, maxSize, threads, w, h, sz);
+    /* the slab buffer starts zeroed (a fresh allocation of that size is) and is
+     * reused for every slab without clearing, as are the counts: cells a slab
+     * does not write keep the previous slab's words, cells it writes average
+     * with them */
+    uint32_t *data = r == 0 ? (uint32_t *)calloc(slab, 4) : NULL;
+    uint32_t *vol = r == 0 ? (uint32_t *)malloc(plane * (size_t)d * 4) : NULL;
+    if (r == 0 && (!data || !vol)) r = -4;
+    for (int z = 0; r == 0 && z < d; z += sz) {
+        process_block(&c, data, 0, 0, z, w, h, sz);
+        if (c.err) { r = -3; break; }
+        memcpy(vol + plane * (size_t)z, data, plane * (size_t)imin(sz, d - z) * 4);
+    }
+    free(data);
+    vctx_free(&c);
+    if (r) { free(vol); return r; }
+    *out = vol;
+    return 0;
 }
 
 void oracle_free(void *p) { free(p); }

@@ -136,6 +136,13 @@ int oracle_build_grid(const uint32_t *grid, int w, int h, int d, uint64_t mem, u
 int oracle_voxelize_mesh(oracle_mesh *m, int resolution, int threads, uint64_t mem,
                          uint32_t *out, int dims[3]);
 
+/* PlyLoader::convertToVolume (src/PlyLoader.cpp:506-534): the .voxel body
+ * (dims[0]*dims[1]*dims[2] uint32, malloc'ed into *out) the reference writes
+ * after its w, h, d header. *sliceZ = slices per slab. Returns -2 when one
+ * slice does not fit the memory budget. */
+int oracle_convert_to_volume(oracle_mesh *m, int maxSize, int threads, uint64_t mem, uint32_t **out,
+                             int dims[3], int *sliceZ);
+
 void oracle_free(void *p);
 
 #ifdef __cplusplus

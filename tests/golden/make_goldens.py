@@ -36,9 +36,11 @@ Fixtures written (all small):
                          volume (scenes.terrain_volume) at 128^3 and 256^3
 
   grid_edges.npz + build_grid_{empty16,one16,slice537,full4}.oct + builds.json["grid_edges"]
+  builds.json["convert"]  md5 and size of the reference's PlyLoader::convertToVolume
+                          .voxel files of the mesh fixtures (single and multi-slab)
                          edge volumes through the reference's file-source builder
 
-`--only frames|ico2048|terrain|edges|extremes` regenerates just that part;
+`--only frames|ico2048|terrain|edges|extremes|convert` regenerates just that part;
 other entries of builds.json are kept.
 """
 import argparse
@@ -317,6 +319,32 @@ BUILD_CASES = [
     ("flat20", 256, 16, 12000000),
 ]
 
+CONVERT_CASES = [
+    # mesh, maxSize, threads, memory budget: PlyLoader::convertToVolume; the
+    # small budgets split the volume into slabs (the reference reuses the slab
+    # buffer and counts without clearing them)
+    ("ico10", 64, 8, 1 << 30),
+    ("ico16n", 128, 8, 1 << 30),
+    ("ico6a", 64, 2, 1 << 30),
+    ("flat20", 128, 16, 1 << 30),
+    ("ico10", 64, 4, 40000),
+    ("flat12", 128, 8, 200000),
+    ("ico16n", 96, 3, 100000),
+]
+
+
+def convert_builds(tmp):
+    """Reference .voxel files of the mesh fixtures (md5 and size only)."""
+    out = []
+    for mesh, res, threads, mem in CONVERT_CASES:
+        vp = os.path.join(tmp, "conv.voxel")
+        run_ref("convert", os.path.join(HERE, f"mesh_{mesh}.ply"), res, threads, mem, vp)
+        out.append({"mesh": mesh, "resolution": res, "threads": threads, "mem": mem, "md5": md5(vp),
+                    "size": os.path.getsize(vp)})
+        print("convert", mesh, res, threads, mem, out[-1]["md5"], out[-1]["size"], flush=True)
+    return out
+
+
 # edge volumes: no voxel, one voxel in a corner, a voxel only on the unscanned
 # last slice of an odd depth, every voxel set
 EDGE_GRIDS = [("empty16", (16, 16, 16)), ("one16", (16, 16, 16)), ("slice537", (5, 3, 7)), ("full4", (4, 4, 4))]
@@ -360,7 +388,7 @@ GRID_CASES = [
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--reference", default="/root/reference")
-    ap.add_argument("--only", choices=["frames", "ico2048", "terrain", "edges", "extremes"], default=None)
+    ap.add_argument("--only", choices=["frames", "ico2048", "terrain", "edges", "extremes", "convert"], default=None)
     args = ap.parse_args()
     if not os.path.exists(REF_BIN):
         subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref", f"REF={args.reference}"], check=True)
@@ -375,13 +403,15 @@ def main():
             return
 
         bpath = os.path.join(HERE, "builds.json")
-        if args.only in ("ico2048", "terrain", "edges"):
+        if args.only in ("ico2048", "terrain", "edges", "convert"):
             with open(bpath) as fp:
                 builds = json.load(fp)
             if args.only == "ico2048":
                 builds["ico2048"] = full_size_build(tmp)
             elif args.only == "terrain":
                 builds["terrain"] = terrain_builds(tmp)
+            elif args.only == "convert":
+                builds["convert"] = convert_builds(tmp)
             else:
                 builds["grid_edges"] = edge_builds(tmp)
             with open(bpath, "w") as fp:
@@ -518,6 +548,7 @@ def main():
             builds["mesh_builds"].append({"mesh": mesh, "resolution": res, "threads": threads,
                                           "mem": mem, "oct": os.path.basename(op), "md5": md5(op)})
             print("mesh build", mesh, res, threads, mem, md5(op), flush=True)
+        builds["convert"] = convert_builds(tmp)
         with open(os.path.join(HERE, "builds.json"), "w") as fp:
             json.dump(builds, fp, indent=1)
     finally:

@@ -50,6 +50,7 @@ def lib():
         L.oracle_build_grid.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
                                         P, P, P, P]
         L.oracle_voxelize_mesh.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, P, P]
+        L.oracle_convert_to_volume.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, P, P, P]
         L.oracle_compress_material_batch.argtypes = [ctypes.c_int64, P, P, P]
         L.oracle_decompress_material_batch.argtypes = [ctypes.c_int64, P, P, P]
         L.oracle_tribox_batch.argtypes = [ctypes.c_int64, P, P]
@@ -371,6 +372,22 @@ class Mesh:
         if r != 0:
             raise RuntimeError("oracle voxelize failed")
         return out.reshape(dims[2], dims[1], dims[0])
+
+
+    def convert_to_volume(self, max_size, threads, mem=1 << 30):
+        """PlyLoader::convertToVolume: the .voxel file's bytes (w, h, d int32
+        header, then the uint32 volume) and the slices per slab."""
+        wp = ctypes.c_void_p()
+        dims = np.zeros(3, np.int32)
+        sz = ctypes.c_int()
+        r = lib().oracle_convert_to_volume(self.h, int(max_size), int(threads), int(mem), ctypes.byref(wp),
+                                           _p(dims), ctypes.byref(sz))
+        if r != 0:
+            raise RuntimeError(f"oracle convert failed ({r})")
+        n = int(dims[0]) * int(dims[1]) * int(dims[2])
+        body = np.ctypeslib.as_array(ctypes.cast(wp, ctypes.POINTER(ctypes.c_uint32)), (n,)).copy()
+        lib().oracle_free(wp)
+        return dims.tobytes() + body.tobytes(), sz.value
 
 
 def build_grid(vol, mem=1 << 30):

@@ -147,6 +147,19 @@ def test_frame_invariants(golden):
     assert 0 < (d1 < f(1e10)).mean() < 1
 
 
+def test_convert_to_volume_matches_reference(golden):
+    """PlyLoader::convertToVolume's .voxel bytes (reference driver: the
+    function's slab loop on the reference's own processBlock). The small
+    budgets split the volume into 9-12-slice slabs; their bytes depend on the
+    slab buffer and counts being reused without clearing."""
+    for c in _builds(golden)["convert"]:
+        m = orc.Mesh.from_ply(os.path.join(golden, f"mesh_{c['mesh']}.ply"))
+        data, slice_z = m.convert_to_volume(c["resolution"], c["threads"], c["mem"])
+        assert len(data) == c["size"], c
+        assert hashlib.md5(data).hexdigest() == c["md5"], c
+        assert (slice_z < int(np.frombuffer(data[8:12], np.int32)[0])) == (c["mem"] < 1 << 30), c
+
+
 def test_edge_volumes_match_reference(golden):
     """No voxel, one voxel, a voxel only on the unscanned last slice of an odd
     depth, a full volume; a one-voxel volume (on which the reference crashes)
