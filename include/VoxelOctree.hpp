@@ -7,6 +7,7 @@
  * (src/Main.cpp viewer and builder) compile unchanged against it:
  *
  *     PlyLoader loader(in);                                 // Main.cpp:323
+ *     loader.convertToVolume(tmp, resolution, dataMemory);  // Main.cpp:317 (GPU voxelization)
  *     VoxelData data(&loader, resolution, dataMemory);      // Main.cpp:324
  *     VoxelOctree tree(&data);                              // Main.cpp:325 (GPU build)
  *     tree.save(out);                                       // Main.cpp:326
@@ -62,14 +63,26 @@ inline bool check(int code, std::string *err) {
 }
 }  // namespace svo_shim
 
-/* PlyLoader(path), src/PlyLoader.cpp:66-79: remembers the mesh file. */
+/* PlyLoader(path), src/PlyLoader.cpp:66-79: remembers the mesh file. The
+ * thread count fixes the sub-block partition, as for VoxelData below. */
 class PlyLoader {
 public:
     explicit PlyLoader(const char *path) : path_(path) {}
     const std::string &path() const { return path_; }
+    void setThreads(int threads) { threads_ = threads; }
+
+    /* convertToVolume, src/PlyLoader.cpp:506-534 (builder mode 1,
+     * src/Main.cpp:315-320): the dense .voxel file, voxelized on the GPU. */
+    void convertToVolume(const char *path, int maxSize, size_t memoryBudget) {
+        svo_build_opts o{};
+        o.memory_budget = memoryBudget;
+        o.threads = threads_;
+        svo_shim::check(svo_convert_to_volume(path_.c_str(), maxSize, &o, path), nullptr);
+    }
 
 private:
     std::string path_;
+    int threads_ = 0;
 };
 
 /* VoxelData(PlyLoader*, sideLength, mem), src/VoxelData.cpp:50-56, and

@@ -290,6 +290,18 @@ typedef struct {
     int reserved[7];
 } svo_build_opts;
 
+/* PlyLoader(path).convertToVolume(voxel_path, resolution, budget),
+ * src/PlyLoader.cpp:506-534 (the builder's on-disk mode, src/Main.cpp:315-320):
+ * the mesh voxelized on the GPU into a dense .voxel file (int32 w, h, d, then
+ * w*h*d uint32 words, x fastest), byte-identical to the voxels the reference's
+ * slab loop produces for the same (resolution, budget, threads): slabs of
+ * budget / (w*h) slices, each one processBlock over a w x h x slices block,
+ * with the slab buffer and counts carried from slab to slab uncleared, as
+ * there. The reference's function itself never writes a slab (its progress
+ * line divides by a zero block count). SVO_ERR_LIMIT when the budget holds
+ * less than one slice; on failure no file is left. */
+int svo_convert_to_volume(const char *ply_path, int resolution, const svo_build_opts *opts, const char *voxel_path);
+
 /* PlyLoader(path) -> VoxelData(loader, resolution, budget) -> VoxelOctree,
  * src/Main.cpp:322-327, built on the GPU; .oct bytes identical to the
  * reference's for the same (resolution, budget, threads). */

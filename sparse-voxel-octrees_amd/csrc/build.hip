@@ -386,13 +386,18 @@ __global__ void __launch_bounds__(256) ply_bounds_kernel(const float *__restrict
     }
 }
 
+// Destination of a mesh voxelized as PlyLoader::convertToVolume writes it.
+struct VolumeSink {
+    FILE *fp;  // the .voxel file, empty
+};
+
 struct MeshParams {
     int scale;  // resolution - 4
     int sb[3];
     int g[3];
     int p[3];
     int data[3];
-    int cache;
+    int blk[3];   // processBlock's block: the cubic cache block (VoxelData) or a convertToVolume slab
     float hx;     // 1.0f / scale (voxel edge in the unit cube)
     float hb[3];  // sub-block edge per axis in the unit cube
 };
@@ -477,7 +482,7 @@ __device__ __forceinline__ bool cell_region(uint32_t f, const MeshParams &mp, in
     const int gg[3] = {gx, gy, gz};
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        const int blockOrigin = (gg[k] / mp.p[k]) * mp.cache;
+        const int blockOrigin = (gg[k] / mp.p[k]) * mp.blk[k];
         if (blockOrigin >= mp.data[k]) return false;
         lo[k] = gg[k] * mp.sb[k];
         hi[k] = min((gg[k] + 1) * mp.sb[k], mp.data[k]) - 1;
@@ -521,7 +526,7 @@ __global__ void block_flags_kernel(const uint32_t *__restrict__ cells, uint64_t 
     const uint32_t gz = f / (uint32_t(mp.g[0]) * uint32_t(mp.g[1]));
     const uint32_t b[3] = {gx / uint32_t(mp.p[0]), gy / uint32_t(mp.p[1]), gz / uint32_t(mp.p[2])};
     for (int k = 0; k < 3; ++k)
-        if (int64_t(b[k]) * mp.cache >= mp.data[k]) return;
+        if (int64_t(b[k]) * mp.blk[k] >= mp.data[k]) return;
     flags[morton3(b[0], b[1], b[2])] = 1u;
 }
 
@@ -683,6 +688,31 @@ __global__ void fold_kernel(const uint64_t *__restrict__ keys, const uint64_t *_
     vkey[s] = keys[segStart[s]] >> tribits;
     vword[s] = word;
     vcount[s] = uint8_t(count);
+}
+
+// One slab of PlyLoader::convertToVolume (src/PlyLoader.cpp:521-530): every
+// voxel whose z lies in [z0, z0 + sliceZ) is folded from the state its cell of
+// the slab buffer holds (word, count), which carries over from earlier slabs
+// because the reference never clears the buffer or its counts.
+__global__ void slab_fold_kernel(const uint64_t *__restrict__ keys, const uint64_t *__restrict__ segStart, uint64_t S,
+                                 int tribits, const float *__restrict__ pos, const float *__restrict__ nrm,
+                                 const float *__restrict__ col, MeshParams mp, int z0, int sliceZ,
+                                 uint32_t *__restrict__ word, uint32_t *__restrict__ count,
+                                 uint32_t *__restrict__ bad) {
+    uint64_t s = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    const uint64_t m = keys[segStart[s]] >> tribits;
+    const int x = int(compact3(m)), y = int(compact3(m >> 1)), z = int(compact3(m >> 2));
+    if (z < z0 || z >= z0 + sliceZ) return;
+    if (x >= mp.data[0] || y >= mp.data[1]) {
+        atomicOr(bad, 1u);
+        return;
+    }
+    const uint64_t cell = uint64_t(x) + uint64_t(mp.data[0]) * (uint64_t(y) + uint64_t(mp.data[1]) * uint64_t(z - z0));
+    uint32_t w = word[cell], c = count[cell];
+    fold_voxel(keys, segStart[s], segStart[s + 1], tribits, pos, nrm, col, mp, w, c);
+    word[cell] = w;
+    count[cell] = c;
 }
 
 // Folds voxel `voxel[i]` again from a carried buffer state (init word/count).
@@ -1584,12 +1614,47 @@ int check_blocks(const Geometry &g, uint64_t &nBlocks) {
 // entry points
 // ---------------------------------------------------------------------------
 
+// convertToVolume's slab loop (src/PlyLoader.cpp:516-530) once every voxel's
+// (voxel, triangle) pairs are sorted: the w, h, d header, then slab after slab
+// the voxels of the slab are folded from the slab buffer's carried state and
+// the slab's first min(sliceZ, d - z) slices are appended.
+static int write_volume(Arena &ar, const Geometry &g, const MeshParams &mp, int tribits, const uint64_t *keys,
+                        const uint64_t *segStart, uint64_t V, const float *pos, const float *nrm, const float *col,
+                        int sliceZ, FILE *fp) {
+    hipStream_t st = ar.stream;
+    const uint64_t plane = uint64_t(g.dataW) * uint64_t(g.dataH), slab = plane * uint64_t(sliceZ);
+    uint32_t *word, *count, *bad;
+    int r;
+    if ((r = ar.alloc(word, slab)) || (r = ar.alloc(count, slab)) || (r = ar.alloc(bad, 1))) return r;
+    // the reference's buffer is a fresh allocation (zero pages) and its counts are
+    // only read for cells written before
+    if ((r = fill32(word, slab, 0, st)) || (r = fill32(count, slab, 0, st)) || (r = fill32(bad, 1, 0, st))) return r;
+    const int32_t hdr[3] = {g.dataW, g.dataH, g.dataD};
+    if (std::fwrite(hdr, 4, 3, fp) != 3) return fail(SVO_ERR_IO, "convert: short write");
+    WordVec host;
+    for (int z0 = 0; z0 < g.dataD; z0 += sliceZ) {
+        if (V)
+            hipLaunchKernelGGL(slab_fold_kernel, dim3(grid_for(V)), dim3(kThreads), 0, st, keys, segStart, V, tribits, pos,
+                               nrm, col, mp, z0, sliceZ, word, count, bad);
+        HIP_TRY(hipGetLastError());
+        const uint64_t n = plane * uint64_t(std::min(sliceZ, g.dataD - z0));
+        if ((r = download_words(word, n, host, st))) return r;
+        if (std::fwrite(host.data(), 4, n, fp) != n) return fail(SVO_ERR_IO, "convert: short write");
+    }
+    uint32_t badH = 0;
+    HIP_TRY(hipMemcpy(&badH, bad, 4, hipMemcpyDeviceToHost));
+    if (badH) return fail(SVO_ERR_DEVICE, "convert: a voxel outside the volume (internal error)");
+    return SVO_OK;
+}
+
 // The mesh build once the vertices (9 floats each: position, normal, colour;
 // not yet rescaled), their bounds and the face table (sizes, first index,
-// first triangle, indices) are on the device in `ar`.
+// first triangle, indices) are on the device in `ar`. With a sink the mesh is
+// instead voxelized into a dense .voxel file as PlyLoader::convertToVolume
+// writes it (out is not touched).
 static int mesh_core(Arena &ar, float *dv, int64_t nverts, int has_normals, int32_t *dfc, int64_t *dfs, int64_t *dts,
                      int32_t *dfi, int64_t nfaces, int64_t T, const float lower[3], const float upper[3],
-                     int resolution, const svo_build_opts *opts, svo_octree &out) {
+                     int resolution, const svo_build_opts *opts, svo_octree &out, const VolumeSink *sink) {
     const float diff[3] = {upper[0] - lower[0], upper[1] - lower[1], upper[2] - lower[2]};
     int largest = 2;
     if (diff[0] > diff[1] && diff[0] > diff[2]) largest = 0;
@@ -1606,24 +1671,45 @@ static int mesh_core(Arena &ar, float *dv, int64_t nverts, int has_normals, int3
     g.dataW = cvt_i32((su[0] - sl[0]) * float(resolution - 2)) + 2;
     g.dataH = cvt_i32((su[1] - sl[1]) * float(resolution - 2)) + 2;
     g.dataD = cvt_i32((su[2] - sl[2]) * float(resolution - 2)) + 2;
-    int r = choose_cache(g, budget(opts), true);
-    if (r) return r;
+    int r;
+    int blk[3];  // processBlock's block
+    int sliceZ = 0;
+    uint64_t nBlocks = 0;
+    if (!sink) {
+        if ((r = choose_cache(g, budget(opts), true))) return r;
+        blk[0] = blk[1] = blk[2] = g.cache;
+    } else {
+        // convertToVolume (src/PlyLoader.cpp:511-517): slabs of the whole w x h
+        // plane, as many slices as one count byte per voxel lets into the budget
+        if (g.dataW < 1 || g.dataH < 1 || g.dataD < 1) return fail(SVO_ERR_UNSUPPORTED, "degenerate mesh extent");
+        const uint64_t slices = budget(opts) / (uint64_t(g.dataW) * uint64_t(g.dataH));
+        sliceZ = int(std::min<uint64_t>(slices, uint64_t(g.dataD)));
+        if (sliceZ == 0)
+            return fail(SVO_ERR_LIMIT, "convert: the memory budget holds less than one slice (the reference asserts)");
+        blk[0] = g.dataW;
+        blk[1] = g.dataH;
+        blk[2] = sliceZ;
+        g.depth = 0;  // Morton bits per axis of the voxel keys
+        while ((1 << g.depth) < std::max(g.dataW, std::max(g.dataH, g.dataD))) g.depth++;
+    }
     // PlyLoader::setupBlockProcessing, src/PlyLoader.cpp:407-437
     g.scale = resolution - 4;
-    g.sbW = g.sbH = g.sbD = g.cache;
+    g.sbW = blk[0];
+    g.sbH = blk[1];
+    g.sbD = blk[2];
     best_partition(g.sbW, g.sbH, g.sbD, host_threads(opts));
-    g.pW = g.cache / g.sbW;
-    g.pH = g.cache / g.sbH;
-    g.pD = g.cache / g.sbD;
-    g.gW = g.pW * (g.dataW + g.cache - 1) / g.cache;
-    g.gH = g.pH * (g.dataH + g.cache - 1) / g.cache;
-    g.gD = g.pD * (g.dataD + g.cache - 1) / g.cache;
+    g.pW = blk[0] / g.sbW;
+    g.pH = blk[1] / g.sbH;
+    g.pD = blk[2] / g.sbD;
+    // multiplied before the division, as the reference's _gridW expression evaluates
+    g.gW = g.pW * (g.dataW + blk[0] - 1) / blk[0];
+    g.gH = g.pH * (g.dataH + blk[1] - 1) / blk[1];
+    g.gD = g.pD * (g.dataD + blk[2] - 1) / blk[2];
     if (uint64_t(g.gW) * g.gH * g.gD >= (uint64_t(1) << 32)) return fail(SVO_ERR_LIMIT, "sub-block grid too large");
     int tribits = 1;
     while ((int64_t(1) << tribits) < std::max<int64_t>(T, 2)) tribits++;
     if (3 * g.depth + tribits > 64) return fail(SVO_ERR_LIMIT, "voxel key and triangle index exceed 64 bits");
-    uint64_t nBlocks;
-    if ((r = check_blocks(g, nBlocks))) return r;
+    if (!sink && (r = check_blocks(g, nBlocks))) return r;
 
     MeshParams mp;
     mp.scale = g.scale;
@@ -1631,7 +1717,9 @@ static int mesh_core(Arena &ar, float *dv, int64_t nverts, int has_normals, int3
     mp.g[0] = g.gW; mp.g[1] = g.gH; mp.g[2] = g.gD;
     mp.p[0] = g.pW; mp.p[1] = g.pH; mp.p[2] = g.pD;
     mp.data[0] = g.dataW; mp.data[1] = g.dataH; mp.data[2] = g.dataD;
-    mp.cache = g.cache;
+    mp.blk[0] = blk[0];
+    mp.blk[1] = blk[1];
+    mp.blk[2] = blk[2];
     mp.hx = 1.0f / float(g.scale);
     mp.hb[0] = float(g.sbW) / float(g.scale);
     mp.hb[1] = float(g.sbH) / float(g.scale);
@@ -1676,9 +1764,11 @@ static int mesh_core(Arena &ar, float *dv, int64_t nverts, int has_normals, int3
     ar.release(binCount);
     ar.release(binOff);
 
-    uint32_t *blockFlags;
-    if ((r = ar.alloc(blockFlags, nBlocks)) || (r = fill32(blockFlags, nBlocks, 0, st))) return r;
-    if (P) hipLaunchKernelGGL(block_flags_kernel, dim3(grid_for(P)), dim3(kThreads), 0, st, cells, P, mp, blockFlags);
+    uint32_t *blockFlags = nullptr;
+    if (!sink) {
+        if ((r = ar.alloc(blockFlags, nBlocks)) || (r = fill32(blockFlags, nBlocks, 0, st))) return r;
+        if (P) hipLaunchKernelGGL(block_flags_kernel, dim3(grid_for(P)), dim3(kThreads), 0, st, cells, P, mp, blockFlags);
+    }
 
     // voxel rows
     uint32_t *rows;
@@ -1738,11 +1828,15 @@ static int mesh_core(Arena &ar, float *dv, int64_t nverts, int has_normals, int3
     uint64_t *segStart, *vkey;
     uint32_t *vword;
     uint8_t *vcount;
-    if ((r = ar.alloc(segStart, V + 1)) || (r = ar.alloc(vkey, V)) || (r = ar.alloc(vword, V)) ||
-        (r = ar.alloc(vcount, V)))
-        return r;
+    if ((r = ar.alloc(segStart, V + 1))) return r;
     if (K) hipLaunchKernelGGL(seg_starts_kernel, dim3(grid_for(K)), dim3(kThreads), 0, st, head, segIdx, K, segStart);
     HIP_TRY(hipMemcpy(segStart + V, &K, 8, hipMemcpyHostToDevice));
+    if (sink) {
+        ar.release(head);
+        ar.release(segIdx);
+        return write_volume(ar, g, mp, tribits, keys2, segStart, V, pos, nrm, col, sliceZ, sink->fp);
+    }
+    if ((r = ar.alloc(vkey, V)) || (r = ar.alloc(vword, V)) || (r = ar.alloc(vcount, V))) return r;
     if (V)
         hipLaunchKernelGGL(fold_kernel, dim3(grid_for(V)), dim3(kThreads), 0, st, keys2, segStart, V, tribits, pos, nrm,
                            col, mp, vkey, vword, vcount);
@@ -1767,8 +1861,9 @@ static int mesh_core(Arena &ar, float *dv, int64_t nverts, int has_normals, int3
     return emit_tree(ar, g, lkey, lword, L, multi ? blockFlags : nullptr, st, out);
 }
 
-int build_mesh(const float *verts, int64_t nverts, int has_normals, const int32_t *face_counts,
-               const int32_t *face_idx, int64_t nfaces, int resolution, const svo_build_opts *opts, svo_octree &out) {
+static int mesh_entry(const float *verts, int64_t nverts, int has_normals, const int32_t *face_counts,
+                      const int32_t *face_idx, int64_t nfaces, int resolution, const svo_build_opts *opts,
+                      svo_octree &out, const VolumeSink *sink) {
     if (resolution < 8 || resolution > 8192) return fail(SVO_ERR_INVALID, "resolution must be in [8, 8192]");
     // PlyLoader::readVertices bounds and rescaleVertices (src/PlyLoader.cpp:122-181)
     float lower[3] = {1e30f, 1e30f, 1e30f}, upper[3] = {-1e30f, -1e30f, -1e30f};
@@ -1812,14 +1907,21 @@ int build_mesh(const float *verts, int64_t nverts, int has_normals, const int32_
     if (at) HIP_TRY(hipMemcpyAsync(dfi, face_idx, uint64_t(at) * 4, hipMemcpyHostToDevice, st));
     // the host vectors must outlive the copies
     HIP_TRY(hipStreamSynchronize(st));
-    return mesh_core(ar, dv, nverts, has_normals, dfc, dfs, dts, dfi, nfaces, T, lower, upper, resolution, opts, out);
+    return mesh_core(ar, dv, nverts, has_normals, dfc, dfs, dts, dfi, nfaces, T, lower, upper, resolution, opts, out,
+                     sink);
+}
+
+int build_mesh(const float *verts, int64_t nverts, int has_normals, const int32_t *face_counts,
+               const int32_t *face_idx, int64_t nfaces, int resolution, const svo_build_opts *opts, svo_octree &out) {
+    return mesh_entry(verts, nverts, has_normals, face_counts, face_idx, nfaces, resolution, opts, out, nullptr);
 }
 
 // PLY path: binary little-endian files whose vertex records have a fixed
 // size and whose faces all have the first face's vertex count are decoded on
 // the device straight from the file bytes; other files (ascii, mixed face
 // sizes, list properties on vertices) go through the host parser.
-int build_ply(const char *path, int resolution, const svo_build_opts *opts, svo_octree &out) {
+static int ply_entry(const char *path, int resolution, const svo_build_opts *opts, svo_octree &out,
+                     const VolumeSink *sink) {
     if (resolution < 8 || resolution > 8192) return fail(SVO_ERR_INVALID, "resolution must be in [8, 8192]");
     PlyLayout L;
     int r = ply_open(path, L);
@@ -1830,8 +1932,8 @@ int build_ply(const char *path, int resolution, const svo_build_opts *opts, svo_
         int hn = 0;
         int rr = ply_parse_host(L, verts, hn, counts, idx);
         if (rr) return rr;
-        return build_mesh(verts.data(), int64_t(verts.size() / 9), hn, counts.data(), idx.data(),
-                          int64_t(counts.size()), resolution, opts, out);
+        return mesh_entry(verts.data(), int64_t(verts.size() / 9), hn, counts.data(), idx.data(),
+                          int64_t(counts.size()), resolution, opts, out, sink);
     };
     if (!L.binary) return host_path();
     // element offsets: every element before the face element needs fixed-size records
@@ -1949,7 +2051,22 @@ int build_ply(const char *path, int resolution, const svo_build_opts *opts, svo_
     ar.release(raw);
     ar.release(parts);
     ar.release(flags);
-    return mesh_core(ar, dv, nv, L.has_normals, dfc, dfs, dts, dfi, nf, T, lower, upper, resolution, opts, out);
+    return mesh_core(ar, dv, nv, L.has_normals, dfc, dfs, dts, dfi, nf, T, lower, upper, resolution, opts, out, sink);
+}
+
+int build_ply(const char *path, int resolution, const svo_build_opts *opts, svo_octree &out) {
+    return ply_entry(path, resolution, opts, out, nullptr);
+}
+
+int convert_ply(const char *path, int resolution, const svo_build_opts *opts, const char *voxelPath) {
+    FILE *fp = std::fopen(voxelPath, "wb");
+    if (!fp) return fail(SVO_ERR_IO, std::string("cannot write ") + voxelPath);
+    VolumeSink sink{fp};
+    svo_octree unused;
+    int r = ply_entry(path, resolution, opts, unused, &sink);
+    if (std::fclose(fp) != 0 && r == SVO_OK) r = fail(SVO_ERR_IO, std::string("short write to ") + voxelPath);
+    if (r != SVO_OK) std::remove(voxelPath);  // no partial volume is left behind
+    return r;
 }
 
 int build_grid(const uint32_t *grid, int w, int h, int d, const svo_build_opts *opts, svo_octree &out) {

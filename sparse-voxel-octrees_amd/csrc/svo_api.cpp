@@ -579,6 +579,14 @@ int svo_build_from_ply(const char *path, int resolution, const svo_build_opts *o
     });
 }
 
+int svo_convert_to_volume(const char *ply_path, int resolution, const svo_build_opts *opts, const char *voxel_path) {
+    return guarded([&]() -> int {
+        clear_error();
+        if (!ply_path || !voxel_path) return fail(SVO_ERR_INVALID, "null argument");
+        return convert_ply(ply_path, resolution, opts, voxel_path);
+    });
+}
+
 int svo_build_from_grid(const uint32_t *grid, int w, int h, int d, const svo_build_opts *opts, svo_octree **out) {
     return guarded([&]() -> int {
         clear_error();

@@ -185,5 +185,7 @@ int ply_parse_host(const PlyLayout &layout, std::vector<float> &verts, int &has_
 int read_ply(const char *path, std::vector<float> &verts, int &has_normals,
              std::vector<int32_t> &face_counts, std::vector<int32_t> &face_idx);
 int build_ply(const char *path, int resolution, const svo_build_opts *opts, svo_octree &out);
+// PlyLoader::convertToVolume: the mesh voxelized into a dense .voxel file.
+int convert_ply(const char *path, int resolution, const svo_build_opts *opts, const char *voxelPath);
 
 }  // namespace svo

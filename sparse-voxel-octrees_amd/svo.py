@@ -27,6 +27,7 @@ EXPORTS = [
     "svo_encode", "svo_encode_words", "svo_free_buffer", "svo_from_words", "svo_center", "svo_word_count", "svo_copy_words",
     "svo_depth", "svo_free", "svo_upload", "svo_raymarch", "svo_raymarch_batch", "svo_raymarch_batch_device",
     "svo_orbit_camera", "svo_render_primary", "svo_render_primary_device", "svo_build_from_ply",
+    "svo_convert_to_volume",
     "svo_build_from_mesh", "svo_build_from_voxels", "svo_build_from_grid",
     "svo_renderer_create", "svo_renderer_free", "svo_render_batch_device",
     "svo_build_from_sdf", "svo_sdf_volume", "svo_serve_replay",
@@ -172,6 +173,7 @@ def _load():
         "svo_render_primary": [P, P, ctypes.c_int, ctypes.c_int, P, P, P, P],
         "svo_render_primary_device": [P, P, ctypes.c_int, ctypes.c_int, P, P, P, P, P],
         "svo_build_from_ply": [ctypes.c_char_p, ctypes.c_int, P, P],
+        "svo_convert_to_volume": [ctypes.c_char_p, ctypes.c_int, P, ctypes.c_char_p],
         "svo_build_from_mesh": [P, I64, ctypes.c_int, P, P, I64, ctypes.c_int, P, P],
         "svo_build_from_voxels": [ctypes.c_char_p, P, P],
         "svo_build_from_grid": [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P],
@@ -475,6 +477,13 @@ class VoxelOctree:
     def render_frame_device(self, cam, width, height, rgba_ptr, depth_ptr=None, stripes=16, stride=1, stream=None):
         _check(lib.svo_render_frame_device(self._h, ctypes.byref(cam), int(width), int(height), int(stripes),
                                            int(stride), rgba_ptr, depth_ptr, stream))
+
+
+def convert_to_volume(ply_path, resolution, voxel_path, threads=0, memory_budget=0):
+    """PlyLoader::convertToVolume on the GPU: writes the dense .voxel file."""
+    o = _opts(memory_budget, threads)
+    _check(lib.svo_convert_to_volume(os.fsencode(ply_path), int(resolution), ctypes.byref(o),
+                                     os.fsencode(voxel_path)))
 
 
 def frame_depth_count(width, height, stripes=16):

@@ -6,6 +6,7 @@
 //   shim_main build <in.ply> <res> <threads> <mem> <out.oct>
 //   shim_main frame <in.oct> <pitch> <yaw> <radius> <w> <h> <stride> <out.rgba>
 #include <cstdio>
+#include <memory>
 #include <cstdlib>
 #include <cstring>
 #include <vector>
@@ -58,6 +59,18 @@ int main(int argc, char **argv) {
         data.setThreads(std::atoi(argv[4]));
         VoxelOctree tree(&data);
         tree.save(argv[6]);
+        return 0;
+    }
+    if (mode == "convert" && argc == 8) {
+        // the builder's on-disk mode (src/Main.cpp:315-320): convertToVolume, then
+        // VoxelData(path, mem) -> VoxelOctree -> save
+        std::unique_ptr<PlyLoader> loader(new PlyLoader(argv[2]));
+        loader->setThreads(std::atoi(argv[4]));
+        const size_t mem = size_t(std::strtoull(argv[5], nullptr, 10));
+        loader->convertToVolume(argv[6], std::atoi(argv[3]), mem);
+        std::unique_ptr<VoxelData> data(new VoxelData(argv[6], mem));
+        std::unique_ptr<VoxelOctree> tree(new VoxelOctree(data.get()));
+        tree->save(argv[7]);
         return 0;
     }
     if (mode == "frame" && argc == 10) {

@@ -63,6 +63,25 @@ def test_edge_volumes_match_reference(svo, golden):
     assert e.value.code == svo.svo.ERR_UNSUPPORTED
 
 
+def test_convert_to_volume_matches_reference(svo, golden, tmp_path):
+    """PlyLoader::convertToVolume on the GPU: the .voxel bytes equal the
+    reference's for single-slab and multi-slab budgets (the latter carry the
+    slab buffer between slabs); a budget below one slice is refused and
+    leaves no file."""
+    import hashlib
+    for c in _builds(golden)["convert"]:
+        print(f"[gpu] convert {c}", flush=True)
+        out = tmp_path / f"{c['mesh']}_{c['mem']}.voxel"
+        svo.svo.convert_to_volume(os.path.join(golden, f"mesh_{c['mesh']}.ply"), c["resolution"], str(out),
+                                  threads=c["threads"], memory_budget=c["mem"])
+        raw = out.read_bytes()
+        assert len(raw) == c["size"] and hashlib.md5(raw).hexdigest() == c["md5"], c
+    bad = tmp_path / "small.voxel"
+    with pytest.raises(svo.SvoError) as e:
+        svo.svo.convert_to_volume(os.path.join(golden, "mesh_ico10.ply"), 64, str(bad), threads=4, memory_budget=1000)
+    assert e.value.code == svo.svo.ERR_LIMIT and not bad.exists()
+
+
 def test_sdf_scene_build_matches_reference(svo, golden):
     b = _builds(golden)["sdf128"]
     print("[gpu] sdf128 grid build", flush=True)

@@ -55,6 +55,27 @@ def test_frontend_builder_matches_reference(shim_bin, golden, tmp_path):
 
 
 @pytest.mark.gpu
+def test_frontend_on_disk_builder_mode(shim_bin, golden, tmp_path):
+    """The builder's on-disk mode as Main.cpp writes it (convertToVolume, then
+    VoxelData(path) -> VoxelOctree -> save): the .voxel equals the reference's,
+    and the octree equals the oracle's file-source build of that volume."""
+    import hashlib
+    import oracle_py as orc
+    c = [x for x in json.load(open(os.path.join(golden, "builds.json")))["convert"] if x["mem"] < 1 << 30][0]
+    print(f"[gpu] C++ front end on-disk builder {c['mesh']} mem={c['mem']}", flush=True)
+    vox, oct_ = tmp_path / "m.voxel", tmp_path / "m.oct"
+    subprocess.run([shim_bin, "convert", os.path.join(golden, f"mesh_{c['mesh']}.ply"), str(c["resolution"]),
+                    str(c["threads"]), str(c["mem"]), str(vox), str(oct_)], check=True, timeout=120)
+    raw = vox.read_bytes()
+    assert hashlib.md5(raw).hexdigest() == c["md5"]
+    w, h, d = np.frombuffer(raw[:12], np.int32)
+    vol = np.frombuffer(raw[12:], np.uint32).reshape(d, h, w)
+    center, words, _ = orc.build_grid(vol, c["mem"])
+    rc, rw = orc.read_oct(str(oct_))
+    assert np.array_equal(rw, words) and np.array_equal(rc, center)
+
+
+@pytest.mark.gpu
 def test_frontend_frame_matches_reference(shim_bin, golden, tmp_path):
     """renderFrame (the viewer's renderBatch/renderTile/shade as one call)
     reproduces the reference driver's frames."""
