@@ -241,11 +241,16 @@ __device__ __forceinline__ int ray_step(Ray<Idx> &r, const uint32_t *__restrict_
         if (stepMask & 1) diff |= f2u(r.posX) ^ f2u(r.posX + r.scaleExp2);
         if (stepMask & 2) diff |= f2u(r.posY) ^ f2u(r.posY + r.scaleExp2);
         if (stepMask & 4) diff |= f2u(r.posZ) ^ f2u(r.posZ + r.scaleExp2);
-        r.scale = int(f2u(float(int(diff))) >> 23) - 127;
+        // The reference takes the highest differing bit from the exponent of
+        // float(int(diff)) (src/VoxelOctree.cpp:318-325). diff is non-zero (a
+        // stepped coordinate in [1,2) changes by at least its ulp); below 2^23
+        // the conversion is exact, so the exponent is 31 - clz(diff); from 2^23
+        // up both forms give a scale >= 23 and the ray leaves the octree. The
+        // reference then reads rayStack[scale] and leaves the loop; that read
+        // cannot reach the result.
+        if (diff >= (1u << kMaxScale)) return kMiss;
+        r.scale = 31 - __builtin_clz(diff);
         r.scaleExp2 = u2f(uint32_t(r.scale - kMaxScale + 127) << 23);
-        // the reference reads rayStack[scale] even when scale >= 23 and then
-        // leaves the loop; that read cannot reach the result
-        if (r.scale >= kMaxScale) return kMiss;
         const int slot = (kMaxScale - 1) - r.scale;  // a slot pushed earlier: the new scale is above the current one
         if (slot >= Reg) {
             stk.pop(slot - Reg, r.parent, r.maxT);
