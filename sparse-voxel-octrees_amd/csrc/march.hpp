@@ -217,21 +217,23 @@ __device__ __forceinline__ int ray_step(Ray<Idx> &r, const uint32_t *__restrict_
             if (r.current & 0x10000u) r.parent += siblingCount;
             r.current = tree[r.parent];
 
-            r.idx = 0;
             r.scale--;
             r.scaleExp2 = half;
-            if (centerTX > r.minT) { r.idx ^= 1; r.posX += r.scaleExp2; }
-            if (centerTY > r.minT) { r.idx ^= 2; r.posY += r.scaleExp2; }
-            if (centerTZ > r.minT) { r.idx ^= 4; r.posZ += r.scaleExp2; }
+            const bool cx = centerTX > r.minT, cy = centerTY > r.minT, cz = centerTZ > r.minT;
+            if (cx) r.posX += r.scaleExp2;
+            if (cy) r.posY += r.scaleExp2;
+            if (cz) r.posZ += r.scaleExp2;
+            r.idx = (cx ? 1 : 0) | (cy ? 2 : 0) | (cz ? 4 : 0);
             r.maxT = maxTV;
             return kRunning;
         }
     }
 
-    int stepMask = 0;
-    if (cornerTX <= maxTC) { stepMask ^= 1; r.posX -= r.scaleExp2; }
-    if (cornerTY <= maxTC) { stepMask ^= 2; r.posY -= r.scaleExp2; }
-    if (cornerTZ <= maxTC) { stepMask ^= 4; r.posZ -= r.scaleExp2; }
+    const bool sx = cornerTX <= maxTC, sy = cornerTY <= maxTC, sz = cornerTZ <= maxTC;
+    if (sx) r.posX -= r.scaleExp2;
+    if (sy) r.posY -= r.scaleExp2;
+    if (sz) r.posZ -= r.scaleExp2;
+    const int stepMask = (sx ? 1 : 0) | (sy ? 2 : 0) | (sz ? 4 : 0);
 
     r.minT = maxTC;
     r.idx ^= stepMask;
@@ -272,7 +274,8 @@ __device__ __forceinline__ int ray_step(Ray<Idx> &r, const uint32_t *__restrict_
         r.posX = u2f(shX << r.scale);
         r.posY = u2f(shY << r.scale);
         r.posZ = u2f(shZ << r.scale);
-        r.idx = int((shX & 1u) | ((shY & 1u) << 1) | ((shZ & 1u) << 2));
+        r.idx = int(__builtin_amdgcn_ubfe(shX, 0, 1) | (__builtin_amdgcn_ubfe(shY, 0, 1) << 1) |
+                    (__builtin_amdgcn_ubfe(shZ, 0, 1) << 2));
     }
     return kRunning;
 }
