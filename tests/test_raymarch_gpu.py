@@ -187,6 +187,44 @@ def test_batched_renderer_matches_plain_kernel(svo, trees, golden):
             _same(mine, ref, f"{w}x{h} rs={rs}")
 
 
+def test_batched_renderer_with_a_deep_octree(svo, trees):
+    """A batch whose deepest octree has depth 13 (4096^3 terrain) keeps the
+    top three stack entries of every view in registers; each view, the
+    shallow dragon's included, equals the plain kernel, and the terrain's
+    plain render equals the oracle on a sample of rays."""
+    import torch
+    dev = torch.device("cuda", 0)
+    deep = svo.VoxelOctree.build_from_terrain(4096, 41, memory_budget=1 << 30)
+    assert deep.depth() == 13
+    pitch, yaw, _ = svo.scenes.terrain_view(41)
+    specs = [(deep, 320, 180, pitch, yaw), (trees["dragon"], 200, 120, 20.0, 35.0), (deep, 77, 45, 30.0, 100.0)]
+    ren = svo.svo.Renderer(8)
+    outs, views = [], []
+    for t, w, h, p, y in specs:
+        cam = t.orbit_camera(p, y, 1.0)
+        o = {"hit": torch.full((w * h,), 9, dtype=torch.uint8, device=dev),
+             "normal": torch.zeros(w * h, dtype=torch.int32, device=dev),
+             "t": torch.zeros(w * h, dtype=torch.float32, device=dev),
+             "leaf": torch.zeros(w * h, dtype=torch.int64, device=dev)}
+        outs.append((t, cam, w, h, o))
+        views.append(ren.view(t, cam, w, h, o["hit"].data_ptr(), o["normal"].data_ptr(), o["t"].data_ptr(),
+                              o["leaf"].data_ptr()))
+    print("[gpu] batched renderer with a depth-13 octree", flush=True)
+    ren.render_device(views, None, 0, 0)
+    torch.cuda.synchronize()
+    for t, cam, w, h, o in outs:
+        mine = {"hit": o["hit"].cpu().numpy(), "normal": o["normal"].cpu().numpy().view(np.uint32),
+                "t": o["t"].cpu().numpy(), "leaf": o["leaf"].cpu().numpy().view(np.uint64)}
+        _same(mine, t.render_primary(cam, w, h), f"{w}x{h}")
+    t, cam, w, h, _ = outs[0]
+    plain = t.render_primary(cam, w, h)
+    sample = np.arange(0, w * h, 7)
+    d = orc.primary_dirs(np.frombuffer(cam.tform, np.float32), w, h)[sample]
+    ref = orc.raymarch(t.words(), np.frombuffer(cam.pos, np.float32)[None, :], d, 0.0)
+    _same({k: v[sample] for k, v in plain.items()}, ref, "terrain 4096 vs oracle")
+    assert (plain["hit"] == 1).mean() > 0.2
+
+
 def _batch_setup(svo, trees, ren, dev, specs):
     import torch
     outs, views = [], []
