@@ -76,9 +76,17 @@ int validate_from(const uint32_t *w, uint64_t n, Item root, int split, std::vect
 // Subtrees below level 3 are checked on all host threads.
 int octree_validate(const uint32_t *w, uint64_t n) {
     if (n == 0) return -1;
+    // the root's child offset, far form included: the traversal reads it like
+    // any other descriptor's (src/VoxelOctree.cpp:276-280)
+    uint64_t rootOff = w[0] >> 18;
+    if (w[0] & 0x20000u) {
+        if (n < 2) return -1;
+        rootOff = (rootOff << 32) | w[1];
+    }
+    if ((w[0] & 0xFF00u) && rootOff == 0) return -1;
     std::vector<Item> frontier;
     const int split = 3;
-    int depth = validate_from(w, n, {0, w[0] >> 18, 0}, split, &frontier);
+    int depth = validate_from(w, n, {0, rootOff, 0}, split, &frontier);
     if (depth < 0) return -1;
     depth = std::max(depth, 1);
     if (frontier.empty()) return depth;

@@ -111,6 +111,46 @@ def test_load_errors_are_reported(svo, golden, tmp_path):
     assert e.value.code == svo.svo.ERR_FORMAT
 
 
+def relocated(words, base):
+    """The tree with every word but the root moved up by `base` and reached
+    through a far pointer in the root (a root the reference never writes, but
+    one a crafted file can hold and the traversal follows)."""
+    n = len(words)
+    root = int(words[0])
+    assert not root & 0x20000
+    target = base + (root >> 18)
+    out = np.zeros(base + n, np.uint32)
+    out[base + 1:] = words[1:]
+    out[0] = ((target >> 32) << 18) | (1 << 17) | (root & 0x1FFFF)
+    out[1] = target & 0xFFFFFFFF
+    return out
+
+
+def test_far_root_is_validated_like_any_descriptor(svo, golden):
+    c, w = orc.read_oct(os.path.join(golden, "dragon.oct"))
+    base = 1000
+    moved = relocated(w, base)
+    t = svo.VoxelOctree.from_words(moved, c)
+    assert t.depth() == 9
+    rng = np.random.default_rng(3)
+    o = rng.uniform(-0.5, 3.5, (3000, 3)).astype(np.float32)
+    d = (rng.uniform(0.9, 2.1, (3000, 3)).astype(np.float32) - o).astype(np.float32)
+    ref = orc.raymarch(w, o, d, 0.0)
+    got = orc.raymarch(moved, o, d, 0.0)
+    leaf = ref["hit"] == 1
+    assert leaf.sum() > 300
+    assert np.array_equal(got["hit"], ref["hit"]) and np.array_equal(got["normal"], ref["normal"])
+    assert np.array_equal(got["t"].view(np.uint32), ref["t"].view(np.uint32))
+    assert np.array_equal(got["leaf"][leaf], ref["leaf"][leaf] + np.uint64(base))
+    # past the end; offset 0; and a far root whose high bits name a valid
+    # one-leaf block at word 1 while the traversal's offset is (1 << 32) | w[1]
+    crafted = np.array([(1 << 18) | (1 << 17) | (0x01 << 8), 0x1234], np.uint32)
+    for bad in (moved[:base], np.array([moved[0], 0] + [0] * 8, np.uint32), crafted):
+        with pytest.raises(svo.SvoError) as e:
+            svo.VoxelOctree.from_words(bad, c)
+        assert e.value.code == svo.svo.ERR_FORMAT
+
+
 def test_host_camera_matches_reference(svo, golden):
     g = np.load(os.path.join(golden, "camera.npz"))
     for inp, tf in zip(g["inputs"], g["tform"]):
