@@ -376,6 +376,11 @@ class VoxelOctree:
         _check(lib.svo_center(self._h, _p(c)))
         return c
 
+    def word_count(self):
+        n = ctypes.c_uint64()
+        _check(lib.svo_word_count(self._h, ctypes.byref(n)))
+        return n.value
+
     def words(self):
         n = ctypes.c_uint64()
         _check(lib.svo_word_count(self._h, ctypes.byref(n)))

@@ -126,6 +126,56 @@ def test_concurrent_first_use_of_a_handle(svo, golden):
         assert rays[k] == t.raymarch(o[k], d[k], 0.0)
 
 
+def test_octree_beyond_2_32_words(svo, golden):
+    """A tree of more than 2^32 words runs the 64-bit-index kernels: the
+    dragon moved to word 2^32 + 8 behind a far root (16 GiB, mostly zero
+    pages) gives the dragon's hits, normals and t, with leaf indices moved by
+    the same amount, for arbitrary rays, primary renders, viewer frames and a
+    batched launch."""
+    import torch
+    from test_format import relocated
+    c, w = orc.read_oct(os.path.join(golden, "dragon.oct"))
+    base = (1 << 32) + 8
+    print("[gpu] 2^32+ word octree: building the relocated array", flush=True)
+    moved = relocated(w, base)
+    big = svo.VoxelOctree.from_words(moved, c)
+    assert big.word_count() == base + len(w) and big.depth() == 9
+    small = svo.VoxelOctree(os.path.join(golden, "dragon.oct"))
+    shift = np.uint64(base)
+
+    def shifted(r):
+        r = dict(r)
+        leaf = r["hit"] == 1
+        r["leaf"] = np.where(leaf, r["leaf"] + shift, r["leaf"])
+        return r
+
+    rng = np.random.default_rng(8)
+    o = rng.uniform(-0.5, 3.5, (20000, 3)).astype(np.float32)
+    d = (rng.uniform(0.9, 2.1, (20000, 3)).astype(np.float32) - o).astype(np.float32)
+    print("[gpu] 2^32+ word octree: rays", flush=True)
+    mine = big.raymarch_batch(o, d, 0.0)
+    assert (mine["hit"] == 1).sum() > 2000
+    _same(mine, shifted(small.raymarch_batch(o, d, 0.0)), "rays")
+    _same({k: v[:500] for k, v in mine.items()}, orc.raymarch(moved, o[:500], d[:500], 0.0), "rays vs oracle")
+    del moved
+    cam = small.orbit_camera(20.0, 35.0, 1.0)
+    print("[gpu] 2^32+ word octree: primary, frame, batch", flush=True)
+    _same(big.render_primary(cam, 320, 180), shifted(small.render_primary(cam, 320, 180)), "primary")
+    rgba, depth = big.render_frame(cam, 320, 176, 16, 1)
+    ref_rgba, ref_depth = small.render_frame(cam, 320, 176, 16, 1)
+    assert np.array_equal(rgba, ref_rgba) and np.array_equal(_bits(depth), _bits(ref_depth))
+    dev = torch.device("cuda", 0)
+    ren = svo.svo.Renderer(4)
+    o4 = {k: torch.zeros(96 * 64, dtype=dt, device=dev)
+          for k, dt in (("hit", torch.uint8), ("normal", torch.int32), ("t", torch.float32), ("leaf", torch.int64))}
+    ren.render_device([ren.view(big, cam, 96, 64, o4["hit"].data_ptr(), o4["normal"].data_ptr(), o4["t"].data_ptr(),
+                                o4["leaf"].data_ptr())], None, 0, 0)
+    torch.cuda.synchronize()
+    got = {"hit": o4["hit"].cpu().numpy(), "normal": o4["normal"].cpu().numpy().view(np.uint32),
+           "t": o4["t"].cpu().numpy(), "leaf": o4["leaf"].cpu().numpy().view(np.uint64)}
+    _same(got, shifted(small.render_primary(cam, 96, 64)), "batched")
+
+
 def test_single_ray_entry_matches_batch(trees):
     t = trees["dragon"]
     rng = np.random.default_rng(5)
