@@ -59,7 +59,9 @@ int svo_device_count(void);
 /* ---- .oct files --------------------------------------------------------- */
 
 /* VoxelOctree::VoxelOctree(const char *path), src/VoxelOctree.cpp:57-90. The
- * word array is validated (every child reference in range) before use. */
+ * word array is validated before use: every child reference in range, each
+ * non-leaf slot the child of one parent, at most 23 levels (SVO_ERR_FORMAT
+ * otherwise). */
 int svo_load(const char *path, svo_octree **out);
 /* Same, from an in-memory .oct image. */
 int svo_load_memory(const void *data, uint64_t size, svo_octree **out);

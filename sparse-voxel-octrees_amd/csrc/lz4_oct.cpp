@@ -323,7 +323,13 @@ int oct_decode(const uint8_t *data, uint64_t size, svo_octree &h) {
         std::memcpy(&cs, data + p, 8);
         p += 8;
         if (cs > size - p) return fail(SVO_ERR_FORMAT, ".oct: truncated slice");
-        refs.push_back({p, cs, off, bytes - off < kSlice ? bytes - off : kSlice});
+        const uint64_t len = bytes - off < kSlice ? bytes - off : kSlice;
+        // An LZ4 sequence of k input bytes writes at most 255*k output bytes
+        // (a match of 4 + 15 + 255*(e-1) + 254 bytes costs 3 + e bytes), so a
+        // slice shorter than len/255 cannot decode to len bytes. Checking it
+        // here bounds the word array by the file's size before allocating.
+        if (len > 255 * cs) return fail(SVO_ERR_FORMAT, ".oct: slice too short for its decoded size");
+        refs.push_back({p, cs, off, len});
         p += cs;
     }
     try {

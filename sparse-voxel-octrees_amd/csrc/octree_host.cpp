@@ -11,6 +11,8 @@
 
 #include <algorithm>
 #include <atomic>
+#include <cstdlib>
+#include <memory>
 #include <thread>
 #include <vector>
 
@@ -27,10 +29,13 @@ struct Item {
 
 // Checks the subtree below `it` (descriptor at it.idx, child block at
 // it.idx + it.off). Returns the depth it reaches (0 when it has no children
-// of its own) or -1 if a pointer leaves the array or the tree is deeper than
-// the traversal can follow. With `frontier`, nodes at level `split` are
-// handed out instead of descended into.
-int validate_from(const uint32_t *w, uint64_t n, Item root, int split, std::vector<Item> *frontier) {
+// of its own) or -1 if a pointer leaves the array, the tree is deeper than
+// the traversal can follow, or a non-leaf slot is the child of two parents
+// (bit set in `seen`, one bit per word, shared by all threads). Child
+// pointers only point forward, so with that rule the check visits each word
+// at most once. With `frontier`, nodes at level `split` are handed out
+// instead of descended into.
+int validate_from(const uint32_t *w, uint64_t n, Item root, int split, std::vector<Item> *frontier, uint64_t *seen) {
     std::vector<Item> stack;
     stack.push_back(root);
     int depth = 0;
@@ -62,6 +67,10 @@ int validate_from(const uint32_t *w, uint64_t n, Item root, int split, std::vect
                     coff = (coff << 32) | w[slot + 1];
                 }
                 if (coff == 0) return -1;  // a node cannot be its own child block
+                // each non-leaf slot has one parent: a slot reached twice would
+                // make the check (and nothing else) revisit shared subtrees
+                const uint64_t bit = uint64_t(1) << (slot & 63);
+                if (__atomic_fetch_or(&seen[slot >> 6], bit, __ATOMIC_RELAXED) & bit) return -1;
                 stack.push_back({slot, coff, it.level + 1});
             }
         }
@@ -71,9 +80,11 @@ int validate_from(const uint32_t *w, uint64_t n, Item root, int split, std::vect
 
 }  // namespace
 
-// Every child pointer reachable from the root stays inside the array and the
-// depth is bounded; returns the octree depth (levels of descriptors + 1) or -1.
-// Subtrees below level 3 are checked on all host threads.
+// Every child pointer reachable from the root stays inside the array, no
+// non-leaf slot is shared, and the depth is bounded; returns the octree depth
+// (levels of descriptors + 1), -1 for a malformed tree or -2 when the check's
+// bitmap (one bit per word) cannot be allocated. Subtrees below level 3 are
+// checked on all host threads.
 int octree_validate(const uint32_t *w, uint64_t n) {
     if (n == 0) return -1;
     // the root's child offset, far form included: the traversal reads it like
@@ -84,9 +95,14 @@ int octree_validate(const uint32_t *w, uint64_t n) {
         rootOff = (rootOff << 32) | w[1];
     }
     if ((w[0] & 0xFF00u) && rootOff == 0) return -1;
+    std::unique_ptr<uint64_t, void (*)(void *)> seenMem(
+        static_cast<uint64_t *>(std::calloc(size_t((n + 63) / 64), sizeof(uint64_t))), std::free);
+    if (!seenMem) return -2;
+    uint64_t *seen = seenMem.get();
+    seen[0] = 1;  // the root
     std::vector<Item> frontier;
     const int split = 3;
-    int depth = validate_from(w, n, {0, rootOff, 0}, split, &frontier);
+    int depth = validate_from(w, n, {0, rootOff, 0}, split, &frontier, seen);
     if (depth < 0) return -1;
     depth = std::max(depth, 1);
     if (frontier.empty()) return depth;
@@ -95,7 +111,7 @@ int octree_validate(const uint32_t *w, uint64_t n) {
     std::atomic<bool> bad{false};
     auto worker = [&]() {
         for (size_t i; !bad.load() && (i = next.fetch_add(1)) < frontier.size();) {
-            int d = validate_from(w, n, frontier[i], split, nullptr);
+            int d = validate_from(w, n, frontier[i], split, nullptr, seen);
             if (d < 0) {
                 bad.store(true);
                 return;

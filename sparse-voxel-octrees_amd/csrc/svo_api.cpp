@@ -28,7 +28,10 @@ static int finish_handle(std::unique_ptr<svo_octree> &h, svo_octree **out) {
     int depth = octree_validate(h->words.data(), h->words.size());
     if (depth < 0) {
         release_device(*h);
-        return fail(SVO_ERR_FORMAT, "malformed octree: a child reference leaves the word array");
+        if (depth == -2) return fail(SVO_ERR_NOMEM, "out of host memory (octree check)");
+        return fail(SVO_ERR_FORMAT,
+                    "malformed octree: a child reference leaves the word array, a subtree has two parents, "
+                    "or the tree is deeper than the traversal follows");
     }
     h->depth = depth;
     *out = h.release();

@@ -102,7 +102,9 @@ void oct_encode(const svo_octree &h, std::vector<uint8_t> &out);
 
 // ---- octree structure (octree_host.cpp) ----
 // Validates every child reference of the tree reachable from the root and
-// returns its depth (descriptor levels + 1), or -1 if malformed.
+// returns its depth (descriptor levels + 1), -1 if malformed (a reference
+// out of the array, a non-leaf slot with two parents, too deep), or -2 if
+// the check's one-bit-per-word bitmap cannot be allocated.
 int octree_validate(const uint32_t *words, uint64_t n);
 
 // ---- camera (camera.cpp) ----

@@ -111,6 +111,79 @@ def test_load_errors_are_reported(svo, golden, tmp_path):
     assert e.value.code == svo.svo.ERR_FORMAT
 
 
+
+def test_corrupt_streams_load_valid_trees_or_fail_cleanly(svo, golden):
+    """Random corruptions of a reference .oct (byte flips in the header, the
+    slice header and the LZ4 body, overwritten runs, truncations) either
+    decode to a tree the structural check accepts or are refused with
+    ERR_FORMAT; none may crash the process."""
+    raw = np.frombuffer(open(os.path.join(golden, "dragon.oct"), "rb").read(), np.uint8)
+    rng = np.random.default_rng(2024)
+    loaded = refused = 0
+    for i in range(300):
+        b = raw.copy()
+        kind = i % 4
+        if kind == 0:  # a few flipped bits in the first 4 KiB
+            for _ in range(int(rng.integers(1, 5))):
+                b[int(rng.integers(0, 4096))] ^= np.uint8(1 << int(rng.integers(0, 8)))
+        elif kind == 1:  # flipped bits anywhere
+            for _ in range(int(rng.integers(1, 9))):
+                b[int(rng.integers(0, len(b)))] ^= np.uint8(1 << int(rng.integers(0, 8)))
+        elif kind == 2:  # a random run overwritten
+            at, k = int(rng.integers(0, len(b) - 64)), int(rng.integers(1, 65))
+            b[at:at + k] = rng.integers(0, 256, k, dtype=np.uint8)
+        else:  # truncated
+            b = b[:int(rng.integers(0, len(b)))]
+        try:
+            t = svo.VoxelOctree.from_bytes(b.tobytes())
+        except svo.SvoError as e:
+            assert e.code == svo.svo.ERR_FORMAT, (i, e.code, str(e))
+            refused += 1
+            continue
+        assert 1 <= t.depth() <= 23 and t.word_count() >= 1
+        loaded += 1
+    assert refused > 100 and loaded > 10, (loaded, refused)
+
+
+def test_word_count_claim_is_bounded_by_the_stream(svo):
+    """A header claiming 2^38 words with one empty slice record per 64 MiB
+    (128 KiB of file) is refused before the word array is allocated: LZ4
+    writes at most 255 bytes per input byte."""
+    n = 1 << 38
+    slices = (n * 4) >> 26
+    data = np.zeros(3, np.float32).tobytes() + int(n).to_bytes(8, "little") + bytes(8 * slices)
+    with pytest.raises(svo.SvoError) as e:
+        svo.VoxelOctree.from_bytes(data)
+    assert e.value.code == svo.svo.ERR_FORMAT and "too short" in str(e.value)
+
+
+def test_shared_subtrees_are_refused_at_once(svo):
+    """Twenty levels of descriptor blocks where all eight children of a node
+    point at the same next block, then one block of leaves (a DAG the
+    reference never writes: each non-leaf child gets its own block), make
+    8^20 root-to-leaf paths out of 169 words, every pointer in range. The
+    check refuses a slot's second parent instead of walking the paths."""
+    import time
+    levels = 20
+
+    def chain(nonleaf):
+        w = np.full(1 + 8 * (levels + 1), 0x00808080, np.uint32)  # last block: leaf material words
+        w[0] = (1 << 18) | (0xFF << 8) | nonleaf
+        for j in range(levels):
+            blk, nxt = 1 + 8 * j, 1 + 8 * (j + 1)
+            mask = 0 if j == levels - 1 else nonleaf
+            for k in range(8 if nonleaf == 0xFF else 1):
+                w[blk + k] = ((nxt - (blk + k)) << 18) | (0xFF << 8) | mask
+        return w
+
+    t0 = time.perf_counter()
+    with pytest.raises(svo.SvoError) as e:
+        svo.VoxelOctree.from_words(chain(0xFF), [0.5, 0.5, 0.5])
+    assert e.value.code == svo.svo.ERR_FORMAT
+    assert time.perf_counter() - t0 < 1.0
+    # one non-leaf child per node (the other seven are leaf words): a tree
+    assert svo.VoxelOctree.from_words(chain(0x01), [0.5, 0.5, 0.5]).depth() == levels + 2
+
 def relocated(words, base):
     """The tree with every word but the root moved up by `base` and reached
     through a far pointer in the root (a root the reference never writes, but
