@@ -10,6 +10,7 @@
 // Big-endian files are rejected: the reference converts before swapping bytes
 // (PlyLoader.cpp:104-120,152), so it has no meaningful output for them.
 #include <cerrno>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -135,6 +136,18 @@ int ply_open(const char *path_in, PlyLayout &L) {
         }
     }
     if (!ended || format < 0) return fail(SVO_ERR_FORMAT, "PLY header incomplete");
+    // Each record takes at least: the sizes of its scalars and list counts
+    // (binary), or one character and one separator per property (ascii; the
+    // file's last token may end without a separator). Counts the body cannot
+    // hold are refused here, before anything is sized or looped by them;
+    // elements without properties are capped at INT32_MAX records.
+    const uint64_t bodyBytes = buf.size() - pos;
+    for (const PlyElem &e : elems) {
+        uint64_t rmin = 0;
+        for (const PlyProp &p : e.props) rmin += format == 1 ? uint64_t(type_size(p.list ? p.count_type : p.type)) : 2;
+        const uint64_t cap = rmin ? (bodyBytes + 1) / rmin : uint64_t(INT32_MAX);
+        if (uint64_t(e.count) > cap) return fail(SVO_ERR_FORMAT, "PLY element " + e.name + ": count exceeds the data");
+    }
     L.binary = format == 1;
     L.body = pos;
     L.vertex = L.face = -1;
@@ -207,14 +220,21 @@ int ply_parse_host(const PlyLayout &L, std::vector<float> &verts, int &has_norma
                 if (pr.list) {
                     double cv;
                     if (!read_item(pr.count_type, cv)) return fail(SVO_ERR_FORMAT, "PLY data truncated");
-                    int64_t cnt = int64_t(cv);
-                    if (cnt < 0) return fail(SVO_ERR_FORMAT, "negative list length");
+                    if (!(cv >= 0.0)) return fail(SVO_ERR_FORMAT, "negative list length");
+                    if (cv > 4294967295.0) return fail(SVO_ERR_FORMAT, "PLY data truncated");
+                    const int64_t cnt = int64_t(cv);
                     const bool take = isF && pr.name == "vertex_indices";
                     if (take) face_counts.push_back(int32_t(cnt));
                     for (int64_t k = 0; k < cnt; ++k) {
                         double v;
                         if (!read_item(pr.type, v)) return fail(SVO_ERR_FORMAT, "PLY data truncated");
-                        if (take) face_idx.push_back(int32_t(v));
+                        if (take) {
+                            // int conversion as plyfile.c's store_item; out of
+                            // int range (or NaN) the index is missing anyway
+                            if (!(v > -2147483649.0 && v < 2147483648.0))
+                                return fail(SVO_ERR_FORMAT, "face references a missing vertex");
+                            face_idx.push_back(int32_t(v));
+                        }
                     }
                 } else {
                     double v;

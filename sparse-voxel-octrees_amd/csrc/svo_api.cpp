@@ -665,7 +665,28 @@ int svo_build_from_voxels(const char *path, const svo_build_opts *opts, svo_octr
             std::fclose(fp);
             return fail(SVO_ERR_FORMAT, ".voxel: bad header");
         }
-        size_t n = size_t(dims[0]) * size_t(dims[1]) * size_t(dims[2]);
+        // the volume's word count, checked against the words the file holds
+        // (w*h*d of them follow the header) before anything is sized by it
+        uint64_t avail = 0;
+        if (std::fseek(fp, 0, SEEK_END) == 0) {
+            const long long end = ftello(fp);
+            if (end >= 12) avail = uint64_t(end - 12) / 4;
+        }
+        if (std::fseek(fp, 12, SEEK_SET) != 0) {
+            std::fclose(fp);
+            return fail(SVO_ERR_IO, std::string("cannot read ") + path);
+        }
+        uint64_t n = uint64_t(dims[0]);
+        bool fits = n <= avail && uint64_t(dims[1]) <= avail / n;
+        if (fits) {
+            n *= uint64_t(dims[1]);
+            fits = uint64_t(dims[2]) <= avail / n;
+        }
+        if (!fits) {
+            std::fclose(fp);
+            return fail(SVO_ERR_FORMAT, ".voxel: truncated (the header's volume exceeds the file)");
+        }
+        n *= uint64_t(dims[2]);
         std::vector<uint32_t> vol;
         try {
             vol.resize(n);

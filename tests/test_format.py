@@ -17,6 +17,15 @@ import oracle_py as orc
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
+
+def _device_count_or_none():
+    """Visible HIP devices as the library counts them (None if it cannot load)."""
+    try:
+        import conftest
+        return conftest.load_package().svo.device_count()
+    except Exception:
+        return None
+
 def test_library_exports_every_declared_symbol(svo):
     header = open(os.path.join(ROOT, "include", "svo.h")).read()
     declared = set(re.findall(r"\b(svo_[a-z_0-9]+)\s*\(", header))
@@ -183,6 +192,85 @@ def test_shared_subtrees_are_refused_at_once(svo):
     assert time.perf_counter() - t0 < 1.0
     # one non-leaf child per node (the other seven are leaf words): a tree
     assert svo.VoxelOctree.from_words(chain(0x01), [0.5, 0.5, 0.5]).depth() == levels + 2
+
+
+def _ply(tmp_path, name, header, body=b""):
+    path = tmp_path / name
+    path.write_bytes(("\n".join(["ply"] + header + ["end_header"]) + "\n").encode() + body)
+    return str(path)
+
+
+def test_crafted_ply_counts_are_refused_before_sizing(svo, tmp_path):
+    """Element counts the body cannot hold are refused from the header, before
+    anything is sized by them: a property-less vertex element whose count
+    times 9 floats wraps to 2 (a heap overflow in an unchecked parser), counts
+    beyond the body of binary and ascii files, a list length of 1e30 and a
+    face index beyond int range."""
+    face = ["element face 1", "property list uchar int vertex_indices"]
+    xyz = ["property float x", "property float y", "property float z"]
+    cases = [
+        _ply(tmp_path, "wrap.ply", ["format ascii 1.0", "element vertex 2049638230412172402"] + face, b"3 0 0 0\n"),
+        _ply(tmp_path, "bin.ply", ["format binary_little_endian 1.0", "element vertex 1000000"] + xyz + face,
+             bytes(64)),
+        _ply(tmp_path, "asc.ply", ["format ascii 1.0", "element vertex 1000000"] + xyz + face, b"0 0 0\n3 0 0 0\n"),
+        _ply(tmp_path, "list.ply", ["format ascii 1.0", "element vertex 3"] + xyz +
+             ["element face 1", "property list float int vertex_indices"], b"0 0 0\n1 0 0\n0 1 0\n1e30 0 1 2\n"),
+        _ply(tmp_path, "idx.ply", ["format ascii 1.0", "element vertex 3"] + xyz +
+             ["element face 1", "property list uchar float vertex_indices"], b"0 0 0\n1 0 0\n0 1 0\n3 0 1 1e12\n"),
+    ]
+    for path in cases:
+        with pytest.raises(svo.SvoError) as e:
+            svo.VoxelOctree.build_from_ply(path, 64)
+        assert e.value.code == svo.svo.ERR_FORMAT, (path, str(e.value))
+
+
+@pytest.mark.skipif(_device_count_or_none() != 0, reason="host-parse fuzz: with a device the mutated meshes would build")
+def test_corrupt_ply_files_fail_cleanly(svo, golden, tmp_path):
+    """Random corruptions of an ascii and a binary PLY (header and body bytes,
+    truncations) are refused or reach the device step (ERR_DEVICE where no
+    device is present); none may crash the process."""
+    scenes = svo.scenes
+    v, f = scenes.icosphere(4, seed=7, displacement=0.05)
+    srcs = []
+    for ascii in (True, False):
+        pth = str(tmp_path / f"src_{int(ascii)}.ply")
+        scenes.write_ply(pth, v, f, normals=v, colors=np.full((len(v), 3), 200, np.uint8), ascii=ascii)
+        srcs.append(np.frombuffer(open(pth, "rb").read(), np.uint8))
+    rng = np.random.default_rng(99)
+    allowed = {svo.svo.ERR_FORMAT, svo.svo.ERR_UNSUPPORTED, svo.svo.ERR_DEVICE, svo.svo.ERR_LIMIT}
+    seen = set()
+    for i in range(400):
+        b = srcs[i % 2].copy()
+        hdr = int(bytes(b).index(b"end_header")) + 11
+        for _ in range(int(rng.integers(1, 6))):
+            k = int(rng.integers(0, hdr)) if rng.random() < 0.5 else int(rng.integers(0, len(b)))
+            b[k] = np.uint8(rng.integers(0, 256)) if rng.random() < 0.5 else np.uint8(b"0123456789 \n-e."[rng.integers(0, 15)])
+        if rng.random() < 0.2:
+            b = b[:int(rng.integers(0, len(b)))]
+        pth = tmp_path / "m.ply"
+        pth.write_bytes(b.tobytes())
+        try:
+            svo.VoxelOctree.build_from_ply(str(pth), 32)
+            raise AssertionError("built without a device")
+        except svo.SvoError as e:
+            assert e.code in allowed, (i, e.code, str(e))
+            seen.add(e.code)
+    assert {svo.svo.ERR_FORMAT, svo.svo.ERR_DEVICE} <= seen
+
+
+def test_voxel_header_is_checked_against_the_file(svo, tmp_path):
+    """A .voxel header whose w*h*d exceeds the words after it (a 12-byte
+    file claiming 2048^3, or sides whose product wraps 64 bits) is refused
+    before the volume is allocated."""
+    import time
+    for dims in ((2048, 2048, 2048), (2**31 - 1,) * 3, (4, 4, 5)):
+        path = tmp_path / "v.voxel"
+        path.write_bytes(np.int32(dims).tobytes() + np.zeros(4 * 4 * 4, np.uint32).tobytes())
+        t0 = time.perf_counter()
+        with pytest.raises(svo.SvoError) as e:
+            svo.VoxelOctree.build_from_voxels(str(path))
+        assert e.value.code == svo.svo.ERR_FORMAT and "exceeds the file" in str(e.value), dims
+        assert time.perf_counter() - t0 < 1.0
 
 def relocated(words, base):
     """The tree with every word but the root moved up by `base` and reached
