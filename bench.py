@@ -183,8 +183,8 @@ def run_mixed(pkg, torch, stream, steps, warmup, waves_per_cu, refill):
             trees[v["tree"]].render_primary_device(cams[i], v["width"], v["height"], None, outs[i][0].data_ptr(),
                                                    outs[i][1].data_ptr(), outs[i][2].data_ptr(), sptr)
 
-    def batched(w=waves_per_cu, r=refill):
-        ren.render_device(bviews, sptr, w, r)
+    def batched(w=waves_per_cu, r=refill, f=None):
+        ren.render_device(bviews, sptr, w, r, f)
 
     def timed(fn, k):
         for _ in range(warmup):
@@ -198,12 +198,13 @@ def run_mixed(pkg, torch, stream, steps, warmup, waves_per_cu, refill):
         torch.cuda.synchronize()
         return a.elapsed_time(b) / k
 
-    # tile-queue launch shape: the library's autotuner on this workload
+    # launch form and shape: the library's autotuner on this workload
     tune = {}
+    form = None
     if not waves_per_cu and not refill:
-        (waves_per_cu, refill), tune = ren.autotune(bviews, sptr)
+        (form, waves_per_cu, refill), tune = ren.autotune(bviews, sptr)
     plain_ms = timed(plain, steps)
-    batched_ms = timed(lambda: batched(waves_per_cu, refill), steps)
+    batched_ms = timed(lambda: batched(waves_per_cu, refill, form), steps)
     # parity: the batched launch reproduces the plain launches bit for bit
     plain()
     torch.cuda.synchronize()
@@ -212,7 +213,7 @@ def run_mixed(pkg, torch, stream, steps, warmup, waves_per_cu, refill):
         a.fill_(-1)
         b.fill_(-1.0)
         c.fill_(-1)
-    batched(waves_per_cu, refill)
+    batched(waves_per_cu, refill, form)
     torch.cuda.synchronize()
     mism = sum(int((a != sa).sum() + (b.view(torch.int32) != sb.view(torch.int32)).sum() + (c != sc).sum())
                for (a, b, c), (sa, sb, sc) in zip(outs, snap))
@@ -221,7 +222,8 @@ def run_mixed(pkg, torch, stream, steps, warmup, waves_per_cu, refill):
             "plain_grays_s": round(rays / (plain_ms / 1e3) / 1e9, 4), "plain_ms_per_step": round(plain_ms, 3),
             "batched_grays_s": round(rays / (batched_ms / 1e3) / 1e9, 4), "batched_ms_per_step": round(batched_ms, 3),
             "batched_vs_plain": round(plain_ms / batched_ms, 3),
-            "launch": {"waves_per_cu": waves_per_cu, "refill": refill, "autotune_ms": tune},
+            "launch": {"form": "grid" if form == pkg.svo.BATCH_GRID else "persistent", "waves_per_cu": waves_per_cu,
+                       "refill": refill, "autotune_ms": tune},
             "mismatched_fields_vs_plain": mism}
 
 
@@ -379,6 +381,37 @@ def run_frames(pkg, torch, stream, tree, cams, width, height, steps, warmup, cpu
            "ms_per_frame": round(ms / len(cams), 4), "frames_per_s": round(len(cams) / (ms / 1e3), 1),
            "rays_per_step": traced, "grays_s": round(traced / (ms / 1e3) / 1e9, 4),
            "fine_pixels_traced_frac": round((traced - nd * len(cams)) / (m * len(cams)), 4)}
+    # The same frames with consecutive frames on two streams (default priority):
+    # a frame's coarse pass, a few hundred waves that leave most of the GPU idle,
+    # runs beside the previous frame's fine pass. Every frame writes its own
+    # buffers; the images and depth tables must equal the one-stream ones.
+    s2 = torch.cuda.Stream()
+    sptrs = [sptr, s2.cuda_stream]
+    rgba2 = [torch.empty_like(x) for x in rgba]
+    depth2 = [torch.empty_like(x) for x in depth]
+
+    def step2():
+        for k, c in enumerate(cams):
+            tree.render_frame_device(c, width, height, rgba2[k].data_ptr(), depth2[k].data_ptr(), 16, 1, sptrs[k % 2])
+
+    for _ in range(max(warmup, 1)):
+        step2()
+    torch.cuda.synchronize()
+    a.record(stream)
+    s2.wait_event(a)
+    for _ in range(steps):
+        step2()
+    done2 = torch.cuda.Event()
+    done2.record(s2)
+    stream.wait_event(done2)
+    b.record(stream)
+    torch.cuda.synchronize()
+    ms2 = a.elapsed_time(b) / steps
+    diff = sum(int(torch.count_nonzero(rgba2[k] != rgba[k])) + int(torch.count_nonzero(
+        depth2[k].view(torch.int32) != depth[k].view(torch.int32))) for k in range(len(cams)))
+    out["two_streams"] = {"ms_per_frame": round(ms2 / len(cams), 4), "frames_per_s": round(len(cams) / (ms2 / 1e3), 1),
+                          "vs_one_stream": round(ms / ms2, 3), "mismatched_vs_one_stream": diff,
+                          "form": "consecutive frames alternate between two streams"}
     if cpu_check:
         words = tree.words()
         tf = np.frombuffer(cams[0].tform, np.float32)
@@ -605,7 +638,9 @@ def main():
     graph_ms = time_graph()
     batched = {"value": round(n * V / (batched_ms / 1e3) / 1e9, 4), "unit": "Grays/s",
                "ms_per_step": round(batched_ms, 4), "vs_plain": round((total_ms / args.steps) / batched_ms, 3),
-               "form": f"one persistent launch for all {V} views",
+               "form": (f"one launch of every tile of the {V} views (one wave per 8x8 tile)"
+                        if not (args.waves_per_cu or args.refill) else
+                        f"one persistent launch for all {V} views (waves/CU {args.waves_per_cu}, refill {args.refill})"),
                "graph_replay": {"value": round(n * V / (graph_ms / 1e3) / 1e9, 4), "ms_per_step": round(graph_ms, 4)}}
 
     if args.no_cpu:
@@ -669,6 +704,10 @@ def main():
     reads_per_ray = float(counts.sum()) / cpu_rays
     b_ray = 4.0 * reads_per_ray + 16.0
     achieved = b_ray * n / avg_launch_s / 1e9
+    # the batched step is one launch of every view's tiles: its time is the launch's
+    batched_gbs = b_ray * n * V / (batched_ms / 1e3) / 1e9
+    batched["roofline"] = {"bound": "hbm", "achieved": round(batched_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": round(batched_gbs / HBM_PEAK_GBS, 5), "bytes_per_ray": round(b_ray, 2)}
     port_rate = cpu_rays / cpu_time / 1e9
     cpu_baseline = {"value": round(port_rate, 6), "unit": "Grays/s", "cores": args.cpu_threads, "kind": "port",
                     "sample": f"{cpu_rays} primary rays ({cpu_rays // n} of the {V} views, full {W}x{H}) through "

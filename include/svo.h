@@ -202,10 +202,19 @@ typedef struct {
     uint64_t *leaf;
 } svo_view;
 
+/* Launch forms of a batched render (svo_batch_opts.form):
+ *  SVO_BATCH_GRID: one ordinary launch with one wave per 8x8 tile of every
+ *    view; a view's slowest tiles (rays grazing the surface) then overlap the
+ *    other views' tiles instead of finishing on an idle GPU.
+ *  SVO_BATCH_PERSISTENT: persistent waves over a tile queue, handing a wave's
+ *    idle lanes new pixels once refill_threshold of them are idle. */
+enum { SVO_BATCH_GRID = 0, SVO_BATCH_PERSISTENT = 1 };
+
 typedef struct {
-    int waves_per_cu;     /* resident waves per CU for the persistent grid; 0 = 32 (LDS permitting) */
-    int refill_threshold; /* idle lanes of a wave (1..64) that trigger handing out new pixels; 0 = 16 */
-    int reserved[6];
+    int waves_per_cu;     /* persistent form: resident waves per CU; 0 = 32 (LDS permitting) */
+    int refill_threshold; /* persistent form: idle lanes of a wave (1..64) that trigger new pixels; 0 = 16 */
+    int form;             /* SVO_BATCH_GRID (default) or SVO_BATCH_PERSISTENT */
+    int reserved[5];
 } svo_batch_opts;
 
 /* A renderer owns the device view table and tile-queue counters of batched
@@ -217,17 +226,19 @@ typedef struct svo_renderer svo_renderer;
 int svo_renderer_create(int max_views, svo_renderer **out);
 void svo_renderer_free(svo_renderer *r);
 
-/* Traces every view in one persistent launch over a single queue of 8x8
- * pixel tiles (Morton order within 64x64-pixel blocks, one queue chunk per
- * XCD). Per-ray results equal svo_render_primary's. Enqueued on `stream`. */
+/* Traces every view in one call over a single sequence of 8x8 pixel tiles
+ * (Morton order within 64x64-pixel blocks), in the launch form of `opts`
+ * (NULL: SVO_BATCH_GRID). Per-ray results equal svo_render_primary's.
+ * Enqueued on `stream`. */
 int svo_render_batch_device(svo_renderer *r, const svo_view *views, int nviews, const svo_batch_opts *opts,
                             void *stream);
 
-/* Launch-shape autotuner: times the batched launch of `views` on `stream` for
- * waves per CU {16, 24, 32} x refill threshold {8, 16, 32} (one warm-up and
- * two timed launches each; the launches write the views' outputs) and stores
- * the fastest in *best. times (may be NULL) receives the 9 averages in ms,
- * waves-major. */
+/* Launch-shape autotuner: times the batched launch of `views` on `stream` in
+ * the grid form and in the persistent form for waves per CU {16, 24, 32} x
+ * refill threshold {16, 32, 64} (one warm-up and two timed launches each; the
+ * launches write the views' outputs) and stores the fastest in *best. times
+ * (may be NULL) receives the 10 averages in ms: the grid form, then the
+ * persistent shapes waves-major. */
 int svo_renderer_autotune(svo_renderer *r, const svo_view *views, int nviews, void *stream, svo_batch_opts *best,
                           float *times);
 
@@ -263,7 +274,8 @@ typedef struct {
     int one_per_launch;     /* baseline: one request per launch, plain kernel */
     int waves_per_cu;       /* batched launch parameters (svo_batch_opts) */
     int refill_threshold;
-    int reserved[6];
+    int batch_form;         /* SVO_BATCH_GRID (default) or SVO_BATCH_PERSISTENT */
+    int reserved[5];
 } svo_server_opts;
 
 /* Runs the request loop over a known arrival schedule (arrival_s ascending,

@@ -187,6 +187,54 @@ __global__ void __launch_bounds__(256) batch_kernel(const ViewDesc *__restrict__
     }
 }
 
+// Grid form: one ordinary launch with one workgroup (one wave) per 8x8 tile
+// of every view, in the queue's tile order. The hardware dispatcher hands out
+// the tiles; a wave traces its tile's 64 rays and exits. A single view's
+// launch ends with a tail where a few slow tiles (rays that graze the
+// surface) run on an otherwise idle GPU; with many views in one launch that
+// tail is paid once. `first` offsets blockIdx for grids split into several
+// launches.
+template <bool Lod, typename Idx, int Reg>
+__global__ void __launch_bounds__(64) tiles_kernel(const ViewDesc *__restrict__ views,
+                                                   const uint64_t *__restrict__ tileBase, int nviews, uint64_t first,
+                                                   int slots) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int lane = threadIdx.x;
+    const uint64_t tile = first + blockIdx.x;
+    const int v = view_of(tileBase, nviews, tile);
+    const ViewDesc &vd = views[v];
+    const uint64_t local = tile - tileBase[v];
+    const uint32_t m = uint32_t(local & 63);
+    const uint64_t block = local >> 6;
+    const int tx = int(block % uint64_t(vd.blocksX)) * 8 + int(even_bits(m));
+    const int ty = int(block / uint64_t(vd.blocksX)) * 8 + int(even_bits(m >> 1));
+    const int x = tx * 8 + (lane & 7);
+    const int y = ty * 8 + (lane >> 3);
+    if (tx >= vd.tilesX || ty >= vd.tilesY || x >= vd.width || y >= vd.height) return;
+    // ray generation, src/Main.cpp:97-113 with closed-form dx, dy
+    const float dy = vd.aspect - float(y) * vd.scale;
+    const float dx = -1.0f + float(x) * vd.scale;
+    const float vx = dx * vd.a11 + dy * vd.a12 + vd.zx;
+    const float vy = dx * vd.a21 + dy * vd.a22 + vd.zy;
+    const float vz = dx * vd.a31 + dy * vd.a32 + vd.zz;
+    const float s = quake_rsqrt(vx * vx + vy * vy + vz * vz);
+    Stack<Idx> stk(lds, 64, slots, lane);
+    Ray<Idx> r;
+    ray_init(r, vd.tree, vd.px, vd.py, vd.pz, vx * s, vy * s, vz * s, vd.rayScale);
+    uint32_t normal = 0;
+    float t = 0.0f;
+    Idx leaf = 0;
+    int code;
+    do {
+        code = ray_step<Lod, true, Idx, true, Reg>(r, vd.tree, stk, normal, t, leaf);
+    } while (code == kRunning);
+    const int64_t i = int64_t(y) * vd.width + x;
+    if (vd.hit) vd.hit[i] = uint8_t(code);
+    if (vd.normal) vd.normal[i] = code == kLeaf ? normal : 0u;
+    if (vd.t) vd.t[i] = code == kMiss ? 0.0f : t;
+    if (vd.leaf) vd.leaf[i] = code == kLeaf ? uint64_t(leaf) : SVO_LEAF_NONE;
+}
+
 __global__ void zero_counters_kernel(uint32_t *c) {
     if (threadIdx.x < kQueues) c[threadIdx.x * kCounterStride] = 0u;
 }
@@ -259,6 +307,8 @@ struct LaunchPlan {
     size_t ldsPerBlock = 0;
     unsigned grid = 0;
     int refill = 16;
+    bool persistent = false;  // SVO_BATCH_PERSISTENT; otherwise one workgroup per tile
+    size_t ldsPerWave = 0;
 };
 
 // Validates the views and fills the host view table, the tile bases and the launch shape.
@@ -315,7 +365,11 @@ int plan_views(const svo_view *views, int nviews, const svo_batch_opts *opts, in
     plan.wide = wide;
     plan.lod = lod;
     const size_t ldsPerWave = size_t(plan.slots) * 64 * ((wide ? 8 : 4) + sizeof(float));
+    plan.ldsPerWave = ldsPerWave;
     plan.ldsPerBlock = ldsPerWave * 4;
+    const int form = opts ? opts->form : SVO_BATCH_GRID;
+    if (form != SVO_BATCH_GRID && form != SVO_BATCH_PERSISTENT) return fail(SVO_ERR_INVALID, "unknown batch form");
+    plan.persistent = form == SVO_BATCH_PERSISTENT;
     const size_t ldsPerCU = 160 * 1024;
     const int wavesPerCU = (opts && opts->waves_per_cu > 0) ? opts->waves_per_cu : 32;
     const int blocksPerCU = std::max(1, std::min(wavesPerCU / 4, int(ldsPerCU / plan.ldsPerBlock)));
@@ -347,8 +401,34 @@ void launch_batch(const LaunchPlan &p, const ViewDesc *dViews, const uint64_t *d
                            dTileBase, p.nviews, dCounters, p.slots, p.refill);
 }
 
+// Grid form: the tiles in launches of at most 2^31 - 1 workgroups.
+template <int Reg>
+void launch_tiles(const LaunchPlan &p, const ViewDesc *dViews, const uint64_t *dTileBase, hipStream_t s) {
+    const uint64_t maxGrid = 0x7FFFFFFFull;
+    for (uint64_t first = 0; first < p.tiles; first += maxGrid) {
+        const dim3 grid(unsigned(std::min(maxGrid, p.tiles - first)));
+        if (!p.wide && !p.lod)
+            hipLaunchKernelGGL((tiles_kernel<false, uint32_t, Reg>), grid, dim3(64), p.ldsPerWave, s, dViews,
+                               dTileBase, p.nviews, first, p.slots);
+        else if (!p.wide)
+            hipLaunchKernelGGL((tiles_kernel<true, uint32_t, Reg>), grid, dim3(64), p.ldsPerWave, s, dViews,
+                               dTileBase, p.nviews, first, p.slots);
+        else if (!p.lod)
+            hipLaunchKernelGGL((tiles_kernel<false, uint64_t, Reg>), grid, dim3(64), p.ldsPerWave, s, dViews,
+                               dTileBase, p.nviews, first, p.slots);
+        else
+            hipLaunchKernelGGL((tiles_kernel<true, uint64_t, Reg>), grid, dim3(64), p.ldsPerWave, s, dViews,
+                               dTileBase, p.nviews, first, p.slots);
+    }
+}
+
 void launch_plan(const LaunchPlan &p, const ViewDesc *dViews, const uint64_t *dTileBase, uint32_t *dCounters,
                  hipStream_t s) {
+    if (!p.persistent) {
+        if (p.reg == 3) launch_tiles<3>(p, dViews, dTileBase, s);
+        else launch_tiles<1>(p, dViews, dTileBase, s);
+        return;
+    }
     hipLaunchKernelGGL(zero_counters_kernel, dim3(1), dim3(64), 0, s, dCounters);
     if (p.reg == 3) launch_batch<3>(p, dViews, dTileBase, dCounters, s);
     else launch_batch<1>(p, dViews, dTileBase, dCounters, s);
@@ -377,7 +457,7 @@ int render_batch(svo_renderer *rd, const svo_view *views, int nviews, const svo_
 int renderer_autotune(svo_renderer *rd, const svo_view *views, int nviews, void *stream, svo_batch_opts *best,
                       float *times) {
     if (!rd || !views || !best) return fail(SVO_ERR_INVALID, "null argument");
-    static const int kWaves[3] = {16, 24, 32}, kRefill[3] = {8, 16, 32};
+    static const int kWaves[3] = {16, 24, 32}, kRefill[3] = {16, 32, 64};
     hipStream_t s = static_cast<hipStream_t>(stream);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     HIP_TRY(hipEventCreate(&e0));
@@ -388,10 +468,15 @@ int renderer_autotune(svo_renderer *rd, const svo_view *views, int nviews, void 
     }
     float bestMs = 0.0f;
     int r = SVO_OK;
-    for (int i = 0; i < 9 && !r; ++i) {
+    for (int i = 0; i < 10 && !r; ++i) {
         svo_batch_opts o{};
-        o.waves_per_cu = kWaves[i / 3];
-        o.refill_threshold = kRefill[i % 3];
+        if (i == 0) {
+            o.form = SVO_BATCH_GRID;
+        } else {
+            o.form = SVO_BATCH_PERSISTENT;
+            o.waves_per_cu = kWaves[(i - 1) / 3];
+            o.refill_threshold = kRefill[(i - 1) % 3];
+        }
         r = render_batch(rd, views, nviews, &o, stream);  // warm-up
         if (!r && (he = hipEventRecord(e0, s)) != hipSuccess) r = fail(SVO_ERR_DEVICE, hipGetErrorString(he));
         for (int k = 0; k < 2 && !r; ++k) r = render_batch(rd, views, nviews, &o, stream);

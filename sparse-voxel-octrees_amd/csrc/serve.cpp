@@ -84,6 +84,7 @@ int serve_replay(svo_octree *const *trees, int ntrees, const svo_request *reqs, 
     if (opts) {
         bo.waves_per_cu = opts->waves_per_cu;
         bo.refill_threshold = opts->refill_threshold;
+        bo.form = opts->batch_form;
     }
     const uint64_t slab = budget / uint64_t(nslots) / kAlign * kAlign;
     for (int i = 0; i < n; ++i) {

@@ -55,8 +55,24 @@ class View(ctypes.Structure):
                 ("t", ctypes.c_void_p), ("leaf", ctypes.c_void_p)]
 
 
+BATCH_GRID, BATCH_PERSISTENT = 0, 1
+
+
 class BatchOpts(ctypes.Structure):
-    _fields_ = [("waves_per_cu", ctypes.c_int), ("refill_threshold", ctypes.c_int), ("reserved", ctypes.c_int * 6)]
+    _fields_ = [("waves_per_cu", ctypes.c_int), ("refill_threshold", ctypes.c_int), ("form", ctypes.c_int),
+                ("reserved", ctypes.c_int * 5)]
+
+
+def _batch_opts(waves_per_cu, refill_threshold, form):
+    """Options of a batched launch: the grid form unless a form or a
+    persistent launch shape is given."""
+    o = BatchOpts()
+    o.waves_per_cu = int(waves_per_cu)
+    o.refill_threshold = int(refill_threshold)
+    if form is None:
+        form = BATCH_PERSISTENT if (waves_per_cu or refill_threshold) else BATCH_GRID
+    o.form = int(form)
+    return o
 
 
 class BuildStats(ctypes.Structure):
@@ -113,11 +129,11 @@ class Request(ctypes.Structure):
 class ServerOpts(ctypes.Structure):
     _fields_ = [("streams", ctypes.c_int), ("output_budget", ctypes.c_uint64), ("max_batch", ctypes.c_int),
                 ("one_per_launch", ctypes.c_int), ("waves_per_cu", ctypes.c_int), ("refill_threshold", ctypes.c_int),
-                ("reserved", ctypes.c_int * 6)]
+                ("batch_form", ctypes.c_int), ("reserved", ctypes.c_int * 5)]
 
 
 def serve_replay(trees, requests, arrivals, streams=4, output_budget=1 << 30, max_batch=256, one_per_launch=False,
-                 waves_per_cu=0, refill_threshold=0, outputs=None):
+                 waves_per_cu=0, refill_threshold=0, outputs=None, batch_form=None):
     """Request loop over a known arrival schedule (see svo_serve_replay).
     requests: list of (tree index, Camera, width, height). outputs: optional
     list of (normal, t, leaf) numpy arrays (or None) per request. Returns
@@ -133,6 +149,7 @@ def serve_replay(trees, requests, arrivals, streams=4, output_budget=1 << 30, ma
     o = ServerOpts()
     o.streams, o.output_budget, o.max_batch = int(streams), int(output_budget), int(max_batch)
     o.one_per_launch, o.waves_per_cu, o.refill_threshold = int(bool(one_per_launch)), int(waves_per_cu), int(refill_threshold)
+    o.batch_form = _batch_opts(waves_per_cu, refill_threshold, batch_form).form
     nout = tout = lout = None
     if outputs is not None:
         nout = (ctypes.c_void_p * n)(*[None if x is None else x[0].ctypes.data for x in outputs])
@@ -529,36 +546,33 @@ class Renderer:
         v.hit, v.normal, v.t, v.leaf = hit, normal, t, leaf
         return v
 
-    def render_device(self, views, stream=None, waves_per_cu=0, refill_threshold=0):
+    def render_device(self, views, stream=None, waves_per_cu=0, refill_threshold=0, form=None):
         arr = (View * len(views))(*views)
-        o = BatchOpts()
-        o.waves_per_cu = int(waves_per_cu)
-        o.refill_threshold = int(refill_threshold)
+        o = _batch_opts(waves_per_cu, refill_threshold, form)
         _check(lib.svo_render_batch_device(self._r, arr, len(views), ctypes.byref(o), stream))
 
     def autotune(self, views, stream=None):
-        """((waves_per_cu, refill), {"WxR": ms}) of the fastest launch shape."""
+        """((form, waves_per_cu, refill), {"grid": ms, "WxR": ms, ...}) of the
+        fastest launch shape (waves and refill are 0 for the grid form)."""
         arr = (View * len(views))(*views)
         best = BatchOpts()
-        times = (ctypes.c_float * 9)()
+        times = (ctypes.c_float * 10)()
         _check(lib.svo_renderer_autotune(self._r, arr, len(views), stream, ctypes.byref(best), times))
-        grid = {f"{w}x{r}": round(times[i], 3) for i, (w, r) in enumerate((w, r) for w in (16, 24, 32) for r in (8, 16, 32))}
-        return (best.waves_per_cu, best.refill_threshold), grid
+        shapes = ["grid"] + [f"{w}x{r}" for w in (16, 24, 32) for r in (16, 32, 64)]
+        return (best.form, best.waves_per_cu, best.refill_threshold), {k: round(times[i], 3) for i, k in enumerate(shapes)}
 
-    def capture(self, views, stream=None, waves_per_cu=0, refill_threshold=0):
+    def capture(self, views, stream=None, waves_per_cu=0, refill_threshold=0, form=None):
         """BatchGraph replaying one batched launch of these views."""
-        return BatchGraph(self, views, stream, waves_per_cu, refill_threshold)
+        return BatchGraph(self, views, stream, waves_per_cu, refill_threshold, form)
 
 
 class BatchGraph:
     """A batched launch captured in a HIP graph (svo_batch_graph)."""
 
-    def __init__(self, renderer, views, stream=None, waves_per_cu=0, refill_threshold=0):
+    def __init__(self, renderer, views, stream=None, waves_per_cu=0, refill_threshold=0, form=None):
         self._g = ctypes.c_void_p()
         arr = (View * len(views))(*views)
-        o = BatchOpts()
-        o.waves_per_cu = int(waves_per_cu)
-        o.refill_threshold = int(refill_threshold)
+        o = _batch_opts(waves_per_cu, refill_threshold, form)
         _check(lib.svo_batch_graph_create(renderer._r, arr, len(views), ctypes.byref(o), stream, ctypes.byref(self._g)))
 
     def launch(self, stream=None):

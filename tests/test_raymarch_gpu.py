@@ -166,14 +166,15 @@ def test_octree_beyond_2_32_words(svo, golden):
     assert np.array_equal(rgba, ref_rgba) and np.array_equal(_bits(depth), _bits(ref_depth))
     dev = torch.device("cuda", 0)
     ren = svo.svo.Renderer(4)
-    o4 = {k: torch.zeros(96 * 64, dtype=dt, device=dev)
-          for k, dt in (("hit", torch.uint8), ("normal", torch.int32), ("t", torch.float32), ("leaf", torch.int64))}
-    ren.render_device([ren.view(big, cam, 96, 64, o4["hit"].data_ptr(), o4["normal"].data_ptr(), o4["t"].data_ptr(),
-                                o4["leaf"].data_ptr())], None, 0, 0)
-    torch.cuda.synchronize()
-    got = {"hit": o4["hit"].cpu().numpy(), "normal": o4["normal"].cpu().numpy().view(np.uint32),
-           "t": o4["t"].cpu().numpy(), "leaf": o4["leaf"].cpu().numpy().view(np.uint64)}
-    _same(got, shifted(small.render_primary(cam, 96, 64)), "batched")
+    for form in (svo.svo.BATCH_GRID, svo.svo.BATCH_PERSISTENT):
+        o4 = {k: torch.zeros(96 * 64, dtype=dt, device=dev)
+              for k, dt in (("hit", torch.uint8), ("normal", torch.int32), ("t", torch.float32), ("leaf", torch.int64))}
+        ren.render_device([ren.view(big, cam, 96, 64, o4["hit"].data_ptr(), o4["normal"].data_ptr(),
+                                    o4["t"].data_ptr(), o4["leaf"].data_ptr())], None, 0, 0, form)
+        torch.cuda.synchronize()
+        got = {"hit": o4["hit"].cpu().numpy(), "normal": o4["normal"].cpu().numpy().view(np.uint32),
+               "t": o4["t"].cpu().numpy(), "leaf": o4["leaf"].cpu().numpy().view(np.uint64)}
+        _same(got, shifted(small.render_primary(cam, 96, 64)), f"batched form {form}")
 
 
 def test_single_ray_entry_matches_batch(trees):
@@ -202,16 +203,18 @@ def test_far_pointers_and_empty_batch(trees, svo):
 
 
 def test_batched_renderer_matches_plain_kernel(svo, trees, golden):
-    """One persistent launch over views of different octrees, sizes (edge
-    tiles, sizes below one tile) and LOD scales equals the plain per-view
-    kernel bit for bit, for every refill threshold / grid size tried."""
+    """One launch over views of different octrees, sizes (edge tiles, sizes
+    below one tile) and LOD scales equals the plain per-view kernel bit for
+    bit, in the grid form and in the persistent form for every refill
+    threshold / wave count tried."""
     import torch
     dev = torch.device("cuda", 0)
     specs = [("dragon", 320, 180, 20.0, 35.0, 0.0), ("sdf128", 257, 131, -30.0, 200.0, 0.0),
              ("dragon", 7, 5, 10.0, 10.0, 0.0), ("sdf128", 640, 360, 45.0, 300.0, 0.0),
              ("dragon", 200, 100, -60.0, 90.0, 0.02)]
     ren = svo.svo.Renderer(16)
-    for waves, refill in [(0, 0), (8, 1), (32, 64), (16, 40)]:
+    for waves, refill, form in [(0, 0, svo.svo.BATCH_GRID), (0, 0, svo.svo.BATCH_PERSISTENT), (8, 1, None),
+                                (32, 64, None), (16, 40, None)]:
         outs, views = [], []
         for name, w, h, p, y, rs in specs:
             t = trees[name]
@@ -223,8 +226,8 @@ def test_batched_renderer_matches_plain_kernel(svo, trees, golden):
             outs.append((t, cam, w, h, rs, o))
             views.append(ren.view(t, cam, w, h, o["hit"].data_ptr(), o["normal"].data_ptr(), o["t"].data_ptr(),
                                   o["leaf"].data_ptr(), ray_scale=rs))
-        print(f"[gpu] batched renderer waves={waves} refill={refill}", flush=True)
-        ren.render_device(views, None, waves, refill)
+        print(f"[gpu] batched renderer waves={waves} refill={refill} form={form}", flush=True)
+        ren.render_device(views, None, waves, refill, form)
         torch.cuda.synchronize()
         for t, cam, w, h, rs, o in outs:
             if rs == 0.0:
@@ -259,13 +262,16 @@ def test_batched_renderer_with_a_deep_octree(svo, trees):
         outs.append((t, cam, w, h, o))
         views.append(ren.view(t, cam, w, h, o["hit"].data_ptr(), o["normal"].data_ptr(), o["t"].data_ptr(),
                               o["leaf"].data_ptr()))
-    print("[gpu] batched renderer with a depth-13 octree", flush=True)
-    ren.render_device(views, None, 0, 0)
-    torch.cuda.synchronize()
-    for t, cam, w, h, o in outs:
-        mine = {"hit": o["hit"].cpu().numpy(), "normal": o["normal"].cpu().numpy().view(np.uint32),
-                "t": o["t"].cpu().numpy(), "leaf": o["leaf"].cpu().numpy().view(np.uint64)}
-        _same(mine, t.render_primary(cam, w, h), f"{w}x{h}")
+    for form in (svo.svo.BATCH_GRID, svo.svo.BATCH_PERSISTENT):
+        print(f"[gpu] batched renderer with a depth-13 octree, form {form}", flush=True)
+        for _, _, _, _, o in outs:
+            o["hit"].fill_(9)
+        ren.render_device(views, None, 0, 0, form)
+        torch.cuda.synchronize()
+        for t, cam, w, h, o in outs:
+            mine = {"hit": o["hit"].cpu().numpy(), "normal": o["normal"].cpu().numpy().view(np.uint32),
+                    "t": o["t"].cpu().numpy(), "leaf": o["leaf"].cpu().numpy().view(np.uint64)}
+            _same(mine, t.render_primary(cam, w, h), f"{w}x{h} form {form}")
     t, cam, w, h, _ = outs[0]
     plain = t.render_primary(cam, w, h)
     sample = np.arange(0, w * h, 7)
@@ -298,7 +304,8 @@ def _check_batch(outs, refs):
         _same(mine, ref, f"{w}x{h} rs={rs}")
 
 
-def test_batch_graph_replays_match_eager(svo, trees):
+@pytest.mark.parametrize("form", [0, 1])
+def test_batch_graph_replays_match_eager(svo, trees, form):
     """A batched launch captured in a HIP graph, replayed several times with
     the program's own eager work and synchronisations in between (outputs
     cleared by eager fills each time): every replay equals the plain kernel."""
@@ -316,8 +323,8 @@ def test_batch_graph_replays_match_eager(svo, trees):
             d = orc.primary_dirs(np.frombuffer(cam.tform, np.float32), w, h)
             refs.append(orc.raymarch(t.words(), np.frombuffer(cam.pos, np.float32)[None, :], d, rs))
     stream = torch.cuda.Stream()
-    print("[gpu] batch graph capture", flush=True)
-    g = ren.capture(views, stream.cuda_stream)
+    print(f"[gpu] batch graph capture, form {form}", flush=True)
+    g = ren.capture(views, stream.cuda_stream, form=form)
     for k in range(4):
         with torch.cuda.stream(stream):
             for _, _, _, _, _, o in outs:
@@ -341,15 +348,22 @@ def test_autotune_picks_a_shape_and_keeps_results(svo, trees):
     ren = svo.svo.Renderer(4)
     outs, views = _batch_setup(svo, trees, ren, dev, specs)
     print("[gpu] batch autotune", flush=True)
-    (w, r), grid = ren.autotune(views)
+    (form, w, r), times = ren.autotune(views)
     torch.cuda.synchronize()
-    assert w in (16, 24, 32) and r in (8, 16, 32) and len(grid) == 9 and all(v > 0 for v in grid.values())
-    assert grid[f"{w}x{r}"] == min(grid.values())
+    assert len(times) == 10 and all(v > 0 for v in times.values())
+    if form == svo.svo.BATCH_GRID:
+        assert w == 0 and r == 0 and times["grid"] == min(times.values())
+    else:
+        assert form == svo.svo.BATCH_PERSISTENT and w in (16, 24, 32) and r in (16, 32, 64)
+        assert times[f"{w}x{r}"] == min(times.values())
+    ren.render_device(views, None, w, r, form)
+    torch.cuda.synchronize()
     _check_batch(outs, [t.render_primary(cam, ww, hh) for t, cam, ww, hh, _, _ in outs])
 
 
-@pytest.mark.parametrize("one_per_launch,streams,budget", [(False, 4, 1 << 28), (False, 2, 1 << 24), (True, 1, 1 << 26)])
-def test_served_results_equal_alone_and_in_the_mix(svo, trees, one_per_launch, streams, budget):
+@pytest.mark.parametrize("one_per_launch,streams,budget,form", [(False, 4, 1 << 28, 0), (False, 2, 1 << 24, 0),
+                                                               (True, 1, 1 << 26, 0), (False, 4, 1 << 28, 1)])
+def test_served_results_equal_alone_and_in_the_mix(svo, trees, one_per_launch, streams, budget, form):
     """Requests arriving while others run are folded into the next launch;
     each served result equals the same request rendered alone."""
     rng = np.random.default_rng(17)
@@ -365,9 +379,10 @@ def test_served_results_equal_alone_and_in_the_mix(svo, trees, one_per_launch, s
     arrivals = np.cumsum(rng.exponential(1.0 / 3000.0, len(reqs)))
     outs = [(np.zeros(w * h, np.uint32), np.zeros(w * h, np.float32), np.zeros(w * h, np.uint64))
             for _, _, w, h in reqs]
-    print(f"[gpu] serve replay one_per_launch={one_per_launch} streams={streams} budget={budget}", flush=True)
+    print(f"[gpu] serve replay one_per_launch={one_per_launch} streams={streams} budget={budget} form={form}",
+          flush=True)
     lat, makespan = svo.svo.serve_replay(tl, reqs, arrivals, streams=streams, output_budget=budget,
-                                         one_per_launch=one_per_launch, outputs=outs)
+                                         one_per_launch=one_per_launch, outputs=outs, batch_form=form)
     assert np.all(lat > 0) and makespan > 0
     for (ti, cam, w, h), (nrm, t, leaf) in zip(reqs, outs):
         ref = tl[ti].render_primary(cam, w, h)
