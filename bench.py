@@ -49,26 +49,6 @@ def profiled_traffic(workload, width, height):
 
 
 
-def profiled_issue(workload, width, height, avg_launch_s):
-    """The primary kernel's actual limiter, VALU issue: SQ_INSTS_VALU per wave
-    from the committed PMC summary (same workload), priced at one wave64 VALU
-    instruction per SIMD every 4 cycles on 256 CUs x 4 SIMDs at 2.4 GHz, as a
-    fraction of the launch time measured here. None when no profile covers it."""
-    path = os.path.join(ROOT, "profiles", "raymarch_primary_ico2048_1080p_pmc.txt")
-    if workload != "icosphere2048_8x1080p" or (width, height) != (1920, 1080) or not os.path.exists(path):
-        return None
-    vals = {}
-    for line in open(path):
-        f = line.split()
-        if len(f) == 2 and f[0] in ("SQ_WAVES", "SQ_INSTS_VALU"):
-            vals[f[0]] = float(f[1])
-    if len(vals) != 2 or avg_launch_s <= 0:
-        return None
-    issue_s = vals["SQ_INSTS_VALU"] * 4 / (256 * 4 * 2.4e9)
-    return {"bound": "valu_issue", "valu_insts_per_wave": round(vals["SQ_INSTS_VALU"] / vals["SQ_WAVES"], 1),
-            "valu_issue_ms": round(issue_s * 1e3, 4), "frac": round(issue_s / avg_launch_s, 4),
-            "source": "profiles/raymarch_primary_ico2048_1080p_pmc.txt (counts) / avg_launch_ms (time)"}
-
 def build_roofline(st):
     """Construction against HBM bandwidth: algorithmic bytes 36*T + 24*K + 4*W
     over the device phase of the build (upload through last scatter), for a
@@ -743,8 +723,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5),
                      "traffic": profiled_traffic(config["workload"], W, H),
-                     "bytes_per_ray": round(b_ray, 2), "avg_launch_ms": round(avg_launch_s * 1e3, 4),
-                     "issue": profiled_issue(config["workload"], W, H, avg_launch_s)},
+                     "bytes_per_ray": round(b_ray, 2), "avg_launch_ms": round(avg_launch_s * 1e3, 4)},
         "cpu_baseline": cpu_baseline,
         "batched": batched,
         "mixed_batched": mixed,
