@@ -361,37 +361,32 @@ def run_frames(pkg, torch, stream, tree, cams, width, height, steps, warmup, cpu
            "ms_per_frame": round(ms / len(cams), 4), "frames_per_s": round(len(cams) / (ms / 1e3), 1),
            "rays_per_step": traced, "grays_s": round(traced / (ms / 1e3) / 1e9, 4),
            "fine_pixels_traced_frac": round((traced - nd * len(cams)) / (m * len(cams)), 4)}
-    # The same frames with consecutive frames on two streams (default priority):
-    # a frame's coarse pass, a few hundred waves that leave most of the GPU idle,
-    # runs beside the previous frame's fine pass. Every frame writes its own
-    # buffers; the images and depth tables must equal the one-stream ones.
-    s2 = torch.cuda.Stream()
-    sptrs = [sptr, s2.cuda_stream]
+    # The same frames in one svo_render_frames_device call per step: consecutive
+    # frames alternate between this stream and the library's side stream, so a
+    # frame's coarse pass (a few hundred waves) and its fine pass's tail run
+    # beside the neighbouring frame's fine pass. Images and depth tables must
+    # equal the one-frame-per-call ones.
     rgba2 = [torch.empty_like(x) for x in rgba]
     depth2 = [torch.empty_like(x) for x in depth]
+    rptrs, dptrs = [x.data_ptr() for x in rgba2], [x.data_ptr() for x in depth2]
 
     def step2():
-        for k, c in enumerate(cams):
-            tree.render_frame_device(c, width, height, rgba2[k].data_ptr(), depth2[k].data_ptr(), 16, 1, sptrs[k % 2])
+        tree.render_frames_device(cams, width, height, rptrs, dptrs, 16, 1, sptr)
 
     for _ in range(max(warmup, 1)):
         step2()
     torch.cuda.synchronize()
     a.record(stream)
-    s2.wait_event(a)
     for _ in range(steps):
         step2()
-    done2 = torch.cuda.Event()
-    done2.record(s2)
-    stream.wait_event(done2)
     b.record(stream)
     torch.cuda.synchronize()
     ms2 = a.elapsed_time(b) / steps
     diff = sum(int(torch.count_nonzero(rgba2[k] != rgba[k])) + int(torch.count_nonzero(
         depth2[k].view(torch.int32) != depth[k].view(torch.int32))) for k in range(len(cams)))
-    out["two_streams"] = {"ms_per_frame": round(ms2 / len(cams), 4), "frames_per_s": round(len(cams) / (ms2 / 1e3), 1),
-                          "vs_one_stream": round(ms / ms2, 3), "mismatched_vs_one_stream": diff,
-                          "form": "consecutive frames alternate between two streams"}
+    out["frames_call"] = {"ms_per_frame": round(ms2 / len(cams), 4), "frames_per_s": round(len(cams) / (ms2 / 1e3), 1),
+                          "vs_one_frame_per_call": round(ms / ms2, 3), "mismatched_vs_one_frame_per_call": diff,
+                          "form": f"one svo_render_frames_device call for the {len(cams)} frames"}
     if cpu_check:
         words = tree.words()
         tf = np.frombuffer(cams[0].tform, np.float32)

@@ -262,6 +262,45 @@ int render_frame_device(svo_octree &h, const svo_camera &cam, int width, int hei
     return SVO_OK;
 }
 
+// Several frames in one call: consecutive frames alternate between the
+// caller's stream and the device's side stream, so one frame's coarse pass
+// and its fine pass's tail (both leave most of the GPU idle) overlap the
+// neighbouring frame's fine pass. Each frame is rendered exactly as by
+// render_frame_device; the caller's stream waits for all of them.
+int render_frames_device(svo_octree &h, const svo_camera *cams, int n, int width, int height, int stripes, int stride,
+                         uint32_t *const *rgba, float *const *depth, void *stream) {
+    if (!cams || !rgba || n <= 0) return fail(SVO_ERR_INVALID, "null camera or image array, or no frames");
+    int r = frame_layout(width, height, stripes);
+    if (r) return r;
+    if (stride <= 0 || stride > 8) return fail(SVO_ERR_INVALID, "stride must be in [1, 8]");
+    for (int f = 0; f < n; ++f) {
+        if (!rgba[f]) return fail(SVO_ERR_INVALID, "null image");
+        if (!camera_ok(cams[f])) return fail(SVO_ERR_INVALID, "camera entries must be finite and within +-1e6");
+    }
+    if (n == 1) return render_frame_device(h, cams[0], width, height, stripes, stride, rgba[0], depth ? depth[0] : nullptr,
+                                           stream);
+    r = ensure_device(h);
+    if (r) return r;
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(side_mutex());
+    SideStream *ss = nullptr;
+    if ((r = side_stream(dev, ss))) return r;
+    hipStream_t s = static_cast<hipStream_t>(stream), side = static_cast<hipStream_t>(ss->stream);
+    hipEvent_t fork = static_cast<hipEvent_t>(ss->fork), join = static_cast<hipEvent_t>(ss->join);
+    HIP_TRY(hipEventRecord(fork, s));
+    HIP_TRY(hipStreamWaitEvent(side, fork, 0));
+    for (int f = 0; f < n && !r; ++f)
+        r = render_frame_device(h, cams[f], width, height, stripes, stride, rgba[f], depth ? depth[f] : nullptr,
+                                (f & 1) ? side : s);
+    // join even after a failed frame, so the caller's stream still orders after the side stream's work
+    hipError_t e = hipEventRecord(join, side);
+    if (e == hipSuccess) e = hipStreamWaitEvent(s, join, 0);
+    if (r) return r;
+    if (e != hipSuccess) return fail(SVO_ERR_DEVICE, std::string("frames: ") + hipGetErrorString(e));
+    return SVO_OK;
+}
+
 int64_t frame_depth_count(int width, int height, int stripes) {
     if (frame_layout(width, height, stripes)) return -1;
     const int sh = (height - 1) / stripes + 1;

@@ -292,28 +292,9 @@ void launch_shadow(const svo_octree &h, const TileLaunch &L, const PrimaryParams
                            light[2], eps, L.slots, hit, t, shadow);
 }
 
-// A second stream (default priority, non-blocking) and two events per device,
-// created on first use and kept for the life of the process, with a lock
-// held while one call enqueues its fork and join.
-struct SideStream {
-    hipStream_t stream = nullptr;
-    hipEvent_t fork = nullptr, join = nullptr;
-};
 constexpr int kMaxDevices = 64;
 std::mutex g_sideMu;
 SideStream g_side[kMaxDevices];
-
-int side_stream(int dev, SideStream *&out) {
-    if (dev < 0 || dev >= kMaxDevices) return fail(SVO_ERR_UNSUPPORTED, "device index beyond the side-stream table");
-    SideStream &ss = g_side[dev];
-    if (!ss.stream) {
-        HIP_TRY(hipStreamCreateWithFlags(&ss.stream, hipStreamNonBlocking));
-        HIP_TRY(hipEventCreateWithFlags(&ss.fork, hipEventDisableTiming));
-        HIP_TRY(hipEventCreateWithFlags(&ss.join, hipEventDisableTiming));
-    }
-    out = &ss;
-    return SVO_OK;
-}
 
 // Images with fewer tiles than this render both passes on the caller's stream;
 // larger ones in kSplitParts pieces of consecutive tiles, alternating between
@@ -323,6 +304,25 @@ constexpr int kSplitTiles = 8192;
 constexpr int kSplitParts = 2;
 
 }  // namespace
+
+std::mutex &side_mutex() { return g_sideMu; }
+
+int side_stream(int dev, SideStream *&out) {
+    if (dev < 0 || dev >= kMaxDevices) return fail(SVO_ERR_UNSUPPORTED, "device index beyond the side-stream table");
+    SideStream &ss = g_side[dev];
+    if (!ss.stream) {
+        hipStream_t st = nullptr;
+        hipEvent_t f = nullptr, j = nullptr;
+        HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        HIP_TRY(hipEventCreateWithFlags(&f, hipEventDisableTiming));
+        HIP_TRY(hipEventCreateWithFlags(&j, hipEventDisableTiming));
+        ss.stream = st;
+        ss.fork = f;
+        ss.join = j;
+    }
+    out = &ss;
+    return SVO_OK;
+}
 
 int render_primary_device(svo_octree &h, const svo_camera &cam, int width, int height, uint8_t *hit,
                           uint32_t *normal, float *t, uint64_t *leaf, void *stream) {
@@ -372,17 +372,19 @@ int render_shadow_device(svo_octree &h, const svo_camera &cam, int width, int he
     std::lock_guard<std::mutex> lock(g_sideMu);
     SideStream *ss = nullptr;
     if ((r = side_stream(dev, ss))) return r;
-    HIP_TRY(hipEventRecord(ss->fork, s));
-    HIP_TRY(hipStreamWaitEvent(ss->stream, ss->fork, 0));
+    hipStream_t side = static_cast<hipStream_t>(ss->stream);
+    hipEvent_t fork = static_cast<hipEvent_t>(ss->fork), join = static_cast<hipEvent_t>(ss->join);
+    HIP_TRY(hipEventRecord(fork, s));
+    HIP_TRY(hipStreamWaitEvent(side, fork, 0));
     for (int k = 0; k < kSplitParts; ++k) {
         const int a = int(int64_t(tiles) * k / kSplitParts), b = int(int64_t(tiles) * (k + 1) / kSplitParts);
-        hipStream_t sk = (k & 1) ? ss->stream : s;
+        hipStream_t sk = (k & 1) ? side : s;
         launch_primary(h, L, p, a, b - a, hit, normal, t, leaf, sk);
         launch_shadow(h, L, p, a, b - a, light, eps, hit, t, shadow, sk);
     }
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipEventRecord(ss->join, ss->stream));
-    HIP_TRY(hipStreamWaitEvent(s, ss->join, 0));
+    HIP_TRY(hipEventRecord(join, side));
+    HIP_TRY(hipStreamWaitEvent(s, join, 0));
     return SVO_OK;
 }
 

@@ -469,6 +469,15 @@ int svo_render_frame_device(svo_octree *h, const svo_camera *cam, int width, int
     });
 }
 
+int svo_render_frames_device(svo_octree *h, const svo_camera *cams, int nframes, int width, int height, int stripes,
+                             int stride, uint32_t *const *rgba, float *const *depth, void *stream) {
+    return guarded([&]() -> int {
+        clear_error();
+        if (!h) return fail(SVO_ERR_INVALID, "null handle");
+        return render_frames_device(*h, cams, nframes, width, height, stripes, stride, rgba, depth, stream);
+    });
+}
+
 int64_t svo_frame_depth_count(int width, int height, int stripes) {
     clear_error();
     return frame_depth_count(width, height, stripes);

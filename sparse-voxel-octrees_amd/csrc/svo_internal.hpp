@@ -118,6 +118,19 @@ int ensure_device(svo_octree &h);
 void release_device(svo_octree &h);
 int raymarch_device(svo_octree &h, int64_t n, const float *o, const float *d, const float *rs,
                     float rs0, uint8_t *hit, uint32_t *normal, float *t, uint64_t *leaf, void *stream);
+// A second stream (default priority, non-blocking) and two events per device
+// (hipStream_t / hipEvent_t), created on first use and kept for the life of
+// the process. Hold side_mutex() while enqueuing a fork (record `fork` on the
+// caller's stream, make `stream` wait on it) and its join (record `join` on
+// `stream`, make the caller's stream wait on it).
+struct SideStream {
+    void *stream = nullptr;
+    void *fork = nullptr;
+    void *join = nullptr;
+};
+std::mutex &side_mutex();
+int side_stream(int dev, SideStream *&out);
+
 int render_primary_device(svo_octree &h, const svo_camera &cam, int width, int height, uint8_t *hit,
                           uint32_t *normal, float *t, uint64_t *leaf, void *stream);
 int device_count();
@@ -128,6 +141,8 @@ int render_shadow_device(svo_octree &h, const svo_camera &cam, int width, int he
 // ---- viewer frame (frame.hip) ----
 int render_frame_device(svo_octree &h, const svo_camera &cam, int width, int height, int stripes, int stride,
                         uint32_t *rgba, float *depth, void *stream);
+int render_frames_device(svo_octree &h, const svo_camera *cams, int n, int width, int height, int stripes, int stride,
+                         uint32_t *const *rgba, float *const *depth, void *stream);
 int64_t frame_depth_count(int width, int height, int stripes);
 
 // ---- batched persistent renderer (render_batch.hip) ----

@@ -31,7 +31,7 @@ EXPORTS = [
     "svo_build_from_mesh", "svo_build_from_voxels", "svo_build_from_grid",
     "svo_renderer_create", "svo_renderer_free", "svo_render_batch_device",
     "svo_build_from_sdf", "svo_sdf_volume", "svo_serve_replay",
-    "svo_render_frame", "svo_render_frame_device", "svo_frame_depth_count",
+    "svo_render_frame", "svo_render_frame_device", "svo_render_frames_device", "svo_frame_depth_count",
     "svo_build_from_terrain", "svo_terrain_volume", "svo_terrain_heights",
     "svo_render_shadow", "svo_render_shadow_device", "svo_decode_words",
     "svo_renderer_autotune", "svo_batch_graph_create", "svo_batch_graph_launch", "svo_batch_graph_free",
@@ -201,6 +201,7 @@ def _load():
         "svo_serve_replay": [P, ctypes.c_int, P, P, ctypes.c_int, P, P, P, P, P, P],
         "svo_render_frame": [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P],
         "svo_render_frame_device": [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, P],
+        "svo_render_frames_device": [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, P],
         "svo_frame_depth_count": [ctypes.c_int, ctypes.c_int, ctypes.c_int],
         "svo_build_from_terrain": [ctypes.c_int, ctypes.c_uint32, P, P],
         "svo_terrain_volume": [ctypes.c_int, ctypes.c_uint32, P],
@@ -499,6 +500,17 @@ class VoxelOctree:
     def render_frame_device(self, cam, width, height, rgba_ptr, depth_ptr=None, stripes=16, stride=1, stream=None):
         _check(lib.svo_render_frame_device(self._h, ctypes.byref(cam), int(width), int(height), int(stripes),
                                            int(stride), rgba_ptr, depth_ptr, stream))
+
+    def render_frames_device(self, cams, width, height, rgba_ptrs, depth_ptrs=None, stripes=16, stride=1,
+                             stream=None):
+        """Several frames in one call (svo_render_frames_device): device
+        pointers per frame; depth_ptrs may be None or hold None entries."""
+        n = len(cams)
+        carr = (Camera * n)(*cams)
+        rarr = (ctypes.c_void_p * n)(*rgba_ptrs)
+        darr = None if depth_ptrs is None else (ctypes.c_void_p * n)(*depth_ptrs)
+        _check(lib.svo_render_frames_device(self._h, carr, n, int(width), int(height), int(stripes), int(stride),
+                                            rarr, darr, stream))
 
 
 def convert_to_volume(ply_path, resolution, voxel_path, threads=0, memory_budget=0):

@@ -448,3 +448,41 @@ def test_frame_device_entry_and_errors(svo, trees):
         t.render_frame(cam, w, 100, 16, 1)  # 16 stripes of 7 rows leave the last one empty
     with pytest.raises(svo.SvoError):
         t.render_frame(cam, w, h, 16, 9)
+
+
+def test_frames_in_one_call_equal_single_frames(svo, trees):
+    """svo_render_frames_device (consecutive frames alternating between the
+    caller's stream and a side stream) gives each frame's RGBA and depth table
+    exactly as a single-frame render, with per-frame NULL depths, an odd frame
+    count and stride 3; bad cameras or null images are refused before any
+    frame is enqueued."""
+    import torch
+    t = trees["dragon"]
+    w, h, stripes = 320, 176, 16
+    nd = svo.svo.frame_depth_count(w, h, stripes)
+    angles = [(30.0, 10.0), (20.0, 35.0), (-15.0, 120.0), (45.0, 200.0), (5.0, 300.0)]
+    cams = [t.orbit_camera(p, y, 1.0) for p, y in angles]
+    stream = torch.cuda.Stream()
+    for stride in (1, 3):
+        rgba = [torch.full((h * w,), 0x12345678, dtype=torch.int32, device="cuda") for _ in cams]
+        depth = [torch.full((nd,), -1.0, dtype=torch.float32, device="cuda") for _ in cams]
+        dptrs = [d.data_ptr() if k != 2 else None for k, d in enumerate(depth)]
+        print(f"[gpu] frames in one call, stride {stride}", flush=True)
+        t.render_frames_device(cams, w, h, [x.data_ptr() for x in rgba], dptrs, stripes, stride, stream.cuda_stream)
+        stream.synchronize()
+        for k, cam in enumerate(cams):
+            ref_rgba, ref_depth = t.render_frame(cam, w, h, stripes, stride)
+            assert np.array_equal(rgba[k].cpu().numpy().view(np.uint32), ref_rgba.reshape(-1)), (stride, k)
+            if k != 2:
+                assert np.array_equal(depth[k].cpu().numpy().view(np.uint32), ref_depth.view(np.uint32)), (stride, k)
+            else:
+                assert (depth[k].cpu().numpy() == -1.0).all()
+    bad = t.orbit_camera(0.0, 0.0, 1.0)
+    bad.pos[0] = float("nan")
+    out = [torch.full((h * w,), 7, dtype=torch.int32, device="cuda") for _ in range(2)]
+    for cs, ptrs in (([cams[0], bad], [x.data_ptr() for x in out]), ([cams[0], cams[1]], [out[0].data_ptr(), None])):
+        with pytest.raises(svo.SvoError) as e:
+            t.render_frames_device(cs, w, h, ptrs, None, stripes, 1, stream.cuda_stream)
+        assert e.value.code == svo.svo.ERR_INVALID
+    stream.synchronize()
+    assert all(bool((x == 7).all()) for x in out)  # nothing was enqueued
