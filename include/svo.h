@@ -238,9 +238,9 @@ int svo_renderer_create(int max_views, svo_renderer **out);
 void svo_renderer_free(svo_renderer *r);
 
 /* Traces every view in one call over a single sequence of 8x8 pixel tiles
- * (Morton order within 64x64-pixel blocks), in the launch form of `opts`
- * (NULL: SVO_BATCH_GRID). Per-ray results equal svo_render_primary's.
- * Enqueued on `stream`. */
+ * (grid form: raster order per view; persistent form: Morton order within
+ * 64x64-pixel blocks), in the launch form of `opts` (NULL: SVO_BATCH_GRID).
+ * Per-ray results equal svo_render_primary's. Enqueued on `stream`. */
 int svo_render_batch_device(svo_renderer *r, const svo_view *views, int nviews, const svo_batch_opts *opts,
                             void *stream);
 

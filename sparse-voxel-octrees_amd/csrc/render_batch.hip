@@ -1,5 +1,13 @@
-// Batched persistent renderer: many views of many resident octrees in one
-// launch.
+// Batched renderer: many views of many resident octrees in one launch, in
+// one of two forms.
+//
+// Grid form (default, tiles_kernel): an ordinary launch with one wave per
+// 8x8-pixel tile of every view. A single view's launch spends about half its
+// time in a tail where a few slow tiles (rays grazing the surface) run on an
+// otherwise idle GPU; with all views in one grid the other views' tiles fill
+// that time, and only the last tail remains.
+//
+// Persistent form (batch_kernel), described below.
 //
 // Work is a single queue of 8x8-pixel tiles over all views. Tiles of a view
 // are numbered in 8x8-tile blocks (64x64 pixels) with a Morton order inside
@@ -188,7 +196,7 @@ __global__ void __launch_bounds__(256) batch_kernel(const ViewDesc *__restrict__
 }
 
 // Grid form: one ordinary launch with one workgroup (one wave) per 8x8 tile
-// of every view, in the queue's tile order. The hardware dispatcher hands out
+// of every view, views one after another, tiles in raster order. The hardware dispatcher hands out
 // the tiles; a wave traces its tile's 64 rays and exits. A single view's
 // launch ends with a tail where a few slow tiles (rays that graze the
 // surface) run on an otherwise idle GPU; with many views in one launch that
@@ -204,10 +212,8 @@ __global__ void __launch_bounds__(64) tiles_kernel(const ViewDesc *__restrict__ 
     const int v = view_of(tileBase, nviews, tile);
     const ViewDesc &vd = views[v];
     const uint64_t local = tile - tileBase[v];
-    const uint32_t m = uint32_t(local & 63);
-    const uint64_t block = local >> 6;
-    const int tx = int(block % uint64_t(vd.blocksX)) * 8 + int(even_bits(m));
-    const int ty = int(block / uint64_t(vd.blocksX)) * 8 + int(even_bits(m >> 1));
+    const int tx = int(local % uint64_t(vd.tilesX));
+    const int ty = int(local / uint64_t(vd.tilesX));
     const int x = tx * 8 + (lane & 7);
     const int y = ty * 8 + (lane >> 3);
     if (tx >= vd.tilesX || ty >= vd.tilesY || x >= vd.width || y >= vd.height) return;
@@ -354,7 +360,10 @@ int plan_views(const svo_view *views, int nviews, const svo_batch_opts *opts, in
         d.pad = 0;
         const int blocksY = (d.tilesY + 7) / 8;
         htb[i] = tiles;
-        tiles += uint64_t(d.blocksX) * uint64_t(blocksY) * 64;
+        // persistent form: Morton order in 64x64-pixel blocks (padded); grid
+        // form: raster order (1% faster there, measured on 8 views at 1080p)
+        tiles += (opts && opts->form == SVO_BATCH_PERSISTENT) ? uint64_t(d.blocksX) * uint64_t(blocksY) * 64
+                                                               : uint64_t(d.tilesX) * uint64_t(d.tilesY);
     }
     htb[nviews] = tiles;
     if (tiles >= (uint64_t(kQueues) << 32)) return fail(SVO_ERR_LIMIT, "too many tiles for one launch");
