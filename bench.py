@@ -138,7 +138,7 @@ def reference_cpu_rate(tree, cam, width, height, threads, cache_dir):
 def run_mixed(pkg, torch, stream, steps, warmup, waves_per_cu, refill):
     """Mixed batch: 64 GPU-built octrees (128^3..2048^3, SDF scenes and
     displaced meshes) and 256 views (320x240..1920x1080): 256 plain per-view
-    launches vs one batched persistent launch, same rays."""
+    launches vs one batched launch (form picked by the library's autotuner), same rays."""
     t0 = time.time()
     trees = [pkg.scenes.build_mixed_octree(pkg, sp) for sp in pkg.scenes.mixed_octrees()]
     build_s = time.time() - t0
@@ -547,7 +547,7 @@ def main():
             if events is not None:
                 events[v][1].record(stream)
 
-    # batched form: every view of the step in one persistent launch
+    # batched form: every view of the step in one launch
     renderer = pkg.svo.Renderer(max(V, 1))
     bviews = [renderer.view(tree, cams[v], W, H, None, normal[v].data_ptr(), tbuf[v].data_ptr(), leaf[v].data_ptr())
               for v in range(V)]

@@ -255,8 +255,8 @@ int svo_renderer_autotune(svo_renderer *r, const svo_view *views, int nviews, vo
 
 /* One batched launch of fixed views (cameras, sizes, output pointers) captured
  * in a HIP graph: the graph holds its own view table and queue counters, so
- * every replay is the queue reset kernel plus the persistent kernel, launched
- * with one call. Replays of one graph must not overlap (launch them on one
+ * every replay is the grid-form launch (or, in the persistent form, the queue
+ * reset kernel plus the persistent kernel), launched with one call. Replays of one graph must not overlap (launch them on one
  * stream); the octrees and output arrays must outlive the graph, and the
  * octrees must stay on the graph's device (using one on another device moves
  * its word array there). */

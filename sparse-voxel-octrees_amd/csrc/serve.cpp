@@ -5,14 +5,14 @@
 // renderer, a slab of device output memory and a pinned host slab. Whenever a
 // slot is free and requests are waiting, it admits waiting requests in
 // arrival order while their output footprint (16 B per ray: normal, t, leaf)
-// fits the slot's slab, traces them in one batched persistent launch, and
+// fits the slot's slab, traces them in one batched launch, and
 // copies the outputs to the host slab. Launches of different slots overlap on
 // the device, so requests arriving while others run start as soon as a slot
 // and memory free up. A request is complete when its outputs are on the host;
 // latency is measured from its scheduled arrival to that moment.
 //
 // A launch that admits a single request uses the plain per-view kernel (no
-// view table to stage, no persistent grid): at low request rates that is the
+// view table to stage): at low request rates that is the
 // shorter path; results are identical either way. Every slot's stream and both
 // kernels are exercised once before the clock starts.
 //
