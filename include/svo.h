@@ -170,11 +170,10 @@ int svo_render_frame_device(svo_octree *h, const svo_camera *cam, int width, int
  * reference, whose viewer renders one frame at a time): frame f from cams[f]
  * into rgba[f] (device pointers, width*height each) and, when depth is not
  * NULL, depth[f] (may be NULL per frame). Each frame equals
- * svo_render_frame_device's; consecutive frames alternate between `stream`
- * and a side stream of the device (default priority), so one frame's coarse
- * pass and tail overlap the next frame's fine pass. `stream` waits for all
- * frames. Every camera and image pointer is checked before anything is
- * enqueued. */
+ * svo_render_frame_device's; every 16 frames are one coarse launch over all
+ * their tile corners and one fine launch over all their tiles, so one frame's
+ * coarse pass and tail overlap the other frames' work. Enqueued on `stream`.
+ * Every camera and image pointer is checked before anything is enqueued. */
 int svo_render_frames_device(svo_octree *h, const svo_camera *cams, int nframes, int width, int height, int stripes,
                              int stride, uint32_t *const *rgba, float *const *depth, void *stream);
 /* Coarse depth entries of a frame, or -1 (message in svo_last_error) when the

@@ -50,6 +50,17 @@ struct FrameParams {
     int full;  // corner rows of a full stripe: (sh - 1) / 8 + 2
 };
 
+// Up to N frames of one size traced by one coarse and one fine launch: frame f
+// has its own camera and light (p[f]), image and depth table. The grids are
+// frame-major, so a frame's tiles are contiguous and the frames' coarse passes
+// and fine-pass tails overlap each other's work.
+template <int N>
+struct FrameSet {
+    FrameParams p[N];
+    uint32_t *rgba[N];
+    float *depth[N];
+};
+
 // Stripe s covers rows [s*sh, min((s+1)*sh, H)) (src/Main.cpp:351-361). Every
 // stripe but the last is full (the layout check guarantees it), so stripe s
 // starts at corner s*full*tilesX of the depth table and at fine tile
@@ -84,12 +95,15 @@ __device__ __forceinline__ void frame_dir(const FrameParams &p, float dx, float 
     d[2] = vz * s;
 }
 
-template <typename Idx>
-__global__ void __launch_bounds__(64) coarse_kernel(const uint32_t *__restrict__ tree, FrameParams p,
-                                                    int64_t corners, int slots, float *__restrict__ depth) {
+template <typename Idx, int N>
+__global__ void __launch_bounds__(64) coarse_kernel(const uint32_t *__restrict__ tree, const FrameSet<N> set,
+                                                    int64_t corners, unsigned blocksPerFrame, int slots) {
     extern __shared__ __align__(16) unsigned char lds[];
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const unsigned f = N == 1 ? 0u : blockIdx.x / blocksPerFrame;
+    const int64_t i = int64_t(blockIdx.x - f * blocksPerFrame) * blockDim.x + threadIdx.x;
     if (i >= corners) return;
+    const FrameParams &p = set.p[f];
+    float *__restrict__ depth = set.depth[f];
     const int64_t perStripe = int64_t(p.full) * p.tilesX;
     const int s = int(min(i / perStripe, int64_t(p.stripes - 1)));
     const int64_t local = i - s * perStripe;
@@ -108,12 +122,15 @@ __global__ void __launch_bounds__(64) coarse_kernel(const uint32_t *__restrict__
     depth[i] = code != march::kMiss ? t : kTreeMiss;
 }
 
-template <typename Idx>
-__global__ void __launch_bounds__(64) fine_kernel(const uint32_t *__restrict__ tree, FrameParams p,
-                                                  const float *__restrict__ depth, int slots,
-                                                  uint32_t *__restrict__ rgba) {
+template <typename Idx, int N>
+__global__ void __launch_bounds__(64) fine_kernel(const uint32_t *__restrict__ tree, const FrameSet<N> set,
+                                                  unsigned tilesPerFrame, int slots) {
     extern __shared__ __align__(16) unsigned char lds[];
-    const int64_t tile = blockIdx.x;
+    const unsigned f = N == 1 ? 0u : blockIdx.x / tilesPerFrame;
+    const int64_t tile = blockIdx.x - f * tilesPerFrame;
+    const FrameParams &p = set.p[f];
+    const float *__restrict__ depth = set.depth[f];
+    uint32_t *__restrict__ rgba = set.rgba[f];
     const int64_t perStripe = int64_t(p.full - 1) * (p.tilesX - 1);
     const int s = int(min(tile / perStripe, int64_t(p.stripes - 1)));
     const int64_t local = tile - s * perStripe;
@@ -183,15 +200,10 @@ int frame_layout(int width, int height, int &stripes) {
 
 }  // namespace
 
-int render_frame_device(svo_octree &h, const svo_camera &cam, int width, int height, int stripes, int stride,
-                        uint32_t *rgba, float *depthOut, void *stream) {
-    int r = frame_layout(width, height, stripes);
-    if (r) return r;
-    if (stride <= 0 || stride > 8) return fail(SVO_ERR_INVALID, "stride must be in [1, 8]");
-    if (!rgba) return fail(SVO_ERR_INVALID, "null image");
-    if (!camera_ok(cam)) return fail(SVO_ERR_INVALID, "camera entries must be finite and within +-1e6");
-    r = ensure_device(h);
-    if (r) return r;
+namespace {
+
+// The viewer's per-frame constants (src/Main.cpp:154-176) for one camera.
+FrameParams frame_params(const svo_camera &cam, int width, int height, int stripes, int stride) {
     const float pi = float(3.14159265358979323846);
     FrameParams p;
     const float *tf = cam.tform;
@@ -228,33 +240,48 @@ int render_frame_device(svo_octree &h, const svo_camera &cam, int width, int hei
     p.tileScale = tileScale;
     p.sh = (height - 1) / stripes + 1;
     p.full = (p.sh - 1) / 8 + 2;
+    return p;
+}
+
+constexpr int kFramesPerLaunch = 16;
+
+// Frames [0, n) of `set` (n <= N), all of the size in set.p[0]: one coarse
+// launch over every frame's corners, then one fine launch over every frame's
+// tiles. Frames with a null depth table get one from a stream-ordered scratch
+// allocation.
+template <int N>
+int launch_frames(svo_octree &h, FrameSet<N> &set, int n, hipStream_t s) {
+    const FrameParams &p = set.p[0];
     // corner and fine-tile counts over the stripes (the last may be shorter)
     int64_t corners = 0, tiles = 0;
-    for (int i = 0; i < stripes; ++i) {
-        const int y0 = i * p.sh, y1 = std::min(y0 + p.sh, height);
+    for (int i = 0; i < p.stripes; ++i) {
+        const int y0 = i * p.sh, y1 = std::min(y0 + p.sh, p.height);
         const int tilesY = (y1 - y0 - 1) / 8 + 2;
         corners += int64_t(tilesY) * p.tilesX;
         tiles += int64_t(tilesY - 1) * (p.tilesX - 1);
     }
-    if (tiles > 0x7FFFFFFF) return fail(SVO_ERR_LIMIT, "image too large");
-    hipStream_t s = static_cast<hipStream_t>(stream);
+    const int64_t cblocks = (corners + 63) / 64;
+    if (tiles * n > 0x7FFFFFFF || cblocks * n > 0x7FFFFFFF) return fail(SVO_ERR_LIMIT, "image too large");
+    int nulls = 0;
+    for (int f = 0; f < n; ++f) nulls += set.depth[f] ? 0 : 1;
     void *scratch = nullptr;
-    if (!depthOut) HIP_TRY(hipMallocAsync(&scratch, sizeof(float) * size_t(corners), s));
-    float *dDepth = depthOut ? depthOut : static_cast<float *>(scratch);
+    if (nulls) HIP_TRY(hipMallocAsync(&scratch, sizeof(float) * size_t(corners) * size_t(nulls), s));
+    for (int f = 0, k = 0; f < n; ++f)
+        if (!set.depth[f]) set.depth[f] = static_cast<float *>(scratch) + size_t(corners) * size_t(k++);
     const int slots = lds_slots(h.depth, 0);  // the whole stack in LDS (Reg = 0)
     const bool wide = h.words.size() >= (uint64_t(1) << 32);
     const size_t lds = size_t(slots) * 64 * ((wide ? 8 : 4) + sizeof(float));
-    const unsigned cgrid = unsigned((corners + 63) / 64);
+    const dim3 cgrid(static_cast<unsigned>(cblocks * n)), fgrid(static_cast<unsigned>(tiles * n)), block(64);
     if (!wide) {
-        hipLaunchKernelGGL(coarse_kernel<uint32_t>, dim3(cgrid), dim3(64), lds, s, h.dwords, p, corners, slots, dDepth);
+        hipLaunchKernelGGL((coarse_kernel<uint32_t, N>), cgrid, block, lds, s, h.dwords, set, corners, unsigned(cblocks),
+                           slots);
         if (tiles)
-            hipLaunchKernelGGL(fine_kernel<uint32_t>, dim3(unsigned(tiles)), dim3(64), lds, s, h.dwords, p, dDepth,
-                               slots, rgba);
+            hipLaunchKernelGGL((fine_kernel<uint32_t, N>), fgrid, block, lds, s, h.dwords, set, unsigned(tiles), slots);
     } else {
-        hipLaunchKernelGGL(coarse_kernel<uint64_t>, dim3(cgrid), dim3(64), lds, s, h.dwords, p, corners, slots, dDepth);
+        hipLaunchKernelGGL((coarse_kernel<uint64_t, N>), cgrid, block, lds, s, h.dwords, set, corners, unsigned(cblocks),
+                           slots);
         if (tiles)
-            hipLaunchKernelGGL(fine_kernel<uint64_t>, dim3(unsigned(tiles)), dim3(64), lds, s, h.dwords, p, dDepth,
-                               slots, rgba);
+            hipLaunchKernelGGL((fine_kernel<uint64_t, N>), fgrid, block, lds, s, h.dwords, set, unsigned(tiles), slots);
     }
     hipError_t e = hipGetLastError();
     if (scratch) (void)hipFreeAsync(scratch, s);
@@ -262,11 +289,29 @@ int render_frame_device(svo_octree &h, const svo_camera &cam, int width, int hei
     return SVO_OK;
 }
 
-// Several frames in one call: consecutive frames alternate between the
-// caller's stream and the device's side stream, so one frame's coarse pass
-// and its fine pass's tail (both leave most of the GPU idle) overlap the
-// neighbouring frame's fine pass. Each frame is rendered exactly as by
-// render_frame_device; the caller's stream waits for all of them.
+}  // namespace
+
+int render_frame_device(svo_octree &h, const svo_camera &cam, int width, int height, int stripes, int stride,
+                        uint32_t *rgba, float *depthOut, void *stream) {
+    int r = frame_layout(width, height, stripes);
+    if (r) return r;
+    if (stride <= 0 || stride > 8) return fail(SVO_ERR_INVALID, "stride must be in [1, 8]");
+    if (!rgba) return fail(SVO_ERR_INVALID, "null image");
+    if (!camera_ok(cam)) return fail(SVO_ERR_INVALID, "camera entries must be finite and within +-1e6");
+    r = ensure_device(h);
+    if (r) return r;
+    FrameSet<1> set;
+    set.p[0] = frame_params(cam, width, height, stripes, stride);
+    set.rgba[0] = rgba;
+    set.depth[0] = depthOut;
+    return launch_frames(h, set, 1, static_cast<hipStream_t>(stream));
+}
+
+// Several frames of one size in one call: each group of up to 16 frames is one
+// coarse launch over all their corners and one fine launch over all their
+// tiles, so one frame's coarse pass and fine-pass tail (both leave most of the
+// GPU idle) overlap the other frames' fine work. Each frame equals
+// render_frame_device's.
 int render_frames_device(svo_octree &h, const svo_camera *cams, int n, int width, int height, int stripes, int stride,
                          uint32_t *const *rgba, float *const *depth, void *stream) {
     if (!cams || !rgba || n <= 0) return fail(SVO_ERR_INVALID, "null camera or image array, or no frames");
@@ -277,28 +322,21 @@ int render_frames_device(svo_octree &h, const svo_camera *cams, int n, int width
         if (!rgba[f]) return fail(SVO_ERR_INVALID, "null image");
         if (!camera_ok(cams[f])) return fail(SVO_ERR_INVALID, "camera entries must be finite and within +-1e6");
     }
-    if (n == 1) return render_frame_device(h, cams[0], width, height, stripes, stride, rgba[0], depth ? depth[0] : nullptr,
-                                           stream);
     r = ensure_device(h);
     if (r) return r;
-    int dev = 0;
-    HIP_TRY(hipGetDevice(&dev));
-    std::lock_guard<std::mutex> lock(side_mutex());
-    SideStream *ss = nullptr;
-    if ((r = side_stream(dev, ss))) return r;
-    hipStream_t s = static_cast<hipStream_t>(stream), side = static_cast<hipStream_t>(ss->stream);
-    hipEvent_t fork = static_cast<hipEvent_t>(ss->fork), join = static_cast<hipEvent_t>(ss->join);
-    HIP_TRY(hipEventRecord(fork, s));
-    HIP_TRY(hipStreamWaitEvent(side, fork, 0));
-    for (int f = 0; f < n && !r; ++f)
-        r = render_frame_device(h, cams[f], width, height, stripes, stride, rgba[f], depth ? depth[f] : nullptr,
-                                (f & 1) ? side : s);
-    // join even after a failed frame, so the caller's stream still orders after the side stream's work
-    hipError_t e = hipEventRecord(join, side);
-    if (e == hipSuccess) e = hipStreamWaitEvent(s, join, 0);
-    if (r) return r;
-    if (e != hipSuccess) return fail(SVO_ERR_DEVICE, std::string("frames: ") + hipGetErrorString(e));
-    return SVO_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    for (int f0 = 0; f0 < n && !r; f0 += kFramesPerLaunch) {
+        const int m = std::min(kFramesPerLaunch, n - f0);
+        FrameSet<kFramesPerLaunch> set;
+        for (int k = 0; k < kFramesPerLaunch; ++k) {
+            const int f = f0 + std::min(k, m - 1);  // unused entries repeat the last frame (never indexed)
+            set.p[k] = frame_params(cams[f], width, height, stripes, stride);
+            set.rgba[k] = rgba[f];
+            set.depth[k] = depth ? depth[f] : nullptr;
+        }
+        r = launch_frames(h, set, m, s);
+    }
+    return r;
 }
 
 int64_t frame_depth_count(int width, int height, int stripes) {
