@@ -477,6 +477,16 @@ def test_frames_in_one_call_equal_single_frames(svo, trees):
                 assert np.array_equal(depth[k].cpu().numpy().view(np.uint32), ref_depth.view(np.uint32)), (stride, k)
             else:
                 assert (depth[k].cpu().numpy() == -1.0).all()
+    # 18 frames: two launch groups (16 + 2)
+    w2, h2 = 72, 40
+    cams18 = [t.orbit_camera(-40.0 + 5.0 * k, 20.0 * k, 1.0) for k in range(18)]
+    rgba18 = [torch.zeros(h2 * w2, dtype=torch.int32, device="cuda") for _ in cams18]
+    print("[gpu] frames in one call, 18 frames", flush=True)
+    t.render_frames_device(cams18, w2, h2, [x.data_ptr() for x in rgba18], None, 4, 1, stream.cuda_stream)
+    stream.synchronize()
+    for k, cam in enumerate(cams18):
+        ref_rgba, _ = t.render_frame(cam, w2, h2, 4, 1)
+        assert np.array_equal(rgba18[k].cpu().numpy().view(np.uint32), ref_rgba.reshape(-1)), k
     bad = t.orbit_camera(0.0, 0.0, 1.0)
     bad.pos[0] = float("nan")
     out = [torch.full((h * w,), 7, dtype=torch.int32, device="cuda") for _ in range(2)]
