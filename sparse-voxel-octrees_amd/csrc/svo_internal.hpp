@@ -118,19 +118,6 @@ int ensure_device(svo_octree &h);
 void release_device(svo_octree &h);
 int raymarch_device(svo_octree &h, int64_t n, const float *o, const float *d, const float *rs,
                     float rs0, uint8_t *hit, uint32_t *normal, float *t, uint64_t *leaf, void *stream);
-// A second stream (default priority, non-blocking) and two events per device
-// (hipStream_t / hipEvent_t), created on first use and kept for the life of
-// the process. Hold side_mutex() while enqueuing a fork (record `fork` on the
-// caller's stream, make `stream` wait on it) and its join (record `join` on
-// `stream`, make the caller's stream wait on it).
-struct SideStream {
-    void *stream = nullptr;
-    void *fork = nullptr;
-    void *join = nullptr;
-};
-std::mutex &side_mutex();
-int side_stream(int dev, SideStream *&out);
-
 int render_primary_device(svo_octree &h, const svo_camera &cam, int width, int height, uint8_t *hit,
                           uint32_t *normal, float *t, uint64_t *leaf, void *stream);
 int device_count();

@@ -250,8 +250,7 @@ def _shadow_reference(words, cam, w, h, light, eps, sample):
     return prim, shadow
 
 
-# 1283x723 is 14,651 tiles: large enough that the primary and shadow passes run
-# as two halves on two streams, split mid-row, with partial edge tiles
+# 1283x723: 14,651 tiles, partial edge tiles on both axes
 @pytest.mark.parametrize("n,seed,w,h", [(256, 31, 320, 180), (1024, 32, 641, 359), (1024, 33, 1283, 723)])
 def test_shadow_rays_match_oracle(svo, n, seed, w, h):
     print(f"[gpu] shadow rays terrain n={n} {w}x{h}", flush=True)
