@@ -3,7 +3,10 @@
 Bar: bit-exact. hit code, normal word, t (compared as bits) and leaf index
 against (a) the reference's own outputs for the same rays (golden npz made
 by the reference build) and (b) the oracle on camera frames and random rays
-at sizes the oracle finishes in seconds.
+at sizes the oracle finishes in seconds. The other launch paths (batched
+renders in the grid and persistent forms, HIP-graph replays, the request
+loop, several viewer frames per call, octrees past 2^32 words) must equal
+the plain per-view kernel or single-frame renders bit for bit.
 """
 import os
 
