@@ -330,3 +330,22 @@ def test_frame_depth_count_matches_oracle_and_refuses_empty_stripes(svo):
         with pytest.raises(svo.SvoError) as e:
             svo.svo.frame_depth_count(w, h, stripes)
         assert e.value.code == svo.svo.ERR_INVALID
+
+
+def test_frames_call_refuses_bad_arguments_before_device_use(svo, golden):
+    """svo_render_frames_device checks the frame count, the layout, the stride,
+    every image pointer and every camera before anything touches a device."""
+    t = svo.VoxelOctree(os.path.join(golden, "dragon.oct"))
+    cam = t.orbit_camera(30.0, 10.0, 1.0)
+    bad = t.orbit_camera(30.0, 10.0, 1.0)
+    bad.pos[1] = float("inf")
+    fake = 0x1000  # never dereferenced: every case is refused first
+    cases = [([], 64, 64, [], 16, 1),                       # no frames
+             ([cam], 64, 100, [fake], 16, 1),               # 16 stripes of 7 rows leave the last one empty
+             ([cam], 64, 64, [fake], 16, 9),                # stride out of range
+             ([cam, cam], 64, 64, [fake, None], 16, 1),     # null image
+             ([cam, bad], 64, 64, [fake, fake], 16, 1)]     # camera entry not finite
+    for cams, w, h, ptrs, stripes, stride in cases:
+        with pytest.raises(svo.SvoError) as e:
+            t.render_frames_device(cams, w, h, ptrs, None, stripes, stride)
+        assert e.value.code == svo.svo.ERR_INVALID, (len(cams), w, h, stride)
