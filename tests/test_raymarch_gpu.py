@@ -361,6 +361,9 @@ def test_autotune_picks_a_shape_and_keeps_results(svo, trees):
         assert times[f"{w}x{r}"] == min(times.values())
     ren.render_device(views, None, w, r, form)
     torch.cuda.synchronize()
+    with pytest.raises(svo.SvoError) as e:  # no such launch form
+        ren.render_device(views, None, 0, 0, 7)
+    assert e.value.code == svo.svo.ERR_INVALID
     _check_batch(outs, [t.render_primary(cam, ww, hh) for t, cam, ww, hh, _, _ in outs])
 
 
