@@ -2,11 +2,17 @@
 // the device: every word the raymarch kernel can read must lie inside the
 // array, so a corrupt or hostile .oct cannot make a kernel read out of bounds.
 //
-// Layout (written by src/VoxelOctree.cpp:199-265, read by :312-399): a
+// Layout (written by src/VoxelOctree.cpp:148-204, read by :252-338): a
 // descriptor holds the child offset in bits 31..18 (relative to its own index),
 // bit 17 = far pointer (offset = bits31..18 << 32 | next word), bit 16 = two
-// words per child slot, bits 15..8 valid mask, bits 7..0 non-leaf mask. The
-// k-th valid child (ascending bit) sits at index + offset + k*(1+large).
+// words per child slot, bits 15..8 valid mask, bits 7..0 non-leaf mask.
+// The check addresses children exactly as the traversal does
+// (src/VoxelOctree.cpp:282,290-293): a valid non-leaf child m at
+// index + offset + popc(non-leaf bits below m) * (1+large), a valid leaf child
+// m at index + offset + popc(valid bits below m). The reference builder writes
+// non-leaf = valid or 0, where both reduce to the k-th valid child at
+// index + offset + k*(1+large); other masks (non-leaf bits without a valid bit
+// included) are checked at the addresses the traversal will read.
 #include <sys/mman.h>
 
 #include <algorithm>
@@ -52,14 +58,18 @@ int validate_from(const uint32_t *w, uint64_t n, Item root, int split, std::vect
         uint64_t per = 1 + ((d >> 16) & 1u);
         uint64_t base = it.idx + it.off;
         if (base < it.idx) return -1;
-        uint64_t k = 0;
         if (it.level + 2 > depth && valid) depth = it.level + 2;
         for (int m = 0; m < 8; ++m) {
             if (!((valid >> m) & 1u)) continue;
-            uint64_t slot = base + k * per;
-            k++;
-            if (slot >= n) return -1;
-            if ((nonleaf >> m) & 1u) {
+            const uint32_t below = (1u << m) - 1u;
+            const bool inner = (nonleaf >> m) & 1u;
+            // the traversal's own slot: non-leaf children count non-leaf bits
+            // (times the slot width), leaf children count valid bits
+            const uint64_t slot =
+                inner ? base + uint64_t(__builtin_popcount(nonleaf & below)) * per
+                      : base + uint64_t(__builtin_popcount(valid & below));
+            if (slot < base || slot >= n) return -1;
+            if (inner) {
                 uint32_t cw = w[slot];
                 uint64_t coff = cw >> 18;
                 if (cw & 0x20000u) {

@@ -150,8 +150,81 @@ def test_corrupt_streams_load_valid_trees_or_fail_cleanly(svo, golden):
             refused += 1
             continue
         assert 1 <= t.depth() <= 23 and t.word_count() >= 1
+        # an accepted tree is one the traversal walks without leaving the array
+        orc.raymarch(t.words(), *_probe_rays(rng, 64))
         loaded += 1
     assert refused > 100 and loaded > 10, (loaded, refused)
+
+
+def _probe_rays(rng, n):
+    """n rays from random points around the unit cube [1,2]^3 toward random
+    points inside it (origins, directions)."""
+    o = rng.uniform(0.2, 2.8, (n, 3)).astype(np.float32)
+    tgt = rng.uniform(1.0, 2.0, (n, 3)).astype(np.float32)
+    d = tgt - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return o, d.astype(np.float32)
+
+
+def test_descriptor_masks_are_checked_where_the_traversal_reads():
+    """The traversal finds a non-leaf child m at base + popc(non-leaf bits below
+    m) * (1 + large) and a leaf child at base + popc(valid bits below m)
+    (src/VoxelOctree.cpp:282,290-293). Masks the reference builder never writes
+    (non-leaf neither 0 nor equal to valid, non-leaf bits without a valid bit)
+    are checked at those addresses: here both send the traversal far outside
+    the array and must be refused."""
+    import conftest
+    svo = conftest.load_package()
+    # child 1 (non-leaf) sits at base + 0, the leaf word 0xFFFFFFFF, read as a
+    # far descriptor pointing ~2^46 words away
+    with pytest.raises(svo.SvoError) as e:
+        svo.VoxelOctree.from_words(np.uint32([0x00040302, 0xFFFFFFFF, 0x00040100, 0x00808080]), [0.5] * 3)
+    assert e.value.code == svo.svo.ERR_FORMAT
+    # a non-leaf bit without its valid bit shifts child 1 to base + 1 (word 2, a leaf word)
+    with pytest.raises(svo.SvoError) as e:
+        svo.VoxelOctree.from_words(np.uint32([(1 << 18) | (0x02 << 8) | 0x03, (1 << 18) | (0x01 << 8), 0x00808080]),
+                                   [0.5] * 3)
+    assert e.value.code == svo.svo.ERR_FORMAT
+    # the same trees with the masks the reference writes are accepted
+    assert svo.VoxelOctree.from_words(np.uint32([(1 << 18) | (0x02 << 8) | 0x02, (1 << 18) | (0x01 << 8), 0x00808080]),
+                                      [0.5] * 3).depth() == 3
+
+
+def test_mutated_descriptor_masks_load_traceable_trees_or_fail_cleanly(golden):
+    """Random bit flips in the masks and flags of the dragon's descriptors
+    (non-leaf, valid, large, far): every tree the check accepts traces in the
+    bounds-checked oracle without leaving the word array."""
+    import conftest
+    svo = conftest.load_package()
+    words = svo.VoxelOctree(os.path.join(golden, "dragon.oct")).words()
+    rng = np.random.default_rng(355)
+    # descriptors of the top levels: the words reached from the root within 3 levels
+    top, frontier = [0], [0]
+    for _ in range(3):
+        nxt = []
+        for i in frontier:
+            w = int(words[i])
+            off, valid, nonleaf, per = w >> 18, (w >> 8) & 0xFF, w & 0xFF, 1 + ((w >> 16) & 1)
+            nxt += [i + off + bin(nonleaf & ((1 << m) - 1)).count("1") * per
+                    for m in range(8) if (valid >> m) & (nonleaf >> m) & 1]
+        top += nxt
+        frontier = nxt
+    loaded = refused = 0
+    o, d = _probe_rays(rng, 256)
+    for _ in range(300):
+        w = words.copy()
+        for _ in range(int(rng.integers(1, 4))):
+            i = int(top[int(rng.integers(0, len(top)))])
+            w[i] ^= np.uint32(1 << int(rng.integers(0, 18)))
+        try:
+            t = svo.VoxelOctree.from_words(w, [0.5] * 3)
+        except svo.SvoError as e:
+            assert e.code == svo.svo.ERR_FORMAT, str(e)
+            refused += 1
+            continue
+        orc.raymarch(t.words(), o, d)
+        loaded += 1
+    assert loaded > 20 and refused > 20, (loaded, refused)
 
 
 def test_word_count_claim_is_bounded_by_the_stream(svo):
