@@ -361,11 +361,11 @@ def run_frames(pkg, torch, stream, tree, cams, width, height, steps, warmup, cpu
            "ms_per_frame": round(ms / len(cams), 4), "frames_per_s": round(len(cams) / (ms / 1e3), 1),
            "rays_per_step": traced, "grays_s": round(traced / (ms / 1e3) / 1e9, 4),
            "fine_pixels_traced_frac": round((traced - nd * len(cams)) / (m * len(cams)), 4)}
-    # The same frames in one svo_render_frames_device call per step: consecutive
-    # frames alternate between this stream and the library's side stream, so a
-    # frame's coarse pass (a few hundred waves) and its fine pass's tail run
-    # beside the neighbouring frame's fine pass. Images and depth tables must
-    # equal the one-frame-per-call ones.
+    # The same frames in one svo_render_frames_device call per step: up to 16
+    # frames share one coarse launch and one fine launch on this one stream, so
+    # a frame's coarse pass (a few hundred waves) and its fine pass's tail run
+    # beside the other frames' work. Images and depth tables must equal the
+    # one-frame-per-call ones.
     rgba2 = [torch.empty_like(x) for x in rgba]
     depth2 = [torch.empty_like(x) for x in depth]
     rptrs, dptrs = [x.data_ptr() for x in rgba2], [x.data_ptr() for x in depth2]

@@ -457,11 +457,11 @@ def test_frame_device_entry_and_errors(svo, trees):
 
 
 def test_frames_in_one_call_equal_single_frames(svo, trees):
-    """svo_render_frames_device (consecutive frames alternating between the
-    caller's stream and a side stream) gives each frame's RGBA and depth table
-    exactly as a single-frame render, with per-frame NULL depths, an odd frame
-    count and stride 3; bad cameras or null images are refused before any
-    frame is enqueued."""
+    """svo_render_frames_device (up to 16 frames per coarse launch and per
+    fine launch, on the caller's one stream) gives each frame's RGBA and depth
+    table exactly as a single-frame render, with per-frame NULL depths, an odd
+    frame count and stride 3, and for 18 frames (two launch groups, 16 + 2);
+    bad cameras or null images are refused before any frame is enqueued."""
     import torch
     t = trees["dragon"]
     w, h, stripes = 320, 176, 16
