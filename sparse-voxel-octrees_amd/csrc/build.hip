@@ -27,7 +27,7 @@
 //  the far-pointer rule, offsets top-down, and a scatter of every word into
 //  the depth-first layout (see size_kernel / place_kernel).
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
 #include <chrono>
@@ -504,14 +504,26 @@ __device__ __forceinline__ bool pair_box(uint32_t f, const float tv[9], const Me
     return true;
 }
 
+// Rows (y, z) of each pair's clipped voxel box; *voxels accumulates the
+// boxes' voxel counts, an upper bound of the (voxel, triangle) candidates.
 __global__ void pair_rows_kernel(const uint32_t *__restrict__ cells, const uint32_t *__restrict__ tris, uint64_t P,
-                                 const float *__restrict__ pos, MeshParams mp, uint32_t *__restrict__ rows) {
+                                 const float *__restrict__ pos, MeshParams mp, uint32_t *__restrict__ rows,
+                                 unsigned long long *__restrict__ voxels) {
     uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= P) return;
-    float tv[9];
-    load_tri(pos, tris[i], tv);
-    int l[3], u[3];
-    rows[i] = pair_box(cells[i], tv, mp, l, u) ? uint32_t((u[1] - l[1] + 1) * (u[2] - l[2] + 1)) : 0u;
+    uint64_t box = 0;
+    if (i < P) {
+        float tv[9];
+        load_tri(pos, tris[i], tv);
+        int l[3], u[3];
+        const bool any = pair_box(cells[i], tv, mp, l, u);
+        const uint32_t nr = any ? uint32_t((u[1] - l[1] + 1) * (u[2] - l[2] + 1)) : 0u;
+        rows[i] = nr;
+        if (any) box = uint64_t(nr) * uint64_t(u[0] - l[0] + 1);
+    }
+    // one atomic per wave
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) box += __shfl_xor(box, o);
+    if ((threadIdx.x & 63) == 0 && box) atomicAdd(voxels, (unsigned long long)box);
 }
 
 // Cache blocks with a non-empty cell list (PlyLoader::isBlockEmpty,
@@ -541,41 +553,84 @@ __device__ __forceinline__ uint64_t find_pair(const uint64_t *__restrict__ rowSt
 }
 
 // The voxel loop of PlyLoader::triangleToVolume (src/PlyLoader.cpp:353-373)
-// for one (y, z) row: centres start at (l - 0.5)*hx and step by +hx, exactly as
-// the reference accumulates them. Count pass (keys == nullptr) or emit pass.
-__global__ void row_voxels_kernel(const uint32_t *__restrict__ cells, const uint32_t *__restrict__ tris, uint64_t P,
-                                  const uint64_t *__restrict__ rowStart, uint64_t R, const float *__restrict__ pos,
-                                  MeshParams mp, int tribits, uint32_t *__restrict__ hitCount,
-                                  const uint64_t *__restrict__ hitStart, uint64_t *__restrict__ keys) {
-    uint64_t r = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (r >= R) return;
-    const uint64_t i = find_pair(rowStart, P, r);
-    const uint32_t t = tris[i];
+// for one (y, z) row per thread: centres start at (l - 0.5)*hx and step by
+// +hx, exactly as the reference accumulates them, in one pass: each row's SAT
+// results are kept as bit masks (256 voxels per pass over the row), the wave
+// reserves space for all its lanes' hits with one atomic, and each lane
+// writes its keys. The keys'
+// order is arbitrary (they are sorted next). cap bounds the key array; a
+// reservation past it raises *overflow (the pair boxes' voxel count is an
+// upper bound, so this does not happen).
+__global__ void __launch_bounds__(kThreads) row_voxels_emit_kernel(
+    const uint32_t *__restrict__ cells, const uint32_t *__restrict__ tris, uint64_t P,
+    const uint64_t *__restrict__ rowStart, uint64_t R, const float *__restrict__ pos, MeshParams mp, int tribits,
+    uint64_t *__restrict__ keys, uint64_t cap, unsigned long long *__restrict__ count, uint32_t *__restrict__ overflow) {
+    const uint64_t r = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    int len = 0, x0 = 0, y = 0, z = 0;
+    uint32_t t = 0;
     float tv[9];
-    load_tri(pos, t, tv);
-    int l[3], u[3];
-    pair_box(cells[i], tv, mp, l, u);
-    const uint64_t local = r - rowStart[i];
-    const int ny = u[1] - l[1] + 1;
-    const int y = l[1] + int(local % uint64_t(ny));
-    const int z = l[2] + int(local / uint64_t(ny));
+    float ctr[3] = {0.0f, 0.0f, 0.0f};
     const float hx = mp.hx;
     const float half[3] = {0.5f * hx, 0.5f * hx, 0.5f * hx};
-    float ctr[3];
-    ctr[2] = (float(l[2]) - 0.5f) * hx;
-    for (int k = l[2]; k < z; ++k) ctr[2] += hx;
-    ctr[1] = (float(l[1]) - 0.5f) * hx;
-    for (int k = l[1]; k < y; ++k) ctr[1] += hx;
-    ctr[0] = (float(l[0]) - 0.5f) * hx;
-    uint32_t n = 0;
-    uint64_t at = keys ? hitStart[r] : 0;
-    for (int x = l[0]; x <= u[0]; ++x, ctr[0] += hx) {
-        if (tri_box(ctr, half, tv)) {
-            if (keys) keys[at + n] = (morton3(uint32_t(x), uint32_t(y), uint32_t(z)) << tribits) | uint64_t(t);
-            n++;
+    if (r < R) {
+        const uint64_t i = find_pair(rowStart, P, r);
+        t = tris[i];
+        load_tri(pos, t, tv);
+        int l[3], u[3];
+        pair_box(cells[i], tv, mp, l, u);
+        const uint64_t local = r - rowStart[i];
+        const int ny = u[1] - l[1] + 1;
+        y = l[1] + int(local % uint64_t(ny));
+        z = l[2] + int(local / uint64_t(ny));
+        ctr[2] = (float(l[2]) - 0.5f) * hx;
+        for (int k = l[2]; k < z; ++k) ctr[2] += hx;
+        ctr[1] = (float(l[1]) - 0.5f) * hx;
+        for (int k = l[1]; k < y; ++k) ctr[1] += hx;
+        ctr[0] = (float(l[0]) - 0.5f) * hx;
+        x0 = l[0];
+        len = u[0] - l[0] + 1;
+    }
+    int maxLen = len;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) maxLen = max(maxLen, __shfl_xor(maxLen, o));
+    for (int seg = 0; seg < maxLen; seg += 256) {
+        uint64_t m[4] = {0, 0, 0, 0};
+        const int segLen = min(max(len - seg, 0), 256);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int n = min(max(segLen - 64 * q, 0), 64);
+            for (int j = 0; j < n; ++j, ctr[0] += hx)
+                if (tri_box(ctr, half, tv)) m[q] |= uint64_t(1) << j;
+        }
+        const uint64_t hits = uint64_t(__popcll(m[0]) + __popcll(m[1]) + __popcll(m[2]) + __popcll(m[3]));
+        // wave-wide exclusive sum of the hit counts, then one reservation
+        uint64_t incl = hits;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint64_t v = __shfl_up(incl, o);
+            if (lane >= o) incl += v;
+        }
+        const uint64_t waveTotal = __shfl(incl, 63);
+        unsigned long long base = 0;
+        if (lane == 0 && waveTotal) base = atomicAdd(count, (unsigned long long)waveTotal);
+        base = __shfl(base, 0);
+        uint64_t at = base + incl - hits;
+        if (at + hits > cap) {
+            if (hits) atomicOr(overflow, 1u);
+            continue;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            uint64_t bits = m[q];
+            while (bits) {
+                const int j = __builtin_ctzll(bits);
+                bits &= bits - 1;
+                const uint32_t x = uint32_t(x0 + seg + 64 * q + j);
+                keys[at++] = (morton3(x, uint32_t(y), uint32_t(z)) << tribits) | uint64_t(t);
+            }
         }
     }
-    if (!keys) hitCount[r] = n;
 }
 
 __global__ void seg_heads_kernel(const uint64_t *__restrict__ keys, uint64_t K, int shift, uint32_t *__restrict__ head) {
@@ -802,18 +857,22 @@ __global__ void compact_kernel(const uint64_t *__restrict__ key, const uint32_t 
 // levels, sizes, placement
 // ---------------------------------------------------------------------------
 
-__global__ void parent_heads_kernel(const uint64_t *__restrict__ child, uint64_t n, uint32_t *__restrict__ head) {
+// Level kernels read the level's node count from a device word (nptr): the
+// count comes from the previous level's scan, so no host round trip sizes
+// them; grids are sized by a host upper bound.
+__global__ void parent_heads_kernel(const uint64_t *__restrict__ child, const uint64_t *__restrict__ nptr,
+                                    uint32_t *__restrict__ head) {
     uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= *nptr) return;
     head[i] = (i == 0 || (child[i] >> 3) != (child[i - 1] >> 3)) ? 1u : 0u;
 }
 
 // parent key / child start / mask from sorted child keys (no extra parents)
-__global__ void parents_kernel(const uint64_t *__restrict__ child, uint64_t n, const uint32_t *__restrict__ head,
-                               const uint64_t *__restrict__ incl, uint64_t *__restrict__ pkey,
-                               uint64_t *__restrict__ pstart, uint32_t *__restrict__ pmask) {
+__global__ void parents_kernel(const uint64_t *__restrict__ child, const uint64_t *__restrict__ nptr,
+                               const uint32_t *__restrict__ head, const uint64_t *__restrict__ incl,
+                               uint64_t *__restrict__ pkey, uint64_t *__restrict__ pstart, uint32_t *__restrict__ pmask) {
     uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= *nptr) return;
     const uint64_t p = incl[i];  // exclusive scan of heads: parent index of child i is p + head - 1
     const uint64_t pi = p + head[i] - 1;
     if (head[i]) {
@@ -825,9 +884,10 @@ __global__ void parents_kernel(const uint64_t *__restrict__ child, uint64_t n, c
 
 // Level that also takes the cache blocks the occupancy table marks: parents
 // are the flagged cells of a dense Morton-indexed table.
-__global__ void dense_mark_kernel(const uint64_t *__restrict__ child, uint64_t n, uint32_t *__restrict__ dense) {
+__global__ void dense_mark_kernel(const uint64_t *__restrict__ child, const uint64_t *__restrict__ nptr,
+                                  uint32_t *__restrict__ dense) {
     uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= *nptr) return;
     dense[child[i] >> 3] = 1u;
 }
 
@@ -840,10 +900,11 @@ __global__ void dense_compact_kernel(const uint32_t *__restrict__ dense, uint64_
     slot[i] = at[i];
 }
 
-__global__ void dense_children_kernel(const uint64_t *__restrict__ child, uint64_t n, const uint64_t *__restrict__ slot,
-                                      uint64_t *__restrict__ pstart, uint32_t *__restrict__ pmask) {
+__global__ void dense_children_kernel(const uint64_t *__restrict__ child, const uint64_t *__restrict__ nptr,
+                                      const uint64_t *__restrict__ slot, uint64_t *__restrict__ pstart,
+                                      uint32_t *__restrict__ pmask) {
     uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= *nptr) return;
     const uint64_t pk = child[i] >> 3;
     const uint64_t pi = slot[pk];
     if (i == 0 || (child[i - 1] >> 3) != pk) pstart[pi] = i;
@@ -856,11 +917,11 @@ __global__ void dense_children_kernel(const uint64_t *__restrict__ child, uint64
 // subtrees S. Child i's offset from its own slot, before the node's far words,
 // is n - i + sum_{j<i} S(c_j); the node needs far pointers (large) if any
 // exceeds 0x3FFF, and then every child slot takes a second word.
-__global__ void size_kernel(const uint32_t *__restrict__ mask, const uint64_t *__restrict__ start, uint64_t n,
-                            int leafParents, const uint64_t *__restrict__ childS, uint64_t *__restrict__ S,
-                            uint8_t *__restrict__ large) {
+__global__ void size_kernel(const uint32_t *__restrict__ mask, const uint64_t *__restrict__ start,
+                            const uint64_t *__restrict__ nptr, int leafParents, const uint64_t *__restrict__ childS,
+                            uint64_t *__restrict__ S, uint8_t *__restrict__ large) {
     uint64_t k = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (k >= n) return;
+    if (k >= *nptr) return;
     const uint32_t c = __popc(mask[k]);
     if (leafParents || c == 0) {
         S[k] = c;
@@ -882,12 +943,13 @@ __global__ void size_kernel(const uint32_t *__restrict__ mask, const uint64_t *_
 // Placement, top-down: a node at `pos` whose child block starts at `B` puts
 // child i at B + i*(1+large); that child's own block starts after the node's
 // whole child block and the subtrees of children 0..i-1.
-__global__ void place_kernel(const uint32_t *__restrict__ mask, const uint64_t *__restrict__ start, uint64_t n,
-                             const uint64_t *__restrict__ B, const uint8_t *__restrict__ large,
-                             const uint64_t *__restrict__ childS, int childLeaves, uint64_t *__restrict__ childPos,
-                             uint64_t *__restrict__ childB, uint8_t *__restrict__ childFar) {
+__global__ void place_kernel(const uint32_t *__restrict__ mask, const uint64_t *__restrict__ start,
+                             const uint64_t *__restrict__ nptr, const uint64_t *__restrict__ B,
+                             const uint8_t *__restrict__ large, const uint64_t *__restrict__ childS, int childLeaves,
+                             uint64_t *__restrict__ childPos, uint64_t *__restrict__ childB,
+                             uint8_t *__restrict__ childFar) {
     uint64_t k = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (k >= n) return;
+    if (k >= *nptr) return;
     const uint32_t c = __popc(mask[k]);
     if (c == 0) return;
     const uint64_t per = large[k] ? 2 : 1;
@@ -906,12 +968,12 @@ __global__ void place_kernel(const uint32_t *__restrict__ mask, const uint64_t *
 // Descriptor word (and far word) of every node of one level.
 // Positions are checked against the word count W; a violation (which the
 // size/placement arithmetic excludes) is reported instead of written.
-__global__ void write_nodes_kernel(const uint32_t *__restrict__ mask, uint64_t n, const uint64_t *__restrict__ pos,
-                                   const uint64_t *__restrict__ B, const uint8_t *__restrict__ large,
-                                   const uint8_t *__restrict__ far, int leafParents, uint32_t *__restrict__ out,
-                                   uint64_t W, uint32_t *__restrict__ bad) {
+__global__ void write_nodes_kernel(const uint32_t *__restrict__ mask, const uint64_t *__restrict__ nptr,
+                                   const uint64_t *__restrict__ pos, const uint64_t *__restrict__ B,
+                                   const uint8_t *__restrict__ large, const uint8_t *__restrict__ far, int leafParents,
+                                   uint32_t *__restrict__ out, uint64_t W, uint32_t *__restrict__ bad) {
     uint64_t k = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (k >= n) return;
+    if (k >= *nptr) return;
     const uint32_t m = mask[k];
     uint32_t w = (m << 8) | (leafParents ? 0u : m);
     if (large[k]) w |= 0x10000u;
@@ -944,6 +1006,26 @@ __global__ void write_leaves_kernel(const uint32_t *__restrict__ word, uint64_t 
 __global__ void fill_u32_kernel(uint32_t *p, uint64_t n, uint32_t v) {
     uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
+}
+
+// The root level of an empty tree or the root's placement: one node.
+__global__ void root_init_kernel(uint64_t *key, uint64_t *start, uint32_t *mask, uint64_t *cnt) {
+    *key = 0;
+    *start = 0;
+    *mask = 0;
+    *cnt = 1;
+}
+
+__global__ void root_place_kernel(uint64_t *pos, uint64_t *B, uint8_t *far) {
+    *pos = 0;
+    *B = 1;
+    *far = 0;
+}
+
+// {root subtree size, root level count} for the host, in one read.
+__global__ void root_report_kernel(const uint64_t *S0, const uint64_t *cnt0, uint64_t *out) {
+    out[0] = *S0;
+    out[1] = *cnt0;
 }
 
 // ---------------------------------------------------------------------------
@@ -1148,43 +1230,173 @@ __global__ void terrain_emit_kernel(int n, const int32_t *__restrict__ H, const 
 }
 
 // ---------------------------------------------------------------------------
+// exclusive prefix sums with the element count in device memory
+// ---------------------------------------------------------------------------
+
+// Three passes over blocks of kScanTile elements: per-block sums, one
+// workgroup scanning the block sums (and writing the grand total), then each
+// block's scan plus its offset. The count is read on the device, so a pass
+// whose input count was produced by an earlier kernel needs no round trip to
+// the host; grids are sized by a host-side upper bound and blocks past the
+// count exit at once.
+constexpr int kScanItems = 8;
+constexpr uint64_t kScanTile = uint64_t(kThreads) * kScanItems;
+
+// Exclusive sum of `v` over the 256 threads of the workgroup; *all = total.
+__device__ __forceinline__ uint64_t block_exclusive(uint64_t v, uint64_t *lds, uint64_t *all) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint64_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint64_t y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) lds[wave] = x;
+    __syncthreads();
+    uint64_t before = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < kThreads / 64; ++w) {
+        if (w < wave) before += lds[w];
+        total += lds[w];
+    }
+    __syncthreads();
+    *all = total;
+    return before + x - v;
+}
+
+template <typename In>
+__global__ void __launch_bounds__(kThreads) scan_sums_kernel(const In *__restrict__ in, const uint64_t *__restrict__ nptr,
+                                                             uint64_t *__restrict__ sums) {
+    __shared__ uint64_t lds[kThreads / 64];
+    const uint64_t n = *nptr;
+    const uint64_t base = uint64_t(blockIdx.x) * kScanTile;
+    if (base >= n) {
+        if (threadIdx.x == 0) sums[blockIdx.x] = 0;
+        return;
+    }
+    uint64_t v = 0;
+    const uint64_t first = base + uint64_t(threadIdx.x) * kScanItems;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k)
+        if (first + k < n) v += uint64_t(in[first + k]);
+    uint64_t total;
+    block_exclusive(v, lds, &total);
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+// One workgroup: exclusive scan of nb block sums in place; *total = their sum.
+__global__ void __launch_bounds__(kThreads) scan_block_sums_kernel(uint64_t *__restrict__ sums, uint64_t nb,
+                                                                   uint64_t *__restrict__ total) {
+    __shared__ uint64_t lds[kThreads / 64];
+    uint64_t carry = 0;
+    for (uint64_t base = 0; base < nb; base += kThreads) {
+        const uint64_t i = base + threadIdx.x;
+        const uint64_t v = i < nb ? sums[i] : 0;
+        uint64_t all;
+        const uint64_t ex = block_exclusive(v, lds, &all);
+        if (i < nb) sums[i] = carry + ex;
+        carry += all;
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+
+template <typename In>
+__global__ void __launch_bounds__(kThreads) scan_down_kernel(const In *__restrict__ in, const uint64_t *__restrict__ nptr,
+                                                             const uint64_t *__restrict__ offs, uint64_t *__restrict__ out) {
+    __shared__ uint64_t lds[kThreads / 64];
+    const uint64_t n = *nptr;
+    const uint64_t base = uint64_t(blockIdx.x) * kScanTile;
+    if (base >= n) return;
+    const uint64_t first = base + uint64_t(threadIdx.x) * kScanItems;
+    uint64_t v[kScanItems];
+    uint64_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        v[k] = first + k < n ? uint64_t(in[first + k]) : 0;
+        sum += v[k];
+    }
+    uint64_t total;
+    uint64_t run = offs[blockIdx.x] + block_exclusive(sum, lds, &total);
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        if (first + k < n) out[first + k] = run;
+        run += v[k];
+    }
+}
+
+__global__ void set_u64_kernel(uint64_t *p, uint64_t v) { *p = v; }
+
+// *out = 1 when a cache block other than the last flagged one (the last
+// built) holds an unscanned voxel; one workgroup.
+__global__ void __launch_bounds__(kThreads) stale_blocks_kernel(const uint32_t *__restrict__ hazard,
+                                                                const uint32_t *__restrict__ flags, uint64_t nBlocks,
+                                                                uint64_t *__restrict__ out) {
+    __shared__ int last;
+    if (threadIdx.x == 0) last = -1;
+    __syncthreads();
+    for (uint64_t b = threadIdx.x; b < nBlocks; b += blockDim.x)
+        if (flags[b]) atomicMax(&last, int(b));
+    __syncthreads();
+    for (uint64_t b = threadIdx.x; b < nBlocks; b += blockDim.x)
+        if (hazard[b] && int(b) != last) *out = 1;
+}
+__global__ void widen_u32_kernel(const uint32_t *src, uint64_t *dst) { *dst = *src; }
+
+// ---------------------------------------------------------------------------
 // host helpers
 // ---------------------------------------------------------------------------
 
-struct ToU64 {
-    template <typename T>
-    __host__ __device__ uint64_t operator()(const T &v) const {
-        return uint64_t(v);
-    }
-};
-
-// Exclusive prefix sum accumulated in 64 bits.
+// Exclusive prefix sum of in[0, n) into out (64-bit), n read from the device
+// word *nptr; `bound` (host) is an upper bound of n that sizes the grids.
+// *total (device, may be null) receives the sum of all n elements.
 template <typename In>
-int exclusive_scan(Arena &ar, const In *in, uint64_t *out, uint64_t n, hipStream_t s) {
-    if (n == 0) return SVO_OK;
-    hipcub::TransformInputIterator<uint64_t, ToU64, const In *> it(in, ToU64());
-    size_t bytes = 0;
-    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, it, out, n, s));
-    uint8_t *tmp;
-    int r = ar.alloc(tmp, bytes);
+int exclusive_scan_dev(Arena &ar, const In *in, uint64_t *out, const uint64_t *nptr, uint64_t bound, uint64_t *total,
+                       hipStream_t s) {
+    const uint64_t nb = std::max<uint64_t>(1, (bound + kScanTile - 1) / kScanTile);
+    if (nb > 0x7FFFFFFFull) return fail(SVO_ERR_LIMIT, "builder: scan too large");
+    uint64_t *sums, *tot = total;
+    int r = ar.alloc(sums, nb);
     if (r) return r;
-    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(tmp, bytes, it, out, n, s));
-    ar.release(tmp);
+    if (!tot && (r = ar.alloc(tot, 1))) return r;
+    hipLaunchKernelGGL(scan_sums_kernel<In>, dim3(unsigned(nb)), dim3(kThreads), 0, s, in, nptr, sums);
+    hipLaunchKernelGGL(scan_block_sums_kernel, dim3(1), dim3(kThreads), 0, s, sums, nb, tot);
+    hipLaunchKernelGGL(scan_down_kernel<In>, dim3(unsigned(nb)), dim3(kThreads), 0, s, in, nptr, sums, out);
+    HIP_TRY(hipGetLastError());
+    ar.release(sums);
+    if (!total) ar.release(tot);
     return SVO_OK;
 }
 
-// total = exclusive[n-1] + last element
+// The same with a host-known count; *total (device, may be null) as above.
 template <typename In>
-int scan_total(const In *in, const uint64_t *ex, uint64_t n, uint64_t &total) {
+int exclusive_scan(Arena &ar, const In *in, uint64_t *out, uint64_t n, hipStream_t s, uint64_t *total = nullptr) {
+    uint64_t *nd;
+    int r = ar.alloc(nd, 1);
+    if (r) return r;
+    hipLaunchKernelGGL(set_u64_kernel, dim3(1), dim3(1), 0, s, nd, n);
+    r = exclusive_scan_dev(ar, in, out, nd, n, total, s);
+    ar.release(nd);
+    return r;
+}
+
+// Reads device words into host values in one copy (one round trip).
+int read_words(const uint64_t *src, uint64_t *dst, int count, hipStream_t s) {
+    HIP_TRY(hipMemcpyAsync(dst, src, sizeof(uint64_t) * count, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return SVO_OK;
+}
+
+// Exclusive scan with a host-known count whose total the host needs next.
+template <typename In>
+int scan_count(Arena &ar, const In *in, uint64_t *out, uint64_t n, hipStream_t s, uint64_t &total) {
     if (n == 0) {
         total = 0;
         return SVO_OK;
     }
-    In last;
-    uint64_t lastEx;
-    HIP_TRY(hipMemcpy(&last, in + n - 1, sizeof(In), hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(&lastEx, ex + n - 1, sizeof(uint64_t), hipMemcpyDeviceToHost));
-    total = lastEx + uint64_t(last);
+    uint64_t *tot;
+    int r = ar.alloc(tot, 1);
+    if (r || (r = exclusive_scan(ar, in, out, n, s, tot)) || (r = read_words(tot, &total, 1, s))) return r;
+    ar.release(tot);
     return SVO_OK;
 }
 
@@ -1195,7 +1407,9 @@ int fill32(uint32_t *p, uint64_t n, uint32_t v, hipStream_t s) {
 }
 
 struct Level {
-    uint64_t n = 0;
+    uint64_t n = 0;             // node count when known on the host (the leaves)
+    uint64_t bound = 0;         // host upper bound of the node count (sizes arrays and grids)
+    uint64_t *cnt = nullptr;    // node count, device word
     uint64_t *key = nullptr;
     uint32_t *mask = nullptr;   // child mask (nodes above the leaves)
     uint64_t *start = nullptr;  // index of the first child in the level below
@@ -1273,11 +1487,20 @@ int emit_tree(Arena &ar, const Geometry &g, uint64_t *leafKey, uint32_t *leafWor
     const int D = g.depth;
     std::vector<Level> lv(size_t(D) + 1);
     lv[D].n = L;
+    lv[D].bound = L;
     lv[D].key = leafKey;
     lv[D].word = leafWord;
     const int cacheLevel = D - g.cacheLog;  // nodes of edge `cache`
     const bool multi = g.cacheLog < D;
     int r;
+    for (Level &l : lv)
+        if ((r = ar.alloc(l.cnt, 1))) return r;
+    hipLaunchKernelGGL(set_u64_kernel, dim3(1), dim3(1), 0, st, lv[D].cnt, L);
+    // Levels bottom-up. Node counts stay on the device: each level's arrays
+    // are sized by a host bound (no more nodes than the level below, nor than
+    // 8^d cells), so no level waits for the host. `empty`: the level below is
+    // known on the host to hold no node (a tree without leaves).
+    bool empty = L == 0;
     for (int d = D - 1; d >= 0; --d) {
         Level &c = lv[d + 1];
         Level &p = lv[d];
@@ -1287,75 +1510,81 @@ int emit_tree(Arena &ar, const Geometry &g, uint64_t *leafKey, uint32_t *leafWor
             uint64_t *at, *slot;
             if ((r = ar.alloc(dense, cells)) || (r = ar.alloc(at, cells)) || (r = ar.alloc(slot, cells))) return r;
             HIP_TRY(hipMemcpyAsync(dense, blockFlags, cells * 4, hipMemcpyDeviceToDevice, st));
-            if (c.n) hipLaunchKernelGGL(dense_mark_kernel, dim3(grid_for(c.n)), dim3(kThreads), 0, st, c.key, c.n, dense);
-            if ((r = exclusive_scan(ar, dense, at, cells, st))) return r;
-            if ((r = scan_total(dense, at, cells, p.n))) return r;
-            if ((r = ar.alloc(p.key, p.n)) || (r = ar.alloc(p.start, p.n)) || (r = ar.alloc(p.mask, p.n))) return r;
-            if ((r = fill32(p.mask, p.n, 0, st))) return r;
+            if (c.bound)
+                hipLaunchKernelGGL(dense_mark_kernel, dim3(grid_for(c.bound)), dim3(kThreads), 0, st, c.key, c.cnt, dense);
+            if ((r = exclusive_scan(ar, dense, at, cells, st, p.cnt))) return r;
+            p.bound = cells;
+            if ((r = ar.alloc(p.key, p.bound)) || (r = ar.alloc(p.start, p.bound)) || (r = ar.alloc(p.mask, p.bound)))
+                return r;
+            if ((r = fill32(p.mask, p.bound, 0, st))) return r;
             hipLaunchKernelGGL(dense_compact_kernel, dim3(grid_for(cells)), dim3(kThreads), 0, st, dense, cells, at,
                                p.key, slot);
-            if (c.n)
-                hipLaunchKernelGGL(dense_children_kernel, dim3(grid_for(c.n)), dim3(kThreads), 0, st, c.key, c.n, slot,
-                                   p.start, p.mask);
+            if (c.bound)
+                hipLaunchKernelGGL(dense_children_kernel, dim3(grid_for(c.bound)), dim3(kThreads), 0, st, c.key, c.cnt,
+                                   slot, p.start, p.mask);
+            if (L == 0) {
+                // no leaves: the table level alone decides whether nodes exist
+                uint64_t nTable = 0;
+                if ((r = read_words(p.cnt, &nTable, 1, st))) return r;
+                empty = nTable == 0;
+            }
         } else {
-            if (c.n == 0) {
-                // empty level below: only the root can remain (an empty tree)
-                p.n = d == 0 ? 1 : 0;
+            // an empty level below: only the root remains (an empty tree)
+            if (empty) {
+                p.bound = d == 0 ? 1 : 0;
                 if ((r = ar.alloc(p.key, 1)) || (r = ar.alloc(p.start, 1)) || (r = ar.alloc(p.mask, 1))) return r;
-                uint64_t z = 0;
-                HIP_TRY(hipMemcpy(p.key, &z, 8, hipMemcpyHostToDevice));
-                HIP_TRY(hipMemcpy(p.start, &z, 8, hipMemcpyHostToDevice));
-                if ((r = fill32(p.mask, 1, 0, st))) return r;
+                if (d == 0) hipLaunchKernelGGL(root_init_kernel, dim3(1), dim3(1), 0, st, p.key, p.start, p.mask, p.cnt);
+                else hipLaunchKernelGGL(set_u64_kernel, dim3(1), dim3(1), 0, st, p.cnt, uint64_t(0));
                 continue;
             }
+            p.bound = d < 21 ? std::min(c.bound, uint64_t(1) << (3 * d)) : c.bound;
             uint32_t *head;
             uint64_t *ex;
-            if ((r = ar.alloc(head, c.n)) || (r = ar.alloc(ex, c.n))) return r;
-            hipLaunchKernelGGL(parent_heads_kernel, dim3(grid_for(c.n)), dim3(kThreads), 0, st, c.key, c.n, head);
-            if ((r = exclusive_scan(ar, head, ex, c.n, st))) return r;
-            if ((r = scan_total(head, ex, c.n, p.n))) return r;
-            if ((r = ar.alloc(p.key, p.n)) || (r = ar.alloc(p.start, p.n)) || (r = ar.alloc(p.mask, p.n))) return r;
-            if ((r = fill32(p.mask, p.n, 0, st))) return r;
-            hipLaunchKernelGGL(parents_kernel, dim3(grid_for(c.n)), dim3(kThreads), 0, st, c.key, c.n, head, ex, p.key,
-                               p.start, p.mask);
+            if ((r = ar.alloc(head, c.bound)) || (r = ar.alloc(ex, c.bound))) return r;
+            hipLaunchKernelGGL(parent_heads_kernel, dim3(grid_for(c.bound)), dim3(kThreads), 0, st, c.key, c.cnt, head);
+            if ((r = exclusive_scan_dev(ar, head, ex, c.cnt, c.bound, p.cnt, st))) return r;
+            if ((r = ar.alloc(p.key, p.bound)) || (r = ar.alloc(p.start, p.bound)) || (r = ar.alloc(p.mask, p.bound)))
+                return r;
+            if ((r = fill32(p.mask, p.bound, 0, st))) return r;
+            hipLaunchKernelGGL(parents_kernel, dim3(grid_for(c.bound)), dim3(kThreads), 0, st, c.key, c.cnt, head, ex,
+                               p.key, p.start, p.mask);
             ar.release(head);
             ar.release(ex);
         }
         HIP_TRY(hipGetLastError());
     }
-    if (lv[0].n != 1) return fail(SVO_ERR_DEVICE, "builder: root level does not hold one node");
 
     // sizes, bottom-up
     for (int d = D - 1; d >= 0; --d) {
         Level &p = lv[d];
-        if ((r = ar.alloc(p.S, p.n)) || (r = ar.alloc(p.large, p.n))) return r;
-        if (p.n)
-            hipLaunchKernelGGL(size_kernel, dim3(grid_for(p.n)), dim3(kThreads), 0, st, p.mask, p.start, p.n,
+        if ((r = ar.alloc(p.S, p.bound)) || (r = ar.alloc(p.large, p.bound))) return r;
+        if (p.bound)
+            hipLaunchKernelGGL(size_kernel, dim3(grid_for(p.bound)), dim3(kThreads), 0, st, p.mask, p.start, p.cnt,
                                int(d == D - 1), lv[d + 1].S, p.S, p.large);
         HIP_TRY(hipGetLastError());
     }
-    uint64_t rootS = 0;
-    HIP_TRY(hipMemcpy(&rootS, lv[0].S, 8, hipMemcpyDeviceToHost));
-    const uint64_t W = 1 + rootS;
+    // the one round trip of the level passes: the word count sizes the array
+    uint64_t *rep, report[2] = {0, 0};
+    if ((r = ar.alloc(rep, 2))) return r;
+    hipLaunchKernelGGL(root_report_kernel, dim3(1), dim3(1), 0, st, lv[0].S, lv[0].cnt, rep);
+    if ((r = read_words(rep, report, 2, st))) return r;
+    if (report[1] != 1) return fail(SVO_ERR_DEVICE, "builder: root level does not hold one node");
+    const uint64_t W = 1 + report[0];
 
     // placement, top-down
     {
         Level &root = lv[0];
         if ((r = ar.alloc(root.pos, 1)) || (r = ar.alloc(root.B, 1)) || (r = ar.alloc(root.far, 1))) return r;
-        const uint64_t zero = 0, one = 1;
-        const uint8_t nf = 0;
-        HIP_TRY(hipMemcpy(root.pos, &zero, 8, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(root.B, &one, 8, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(root.far, &nf, 1, hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(root_place_kernel, dim3(1), dim3(1), 0, st, root.pos, root.B, root.far);
     }
     for (int d = 0; d < D; ++d) {
         Level &p = lv[d], &c = lv[d + 1];
         const bool childLeaves = d + 1 == D;
-        if ((r = ar.alloc(c.pos, c.n))) return r;
-        if (!childLeaves && ((r = ar.alloc(c.B, c.n)) || (r = ar.alloc(c.far, c.n)))) return r;
-        if (p.n)
-            hipLaunchKernelGGL(place_kernel, dim3(grid_for(p.n)), dim3(kThreads), 0, st, p.mask, p.start, p.n, p.B,
-                               p.large, childLeaves ? nullptr : c.S, int(childLeaves), c.pos, c.B, c.far);
+        if ((r = ar.alloc(c.pos, c.bound))) return r;
+        if (!childLeaves && ((r = ar.alloc(c.B, c.bound)) || (r = ar.alloc(c.far, c.bound)))) return r;
+        if (p.bound)
+            hipLaunchKernelGGL(place_kernel, dim3(grid_for(p.bound)), dim3(kThreads), 0, st, p.mask, p.start, p.cnt,
+                               p.B, p.large, childLeaves ? nullptr : c.S, int(childLeaves), c.pos, c.B, c.far);
         HIP_TRY(hipGetLastError());
     }
 
@@ -1374,9 +1603,9 @@ int emit_tree(Arena &ar, const Geometry &g, uint64_t *leafKey, uint32_t *leafWor
     }
     for (int d = 0; d < D; ++d) {
         Level &p = lv[d];
-        if (p.n)
-            hipLaunchKernelGGL(write_nodes_kernel, dim3(grid_for(p.n)), dim3(kThreads), 0, st, p.mask, p.n, p.pos, p.B,
-                               p.large, p.far, int(d == D - 1), words, W, bad);
+        if (p.bound)
+            hipLaunchKernelGGL(write_nodes_kernel, dim3(grid_for(p.bound)), dim3(kThreads), 0, st, p.mask, p.cnt, p.pos,
+                               p.B, p.large, p.far, int(d == D - 1), words, W, bad);
     }
     if (L)
         hipLaunchKernelGGL(write_leaves_kernel, dim3(grid_for(L)), dim3(kThreads), 0, st, leafWord, L, lv[D].pos, words,
@@ -1426,22 +1655,20 @@ int finish_leaves(Arena &ar, const Geometry &g, uint64_t *key, uint32_t *word, u
         hipLaunchKernelGGL(keep_kernel, dim3(grid_for(V)), dim3(kThreads), 0, st, key, word, V, bp, int(!mesh),
                            blockFlags, hazard, keep);
     HIP_TRY(hipGetLastError());
-    if (mesh && nBlocks > 1) {
-        std::vector<uint32_t> hz(nBlocks), fl(nBlocks);
-        HIP_TRY(hipMemcpy(hz.data(), hazard, nBlocks * 4, hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(fl.data(), blockFlags, nBlocks * 4, hipMemcpyDeviceToHost));
-        int64_t last = -1;
-        for (uint64_t b = 0; b < nBlocks; ++b)
-            if (fl[b]) last = int64_t(b);
-        for (uint64_t b = 0; b < nBlocks; ++b)
-            if (hz[b] && int64_t(b) != last) {
-                if (!staleCarry) return fail(SVO_ERR_UNSUPPORTED, "unscanned voxels in a non-final cache block");
-                *staleCarry = true;  // the caller replays the block buffer (stale_buffer_leaves)
-                return SVO_OK;
-            }
+    // info[0] = leaves kept, info[1] = an unscanned voxel in a cache block that
+    // is not the last one built (mesh source), read together
+    uint64_t *info, infoH[2] = {0, 0};
+    if ((r = ar.alloc(info, 2)) || (r = fill32(reinterpret_cast<uint32_t *>(info), 4, 0, st))) return r;
+    if (mesh && nBlocks > 1)
+        hipLaunchKernelGGL(stale_blocks_kernel, dim3(1), dim3(kThreads), 0, st, hazard, blockFlags, nBlocks, info + 1);
+    if (V && (r = exclusive_scan(ar, keep, at, V, st, info))) return r;
+    if ((r = read_words(info, infoH, 2, st))) return r;
+    if (infoH[1]) {
+        if (!staleCarry) return fail(SVO_ERR_UNSUPPORTED, "unscanned voxels in a non-final cache block");
+        *staleCarry = true;  // the caller replays the block buffer (stale_buffer_leaves)
+        return SVO_OK;
     }
-    if ((r = exclusive_scan(ar, keep, at, V, st))) return r;
-    if ((r = scan_total(keep, at, V, L))) return r;
+    L = infoH[0];
     if ((r = ar.alloc(lkey, L)) || (r = ar.alloc(lword, L))) return r;
     if (V) hipLaunchKernelGGL(compact_kernel, dim3(grid_for(V)), dim3(kThreads), 0, st, key, word, V, keep, at, lkey, lword);
     HIP_TRY(hipGetLastError());
@@ -1750,17 +1977,20 @@ static int mesh_core(Arena &ar, float *dv, int64_t nverts, int has_normals, int3
         hipLaunchKernelGGL(bin_kernel, dim3(grid_for(T)), dim3(kThreads), 0, st, pos, T, mp, binCount, nullptr, nullptr,
                            nullptr, bad);
     HIP_TRY(hipGetLastError());
-    if ((r = exclusive_scan(ar, binCount, binOff, T, st))) return r;
-    uint64_t P;
-    if ((r = scan_total(binCount, binOff, T, P))) return r;
+    // the pair count and the out-of-table flag (the count pass already sets
+    // it for every offending cell) in one read
+    uint64_t *rb, binInfo[2] = {0, 0};
+    if ((r = ar.alloc(rb, 2))) return r;
+    if ((r = exclusive_scan(ar, binCount, binOff, uint64_t(T), st, rb))) return r;
+    hipLaunchKernelGGL(widen_u32_kernel, dim3(1), dim3(1), 0, st, bad, rb + 1);
+    if ((r = read_words(rb, binInfo, 2, st))) return r;
+    if (binInfo[1]) return fail(SVO_ERR_UNSUPPORTED, "a triangle bins outside the sub-block table (reference undefined behaviour)");
+    const uint64_t P = T ? binInfo[0] : 0;
     uint32_t *cells, *ptri;
     if ((r = ar.alloc(cells, P)) || (r = ar.alloc(ptri, P))) return r;
     if (T)
         hipLaunchKernelGGL(bin_kernel, dim3(grid_for(T)), dim3(kThreads), 0, st, pos, T, mp, binCount, binOff, cells, ptri,
                            bad);
-    uint32_t badH = 0;
-    HIP_TRY(hipMemcpy(&badH, bad, 4, hipMemcpyDeviceToHost));
-    if (badH) return fail(SVO_ERR_UNSUPPORTED, "a triangle bins outside the sub-block table (reference undefined behaviour)");
     ar.release(binCount);
     ar.release(binOff);
 
@@ -1770,37 +2000,37 @@ static int mesh_core(Arena &ar, float *dv, int64_t nverts, int has_normals, int3
         if (P) hipLaunchKernelGGL(block_flags_kernel, dim3(grid_for(P)), dim3(kThreads), 0, st, cells, P, mp, blockFlags);
     }
 
-    // voxel rows
+    // voxel rows; rb[0] = rows, rb[1] = voxels in the pairs' boxes (a bound
+    // of the candidates), read together
     uint32_t *rows;
     uint64_t *rowStart;
     if ((r = ar.alloc(rows, P)) || (r = ar.alloc(rowStart, P + 1))) return r;
-    if (P) hipLaunchKernelGGL(pair_rows_kernel, dim3(grid_for(P)), dim3(kThreads), 0, st, cells, ptri, P, pos, mp, rows);
+    uint64_t rowInfo[2] = {0, 0};
+    if ((r = fill32(reinterpret_cast<uint32_t *>(rb), 4, 0, st))) return r;
+    if (P)
+        hipLaunchKernelGGL(pair_rows_kernel, dim3(grid_for(P)), dim3(kThreads), 0, st, cells, ptri, P, pos, mp, rows,
+                           reinterpret_cast<unsigned long long *>(rb + 1));
     HIP_TRY(hipGetLastError());
-    if ((r = exclusive_scan(ar, rows, rowStart, P, st))) return r;
-    uint64_t R;
-    if ((r = scan_total(rows, rowStart, P, R))) return r;
-    HIP_TRY(hipMemcpy(rowStart + P, &R, 8, hipMemcpyHostToDevice));
+    if (P && (r = exclusive_scan(ar, rows, rowStart, P, st, rb))) return r;
+    if ((r = read_words(rb, rowInfo, 2, st))) return r;
+    const uint64_t R = rowInfo[0], cap = rowInfo[1];
+    hipLaunchKernelGGL(set_u64_kernel, dim3(1), dim3(1), 0, st, rowStart + P, R);
     ar.release(rows);
-    uint32_t *hitCount;
-    uint64_t *hitStart;
-    if ((r = ar.alloc(hitCount, R)) || (r = ar.alloc(hitStart, R))) return r;
+    // (voxel, triangle) keys in one SAT pass; rb[0] = candidates, rb[1] = overflow
+    uint64_t *keys, *keys2;
+    if ((r = ar.alloc(keys, cap)) || (r = ar.alloc(keys2, cap))) return r;
+    if ((r = fill32(reinterpret_cast<uint32_t *>(rb), 4, 0, st))) return r;
     if (R)
-        hipLaunchKernelGGL(row_voxels_kernel, dim3(grid_for(R)), dim3(kThreads), 0, st, cells, ptri, P, rowStart, R, pos,
-                           mp, tribits, hitCount, nullptr, nullptr);
+        hipLaunchKernelGGL(row_voxels_emit_kernel, dim3(grid_for(R)), dim3(kThreads), 0, st, cells, ptri, P, rowStart, R,
+                           pos, mp, tribits, keys, cap, reinterpret_cast<unsigned long long *>(rb),
+                           reinterpret_cast<uint32_t *>(rb + 1));
     HIP_TRY(hipGetLastError());
-    if ((r = exclusive_scan(ar, hitCount, hitStart, R, st))) return r;
-    uint64_t K;
-    if ((r = scan_total(hitCount, hitStart, R, K))) return r;
+    uint64_t keyInfo[2] = {0, 0};
+    if ((r = read_words(rb, keyInfo, 2, st))) return r;
+    if (keyInfo[1]) return fail(SVO_ERR_DEVICE, "builder: candidate keys exceed their bound (internal error)");
+    const uint64_t K = keyInfo[0];
     out.build.triangles = uint64_t(T);
     out.build.candidates = K;
-    uint64_t *keys, *keys2;
-    if ((r = ar.alloc(keys, K)) || (r = ar.alloc(keys2, K))) return r;
-    if (R)
-        hipLaunchKernelGGL(row_voxels_kernel, dim3(grid_for(R)), dim3(kThreads), 0, st, cells, ptri, P, rowStart, R, pos,
-                           mp, tribits, hitCount, hitStart, keys);
-    HIP_TRY(hipGetLastError());
-    ar.release(hitCount);
-    ar.release(hitStart);
     ar.release(cells);
     ar.release(ptri);
     ar.release(rowStart);
@@ -1809,10 +2039,10 @@ static int mesh_core(Arena &ar, float *dv, int64_t nverts, int has_normals, int3
     if (K) {
         size_t bytes = 0;
         const int endBit = 3 * g.depth + tribits;
-        HIP_TRY(hipcub::DeviceRadixSort::SortKeys(nullptr, bytes, keys, keys2, K, 0, endBit, st));
+        HIP_TRY(rocprim::radix_sort_keys(nullptr, bytes, keys, keys2, K, 0, endBit, st));
         uint8_t *tmp;
         if ((r = ar.alloc(tmp, bytes))) return r;
-        HIP_TRY(hipcub::DeviceRadixSort::SortKeys(tmp, bytes, keys, keys2, K, 0, endBit, st));
+        HIP_TRY(rocprim::radix_sort_keys(tmp, bytes, keys, keys2, K, 0, endBit, st));
         ar.release(tmp);
     }
     ar.release(keys);
@@ -1822,15 +2052,14 @@ static int mesh_core(Arena &ar, float *dv, int64_t nverts, int has_normals, int3
     uint64_t *segIdx;
     if ((r = ar.alloc(head, K)) || (r = ar.alloc(segIdx, K))) return r;
     if (K) hipLaunchKernelGGL(seg_heads_kernel, dim3(grid_for(K)), dim3(kThreads), 0, st, keys2, K, tribits, head);
-    if ((r = exclusive_scan(ar, head, segIdx, K, st))) return r;
     uint64_t V;
-    if ((r = scan_total(head, segIdx, K, V))) return r;
+    if ((r = scan_count(ar, head, segIdx, K, st, V))) return r;
     uint64_t *segStart, *vkey;
     uint32_t *vword;
     uint8_t *vcount;
     if ((r = ar.alloc(segStart, V + 1))) return r;
     if (K) hipLaunchKernelGGL(seg_starts_kernel, dim3(grid_for(K)), dim3(kThreads), 0, st, head, segIdx, K, segStart);
-    HIP_TRY(hipMemcpy(segStart + V, &K, 8, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(set_u64_kernel, dim3(1), dim3(1), 0, st, segStart + V, K);
     if (sink) {
         ar.release(head);
         ar.release(segIdx);
@@ -2088,9 +2317,8 @@ int build_grid(const uint32_t *grid, int w, int h, int d, const svo_build_opts *
     if ((r = ar.alloc(dg, N)) || (r = ar.alloc(flag, N)) || (r = ar.alloc(at, N))) return r;
     HIP_TRY(hipMemcpyAsync(dg, grid, N * 4, hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(grid_keys_kernel, dim3(grid_for(N)), dim3(kThreads), 0, st, dg, w, h, d, flag);
-    if ((r = exclusive_scan(ar, flag, at, N, st))) return r;
     uint64_t V;
-    if ((r = scan_total(flag, at, N, V))) return r;
+    if ((r = scan_count(ar, flag, at, N, st, V))) return r;
     out.build.candidates = V;
     uint64_t *key, *key2;
     uint32_t *word, *word2;
@@ -2102,10 +2330,10 @@ int build_grid(const uint32_t *grid, int w, int h, int d, const svo_build_opts *
     ar.release(at);
     if (V) {
         size_t bytes = 0;
-        HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, key, key2, word, word2, V, 0, 3 * g.depth, st));
+        HIP_TRY(rocprim::radix_sort_pairs(nullptr, bytes, key, key2, word, word2, V, 0, 3 * g.depth, st));
         uint8_t *tmp;
         if ((r = ar.alloc(tmp, bytes))) return r;
-        HIP_TRY(hipcub::DeviceRadixSort::SortPairs(tmp, bytes, key, key2, word, word2, V, 0, 3 * g.depth, st));
+        HIP_TRY(rocprim::radix_sort_pairs(tmp, bytes, key, key2, word, word2, V, 0, 3 * g.depth, st));
         ar.release(tmp);
     }
     uint32_t *blockFlags;
@@ -2164,10 +2392,10 @@ static int sdf_leaves(Arena &ar, const svo_sdf_primitive *prims, int count, int 
             if ((r = ar.alloc(k2, V)) || (r = ar.alloc(w2, V))) return r;
             if (V) {
                 size_t bytes = 0;
-                HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, k, k2, w, w2, V, 0, 3 * log2i(n), st));
+                HIP_TRY(rocprim::radix_sort_pairs(nullptr, bytes, k, k2, w, w2, V, 0, 3 * log2i(n), st));
                 uint8_t *tmp;
                 if ((r = ar.alloc(tmp, bytes))) return r;
-                HIP_TRY(hipcub::DeviceRadixSort::SortPairs(tmp, bytes, k, k2, w, w2, V, 0, 3 * log2i(n), st));
+                HIP_TRY(rocprim::radix_sort_pairs(tmp, bytes, k, k2, w, w2, V, 0, 3 * log2i(n), st));
                 ar.release(tmp);
             }
             ar.release(k);
@@ -2211,7 +2439,7 @@ static int terrain_leaves(Arena &ar, int n, uint32_t seed, uint64_t *&key, uint3
     if ((r = ar.alloc(cnt, cols)) || (r = ar.alloc(at, cols))) return r;
     hipLaunchKernelGGL(terrain_count_kernel, dim3(grid_for(cols)), dim3(kThreads), 0, st, n, H, cnt);
     HIP_TRY(hipGetLastError());
-    if ((r = exclusive_scan(ar, cnt, at, cols, st)) || (r = scan_total(cnt, at, cols, V))) return r;
+    if ((r = scan_count(ar, cnt, at, cols, st, V))) return r;
     ar.release(cnt);
     uint64_t *k;
     uint32_t *w;
@@ -2225,10 +2453,10 @@ static int terrain_leaves(Arena &ar, int n, uint32_t seed, uint64_t *&key, uint3
     uint32_t *w2;
     if ((r = ar.alloc(k2, V)) || (r = ar.alloc(w2, V))) return r;
     size_t bytes = 0;
-    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, k, k2, w, w2, V, 0, 3 * log2i(n), st));
+    HIP_TRY(rocprim::radix_sort_pairs(nullptr, bytes, k, k2, w, w2, V, 0, 3 * log2i(n), st));
     uint8_t *tmp;
     if ((r = ar.alloc(tmp, bytes))) return r;
-    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(tmp, bytes, k, k2, w, w2, V, 0, 3 * log2i(n), st));
+    HIP_TRY(rocprim::radix_sort_pairs(tmp, bytes, k, k2, w, w2, V, 0, 3 * log2i(n), st));
     ar.release(tmp);
     ar.release(k);
     ar.release(w);
