@@ -84,12 +84,23 @@ int main(int argc, char **argv) {
     std::string cmd = argv[1];
 
     if (cmd == "build-ply" && argc == 7) {
+        // the builder's in-memory mode (src/Main.cpp:322-327), each phase timed:
+        // PLY parse, voxelization + octree build, LZ4 save
         ThreadUtils::startThreads(atoi(argv[4]));
         size_t mem = size_t(strtoull(argv[5], 0, 10));
+        auto t0 = std::chrono::steady_clock::now();
         std::unique_ptr<PlyLoader> loader(new PlyLoader(argv[2]));
+        auto t1 = std::chrono::steady_clock::now();
         std::unique_ptr<VoxelData> data(new VoxelData(loader.get(), atoi(argv[3]), mem));
         std::unique_ptr<VoxelOctree> tree(new VoxelOctree(data.get()));
+        auto t2 = std::chrono::steady_clock::now();
         tree->save(argv[6]);
+        auto t3 = std::chrono::steady_clock::now();
+        auto sec = [](std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
+            return std::chrono::duration<double>(b - a).count();
+        };
+        printf("build_seconds parse %.6f build %.6f save %.6f threads %d\n", sec(t0, t1), sec(t1, t2), sec(t2, t3),
+               atoi(argv[4]));
         return 0;
     }
     if (cmd == "convert" && argc == 7) {
