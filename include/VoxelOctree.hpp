@@ -12,7 +12,7 @@
  *     VoxelOctree tree(&data);                              // Main.cpp:325 (GPU build)
  *     tree.save(out);                                       // Main.cpp:326
  *     VoxelOctree viewer(file);                             // Main.cpp:333
- *     viewer.raymarch(o, d, rayScale, normal, t);           // Main.cpp:118,181 (GPU)
+ *     viewer.raymarch(o, d, rayScale, normal, t);           // Main.cpp:118,181
  *
  * Every member forwards to the C ABI in svo.h; nothing is computed here.
  * Build inputs only record their arguments: the PLY is parsed and voxelized
@@ -135,10 +135,11 @@ public:
     /* VoxelOctree::save, src/VoxelOctree.cpp:92-123 (byte-identical .oct) */
     void save(const char *path) { svo_shim::check(svo_save(h_, path), &err_); }
 
-    /* VoxelOctree::raymarch, src/VoxelOctree.cpp:207-346. One ray per call
-     * round-trips to the GPU; prefer raymarchBatch / renderPrimary for tiles
-     * and frames. normal is written on a leaf hit, t on any hit, as in the
-     * reference. */
+    /* VoxelOctree::raymarch, src/VoxelOctree.cpp:207-346. One ray per call is
+     * walked on the calling thread by the GPU kernels' traversal source (a
+     * device round trip per ray would cost ~30 us); raymarchBatch,
+     * renderPrimary and renderFrame run tiles and frames on the GPU. normal is
+     * written on a leaf hit, t on any hit, as in the reference. */
     bool raymarch(const Vec3 &o, const Vec3 &d, float rayScale, uint32_t &normal, float &t) {
         const float of[3] = {o.x, o.y, o.z}, df[3] = {d.x, d.y, d.z};
         int hit = 0;

@@ -100,7 +100,11 @@ int svo_upload(svo_octree *h, const uint32_t **device_words);
 /* ---- raymarch ------------------------------------------------------------ */
 
 /* VoxelOctree::raymarch(o, d, rayScale, normal, t), src/VoxelOctree.cpp:207-346,
- * one ray, synchronous, on the GPU. *hit gets the hit code; normal/t are
+ * one ray, synchronous. The ray is walked on the calling thread by the same
+ * traversal source the GPU kernels run (a device launch per ray would cost
+ * tens of microseconds of upload, launch and download against well under one
+ * of walk); the batch, render and frame entry points run on the GPU. Safe
+ * from many threads at once. *hit gets the hit code; normal/t are
  * written exactly when the reference writes them (normal on a leaf hit; t on
  * any hit). For every raymarch entry point: a ray with a NaN origin or
  * direction component, or an infinite origin on an axis whose direction is
@@ -224,13 +228,17 @@ typedef struct {
     int waves_per_cu;     /* persistent form: resident waves per CU; 0 = 32 (LDS permitting) */
     int refill_threshold; /* persistent form: idle lanes of a wave (1..64) that trigger new pixels; 0 = 16 */
     int form;             /* SVO_BATCH_GRID (default) or SVO_BATCH_PERSISTENT */
-    int reserved[5];
+    int tile_w;           /* grid form: pixels per tile row, 4, 8, 16 or 32 (tile = tile_w x 64/tile_w;
+                             0 = 8, the reference's TileSize, src/Main.cpp:63); the persistent form uses 8 */
+    int reserved[4];
 } svo_batch_opts;
 
 /* A renderer owns the device view table and tile-queue counters of batched
  * launches. Launches on one renderer are ordered: a call waits on the host
- * until the previous launch of the same renderer has finished. Use one
- * renderer per concurrent stream. No counterpart in the reference (its viewer
+ * only until the previous call's small staging copies have been read, and its
+ * launch waits on the device for the previous launch of the same renderer
+ * (which may be on another stream). Use one renderer per concurrent stream to
+ * let launches overlap. No counterpart in the reference (its viewer
  * renders one frame with a CPU thread pool, src/Main.cpp:204-262,342-367). */
 typedef struct svo_renderer svo_renderer;
 int svo_renderer_create(int max_views, svo_renderer **out);
@@ -244,11 +252,12 @@ int svo_render_batch_device(svo_renderer *r, const svo_view *views, int nviews, 
                             void *stream);
 
 /* Launch-shape autotuner: times the batched launch of `views` on `stream` in
- * the grid form and in the persistent form for waves per CU {16, 24, 32} x
- * refill threshold {16, 32, 64} (one warm-up and two timed launches each; the
- * launches write the views' outputs) and stores the fastest in *best. times
- * (may be NULL) receives the 10 averages in ms: the grid form, then the
- * persistent shapes waves-major. */
+ * the grid form for tiles 8x8, 16x4, 4x16 and 32x2 and in the persistent form
+ * for waves per CU {16, 24, 32} x refill threshold {16, 32, 64} (one warm-up
+ * and two timed launches each; the launches write the views' outputs) and
+ * stores the fastest in *best. times (may be NULL) receives the 13 averages in
+ * ms: the four grid tile shapes in that order, then the persistent shapes
+ * waves-major. */
 int svo_renderer_autotune(svo_renderer *r, const svo_view *views, int nviews, void *stream, svo_batch_opts *best,
                           float *times);
 
@@ -285,14 +294,17 @@ typedef struct {
     int waves_per_cu;       /* batched launch parameters (svo_batch_opts) */
     int refill_threshold;
     int batch_form;         /* SVO_BATCH_GRID (default) or SVO_BATCH_PERSISTENT */
-    int reserved[5];
+    int device_outputs;     /* 1: outputs stay in device memory (no copy to the host;
+                               latency ends when the launch completes) */
+    int reserved[4];
 } svo_server_opts;
 
 /* Runs the request loop over a known arrival schedule (arrival_s ascending,
  * seconds from the start) with the octrees resident: requests are admitted
  * by output footprint and folded into the next launch as slots free up, with
  * launches of different slots overlapping on the device. latency_s[i] gets
- * scheduled arrival -> outputs visible in host memory; makespan_s the time
+ * scheduled arrival -> outputs visible in host memory (with device_outputs:
+ * -> the launch holding it has completed); makespan_s the time
  * from the first arrival to the last completion. Optional per-request host
  * destinations (arrays of n pointers, entries may be NULL) receive the
  * outputs. Synchronous; spins one host thread. */
@@ -300,6 +312,80 @@ int svo_serve_replay(svo_octree *const *trees, int ntrees, const svo_request *re
                      const double *arrival_s, int n, const svo_server_opts *opts, double *latency_s,
                      double *makespan_s, uint32_t *const *normal_out, float *const *t_out,
                      uint64_t *const *leaf_out);
+
+/* ---- live server: requests folded into one running persistent launch ------ */
+
+/* A request server keeps octrees resident and traces requests submitted at
+ * any time, from any thread, in one persistent launch that runs while work is
+ * in flight: a leader wave copies newly published requests from a ring in
+ * pinned host memory into device tables, the other waves claim 8x8 tiles of
+ * every published request from one queue, and a request that arrives while
+ * others are being traced is picked up by the next waves that free up.
+ * Requests are admitted in submission order while their outputs (16 B per
+ * ray: normal u32, t f32, leaf u64) fit the output arena and a ring slot is
+ * free; the others wait in the server. Results equal svo_render_primary's for
+ * the same camera and size, alone or in any mix. No counterpart in the
+ * reference (its viewer renders one frame at a time with CPU threads,
+ * src/Main.cpp:204-262,342-367). The launch ends after idle_stop_us with
+ * nothing in flight (requests waiting for arena memory do not keep it
+ * running); the next admission starts a new one. While it runs it holds
+ * waves_per_cu wave slots per CU, and a device-wide synchronisation
+ * (hipDeviceSynchronize, hipFree) waits until it has ended. */
+typedef struct svo_server svo_server;
+
+typedef struct {
+    uint64_t output_budget; /* bytes of the output arena (0 = 1 GiB) */
+    int ring_slots;         /* requests in flight at most (0 = 1024, at most 2^20) */
+    int waves_per_cu;       /* resident waves of the serving launch per CU, 4..32 (0 = 24:
+                               the other slots stay free for the caller's kernels) */
+    int host_outputs;       /* 0: outputs stay in device memory (svo_server_outputs gives
+                               device pointers); 1: pinned host memory written by the kernel */
+    int idle_stop_us;       /* the launch ends after this long with nothing in flight (0 = 2000) */
+    int reserved[6];
+} svo_server_config;
+
+enum { SVO_REQ_WAITING = 0, SVO_REQ_IN_FLIGHT = 1, SVO_REQ_DONE = 2 };
+
+typedef struct {
+    uint64_t launches;  /* persistent launches started */
+    uint64_t published; /* requests handed to the device */
+    uint64_t in_flight; /* published, not yet complete */
+    uint64_t waiting;   /* submitted, not yet admitted (arena or ring full) */
+    int live;           /* a launch is running */
+    int reserved[7];
+} svo_server_stats;
+
+/* Makes the octrees resident on the current device and starts the request
+ * loop thread. The octrees must outlive the server. */
+int svo_server_create(svo_octree *const *trees, int ntrees, const svo_server_config *cfg, svo_server **out);
+/* Queues one request (octree index into the create call's array) and returns
+ * its ticket at once. SVO_ERR_LIMIT when its outputs exceed the whole arena. */
+int svo_server_submit(svo_server *s, const svo_request *req, uint64_t *ticket);
+/* State (SVO_REQ_*) of a ticket; latency_s (may be NULL) = submit -> completion
+ * seen by the request loop, once done. */
+int svo_server_poll(svo_server *s, uint64_t ticket, int *state, double *latency_s);
+/* Blocks until the ticket is done (timeout_s <= 0: no limit; SVO_ERR_LIMIT on timeout). */
+int svo_server_wait(svo_server *s, uint64_t ticket, double timeout_s, double *latency_s);
+/* Output arrays of a completed request (width*height each, row-major): device
+ * pointers, or host pointers with host_outputs. Valid until svo_server_release. */
+int svo_server_outputs(svo_server *s, uint64_t ticket, uint32_t **normal, float **t, uint64_t **leaf);
+/* Copies a completed request's outputs (any pointer may be NULL) into host
+ * arrays of width*height entries; synchronous. */
+int svo_server_copy_outputs(svo_server *s, uint64_t ticket, uint32_t *normal, float *t, uint64_t *leaf);
+/* Frees a completed (or still waiting) request's outputs and its ticket. */
+int svo_server_release(svo_server *s, uint64_t ticket);
+int svo_server_stats_get(svo_server *s, svo_server_stats *out);
+/* Drives a server with a request schedule: request i is submitted at
+ * arrival_s[i] seconds after the call starts (ascending), completions are
+ * recorded as they come; latency_s[i] = completion seen - scheduled arrival.
+ * With tickets non-NULL the requests are kept (tickets[i] is request i's; the
+ * caller reads and releases them, and the arena must hold every output);
+ * otherwise each request is released once complete. makespan_s (may be NULL)
+ * = first arrival -> last completion. */
+int svo_server_run_schedule(svo_server *s, const svo_request *requests, const double *arrival_s, int n,
+                            uint64_t *tickets, double *latency_s, double *makespan_s);
+/* Ends the launch (finishing requests in flight) and frees the server. */
+void svo_server_destroy(svo_server *s);
 
 /* ---- construction --------------------------------------------------------- */
 

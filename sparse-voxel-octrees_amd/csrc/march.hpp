@@ -5,6 +5,11 @@
 // (:216-250), ray_step one pass of the loop body (:252-338). Splitting the
 // loop into resumable steps lets a persistent wave hand a finished lane a new
 // ray while the other lanes keep walking.
+//
+// The same source also compiles for the host (single-ray calls, where a
+// device round trip costs far more than the walk itself): every operation is
+// a correctly rounded FP32 add, multiply or divide, a comparison or integer
+// bit work, so host and device give the same bits.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -14,15 +19,26 @@
 namespace svo {
 namespace march {
 
+#define SVO_HD __host__ __device__ __forceinline__
+
 constexpr int kMaxScale = 23;  // VoxelOctree::MaxScale, src/VoxelOctree.hpp:38
 
-__device__ __forceinline__ float smin(float a, float b) { return (b < a) ? b : a; }  // std::min
-__device__ __forceinline__ float smax(float a, float b) { return (a < b) ? b : a; }  // std::max
-__device__ __forceinline__ uint32_t f2u(float f) { return __float_as_uint(f); }
-__device__ __forceinline__ float u2f(uint32_t u) { return __uint_as_float(u); }
+SVO_HD float smin(float a, float b) { return (b < a) ? b : a; }  // std::min
+SVO_HD float smax(float a, float b) { return (a < b) ? b : a; }  // std::max
+SVO_HD uint32_t f2u(float f) { return __builtin_bit_cast(uint32_t, f); }
+SVO_HD float u2f(uint32_t u) { return __builtin_bit_cast(float, u); }
+SVO_HD int popc32(uint32_t x) { return __builtin_popcount(x); }
+// bit 0 of x (a v_bfe_u32 on the device)
+SVO_HD uint32_t bit0(uint32_t x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_ubfe(x, 0, 1);
+#else
+    return x & 1u;
+#endif
+}
 
 // invSqrt, src/Util.hpp:47-58
-__device__ __forceinline__ float quake_rsqrt(float x) {
+SVO_HD float quake_rsqrt(float x) {
     float half = x * 0.5f;
     float y = u2f(0x5f3759dfu - (f2u(x) >> 1));
     return y * (1.5f - half * y * y);
@@ -48,15 +64,15 @@ struct Stack {
     int stride;
     int slots;
     int lane;
-    __device__ __forceinline__ Stack(void *base, int stride_, int slots_, int lane_)
+    SVO_HD Stack(void *base, int stride_, int slots_, int lane_)
         : off(reinterpret_cast<Idx *>(base)),
           tmax(reinterpret_cast<float *>(reinterpret_cast<Idx *>(base) + size_t(slots_) * stride_)),
           stride(stride_), slots(slots_), lane(lane_) {}
-    __device__ __forceinline__ void push(int slot, Idx parent, float t) const {
+    SVO_HD void push(int slot, Idx parent, float t) const {
         off[slot * stride + lane] = parent;
         tmax[slot * stride + lane] = t;
     }
-    __device__ __forceinline__ void pop(int slot, Idx &parent, float &t) const {
+    SVO_HD void pop(int slot, Idx &parent, float &t) const {
         parent = off[slot * stride + lane];
         t = tmax[slot * stride + lane];
     }
@@ -69,15 +85,13 @@ struct Stack<uint32_t> {
     uint2 *e;
     int stride;
     int slots;
-    __device__ __forceinline__ Stack(void *base, int stride_, int slots_, int lane_)
+    SVO_HD Stack(void *base, int stride_, int slots_, int lane_)
         : e(reinterpret_cast<uint2 *>(base) + lane_), stride(stride_), slots(slots_) {}
-    __device__ __forceinline__ void push(int slot, uint32_t parent, float t) const {
-        e[slot * stride] = make_uint2(parent, __float_as_uint(t));
-    }
-    __device__ __forceinline__ void pop(int slot, uint32_t &parent, float &t) const {
+    SVO_HD void push(int slot, uint32_t parent, float t) const { e[slot * stride] = make_uint2(parent, f2u(t)); }
+    SVO_HD void pop(int slot, uint32_t &parent, float &t) const {
         const uint2 v = e[slot * stride];
         parent = v.x;
-        t = __uint_as_float(v.y);
+        t = u2f(v.y);
     }
 };
 
@@ -98,8 +112,8 @@ enum : int { kRunning = -1, kMiss = 0, kLeaf = 1, kLod = 2 };
 // Set-up before the loop (:216-250). The root descriptor is read here, the
 // read the first loop pass makes (:254).
 template <typename Idx>
-__device__ __forceinline__ void ray_init(Ray<Idx> &r, const uint32_t *__restrict__ tree, float ox, float oy, float oz,
-                                         float dx, float dy, float dz, float rayScale) {
+SVO_HD void ray_init(Ray<Idx> &r, const uint32_t *__restrict__ tree, float ox, float oy, float oz, float dx, float dy,
+                     float dz, float rayScale) {
     if (fabsf(dx) < 1e-4f) dx = 1e-4f;
     if (fabsf(dy) < 1e-4f) dy = 1e-4f;
     if (fabsf(dz) < 1e-4f) dz = 1e-4f;
@@ -139,7 +153,7 @@ __device__ __forceinline__ void ray_init(Ray<Idx> &r, const uint32_t *__restrict
 // time equals at least one corner time), each node is entered at most once and
 // the walk ends.
 template <typename Idx>
-__device__ __forceinline__ bool ray_defined(const Ray<Idx> &r) {
+SVO_HD bool ray_defined(const Ray<Idx> &r) {
     return r.dTx == r.dTx && r.dTy == r.dTy && r.dTz == r.dTz && r.bTx == r.bTx && r.bTy == r.bTy &&
            r.bTz == r.bTz;
 }
@@ -162,7 +176,7 @@ __device__ __forceinline__ bool ray_defined(const Ray<Idx> &r) {
 //    the three corner times does not depend on the order of ties or on NaN
 //    handling and one hardware min3 returns the reference's std::min result.
 template <bool Lod, bool Finite, typename Idx, bool Normal = true, int Reg = 1>
-__device__ __forceinline__ int ray_step(Ray<Idx> &r, const uint32_t *__restrict__ tree, const Stack<Idx> &stk,
+SVO_HD int ray_step(Ray<Idx> &r, const uint32_t *__restrict__ tree, const Stack<Idx> &stk,
                                         uint32_t &normal, float &tOut, Idx &leaf) {
     const float cornerTX = r.posX * r.dTx - r.bTx;
     const float cornerTY = r.posY * r.dTy - r.bTy;
@@ -190,7 +204,7 @@ __device__ __forceinline__ int ray_step(Ray<Idx> &r, const uint32_t *__restrict_
             if (!(childMasks & 0x80u)) {
                 if (Normal) {
                     leaf = childOffset + r.parent +
-                           Idx(__popc((((childMasks >> (8 + childShift)) << childShift) & 127u)));
+                           Idx(popc32((((childMasks >> (8 + childShift)) << childShift) & 127u)));
                     normal = tree[leaf];
                 }
                 tOut = r.minT;
@@ -212,7 +226,7 @@ __device__ __forceinline__ int ray_step(Ray<Idx> &r, const uint32_t *__restrict_
                 r.maxT2 = r.maxT;
             }
 
-            const Idx siblingCount = Idx(__popc(childMasks & 127u));
+            const Idx siblingCount = Idx(popc32(childMasks & 127u));
             r.parent += childOffset + siblingCount;
             if (r.current & 0x10000u) r.parent += siblingCount;
             r.current = tree[r.parent];
@@ -274,8 +288,7 @@ __device__ __forceinline__ int ray_step(Ray<Idx> &r, const uint32_t *__restrict_
         r.posX = u2f(shX << r.scale);
         r.posY = u2f(shY << r.scale);
         r.posZ = u2f(shZ << r.scale);
-        r.idx = int(__builtin_amdgcn_ubfe(shX, 0, 1) | (__builtin_amdgcn_ubfe(shY, 0, 1) << 1) |
-                    (__builtin_amdgcn_ubfe(shZ, 0, 1) << 2));
+        r.idx = int(bit0(shX) | (bit0(shY) << 1) | (bit0(shZ) << 2));
     }
     return kRunning;
 }

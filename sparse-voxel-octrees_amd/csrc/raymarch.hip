@@ -165,7 +165,34 @@ __global__ void __launch_bounds__(64) primary_shadow_kernel(const uint32_t *__re
 
 int stack_slots(const svo_octree &h, int reg) { return lds_slots(h.depth, reg); }
 
+template <typename Idx>
+int walk_one(const svo_octree &h, const float o[3], const float d[3], float rs, uint32_t &normal, float &t,
+             uint64_t &leaf) {
+    const uint32_t *tree = h.words.data();
+    uint64_t buf[48];  // the restart stack: at most 21 entries of (index, maxT)
+    const Stack<Idx> stk(buf, 1, stack_slots(h, 1), 0);
+    Ray<Idx> r;
+    ray_init(r, tree, o[0], o[1], o[2], d[0], d[1], d[2], rs);
+    if (!ray_defined(r)) return kMiss;  // the reference never ends such a ray
+    Idx lf = 0;
+    int code;
+    do {
+        code = ray_step<true, false, Idx, true, 1>(r, tree, stk, normal, t, lf);
+    } while (code == kRunning);
+    leaf = uint64_t(lf);
+    return code;
+}
+
 }  // namespace
+
+// One ray walked on the calling host thread by the same traversal source as
+// the kernels (march.hpp): hit code 0/1/2, normal and leaf on a leaf hit, t on
+// any hit.
+int raymarch_one(const svo_octree &h, const float o[3], const float d[3], float rs, uint32_t &normal, float &t,
+                 uint64_t &leaf) {
+    if (h.words.size() < (uint64_t(1) << 32)) return walk_one<uint32_t>(h, o, d, rs, normal, t, leaf);
+    return walk_one<uint64_t>(h, o, d, rs, normal, t, leaf);
+}
 
 int device_count() {
     int n = 0;

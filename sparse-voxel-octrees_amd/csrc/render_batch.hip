@@ -33,6 +33,7 @@
 
 #include "march.hpp"
 #include "svo_internal.hpp"
+#include "view_desc.hpp"
 
 namespace svo {
 
@@ -47,22 +48,9 @@ namespace {
 
 using namespace march;
 
-constexpr int kQueues = 8;             // one tile-queue chunk per XCD
-constexpr int kCounterStride = 64;     // 256 B between queue counters
 constexpr uint64_t kNoTile = ~uint64_t(0);
-
-struct ViewDesc {
-    const uint32_t *tree;
-    uint8_t *hit;
-    uint32_t *normal;
-    float *t;
-    uint64_t *leaf;
-    float a11, a12, a21, a22, a31, a32;  // camera rotation columns used by ray generation
-    float zx, zy, zz;                    // planeDist * third column
-    float px, py, pz;                    // ray origin
-    float aspect, scale, rayScale;       // H/W, 2/W, LOD scale
-    int width, height, tilesX, tilesY, blocksX, pad;
-};
+constexpr int kTileShapes = 4;                      // grid-form tile shapes the autotuner times
+constexpr int kAutotuneShapes = kTileShapes + 9;    // plus 3 x 3 persistent shapes
 
 __device__ __forceinline__ int view_of(const uint64_t *__restrict__ tileBase, int nviews, uint64_t tile) {
     int lo = 0, hi = nviews;  // tileBase[lo] <= tile < tileBase[hi]
@@ -148,14 +136,9 @@ __global__ void __launch_bounds__(256) batch_kernel(const ViewDesc *__restrict__
                     const int x = tx * 8 + (p & 7);
                     const int y = ty * 8 + (p >> 3);
                     if (x < vd.width && y < vd.height) {
-                        // ray generation, src/Main.cpp:97-113 with closed-form dx, dy
-                        const float dy = vd.aspect - float(y) * vd.scale;
-                        const float dx = -1.0f + float(x) * vd.scale;
-                        const float vx = dx * vd.a11 + dy * vd.a12 + vd.zx;
-                        const float vy = dx * vd.a21 + dy * vd.a22 + vd.zy;
-                        const float vz = dx * vd.a31 + dy * vd.a32 + vd.zz;
-                        const float s = quake_rsqrt(vx * vx + vy * vy + vz * vz);
-                        ray_init(r, vd.tree, vd.px, vd.py, vd.pz, vx * s, vy * s, vz * s, vd.rayScale);
+                        float dx, dy, dz;
+                        view_ray_dir(vd, x, y, dx, dy, dz);
+                        ray_init(r, vd.tree, vd.px, vd.py, vd.pz, dx, dy, dz, vd.rayScale);
                         tree = vd.tree;
                         rView = tView;
                         rPix = int64_t(y) * vd.width + x;
@@ -180,11 +163,7 @@ __global__ void __launch_bounds__(256) batch_kernel(const ViewDesc *__restrict__
                 Idx leaf = 0;
                 int code = ray_step<Lod, true, Idx, true, Reg>(r, tree, stk, normal, t, leaf);
                 if (code != kRunning) {
-                    const ViewDesc &vd = views[rView];
-                    if (vd.hit) vd.hit[rPix] = uint8_t(code);
-                    if (vd.normal) vd.normal[rPix] = code == kLeaf ? normal : 0u;
-                    if (vd.t) vd.t[rPix] = code == kMiss ? 0.0f : t;
-                    if (vd.leaf) vd.leaf[rPix] = code == kLeaf ? uint64_t(leaf) : SVO_LEAF_NONE;
+                    view_store<Idx>(views[rView], rPix, code, normal, t, leaf);
                     active = false;
                 }
             }
@@ -201,11 +180,13 @@ __global__ void __launch_bounds__(256) batch_kernel(const ViewDesc *__restrict__
 // launch ends with a tail where a few slow tiles (rays that graze the
 // surface) run on an otherwise idle GPU; with many views in one launch that
 // tail is paid once. `first` offsets blockIdx for grids split into several
-// launches.
+// launches. A tile is 2^twLog pixels wide and 64 / 2^twLog high (8x8 by
+// default; 16x4, 4x16 and 32x2 are the autotuner's other shapes); the views'
+// tilesX / tilesY are counted in that shape.
 template <bool Lod, typename Idx, int Reg>
 __global__ void __launch_bounds__(64) tiles_kernel(const ViewDesc *__restrict__ views,
                                                    const uint64_t *__restrict__ tileBase, int nviews, uint64_t first,
-                                                   int slots) {
+                                                   int slots, int twLog) {
     extern __shared__ __align__(16) unsigned char lds[];
     const int lane = threadIdx.x;
     const uint64_t tile = first + blockIdx.x;
@@ -214,19 +195,14 @@ __global__ void __launch_bounds__(64) tiles_kernel(const ViewDesc *__restrict__ 
     const uint64_t local = tile - tileBase[v];
     const int tx = int(local % uint64_t(vd.tilesX));
     const int ty = int(local / uint64_t(vd.tilesX));
-    const int x = tx * 8 + (lane & 7);
-    const int y = ty * 8 + (lane >> 3);
+    const int x = (tx << twLog) + (lane & ((1 << twLog) - 1));
+    const int y = (ty << (6 - twLog)) + (lane >> twLog);
     if (tx >= vd.tilesX || ty >= vd.tilesY || x >= vd.width || y >= vd.height) return;
-    // ray generation, src/Main.cpp:97-113 with closed-form dx, dy
-    const float dy = vd.aspect - float(y) * vd.scale;
-    const float dx = -1.0f + float(x) * vd.scale;
-    const float vx = dx * vd.a11 + dy * vd.a12 + vd.zx;
-    const float vy = dx * vd.a21 + dy * vd.a22 + vd.zy;
-    const float vz = dx * vd.a31 + dy * vd.a32 + vd.zz;
-    const float s = quake_rsqrt(vx * vx + vy * vy + vz * vz);
+    float dx, dy, dz;
+    view_ray_dir(vd, x, y, dx, dy, dz);
     Stack<Idx> stk(lds, 64, slots, lane);
     Ray<Idx> r;
-    ray_init(r, vd.tree, vd.px, vd.py, vd.pz, vx * s, vy * s, vz * s, vd.rayScale);
+    ray_init(r, vd.tree, vd.px, vd.py, vd.pz, dx, dy, dz, vd.rayScale);
     uint32_t normal = 0;
     float t = 0.0f;
     Idx leaf = 0;
@@ -234,11 +210,7 @@ __global__ void __launch_bounds__(64) tiles_kernel(const ViewDesc *__restrict__ 
     do {
         code = ray_step<Lod, true, Idx, true, Reg>(r, vd.tree, stk, normal, t, leaf);
     } while (code == kRunning);
-    const int64_t i = int64_t(y) * vd.width + x;
-    if (vd.hit) vd.hit[i] = uint8_t(code);
-    if (vd.normal) vd.normal[i] = code == kLeaf ? normal : 0u;
-    if (vd.t) vd.t[i] = code == kMiss ? 0.0f : t;
-    if (vd.leaf) vd.leaf[i] = code == kLeaf ? uint64_t(leaf) : SVO_LEAF_NONE;
+    view_store<Idx>(vd, int64_t(y) * vd.width + x, code, normal, t, leaf);
 }
 
 __global__ void zero_counters_kernel(uint32_t *c) {
@@ -256,7 +228,8 @@ struct svo_renderer {
     uint32_t *dCounters = nullptr;
     svo::ViewDesc *hViews = nullptr;  // pinned staging
     uint64_t *hTileBase = nullptr;  // pinned staging
-    hipEvent_t done = nullptr;      // last launch finished reading the staging copies
+    hipEvent_t staged = nullptr;    // the last call's staging copies have been read
+    hipEvent_t done = nullptr;      // the last call's launch has finished with the device tables
     int cus = 0;
     std::mutex mu;
 };
@@ -273,6 +246,7 @@ int renderer_create(int maxViews, svo_renderer **out) {
     if (e == hipSuccess) e = hipMalloc(&r->dCounters, sizeof(uint32_t) * kQueues * kCounterStride);
     if (e == hipSuccess) e = hipHostMalloc(&r->hViews, sizeof(ViewDesc) * maxViews, hipHostMallocDefault);
     if (e == hipSuccess) e = hipHostMalloc(&r->hTileBase, sizeof(uint64_t) * (maxViews + 1), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&r->staged, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&r->done, hipEventDisableTiming);
     int dev = 0;
     hipDeviceProp_t prop;
@@ -293,6 +267,7 @@ void renderer_free(svo_renderer *r) {
         (void)hipEventSynchronize(r->done);
         (void)hipEventDestroy(r->done);
     }
+    if (r->staged) (void)hipEventDestroy(r->staged);
     if (r->dViews) (void)hipFree(r->dViews);
     if (r->dTileBase) (void)hipFree(r->dTileBase);
     if (r->dCounters) (void)hipFree(r->dCounters);
@@ -301,70 +276,95 @@ void renderer_free(svo_renderer *r) {
     delete r;
 }
 
+int fill_view_desc(const svo_view &v, ViewDesc &d) {
+    if (!v.tree || !v.tree->dwords) return fail(SVO_ERR_INVALID, "view without a resident octree");
+    if (v.width <= 0 || v.height <= 0 || v.width > 65536 || v.height > 65536)
+        return fail(SVO_ERR_INVALID, "view size out of range");
+    if (!camera_ok(v.camera)) return fail(SVO_ERR_INVALID, "camera entries must be finite and within +-1e6");
+    if (!(std::fabs(v.ray_scale) <= 1e6f)) return fail(SVO_ERR_INVALID, "ray_scale must be finite");
+    const float pi = float(3.14159265358979323846);
+    const float planeDist = 1.0f / std::tan(pi / 6.0f);  // src/Main.cpp:158
+    d.tree = v.tree->dwords;
+    d.hit = v.hit;
+    d.normal = v.normal;
+    d.t = v.t;
+    d.leaf = v.leaf;
+    const float *tf = v.camera.tform;
+    d.a11 = tf[0]; d.a12 = tf[1]; d.a21 = tf[4]; d.a22 = tf[5]; d.a31 = tf[8]; d.a32 = tf[9];
+    d.zx = planeDist * tf[2];
+    d.zy = planeDist * tf[6];
+    d.zz = planeDist * tf[10];
+    d.px = v.camera.pos[0]; d.py = v.camera.pos[1]; d.pz = v.camera.pos[2];
+    d.aspect = float(v.height) / float(v.width);
+    d.scale = 2.0f / float(v.width);
+    d.rayScale = v.ray_scale;
+    d.width = v.width;
+    d.height = v.height;
+    d.tilesX = (v.width + 7) / 8;
+    d.tilesY = (v.height + 7) / 8;
+    d.blocksX = (d.tilesX + 7) / 8;
+    d.pad = 0;
+    return SVO_OK;
+}
+
 namespace {
 
-// Everything one batched launch needs besides the device view table.
-struct LaunchPlan {
-    int nviews = 0;
-    uint64_t tiles = 0;
-    int slots = 1;
-    int reg = 1;  // stack entries held in registers
-    bool wide = false, lod = false;
-    size_t ldsPerBlock = 0;
-    unsigned grid = 0;
-    int refill = 16;
-    bool persistent = false;  // SVO_BATCH_PERSISTENT; otherwise one workgroup per tile
-    size_t ldsPerWave = 0;
-};
+// log2 of a grid-form tile width (0 = 8); -1 for a width that is not 4, 8, 16 or 32.
+int tile_width_log(int w) {
+    switch (w) {
+        case 0:
+        case 8: return 3;
+        case 4: return 2;
+        case 16: return 4;
+        case 32: return 5;
+        default: return -1;
+    }
+}
 
-// Validates the views and fills the host view table, the tile bases and the launch shape.
+}  // namespace
+
+// Validates the views and fills the host view table, the tile bases and the
+// launch shape. upload: octrees not yet resident are uploaded (otherwise a
+// view of an octree not resident on the current device is refused).
 int plan_views(const svo_view *views, int nviews, const svo_batch_opts *opts, int cus, ViewDesc *hv, uint64_t *htb,
-               LaunchPlan &plan) {
+               LaunchPlan &plan, bool upload) {
     int maxDepth = 1;
     bool wide = false, lod = false;
     uint64_t tiles = 0;
-    const float pi = float(3.14159265358979323846);
-    const float planeDist = 1.0f / std::tan(pi / 6.0f);  // src/Main.cpp:158
+    const bool persistentForm = opts && opts->form == SVO_BATCH_PERSISTENT;
+    const int twLog = tile_width_log(opts ? opts->tile_w : 0);
+    if (twLog < 0) return fail(SVO_ERR_INVALID, "tile_w must be 4, 8, 16 or 32");
+    if (persistentForm && twLog != 3) return fail(SVO_ERR_INVALID, "the persistent form traces 8x8 tiles");
     for (int i = 0; i < nviews; ++i) {
         const svo_view &v = views[i];
         if (!v.tree) return fail(SVO_ERR_INVALID, "view without an octree");
-        if (v.width <= 0 || v.height <= 0 || v.width > 65536 || v.height > 65536)
-            return fail(SVO_ERR_INVALID, "view size out of range");
-        if (!camera_ok(v.camera)) return fail(SVO_ERR_INVALID, "camera entries must be finite and within +-1e6");
-        if (!(std::fabs(v.ray_scale) <= 1e6f)) return fail(SVO_ERR_INVALID, "ray_scale must be finite");
-        lod = lod || v.ray_scale != 0.0f;
-        int r = ensure_device(*v.tree);
+        int r = SVO_OK;
+        if (upload) {
+            r = ensure_device(*v.tree);
+        } else {
+            int dev = -1;
+            if (hipGetDevice(&dev) != hipSuccess || !v.tree->dwords || v.tree->device != dev)
+                r = fail(SVO_ERR_INVALID, "view of an octree not resident on this device (svo_upload it first)");
+        }
         if (r) return r;
+        ViewDesc &d = hv[i];
+        r = fill_view_desc(v, d);
+        if (r) return r;
+        lod = lod || v.ray_scale != 0.0f;
         maxDepth = std::max(maxDepth, v.tree->depth);
         wide = wide || v.tree->words.size() >= (uint64_t(1) << 32);
-        ViewDesc &d = hv[i];
-        d.tree = v.tree->dwords;
-        d.hit = v.hit;
-        d.normal = v.normal;
-        d.t = v.t;
-        d.leaf = v.leaf;
-        const float *tf = v.camera.tform;
-        d.a11 = tf[0]; d.a12 = tf[1]; d.a21 = tf[4]; d.a22 = tf[5]; d.a31 = tf[8]; d.a32 = tf[9];
-        d.zx = planeDist * tf[2];
-        d.zy = planeDist * tf[6];
-        d.zz = planeDist * tf[10];
-        d.px = v.camera.pos[0]; d.py = v.camera.pos[1]; d.pz = v.camera.pos[2];
-        d.aspect = float(v.height) / float(v.width);
-        d.scale = 2.0f / float(v.width);
-        d.rayScale = v.ray_scale;
-        d.width = v.width;
-        d.height = v.height;
-        d.tilesX = (v.width + 7) / 8;
-        d.tilesY = (v.height + 7) / 8;
-        d.blocksX = (d.tilesX + 7) / 8;
-        d.pad = 0;
         const int blocksY = (d.tilesY + 7) / 8;
+        if (!persistentForm) {  // tiles of the chosen shape
+            const int tw = 1 << twLog, th = 64 >> twLog;
+            d.tilesX = (v.width + tw - 1) / tw;
+            d.tilesY = (v.height + th - 1) / th;
+        }
         htb[i] = tiles;
         // persistent form: Morton order in 64x64-pixel blocks (padded); grid
         // form: raster order (1% faster there, measured on 8 views at 1080p)
-        tiles += (opts && opts->form == SVO_BATCH_PERSISTENT) ? uint64_t(d.blocksX) * uint64_t(blocksY) * 64
-                                                               : uint64_t(d.tilesX) * uint64_t(d.tilesY);
+        tiles += persistentForm ? uint64_t(d.blocksX) * uint64_t(blocksY) * 64 : uint64_t(d.tilesX) * uint64_t(d.tilesY);
     }
+    plan.twLog = twLog;
     htb[nviews] = tiles;
     if (tiles >= (uint64_t(kQueues) << 32)) return fail(SVO_ERR_LIMIT, "too many tiles for one launch");
     plan.nviews = nviews;
@@ -390,6 +390,8 @@ int plan_views(const svo_view *views, int nviews, const svo_batch_opts *opts, in
     plan.grid = unsigned(std::max<uint64_t>(kQueues, std::min<uint64_t>(uint64_t(cus) * blocksPerCU, workBlocks)));
     return SVO_OK;
 }
+
+namespace {
 
 // Queue reset and the persistent kernel (no memset: the counters are cleared
 // by a kernel, so the pair can be captured in a graph).
@@ -418,18 +420,20 @@ void launch_tiles(const LaunchPlan &p, const ViewDesc *dViews, const uint64_t *d
         const dim3 grid(unsigned(std::min(maxGrid, p.tiles - first)));
         if (!p.wide && !p.lod)
             hipLaunchKernelGGL((tiles_kernel<false, uint32_t, Reg>), grid, dim3(64), p.ldsPerWave, s, dViews,
-                               dTileBase, p.nviews, first, p.slots);
+                               dTileBase, p.nviews, first, p.slots, p.twLog);
         else if (!p.wide)
             hipLaunchKernelGGL((tiles_kernel<true, uint32_t, Reg>), grid, dim3(64), p.ldsPerWave, s, dViews,
-                               dTileBase, p.nviews, first, p.slots);
+                               dTileBase, p.nviews, first, p.slots, p.twLog);
         else if (!p.lod)
             hipLaunchKernelGGL((tiles_kernel<false, uint64_t, Reg>), grid, dim3(64), p.ldsPerWave, s, dViews,
-                               dTileBase, p.nviews, first, p.slots);
+                               dTileBase, p.nviews, first, p.slots, p.twLog);
         else
             hipLaunchKernelGGL((tiles_kernel<true, uint64_t, Reg>), grid, dim3(64), p.ldsPerWave, s, dViews,
-                               dTileBase, p.nviews, first, p.slots);
+                               dTileBase, p.nviews, first, p.slots, p.twLog);
     }
 }
+
+}  // namespace
 
 void launch_plan(const LaunchPlan &p, const ViewDesc *dViews, const uint64_t *dTileBase, uint32_t *dCounters,
                  hipStream_t s) {
@@ -443,20 +447,23 @@ void launch_plan(const LaunchPlan &p, const ViewDesc *dViews, const uint64_t *dT
     else launch_batch<1>(p, dViews, dTileBase, dCounters, s);
 }
 
-}  // namespace
 
 int render_batch(svo_renderer *rd, const svo_view *views, int nviews, const svo_batch_opts *opts, void *stream) {
     if (!rd || !views) return fail(SVO_ERR_INVALID, "null argument");
     if (nviews <= 0 || nviews > rd->maxViews) return fail(SVO_ERR_LIMIT, "view count exceeds the renderer's max_views");
     std::lock_guard<std::mutex> lock(rd->mu);
-    // the previous launch must be done with the staging buffers before they are rewritten
-    HIP_TRY(hipEventSynchronize(rd->done));
+    // the host waits only until the previous call's copies have read the
+    // pinned staging buffers; the device tables are protected on the device:
+    // this call's stream waits for the previous launch (a no-op on one stream)
+    HIP_TRY(hipEventSynchronize(rd->staged));
     LaunchPlan plan;
-    int r = plan_views(views, nviews, opts, rd->cus, rd->hViews, rd->hTileBase, plan);
+    int r = plan_views(views, nviews, opts, rd->cus, rd->hViews, rd->hTileBase, plan, true);
     if (r) return r;
     hipStream_t s = static_cast<hipStream_t>(stream);
+    HIP_TRY(hipStreamWaitEvent(s, rd->done, 0));
     HIP_TRY(hipMemcpyAsync(rd->dViews, rd->hViews, sizeof(ViewDesc) * nviews, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(rd->dTileBase, rd->hTileBase, sizeof(uint64_t) * (nviews + 1), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipEventRecord(rd->staged, s));
     launch_plan(plan, rd->dViews, rd->dTileBase, rd->dCounters, s);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(rd->done, s));
@@ -466,7 +473,7 @@ int render_batch(svo_renderer *rd, const svo_view *views, int nviews, const svo_
 int renderer_autotune(svo_renderer *rd, const svo_view *views, int nviews, void *stream, svo_batch_opts *best,
                       float *times) {
     if (!rd || !views || !best) return fail(SVO_ERR_INVALID, "null argument");
-    static const int kWaves[3] = {16, 24, 32}, kRefill[3] = {16, 32, 64};
+    static const int kTileW[kTileShapes] = {8, 16, 4, 32}, kWaves[3] = {16, 24, 32}, kRefill[3] = {16, 32, 64};
     hipStream_t s = static_cast<hipStream_t>(stream);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     HIP_TRY(hipEventCreate(&e0));
@@ -477,14 +484,16 @@ int renderer_autotune(svo_renderer *rd, const svo_view *views, int nviews, void 
     }
     float bestMs = 0.0f;
     int r = SVO_OK;
-    for (int i = 0; i < 10 && !r; ++i) {
+    for (int i = 0; i < kAutotuneShapes && !r; ++i) {
         svo_batch_opts o{};
-        if (i == 0) {
+        if (i < kTileShapes) {
             o.form = SVO_BATCH_GRID;
+            o.tile_w = kTileW[i];
         } else {
+            const int k = i - kTileShapes;
             o.form = SVO_BATCH_PERSISTENT;
-            o.waves_per_cu = kWaves[(i - 1) / 3];
-            o.refill_threshold = kRefill[(i - 1) % 3];
+            o.waves_per_cu = kWaves[k / 3];
+            o.refill_threshold = kRefill[k % 3];
         }
         r = render_batch(rd, views, nviews, &o, stream);  // warm-up
         if (!r && (he = hipEventRecord(e0, s)) != hipSuccess) r = fail(SVO_ERR_DEVICE, hipGetErrorString(he));
@@ -538,7 +547,7 @@ int batch_graph_create(svo_renderer *rd, const svo_view *views, int nviews, cons
     std::vector<ViewDesc> hv(static_cast<size_t>(nviews));
     std::vector<uint64_t> htb(static_cast<size_t>(nviews) + 1);
     LaunchPlan plan;
-    int r = plan_views(views, nviews, opts, rd->cus, hv.data(), htb.data(), plan);
+    int r = plan_views(views, nviews, opts, rd->cus, hv.data(), htb.data(), plan, true);
     if (r) return r;
     svo_batch_graph *g = new (std::nothrow) svo_batch_graph();
     if (!g) return fail(SVO_ERR_NOMEM, "out of host memory");

@@ -80,6 +80,7 @@ int serve_replay(svo_octree *const *trees, int ntrees, const svo_request *reqs, 
     const uint64_t budget = (opts && opts->output_budget) ? opts->output_budget : (uint64_t(1) << 30);
     const int maxBatch = (opts && opts->max_batch > 0) ? opts->max_batch : 256;
     const bool single = opts && opts->one_per_launch;
+    const bool onDevice = opts && opts->device_outputs;
     svo_batch_opts bo{};
     if (opts) {
         bo.waves_per_cu = opts->waves_per_cu;
@@ -160,6 +161,7 @@ int serve_replay(svo_octree *const *trees, int ntrees, const svo_request *reqs, 
                 const uint64_t m = uint64_t(reqs[id].width) * reqs[id].height;
                 const uint8_t *base = s.host + s.offs[k];
                 auto up = [](uint64_t b) { return (b + kAlign - 1) / kAlign * kAlign; };
+                if (onDevice) continue;
                 if (normalOut && normalOut[id]) std::memcpy(normalOut[id], base, m * 4);
                 if (tOut && tOut[id]) std::memcpy(tOut[id], base + up(m * 4), m * 4);
                 if (leafOut && leafOut[id]) std::memcpy(leafOut[id], base + 2 * up(m * 4), m * 8);
@@ -206,7 +208,7 @@ int serve_replay(svo_octree *const *trees, int ntrees, const svo_request *reqs, 
             else
                 r = render_batch(s.renderer, views.data(), int(views.size()), &bo, s.stream);
             if (r) return r;
-            hipError_t e = hipMemcpyAsync(s.host, s.dev, s.used, hipMemcpyDeviceToHost, s.stream);
+            hipError_t e = onDevice ? hipSuccess : hipMemcpyAsync(s.host, s.dev, s.used, hipMemcpyDeviceToHost, s.stream);
             if (e == hipSuccess) e = hipEventRecord(s.done, s.stream);
             if (e != hipSuccess) return fail(SVO_ERR_DEVICE, std::string("serve: ") + hipGetErrorString(e));
             s.busy = true;

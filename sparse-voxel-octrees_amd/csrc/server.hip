@@ -1,0 +1,844 @@
+// Live request server: continuous batching of render requests into one
+// running persistent launch.
+//
+// A caller submits render requests (octree id, camera, resolution) at any
+// time from any thread. The server keeps the octrees resident, admits
+// requests in arrival order while their outputs (normal 4 B, t 4 B, leaf 8 B
+// per ray) fit the output arena and a ring slot is free (admission by HBM
+// footprint), and publishes each admitted request to a ring in pinned,
+// coherent host memory. One persistent kernel runs while there is work:
+//
+//  - a leader wave (wave 0 of workgroup 0) polls the ring's tail, copies each
+//    new request's view descriptor into device memory, extends the launch's
+//    table of first-tile indices (one entry per request), and publishes the
+//    new (request count, tile count) pair with a release store;
+//  - every other wave claims one global tile index at a time from a device
+//    counter; indices beyond the published tiles are held until published
+//    (the wave polls the published pair), so a request that arrives while
+//    the kernel is tracing others is traced by the next waves that free up;
+//  - a wave traces its 8x8 tile (march.hpp, the reference's raymarch per
+//    lane: src/VoxelOctree.cpp:207-346), releases its outputs at system scope
+//    and counts the tile done; the wave that completes a request's last tile
+//    writes the request's sequence number into a completion word in host
+//    memory.
+//
+// The host side is a request loop on its own thread: it records completions
+// (latency = submit -> completion seen on the host), admits waiting requests
+// as memory and ring slots free up, keeps a heartbeat the kernel checks, and
+// ends the launch when no request has been in flight for `idle_stop_us`
+// (the next admission starts a new one). Work the caller synchronises with
+// device-wide (hipDeviceSynchronize, hipFree) therefore waits at most until
+// the requests in flight are done and the launch has ended.
+//
+// Every wait inside the kernel is bounded: waves leave once the host asks the
+// launch to stop (only when nothing is in flight) or when the host's
+// heartbeat has not moved for 2 s; a wave holding a published tile always
+// traces it first, so every published tile is traced before the launch ends.
+//
+// Outputs live in the server's arena: device memory (results stay in HBM, the
+// caller reads them with its own kernels or copies) or pinned host memory
+// (the kernel writes them straight over the host link). No counterpart in the
+// reference, whose viewer renders one frame at a time on CPU threads
+// (src/Main.cpp:204-262,342-367).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "march.hpp"
+#include "svo_internal.hpp"
+#include "view_desc.hpp"
+
+namespace svo {
+
+namespace {
+
+using namespace march;
+
+constexpr uint64_t kAlign = 256;
+constexpr int kCountShift = 44;                     // published word: requests << 44 | tiles
+constexpr uint64_t kTileMask = (uint64_t(1) << kCountShift) - 1;
+constexpr uint64_t kHeartbeatTicks = 200000000ull;  // 2 s at the 100 MHz wall clock
+
+// Pinned host memory, shared with the kernel (coherent).
+struct ServeCtrl {
+    uint64_t tail;       // requests published (global sequence numbers < tail)
+    uint64_t stop;       // host: end the launch (only set with nothing in flight)
+    uint64_t heartbeat;  // host loop counter
+    uint64_t mirrored;   // kernel: requests of this launch copied to the device tables (global)
+    uint64_t pad[4];
+};
+struct ServeSlot {       // one ring entry, written by the host before it publishes
+    ViewDesc desc;
+    uint64_t seq;
+    uint32_t tiles;
+    uint32_t pad[13];
+};
+static_assert(sizeof(ServeSlot) == 192, "ServeSlot layout");
+
+// Device memory: launch state (cleared before each launch).
+struct ServeState {
+    unsigned long long cursor;     // next launch-relative tile index to claim
+    unsigned long long published;  // (requests << 44) | tiles, release-stored by the leader
+    unsigned int stop;             // set by the leader: waiting waves leave
+    unsigned int pad[11];
+};
+
+__device__ __forceinline__ uint64_t load_sys(uint64_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// Persistent serving kernel. Workgroups of 256 threads (4 waves); wave 0 of
+// workgroup 0 is the leader, every other wave traces tiles.
+template <typename Idx, int Reg>
+__global__ void __launch_bounds__(256) serve_kernel(ServeCtrl *ctrl, const ServeSlot *hslots, ServeState *st,
+                                                    ViewDesc *dslots, uint32_t *tilesOf, uint64_t *dbase,
+                                                    uint32_t *done, uint64_t *complete, int ring, uint64_t seq0,
+                                                    uint32_t maxReqs, int slots) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+
+    if (blockIdx.x == 0 && wave == 0) {
+        // ---- leader: mirror the host ring into the device tables ----
+        uint32_t mirrored = 0;
+        uint64_t total = 0;
+        uint64_t hb = __hip_atomic_load(&ctrl->heartbeat, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        uint64_t hbTime = wall_clock64();
+        for (;;) {
+            const uint64_t tail = load_sys(&ctrl->tail);
+            uint64_t avail = tail > seq0 ? tail - seq0 : 0;
+            if (avail > maxReqs) avail = maxReqs;
+            if (avail > mirrored) {
+                // two requests per pass: lanes 0-31 copy the first's 32
+                // descriptor words, lanes 32-63 the second's
+                for (uint32_t i = mirrored; i < uint32_t(avail); i += 2) {
+                    const uint32_t k = i + uint32_t(lane >> 5);
+                    const bool mine = k < uint32_t(avail);
+                    uint32_t tiles = 0;
+                    if (mine) {
+                        const int s = int((seq0 + k) % uint64_t(ring));
+                        const uint32_t *src = reinterpret_cast<const uint32_t *>(&hslots[s].desc) + (lane & 31);
+                        uint32_t *dst = reinterpret_cast<uint32_t *>(&dslots[s]) + (lane & 31);
+                        *dst = __hip_atomic_load(const_cast<uint32_t *>(src), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                        tiles = __hip_atomic_load(const_cast<uint32_t *>(&hslots[s].tiles), __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_SYSTEM);
+                        if ((lane & 31) == 0) tilesOf[s] = tiles;
+                    }
+                    const uint32_t t0 = __shfl(tiles, 0), t1 = __shfl(tiles, 32);
+                    if (lane == 0) dbase[i] = total;
+                    total += t0;
+                    if (i + 1 < uint32_t(avail)) {
+                        if (lane == 0) dbase[i + 1] = total;
+                        total += t1;
+                    }
+                }
+                mirrored = uint32_t(avail);
+                // every lane's descriptor words before the publishing store
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+                __builtin_amdgcn_wave_barrier();
+                if (lane == 0) {
+                    dbase[mirrored] = total;
+                    __hip_atomic_store(&st->published, (uint64_t(mirrored) << kCountShift) | total, __ATOMIC_RELEASE,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_store(&ctrl->mirrored, seq0 + mirrored, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                }
+            }
+            bool quit = load_sys(&ctrl->stop) != 0;
+            const uint64_t h = __hip_atomic_load(&ctrl->heartbeat, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            const uint64_t now = wall_clock64();
+            if (h != hb) {
+                hb = h;
+                hbTime = now;
+            } else if (now - hbTime > kHeartbeatTicks) {
+                quit = true;  // the host loop is gone: end the launch
+            }
+            if (quit) {
+                if (lane == 0) __hip_atomic_store(&st->stop, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                return;
+            }
+            __builtin_amdgcn_s_sleep(2);
+        }
+    }
+
+    // ---- tracing waves ----
+    const int waveInBlock = wave;
+    Stack<Idx> stk(lds + size_t(waveInBlock) * slots * 64 * (sizeof(Idx) + sizeof(float)), 64, slots, lane);
+    uint64_t known = 0;  // last published pair seen
+    uint32_t q = 0;      // launch-relative request of the last tile
+    for (;;) {
+        unsigned long long g = 0;
+        if (lane == 0) g = atomicAdd(&st->cursor, 1ull);
+        g = __shfl(g, 0);
+        // hold the tile index until it is published (or the launch ends)
+        while (g >= (known & kTileMask)) {
+            uint64_t p = 0;
+            if (lane == 0) p = __hip_atomic_load(&st->published, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+            p = __shfl(p, 0);
+            known = p;
+            if (g < (known & kTileMask)) break;
+            uint32_t stop = 0;
+            if (lane == 0) stop = __hip_atomic_load(&st->stop, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+            if (__shfl(stop, 0)) return;
+            __builtin_amdgcn_s_sleep(2);
+        }
+        // the request holding tile g: dbase[q] <= g < dbase[q + 1] (g only grows per wave)
+        const uint32_t count = uint32_t(known >> kCountShift);
+        if (dbase[q + 1] <= g) {
+            uint32_t lo = q + 1, hi = count;  // dbase[lo] <= g < dbase[hi]
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (dbase[mid] <= g) lo = mid; else hi = mid;
+            }
+            q = lo;
+        }
+        const uint64_t seq = seq0 + q;
+        const int s = int(seq % uint64_t(ring));
+        const ViewDesc &vd = dslots[s];
+        const uint64_t local = g - dbase[q];
+        const int tx = int(local % uint64_t(vd.tilesX));
+        const int ty = int(local / uint64_t(vd.tilesX));
+        const int x = tx * 8 + (lane & 7);
+        const int y = ty * 8 + (lane >> 3);
+        if (x < vd.width && y < vd.height) {
+            float dx, dy, dz;
+            view_ray_dir(vd, x, y, dx, dy, dz);
+            Ray<Idx> r;
+            ray_init(r, vd.tree, vd.px, vd.py, vd.pz, dx, dy, dz, 0.0f);
+            uint32_t normal = 0;
+            float t = 0.0f;
+            Idx leaf = 0;
+            int code;
+            do {
+                code = ray_step<false, true, Idx, true, Reg>(r, vd.tree, stk, normal, t, leaf);
+            } while (code == kRunning);
+            view_store<Idx>(vd, int64_t(y) * vd.width + x, code, normal, t, leaf);
+        }
+        // release this tile's outputs system-wide, then count it
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+        __builtin_amdgcn_wave_barrier();
+        uint32_t prior = 0;
+        if (lane == 0) prior = __hip_atomic_fetch_add(&done[s], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_SYSTEM);
+        prior = __shfl(prior, 0);
+        if (prior + 1 == tilesOf[s] && lane == 0) {
+            // the last tile of the request: reset the slot's counter for its
+            // next request, then publish completion to the host
+            __hip_atomic_store(&done[s], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&complete[s], seq + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+    }
+}
+
+struct Ticket {
+    svo_request req;
+    uint64_t off = 0, bytes = 0;  // arena range
+    uint64_t seq = 0;             // global sequence number once admitted
+    uint32_t tiles = 0;
+    int state = 0;                // 0 waiting, 1 in flight, 2 done
+    double submitted = 0.0, completed = 0.0;
+};
+
+// First-fit allocator over the output arena (offsets in bytes, kAlign units).
+class Arena {
+public:
+    void reset(uint64_t size) {
+        free_.clear();
+        if (size) free_[0] = size;
+    }
+    bool alloc(uint64_t bytes, uint64_t &off) {
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < bytes) continue;
+            off = it->first;
+            const uint64_t rest = it->second - bytes;
+            free_.erase(it);
+            if (rest) free_[off + bytes] = rest;
+            return true;
+        }
+        return false;
+    }
+    void release(uint64_t off, uint64_t bytes) {
+        auto next = free_.lower_bound(off);
+        if (next != free_.end() && off + bytes == next->first) {
+            bytes += next->second;
+            next = free_.erase(next);
+        }
+        if (next != free_.begin()) {
+            auto prev = std::prev(next);
+            if (prev->first + prev->second == off) {
+                prev->second += bytes;
+                return;
+            }
+        }
+        free_[off] = bytes;
+    }
+
+private:
+    std::map<uint64_t, uint64_t> free_;
+};
+
+uint64_t round_up(uint64_t b) { return (b + kAlign - 1) / kAlign * kAlign; }
+
+uint64_t request_bytes(const svo_request &r) {
+    const uint64_t n = uint64_t(r.width) * uint64_t(r.height);
+    return round_up(n * 4) * 2 + round_up(n * 8);
+}
+
+using Clock = std::chrono::steady_clock;
+
+}  // namespace
+
+}  // namespace svo
+
+struct svo_server {
+    std::vector<svo_octree *> trees;
+    int ring = 0;
+    uint32_t maxReqs = 0;
+    bool hostOutputs = false;
+    uint64_t budget = 0;
+    double idleStop = 0.0;
+    int device = 0, cus = 0, wavesPerCU = 24;
+    int reg = 1, stackSlots = 1;
+    bool wide = false;
+    // memory
+    svo::ServeCtrl *ctrl = nullptr;   // pinned
+    svo::ServeSlot *hslots = nullptr; // pinned
+    uint64_t *complete = nullptr;     // pinned, one word per ring slot
+    svo::ServeState *st = nullptr;    // device
+    svo::ViewDesc *dslots = nullptr;  // device
+    uint32_t *tilesOf = nullptr;      // device
+    uint64_t *dbase = nullptr;        // device
+    uint32_t *done = nullptr;         // device
+    uint8_t *arena = nullptr;         // device or pinned
+    hipStream_t stream = nullptr;
+    hipStream_t copyStream = nullptr;  // copies of completed outputs to the host
+    hipEvent_t ended = nullptr;
+    svo::Arena alloc;
+    // request state
+    std::mutex mu;
+    std::condition_variable cv;
+    std::unordered_map<uint64_t, svo::Ticket> tickets;
+    std::deque<uint64_t> waiting;        // submitted, not yet admitted
+    std::deque<uint64_t> inflight;       // admitted, in sequence order
+    std::vector<uint64_t> slotTicket;    // ring slot -> ticket
+    uint64_t nextTicket = 1, nextSeq = 0;
+    bool live = false;
+    uint64_t seq0 = 0;                   // first sequence number of the running launch
+    svo::Clock::time_point t0, lastBusy;
+    int error = SVO_OK;
+    std::string errorMsg;
+    // loop thread
+    std::thread loop;
+    bool quit = false;
+    uint64_t launches = 0, published = 0;
+};
+
+namespace svo {
+
+namespace {
+
+double now_s(const svo_server &s) { return std::chrono::duration<double>(Clock::now() - s.t0).count(); }
+
+void set_error(svo_server &s, const std::string &msg) {
+    if (s.error == SVO_OK) {
+        s.error = SVO_ERR_DEVICE;
+        s.errorMsg = msg;
+    }
+}
+
+// Starts a launch whose first request is sequence number s.nextSeq (called
+// with the lock held, no launch running).
+bool start_launch(svo_server &s) {
+    s.seq0 = s.nextSeq;
+    __atomic_store_n(&s.ctrl->stop, 0ull, __ATOMIC_RELEASE);
+    __atomic_store_n(&s.ctrl->mirrored, s.seq0, __ATOMIC_RELEASE);
+    hipError_t e = hipMemsetAsync(s.st, 0, sizeof(ServeState), s.stream);
+    if (e != hipSuccess) {
+        set_error(s, std::string("server launch: ") + hipGetErrorString(e));
+        return false;
+    }
+    const size_t lds = size_t(4) * s.stackSlots * 64 * ((s.wide ? 8 : 4) + sizeof(float));
+    const int blocksPerCU = std::max(1, std::min(s.wavesPerCU / 4, int((160 * 1024) / lds)));
+    const dim3 grid(unsigned(s.cus * blocksPerCU)), block(256);
+    if (!s.wide && s.reg == 1)
+        hipLaunchKernelGGL((serve_kernel<uint32_t, 1>), grid, block, lds, s.stream, s.ctrl, s.hslots, s.st, s.dslots,
+                           s.tilesOf, s.dbase, s.done, s.complete, s.ring, s.seq0, s.maxReqs, s.stackSlots);
+    else if (!s.wide)
+        hipLaunchKernelGGL((serve_kernel<uint32_t, 3>), grid, block, lds, s.stream, s.ctrl, s.hslots, s.st, s.dslots,
+                           s.tilesOf, s.dbase, s.done, s.complete, s.ring, s.seq0, s.maxReqs, s.stackSlots);
+    else if (s.reg == 1)
+        hipLaunchKernelGGL((serve_kernel<uint64_t, 1>), grid, block, lds, s.stream, s.ctrl, s.hslots, s.st, s.dslots,
+                           s.tilesOf, s.dbase, s.done, s.complete, s.ring, s.seq0, s.maxReqs, s.stackSlots);
+    else
+        hipLaunchKernelGGL((serve_kernel<uint64_t, 3>), grid, block, lds, s.stream, s.ctrl, s.hslots, s.st, s.dslots,
+                           s.tilesOf, s.dbase, s.done, s.complete, s.ring, s.seq0, s.maxReqs, s.stackSlots);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipEventRecord(s.ended, s.stream);
+    if (e != hipSuccess) {
+        set_error(s, std::string("server launch: ") + hipGetErrorString(e));
+        return false;
+    }
+    s.live = true;
+    s.launches++;
+    return true;
+}
+
+// Asks the running launch to end and waits for it (lock held; nothing in flight).
+void stop_launch(svo_server &s) {
+    if (!s.live) return;
+    __atomic_store_n(&s.ctrl->stop, 1ull, __ATOMIC_RELEASE);
+    hipError_t e = hipEventSynchronize(s.ended);
+    if (e != hipSuccess) set_error(s, std::string("server stop: ") + hipGetErrorString(e));
+    s.live = false;
+}
+
+// Admits waiting requests in arrival order while arena memory, a ring slot
+// and room in the launch's request table are free, and publishes them.
+void admit(svo_server &s) {
+    if (s.error) return;
+    while (!s.waiting.empty()) {
+        Ticket &tk = s.tickets[s.waiting.front()];
+        // ring window: the oldest request in flight holds its slot
+        const uint64_t oldest = s.inflight.empty() ? s.nextSeq : s.tickets[s.inflight.front()].seq;
+        if (s.nextSeq - oldest >= uint64_t(s.ring)) return;
+        if (s.live && s.nextSeq - s.seq0 >= s.maxReqs) {
+            // this launch's request table is full: start a new launch once it drains
+            if (!s.inflight.empty()) return;
+            stop_launch(s);
+            if (s.error) return;
+        }
+        uint64_t off = 0;
+        if (!s.alloc.alloc(tk.bytes, off)) return;  // admission by output footprint
+        if (!s.live && !start_launch(s)) {
+            s.alloc.release(off, tk.bytes);
+            return;
+        }
+        tk.off = off;
+        const uint64_t id = s.waiting.front();
+        s.waiting.pop_front();
+        tk.seq = s.nextSeq++;
+        const int slot = int(tk.seq % uint64_t(s.ring));
+        s.slotTicket[size_t(slot)] = id;
+        ServeSlot &hs = s.hslots[slot];
+        const svo_request &r = tk.req;
+        const uint64_t n = uint64_t(r.width) * uint64_t(r.height);
+        svo_view v{};
+        v.tree = s.trees[size_t(r.tree)];
+        v.camera = r.camera;
+        v.width = r.width;
+        v.height = r.height;
+        v.normal = reinterpret_cast<uint32_t *>(s.arena + off);
+        v.t = reinterpret_cast<float *>(s.arena + off + round_up(n * 4));
+        v.leaf = reinterpret_cast<uint64_t *>(s.arena + off + 2 * round_up(n * 4));
+        ViewDesc d;
+        (void)fill_view_desc(v, d);  // checked at submit
+        std::memcpy(&hs.desc, &d, sizeof(d));
+        hs.seq = tk.seq;
+        hs.tiles = tk.tiles;
+        tk.state = 1;
+        s.inflight.push_back(id);
+        __atomic_store_n(&s.ctrl->tail, tk.seq + 1, __ATOMIC_RELEASE);  // publish
+        s.published++;
+        s.lastBusy = Clock::now();
+    }
+}
+
+// Records completions of requests in flight (lock held).
+bool reap(svo_server &s) {
+    bool any = false;
+    for (auto it = s.inflight.begin(); it != s.inflight.end();) {
+        Ticket &tk = s.tickets[*it];
+        const int slot = int(tk.seq % uint64_t(s.ring));
+        if (__atomic_load_n(&s.complete[slot], __ATOMIC_ACQUIRE) == tk.seq + 1) {
+            tk.state = 2;
+            tk.completed = now_s(s);
+            it = s.inflight.erase(it);
+            any = true;
+        } else {
+            ++it;
+        }
+    }
+    return any;
+}
+
+// Drops a ticket that is waiting or done (lock held): its arena range goes
+// back to the allocator and waiting requests may now fit.
+void alloc_release(svo_server &s, std::unordered_map<uint64_t, Ticket>::iterator it) {
+    if (it->second.state == 0) {
+        auto w = std::find(s.waiting.begin(), s.waiting.end(), it->first);
+        if (w != s.waiting.end()) s.waiting.erase(w);
+        s.tickets.erase(it);
+        return;
+    }
+    s.alloc.release(it->second.off, it->second.bytes);
+    s.tickets.erase(it);
+    admit(s);
+}
+
+void loop_main(svo_server *sp) {
+    svo_server &s = *sp;
+    (void)hipSetDevice(s.device);
+    std::unique_lock<std::mutex> lock(s.mu);
+    while (!s.quit) {
+        __atomic_fetch_add(&s.ctrl->heartbeat, 1ull, __ATOMIC_RELEASE);
+        if (s.live) {
+            if (reap(s)) {
+                s.lastBusy = Clock::now();
+                admit(s);
+                s.cv.notify_all();
+            }
+            if (s.live && hipEventQuery(s.ended) == hipSuccess) {
+                // the launch ended by itself (heartbeat lost): requests it did
+                // not take over are published again in a new launch
+                s.live = false;
+                if (!s.inflight.empty()) {
+                    const uint64_t mirrored = __atomic_load_n(&s.ctrl->mirrored, __ATOMIC_ACQUIRE);
+                    reap(s);
+                    if (!s.inflight.empty()) {
+                        const uint64_t first = s.tickets[s.inflight.front()].seq;
+                        if (mirrored <= first) {
+                            // not yet taken over: republish from `first`
+                            s.nextSeq = first;
+                            for (auto it = s.inflight.rbegin(); it != s.inflight.rend(); ++it) {
+                                Ticket &tk = s.tickets[*it];
+                                tk.state = 0;
+                                s.waiting.push_front(*it);
+                            }
+                            s.inflight.clear();
+                        } else {
+                            set_error(s, "server: a launch ended with requests in flight");
+                        }
+                    }
+                }
+                admit(s);
+                s.cv.notify_all();
+            } else if (s.inflight.empty() &&
+                       std::chrono::duration<double>(Clock::now() - s.lastBusy).count() > s.idleStop) {
+                // nothing in flight: requests still waiting for arena memory
+                // cannot start before the caller releases outputs, so the
+                // launch ends (a device-wide synchronisation by the caller
+                // would otherwise wait on it); admission starts a new one
+                stop_launch(s);
+                s.cv.notify_all();
+            }
+        } else if (!s.waiting.empty()) {
+            admit(s);
+        }
+        if (s.error) s.cv.notify_all();
+        // spin briefly without the lock, so submitters and pollers get in
+        lock.unlock();
+        if (s.live) {
+            for (int k = 0; k < 64; ++k) __builtin_ia32_pause();
+        } else {
+            std::this_thread::sleep_for(std::chrono::microseconds(200));
+        }
+        lock.lock();
+    }
+    if (!s.inflight.empty()) {
+        // destroy with requests in flight: let the launch finish them
+        const auto deadline = Clock::now() + std::chrono::seconds(10);
+        while (!s.inflight.empty() && Clock::now() < deadline) {
+            __atomic_fetch_add(&s.ctrl->heartbeat, 1ull, __ATOMIC_RELEASE);
+            reap(s);
+        }
+    }
+    stop_launch(s);
+}
+
+void free_server(svo_server *s) {
+    if (!s) return;
+    if (s->loop.joinable()) {
+        {
+            std::lock_guard<std::mutex> lock(s->mu);
+            s->quit = true;
+        }
+        s->loop.join();
+    }
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    if (s->ended) (void)hipEventDestroy(s->ended);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    if (s->copyStream) (void)hipStreamDestroy(s->copyStream);
+    if (s->arena) {
+        if (s->hostOutputs) (void)hipHostFree(s->arena);
+        else (void)hipFree(s->arena);
+    }
+    if (s->st) (void)hipFree(s->st);
+    if (s->dslots) (void)hipFree(s->dslots);
+    if (s->tilesOf) (void)hipFree(s->tilesOf);
+    if (s->dbase) (void)hipFree(s->dbase);
+    if (s->done) (void)hipFree(s->done);
+    if (s->ctrl) (void)hipHostFree(s->ctrl);
+    if (s->hslots) (void)hipHostFree(s->hslots);
+    if (s->complete) (void)hipHostFree(s->complete);
+    delete s;
+}
+
+}  // namespace
+
+int server_create(svo_octree *const *trees, int ntrees, const svo_server_config *cfg, svo_server **out) {
+    if (!out) return fail(SVO_ERR_INVALID, "server: null output");
+    *out = nullptr;
+    if (!trees || ntrees <= 0) return fail(SVO_ERR_INVALID, "server: no octrees");
+    for (int i = 0; i < ntrees; ++i)
+        if (!trees[i]) return fail(SVO_ERR_INVALID, "server: null octree");
+    const int ring = (cfg && cfg->ring_slots > 0) ? cfg->ring_slots : 1024;
+    if (ring > (1 << 20)) return fail(SVO_ERR_INVALID, "server: ring_slots above 2^20");
+    const uint64_t budget = (cfg && cfg->output_budget) ? cfg->output_budget : (uint64_t(1) << 30);
+    const int waves = (cfg && cfg->waves_per_cu > 0) ? cfg->waves_per_cu : 24;
+    if (waves < 4 || waves > 32) return fail(SVO_ERR_INVALID, "server: waves_per_cu must be 4..32");
+    const double idle = (cfg && cfg->idle_stop_us > 0) ? cfg->idle_stop_us * 1e-6 : 2e-3;
+    if (device_count() < 1) return fail(SVO_ERR_DEVICE, "server: no HIP device");
+    svo_server *s = new (std::nothrow) svo_server();
+    if (!s) return fail(SVO_ERR_NOMEM, "out of host memory");
+    s->trees.assign(trees, trees + ntrees);
+    s->ring = ring;
+    s->maxReqs = 1u << 20;
+    s->hostOutputs = cfg && cfg->host_outputs;
+    s->budget = budget / kAlign * kAlign;
+    s->idleStop = idle;
+    s->wavesPerCU = waves;
+    int maxDepth = 1;
+    for (svo_octree *t : s->trees) {
+        int r = ensure_device(*t);  // octrees stay resident while the server lives
+        if (r) {
+            free_server(s);
+            return r;
+        }
+        maxDepth = std::max(maxDepth, t->depth);
+        s->wide = s->wide || t->words.size() >= (uint64_t(1) << 32);
+    }
+    s->reg = reg_entries(maxDepth);
+    s->stackSlots = lds_slots(maxDepth, s->reg);
+    hipDeviceProp_t prop;
+    hipError_t e = hipGetDevice(&s->device);
+    if (e == hipSuccess) e = hipGetDeviceProperties(&prop, s->device);
+    if (e == hipSuccess) {
+        s->cus = prop.multiProcessorCount;
+        const unsigned pinned = hipHostMallocCoherent | hipHostMallocMapped;
+        e = hipHostMalloc(reinterpret_cast<void **>(&s->ctrl), sizeof(ServeCtrl), pinned);
+        if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void **>(&s->hslots), sizeof(ServeSlot) * ring, pinned);
+        if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void **>(&s->complete), sizeof(uint64_t) * ring, pinned);
+        if (e == hipSuccess) e = hipMalloc(&s->st, sizeof(ServeState));
+        if (e == hipSuccess) e = hipMalloc(&s->dslots, sizeof(ViewDesc) * ring);
+        if (e == hipSuccess) e = hipMalloc(&s->tilesOf, sizeof(uint32_t) * ring);
+        if (e == hipSuccess) e = hipMalloc(&s->dbase, sizeof(uint64_t) * (size_t(s->maxReqs) + 1));
+        if (e == hipSuccess) e = hipMalloc(&s->done, sizeof(uint32_t) * ring);
+        if (e == hipSuccess) e = hipMemset(s->done, 0, sizeof(uint32_t) * ring);
+        if (e == hipSuccess) {
+            if (s->hostOutputs) e = hipHostMalloc(reinterpret_cast<void **>(&s->arena), s->budget, pinned);
+            else e = hipMalloc(&s->arena, s->budget);
+        }
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->copyStream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&s->ended, hipEventDisableTiming);
+    }
+    if (e != hipSuccess) {
+        const int code = (e == hipErrorOutOfMemory) ? SVO_ERR_NOMEM : SVO_ERR_DEVICE;
+        free_server(s);
+        return fail(code, std::string("server: ") + hipGetErrorString(e));
+    }
+    std::memset(s->ctrl, 0, sizeof(ServeCtrl));
+    std::memset(s->hslots, 0, sizeof(ServeSlot) * ring);
+    std::memset(s->complete, 0, sizeof(uint64_t) * ring);
+    s->slotTicket.assign(size_t(ring), 0);
+    s->alloc.reset(s->budget);
+    s->t0 = Clock::now();
+    s->lastBusy = s->t0;
+    try {
+        s->loop = std::thread(loop_main, s);
+    } catch (...) {
+        free_server(s);
+        return fail(SVO_ERR_NOMEM, "server: cannot start the request loop thread");
+    }
+    *out = s;
+    return SVO_OK;
+}
+
+void server_destroy(svo_server *s) { free_server(s); }
+
+int server_submit(svo_server *s, const svo_request *r, uint64_t *ticket) {
+    if (!s || !r || !ticket) return fail(SVO_ERR_INVALID, "server: null argument");
+    if (r->tree < 0 || size_t(r->tree) >= s->trees.size()) return fail(SVO_ERR_INVALID, "server: bad octree index");
+    if (r->width <= 0 || r->height <= 0 || r->width > 65536 || r->height > 65536)
+        return fail(SVO_ERR_INVALID, "server: bad request size");
+    if (!camera_ok(r->camera)) return fail(SVO_ERR_INVALID, "camera entries must be finite and within +-1e6");
+    Ticket tk;
+    tk.req = *r;
+    tk.bytes = request_bytes(*r);
+    if (tk.bytes > s->budget) return fail(SVO_ERR_LIMIT, "server: a request's outputs exceed the output budget");
+    tk.tiles = uint32_t(((r->width + 7) / 8) * ((r->height + 7) / 8));
+    std::lock_guard<std::mutex> lock(s->mu);
+    if (s->error) return fail(s->error, s->errorMsg);
+    tk.submitted = now_s(*s);
+    const uint64_t id = s->nextTicket++;
+    s->tickets.emplace(id, tk);
+    s->waiting.push_back(id);
+    admit(*s);
+    if (s->error) return fail(s->error, s->errorMsg);
+    *ticket = id;
+    return SVO_OK;
+}
+
+int server_poll(svo_server *s, uint64_t ticket, int *state, double *latency) {
+    if (!s || !state) return fail(SVO_ERR_INVALID, "server: null argument");
+    std::lock_guard<std::mutex> lock(s->mu);
+    auto it = s->tickets.find(ticket);
+    if (it == s->tickets.end()) return fail(SVO_ERR_INVALID, "server: unknown ticket");
+    if (s->error && it->second.state != 2) return fail(s->error, s->errorMsg);
+    *state = it->second.state;
+    if (latency) *latency = it->second.state == 2 ? it->second.completed - it->second.submitted : 0.0;
+    return SVO_OK;
+}
+
+int server_wait(svo_server *s, uint64_t ticket, double timeout, double *latency) {
+    if (!s) return fail(SVO_ERR_INVALID, "server: null argument");
+    std::unique_lock<std::mutex> lock(s->mu);
+    auto it = s->tickets.find(ticket);
+    if (it == s->tickets.end()) return fail(SVO_ERR_INVALID, "server: unknown ticket");
+    const auto deadline = Clock::now() + std::chrono::duration_cast<Clock::duration>(
+                                             std::chrono::duration<double>(timeout > 0 ? timeout : 1e9));
+    while (it->second.state != 2 && !s->error) {
+        if (s->cv.wait_until(lock, deadline) == std::cv_status::timeout && it->second.state != 2)
+            return fail(SVO_ERR_LIMIT, "server: wait timed out");
+    }
+    if (it->second.state != 2) return fail(s->error, s->errorMsg);
+    if (latency) *latency = it->second.completed - it->second.submitted;
+    return SVO_OK;
+}
+
+int server_outputs(svo_server *s, uint64_t ticket, uint32_t **normal, float **t, uint64_t **leaf) {
+    if (!s) return fail(SVO_ERR_INVALID, "server: null argument");
+    std::lock_guard<std::mutex> lock(s->mu);
+    auto it = s->tickets.find(ticket);
+    if (it == s->tickets.end()) return fail(SVO_ERR_INVALID, "server: unknown ticket");
+    if (it->second.state != 2) return fail(SVO_ERR_INVALID, "server: request not complete");
+    const Ticket &tk = it->second;
+    const uint64_t n = uint64_t(tk.req.width) * tk.req.height;
+    if (normal) *normal = reinterpret_cast<uint32_t *>(s->arena + tk.off);
+    if (t) *t = reinterpret_cast<float *>(s->arena + tk.off + round_up(n * 4));
+    if (leaf) *leaf = reinterpret_cast<uint64_t *>(s->arena + tk.off + 2 * round_up(n * 4));
+    return SVO_OK;
+}
+
+int server_copy_outputs(svo_server *s, uint64_t ticket, uint32_t *normal, float *t, uint64_t *leaf) {
+    uint32_t *dn = nullptr;
+    float *dt = nullptr;
+    uint64_t *dl = nullptr;
+    int r = server_outputs(s, ticket, &dn, &dt, &dl);
+    if (r) return r;
+    uint64_t n = 0;
+    {
+        std::lock_guard<std::mutex> lock(s->mu);
+        const Ticket &tk = s->tickets[ticket];
+        n = uint64_t(tk.req.width) * tk.req.height;
+    }
+    if (s->hostOutputs) {
+        if (normal) std::memcpy(normal, dn, n * 4);
+        if (t) std::memcpy(t, dt, n * 4);
+        if (leaf) std::memcpy(leaf, dl, n * 8);
+        return SVO_OK;
+    }
+    // the copy stream is non-blocking, so the copies do not wait for the
+    // running launch (the request's outputs were released by it)
+    hipError_t e = hipSuccess;
+    if (normal) e = hipMemcpyAsync(normal, dn, n * 4, hipMemcpyDeviceToHost, s->copyStream);
+    if (e == hipSuccess && t) e = hipMemcpyAsync(t, dt, n * 4, hipMemcpyDeviceToHost, s->copyStream);
+    if (e == hipSuccess && leaf) e = hipMemcpyAsync(leaf, dl, n * 8, hipMemcpyDeviceToHost, s->copyStream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->copyStream);
+    if (e != hipSuccess) return fail(SVO_ERR_DEVICE, std::string("server copy: ") + hipGetErrorString(e));
+    return SVO_OK;
+}
+
+int server_release(svo_server *s, uint64_t ticket) {
+    if (!s) return fail(SVO_ERR_INVALID, "server: null argument");
+    std::lock_guard<std::mutex> lock(s->mu);
+    auto it = s->tickets.find(ticket);
+    if (it == s->tickets.end()) return fail(SVO_ERR_INVALID, "server: unknown ticket");
+    if (it->second.state == 1) return fail(SVO_ERR_INVALID, "server: request still in flight");
+    alloc_release(*s, it);
+    return SVO_OK;
+}
+
+int server_stats(svo_server *s, svo_server_stats *out) {
+    if (!s || !out) return fail(SVO_ERR_INVALID, "server: null argument");
+    std::lock_guard<std::mutex> lock(s->mu);
+    std::memset(out, 0, sizeof(*out));
+    out->launches = s->launches;
+    out->published = s->published;
+    out->in_flight = s->inflight.size();
+    out->waiting = s->waiting.size();
+    out->live = s->live ? 1 : 0;
+    return SVO_OK;
+}
+
+}  // namespace svo
+
+namespace svo {
+
+int server_run_schedule(svo_server *s, const svo_request *reqs, const double *arrival, int n, uint64_t *tickets,
+                        double *latency, double *makespan) {
+    if (!s || !reqs || !arrival || !latency || n <= 0) return fail(SVO_ERR_INVALID, "schedule: null or empty argument");
+    for (int i = 1; i < n; ++i)
+        if (!(arrival[i] >= arrival[i - 1])) return fail(SVO_ERR_INVALID, "schedule: arrivals must be ascending");
+    std::vector<uint64_t> ids(size_t(n), 0);
+    std::vector<char> finished(size_t(n), 0);
+    double base;
+    {
+        std::lock_guard<std::mutex> lock(s->mu);
+        base = now_s(*s);
+    }
+    const auto t0 = Clock::now();
+    int next = 0, completed = 0, first = 0;
+    double lastDone = 0.0;
+    auto stalled = Clock::now();
+    while (completed < n) {
+        const double t = std::chrono::duration<double>(Clock::now() - t0).count();
+        while (next < n && arrival[next] <= t) {
+            int r = server_submit(s, &reqs[next], &ids[size_t(next)]);
+            if (r) return r;
+            next++;
+        }
+        bool progress = false;
+        {
+            std::lock_guard<std::mutex> lock(s->mu);
+            if (s->error) return fail(s->error, s->errorMsg);
+            for (int i = first; i < next; ++i) {
+                if (finished[size_t(i)]) continue;
+                auto it = s->tickets.find(ids[size_t(i)]);
+                if (it == s->tickets.end()) return fail(SVO_ERR_INVALID, "schedule: a ticket was released by another caller");
+                if (it->second.state != 2) continue;
+                latency[i] = it->second.completed - (base + arrival[i]);
+                lastDone = std::max(lastDone, it->second.completed - base);
+                finished[size_t(i)] = 1;
+                completed++;
+                progress = true;
+                if (!tickets) {
+                    alloc_release(*s, it);
+                }
+            }
+            while (first < next && finished[size_t(first)]) first++;
+            if (progress) {
+                stalled = Clock::now();
+            } else if (next == n && s->inflight.empty() && !s->waiting.empty() &&
+                       std::chrono::duration<double>(Clock::now() - stalled).count() > 0.5) {
+                return fail(SVO_ERR_LIMIT, "schedule: the kept outputs fill the arena; raise output_budget");
+            }
+        }
+        for (int k = 0; k < 32; ++k) __builtin_ia32_pause();
+    }
+    if (tickets)
+        for (int i = 0; i < n; ++i) tickets[i] = ids[size_t(i)];
+    if (makespan) *makespan = lastDone - arrival[0];
+    return SVO_OK;
+}
+
+}  // namespace svo

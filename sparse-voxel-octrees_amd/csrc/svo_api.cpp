@@ -317,7 +317,7 @@ int svo_raymarch_batch(svo_octree *h, int64_t n, const float *o, const float *d,
             return r;
         r = raymarch_device(*h, n, dO.p, dD.p, ray_scale ? dS.p : nullptr, ray_scale0, dH.p, dN.p, dT.p, dL.p, nullptr);
         if (r) return r;
-        hipError_t e = hipDeviceSynchronize();
+        hipError_t e = hipStreamSynchronize(nullptr);  // the null stream the launch went to
         if (e != hipSuccess) return fail(SVO_ERR_DEVICE, std::string("raymarch: ") + hipGetErrorString(e));
         if ((r = from_device(hit, dH, m)) || (r = from_device(normal, dN, m)) || (r = from_device(t, dT, m)) ||
             (r = from_device(leaf, dL, m)))
@@ -331,49 +331,15 @@ int svo_raymarch(svo_octree *h, const float o[3], const float d[3], float ray_sc
     return guarded([&]() -> int {
         clear_error();
         if (!h || !o || !d) return fail(SVO_ERR_INVALID, "null argument");
-        // One ray per call (the reference's per-pixel interface): a small device
-        // block per thread and device, reused by every call, so a call is one
-        // upload, one launch and one download. It is not freed at thread exit
-        // (64 bytes per thread; a thread-exit handler can outlive the runtime).
-        struct Block {
-            int device = -1;
-            uint8_t *p = nullptr;
-        };
-        thread_local Block blk;
-        int dev = 0;
-        hipError_t e = hipGetDevice(&dev);
-        if (e == hipSuccess && (!blk.p || blk.device != dev)) {
-            if (blk.p) {  // the previous device's block; calls are synchronous, so nothing still uses it
-                int cur = dev;
-                if (hipSetDevice(blk.device) == hipSuccess) (void)hipFree(blk.p);
-                (void)hipSetDevice(cur);
-                blk.p = nullptr;
-            }
-            void *q = nullptr;
-            e = hipMalloc(&q, 64);
-            if (e == hipSuccess) {
-                blk.p = static_cast<uint8_t *>(q);
-                blk.device = dev;
-            }
-        }
-        if (e != hipSuccess) return fail(SVO_ERR_DEVICE, std::string("raymarch: ") + hipGetErrorString(e));
-        float in[6] = {o[0], o[1], o[2], d[0], d[1], d[2]};
-        e = hipMemcpy(blk.p, in, sizeof(in), hipMemcpyHostToDevice);
-        if (e != hipSuccess) return fail(SVO_ERR_DEVICE, std::string("raymarch: ") + hipGetErrorString(e));
-        const float *dO = reinterpret_cast<const float *>(blk.p), *dD = dO + 3;
-        uint8_t *dHit = blk.p + 32;
-        uint32_t *dNrm = reinterpret_cast<uint32_t *>(blk.p + 36);
-        float *dT = reinterpret_cast<float *>(blk.p + 40);
-        int r = raymarch_device(*h, 1, dO, dD, nullptr, ray_scale, dHit, dNrm, dT, nullptr, nullptr);
-        if (r) return r;
-        uint8_t out[12];
-        e = hipMemcpy(out, blk.p + 32, sizeof(out), hipMemcpyDeviceToHost);  // waits for the launch (null stream)
-        if (e != hipSuccess) return fail(SVO_ERR_DEVICE, std::string("raymarch: ") + hipGetErrorString(e));
-        const uint8_t code = out[0];
-        uint32_t nrm;
-        float tt;
-        std::memcpy(&nrm, out + 4, 4);
-        std::memcpy(&tt, out + 8, 4);
+        // One ray per call (the reference's per-pixel interface, src/Main.cpp:118,181):
+        // walked on the calling thread by the kernels' own traversal source
+        // (march.hpp compiled for the host). A device launch per ray costs about
+        // 30 us of upload, launch and download against well under 1 us of walk;
+        // tiles, frames and ray batches go to the GPU entry points.
+        uint32_t nrm = 0;
+        float tt = 0.0f;
+        uint64_t leaf = 0;
+        const int code = raymarch_one(*h, o, d, ray_scale, nrm, tt, leaf);
         if (hit) *hit = code;
         if (code == 1 && normal) *normal = nrm;
         if (code != 0 && t) *t = tt;
@@ -415,7 +381,7 @@ int svo_render_primary(svo_octree *h, const svo_camera *cam, int width, int heig
             return r;
         r = render_primary_device(*h, *cam, width, height, dH.p, dN.p, dT.p, dL.p, nullptr);
         if (r) return r;
-        hipError_t e = hipDeviceSynchronize();
+        hipError_t e = hipStreamSynchronize(nullptr);  // the null stream the launch went to
         if (e != hipSuccess) return fail(SVO_ERR_DEVICE, std::string("render: ") + hipGetErrorString(e));
         if ((r = from_device(hit, dH, m)) || (r = from_device(normal, dN, m)) || (r = from_device(t, dT, m)) ||
             (r = from_device(leaf, dL, m)))
@@ -451,7 +417,7 @@ int svo_render_shadow(svo_octree *h, const svo_camera *cam, int width, int heigh
             return r;
         r = render_shadow_device(*h, *cam, width, height, light, eps, dH.p, dN.p, dT.p, dL.p, dS.p, nullptr);
         if (r) return r;
-        hipError_t e = hipDeviceSynchronize();
+        hipError_t e = hipStreamSynchronize(nullptr);  // the null stream the launch went to
         if (e != hipSuccess) return fail(SVO_ERR_DEVICE, std::string("render: ") + hipGetErrorString(e));
         if ((r = from_device(hit, dH, m)) || (r = from_device(normal, dN, m)) || (r = from_device(t, dT, m)) ||
             (r = from_device(leaf, dL, m)) || (r = from_device(shadow, dS, m)))
@@ -498,7 +464,7 @@ int svo_render_frame(svo_octree *h, const svo_camera *cam, int width, int height
         if ((r = dR.alloc(m)) || (r = out_buf(dD, depth, size_t(nd)))) return r;
         r = render_frame_device(*h, *cam, width, height, stripes, stride, dR.p, dD.p, nullptr);
         if (r) return r;
-        hipError_t e = hipDeviceSynchronize();
+        hipError_t e = hipStreamSynchronize(nullptr);  // the null stream the launch went to
         if (e != hipSuccess) return fail(SVO_ERR_DEVICE, std::string("frame: ") + hipGetErrorString(e));
         if ((r = from_device(rgba, dR, m)) || (r = from_device(depth, dD, size_t(nd)))) return r;
         return SVO_OK;
@@ -559,6 +525,74 @@ int svo_serve_replay(svo_octree *const *trees, int ntrees, const svo_request *re
         return serve_replay(trees, ntrees, requests, arrival_s, n, opts, latency_s, makespan_s, normal_out, t_out, leaf_out);
     });
 }
+
+// ---- live request server ---------------------------------------------------------
+
+int svo_server_create(svo_octree *const *trees, int ntrees, const svo_server_config *cfg, svo_server **out) {
+    return guarded([&]() -> int {
+        clear_error();
+        return server_create(trees, ntrees, cfg, out);
+    });
+}
+
+int svo_server_submit(svo_server *s, const svo_request *req, uint64_t *ticket) {
+    return guarded([&]() -> int {
+        clear_error();
+        return server_submit(s, req, ticket);
+    });
+}
+
+int svo_server_poll(svo_server *s, uint64_t ticket, int *state, double *latency_s) {
+    return guarded([&]() -> int {
+        clear_error();
+        return server_poll(s, ticket, state, latency_s);
+    });
+}
+
+int svo_server_wait(svo_server *s, uint64_t ticket, double timeout_s, double *latency_s) {
+    return guarded([&]() -> int {
+        clear_error();
+        return server_wait(s, ticket, timeout_s, latency_s);
+    });
+}
+
+int svo_server_outputs(svo_server *s, uint64_t ticket, uint32_t **normal, float **t, uint64_t **leaf) {
+    return guarded([&]() -> int {
+        clear_error();
+        return server_outputs(s, ticket, normal, t, leaf);
+    });
+}
+
+int svo_server_copy_outputs(svo_server *s, uint64_t ticket, uint32_t *normal, float *t, uint64_t *leaf) {
+    return guarded([&]() -> int {
+        clear_error();
+        return server_copy_outputs(s, ticket, normal, t, leaf);
+    });
+}
+
+int svo_server_release(svo_server *s, uint64_t ticket) {
+    return guarded([&]() -> int {
+        clear_error();
+        return server_release(s, ticket);
+    });
+}
+
+int svo_server_stats_get(svo_server *s, svo_server_stats *out) {
+    return guarded([&]() -> int {
+        clear_error();
+        return server_stats(s, out);
+    });
+}
+
+int svo_server_run_schedule(svo_server *s, const svo_request *requests, const double *arrival_s, int n,
+                            uint64_t *tickets, double *latency_s, double *makespan_s) {
+    return guarded([&]() -> int {
+        clear_error();
+        return server_run_schedule(s, requests, arrival_s, n, tickets, latency_s, makespan_s);
+    });
+}
+
+void svo_server_destroy(svo_server *s) { server_destroy(s); }
 
 // ---- construction -------------------------------------------------------------
 

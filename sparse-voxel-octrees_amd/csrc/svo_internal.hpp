@@ -67,6 +67,7 @@ struct svo_octree {
 
 struct svo_renderer;     // render_batch.hip
 struct svo_batch_graph;  // render_batch.hip
+struct svo_server;       // server.hip
 
 namespace svo {
 
@@ -121,6 +122,8 @@ int raymarch_device(svo_octree &h, int64_t n, const float *o, const float *d, co
 int render_primary_device(svo_octree &h, const svo_camera &cam, int width, int height, uint8_t *hit,
                           uint32_t *normal, float *t, uint64_t *leaf, void *stream);
 int device_count();
+int raymarch_one(const svo_octree &h, const float o[3], const float d[3], float rs, uint32_t &normal, float &t,
+                 uint64_t &leaf);
 
 int render_shadow_device(svo_octree &h, const svo_camera &cam, int width, int height, const float light[3], float eps,
                          uint8_t *hit, uint32_t *normal, float *t, uint64_t *leaf, uint8_t *shadow, void *stream);
@@ -147,6 +150,19 @@ void batch_graph_free(svo_batch_graph *g);
 int serve_replay(svo_octree *const *trees, int ntrees, const svo_request *reqs, const double *arrival, int n,
                  const svo_server_opts *opts, double *latency, double *makespan, uint32_t *const *normalOut,
                  float *const *tOut, uint64_t *const *leafOut);
+
+// ---- live request server (server.hip) ----
+int server_create(svo_octree *const *trees, int ntrees, const svo_server_config *cfg, svo_server **out);
+void server_destroy(svo_server *s);
+int server_submit(svo_server *s, const svo_request *r, uint64_t *ticket);
+int server_poll(svo_server *s, uint64_t ticket, int *state, double *latency);
+int server_wait(svo_server *s, uint64_t ticket, double timeout, double *latency);
+int server_outputs(svo_server *s, uint64_t ticket, uint32_t **normal, float **t, uint64_t **leaf);
+int server_copy_outputs(svo_server *s, uint64_t ticket, uint32_t *normal, float *t, uint64_t *leaf);
+int server_release(svo_server *s, uint64_t ticket);
+int server_stats(svo_server *s, svo_server_stats *out);
+int server_run_schedule(svo_server *s, const svo_request *reqs, const double *arrival, int n, uint64_t *tickets,
+                        double *latency, double *makespan);
 
 // ---- construction (build.hip) ----
 int build_mesh(const float *verts, int64_t nverts, int has_normals, const int32_t *face_counts,

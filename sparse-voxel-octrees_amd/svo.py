@@ -35,6 +35,9 @@ EXPORTS = [
     "svo_build_from_terrain", "svo_terrain_volume", "svo_terrain_heights",
     "svo_render_shadow", "svo_render_shadow_device", "svo_decode_words",
     "svo_renderer_autotune", "svo_batch_graph_create", "svo_batch_graph_launch", "svo_batch_graph_free",
+    "svo_server_create", "svo_server_submit", "svo_server_poll", "svo_server_wait", "svo_server_outputs",
+    "svo_server_copy_outputs", "svo_server_release", "svo_server_stats_get", "svo_server_run_schedule",
+    "svo_server_destroy",
     "svo_build_stats_get",
 ]
 
@@ -60,19 +63,25 @@ BATCH_GRID, BATCH_PERSISTENT = 0, 1
 
 class BatchOpts(ctypes.Structure):
     _fields_ = [("waves_per_cu", ctypes.c_int), ("refill_threshold", ctypes.c_int), ("form", ctypes.c_int),
-                ("reserved", ctypes.c_int * 5)]
+                ("tile_w", ctypes.c_int), ("reserved", ctypes.c_int * 4)]
 
 
-def _batch_opts(waves_per_cu, refill_threshold, form):
+def _batch_opts(waves_per_cu, refill_threshold, form, tile_w=0):
     """Options of a batched launch: the grid form unless a form or a
-    persistent launch shape is given."""
+    persistent launch shape is given; tile_w picks the grid form's tile shape
+    (tile_w x 64/tile_w pixels; 0 = 8x8)."""
     o = BatchOpts()
     o.waves_per_cu = int(waves_per_cu)
     o.refill_threshold = int(refill_threshold)
     if form is None:
         form = BATCH_PERSISTENT if (waves_per_cu or refill_threshold) else BATCH_GRID
     o.form = int(form)
+    o.tile_w = int(tile_w)
     return o
+
+
+AUTOTUNE_SHAPES = ["grid8x8", "grid16x4", "grid4x16", "grid32x2"] + [f"{w}x{r}" for w in (16, 24, 32)
+                                                                       for r in (16, 32, 64)]
 
 
 class BuildStats(ctypes.Structure):
@@ -129,11 +138,11 @@ class Request(ctypes.Structure):
 class ServerOpts(ctypes.Structure):
     _fields_ = [("streams", ctypes.c_int), ("output_budget", ctypes.c_uint64), ("max_batch", ctypes.c_int),
                 ("one_per_launch", ctypes.c_int), ("waves_per_cu", ctypes.c_int), ("refill_threshold", ctypes.c_int),
-                ("batch_form", ctypes.c_int), ("reserved", ctypes.c_int * 5)]
+                ("batch_form", ctypes.c_int), ("device_outputs", ctypes.c_int), ("reserved", ctypes.c_int * 4)]
 
 
 def serve_replay(trees, requests, arrivals, streams=4, output_budget=1 << 30, max_batch=256, one_per_launch=False,
-                 waves_per_cu=0, refill_threshold=0, outputs=None, batch_form=None):
+                 waves_per_cu=0, refill_threshold=0, outputs=None, batch_form=None, device_outputs=False):
     """Request loop over a known arrival schedule (see svo_serve_replay).
     requests: list of (tree index, Camera, width, height). outputs: optional
     list of (normal, t, leaf) numpy arrays (or None) per request. Returns
@@ -150,6 +159,7 @@ def serve_replay(trees, requests, arrivals, streams=4, output_budget=1 << 30, ma
     o.streams, o.output_budget, o.max_batch = int(streams), int(output_budget), int(max_batch)
     o.one_per_launch, o.waves_per_cu, o.refill_threshold = int(bool(one_per_launch)), int(waves_per_cu), int(refill_threshold)
     o.batch_form = _batch_opts(waves_per_cu, refill_threshold, batch_form).form
+    o.device_outputs = int(bool(device_outputs))
     nout = tout = lout = None
     if outputs is not None:
         nout = (ctypes.c_void_p * n)(*[None if x is None else x[0].ctypes.data for x in outputs])
@@ -158,6 +168,102 @@ def serve_replay(trees, requests, arrivals, streams=4, output_budget=1 << 30, ma
     _check(lib.svo_serve_replay(tarr, len(trees), rarr, _p(arr), n, ctypes.byref(o), _p(lat), ctypes.byref(mk),
                                 nout, tout, lout))
     return lat, mk.value
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("output_budget", ctypes.c_uint64), ("ring_slots", ctypes.c_int), ("waves_per_cu", ctypes.c_int),
+                ("host_outputs", ctypes.c_int), ("idle_stop_us", ctypes.c_int), ("reserved", ctypes.c_int * 6)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("launches", ctypes.c_uint64), ("published", ctypes.c_uint64), ("in_flight", ctypes.c_uint64),
+                ("waiting", ctypes.c_uint64), ("live", ctypes.c_int), ("reserved", ctypes.c_int * 7)]
+
+
+REQ_WAITING, REQ_IN_FLIGHT, REQ_DONE = 0, 1, 2
+
+
+def _requests(requests):
+    rarr = (Request * len(requests))()
+    for i, (ti, cam, w, h) in enumerate(requests):
+        rarr[i].tree, rarr[i].camera, rarr[i].width, rarr[i].height = int(ti), cam, int(w), int(h)
+    return rarr
+
+
+class Server:
+    """Live request server (svo_server): requests submitted at any time are
+    folded into one running persistent launch. requests are (tree index,
+    Camera, width, height) over the octrees given here."""
+
+    def __init__(self, trees, output_budget=0, ring_slots=0, waves_per_cu=0, host_outputs=False, idle_stop_us=0):
+        self._trees = list(trees)
+        tarr = (ctypes.c_void_p * len(self._trees))(*[t.handle.value for t in self._trees])
+        c = ServerConfig()
+        c.output_budget, c.ring_slots, c.waves_per_cu = int(output_budget), int(ring_slots), int(waves_per_cu)
+        c.host_outputs, c.idle_stop_us = int(bool(host_outputs)), int(idle_stop_us)
+        self._s = ctypes.c_void_p()
+        _check(lib.svo_server_create(tarr, len(self._trees), ctypes.byref(c), ctypes.byref(self._s)))
+
+    def submit(self, tree, camera, width, height):
+        r = Request()
+        r.tree, r.camera, r.width, r.height = int(tree), camera, int(width), int(height)
+        t = ctypes.c_uint64()
+        _check(lib.svo_server_submit(self._s, ctypes.byref(r), ctypes.byref(t)))
+        return t.value
+
+    def poll(self, ticket):
+        """(state, latency seconds or 0)."""
+        st, lat = ctypes.c_int(), ctypes.c_double()
+        _check(lib.svo_server_poll(self._s, int(ticket), ctypes.byref(st), ctypes.byref(lat)))
+        return st.value, lat.value
+
+    def wait(self, ticket, timeout=0.0):
+        lat = ctypes.c_double()
+        _check(lib.svo_server_wait(self._s, int(ticket), float(timeout), ctypes.byref(lat)))
+        return lat.value
+
+    def outputs(self, ticket):
+        """(normal, t, leaf) pointers (ints) of a completed request."""
+        n, t, l = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        _check(lib.svo_server_outputs(self._s, int(ticket), ctypes.byref(n), ctypes.byref(t), ctypes.byref(l)))
+        return n.value, t.value, l.value
+
+    def copy_outputs(self, ticket, width, height):
+        """Host copies {"normal", "t", "leaf"} of a completed request."""
+        n = int(width) * int(height)
+        out = {"normal": np.zeros(n, np.uint32), "t": np.zeros(n, np.float32), "leaf": np.zeros(n, np.uint64)}
+        _check(lib.svo_server_copy_outputs(self._s, int(ticket), _p(out["normal"]), _p(out["t"]), _p(out["leaf"])))
+        return out
+
+    def release(self, ticket):
+        _check(lib.svo_server_release(self._s, int(ticket)))
+
+    def stats(self):
+        st = ServerStats()
+        _check(lib.svo_server_stats_get(self._s, ctypes.byref(st)))
+        return {k: getattr(st, k) for k in ("launches", "published", "in_flight", "waiting", "live")}
+
+    def run_schedule(self, requests, arrivals, keep=False):
+        """Submits request i at arrivals[i] seconds; returns (latencies,
+        makespan, tickets or None). With keep the outputs stay until released."""
+        n = len(requests)
+        rarr = _requests(requests)
+        arr = np.ascontiguousarray(arrivals, dtype=np.float64)
+        lat = np.zeros(n, np.float64)
+        mk = ctypes.c_double()
+        tk = np.zeros(n, np.uint64) if keep else None
+        _check(lib.svo_server_run_schedule(self._s, rarr, _p(arr), n, _p(tk), _p(lat), ctypes.byref(mk)))
+        return lat, mk.value, tk
+
+    def close(self):
+        if lib is None:  # interpreter shutdown
+            return
+        if getattr(self, "_s", None) and self._s.value:
+            lib.svo_server_destroy(self._s)
+            self._s = ctypes.c_void_p()
+
+    def __del__(self):
+        self.close()
 
 
 class BuildOpts(ctypes.Structure):
@@ -214,12 +320,18 @@ def _load():
         "svo_batch_graph_free": [P],
         "svo_render_shadow": [P, P, ctypes.c_int, ctypes.c_int, P, F, P, P, P, P, P],
         "svo_render_shadow_device": [P, P, ctypes.c_int, ctypes.c_int, P, F, P, P, P, P, P, P],
+        "svo_server_create": [P, ctypes.c_int, P, P], "svo_server_submit": [P, P, P],
+        "svo_server_poll": [P, U64, P, P], "svo_server_wait": [P, U64, ctypes.c_double, P],
+        "svo_server_outputs": [P, U64, P, P, P], "svo_server_release": [P, U64],
+        "svo_server_copy_outputs": [P, U64, P, P, P],
+        "svo_server_stats_get": [P, P], "svo_server_run_schedule": [P, P, P, ctypes.c_int, P, P, P],
+        "svo_server_destroy": [P],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
         fn.argtypes = args
         fn.restype = None if name in ("svo_free", "svo_free_buffer", "svo_renderer_free",
-                                      "svo_batch_graph_free") else ctypes.c_int
+                                      "svo_batch_graph_free", "svo_server_destroy") else ctypes.c_int
     L.svo_frame_depth_count.restype = I64
     return L
 
@@ -558,33 +670,34 @@ class Renderer:
         v.hit, v.normal, v.t, v.leaf = hit, normal, t, leaf
         return v
 
-    def render_device(self, views, stream=None, waves_per_cu=0, refill_threshold=0, form=None):
+    def render_device(self, views, stream=None, waves_per_cu=0, refill_threshold=0, form=None, tile_w=0):
         arr = (View * len(views))(*views)
-        o = _batch_opts(waves_per_cu, refill_threshold, form)
+        o = _batch_opts(waves_per_cu, refill_threshold, form, tile_w)
         _check(lib.svo_render_batch_device(self._r, arr, len(views), ctypes.byref(o), stream))
 
     def autotune(self, views, stream=None):
-        """((form, waves_per_cu, refill), {"grid": ms, "WxR": ms, ...}) of the
-        fastest launch shape (waves and refill are 0 for the grid form)."""
+        """((form, waves_per_cu, refill, tile_w), {"grid8x8": ms, ..., "WxR": ms})
+        of the fastest launch shape (waves and refill are 0 for the grid form,
+        tile_w 8 for the persistent form)."""
         arr = (View * len(views))(*views)
         best = BatchOpts()
-        times = (ctypes.c_float * 10)()
+        times = (ctypes.c_float * len(AUTOTUNE_SHAPES))()
         _check(lib.svo_renderer_autotune(self._r, arr, len(views), stream, ctypes.byref(best), times))
-        shapes = ["grid"] + [f"{w}x{r}" for w in (16, 24, 32) for r in (16, 32, 64)]
-        return (best.form, best.waves_per_cu, best.refill_threshold), {k: round(times[i], 3) for i, k in enumerate(shapes)}
+        return ((best.form, best.waves_per_cu, best.refill_threshold, best.tile_w or 8),
+                {k: round(times[i], 3) for i, k in enumerate(AUTOTUNE_SHAPES)})
 
-    def capture(self, views, stream=None, waves_per_cu=0, refill_threshold=0, form=None):
+    def capture(self, views, stream=None, waves_per_cu=0, refill_threshold=0, form=None, tile_w=0):
         """BatchGraph replaying one batched launch of these views."""
-        return BatchGraph(self, views, stream, waves_per_cu, refill_threshold, form)
+        return BatchGraph(self, views, stream, waves_per_cu, refill_threshold, form, tile_w)
 
 
 class BatchGraph:
     """A batched launch captured in a HIP graph (svo_batch_graph)."""
 
-    def __init__(self, renderer, views, stream=None, waves_per_cu=0, refill_threshold=0, form=None):
+    def __init__(self, renderer, views, stream=None, waves_per_cu=0, refill_threshold=0, form=None, tile_w=0):
         self._g = ctypes.c_void_p()
         arr = (View * len(views))(*views)
-        o = _batch_opts(waves_per_cu, refill_threshold, form)
+        o = _batch_opts(waves_per_cu, refill_threshold, form, tile_w)
         _check(lib.svo_batch_graph_create(renderer._r, arr, len(views), ctypes.byref(o), stream, ctypes.byref(self._g)))
 
     def launch(self, stream=None):

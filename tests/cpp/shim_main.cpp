@@ -2,9 +2,11 @@
 // uses VoxelOctree (load, center, save; build from PlyLoader/VoxelData;
 // raymarch). Modes:
 //   shim_main cpu <in.oct> <out.oct>            load + center + save, error path
-//   shim_main gpu <in.oct> <rays.f32> <out.bin> per-ray raymarch of n x 7 floats
+//   shim_main rays <in.oct> <rays.f32> <out.bin> per-ray raymarch of n x 7 floats;
+//                                              prints the calls per second
 //   shim_main build <in.ply> <res> <threads> <mem> <out.oct>
 //   shim_main frame <in.oct> <pitch> <yaw> <radius> <w> <h> <stride> <out.rgba>
+#include <chrono>
 #include <cstdio>
 #include <memory>
 #include <cstdlib>
@@ -29,7 +31,7 @@ int main(int argc, char **argv) {
         }
         return 0;
     }
-    if (mode == "gpu" && argc == 5) {
+    if (mode == "rays" && argc == 5) {
         VoxelOctree tree(argv[2]);
         FILE *fp = std::fopen(argv[3], "rb");
         std::vector<float> rays;
@@ -37,7 +39,22 @@ int main(int argc, char **argv) {
         while (std::fread(buf, 4, 7, fp) == 7) rays.insert(rays.end(), buf, buf + 7);
         std::fclose(fp);
         std::vector<uint32_t> out;
-        for (size_t i = 0; i < rays.size() / 7; ++i) {
+        const size_t n = rays.size() / 7;
+        // the reference's per-pixel call pattern, timed over at least 2^20 calls
+        size_t calls = 0;
+        uint32_t sink = 0;
+        const auto t0 = std::chrono::steady_clock::now();
+        while (calls < (size_t(1) << 20)) {
+            for (size_t i = 0; i < n; ++i, ++calls) {
+                const float *r = &rays[i * 7];
+                uint32_t normal = 0;
+                float t = 0.0f;
+                sink += tree.raymarch(Vec3(r[0], r[1], r[2]), Vec3(r[3], r[4], r[5]), r[6], normal, t) ? normal : 0u;
+            }
+        }
+        const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        std::printf("calls_per_s %.0f calls %zu sink %u\n", double(calls) / sec, calls, sink);
+        for (size_t i = 0; i < n; ++i) {
             const float *r = &rays[i * 7];
             uint32_t normal = 0;
             float t = 0.0f;

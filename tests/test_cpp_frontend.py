@@ -1,8 +1,9 @@
 """The drop-in C++ front end (include/VoxelOctree.hpp) compiled with g++ into
 a caller written like the reference's Main.cpp, linked against
-libsvo_hip.so: .oct load/center/save byte-identical and errors surfaced
-(CPU); per-ray raymarch equal to the reference's goldens and the GPU build
-equal to the reference's .oct (GPU)."""
+libsvo_hip.so: .oct load/center/save byte-identical and errors surfaced,
+per-ray raymarch equal to the reference's goldens (CPU: single rays are
+walked on the calling thread); the GPU build equal to the reference's .oct,
+the viewer frame and the on-disk builder mode (GPU)."""
 import json
 import os
 import subprocess
@@ -31,18 +32,27 @@ def test_frontend_load_save_and_errors(shim_bin, golden, tmp_path):
     assert out.read_bytes() == open(os.path.join(golden, "build_ico10_r64_t8_m1073741824.oct"), "rb").read()
 
 
-@pytest.mark.gpu
-def test_frontend_raymarch_matches_reference(shim_bin, golden, tmp_path):
-    g = np.load(os.path.join(golden, "raymarch_dragon.npz"))
-    rays = g["rays"][:3000]
+@pytest.mark.parametrize("name", ["dragon", "sdf128", "extremes"])
+def test_frontend_raymarch_matches_reference(shim_bin, golden, tmp_path, name):
+    """VoxelOctree::raymarch called once per ray, as Main.cpp's tile loop
+    calls it (src/Main.cpp:118,181): hit, normal and t equal the reference's
+    outputs on its own golden rays, at >= 1 M calls/s on one thread."""
+    g = np.load(os.path.join(golden, f"raymarch_{name}.npz"))
+    tree = "build_sdf128.oct" if name == "sdf128" else "dragon.oct"
+    rays = g["rays"]
     rays.tofile(tmp_path / "rays.f32")
-    print("[gpu] C++ front end per-ray raymarch, 3000 rays", flush=True)
-    subprocess.run([shim_bin, "gpu", os.path.join(golden, "dragon.oct"), str(tmp_path / "rays.f32"),
-                    str(tmp_path / "out.bin")], check=True, timeout=120)
+    r = subprocess.run([shim_bin, "rays", os.path.join(golden, tree), str(tmp_path / "rays.f32"),
+                        str(tmp_path / "out.bin")], check=True, timeout=120, capture_output=True, text=True)
     out = np.fromfile(tmp_path / "out.bin", np.uint32).reshape(-1, 3)
-    hit = g["hit"][:3000] != 0
+    hit = g["hit"] != 0
     assert np.array_equal(out[:, 0] != 0, hit)
-    assert np.array_equal(out[hit, 2], g["t"][:3000][hit].view(np.uint32))
+    leaf = g["hit"] == 1
+    assert np.array_equal(out[leaf, 1], g["normal"][leaf])
+    assert np.array_equal(out[hit, 2], g["t"][hit].view(np.uint32))
+    rate = float(r.stdout.split()[1])
+    print(f"{name}: {rate / 1e6:.2f} M calls/s", flush=True)
+    if name != "extremes":
+        assert rate >= 1e6, rate
 
 
 @pytest.mark.gpu

@@ -351,20 +351,47 @@ def test_autotune_picks_a_shape_and_keeps_results(svo, trees):
     ren = svo.svo.Renderer(4)
     outs, views = _batch_setup(svo, trees, ren, dev, specs)
     print("[gpu] batch autotune", flush=True)
-    (form, w, r), times = ren.autotune(views)
+    (form, w, r, tw), times = ren.autotune(views)
     torch.cuda.synchronize()
-    assert len(times) == 10 and all(v > 0 for v in times.values())
+    assert len(times) == 13 and all(v > 0 for v in times.values())
     if form == svo.svo.BATCH_GRID:
-        assert w == 0 and r == 0 and times["grid"] == min(times.values())
+        assert w == 0 and r == 0 and tw in (4, 8, 16, 32)
+        assert times[f"grid{tw}x{64 // tw}"] == min(times.values())
     else:
-        assert form == svo.svo.BATCH_PERSISTENT and w in (16, 24, 32) and r in (16, 32, 64)
+        assert form == svo.svo.BATCH_PERSISTENT and w in (16, 24, 32) and r in (16, 32, 64) and tw == 8
         assert times[f"{w}x{r}"] == min(times.values())
-    ren.render_device(views, None, w, r, form)
+    ren.render_device(views, None, w, r, form, tw)
     torch.cuda.synchronize()
     with pytest.raises(svo.SvoError) as e:  # no such launch form
         ren.render_device(views, None, 0, 0, 7)
     assert e.value.code == svo.svo.ERR_INVALID
+    with pytest.raises(svo.SvoError) as e:  # no such tile shape
+        ren.render_device(views, None, 0, 0, svo.svo.BATCH_GRID, 12)
+    assert e.value.code == svo.svo.ERR_INVALID
     _check_batch(outs, [t.render_primary(cam, ww, hh) for t, cam, ww, hh, _, _ in outs])
+
+
+@pytest.mark.parametrize("tile_w", [4, 16, 32])
+def test_grid_tile_shapes_match_plain_kernel(svo, trees, tile_w):
+    """The grid form with 4x16, 16x4 and 32x2 tiles (edge tiles, views below
+    one tile, LOD scale) equals the plain 8x8 kernel bit for bit."""
+    import torch
+    dev = torch.device("cuda", 0)
+    specs = [("dragon", 320, 180, 20.0, 35.0, 0.0), ("sdf128", 257, 131, -30.0, 200.0, 0.0),
+             ("dragon", 7, 5, 10.0, 10.0, 0.0), ("dragon", 200, 100, -60.0, 90.0, 0.02)]
+    ren = svo.svo.Renderer(8)
+    outs, views = _batch_setup(svo, trees, ren, dev, specs)
+    print(f"[gpu] grid form tile width {tile_w}", flush=True)
+    ren.render_device(views, None, 0, 0, svo.svo.BATCH_GRID, tile_w)
+    torch.cuda.synchronize()
+    refs = []
+    for t, cam, w, h, rs, o in outs:
+        if rs == 0.0:
+            refs.append(t.render_primary(cam, w, h))
+        else:
+            d = orc.primary_dirs(np.frombuffer(cam.tform, np.float32), w, h)
+            refs.append(orc.raymarch(t.words(), np.frombuffer(cam.pos, np.float32)[None, :], d, rs))
+    _check_batch(outs, refs)
 
 
 @pytest.mark.parametrize("one_per_launch,streams,budget,form", [(False, 4, 1 << 28, 0), (False, 2, 1 << 24, 0),
