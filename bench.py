@@ -62,7 +62,7 @@ def build_roofline(st):
             "leaf_voxels_per_s": round(st["leaves"] / sec, 1) if sec > 0 else None}
 
 
-def make_workload(pkg, name, cache_dir):
+def make_workload(pkg, name, cache_dir, no_cpu=False):
     if name == "dragon":
         tree = pkg.VoxelOctree(os.path.join(ROOT, "tests", "golden", "dragon.oct"))
         return tree, {"workload": "dragon256_8x1080p", "octree": "reference dragon 256^3",
@@ -88,13 +88,67 @@ def make_workload(pkg, name, cache_dir):
         with open(os.path.join(ROOT, "tests", "golden", "builds.json")) as fp:
             ref = json.load(fp)["ico2048"]
         md5 = hashlib.md5(tree.encode()).hexdigest()
+        # the reference's own builder on this host, same PLY, partition (8 threads) and budget
+        model, ncpu = host_cpu()
+        rb = reference_build(ply, 2048, 8, 1 << 30, cache_dir) if not no_cpu else None
+        build_cpu = None
+        if rb is not None:
+            total = rb["parse"] + rb["build"] + rb["save"]
+            build_cpu = {"value": round(ref["leaves"] / (rb["parse"] + rb["build"]) / 1e6, 3), "unit": "M leaf voxels/s",
+                         "cores": rb["threads"], "kind": "reference", "cpu_model": model, "host_cpu_count": ncpu,
+                         "seconds_without_save": round(rb["parse"] + rb["build"], 3),
+                         "seconds_with_lz4_save": round(total, 3),
+                         "phases_s": {"ply_parse": rb["parse"], "voxelize_and_build": rb["build"], "lz4_save": rb["save"]},
+                         "oct_md5_equals_gpu_build": rb["md5"] == md5,
+                         "sample": "the whole 2048^3 build: oracle/_ref/svo_ref build-ply (the reference's PlyLoader, "
+                                   "VoxelData, VoxelOctree, save) with the 8-thread partition and a 1 GiB budget",
+                         "gpu_speedup_without_save": round((rb["parse"] + rb["build"]) / warm_s, 1)}
         return tree, {"workload": "icosphere2048_8x1080p", "octree": "GPU-built 2048^3 displaced icosphere",
                       "triangles": 1997120, "resolution": 2048, "build_seconds": round(build_s, 3),
                       "build_leaf_voxels_per_s": round(ref["leaves"] / build_s, 1),
                       "build_seconds_second_build": round(warm_s, 3),
                       "build_roofline": build_roofline(bst),
-                      "build_oct_md5_matches_reference": md5 == ref["md5"]}
+                      "build_oct_md5_matches_reference": md5 == ref["md5"],
+                      "build_cpu_baseline": build_cpu}
     raise SystemExit(f"unknown workload {name}")
+
+
+def host_cpu():
+    """(model name, os.cpu_count()) of this host."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
+def reference_build(ply, resolution, threads, mem, cache_dir):
+    """The reference builder's in-memory mode (oracle/_ref/svo_ref build-ply:
+    PlyLoader -> VoxelData -> VoxelOctree -> save, src/Main.cpp:322-327) on
+    `threads` host threads, each phase timed: {"parse", "build", "save",
+    "md5", "threads"}, or None when that build is not present."""
+    import hashlib
+    import subprocess
+    exe = os.path.join(ROOT, "oracle", "_ref", "svo_ref")
+    if not os.access(exe, os.X_OK):
+        return None
+    os.makedirs(cache_dir, exist_ok=True)
+    out = os.path.join(cache_dir, "ref_build.oct")
+    try:
+        r = subprocess.run([exe, "build-ply", ply, str(resolution), str(threads), str(mem), out], capture_output=True,
+                           text=True, timeout=900)
+    except (OSError, subprocess.SubprocessError):
+        return None
+    for line in r.stdout.splitlines():
+        if line.startswith("build_seconds"):
+            f = line.split()
+            return {"parse": float(f[2]), "build": float(f[4]), "save": float(f[6]), "threads": int(f[8]),
+                    "md5": hashlib.md5(open(out, "rb").read()).hexdigest()}
+    return None
 
 
 def reference_rays(tree, rays7, threads, cache_dir):
@@ -163,8 +217,8 @@ def run_mixed(pkg, torch, stream, steps, warmup, waves_per_cu, refill):
             trees[v["tree"]].render_primary_device(cams[i], v["width"], v["height"], None, outs[i][0].data_ptr(),
                                                    outs[i][1].data_ptr(), outs[i][2].data_ptr(), sptr)
 
-    def batched(w=waves_per_cu, r=refill, f=None):
-        ren.render_device(bviews, sptr, w, r, f)
+    def batched(w=waves_per_cu, r=refill, f=None, tw=0):
+        ren.render_device(bviews, sptr, w, r, f, tw)
 
     def timed(fn, k):
         for _ in range(warmup):
@@ -181,11 +235,15 @@ def run_mixed(pkg, torch, stream, steps, warmup, waves_per_cu, refill):
     # launch form and shape: the library's autotuner on this workload
     tune = {}
     form = None
+    tile_w = 0
     if not waves_per_cu and not refill:
-        (form, waves_per_cu, refill), tune = ren.autotune(bviews, sptr)
+        (form, waves_per_cu, refill, tile_w), tune = ren.autotune(bviews, sptr)
     plain_ms = timed(plain, steps)
-    batched_ms = timed(lambda: batched(waves_per_cu, refill, form), steps)
-    # parity: the batched launch reproduces the plain launches bit for bit
+    batched_ms = timed(lambda: batched(waves_per_cu, refill, form, tile_w), steps)
+    # the same launch captured once in a HIP graph and replayed
+    graph = ren.capture(bviews, sptr, waves_per_cu, refill, form, tile_w)
+    graph_ms = timed(lambda: graph.launch(sptr), steps)
+    # parity: the graph-replayed batched launch reproduces the plain launches bit for bit
     plain()
     torch.cuda.synchronize()
     snap = [(a.clone(), b.clone(), c.clone()) for a, b, c in outs]
@@ -193,8 +251,9 @@ def run_mixed(pkg, torch, stream, steps, warmup, waves_per_cu, refill):
         a.fill_(-1)
         b.fill_(-1.0)
         c.fill_(-1)
-    batched(waves_per_cu, refill, form)
+    graph.launch(sptr)
     torch.cuda.synchronize()
+    graph.close()
     mism = sum(int((a != sa).sum() + (b.view(torch.int32) != sb.view(torch.int32)).sum() + (c != sc).sum())
                for (a, b, c), (sa, sb, sc) in zip(outs, snap))
     run_mixed.trees = trees
@@ -202,8 +261,11 @@ def run_mixed(pkg, torch, stream, steps, warmup, waves_per_cu, refill):
             "plain_grays_s": round(rays / (plain_ms / 1e3) / 1e9, 4), "plain_ms_per_step": round(plain_ms, 3),
             "batched_grays_s": round(rays / (batched_ms / 1e3) / 1e9, 4), "batched_ms_per_step": round(batched_ms, 3),
             "batched_vs_plain": round(plain_ms / batched_ms, 3),
+            "graph_replay": {"grays_s": round(rays / (graph_ms / 1e3) / 1e9, 4), "ms_per_step": round(graph_ms, 3),
+                             "vs_plain": round(plain_ms / graph_ms, 3),
+                             "form": "the autotuned batched launch captured in one HIP graph, replayed per step"},
             "launch": {"form": "grid" if form == pkg.svo.BATCH_GRID else "persistent", "waves_per_cu": waves_per_cu,
-                       "refill": refill, "autotune_ms": tune},
+                       "refill": refill, "tile": f"{tile_w or 8}x{64 // (tile_w or 8)}", "autotune_ms": tune},
             "mismatched_fields_vs_plain": mism}
 
 
@@ -440,14 +502,18 @@ def reference_frame(tree, angle, width, height, threads, cache_dir):
     return None
 
 
-def run_served(pkg, trees, rates, seconds, seed=2000):
-    """Served stream: Poisson request streams (seeded) over the 64 resident
-    octrees, each request (octree U{0..63}, seeded orbit camera, resolution as
-    the mixed views). For each mean rate, the continuous-batching loop and the
-    one-request-per-launch baseline replay the same schedule; a final burst
-    (every request at t=0) measures saturated throughput."""
+def run_served(pkg, trees, rates, seconds, seed=2000, full=False):
+    """Served stream over the 64 resident octrees: seeded Poisson request
+    streams, each request (octree U{0..63}, seeded orbit camera, resolution as
+    the mixed views). For each mean rate the live server (svo_server_*: one
+    running persistent launch, requests folded in as they arrive, outputs
+    left in device memory) and the one-request-per-launch baseline (plain
+    kernel, outputs in device memory) replay the same schedule; latency is
+    scheduled arrival -> completion seen on the host. Requests per rate:
+    min(5000, rate * seconds), or 5000 with `full`. A final burst (every
+    request at t=0) measures saturated throughput."""
     rng = np.random.default_rng(seed)
-    out = {"rates": []}
+    out = {"rates": [], "latency": "scheduled arrival -> completion seen by the host (outputs resident in HBM)"}
 
     def make(n):
         reqs = []
@@ -462,20 +528,30 @@ def run_served(pkg, trees, rates, seconds, seed=2000):
                 "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
                 "grays_s": round(rays / mk / 1e9, 4), "requests": len(reqs)}
 
+    srv = pkg.svo.Server(trees, output_budget=4 << 30)
+    # first use of the serving kernel, outside the measurements
+    warm = make(8)
+    srv.run_schedule(warm, np.zeros(len(warm)))
     for lam in rates:
-        n = int(max(50, min(5000, lam * seconds)))
+        n = 5000 if full else int(max(200, min(5000, lam * seconds)))
         reqs = make(n)
         arr = np.cumsum(rng.exponential(1.0 / lam, n))
         arr -= arr[0]
-        lat_b, mk_b = pkg.svo.serve_replay(trees, reqs, arr, streams=4)
-        lat_p, mk_p = pkg.svo.serve_replay(trees, reqs, arr, streams=1, one_per_launch=True)
-        out["rates"].append({"req_per_s": lam, "batched": stats(lat_b, mk_b, reqs), "one_per_launch": stats(lat_p, mk_p, reqs)})
+        lat_s, mk_s, _ = srv.run_schedule(reqs, arr)
+        lat_p, mk_p = pkg.svo.serve_replay(trees, reqs, arr, streams=1, one_per_launch=True, device_outputs=True)
+        out["rates"].append({"req_per_s": lam, "live_server": stats(lat_s, mk_s, reqs),
+                             "one_per_launch": stats(lat_p, mk_p, reqs)})
     reqs = make(1000)
     arr = np.zeros(len(reqs))
-    lat_b, mk_b = pkg.svo.serve_replay(trees, reqs, arr, streams=4)
-    lat_p, mk_p = pkg.svo.serve_replay(trees, reqs, arr, streams=1, one_per_launch=True)
-    b, p = stats(lat_b, mk_b, reqs), stats(lat_p, mk_p, reqs)
-    out["burst"] = {"batched": b, "one_per_launch": p, "batched_vs_one_per_launch": round(b["grays_s"] / p["grays_s"], 3)}
+    lat_s, mk_s, _ = srv.run_schedule(reqs, arr)
+    st = srv.stats()
+    srv.close()
+    lat_p, mk_p = pkg.svo.serve_replay(trees, reqs, arr, streams=1, one_per_launch=True, device_outputs=True)
+    lat_r, mk_r = pkg.svo.serve_replay(trees, reqs, arr, streams=4, device_outputs=True)
+    s_, p_, r_ = stats(lat_s, mk_s, reqs), stats(lat_p, mk_p, reqs), stats(lat_r, mk_r, reqs)
+    out["burst"] = {"live_server": s_, "one_per_launch": p_, "relaunch_batched_4_streams": r_,
+                    "live_vs_one_per_launch": round(s_["grays_s"] / p_["grays_s"], 3)}
+    out["server_launches"] = st["launches"]
     return out
 
 
@@ -497,8 +573,9 @@ def main():
     ap.add_argument("--refill", type=int, default=0, help="batched form: idle lanes that trigger a refill (0 = default)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the 64-octree / 256-view batched workload")
     ap.add_argument("--no-serve", action="store_true", help="skip the served (request stream) workload")
-    ap.add_argument("--serve-rates", default="100,500,1000,2000", help="mean request rates (req/s) to replay")
-    ap.add_argument("--serve-seconds", type=float, default=2.0, help="schedule length per rate")
+    ap.add_argument("--serve-rates", default="100,200,500,1000,2000", help="mean request rates (req/s) to replay")
+    ap.add_argument("--serve-seconds", type=float, default=4.0, help="schedule length per rate (at most 5000 requests)")
+    ap.add_argument("--serve-full", action="store_true", help="5000 requests at every rate")
     ap.add_argument("--no-terrain", action="store_true", help="skip the 8192^3 terrain + shadow-ray workload")
     ap.add_argument("--no-frames", action="store_true", help="skip the viewer-frame workload")
     ap.add_argument("--only", choices=["terrain", "frames"], default=None,
@@ -519,7 +596,7 @@ def main():
         print(json.dumps(run_terrain(pkg, torch, torch.cuda.current_stream(), args.steps, args.warmup,
                                      args.cpu_threads, args.cache)), flush=True)
         return
-    tree, config = make_workload(pkg, args.workload, args.cache)
+    tree, config = make_workload(pkg, args.workload, args.cache, no_cpu=args.no_cpu)
     tree.upload()
     W, H, V = args.width, args.height, args.views
     n = W * H
@@ -631,7 +708,8 @@ def main():
         mixed = run_mixed(pkg, torch, stream, max(1, min(args.steps, 5)), 1, args.waves_per_cu, args.refill)
     served = None
     if not args.no_mixed and not args.no_serve:
-        served = run_served(pkg, run_mixed.trees, [float(x) for x in args.serve_rates.split(",")], args.serve_seconds)
+        served = run_served(pkg, run_mixed.trees, [float(x) for x in args.serve_rates.split(",")], args.serve_seconds,
+                            full=args.serve_full)
 
     frames = None
     if not args.no_frames:
@@ -687,19 +765,33 @@ def main():
     cpu_baseline = {"value": round(port_rate, 6), "unit": "Grays/s", "cores": args.cpu_threads, "kind": "port",
                     "sample": f"{cpu_rays} primary rays ({cpu_rays // n} of the {V} views, full {W}x{H}) through "
                               f"the oracle raymarch"}
-    ref = reference_cpu_rate(tree, cams[0], W, H, args.cpu_threads, args.cache)
+    # every view through the reference's own raymarch, timed on this host
+    ref_views, ref_mism, ref_sec, ref_rays = 0, 0, 0.0, 0
+    for v in range(V):
+        ref = reference_cpu_rate(tree, cams[v], W, H, args.cpu_threads, args.cache)
+        if ref is None:
+            break
+        # normal and t bits of every ray (0 where a miss leaves them unwritten)
+        g_nrm = normal[v].cpu().numpy().view(np.uint32)
+        g_t = tbuf[v].cpu().numpy().view(np.uint32)
+        ref_mism += int(np.count_nonzero(g_nrm != ref["out"][:, 1]) + np.count_nonzero(g_t != ref["out"][:, 2]))
+        ref_sec += ref["rays"] / (ref["rate"] * 1e9)
+        ref_rays += ref["rays"]
+        ref_views += 1
+        ref_threads = ref["threads"]
     ref_parity = None
-    if ref is not None:
-        # view 0 against the reference itself: normal and t bits of every ray (0 where a miss leaves them unwritten)
-        g_nrm = normal[0].cpu().numpy().view(np.uint32)
-        g_t = tbuf[0].cpu().numpy().view(np.uint32)
-        ref_parity = {"rays": int(len(g_nrm)), "mismatched_fields": int(np.count_nonzero(g_nrm != ref["out"][:, 1])
-                                                                        + np.count_nonzero(g_t != ref["out"][:, 2]))}
-        mismatches += ref_parity["mismatched_fields"]
-        cpu_baseline = {"value": round(ref["rate"], 6), "unit": "Grays/s", "cores": ref["threads"], "kind": "reference",
-                        "sample": f"{ref['rays']} primary rays (view 0, {W}x{H}) through the reference's own "
-                                  f"VoxelOctree::raymarch (oracle/_ref/svo_ref, {ref['threads']} threads, raymarch loop "
-                                  f"only); the oracle port on all {cpu_rays} rays: {port_rate:.6f} Grays/s"}
+    model, ncpu = host_cpu()
+    cpu_baseline["cpu_model"], cpu_baseline["host_cpu_count"] = model, ncpu
+    if ref_views:
+        ref_parity = {"rays": ref_rays, "views": ref_views, "mismatched_fields": ref_mism}
+        mismatches += ref_mism
+        cpu_baseline = {"value": round(ref_rays / ref_sec / 1e9, 6), "unit": "Grays/s", "cores": ref_threads,
+                        "kind": "reference", "cpu_model": model, "host_cpu_count": ncpu,
+                        "sample": f"{ref_rays} primary rays (all {ref_views} views, {W}x{H}) through the reference's "
+                                  f"own VoxelOctree::raymarch (oracle/_ref/svo_ref, {ref_threads} threads = this "
+                                  f"host's CPU share for one GPU, 16 = the reference viewer's NumThreads, "
+                                  f"src/Main.cpp:57; raymarch loop only); the oracle port on all {cpu_rays} rays: "
+                                  f"{port_rate:.6f} Grays/s"}
 
     out = {
         "metric": METRIC,
