@@ -55,6 +55,11 @@ const char *svo_last_error(void);
 int svo_version(void);
 /* Number of visible HIP devices (0 when none); never fails. */
 int svo_device_count(void);
+/* Loads all of the library's device code onto the current device now. The
+ * HIP runtime otherwise loads a source file's kernels at the first launch of
+ * one of them, which makes the first build of a process (the builder's radix
+ * sorts included) take ~0.1 s longer than the next. Optional. */
+int svo_load_device_code(void);
 
 /* ---- .oct files --------------------------------------------------------- */
 
@@ -143,6 +148,24 @@ int svo_render_primary(svo_octree *h, const svo_camera *cam, int width, int heig
 int svo_render_primary_device(svo_octree *h, const svo_camera *cam, int width, int height,
                               uint8_t *hit, uint32_t *normal, float *t, uint64_t *leaf,
                               void *stream);
+
+/* Cost-ordered primary renders (no counterpart in the reference): a tile plan
+ * keeps, on the device, each 8x8 tile's cost in the last render of a view
+ * size (its largest loop-pass count) and the tile order those costs give.
+ * svo_render_primary_planned_device traces every tile as
+ * svo_render_primary_device does (same rays, same results) but starts the
+ * tiles in the plan's order, most expensive first, records this render's
+ * costs and then builds the next order (a one-workgroup kernel), so a view
+ * rendered again (the next frame) starts its slow tiles first instead of
+ * leaving them to run alone at the end of the launch. The first render of a
+ * plan runs in raster order. One plan per image size and stream: renders on
+ * one plan must be ordered (one stream). */
+typedef struct svo_tile_plan svo_tile_plan;
+int svo_tile_plan_create(int width, int height, svo_tile_plan **out);
+void svo_tile_plan_free(svo_tile_plan *p);
+int svo_render_primary_planned_device(svo_octree *h, svo_tile_plan *plan, const svo_camera *cam, int width,
+                                      int height, uint8_t *hit, uint32_t *normal, float *t, uint64_t *leaf,
+                                      void *stream);
 
 /* Primary rays (as svo_render_primary) plus one hard-shadow ray per primary
  * leaf hit: origin pos + d*(t - eps) (the hit leaf's entry point pulled back

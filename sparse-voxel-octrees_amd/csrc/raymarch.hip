@@ -119,6 +119,68 @@ __global__ void __launch_bounds__(64) primary_kernel(const uint32_t *__restrict_
     store_result<Idx>(code, normal, t, leaf, int64_t(y) * p.width + x, hit, nrm, tt, lf);
 }
 
+// Cost-ordered form of primary_kernel: workgroup b traces tile order[b]
+// (order == nullptr: raster order) and records the tile's cost, the largest
+// loop-pass count of its 64 rays, in cost[tile]. A view rendered again starts
+// its slowest tiles (rays grazing the surface, up to ~500 dependent passes
+// against a median ~40) first, so they run beside the bulk of the tiles
+// instead of in the launch's tail on an otherwise idle GPU. The order only
+// decides when a tile runs: every ray and its result are those of
+// primary_kernel.
+template <typename Idx, int Reg>
+__global__ void __launch_bounds__(64) primary_ordered_kernel(const uint32_t *__restrict__ tree, PrimaryParams p,
+                                                             int slots, const int *__restrict__ order,
+                                                             uint32_t *__restrict__ cost, uint8_t *hit, uint32_t *nrm,
+                                                             float *tt, uint64_t *lf) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    const int tile = order ? order[blockIdx.x] : int(blockIdx.x);
+    const int lane = threadIdx.x;
+    const int x = (tile % p.tilesX) * 8 + (lane & 7);
+    const int y = (tile / p.tilesX) * 8 + (lane >> 3);
+    uint32_t passes = 0;
+    if (x < p.width && y < p.height) {
+        float d[3];
+        primary_dir(p, x, y, d);
+        Stack<Idx> stk(lds, 64, slots, lane);
+        Ray<Idx> r;
+        ray_init(r, tree, p.px, p.py, p.pz, d[0], d[1], d[2], 0.0f);
+        uint32_t normal = 0;
+        float t = 0.0f;
+        Idx leaf = 0;
+        int code;
+        do {
+            code = ray_step<false, true, Idx, true, Reg>(r, tree, stk, normal, t, leaf);
+            passes++;
+        } while (code == kRunning);
+        store_result<Idx>(code, normal, t, leaf, int64_t(y) * p.width + x, hit, nrm, tt, lf);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) passes = max(passes, uint32_t(__shfl_xor(passes, o)));
+    if (lane == 0) cost[tile] = passes;
+}
+
+// The next render's order from the tile costs: tiles in descending buckets of
+// log2(cost), 16 buckets (the order inside a bucket does not matter). One
+// workgroup; a few microseconds for a 4K image's tiles.
+__global__ void __launch_bounds__(1024) tile_order_kernel(const uint32_t *__restrict__ cost, int tiles,
+                                                          int *__restrict__ order) {
+    __shared__ int hist[16], base[16];
+    const int t = threadIdx.x;
+    if (t < 16) hist[t] = 0;
+    __syncthreads();
+    for (int i = t; i < tiles; i += blockDim.x) atomicAdd(&hist[min(15, 31 - __builtin_clz(cost[i] | 1u))], 1);
+    __syncthreads();
+    if (t == 0) {
+        int pos = 0;
+        for (int b = 15; b >= 0; --b) {
+            base[b] = pos;
+            pos += hist[b];
+        }
+    }
+    __syncthreads();
+    for (int i = t; i < tiles; i += blockDim.x) order[atomicAdd(&base[min(15, 31 - __builtin_clz(cost[i] | 1u))], 1)] = i;
+}
+
 // Primary rays and hard shadows of one 8x8 tile in one wave: every lane's
 // primary ray (as primary_kernel), then one shadow ray per primary leaf hit,
 // from o + d*(t - eps) (the hit leaf's entry point pulled back along the
@@ -313,6 +375,85 @@ void launch_primary(const svo_octree &h, const TileLaunch &L, const PrimaryParam
 
 }  // namespace
 
+}  // namespace svo
+
+// Per-tile costs and tile order of one view size, kept on the device between
+// renders (svo_render_primary_planned_device).
+struct svo_tile_plan {
+    int width = 0, height = 0, tiles = 0;
+    int device = -1;
+    bool ordered = false;  // `order` holds the order from the last render's costs
+    uint32_t *cost = nullptr;
+    int *order = nullptr;
+};
+
+namespace svo {
+
+int tile_plan_create(int width, int height, svo_tile_plan **out) {
+    if (!out) return fail(SVO_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (width <= 0 || height <= 0 || width > 65536 || height > 65536)
+        return fail(SVO_ERR_INVALID, "image size out of range");
+    const int64_t tiles = int64_t((width + 7) / 8) * ((height + 7) / 8);
+    if (tiles > 0x7FFFFFFF) return fail(SVO_ERR_LIMIT, "too many tiles");
+    svo_tile_plan *p = new (std::nothrow) svo_tile_plan();
+    if (!p) return fail(SVO_ERR_NOMEM, "out of host memory");
+    p->width = width;
+    p->height = height;
+    p->tiles = int(tiles);
+    hipError_t e = hipGetDevice(&p->device);
+    if (e == hipSuccess) e = hipMalloc(&p->cost, sizeof(uint32_t) * size_t(tiles));
+    if (e == hipSuccess) e = hipMalloc(&p->order, sizeof(int) * size_t(tiles));
+    if (e != hipSuccess) {
+        tile_plan_free(p);
+        return fail(SVO_ERR_NOMEM, std::string("tile plan: ") + hipGetErrorString(e));
+    }
+    *out = p;
+    return SVO_OK;
+}
+
+void tile_plan_free(svo_tile_plan *p) {
+    if (!p) return;
+    if (p->cost) (void)hipFree(p->cost);
+    if (p->order) (void)hipFree(p->order);
+    delete p;
+}
+
+namespace {
+
+template <typename Idx, int Reg>
+void launch_ordered(const svo_octree &h, const TileLaunch &L, const PrimaryParams &p, svo_tile_plan &plan,
+                    uint8_t *hit, uint32_t *normal, float *t, uint64_t *leaf, hipStream_t s) {
+    hipLaunchKernelGGL((primary_ordered_kernel<Idx, Reg>), dim3(unsigned(plan.tiles)), dim3(64), L.lds, s, h.dwords, p,
+                       L.slots, plan.ordered ? plan.order : nullptr, plan.cost, hit, normal, t, leaf);
+}
+
+}  // namespace
+
+int render_primary_planned_device(svo_octree &h, svo_tile_plan *plan, const svo_camera &cam, int width, int height,
+                                  uint8_t *hit, uint32_t *normal, float *t, uint64_t *leaf, void *stream) {
+    if (!plan) return fail(SVO_ERR_INVALID, "null tile plan");
+    if (width != plan->width || height != plan->height)
+        return fail(SVO_ERR_INVALID, "the tile plan was made for another image size");
+    if (!camera_ok(cam)) return fail(SVO_ERR_INVALID, "camera entries must be finite and within +-1e6");
+    int dev = -1;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev != plan->device) return fail(SVO_ERR_INVALID, "the tile plan lives on another device");
+    int r = ensure_device(h);
+    if (r) return r;
+    const PrimaryParams p = primary_params(cam, width, height);
+    const TileLaunch L = tile_launch(h);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (!L.wide && L.reg == 1) launch_ordered<uint32_t, 1>(h, L, p, *plan, hit, normal, t, leaf, s);
+    else if (!L.wide) launch_ordered<uint32_t, 3>(h, L, p, *plan, hit, normal, t, leaf, s);
+    else if (L.reg == 1) launch_ordered<uint64_t, 1>(h, L, p, *plan, hit, normal, t, leaf, s);
+    else launch_ordered<uint64_t, 3>(h, L, p, *plan, hit, normal, t, leaf, s);
+    hipLaunchKernelGGL(tile_order_kernel, dim3(1), dim3(1024), 0, s, plan->cost, plan->tiles, plan->order);
+    HIP_TRY(hipGetLastError());
+    plan->ordered = true;
+    return SVO_OK;
+}
+
 int render_primary_device(svo_octree &h, const svo_camera &cam, int width, int height, uint8_t *hit,
                           uint32_t *normal, float *t, uint64_t *leaf, void *stream) {
     if (width <= 0 || height <= 0 || width > 65536 || height > 65536)
@@ -361,6 +502,21 @@ int render_shadow_device(svo_octree &h, const svo_camera &cam, int width, int he
                            light[2], eps, L.slots, hit, normal, t, leaf, shadow);
     HIP_TRY(hipGetLastError());
     return SVO_OK;
+}
+
+}  // namespace svo
+
+namespace svo {
+
+namespace {
+__global__ void code_probe_kernel() {}
+}  // namespace
+
+// Makes the HIP runtime load this source file's device code on the current
+// device now (it loads a code object at the first launch of one of its kernels).
+hipError_t load_code_raymarch() {
+    hipFuncAttributes a;
+    return hipFuncGetAttributes(&a, reinterpret_cast<const void *>(&code_probe_kernel));
 }
 
 }  // namespace svo

@@ -586,3 +586,18 @@ int batch_graph_launch(svo_batch_graph *g, void *stream) {
 }
 
 }  // namespace svo
+
+namespace svo {
+
+namespace {
+__global__ void code_probe_kernel() {}
+}  // namespace
+
+// Makes the HIP runtime load this source file's device code on the current
+// device now (it loads a code object at the first launch of one of its kernels).
+hipError_t load_code_render_batch() {
+    hipFuncAttributes a;
+    return hipFuncGetAttributes(&a, reinterpret_cast<const void *>(&code_probe_kernel));
+}
+
+}  // namespace svo

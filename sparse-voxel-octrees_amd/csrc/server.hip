@@ -842,3 +842,18 @@ int server_run_schedule(svo_server *s, const svo_request *reqs, const double *ar
 }
 
 }  // namespace svo
+
+namespace svo {
+
+namespace {
+__global__ void code_probe_kernel() {}
+}  // namespace
+
+// Makes the HIP runtime load this source file's device code on the current
+// device now (it loads a code object at the first launch of one of its kernels).
+hipError_t load_code_server() {
+    hipFuncAttributes a;
+    return hipFuncGetAttributes(&a, reinterpret_cast<const void *>(&code_probe_kernel));
+}
+
+}  // namespace svo

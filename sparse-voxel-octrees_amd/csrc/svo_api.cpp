@@ -121,6 +121,19 @@ int svo_version(void) { return 100; }
 
 int svo_device_count(void) { return device_count(); }
 
+int svo_load_device_code(void) {
+    return guarded([&]() -> int {
+        clear_error();
+        hipError_t (*const loads[])() = {load_code_raymarch, load_code_render_batch, load_code_server,
+                                         load_code_frame, load_code_build};
+        for (auto load : loads) {
+            const hipError_t e = load();
+            if (e != hipSuccess) return fail(SVO_ERR_DEVICE, std::string("loading device code: ") + hipGetErrorString(e));
+        }
+        return SVO_OK;
+    });
+}
+
 int svo_load_memory(const void *data, uint64_t size, svo_octree **out) {
     return guarded([&]() -> int {
         clear_error();
@@ -362,6 +375,25 @@ int svo_render_primary_device(svo_octree *h, const svo_camera *cam, int width, i
         clear_error();
         if (!h || !cam) return fail(SVO_ERR_INVALID, "null argument");
         return render_primary_device(*h, *cam, width, height, hit, normal, t, leaf, stream);
+    });
+}
+
+int svo_tile_plan_create(int width, int height, svo_tile_plan **out) {
+    return guarded([&]() -> int {
+        clear_error();
+        return tile_plan_create(width, height, out);
+    });
+}
+
+void svo_tile_plan_free(svo_tile_plan *p) { tile_plan_free(p); }
+
+int svo_render_primary_planned_device(svo_octree *h, svo_tile_plan *plan, const svo_camera *cam, int width,
+                                      int height, uint8_t *hit, uint32_t *normal, float *t, uint64_t *leaf,
+                                      void *stream) {
+    return guarded([&]() -> int {
+        clear_error();
+        if (!h || !cam) return fail(SVO_ERR_INVALID, "null argument");
+        return render_primary_planned_device(*h, plan, *cam, width, height, hit, normal, t, leaf, stream);
     });
 }
 

@@ -9,6 +9,8 @@
 #include <utility>
 #include <vector>
 
+#include <hip/hip_runtime.h>
+
 #include "svo.h"
 
 namespace svo {
@@ -68,6 +70,7 @@ struct svo_octree {
 struct svo_renderer;     // render_batch.hip
 struct svo_batch_graph;  // render_batch.hip
 struct svo_server;       // server.hip
+struct svo_tile_plan;    // raymarch.hip
 
 namespace svo {
 
@@ -121,6 +124,10 @@ int raymarch_device(svo_octree &h, int64_t n, const float *o, const float *d, co
                     float rs0, uint8_t *hit, uint32_t *normal, float *t, uint64_t *leaf, void *stream);
 int render_primary_device(svo_octree &h, const svo_camera &cam, int width, int height, uint8_t *hit,
                           uint32_t *normal, float *t, uint64_t *leaf, void *stream);
+int tile_plan_create(int width, int height, svo_tile_plan **out);
+void tile_plan_free(svo_tile_plan *p);
+int render_primary_planned_device(svo_octree &h, svo_tile_plan *plan, const svo_camera &cam, int width, int height,
+                                  uint8_t *hit, uint32_t *normal, float *t, uint64_t *leaf, void *stream);
 int device_count();
 int raymarch_one(const svo_octree &h, const float o[3], const float d[3], float rs, uint32_t &normal, float &t,
                  uint64_t &leaf);
@@ -150,6 +157,13 @@ void batch_graph_free(svo_batch_graph *g);
 int serve_replay(svo_octree *const *trees, int ntrees, const svo_request *reqs, const double *arrival, int n,
                  const svo_server_opts *opts, double *latency, double *makespan, uint32_t *const *normalOut,
                  float *const *tOut, uint64_t *const *leafOut);
+
+// ---- device code loading (one probe per source file) ----
+hipError_t load_code_raymarch();
+hipError_t load_code_render_batch();
+hipError_t load_code_server();
+hipError_t load_code_frame();
+hipError_t load_code_build();
 
 // ---- live request server (server.hip) ----
 int server_create(svo_octree *const *trees, int ntrees, const svo_server_config *cfg, svo_server **out);

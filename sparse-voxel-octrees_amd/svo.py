@@ -37,7 +37,8 @@ EXPORTS = [
     "svo_renderer_autotune", "svo_batch_graph_create", "svo_batch_graph_launch", "svo_batch_graph_free",
     "svo_server_create", "svo_server_submit", "svo_server_poll", "svo_server_wait", "svo_server_outputs",
     "svo_server_copy_outputs", "svo_server_release", "svo_server_stats_get", "svo_server_run_schedule",
-    "svo_server_destroy",
+    "svo_server_destroy", "svo_tile_plan_create", "svo_tile_plan_free", "svo_render_primary_planned_device",
+    "svo_load_device_code",
     "svo_build_stats_get",
 ]
 
@@ -326,12 +327,16 @@ def _load():
         "svo_server_copy_outputs": [P, U64, P, P, P],
         "svo_server_stats_get": [P, P], "svo_server_run_schedule": [P, P, P, ctypes.c_int, P, P, P],
         "svo_server_destroy": [P],
+        "svo_tile_plan_create": [ctypes.c_int, ctypes.c_int, P], "svo_tile_plan_free": [P],
+        "svo_render_primary_planned_device": [P, P, P, ctypes.c_int, ctypes.c_int, P, P, P, P, P],
+        "svo_load_device_code": [],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
         fn.argtypes = args
         fn.restype = None if name in ("svo_free", "svo_free_buffer", "svo_renderer_free",
-                                      "svo_batch_graph_free", "svo_server_destroy") else ctypes.c_int
+                                      "svo_batch_graph_free", "svo_server_destroy",
+                                      "svo_tile_plan_free") else ctypes.c_int
     L.svo_frame_depth_count.restype = I64
     return L
 
@@ -372,6 +377,11 @@ def encode_words(words, center):
         return ctypes.string_at(p, n.value)
     finally:
         lib.svo_free_buffer(p)
+
+
+def load_device_code():
+    """Loads all of the library's kernels onto the current device now."""
+    _check(lib.svo_load_device_code())
 
 
 def device_count():
@@ -639,6 +649,30 @@ def frame_depth_count(width, height, stripes=16):
     if n < 0:
         raise SvoError(ERR_INVALID, lib.svo_last_error().decode(errors="replace"))
     return n
+
+
+class TilePlan:
+    """Per-tile costs and tile order of one image size on the device
+    (svo_tile_plan): renders through it start the slowest tiles first."""
+
+    def __init__(self, width, height):
+        self.width, self.height = int(width), int(height)
+        self._p = ctypes.c_void_p()
+        _check(lib.svo_tile_plan_create(self.width, self.height, ctypes.byref(self._p)))
+
+    def render_device(self, tree, cam, hit_ptr=None, normal_ptr=None, t_ptr=None, leaf_ptr=None, stream=None):
+        _check(lib.svo_render_primary_planned_device(tree.handle, self._p, ctypes.byref(cam), self.width, self.height,
+                                                     hit_ptr, normal_ptr, t_ptr, leaf_ptr, stream))
+
+    def close(self):
+        if lib is None:  # interpreter shutdown
+            return
+        if getattr(self, "_p", None) and self._p.value:
+            lib.svo_tile_plan_free(self._p)
+            self._p = ctypes.c_void_p()
+
+    def __del__(self):
+        self.close()
 
 
 class Renderer:

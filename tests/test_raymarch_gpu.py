@@ -529,3 +529,37 @@ def test_frames_in_one_call_equal_single_frames(svo, trees):
         assert e.value.code == svo.svo.ERR_INVALID
     stream.synchronize()
     assert all(bool((x == 7).all()) for x in out)  # nothing was enqueued
+
+
+def test_cost_ordered_renders_match_plain_kernel(svo, trees):
+    """svo_render_primary_planned_device: the first render of a plan (raster
+    order), the next ones (tiles in the order the previous render's costs
+    give) and a render of another camera through the same plan (a stale
+    order) all equal the plain kernel bit for bit; a plan refuses another
+    image size."""
+    import torch
+    dev = torch.device("cuda", 0)
+    t = trees["dragon"]
+    w, h = 333, 197
+    plan = svo.svo.TilePlan(w, h)
+    cams = [t.orbit_camera(20.0, 35.0, 1.0), t.orbit_camera(20.0, 35.0, 1.0), t.orbit_camera(-40.0, 250.0, 1.0)]
+    for k, cam in enumerate(cams):
+        o = {"hit": torch.full((w * h,), 9, dtype=torch.uint8, device=dev),
+             "normal": torch.zeros(w * h, dtype=torch.int32, device=dev),
+             "t": torch.zeros(w * h, dtype=torch.float32, device=dev),
+             "leaf": torch.zeros(w * h, dtype=torch.int64, device=dev)}
+        print(f"[gpu] cost-ordered render {k}", flush=True)
+        plan.render_device(t, cam, o["hit"].data_ptr(), o["normal"].data_ptr(), o["t"].data_ptr(), o["leaf"].data_ptr())
+        torch.cuda.synchronize()
+        mine = {"hit": o["hit"].cpu().numpy(), "normal": o["normal"].cpu().numpy().view(np.uint32),
+                "t": o["t"].cpu().numpy(), "leaf": o["leaf"].cpu().numpy().view(np.uint64)}
+        _same(mine, t.render_primary(cam, w, h), f"planned render {k}")
+    other = svo.svo.TilePlan(w + 1, h)
+    with pytest.raises(svo.SvoError) as e:
+        lib = svo.svo.lib
+        import ctypes
+        svo.svo._check(lib.svo_render_primary_planned_device(t.handle, other._p, ctypes.byref(cams[0]), w, h,
+                                                             None, None, None, None, None))
+    assert e.value.code == svo.svo.ERR_INVALID
+    plan.close()
+    other.close()
