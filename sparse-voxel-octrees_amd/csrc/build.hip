@@ -73,16 +73,16 @@ struct Arena {
             (void)hipStreamSynchronize(stream);
             (void)hipStreamDestroy(stream);
         }
-        // keep a bounded reserve for the next build (1/64 of the device, at most
-        // 4 GiB): memory newly obtained from the driver costs milliseconds per
-        // large allocation
+        // keep a bounded reserve for the next build (1/16 of the device, at most
+        // 16 GiB: a 2048^3 mesh build peaks near 4 GiB): memory newly obtained
+        // from the driver costs milliseconds per large allocation
         hipMemPool_t pool;
         if (default_pool(pool)) (void)hipMemPoolTrimTo(pool, pool_reserve());
     }
     static size_t pool_reserve() {
         size_t freeB = 0, total = 0;
         if (hipMemGetInfo(&freeB, &total) != hipSuccess) return 0;
-        return std::min<size_t>(total / 64, size_t(4) << 30);
+        return std::min<size_t>(total / 16, size_t(16) << 30);
     }
     static bool default_pool(hipMemPool_t &pool) {
         int dev = 0;
@@ -504,26 +504,14 @@ __device__ __forceinline__ bool pair_box(uint32_t f, const float tv[9], const Me
     return true;
 }
 
-// Rows (y, z) of each pair's clipped voxel box; *voxels accumulates the
-// boxes' voxel counts, an upper bound of the (voxel, triangle) candidates.
 __global__ void pair_rows_kernel(const uint32_t *__restrict__ cells, const uint32_t *__restrict__ tris, uint64_t P,
-                                 const float *__restrict__ pos, MeshParams mp, uint32_t *__restrict__ rows,
-                                 unsigned long long *__restrict__ voxels) {
+                                 const float *__restrict__ pos, MeshParams mp, uint32_t *__restrict__ rows) {
     uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    uint64_t box = 0;
-    if (i < P) {
-        float tv[9];
-        load_tri(pos, tris[i], tv);
-        int l[3], u[3];
-        const bool any = pair_box(cells[i], tv, mp, l, u);
-        const uint32_t nr = any ? uint32_t((u[1] - l[1] + 1) * (u[2] - l[2] + 1)) : 0u;
-        rows[i] = nr;
-        if (any) box = uint64_t(nr) * uint64_t(u[0] - l[0] + 1);
-    }
-    // one atomic per wave
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) box += __shfl_xor(box, o);
-    if ((threadIdx.x & 63) == 0 && box) atomicAdd(voxels, (unsigned long long)box);
+    if (i >= P) return;
+    float tv[9];
+    load_tri(pos, tris[i], tv);
+    int l[3], u[3];
+    rows[i] = pair_box(cells[i], tv, mp, l, u) ? uint32_t((u[1] - l[1] + 1) * (u[2] - l[2] + 1)) : 0u;
 }
 
 // Cache blocks with a non-empty cell list (PlyLoader::isBlockEmpty,
@@ -553,84 +541,41 @@ __device__ __forceinline__ uint64_t find_pair(const uint64_t *__restrict__ rowSt
 }
 
 // The voxel loop of PlyLoader::triangleToVolume (src/PlyLoader.cpp:353-373)
-// for one (y, z) row per thread: centres start at (l - 0.5)*hx and step by
-// +hx, exactly as the reference accumulates them, in one pass: each row's SAT
-// results are kept as bit masks (256 voxels per pass over the row), the wave
-// reserves space for all its lanes' hits with one atomic, and each lane
-// writes its keys. The keys'
-// order is arbitrary (they are sorted next). cap bounds the key array; a
-// reservation past it raises *overflow (the pair boxes' voxel count is an
-// upper bound, so this does not happen).
-__global__ void __launch_bounds__(kThreads) row_voxels_emit_kernel(
-    const uint32_t *__restrict__ cells, const uint32_t *__restrict__ tris, uint64_t P,
-    const uint64_t *__restrict__ rowStart, uint64_t R, const float *__restrict__ pos, MeshParams mp, int tribits,
-    uint64_t *__restrict__ keys, uint64_t cap, unsigned long long *__restrict__ count, uint32_t *__restrict__ overflow) {
-    const uint64_t r = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    const int lane = threadIdx.x & 63;
-    int len = 0, x0 = 0, y = 0, z = 0;
-    uint32_t t = 0;
+// for one (y, z) row: centres start at (l - 0.5)*hx and step by +hx, exactly as
+// the reference accumulates them. Count pass (keys == nullptr) or emit pass.
+__global__ void row_voxels_kernel(const uint32_t *__restrict__ cells, const uint32_t *__restrict__ tris, uint64_t P,
+                                  const uint64_t *__restrict__ rowStart, uint64_t R, const float *__restrict__ pos,
+                                  MeshParams mp, int tribits, uint32_t *__restrict__ hitCount,
+                                  const uint64_t *__restrict__ hitStart, uint64_t *__restrict__ keys) {
+    uint64_t r = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r >= R) return;
+    const uint64_t i = find_pair(rowStart, P, r);
+    const uint32_t t = tris[i];
     float tv[9];
-    float ctr[3] = {0.0f, 0.0f, 0.0f};
+    load_tri(pos, t, tv);
+    int l[3], u[3];
+    pair_box(cells[i], tv, mp, l, u);
+    const uint64_t local = r - rowStart[i];
+    const int ny = u[1] - l[1] + 1;
+    const int y = l[1] + int(local % uint64_t(ny));
+    const int z = l[2] + int(local / uint64_t(ny));
     const float hx = mp.hx;
     const float half[3] = {0.5f * hx, 0.5f * hx, 0.5f * hx};
-    if (r < R) {
-        const uint64_t i = find_pair(rowStart, P, r);
-        t = tris[i];
-        load_tri(pos, t, tv);
-        int l[3], u[3];
-        pair_box(cells[i], tv, mp, l, u);
-        const uint64_t local = r - rowStart[i];
-        const int ny = u[1] - l[1] + 1;
-        y = l[1] + int(local % uint64_t(ny));
-        z = l[2] + int(local / uint64_t(ny));
-        ctr[2] = (float(l[2]) - 0.5f) * hx;
-        for (int k = l[2]; k < z; ++k) ctr[2] += hx;
-        ctr[1] = (float(l[1]) - 0.5f) * hx;
-        for (int k = l[1]; k < y; ++k) ctr[1] += hx;
-        ctr[0] = (float(l[0]) - 0.5f) * hx;
-        x0 = l[0];
-        len = u[0] - l[0] + 1;
-    }
-    int maxLen = len;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) maxLen = max(maxLen, __shfl_xor(maxLen, o));
-    for (int seg = 0; seg < maxLen; seg += 256) {
-        uint64_t m[4] = {0, 0, 0, 0};
-        const int segLen = min(max(len - seg, 0), 256);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int n = min(max(segLen - 64 * q, 0), 64);
-            for (int j = 0; j < n; ++j, ctr[0] += hx)
-                if (tri_box(ctr, half, tv)) m[q] |= uint64_t(1) << j;
-        }
-        const uint64_t hits = uint64_t(__popcll(m[0]) + __popcll(m[1]) + __popcll(m[2]) + __popcll(m[3]));
-        // wave-wide exclusive sum of the hit counts, then one reservation
-        uint64_t incl = hits;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint64_t v = __shfl_up(incl, o);
-            if (lane >= o) incl += v;
-        }
-        const uint64_t waveTotal = __shfl(incl, 63);
-        unsigned long long base = 0;
-        if (lane == 0 && waveTotal) base = atomicAdd(count, (unsigned long long)waveTotal);
-        base = __shfl(base, 0);
-        uint64_t at = base + incl - hits;
-        if (at + hits > cap) {
-            if (hits) atomicOr(overflow, 1u);
-            continue;
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            uint64_t bits = m[q];
-            while (bits) {
-                const int j = __builtin_ctzll(bits);
-                bits &= bits - 1;
-                const uint32_t x = uint32_t(x0 + seg + 64 * q + j);
-                keys[at++] = (morton3(x, uint32_t(y), uint32_t(z)) << tribits) | uint64_t(t);
-            }
+    float ctr[3];
+    ctr[2] = (float(l[2]) - 0.5f) * hx;
+    for (int k = l[2]; k < z; ++k) ctr[2] += hx;
+    ctr[1] = (float(l[1]) - 0.5f) * hx;
+    for (int k = l[1]; k < y; ++k) ctr[1] += hx;
+    ctr[0] = (float(l[0]) - 0.5f) * hx;
+    uint32_t n = 0;
+    uint64_t at = keys ? hitStart[r] : 0;
+    for (int x = l[0]; x <= u[0]; ++x, ctr[0] += hx) {
+        if (tri_box(ctr, half, tv)) {
+            if (keys) keys[at + n] = (morton3(uint32_t(x), uint32_t(y), uint32_t(z)) << tribits) | uint64_t(t);
+            n++;
         }
     }
+    if (!keys) hitCount[r] = n;
 }
 
 __global__ void seg_heads_kernel(const uint64_t *__restrict__ keys, uint64_t K, int shift, uint32_t *__restrict__ head) {
@@ -1493,8 +1438,9 @@ int emit_tree(Arena &ar, const Geometry &g, uint64_t *leafKey, uint32_t *leafWor
     const int cacheLevel = D - g.cacheLog;  // nodes of edge `cache`
     const bool multi = g.cacheLog < D;
     int r;
-    for (Level &l : lv)
-        if ((r = ar.alloc(l.cnt, 1))) return r;
+    uint64_t *cnts;  // every level's node count, one word each, read together once the levels exist
+    if ((r = ar.alloc(cnts, uint64_t(D) + 1))) return r;
+    for (int d = 0; d <= D; ++d) lv[size_t(d)].cnt = cnts + d;
     hipLaunchKernelGGL(set_u64_kernel, dim3(1), dim3(1), 0, st, lv[D].cnt, L);
     // Levels bottom-up. Node counts stay on the device: each level's arrays
     // are sized by a host bound (no more nodes than the level below, nor than
@@ -1554,6 +1500,16 @@ int emit_tree(Arena &ar, const Geometry &g, uint64_t *leafKey, uint32_t *leafWor
         HIP_TRY(hipGetLastError());
     }
 
+    // the level counts (one read): the size, placement and scatter arrays
+    // below are allocated and launched at the exact counts
+    {
+        std::vector<uint64_t> n(size_t(D) + 1);
+        if ((r = read_words(cnts, n.data(), D + 1, st))) return r;
+        for (int d = 0; d < D; ++d) {
+            if (n[size_t(d)] > lv[size_t(d)].bound) return fail(SVO_ERR_DEVICE, "builder: level above its bound (internal error)");
+            lv[size_t(d)].bound = n[size_t(d)];
+        }
+    }
     // sizes, bottom-up
     for (int d = D - 1; d >= 0; --d) {
         Level &p = lv[d];
@@ -2000,35 +1956,38 @@ static int mesh_core(Arena &ar, float *dv, int64_t nverts, int has_normals, int3
         if (P) hipLaunchKernelGGL(block_flags_kernel, dim3(grid_for(P)), dim3(kThreads), 0, st, cells, P, mp, blockFlags);
     }
 
-    // voxel rows; rb[0] = rows, rb[1] = voxels in the pairs' boxes (a bound
-    // of the candidates), read together
+    // voxel rows
     uint32_t *rows;
     uint64_t *rowStart;
     if ((r = ar.alloc(rows, P)) || (r = ar.alloc(rowStart, P + 1))) return r;
-    uint64_t rowInfo[2] = {0, 0};
-    if ((r = fill32(reinterpret_cast<uint32_t *>(rb), 4, 0, st))) return r;
-    if (P)
-        hipLaunchKernelGGL(pair_rows_kernel, dim3(grid_for(P)), dim3(kThreads), 0, st, cells, ptri, P, pos, mp, rows,
-                           reinterpret_cast<unsigned long long *>(rb + 1));
+    if (P) hipLaunchKernelGGL(pair_rows_kernel, dim3(grid_for(P)), dim3(kThreads), 0, st, cells, ptri, P, pos, mp, rows);
     HIP_TRY(hipGetLastError());
-    if (P && (r = exclusive_scan(ar, rows, rowStart, P, st, rb))) return r;
-    if ((r = read_words(rb, rowInfo, 2, st))) return r;
-    const uint64_t R = rowInfo[0], cap = rowInfo[1];
+    uint64_t R;
+    if ((r = scan_count(ar, rows, rowStart, P, st, R))) return r;
     hipLaunchKernelGGL(set_u64_kernel, dim3(1), dim3(1), 0, st, rowStart + P, R);
     ar.release(rows);
-    // (voxel, triangle) keys in one SAT pass; rb[0] = candidates, rb[1] = overflow
-    uint64_t *keys, *keys2;
-    if ((r = ar.alloc(keys, cap)) || (r = ar.alloc(keys2, cap))) return r;
-    if ((r = fill32(reinterpret_cast<uint32_t *>(rb), 4, 0, st))) return r;
+    // (voxel, triangle) keys: a count pass sizes each row's output, an emit
+    // pass writes the keys at the rows' exclusive offsets (a single pass that
+    // reserved space per wave with device-scope atomics on one counter
+    // measured 5.7 ms against 1.8 ms for the pair: the ~300 K reservations
+    // serialise)
+    uint32_t *hitCount;
+    uint64_t *hitStart;
+    if ((r = ar.alloc(hitCount, R)) || (r = ar.alloc(hitStart, R))) return r;
     if (R)
-        hipLaunchKernelGGL(row_voxels_emit_kernel, dim3(grid_for(R)), dim3(kThreads), 0, st, cells, ptri, P, rowStart, R,
-                           pos, mp, tribits, keys, cap, reinterpret_cast<unsigned long long *>(rb),
-                           reinterpret_cast<uint32_t *>(rb + 1));
+        hipLaunchKernelGGL(row_voxels_kernel, dim3(grid_for(R)), dim3(kThreads), 0, st, cells, ptri, P, rowStart, R, pos,
+                           mp, tribits, hitCount, nullptr, nullptr);
     HIP_TRY(hipGetLastError());
-    uint64_t keyInfo[2] = {0, 0};
-    if ((r = read_words(rb, keyInfo, 2, st))) return r;
-    if (keyInfo[1]) return fail(SVO_ERR_DEVICE, "builder: candidate keys exceed their bound (internal error)");
-    const uint64_t K = keyInfo[0];
+    uint64_t K;
+    if ((r = scan_count(ar, hitCount, hitStart, R, st, K))) return r;
+    uint64_t *keys, *keys2;
+    if ((r = ar.alloc(keys, K)) || (r = ar.alloc(keys2, K))) return r;
+    if (R)
+        hipLaunchKernelGGL(row_voxels_kernel, dim3(grid_for(R)), dim3(kThreads), 0, st, cells, ptri, P, rowStart, R, pos,
+                           mp, tribits, hitCount, hitStart, keys);
+    HIP_TRY(hipGetLastError());
+    ar.release(hitCount);
+    ar.release(hitStart);
     out.build.triangles = uint64_t(T);
     out.build.candidates = K;
     ar.release(cells);
@@ -2566,6 +2525,21 @@ int sdf_volume(const svo_sdf_primitive *prims, int count, int n, float shade, ui
     HIP_TRY(hipStreamSynchronize(st));
     HIP_TRY(hipMemcpy(grid, dg, N * 4, hipMemcpyDeviceToHost));
     return SVO_OK;
+}
+
+}  // namespace svo
+
+namespace svo {
+
+namespace {
+__global__ void code_probe_kernel() {}
+}  // namespace
+
+// Makes the HIP runtime load this source file's device code on the current
+// device now (it loads a code object at the first launch of one of its kernels).
+hipError_t load_code_build() {
+    hipFuncAttributes a;
+    return hipFuncGetAttributes(&a, reinterpret_cast<const void *>(&code_probe_kernel));
 }
 
 }  // namespace svo
