@@ -274,6 +274,18 @@ void svo_renderer_free(svo_renderer *r);
 int svo_render_batch_device(svo_renderer *r, const svo_view *views, int nviews, const svo_batch_opts *opts,
                             void *stream);
 
+/* svo_render_batch_device that allocates nothing. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid allocation).
+ * The view table, tile bases and queue counters live in a fixed device array
+ * of the library (about 72 KiB per device, held until the process ends), so
+ * a call takes at most 512 views (SVO_ERR_LIMIT beyond) and every octree must
+ * already be resident on the current device (svo_upload; SVO_ERR_INVALID
+ * otherwise). Outputs and launch forms are those of svo_render_batch_device.
+ * All calls on one device share that array: calls on one stream are ordered;
+ * calls that overlap on one device (several streams or threads) race. */
+int svo_render_batch_noalloc(const svo_view *views, int nviews, const svo_batch_opts *opts, void *stream);
+
 /* Launch-shape autotuner: times the batched launch of `views` on `stream` in
  * the grid form for tiles 8x8, 16x4, 4x16 and 32x2 and in the persistent form
  * for waves per CU {16, 24, 32} x refill threshold {16, 32, 64} (one warm-up

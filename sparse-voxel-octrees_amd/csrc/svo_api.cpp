@@ -524,6 +524,13 @@ int svo_render_batch_device(svo_renderer *r, const svo_view *views, int nviews, 
     });
 }
 
+int svo_render_batch_noalloc(const svo_view *views, int nviews, const svo_batch_opts *opts, void *stream) {
+    return guarded([&]() -> int {
+        clear_error();
+        return render_batch_noalloc(views, nviews, opts, stream);
+    });
+}
+
 int svo_renderer_autotune(svo_renderer *r, const svo_view *views, int nviews, void *stream, svo_batch_opts *best,
                           float *times) {
     return guarded([&]() -> int {

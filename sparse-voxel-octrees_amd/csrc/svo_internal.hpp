@@ -146,6 +146,7 @@ int64_t frame_depth_count(int width, int height, int stripes);
 int renderer_create(int maxViews, svo_renderer **out);
 void renderer_free(svo_renderer *r);
 int render_batch(svo_renderer *r, const svo_view *views, int nviews, const svo_batch_opts *opts, void *stream);
+int render_batch_noalloc(const svo_view *views, int nviews, const svo_batch_opts *opts, void *stream);
 int renderer_autotune(svo_renderer *r, const svo_view *views, int nviews, void *stream, svo_batch_opts *best,
                       float *times);
 int batch_graph_create(svo_renderer *r, const svo_view *views, int nviews, const svo_batch_opts *opts, void *stream,

@@ -38,7 +38,7 @@ EXPORTS = [
     "svo_server_create", "svo_server_submit", "svo_server_poll", "svo_server_wait", "svo_server_outputs",
     "svo_server_copy_outputs", "svo_server_release", "svo_server_stats_get", "svo_server_run_schedule",
     "svo_server_destroy", "svo_tile_plan_create", "svo_tile_plan_free", "svo_render_primary_planned_device",
-    "svo_load_device_code",
+    "svo_render_batch_noalloc", "svo_load_device_code",
     "svo_build_stats_get",
 ]
 
@@ -329,6 +329,7 @@ def _load():
         "svo_server_destroy": [P],
         "svo_tile_plan_create": [ctypes.c_int, ctypes.c_int, P], "svo_tile_plan_free": [P],
         "svo_render_primary_planned_device": [P, P, P, ctypes.c_int, ctypes.c_int, P, P, P, P, P],
+        "svo_render_batch_noalloc": [P, ctypes.c_int, P, P],
         "svo_load_device_code": [],
     }
     for name, args in sig.items():
@@ -649,6 +650,14 @@ def frame_depth_count(width, height, stripes=16):
     if n < 0:
         raise SvoError(ERR_INVALID, lib.svo_last_error().decode(errors="replace"))
     return n
+
+
+def render_batch_noalloc(views, stream=None, waves_per_cu=0, refill_threshold=0, form=None, tile_w=0):
+    """svo_render_batch_noalloc: a batched launch whose device tables live in
+    the library's fixed scratch array (at most 512 views; octrees resident)."""
+    arr = (View * len(views))(*views)
+    o = _batch_opts(waves_per_cu, refill_threshold, form, tile_w)
+    _check(lib.svo_render_batch_noalloc(arr, len(views), ctypes.byref(o), stream))
 
 
 class TilePlan:
