@@ -161,24 +161,55 @@ __global__ void __launch_bounds__(64) primary_ordered_kernel(const uint32_t *__r
 
 // The next render's order from the tile costs: tiles in descending buckets of
 // log2(cost), 16 buckets (the order inside a bucket does not matter). One
-// workgroup; a few microseconds for a 4K image's tiles.
+// workgroup. Each wave groups its 64 tiles by bucket (ballots) and makes one
+// LDS reservation per group, so the histogram and the scatter take a handful
+// of LDS atomics per wave and round instead of one per tile.
+__device__ __forceinline__ int cost_bucket(uint32_t c) { return min(15, 31 - __builtin_clz(c | 1u)); }
+
 __global__ void __launch_bounds__(1024) tile_order_kernel(const uint32_t *__restrict__ cost, int tiles,
                                                           int *__restrict__ order) {
-    __shared__ int hist[16], base[16];
-    const int t = threadIdx.x;
+    __shared__ int hist[16];
+    const int t = threadIdx.x, lane = t & 63;
+    const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
     if (t < 16) hist[t] = 0;
     __syncthreads();
-    for (int i = t; i < tiles; i += blockDim.x) atomicAdd(&hist[min(15, 31 - __builtin_clz(cost[i] | 1u))], 1);
-    __syncthreads();
-    if (t == 0) {
-        int pos = 0;
-        for (int b = 15; b >= 0; --b) {
-            base[b] = pos;
-            pos += hist[b];
+    for (int i0 = 0; i0 < tiles; i0 += blockDim.x) {
+        const int i = i0 + t;
+        const int b = i < tiles ? cost_bucket(cost[i]) : -1;
+        uint64_t pending = __ballot(b >= 0);
+        while (pending) {
+            const int leader = __ffsll(static_cast<unsigned long long>(pending)) - 1;
+            const int bb = __shfl(b, leader);
+            const uint64_t m = __ballot(b == bb);
+            if (lane == leader) atomicAdd(&hist[bb], __popcll(m));
+            pending &= ~m;
         }
     }
     __syncthreads();
-    for (int i = t; i < tiles; i += blockDim.x) order[atomicAdd(&base[min(15, 31 - __builtin_clz(cost[i] | 1u))], 1)] = i;
+    if (t == 0) {
+        int pos = 0;
+        for (int k = 15; k >= 0; --k) {
+            const int n = hist[k];
+            hist[k] = pos;
+            pos += n;
+        }
+    }
+    __syncthreads();
+    for (int i0 = 0; i0 < tiles; i0 += blockDim.x) {
+        const int i = i0 + t;
+        const int b = i < tiles ? cost_bucket(cost[i]) : -1;
+        uint64_t pending = __ballot(b >= 0);
+        while (pending) {
+            const int leader = __ffsll(static_cast<unsigned long long>(pending)) - 1;
+            const int bb = __shfl(b, leader);
+            const uint64_t m = __ballot(b == bb);
+            int start = 0;
+            if (lane == leader) start = atomicAdd(&hist[bb], __popcll(m));
+            start = __shfl(start, leader);
+            if (b == bb) order[start + __popcll(m & below)] = i;
+            pending &= ~m;
+        }
+    }
 }
 
 // Primary rays and hard shadows of one 8x8 tile in one wave: every lane's

@@ -89,13 +89,52 @@ static_assert(sizeof(ServeSlot) == 192, "ServeSlot layout");
 // Device memory: launch state (cleared before each launch).
 struct ServeState {
     unsigned long long cursor;     // next launch-relative tile index to claim
-    unsigned long long published;  // (requests << 44) | tiles, release-stored by the leader
+    uint64_t published;            // (requests << 44) | tiles, stored by the leader after the tables
     unsigned int stop;             // set by the leader: waiting waves leave
     unsigned int pad[11];
 };
 
-__device__ __forceinline__ uint64_t load_sys(uint64_t *p) {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+// Memory ordering without cache maintenance. A release or acquire at agent
+// or system scope writes back or invalidates the whole L2 of the wave's XCD
+// (buffer_wbl2 / buffer_inv), which, once per tile, costs far more than the
+// tile and evicts the octree. Instead every word shared between waves of
+// different XCDs or with the host is read and written with scoped relaxed
+// atomics (coherent at that scope, so never a stale cached copy), and a
+// writer waits for its stores to be performed (s_waitcnt vmcnt(0)) before
+// the store or atomic that publishes them.
+template <typename T>
+__device__ __forceinline__ T ld_sys(T *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM); }
+template <typename T>
+__device__ __forceinline__ T ld_dev(T *p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+template <typename T>
+__device__ __forceinline__ void st_sys(T *p, T v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM); }
+template <typename T>
+__device__ __forceinline__ void st_dev(T *p, T v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+// s_waitcnt vmcnt(0): this wave's vector memory operations are performed
+__device__ __forceinline__ void wait_stores() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// A request's view descriptor, read coherently: lane k < 32 reads word k,
+// every field is then taken from its lane into a wave-uniform register.
+__device__ __forceinline__ ViewDesc load_desc(const ViewDesc *p, int lane) {
+    uint32_t w = 0;
+    if (lane < 32) w = ld_dev(const_cast<uint32_t *>(reinterpret_cast<const uint32_t *>(p)) + lane);
+    uint32_t words[32];
+#pragma unroll
+    for (int k = 0; k < 32; ++k) words[k] = __builtin_amdgcn_readlane(w, k);
+    ViewDesc d;
+    __builtin_memcpy(&d, words, sizeof(d));
+    return d;
+}
+
+// One ray's outputs as system-scope stores (performed at memory, visible to
+// the host and to later kernels once the wave has waited for them). Requests
+// carry no hit array.
+template <typename Idx>
+__device__ __forceinline__ void store_sys(const ViewDesc &vd, int64_t i, int code, uint32_t normal, float t, Idx leaf) {
+    if (vd.normal) st_sys(vd.normal + i, code == kLeaf ? normal : 0u);
+    if (vd.t) st_sys(reinterpret_cast<uint32_t *>(vd.t) + i, code == kMiss ? 0u : __float_as_uint(t));
+    if (vd.leaf) st_sys(reinterpret_cast<unsigned long long *>(vd.leaf) + i,
+                        (unsigned long long)(code == kLeaf ? uint64_t(leaf) : SVO_LEAF_NONE));
 }
 
 // Persistent serving kernel. Workgroups of 256 threads (4 waves); wave 0 of
@@ -113,10 +152,12 @@ __global__ void __launch_bounds__(256) serve_kernel(ServeCtrl *ctrl, const Serve
         // ---- leader: mirror the host ring into the device tables ----
         uint32_t mirrored = 0;
         uint64_t total = 0;
-        uint64_t hb = __hip_atomic_load(&ctrl->heartbeat, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        uint64_t hb = ld_sys(&ctrl->heartbeat);
         uint64_t hbTime = wall_clock64();
         for (;;) {
-            const uint64_t tail = load_sys(&ctrl->tail);
+            // the host writes a slot before it moves the tail; the slot reads
+            // below depend on the tail read, so they follow it
+            const uint64_t tail = ld_sys(&ctrl->tail);
             uint64_t avail = tail > seq0 ? tail - seq0 : 0;
             if (avail > maxReqs) avail = maxReqs;
             if (avail > mirrored) {
@@ -128,34 +169,31 @@ __global__ void __launch_bounds__(256) serve_kernel(ServeCtrl *ctrl, const Serve
                     uint32_t tiles = 0;
                     if (mine) {
                         const int s = int((seq0 + k) % uint64_t(ring));
-                        const uint32_t *src = reinterpret_cast<const uint32_t *>(&hslots[s].desc) + (lane & 31);
-                        uint32_t *dst = reinterpret_cast<uint32_t *>(&dslots[s]) + (lane & 31);
-                        *dst = __hip_atomic_load(const_cast<uint32_t *>(src), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-                        tiles = __hip_atomic_load(const_cast<uint32_t *>(&hslots[s].tiles), __ATOMIC_RELAXED,
-                                                  __HIP_MEMORY_SCOPE_SYSTEM);
-                        if ((lane & 31) == 0) tilesOf[s] = tiles;
+                        uint32_t *src = const_cast<uint32_t *>(reinterpret_cast<const uint32_t *>(&hslots[s].desc)) + (lane & 31);
+                        st_dev(reinterpret_cast<uint32_t *>(&dslots[s]) + (lane & 31), ld_sys(src));
+                        tiles = ld_sys(const_cast<uint32_t *>(&hslots[s].tiles));
+                        if ((lane & 31) == 0) st_dev(&tilesOf[s], tiles);
                     }
                     const uint32_t t0 = __shfl(tiles, 0), t1 = __shfl(tiles, 32);
-                    if (lane == 0) dbase[i] = total;
+                    if (lane == 0) st_dev(&dbase[i], total);
                     total += t0;
                     if (i + 1 < uint32_t(avail)) {
-                        if (lane == 0) dbase[i + 1] = total;
+                        if (lane == 0) st_dev(&dbase[i + 1], total);
                         total += t1;
                     }
                 }
                 mirrored = uint32_t(avail);
-                // every lane's descriptor words before the publishing store
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+                if (lane == 0) st_dev(&dbase[mirrored], total);
+                // every lane's table stores performed before the publishing store
+                wait_stores();
                 __builtin_amdgcn_wave_barrier();
                 if (lane == 0) {
-                    dbase[mirrored] = total;
-                    __hip_atomic_store(&st->published, (uint64_t(mirrored) << kCountShift) | total, __ATOMIC_RELEASE,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-                    __hip_atomic_store(&ctrl->mirrored, seq0 + mirrored, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                    st_dev(&st->published, (uint64_t(mirrored) << kCountShift) | total);
+                    st_sys(&ctrl->mirrored, seq0 + mirrored);
                 }
             }
-            bool quit = load_sys(&ctrl->stop) != 0;
-            const uint64_t h = __hip_atomic_load(&ctrl->heartbeat, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            bool quit = ld_sys(&ctrl->stop) != 0;
+            const uint64_t h = ld_sys(&ctrl->heartbeat);
             const uint64_t now = wall_clock64();
             if (h != hb) {
                 hb = h;
@@ -164,7 +202,7 @@ __global__ void __launch_bounds__(256) serve_kernel(ServeCtrl *ctrl, const Serve
                 quit = true;  // the host loop is gone: end the launch
             }
             if (quit) {
-                if (lane == 0) __hip_atomic_store(&st->stop, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                if (lane == 0) st_dev(&st->stop, 1u);
                 return;
             }
             __builtin_amdgcn_s_sleep(2);
@@ -172,8 +210,7 @@ __global__ void __launch_bounds__(256) serve_kernel(ServeCtrl *ctrl, const Serve
     }
 
     // ---- tracing waves ----
-    const int waveInBlock = wave;
-    Stack<Idx> stk(lds + size_t(waveInBlock) * slots * 64 * (sizeof(Idx) + sizeof(float)), 64, slots, lane);
+    Stack<Idx> stk(lds + size_t(wave) * slots * 64 * (sizeof(Idx) + sizeof(float)), 64, slots, lane);
     uint64_t known = 0;  // last published pair seen
     uint32_t q = 0;      // launch-relative request of the last tile
     for (;;) {
@@ -183,29 +220,28 @@ __global__ void __launch_bounds__(256) serve_kernel(ServeCtrl *ctrl, const Serve
         // hold the tile index until it is published (or the launch ends)
         while (g >= (known & kTileMask)) {
             uint64_t p = 0;
-            if (lane == 0) p = __hip_atomic_load(&st->published, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-            p = __shfl(p, 0);
-            known = p;
+            if (lane == 0) p = ld_dev(&st->published);
+            known = __shfl(p, 0);
             if (g < (known & kTileMask)) break;
             uint32_t stop = 0;
-            if (lane == 0) stop = __hip_atomic_load(&st->stop, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+            if (lane == 0) stop = ld_dev(&st->stop);
             if (__shfl(stop, 0)) return;
             __builtin_amdgcn_s_sleep(2);
         }
         // the request holding tile g: dbase[q] <= g < dbase[q + 1] (g only grows per wave)
         const uint32_t count = uint32_t(known >> kCountShift);
-        if (dbase[q + 1] <= g) {
+        if (ld_dev(&dbase[q + 1]) <= g) {
             uint32_t lo = q + 1, hi = count;  // dbase[lo] <= g < dbase[hi]
             while (hi - lo > 1) {
                 const uint32_t mid = (lo + hi) >> 1;
-                if (dbase[mid] <= g) lo = mid; else hi = mid;
+                if (ld_dev(&dbase[mid]) <= g) lo = mid; else hi = mid;
             }
             q = lo;
         }
         const uint64_t seq = seq0 + q;
         const int s = int(seq % uint64_t(ring));
-        const ViewDesc &vd = dslots[s];
-        const uint64_t local = g - dbase[q];
+        const ViewDesc vd = load_desc(&dslots[s], lane);
+        const uint64_t local = g - ld_dev(&dbase[q]);
         const int tx = int(local % uint64_t(vd.tilesX));
         const int ty = int(local / uint64_t(vd.tilesX));
         const int x = tx * 8 + (lane & 7);
@@ -222,19 +258,21 @@ __global__ void __launch_bounds__(256) serve_kernel(ServeCtrl *ctrl, const Serve
             do {
                 code = ray_step<false, true, Idx, true, Reg>(r, vd.tree, stk, normal, t, leaf);
             } while (code == kRunning);
-            view_store<Idx>(vd, int64_t(y) * vd.width + x, code, normal, t, leaf);
+            store_sys<Idx>(vd, int64_t(y) * vd.width + x, code, normal, t, leaf);
         }
-        // release this tile's outputs system-wide, then count it
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+        // this tile's outputs performed, then counted
+        wait_stores();
         __builtin_amdgcn_wave_barrier();
         uint32_t prior = 0;
-        if (lane == 0) prior = __hip_atomic_fetch_add(&done[s], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_SYSTEM);
+        if (lane == 0) prior = __hip_atomic_fetch_add(&done[s], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         prior = __shfl(prior, 0);
-        if (prior + 1 == tilesOf[s] && lane == 0) {
-            // the last tile of the request: reset the slot's counter for its
-            // next request, then publish completion to the host
-            __hip_atomic_store(&done[s], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&complete[s], seq + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        if (lane == 0 && prior + 1 == ld_dev(&tilesOf[s])) {
+            // the request's last tile: every tile's outputs were performed
+            // before its count; reset the slot's counter for its next request,
+            // then publish completion to the host
+            st_dev(&done[s], 0u);
+            wait_stores();
+            st_sys(&complete[s], seq + 1);
         }
     }
 }
@@ -809,7 +847,7 @@ int server_run_schedule(svo_server *s, const svo_request *reqs, const double *ar
         }
         bool progress = false;
         {
-            std::lock_guard<std::mutex> lock(s->mu);
+            std::unique_lock<std::mutex> lock(s->mu);
             if (s->error) return fail(s->error, s->errorMsg);
             for (int i = first; i < next; ++i) {
                 if (finished[size_t(i)]) continue;
@@ -831,9 +869,18 @@ int server_run_schedule(svo_server *s, const svo_request *reqs, const double *ar
             } else if (next == n && s->inflight.empty() && !s->waiting.empty() &&
                        std::chrono::duration<double>(Clock::now() - stalled).count() > 0.5) {
                 return fail(SVO_ERR_LIMIT, "schedule: the kept outputs fill the arena; raise output_budget");
+            } else if (completed < n) {
+                // sleep on the request loop's completion signal (which releases
+                // the lock to it) until the next arrival is due, at most 50 us
+                auto until = Clock::now() + std::chrono::microseconds(50);
+                if (next < n) {
+                    const auto due = t0 + std::chrono::duration_cast<Clock::duration>(
+                                              std::chrono::duration<double>(arrival[next]));
+                    if (due < until) until = due;
+                }
+                s->cv.wait_until(lock, until);
             }
         }
-        for (int k = 0; k < 32; ++k) __builtin_ia32_pause();
     }
     if (tickets)
         for (int i = 0; i < n; ++i) tickets[i] = ids[size_t(i)];
