@@ -588,8 +588,9 @@ def main():
     ap.add_argument("--serve-full", action="store_true", help="5000 requests at every rate")
     ap.add_argument("--no-terrain", action="store_true", help="skip the 8192^3 terrain + shadow-ray workload")
     ap.add_argument("--no-frames", action="store_true", help="skip the viewer-frame workload")
-    ap.add_argument("--only", choices=["terrain", "frames"], default=None,
-                    help="run just that workload and print its JSON (profiling runs)")
+    ap.add_argument("--only", choices=["terrain", "frames", "build"], default=None,
+                    help="run just that workload and print its JSON (profiling runs); build: the headline "
+                         "octree's two GPU builds from its PLY file")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (no multi-GPU path); run with --gpus 1",
@@ -607,6 +608,11 @@ def main():
                                      args.cpu_threads, args.cache)), flush=True)
         return
     tree, config = make_workload(pkg, args.workload, args.cache, no_cpu=args.no_cpu)
+    if args.only == "build":
+        print(json.dumps({k: v for k, v in config.items() if k.startswith(("workload", "octree", "triangles",
+                                                                           "resolution", "device_code", "build"))}),
+              flush=True)
+        return
     tree.upload()
     W, H, V = args.width, args.height, args.views
     n = W * H

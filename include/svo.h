@@ -150,16 +150,17 @@ int svo_render_primary_device(svo_octree *h, const svo_camera *cam, int width, i
                               void *stream);
 
 /* Cost-ordered primary renders (no counterpart in the reference): a tile plan
- * keeps, on the device, each 8x8 tile's cost in the last render of a view
- * size (its largest loop-pass count) and the tile order those costs give.
+ * keeps, on the device, the 8x8 tiles of the last render of a view size
+ * grouped by cost (log2 of the tile's largest loop-pass count, 16 groups).
  * svo_render_primary_planned_device traces every tile as
  * svo_render_primary_device does (same rays, same results) but starts the
- * tiles in the plan's order, most expensive first, records this render's
- * costs and then builds the next order (a one-workgroup kernel), so a view
- * rendered again (the next frame) starts its slow tiles first instead of
+ * tiles in the plan's order, most expensive group first, and files each tile
+ * by its cost in this render for the next one (inside the same launch), so a
+ * view rendered again (the next frame) starts its slow tiles first instead of
  * leaving them to run alone at the end of the launch. The first render of a
- * plan runs in raster order. One plan per image size and stream: renders on
- * one plan must be ordered (one stream). */
+ * plan runs in raster order. A plan holds 128 bytes of device memory per tile
+ * (4 MiB at 1920x1080). One plan per image size and stream: renders on one
+ * plan must be ordered (one stream). */
 typedef struct svo_tile_plan svo_tile_plan;
 int svo_tile_plan_create(int width, int height, svo_tile_plan **out);
 void svo_tile_plan_free(svo_tile_plan *p);
@@ -212,14 +213,16 @@ int64_t svo_frame_depth_count(int width, int height, int stripes);
  * reads) + 24*candidates (each (voxel, triangle) key or source voxel written
  * and read once) + 4*words (output). device_ms is the wall time from the
  * first upload to the last scatter kernel (file reading and the download of
- * the words excluded). */
+ * the words excluded). peak_bytes is the most device memory the build's
+ * temporary arrays held at once (the word array excluded). */
 typedef struct {
     uint64_t triangles;
     uint64_t candidates;
     uint64_t leaves;
     uint64_t words;
     double device_ms;
-    uint64_t reserved[3];
+    uint64_t peak_bytes;
+    uint64_t reserved[2];
 } svo_build_stats;
 int svo_build_stats_get(const svo_octree *h, svo_build_stats *out);
 

@@ -30,14 +30,18 @@
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <future>
 #include <mutex>
+#include <thread>
 #include <unistd.h>
 #include <vector>
 
 #include "build_common.hpp"
+#include "range_alloc.hpp"
 #include "svo_internal.hpp"
 
 namespace svo {
@@ -57,67 +61,165 @@ constexpr int kThreads = 256;
 
 inline unsigned grid_for(uint64_t n) { return unsigned((n + kThreads - 1) / kThreads); }
 
-// Every device allocation of one build, from the device's stream-ordered
-// pool: allocations and frees are ordered on the build stream, so a free never
-// waits for the device and no kernel still in flight can see freed memory.
-// The pool keeps what a build frees for its later allocations (release
-// threshold raised) and is trimmed when the build ends.
+// Device memory and stream of one build, kept between builds: a workspace
+// holds one slab from the device's stream-ordered pool and a stream; a build
+// carves its arrays from the slab first-fit on the host. All of a build's
+// kernels and copies run on the workspace's stream, so an array released and
+// carved again is reused in stream order and no carve waits for the device.
+// A request the slab cannot hold comes from the pool itself; after the build
+// the slab is regrown to the build's peak (up to the reserve bound) so that the
+// next build of that size allocates nothing from the runtime. Concurrent builds
+// on one device take separate workspaces.
+constexpr uint64_t kCarveAlign = 256;
+
+struct Workspace {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    char *slab = nullptr;
+    uint64_t size = 0;
+    RangeAlloc fit;
+};
+
+struct WorkspaceCache {
+    std::mutex mu;
+    std::vector<Workspace *> idle;
+};
+
+WorkspaceCache &workspaces() {
+    static WorkspaceCache *c = new WorkspaceCache();  // never freed: a static destructor may outlive the runtime
+    return *c;
+}
+
+// What the device keeps for later builds: idle slabs plus the pool's free
+// memory, 1/16 of the device and at most 16 GiB (a 2048^3 mesh build peaks
+// near 2 GiB): memory newly obtained from the driver costs milliseconds per
+// large allocation.
+size_t build_reserve() {
+    size_t freeB = 0, total = 0;
+    if (hipMemGetInfo(&freeB, &total) != hipSuccess) return 0;
+    return std::min<size_t>(total / 16, size_t(16) << 30);
+}
+
+bool default_pool(hipMemPool_t &pool) {
+    int dev = 0;
+    return hipGetDevice(&dev) == hipSuccess && hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess;
+}
+
 struct Arena {
-    std::vector<void *> ptrs;
+    Workspace *ws = nullptr;
     hipStream_t stream = nullptr;
     std::chrono::steady_clock::time_point started = std::chrono::steady_clock::now();  // device phase begins
+    struct Block {
+        void *p;
+        uint64_t bytes;
+        bool carved;
+    };
+    std::vector<Block> blocks;
+    uint64_t live = 0, peak = 0;
+
+    // takes an idle workspace of the current device (or a new one)
+    int open() {
+        int dev = 0;
+        HIP_TRY(hipGetDevice(&dev));
+        {
+            WorkspaceCache &c = workspaces();
+            std::lock_guard<std::mutex> lock(c.mu);
+            for (size_t i = 0; i < c.idle.size(); ++i)
+                if (c.idle[i]->device == dev) {
+                    ws = c.idle[i];
+                    c.idle.erase(c.idle.begin() + long(i));
+                    break;
+                }
+        }
+        if (!ws) {
+            ws = new Workspace();
+            ws->device = dev;
+            hipError_t e = hipStreamCreate(&ws->stream);
+            if (e != hipSuccess) {
+                delete ws;
+                ws = nullptr;
+                return fail(SVO_ERR_DEVICE, std::string("builder: hipStreamCreate: ") + hipGetErrorString(e));
+            }
+            hipMemPool_t pool;
+            uint64_t keep = ~uint64_t(0);  // the pool keeps what builds free; trimmed after each build
+            if (default_pool(pool)) (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+        }
+        ws->fit.reset(ws->size);
+        stream = ws->stream;
+        started = std::chrono::steady_clock::now();
+        return SVO_OK;
+    }
+
     ~Arena() {
-        for (void *p : ptrs)
-            if (p) (void)hipFreeAsync(p, stream);
-        if (stream) {
+        if (!ws) return;
+        clear();
+        (void)hipStreamSynchronize(stream);
+        const uint64_t reserve = build_reserve();
+        // regrow the slab to this build's peak (with slack for fragmentation)
+        const uint64_t want = std::min<uint64_t>((peak + peak / 4 + (uint64_t(64) << 20) - 1) >> 26 << 26, reserve);
+        if (want > ws->size) {
+            if (ws->slab) (void)hipFreeAsync(ws->slab, stream);
+            ws->slab = nullptr;
+            ws->size = 0;
+            void *p = nullptr;
+            if (hipMallocAsync(&p, want, stream) == hipSuccess) {
+                ws->slab = static_cast<char *>(p);
+                ws->size = want;
+            }
+            (void)hipStreamSynchronize(stream);
+        }
+        hipMemPool_t pool;
+        WorkspaceCache &c = workspaces();
+        std::lock_guard<std::mutex> lock(c.mu);
+        uint64_t held = ws->size;
+        for (Workspace *w : c.idle)
+            if (w->device == ws->device) held += w->size;
+        if (held > reserve) {  // more idle slabs than the reserve: drop this one
+            if (ws->slab) (void)hipFreeAsync(ws->slab, stream);
             (void)hipStreamSynchronize(stream);
             (void)hipStreamDestroy(stream);
+            delete ws;
+        } else {
+            c.idle.push_back(ws);
         }
-        // keep a bounded reserve for the next build (1/16 of the device, at most
-        // 16 GiB: a 2048^3 mesh build peaks near 4 GiB): memory newly obtained
-        // from the driver costs milliseconds per large allocation
-        hipMemPool_t pool;
-        if (default_pool(pool)) (void)hipMemPoolTrimTo(pool, pool_reserve());
+        ws = nullptr;
+        if (default_pool(pool)) (void)hipMemPoolTrimTo(pool, reserve > held ? reserve - held : 0);
     }
-    static size_t pool_reserve() {
-        size_t freeB = 0, total = 0;
-        if (hipMemGetInfo(&freeB, &total) != hipSuccess) return 0;
-        return std::min<size_t>(total / 16, size_t(16) << 30);
-    }
-    static bool default_pool(hipMemPool_t &pool) {
-        int dev = 0;
-        return hipGetDevice(&dev) == hipSuccess && hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess;
-    }
-    bool tuned = false;
+
     template <typename T>
     int alloc(T *&out, uint64_t n) {
-        if (!tuned) {
-            hipMemPool_t pool;
-            uint64_t keep = ~uint64_t(0);
-            if (default_pool(pool)) (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
-            tuned = true;
-        }
         out = nullptr;
+        const uint64_t bytes = (std::max<uint64_t>(n, 1) * sizeof(T) + kCarveAlign - 1) / kCarveAlign * kCarveAlign;
+        uint64_t off = 0;
         void *p = nullptr;
-        hipError_t e = hipMallocAsync(&p, std::max<uint64_t>(n, 1) * sizeof(T), stream);
-        if (e != hipSuccess) return fail(SVO_ERR_NOMEM, std::string("builder: hipMallocAsync: ") + hipGetErrorString(e));
-        ptrs.push_back(p);
+        bool carved = ws->slab && ws->fit.alloc(bytes, off);
+        if (carved) {
+            p = ws->slab + off;
+        } else {
+            hipError_t e = hipMallocAsync(&p, bytes, stream);
+            if (e != hipSuccess)
+                return fail(SVO_ERR_NOMEM, std::string("builder: hipMallocAsync: ") + hipGetErrorString(e));
+        }
+        blocks.push_back({p, bytes, carved});
+        live += bytes;
+        peak = std::max(peak, live);
         out = static_cast<T *>(p);
         return SVO_OK;
     }
+    void drop(Block &b) {
+        if (b.carved) ws->fit.release(uint64_t(static_cast<char *>(b.p) - ws->slab), b.bytes);
+        else (void)hipFreeAsync(b.p, stream);
+        live -= b.bytes;
+        b.p = nullptr;
+    }
     void release(void *p) {
-        for (auto &q : ptrs)
-            if (q == p) {
-                (void)hipFreeAsync(q, stream);
-                q = nullptr;
-            }
+        for (Block &b : blocks)
+            if (b.p == p) drop(b);
     }
     void clear() {
-        for (void *&p : ptrs)
-            if (p) {
-                (void)hipFreeAsync(p, stream);
-                p = nullptr;
-            }
+        for (Block &b : blocks)
+            if (b.p) drop(b);
+        blocks.clear();
     }
 };
 
@@ -924,7 +1026,11 @@ __global__ void write_nodes_kernel(const uint32_t *__restrict__ mask, const uint
     if (large[k]) w |= 0x10000u;
     const uint64_t p = pos[k];
     const uint64_t off = B[k] - p;
-    if (p >= W || (far[k] && p + 1 >= W)) {
+    // the traversal reads child slot B + (non-leaf children before it) * (1 + large)
+    // or B + (children before it) for a leaf: the block must follow the node
+    // and end inside the array
+    const uint64_t kids = uint64_t(__popc(m)), extent = leafParents ? kids : kids * (large[k] ? 2u : 1u);
+    if (p >= W || (far[k] && p + 1 >= W) || (kids && (B[k] <= p || B[k] > W || extent > W - B[k]))) {
         atomicOr(bad, 1u);
         return;
     }
@@ -1366,12 +1472,18 @@ struct Level {
     uint32_t *word = nullptr;   // leaves: material word
 };
 
-// Two pinned chunks shared by every download of the process (never freed: a
-// static destructor may run after the HIP runtime has gone).
+// Large copies between pageable host memory and the device go through pinned
+// chunks shared by the process (never freed: a static destructor may run after
+// the HIP runtime has gone): the DMA moves one chunk while a few host threads
+// copy another between the chunk and the pageable side, each a slice of it (a
+// copy straight from or into pageable memory goes through the runtime's own
+// staging at a few GB/s).
 struct Staging {
     static constexpr uint64_t kChunk = uint64_t(8) << 20;  // bytes
+    static constexpr int kChunks = 4;
     std::mutex mu;
-    void *buf[2] = {nullptr, nullptr};
+    void *buf[kChunks] = {};
+    hipEvent_t ev[kChunks] = {};
 };
 
 Staging &staging() {
@@ -1379,14 +1491,86 @@ Staging &staging() {
     return *s;
 }
 
-// Copies W device words into out: the DMA fills one pinned chunk while the host
-// copies the previous one out (a copy straight into pageable memory goes
-// through the runtime's own staging at a few GB/s).
-int download_words(const uint32_t *src, uint64_t W, WordVec &out, hipStream_t st) {
+// toDevice: host -> pinned chunk -> device; else device -> pinned chunk -> host.
+int staged_copy(void *dev, void *host, uint64_t bytes, bool toDevice, hipStream_t st) {
+    Staging &sg = staging();
+    std::lock_guard<std::mutex> lock(sg.mu);
+    for (int k = 0; k < Staging::kChunks; ++k) {
+        if (!sg.buf[k]) HIP_TRY(hipHostMalloc(&sg.buf[k], Staging::kChunk, hipHostMallocDefault));
+        if (!sg.ev[k]) HIP_TRY(hipEventCreateWithFlags(&sg.ev[k], hipEventDisableTiming));
+    }
+    uint8_t *d8 = static_cast<uint8_t *>(dev), *h8 = static_cast<uint8_t *>(host);
+    const uint64_t n = (bytes + Staging::kChunk - 1) / Staging::kChunk;
+    auto len_of = [&](uint64_t j) { return std::min(Staging::kChunk, bytes - j * Staging::kChunk); };
+    const int P = int(std::min<uint64_t>({4, std::max(1u, std::thread::hardware_concurrency()), n}));
+    // chunk j is ready for the host threads once ready > j; slices done per chunk slot
+    std::atomic<uint64_t> ready{0};
+    std::atomic<int> done[Staging::kChunks];
+    for (auto &d : done) d.store(0);
+    std::atomic<bool> abort{false};
+    auto host_slice = [&](uint64_t j, int k) {
+        const uint64_t len = len_of(j), a = len * uint64_t(k) / uint64_t(P), b = len * uint64_t(k + 1) / uint64_t(P);
+        uint8_t *chunk = static_cast<uint8_t *>(sg.buf[j % Staging::kChunks]);
+        uint8_t *side = h8 + j * Staging::kChunk;
+        if (toDevice) std::memcpy(chunk + a, side + a, b - a);
+        else std::memcpy(side + a, chunk + a, b - a);
+        done[j % Staging::kChunks].fetch_add(1, std::memory_order_acq_rel);
+    };
+    auto helper = [&](int k) {
+        for (uint64_t j = 0; j < n; ++j) {
+            while (ready.load(std::memory_order_acquire) <= j) {
+                if (abort.load(std::memory_order_relaxed)) return;
+                std::this_thread::yield();
+            }
+            host_slice(j, k);
+        }
+    };
+    std::vector<std::thread> helpers;
     try {
-        out.resize(W);
+        for (int k = 1; k < P; ++k) helpers.emplace_back(helper, k);
     } catch (...) {
-        return fail(SVO_ERR_NOMEM, "builder: host word array");
+        abort.store(true);
+        for (std::thread &t : helpers) t.join();
+        return fail(SVO_ERR_NOMEM, "builder: copy threads");
+    }
+    hipError_t e = hipSuccess;
+    auto dma = [&](uint64_t j) {
+        const int slot = int(j % Staging::kChunks);
+        if (toDevice) e = hipMemcpyAsync(d8 + j * Staging::kChunk, sg.buf[slot], len_of(j), hipMemcpyHostToDevice, st);
+        else e = hipMemcpyAsync(sg.buf[slot], d8 + j * Staging::kChunk, len_of(j), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipEventRecord(sg.ev[slot], st);
+    };
+    if (!toDevice)
+        for (uint64_t j = 0; j < std::min<uint64_t>(n, Staging::kChunks) && e == hipSuccess; ++j) dma(j);
+    for (uint64_t j = 0; j < n && e == hipSuccess; ++j) {
+        const int slot = int(j % Staging::kChunks);
+        // download: chunk j has landed; upload: the chunk's previous DMA has left it
+        if (!toDevice || j >= uint64_t(Staging::kChunks)) e = hipEventSynchronize(sg.ev[slot]);
+        if (e != hipSuccess) break;
+        ready.store(j + 1, std::memory_order_release);
+        host_slice(j, 0);
+        while (done[slot].load(std::memory_order_acquire) < P) std::this_thread::yield();
+        done[slot].store(0, std::memory_order_relaxed);
+        if (toDevice) dma(j);
+        else if (j + Staging::kChunks < n) dma(j + Staging::kChunks);
+    }
+    if (e != hipSuccess) abort.store(true);
+    for (std::thread &t : helpers) t.join();
+    // no DMA may still use the chunks once the lock is released
+    hipError_t e2 = hipStreamSynchronize(st);
+    if (e == hipSuccess) e = e2;
+    if (e != hipSuccess) return fail(SVO_ERR_DEVICE, std::string("builder: staged copy: ") + hipGetErrorString(e));
+    return SVO_OK;
+}
+
+// Copies W device words into out (sized to W by the caller or here).
+int download_words(const uint32_t *src, uint64_t W, WordVec &out, hipStream_t st) {
+    if (out.size() != W) {
+        try {
+            out.resize(W);
+        } catch (...) {
+            return fail(SVO_ERR_NOMEM, "builder: host word array");
+        }
     }
     const uint64_t bytes = W * sizeof(uint32_t);
     if (bytes <= Staging::kChunk) {
@@ -1394,34 +1578,7 @@ int download_words(const uint32_t *src, uint64_t W, WordVec &out, hipStream_t st
         HIP_TRY(hipStreamSynchronize(st));
         return SVO_OK;
     }
-    Staging &sg = staging();
-    std::lock_guard<std::mutex> lock(sg.mu);
-    for (void *&b : sg.buf)
-        if (!b) HIP_TRY(hipHostMalloc(&b, Staging::kChunk, hipHostMallocDefault));
-    hipEvent_t ev[2] = {nullptr, nullptr};
-    hipError_t e = hipEventCreateWithFlags(&ev[0], hipEventDisableTiming);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&ev[1], hipEventDisableTiming);
-    const uint8_t *s8 = reinterpret_cast<const uint8_t *>(src);
-    uint8_t *d8 = reinterpret_cast<uint8_t *>(out.data());
-    const uint64_t n = (bytes + Staging::kChunk - 1) / Staging::kChunk;
-    auto len_of = [&](uint64_t i) { return std::min(Staging::kChunk, bytes - i * Staging::kChunk); };
-    for (uint64_t i = 0; i <= n && e == hipSuccess; ++i) {
-        // chunk i goes into the buffer chunk i-2 used; the host copied that out in the previous pass
-        if (i < n) {
-            e = hipMemcpyAsync(sg.buf[i & 1], s8 + i * Staging::kChunk, len_of(i), hipMemcpyDeviceToHost, st);
-            if (e == hipSuccess) e = hipEventRecord(ev[i & 1], st);
-        }
-        if (i > 0 && e == hipSuccess) {
-            const uint64_t j = i - 1;
-            e = hipEventSynchronize(ev[j & 1]);
-            if (e == hipSuccess) std::memcpy(d8 + j * Staging::kChunk, sg.buf[j & 1], len_of(j));
-        }
-    }
-    if (e != hipSuccess) (void)hipStreamSynchronize(st);  // no copy may still target the staging chunks
-    for (hipEvent_t x : ev)
-        if (x) (void)hipEventDestroy(x);
-    if (e != hipSuccess) return fail(SVO_ERR_DEVICE, std::string("builder: download: ") + hipGetErrorString(e));
-    return SVO_OK;
+    return staged_copy(const_cast<uint32_t *>(src), out.data(), bytes, false, st);
 }
 
 // Builds the word array from sorted, unique leaf keys/words. blockFlags marks
@@ -1526,6 +1683,33 @@ int emit_tree(Arena &ar, const Geometry &g, uint64_t *leafKey, uint32_t *leafWor
     if ((r = read_words(rep, report, 2, st))) return r;
     if (report[1] != 1) return fail(SVO_ERR_DEVICE, "builder: root level does not hold one node");
     const uint64_t W = 1 + report[0];
+    // the word array, allocated before the host array below: both map memory
+    // into the process, and a device allocation waits while other threads
+    // populate host pages
+    uint32_t *words;
+    {
+        void *pw = nullptr;
+        hipError_t e = hipMalloc(&pw, W * sizeof(uint32_t));
+        if (e != hipSuccess) return fail(SVO_ERR_NOMEM, std::string("builder: word array: ") + hipGetErrorString(e));
+        words = static_cast<uint32_t *>(pw);
+    }
+    struct WordsGuard {  // frees the word array on every early return
+        uint32_t *&w;
+        bool keep = false;
+        ~WordsGuard() {
+            if (!keep && w) (void)hipFree(w);
+        }
+    } guard{words};
+    // the host array is mapped and populated by other threads while the
+    // device places and scatters (the future's destructor waits on every path)
+    std::future<bool> hostArray = std::async(std::launch::async, [&out, W]() {
+        try {
+            out.words.resize(W);
+            return true;
+        } catch (...) {
+            return false;
+        }
+    });
 
     // placement, top-down
     {
@@ -1545,18 +1729,8 @@ int emit_tree(Arena &ar, const Geometry &g, uint64_t *leafKey, uint32_t *leafWor
     }
 
     // scatter
-    uint32_t *words;
-    {
-        void *pw = nullptr;
-        hipError_t e = hipMalloc(&pw, W * sizeof(uint32_t));
-        if (e != hipSuccess) return fail(SVO_ERR_NOMEM, std::string("builder: word array: ") + hipGetErrorString(e));
-        words = static_cast<uint32_t *>(pw);
-    }
     uint32_t *bad;
-    if ((r = ar.alloc(bad, 1)) || (r = fill32(bad, 1, 0, st))) {
-        (void)hipFree(words);
-        return r;
-    }
+    if ((r = ar.alloc(bad, 1)) || (r = fill32(bad, 1, 0, st))) return r;
     for (int d = 0; d < D; ++d) {
         Level &p = lv[d];
         if (p.bound)
@@ -1571,21 +1745,25 @@ int emit_tree(Arena &ar, const Geometry &g, uint64_t *leafKey, uint32_t *leafWor
     out.build.leaves = L;
     out.build.words = W;
     out.build.device_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ar.started).count();
+    out.build.peak_bytes = ar.peak;
     uint32_t badH = 0;
-    if (e == hipSuccess) e = hipMemcpy(&badH, bad, 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpyAsync(&badH, bad, 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess || badH) {
-        (void)hipFree(words);
         if (badH) return fail(SVO_ERR_DEVICE, "builder: placement outside the word array (internal error)");
         return fail(SVO_ERR_DEVICE, std::string("builder: scatter: ") + hipGetErrorString(e));
     }
-    if ((r = download_words(words, W, out.words, st))) {
-        (void)hipFree(words);
-        return r;
-    }
+    if (!hostArray.get()) return fail(SVO_ERR_NOMEM, "builder: host word array");
+    if ((r = download_words(words, W, out.words, st))) return r;
     int dev = 0;
     (void)hipGetDevice(&dev);
+    guard.keep = true;
     out.dwords = words;
     out.device = dev;
+    // with leaves, the deepest descriptors with children are the leaves'
+    // parents (level D - 1): the depth octree_validate reports, D + 1. The
+    // scatter checked every node's slot and child block against the array.
+    out.depth = L ? D + 1 : 0;
     out.center[0] = float(g.dataW) * 0.5f / float(g.side);  // VoxelData::getCenter, VoxelData.cpp:297-303
     out.center[1] = float(g.dataH) * 0.5f / float(g.side);
     out.center[2] = float(g.dataD) * 0.5f / float(g.side);
@@ -2077,7 +2255,7 @@ static int mesh_entry(const float *verts, int64_t nverts, int has_normals, const
     if (T >= (int64_t(1) << 32)) return fail(SVO_ERR_LIMIT, "too many triangles");
 
     Arena ar;
-    HIP_TRY(hipStreamCreate(&ar.stream));
+    if (int ro = ar.open()) return ro;
     hipStream_t st = ar.stream;
     float *dv;
     int32_t *dfc, *dfi;
@@ -2191,7 +2369,7 @@ static int ply_entry(const char *path, int resolution, const svo_build_opts *opt
     if (T >= (int64_t(1) << 32)) return fail(SVO_ERR_LIMIT, "too many triangles");
 
     Arena ar;
-    HIP_TRY(hipStreamCreate(&ar.stream));
+    if (int ro = ar.open()) return ro;
     hipStream_t st = ar.stream;
     uint8_t *raw;
     float *dv;
@@ -2206,7 +2384,11 @@ static int ply_entry(const char *path, int resolution, const svo_build_opts *opt
         (r = ar.alloc(parts, nparts)))
         return r;
     if ((r = fill32(flags, 1, 0, st))) return r;
-    HIP_TRY(hipMemcpyAsync(raw, L.buf.data() + vbody, span, hipMemcpyHostToDevice, st));
+    if (span > Staging::kChunk) {
+        if ((r = staged_copy(raw, const_cast<uint8_t *>(L.buf.data() + vbody), span, true, st))) return r;
+    } else {
+        HIP_TRY(hipMemcpyAsync(raw, L.buf.data() + vbody, span, hipMemcpyHostToDevice, st));
+    }
     if (nv) hipLaunchKernelGGL(ply_vertices_kernel, dim3(grid_for(nv)), dim3(kThreads), 0, st, raw, nv, vl, dv);
     hipLaunchKernelGGL(ply_faces_kernel, dim3(grid_for(nf)), dim3(kThreads), 0, st, raw + (fbody - vbody), nf, fl, nv, dfc,
                        dfs, dts, dfi, flags);
@@ -2269,7 +2451,7 @@ int build_grid(const uint32_t *grid, int w, int h, int d, const svo_build_opts *
     if ((r = check_blocks(g, nBlocks))) return r;
     const uint64_t N = uint64_t(w) * h * d;
     Arena ar;
-    HIP_TRY(hipStreamCreate(&ar.stream));
+    if (int ro = ar.open()) return ro;
     hipStream_t st = ar.stream;
     uint32_t *dg, *flag;
     uint64_t *at;
@@ -2433,7 +2615,7 @@ int build_terrain(int n, uint32_t seed, const svo_build_opts *opts, svo_octree &
     uint64_t nBlocks;
     if ((r = check_blocks(g, nBlocks))) return r;
     Arena ar;
-    HIP_TRY(hipStreamCreate(&ar.stream));
+    if (int ro = ar.open()) return ro;
     hipStream_t st = ar.stream;
     uint64_t *key, V;
     uint32_t *word;
@@ -2455,7 +2637,7 @@ int terrain_volume(int n, uint32_t seed, uint32_t *grid) {
     if (r) return r;
     if (n > 1024) return fail(SVO_ERR_LIMIT, "dense terrain volumes are limited to 1024^3");
     Arena ar;
-    HIP_TRY(hipStreamCreate(&ar.stream));
+    if (int ro = ar.open()) return ro;
     hipStream_t st = ar.stream;
     uint64_t *key, V;
     uint32_t *word;
@@ -2474,7 +2656,7 @@ int terrain_heights(int n, uint32_t seed, int32_t *heights, float *h) {
     int r = terrain_check(n);
     if (r) return r;
     Arena ar;
-    HIP_TRY(hipStreamCreate(&ar.stream));
+    if (int ro = ar.open()) return ro;
     float *hh;
     int32_t *H;
     if ((r = terrain_columns(ar, n, seed, hh, H))) return r;
@@ -2493,7 +2675,7 @@ int build_sdf(const svo_sdf_primitive *prims, int count, int n, float shade, con
     uint64_t nBlocks;
     if ((r = check_blocks(g, nBlocks))) return r;
     Arena ar;
-    HIP_TRY(hipStreamCreate(&ar.stream));
+    if (int ro = ar.open()) return ro;
     hipStream_t st = ar.stream;
     uint64_t *key, V;
     uint32_t *word;
@@ -2511,7 +2693,7 @@ int build_sdf(const svo_sdf_primitive *prims, int count, int n, float shade, con
 int sdf_volume(const svo_sdf_primitive *prims, int count, int n, float shade, uint32_t *grid) {
     if (n > 1024) return fail(SVO_ERR_LIMIT, "dense SDF volumes are limited to 1024^3");
     Arena ar;
-    HIP_TRY(hipStreamCreate(&ar.stream));
+    if (int ro = ar.open()) return ro;
     hipStream_t st = ar.stream;
     uint64_t *key, V;
     uint32_t *word;

@@ -13,7 +13,10 @@
 // non-leaf = valid or 0, where both reduce to the k-th valid child at
 // index + offset + k*(1+large); other masks (non-leaf bits without a valid bit
 // included) are checked at the addresses the traversal will read.
+#include <fcntl.h>
 #include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
 
 #include <algorithm>
 #include <atomic>
@@ -151,13 +154,56 @@ void *map_large(size_t bytes) {
     (void)madvise(p, bytes, MADV_HUGEPAGE);
 #endif
 #ifdef MADV_POPULATE_WRITE
-    (void)madvise(p, bytes, MADV_POPULATE_WRITE);
+    // the kernel clears every page it populates: arrays of tens of MiB and
+    // more are populated by a few threads, in 2 MiB-aligned parts
+    constexpr size_t kPart = size_t(2) << 20;
+    const size_t parts = (bytes + kPart - 1) / kPart;
+    const size_t nt = std::min<size_t>({size_t(4), std::max(1u, std::thread::hardware_concurrency()), parts / 8 + 1});
+    auto populate = [p, bytes, parts, nt](size_t k) {
+        const size_t a = parts * k / nt * kPart, b = std::min(bytes, parts * (k + 1) / nt * kPart);
+        if (a < b) (void)madvise(static_cast<char *>(p) + a, b - a, MADV_POPULATE_WRITE);
+    };
+    std::vector<std::thread> pool;
+    try {
+        for (size_t k = 1; k < nt; ++k) pool.emplace_back(populate, k);
+    } catch (...) {
+        pool.clear();  // no threads: populate the rest on this one
+        for (size_t k = 1; k < nt; ++k) populate(k);
+    }
+    populate(0);
+    for (std::thread &t : pool) t.join();
 #endif
     return p;
 }
 
 void unmap_large(void *p, size_t bytes) {
     if (p) (void)munmap(p, bytes);
+}
+
+// ---- read-only file mappings ----
+
+bool FileView::open(const char *path) {
+    close();
+    const int fd = ::open(path, O_RDONLY | O_CLOEXEC);
+    if (fd < 0) return false;
+    struct stat sb;
+    bool ok = fstat(fd, &sb) == 0 && S_ISREG(sb.st_mode);
+    if (ok && sb.st_size > 0) {
+        void *p = mmap(nullptr, size_t(sb.st_size), PROT_READ, MAP_PRIVATE | MAP_POPULATE, fd, 0);
+        if (p == MAP_FAILED) ok = false;
+        else {
+            data_ = static_cast<const uint8_t *>(p);
+            size_ = size_t(sb.st_size);
+        }
+    }
+    ::close(fd);
+    return ok;
+}
+
+void FileView::close() {
+    if (data_) (void)munmap(const_cast<uint8_t *>(data_), size_);
+    data_ = nullptr;
+    size_ = 0;
 }
 
 }  // namespace svo

@@ -58,18 +58,8 @@ bool is_int_type(PType t) { return t != P_FLOAT && t != P_DOUBLE; }
 int ply_open(const char *path_in, PlyLayout &L) {
     std::string path(path_in);
     if (path.size() < 4 || path.compare(path.size() - 4, 4, ".ply") != 0) path += ".ply";
-    FILE *fp = std::fopen(path.c_str(), "rb");
-    if (!fp) return fail(SVO_ERR_IO, "cannot open PLY " + path);
-    std::vector<uint8_t> &buf = L.buf;
-    {
-        std::fseek(fp, 0, SEEK_END);
-        long n = std::ftell(fp);
-        std::fseek(fp, 0, SEEK_SET);
-        if (n < 0) { std::fclose(fp); return fail(SVO_ERR_IO, "cannot read PLY " + path); }
-        buf.resize(size_t(n));
-        if (n && std::fread(buf.data(), 1, size_t(n), fp) != size_t(n)) { std::fclose(fp); return fail(SVO_ERR_IO, "short read " + path); }
-        std::fclose(fp);
-    }
+    if (!L.buf.open(path.c_str())) return fail(SVO_ERR_IO, "cannot open PLY " + path);
+    const FileView &buf = L.buf;
     // ---- header ----
     size_t pos = 0;
     auto next_line = [&](std::string &line) -> bool {
@@ -179,7 +169,7 @@ int read_ply(const char *path_in, std::vector<float> &verts, int &has_normals, s
 
 int ply_parse_host(const PlyLayout &L, std::vector<float> &verts, int &has_normals, std::vector<int32_t> &face_counts,
                    std::vector<int32_t> &face_idx) {
-    const std::vector<uint8_t> &buf = L.buf;
+    const FileView &buf = L.buf;
     const std::vector<PlyElem> &elems = L.elems;
     const PlyElem *ve = &elems[size_t(L.vertex)], *fe = &elems[size_t(L.face)];
     const int format = L.binary ? 1 : 0;

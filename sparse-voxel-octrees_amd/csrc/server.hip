@@ -12,8 +12,8 @@
 //    new request's view descriptor into device memory, extends the launch's
 //    table of first-tile indices (one entry per request), and publishes the
 //    new (request count, tile count) pair with a release store;
-//  - every other wave claims one global tile index at a time from a device
-//    counter; indices beyond the published tiles are held until published
+//  - every other wave claims one global tile index at a time from one of 8
+//    device counters; indices beyond the published tiles are held until published
 //    (the wave polls the published pair), so a request that arrives while
 //    the kernel is tracing others is traced by the next waves that free up;
 //  - a wave traces its 8x8 tile (march.hpp, the reference's raymarch per
@@ -56,6 +56,7 @@
 #include <vector>
 
 #include "march.hpp"
+#include "range_alloc.hpp"
 #include "svo_internal.hpp"
 #include "view_desc.hpp"
 
@@ -86,13 +87,37 @@ struct ServeSlot {       // one ring entry, written by the host before it publis
 };
 static_assert(sizeof(ServeSlot) == 192, "ServeSlot layout");
 
+// Tiles are claimed from kCursors counters, each on a 128-byte line of its
+// own: workgroup b claims from counter b % kCursors, and that counter's i-th
+// claim is tile i * kCursors + b % kCursors. One counter for every wave would
+// take one device-scope atomic per tile on one address, and those serialise
+// (about 12 ns each: a 9,500-tile request spends ~110 us in them). A request's
+// finished tiles are counted the same way, per class of tile index modulo
+// kCursors, and the class counters that reach their share count the request.
+constexpr int kCursors = 8;
+constexpr int kLineWords = 32;  // uint32 per 128-byte line
+
 // Device memory: launch state (cleared before each launch).
 struct ServeState {
-    unsigned long long cursor;     // next launch-relative tile index to claim
+    struct {
+        unsigned long long next;   // claims made from this counter
+        uint64_t pad[15];
+    } cursor[kCursors];
     uint64_t published;            // (requests << 44) | tiles, stored by the leader after the tables
     unsigned int stop;             // set by the leader: waiting waves leave
-    unsigned int pad[11];
+    unsigned int pad[29];
 };
+
+// done[]: per ring slot, kCursors class counters and one count of finished
+// classes, each on its own line.
+__device__ __forceinline__ uint32_t *done_word(uint32_t *done, int slot, int k) {
+    return done + (size_t(slot) * (kCursors + 1) + size_t(k)) * kLineWords;
+}
+// tiles t in [a, b) with t % kCursors == c
+__device__ __forceinline__ uint32_t class_tiles(uint64_t a, uint64_t b, int c) {
+    const auto below = [c](uint64_t x) { return (x + uint64_t(kCursors - 1 - c)) / kCursors; };
+    return uint32_t(below(b) - below(a));
+}
 
 // Memory ordering without cache maintenance. A release or acquire at agent
 // or system scope writes back or invalidates the whole L2 of the wave's XCD
@@ -141,7 +166,7 @@ __device__ __forceinline__ void store_sys(const ViewDesc &vd, int64_t i, int cod
 // workgroup 0 is the leader, every other wave traces tiles.
 template <typename Idx, int Reg>
 __global__ void __launch_bounds__(256) serve_kernel(ServeCtrl *ctrl, const ServeSlot *hslots, ServeState *st,
-                                                    ViewDesc *dslots, uint32_t *tilesOf, uint64_t *dbase,
+                                                    ViewDesc *dslots, uint64_t *dbase,
                                                     uint32_t *done, uint64_t *complete, int ring, uint64_t seq0,
                                                     uint32_t maxReqs, int slots) {
     extern __shared__ __align__(16) unsigned char lds[];
@@ -172,7 +197,6 @@ __global__ void __launch_bounds__(256) serve_kernel(ServeCtrl *ctrl, const Serve
                         uint32_t *src = const_cast<uint32_t *>(reinterpret_cast<const uint32_t *>(&hslots[s].desc)) + (lane & 31);
                         st_dev(reinterpret_cast<uint32_t *>(&dslots[s]) + (lane & 31), ld_sys(src));
                         tiles = ld_sys(const_cast<uint32_t *>(&hslots[s].tiles));
-                        if ((lane & 31) == 0) st_dev(&tilesOf[s], tiles);
                     }
                     const uint32_t t0 = __shfl(tiles, 0), t1 = __shfl(tiles, 32);
                     if (lane == 0) st_dev(&dbase[i], total);
@@ -213,9 +237,10 @@ __global__ void __launch_bounds__(256) serve_kernel(ServeCtrl *ctrl, const Serve
     Stack<Idx> stk(lds + size_t(wave) * slots * 64 * (sizeof(Idx) + sizeof(float)), 64, slots, lane);
     uint64_t known = 0;  // last published pair seen
     uint32_t q = 0;      // launch-relative request of the last tile
+    const int cls = int(blockIdx.x % kCursors);
     for (;;) {
         unsigned long long g = 0;
-        if (lane == 0) g = atomicAdd(&st->cursor, 1ull);
+        if (lane == 0) g = atomicAdd(&st->cursor[cls].next, 1ull) * kCursors + uint64_t(cls);
         g = __shfl(g, 0);
         // hold the tile index until it is published (or the launch ends)
         while (g >= (known & kTileMask)) {
@@ -260,19 +285,28 @@ __global__ void __launch_bounds__(256) serve_kernel(ServeCtrl *ctrl, const Serve
             } while (code == kRunning);
             store_sys<Idx>(vd, int64_t(y) * vd.width + x, code, normal, t, leaf);
         }
-        // this tile's outputs performed, then counted
+        // this tile's outputs performed, then counted in its class
         wait_stores();
         __builtin_amdgcn_wave_barrier();
-        uint32_t prior = 0;
-        if (lane == 0) prior = __hip_atomic_fetch_add(&done[s], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-        prior = __shfl(prior, 0);
-        if (lane == 0 && prior + 1 == ld_dev(&tilesOf[s])) {
-            // the request's last tile: every tile's outputs were performed
-            // before its count; reset the slot's counter for its next request,
-            // then publish completion to the host
-            st_dev(&done[s], 0u);
-            wait_stores();
-            st_sys(&complete[s], seq + 1);
+        if (lane == 0) {
+            const uint64_t a = ld_dev(&dbase[q]), b = ld_dev(&dbase[q + 1]);  // the request's tiles
+            uint32_t *cnt = done_word(done, s, cls);
+            const uint32_t prior = __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (prior + 1 == class_tiles(a, b, cls)) {
+                // the class's last tile: reset its counter for the slot's next
+                // request (performed before the class is counted), count the class
+                st_dev(cnt, 0u);
+                wait_stores();
+                uint32_t *all = done_word(done, s, kCursors);
+                const uint32_t classes = uint32_t(b - a < uint64_t(kCursors) ? b - a : uint64_t(kCursors));
+                if (__hip_atomic_fetch_add(all, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1 == classes) {
+                    // the request's last class: every tile's outputs were
+                    // performed before its count; reset, then publish completion
+                    st_dev(all, 0u);
+                    wait_stores();
+                    st_sys(&complete[s], seq + 1);
+                }
+            }
         }
     }
 }
@@ -284,44 +318,6 @@ struct Ticket {
     uint32_t tiles = 0;
     int state = 0;                // 0 waiting, 1 in flight, 2 done
     double submitted = 0.0, completed = 0.0;
-};
-
-// First-fit allocator over the output arena (offsets in bytes, kAlign units).
-class Arena {
-public:
-    void reset(uint64_t size) {
-        free_.clear();
-        if (size) free_[0] = size;
-    }
-    bool alloc(uint64_t bytes, uint64_t &off) {
-        for (auto it = free_.begin(); it != free_.end(); ++it) {
-            if (it->second < bytes) continue;
-            off = it->first;
-            const uint64_t rest = it->second - bytes;
-            free_.erase(it);
-            if (rest) free_[off + bytes] = rest;
-            return true;
-        }
-        return false;
-    }
-    void release(uint64_t off, uint64_t bytes) {
-        auto next = free_.lower_bound(off);
-        if (next != free_.end() && off + bytes == next->first) {
-            bytes += next->second;
-            next = free_.erase(next);
-        }
-        if (next != free_.begin()) {
-            auto prev = std::prev(next);
-            if (prev->first + prev->second == off) {
-                prev->second += bytes;
-                return;
-            }
-        }
-        free_[off] = bytes;
-    }
-
-private:
-    std::map<uint64_t, uint64_t> free_;
 };
 
 uint64_t round_up(uint64_t b) { return (b + kAlign - 1) / kAlign * kAlign; }
@@ -353,14 +349,13 @@ struct svo_server {
     uint64_t *complete = nullptr;     // pinned, one word per ring slot
     svo::ServeState *st = nullptr;    // device
     svo::ViewDesc *dslots = nullptr;  // device
-    uint32_t *tilesOf = nullptr;      // device
     uint64_t *dbase = nullptr;        // device
     uint32_t *done = nullptr;         // device
     uint8_t *arena = nullptr;         // device or pinned
     hipStream_t stream = nullptr;
     hipStream_t copyStream = nullptr;  // copies of completed outputs to the host
     hipEvent_t ended = nullptr;
-    svo::Arena alloc;
+    svo::RangeAlloc alloc;  // first fit over the output arena (offsets in bytes, kAlign units)
     // request state
     std::mutex mu;
     std::condition_variable cv;
@@ -409,16 +404,16 @@ bool start_launch(svo_server &s) {
     const dim3 grid(unsigned(s.cus * blocksPerCU)), block(256);
     if (!s.wide && s.reg == 1)
         hipLaunchKernelGGL((serve_kernel<uint32_t, 1>), grid, block, lds, s.stream, s.ctrl, s.hslots, s.st, s.dslots,
-                           s.tilesOf, s.dbase, s.done, s.complete, s.ring, s.seq0, s.maxReqs, s.stackSlots);
+                           s.dbase, s.done, s.complete, s.ring, s.seq0, s.maxReqs, s.stackSlots);
     else if (!s.wide)
         hipLaunchKernelGGL((serve_kernel<uint32_t, 3>), grid, block, lds, s.stream, s.ctrl, s.hslots, s.st, s.dslots,
-                           s.tilesOf, s.dbase, s.done, s.complete, s.ring, s.seq0, s.maxReqs, s.stackSlots);
+                           s.dbase, s.done, s.complete, s.ring, s.seq0, s.maxReqs, s.stackSlots);
     else if (s.reg == 1)
         hipLaunchKernelGGL((serve_kernel<uint64_t, 1>), grid, block, lds, s.stream, s.ctrl, s.hslots, s.st, s.dslots,
-                           s.tilesOf, s.dbase, s.done, s.complete, s.ring, s.seq0, s.maxReqs, s.stackSlots);
+                           s.dbase, s.done, s.complete, s.ring, s.seq0, s.maxReqs, s.stackSlots);
     else
         hipLaunchKernelGGL((serve_kernel<uint64_t, 3>), grid, block, lds, s.stream, s.ctrl, s.hslots, s.st, s.dslots,
-                           s.tilesOf, s.dbase, s.done, s.complete, s.ring, s.seq0, s.maxReqs, s.stackSlots);
+                           s.dbase, s.done, s.complete, s.ring, s.seq0, s.maxReqs, s.stackSlots);
     e = hipGetLastError();
     if (e == hipSuccess) e = hipEventRecord(s.ended, s.stream);
     if (e != hipSuccess) {
@@ -611,7 +606,6 @@ void free_server(svo_server *s) {
     }
     if (s->st) (void)hipFree(s->st);
     if (s->dslots) (void)hipFree(s->dslots);
-    if (s->tilesOf) (void)hipFree(s->tilesOf);
     if (s->dbase) (void)hipFree(s->dbase);
     if (s->done) (void)hipFree(s->done);
     if (s->ctrl) (void)hipHostFree(s->ctrl);
@@ -667,10 +661,10 @@ int server_create(svo_octree *const *trees, int ntrees, const svo_server_config 
         if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void **>(&s->complete), sizeof(uint64_t) * ring, pinned);
         if (e == hipSuccess) e = hipMalloc(&s->st, sizeof(ServeState));
         if (e == hipSuccess) e = hipMalloc(&s->dslots, sizeof(ViewDesc) * ring);
-        if (e == hipSuccess) e = hipMalloc(&s->tilesOf, sizeof(uint32_t) * ring);
         if (e == hipSuccess) e = hipMalloc(&s->dbase, sizeof(uint64_t) * (size_t(s->maxReqs) + 1));
-        if (e == hipSuccess) e = hipMalloc(&s->done, sizeof(uint32_t) * ring);
-        if (e == hipSuccess) e = hipMemset(s->done, 0, sizeof(uint32_t) * ring);
+        const size_t doneBytes = sizeof(uint32_t) * kLineWords * (kCursors + 1) * size_t(ring);
+        if (e == hipSuccess) e = hipMalloc(&s->done, doneBytes);
+        if (e == hipSuccess) e = hipMemset(s->done, 0, doneBytes);
         if (e == hipSuccess) {
             if (s->hostOutputs) e = hipHostMalloc(reinterpret_cast<void **>(&s->arena), s->budget, pinned);
             else e = hipMalloc(&s->arena, s->budget);

@@ -24,8 +24,12 @@ int fail(int code, const std::string &msg) {
 
 void clear_error() { g_error.clear(); }
 
+// Trees from files and caller arrays are walked on the host (octree_validate)
+// before anything traverses them. The GPU builder sets the depth of a tree
+// with leaves itself, after its scatter checked every node's slot and child
+// block against the array; its other trees are walked like the rest.
 static int finish_handle(std::unique_ptr<svo_octree> &h, svo_octree **out) {
-    int depth = octree_validate(h->words.data(), h->words.size());
+    int depth = h->depth > 0 ? h->depth : octree_validate(h->words.data(), h->words.size());
     if (depth < 0) {
         release_device(*h);
         if (depth == -2) return fail(SVO_ERR_NOMEM, "out of host memory (octree check)");
@@ -36,20 +40,6 @@ static int finish_handle(std::unique_ptr<svo_octree> &h, svo_octree **out) {
     h->depth = depth;
     *out = h.release();
     return SVO_OK;
-}
-
-static bool read_file(const char *path, std::vector<uint8_t> &buf) {
-    FILE *fp = std::fopen(path, "rb");
-    if (!fp) return false;
-    bool ok = std::fseek(fp, 0, SEEK_END) == 0;
-    long n = ok ? std::ftell(fp) : -1;
-    ok = ok && n >= 0 && std::fseek(fp, 0, SEEK_SET) == 0;
-    if (ok) {
-        buf.resize(size_t(n));
-        ok = n == 0 || std::fread(buf.data(), 1, size_t(n), fp) == size_t(n);
-    }
-    std::fclose(fp);
-    return ok;
 }
 
 }  // namespace svo
@@ -152,9 +142,10 @@ int svo_load(const char *path, svo_octree **out) {
         clear_error();
         if (!path || !out) return fail(SVO_ERR_INVALID, "null argument");
         *out = nullptr;
-        std::vector<uint8_t> buf;
-        if (!read_file(path, buf)) return fail(SVO_ERR_IO, std::string("cannot read ") + path);
-        return svo_load_memory(buf.data(), buf.size(), out);
+        FileView file;
+        if (!file.open(path)) return fail(SVO_ERR_IO, std::string("cannot read ") + path);
+        static const uint8_t kNone = 0;  // an empty file decodes (and fails) like any short stream
+        return svo_load_memory(file.size() ? file.data() : &kNone, file.size(), out);
     });
 }
 

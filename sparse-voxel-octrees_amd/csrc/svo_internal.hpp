@@ -22,6 +22,25 @@ void *map_large(size_t bytes);
 void unmap_large(void *p, size_t bytes);
 constexpr size_t kLargeArray = size_t(4) << 20;
 
+// A whole file mapped read-only (pages read in by the mapping call): parsing
+// and uploads read the page cache directly instead of a copy of the file.
+class FileView {
+public:
+    FileView() = default;
+    FileView(const FileView &) = delete;
+    FileView &operator=(const FileView &) = delete;
+    ~FileView() { close(); }
+    bool open(const char *path);  // false if the file cannot be opened or mapped
+    void close();
+    const uint8_t *data() const { return data_; }
+    size_t size() const { return size_; }
+    uint8_t operator[](size_t i) const { return data_[i]; }
+
+private:
+    const uint8_t *data_ = nullptr;
+    size_t size_ = 0;
+};
+
 // Allocator that leaves default-constructed elements uninitialised, so that
 // sizing a word array before it is overwritten does not first write zeros over
 // it, and that maps large arrays with map_large.
@@ -205,7 +224,7 @@ struct PlyElem {
     std::vector<PlyProp> props;
 };
 struct PlyLayout {
-    std::vector<uint8_t> buf;  // the whole file
+    FileView buf;              // the whole file
     std::vector<PlyElem> elems;
     size_t body = 0;           // first byte after end_header
     bool binary = false;       // binary_little_endian (else ascii)

@@ -87,7 +87,8 @@ AUTOTUNE_SHAPES = ["grid8x8", "grid16x4", "grid4x16", "grid32x2"] + [f"{w}x{r}" 
 
 class BuildStats(ctypes.Structure):
     _fields_ = [("triangles", ctypes.c_uint64), ("candidates", ctypes.c_uint64), ("leaves", ctypes.c_uint64),
-                ("words", ctypes.c_uint64), ("device_ms", ctypes.c_double), ("reserved", ctypes.c_uint64 * 3)]
+                ("words", ctypes.c_uint64), ("device_ms", ctypes.c_double), ("peak_bytes", ctypes.c_uint64),
+                ("reserved", ctypes.c_uint64 * 2)]
 
 
 class SdfPrimitive(ctypes.Structure):
@@ -508,7 +509,7 @@ class VoxelOctree:
         bytes 36*triangles + 24*candidates + 4*words."""
         st = BuildStats()
         _check(lib.svo_build_stats_get(self._h, ctypes.byref(st)))
-        d = {k: getattr(st, k) for k in ("triangles", "candidates", "leaves", "words", "device_ms")}
+        d = {k: getattr(st, k) for k in ("triangles", "candidates", "leaves", "words", "device_ms", "peak_bytes")}
         d["algorithmic_bytes"] = 36 * st.triangles + 24 * st.candidates + 4 * st.words
         return d
 

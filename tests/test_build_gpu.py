@@ -26,6 +26,13 @@ def _mesh_cases():
         return [(b["mesh"], b["resolution"], b["threads"], b["mem"], b["oct"]) for b in json.load(fp)["mesh_builds"]]
 
 
+def _depth_as_walked(svo, t):
+    """A built tree with leaves takes its depth from the builder (no host walk
+    of the words); it must equal what the host walk (octree_validate, run for
+    trees made from words) reports for the same words."""
+    assert t.depth() == svo.VoxelOctree.from_words(t.words(), t.center()).depth()
+
+
 @pytest.mark.parametrize("mesh,res,threads,mem,octfile", _mesh_cases())
 def test_mesh_build_matches_reference(svo, golden, mesh, res, threads, mem, octfile):
     print(f"[gpu] build {mesh} R={res} threads={threads} mem={mem}", flush=True)
@@ -38,6 +45,7 @@ def test_mesh_build_matches_reference(svo, golden, mesh, res, threads, mem, octf
     assert np.array_equal(w, gw)
     assert np.array_equal(t.center().view(np.uint32), gc.view(np.uint32))
     assert t.encode() == raw
+    _depth_as_walked(svo, t)
 
 
 def test_grid_builds_match_reference(svo, golden):
@@ -47,6 +55,7 @@ def test_grid_builds_match_reference(svo, golden):
         t = svo.VoxelOctree.build_from_grid(vol, memory_budget=b["mem"])
         raw = open(os.path.join(golden, b["oct"]), "rb").read()
         assert t.encode() == raw, b["name"]
+        _depth_as_walked(svo, t)
 
 
 def test_edge_volumes_match_reference(svo, golden):
@@ -55,6 +64,7 @@ def test_edge_volumes_match_reference(svo, golden):
         print(f"[gpu] edge volume {b['name']}", flush=True)
         t = svo.VoxelOctree.build_from_grid(vols[b["name"]], memory_budget=1 << 30)
         assert t.encode() == open(os.path.join(golden, b["oct"]), "rb").read(), b["name"]
+        _depth_as_walked(svo, t)
     empty = svo.VoxelOctree.build_from_grid(vols["empty16"])
     r = empty.render_primary(empty.orbit_camera(20.0, 30.0, 1.0), 64, 48)
     assert not r["hit"].any()
@@ -412,10 +422,10 @@ def test_build_stats_are_consistent(svo, tmp_path):
     words = t.words()
     kinds = orc.octree_stats(words)
     assert st["triangles"] == len(f) and st["words"] == len(words) and st["leaves"] == kinds["leaves"]
-    assert st["candidates"] >= st["leaves"] > 0 and st["device_ms"] > 0
+    assert st["candidates"] >= st["leaves"] > 0 and st["device_ms"] > 0 and st["peak_bytes"] > 0
     assert st["algorithmic_bytes"] == 36 * st["triangles"] + 24 * st["candidates"] + 4 * st["words"]
     g = svo.VoxelOctree.build_from_terrain(64, 5)
     gs = g.build_stats()
     assert gs["triangles"] == 0 and gs["candidates"] == gs["leaves"] == orc.octree_stats(g.words())["leaves"]
     loaded = svo.VoxelOctree.from_words(words, t.center())
-    assert loaded.build_stats()["words"] == 0
+    assert loaded.build_stats()["words"] == 0 and loaded.build_stats()["peak_bytes"] == 0

@@ -533,16 +533,19 @@ def test_frames_in_one_call_equal_single_frames(svo, trees):
 
 def test_cost_ordered_renders_match_plain_kernel(svo, trees):
     """svo_render_primary_planned_device: the first render of a plan (raster
-    order), the next ones (tiles in the order the previous render's costs
-    give) and a render of another camera through the same plan (a stale
-    order) all equal the plain kernel bit for bit; a plan refuses another
-    image size."""
+    order), the next ones (tiles in the order the previous render filed them
+    by cost) and renders of another camera through the same plan (a stale
+    order) all equal the plain kernel bit for bit over seven renders, which
+    take every counter set and id region of the plan through filling, reading
+    and clearing twice; every pixel is written (outputs start from a fill no
+    result has); a plan refuses another image size."""
     import torch
     dev = torch.device("cuda", 0)
     t = trees["dragon"]
     w, h = 333, 197
     plan = svo.svo.TilePlan(w, h)
-    cams = [t.orbit_camera(20.0, 35.0, 1.0), t.orbit_camera(20.0, 35.0, 1.0), t.orbit_camera(-40.0, 250.0, 1.0)]
+    a, b = t.orbit_camera(20.0, 35.0, 1.0), t.orbit_camera(-40.0, 250.0, 1.0)
+    cams = [a, a, b, a, b, b, a]
     for k, cam in enumerate(cams):
         o = {"hit": torch.full((w * h,), 9, dtype=torch.uint8, device=dev),
              "normal": torch.zeros(w * h, dtype=torch.int32, device=dev),
