@@ -35,6 +35,11 @@ def _pkg():
     return _import_package()
 
 
+def progress(msg):
+    """A line on stderr per finished stage (long runs show they are moving)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def profiled_traffic(workload, width, height):
     """HBM bytes per primary launch measured by rocprofv3 (FETCH_SIZE and
     WRITE_SIZE, calibrated on a 1 GiB copy) for this workload, from the
@@ -545,7 +550,9 @@ def run_served(pkg, trees, rates, seconds, seed=2000, full=False):
         arr = np.cumsum(rng.exponential(1.0 / lam, n))
         arr -= arr[0]
         lat_s, mk_s, _ = srv.run_schedule(reqs, arr)
+        progress(f"served {lam:g} req/s: live server, {n} requests")
         lat_p, mk_p = pkg.svo.serve_replay(trees, reqs, arr, streams=1, one_per_launch=True, device_outputs=True)
+        progress(f"served {lam:g} req/s: one per launch")
         out["rates"].append({"req_per_s": lam, "live_server": stats(lat_s, mk_s, reqs),
                              "one_per_launch": stats(lat_p, mk_p, reqs)})
     reqs = make(1000)
@@ -746,18 +753,22 @@ def main():
     mixed = None
     if not args.no_mixed:
         mixed = run_mixed(pkg, torch, stream, max(1, min(args.steps, 5)), 1, args.waves_per_cu, args.refill)
+        progress("mixed 64-octree batch done")
     served = None
     if not args.no_mixed and not args.no_serve:
         served = run_served(pkg, run_mixed.trees, [float(x) for x in args.serve_rates.split(",")], args.serve_seconds,
                             full=args.serve_full)
+        progress("served stream done")
 
     frames = None
     if not args.no_frames:
         frames = run_frames(pkg, torch, stream, tree, cams, W, H, max(1, min(args.steps, 5)), 1, angles=angles,
                             threads=args.cpu_threads, cache_dir=args.cache)
+        progress("viewer frames done")
     terrain = None
     if not args.no_terrain:
         terrain = run_terrain(pkg, torch, stream, max(1, min(args.steps, 5)), 1, args.cpu_threads, args.cache)
+        progress("terrain done")
 
     # batched outputs must equal the plain kernel's: snapshot plain, run batched, compare
     step()

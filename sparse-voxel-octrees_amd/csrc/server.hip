@@ -359,6 +359,7 @@ struct svo_server {
     // request state
     std::mutex mu;
     std::condition_variable cv;
+    std::condition_variable loopCv;  // wakes the request loop when a launch starts
     std::unordered_map<uint64_t, svo::Ticket> tickets;
     std::deque<uint64_t> waiting;        // submitted, not yet admitted
     std::deque<uint64_t> inflight;       // admitted, in sequence order
@@ -422,6 +423,7 @@ bool start_launch(svo_server &s) {
     }
     s.live = true;
     s.launches++;
+    s.loopCv.notify_one();  // the loop watches the launch from now on
     return true;
 }
 
@@ -567,14 +569,16 @@ void loop_main(svo_server *sp) {
             admit(s);
         }
         if (s.error) s.cv.notify_all();
-        // spin briefly without the lock, so submitters and pollers get in
-        lock.unlock();
         if (s.live) {
+            // spin briefly without the lock, so submitters and pollers get in
+            lock.unlock();
             for (int k = 0; k < 64; ++k) __builtin_ia32_pause();
+            lock.lock();
         } else {
-            std::this_thread::sleep_for(std::chrono::microseconds(200));
+            // no launch: sleep until one starts (start_launch signals) or a
+            // waiting request may be admitted
+            s.loopCv.wait_for(lock, std::chrono::microseconds(200));
         }
-        lock.lock();
     }
     if (!s.inflight.empty()) {
         // destroy with requests in flight: let the launch finish them
@@ -594,6 +598,7 @@ void free_server(svo_server *s) {
             std::lock_guard<std::mutex> lock(s->mu);
             s->quit = true;
         }
+        s->loopCv.notify_one();
         s->loop.join();
     }
     if (s->stream) (void)hipStreamSynchronize(s->stream);
