@@ -255,6 +255,14 @@ def run_mixed(pkg, torch, stream, steps, warmup, waves_per_cu, refill):
     # the same launch captured once in a HIP graph and replayed
     graph = ren.capture(bviews, sptr, waves_per_cu, refill, form, tile_w)
     graph_ms = timed(lambda: graph.launch(sptr), steps)
+    # the grid form with the tiles of all 256 views in a batch plan's cost order
+    planned = None
+    if form == pkg.svo.BATCH_GRID:
+        bplan = pkg.svo.TilePlan.for_batch(bviews, tile_w=tile_w)
+        planned_ms = timed(lambda: ren.render_device(bviews, sptr, 0, 0, form, tile_w, plan=bplan), steps)
+        planned = {"grays_s": round(rays / (planned_ms / 1e3) / 1e9, 4), "ms_per_step": round(planned_ms, 3),
+                   "vs_plain": round(plain_ms / planned_ms, 3),
+                   "form": "the autotuned grid launch through a batch tile plan (svo_render_batch_planned_device)"}
     # parity: the graph-replayed batched launch reproduces the plain launches bit for bit
     plain()
     torch.cuda.synchronize()
@@ -268,6 +276,18 @@ def run_mixed(pkg, torch, stream, steps, warmup, waves_per_cu, refill):
     graph.close()
     mism = sum(int((a != sa).sum() + (b.view(torch.int32) != sb.view(torch.int32)).sum() + (c != sc).sum())
                for (a, b, c), (sa, sb, sc) in zip(outs, snap))
+    if planned is not None:
+        for a, b, c in outs:
+            a.fill_(-1)
+            b.fill_(-1.0)
+            c.fill_(-1)
+        ren.render_device(bviews, sptr, 0, 0, form, tile_w, plan=bplan)
+        torch.cuda.synchronize()
+        planned["mismatched_fields_vs_plain"] = sum(
+            int((a != sa).sum() + (b.view(torch.int32) != sb.view(torch.int32)).sum() + (c != sc).sum())
+            for (a, b, c), (sa, sb, sc) in zip(outs, snap))
+        mism += planned["mismatched_fields_vs_plain"]
+        bplan.close()
     run_mixed.trees = trees
     return {"workload": "mixed_64octrees_256views", "rays_per_step": rays, "octree_build_seconds": round(build_s, 2),
             "plain_grays_s": round(rays / (plain_ms / 1e3) / 1e9, 4), "plain_ms_per_step": round(plain_ms, 3),
@@ -276,6 +296,7 @@ def run_mixed(pkg, torch, stream, steps, warmup, waves_per_cu, refill):
             "graph_replay": {"grays_s": round(rays / (graph_ms / 1e3) / 1e9, 4), "ms_per_step": round(graph_ms, 3),
                              "vs_plain": round(plain_ms / graph_ms, 3),
                              "form": "the autotuned batched launch captured in one HIP graph, replayed per step"},
+            "planned": planned,
             "launch": {"form": "grid" if form == pkg.svo.BATCH_GRID else "persistent", "waves_per_cu": waves_per_cu,
                        "refill": refill, "tile": f"{tile_w or 8}x{64 // (tile_w or 8)}", "autotune_ms": tune},
             "mismatched_fields_vs_plain": mism}
@@ -660,8 +681,12 @@ def main():
     bviews = [renderer.view(tree, cams[v], W, H, None, normal[v].data_ptr(), tbuf[v].data_ptr(), leaf[v].data_ptr())
               for v in range(V)]
 
-    def batched_step():
-        renderer.render_device(bviews, sptr, args.waves_per_cu, args.refill)
+    # the grid form through a batch tile plan: each launch starts the tiles the
+    # previous one found slowest (over all views) first
+    bplan = pkg.svo.TilePlan.for_batch(bviews) if not (args.waves_per_cu or args.refill) else None
+
+    def batched_step(ordered=True):
+        renderer.render_device(bviews, sptr, args.waves_per_cu, args.refill, plan=bplan if ordered else None)
 
     def time_graph():
         g = renderer.capture(bviews, sptr, args.waves_per_cu, args.refill)
@@ -677,14 +702,14 @@ def main():
         g.close()
         return a.elapsed_time(b) / args.steps
 
-    def time_batched():
+    def time_batched(ordered=True):
         for _ in range(args.warmup):
-            batched_step()
+            batched_step(ordered)
         torch.cuda.synchronize()
         a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         a.record(stream)
         for _ in range(args.steps):
-            batched_step()
+            batched_step(ordered)
         b.record(stream)
         torch.cuda.synchronize()
         return a.elapsed_time(b) / args.steps
@@ -734,13 +759,19 @@ def main():
     compare = {"order": "raster" if other else "cost-ordered", "value": round(rays / (other_ms / 1e3) / 1e9, 4),
                "ms_per_step": round(other_ms / args.steps, 4), "avg_launch_ms": round(other_launch, 4)}
     batched_ms = time_batched()
+    batched_raster_ms = time_batched(False) if bplan is not None else None
     graph_ms = time_graph()
     batched = {"value": round(n * V / (batched_ms / 1e3) / 1e9, 4), "unit": "Grays/s",
                "ms_per_step": round(batched_ms, 4), "vs_plain": round((total_ms / args.steps) / batched_ms, 3),
-               "form": (f"one launch of every tile of the {V} views (one wave per 8x8 tile)"
-                        if not (args.waves_per_cu or args.refill) else
+               "form": (f"one launch of every tile of the {V} views (one wave per 8x8 tile), tiles in the cost order "
+                        "of the previous launch (svo_render_batch_planned_device)"
+                        if bplan is not None else
                         f"one persistent launch for all {V} views (waves/CU {args.waves_per_cu}, refill {args.refill})"),
-               "graph_replay": {"value": round(n * V / (graph_ms / 1e3) / 1e9, 4), "ms_per_step": round(graph_ms, 4)}}
+               "graph_replay": {"value": round(n * V / (graph_ms / 1e3) / 1e9, 4), "ms_per_step": round(graph_ms, 4),
+                                "order": "the views' tile order (a graph replays one launch's parameters)"}}
+    if batched_raster_ms is not None:
+        batched["other_order"] = {"order": "the views' tile order", "value": round(n * V / (batched_raster_ms / 1e3) / 1e9, 4),
+                                  "ms_per_step": round(batched_raster_ms, 4)}
 
     if args.no_cpu:
         print(json.dumps({"metric": METRIC, "value": round(value, 4), "unit": "Grays/s", "n_gpus": 1,

@@ -277,6 +277,21 @@ void svo_renderer_free(svo_renderer *r);
 int svo_render_batch_device(svo_renderer *r, const svo_view *views, int nviews, const svo_batch_opts *opts,
                             void *stream);
 
+/* The grid form of svo_render_batch_device with every tile of the batch in a
+ * tile plan's cost order (no counterpart in the reference): the first launch
+ * through a plan runs in the batch's tile order, each later one starts the
+ * tiles the previous launch found slowest (over all views) first, so the slow
+ * tiles of the last views no longer form the launch's tail. Same rays and
+ * results. The plan is made for the batch's tile count
+ * (svo_tile_plan_create_batch with the same views' sizes and tile shape;
+ * SVO_ERR_INVALID otherwise, and for the persistent form); renders through
+ * one plan must be ordered (one stream). Not for graph capture: a launch
+ * reads the plan's order from where its predecessor filed it, chosen on the
+ * host per call. */
+int svo_tile_plan_create_batch(const svo_view *views, int nviews, const svo_batch_opts *opts, svo_tile_plan **out);
+int svo_render_batch_planned_device(svo_renderer *r, const svo_view *views, int nviews, const svo_batch_opts *opts,
+                                    svo_tile_plan *plan, void *stream);
+
 /* svo_render_batch_device that allocates nothing. No counterpart in the
  * reference: it is for callers that cannot allocate device memory (code under
  * HIP graph capture, tight serving loops, harnesses that forbid allocation).

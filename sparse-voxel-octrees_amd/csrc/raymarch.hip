@@ -17,6 +17,7 @@
 #include <mutex>
 
 #include "march.hpp"
+#include "tile_order.hpp"
 #include "svo_internal.hpp"
 
 namespace svo {
@@ -119,69 +120,8 @@ __global__ void __launch_bounds__(64) primary_kernel(const uint32_t *__restrict_
     store_result<Idx>(code, normal, t, leaf, int64_t(y) * p.width + x, hit, nrm, tt, lf);
 }
 
-// Cost-ordered form of primary_kernel. A tile's cost is the largest loop-pass
-// count of its 64 rays; tiles are grouped in 16 buckets by log2(cost). A view
-// rendered again starts its slowest tiles (rays grazing the surface, up to
-// ~500 dependent passes against a median ~40) first, so they run beside the
-// bulk of the tiles instead of in the launch's tail on an otherwise idle GPU.
-// The order only decides when a tile runs: every ray and its result are those
-// of primary_kernel.
-//
-// The order is kept by the launches themselves, with no pass between them.
-// Workgroup b files its tile for the next launch in replica b % 64, under its
-// cost bucket (one atomic on that replica's bucket counter, one store into the
-// replica's id region); workgroup b of the next launch takes entry b / 64 of
-// replica b % 64's tiles in descending bucket order (its 16 counts give the
-// bucket). Replica r holds exactly the tiles of the workgroups b with
-// b % 64 == r, the count the next launch takes from it, so every tile runs
-// once; dispatch in workgroup order starts the costliest tiles of all replicas
-// first. A single set of 16 counters would take one device-scope atomic per
-// tile on one cache line, which serialises (measured: a 2.07 M-ray view at
-// 0.73 ms against 0.33); 64 replicas on lines of their own take ~500 each.
-// `plan` holds the counter sets and id regions; `src` < 0 is raster order (no
-// filing yet).
-constexpr int kCostBuckets = 16;
-constexpr int kReplicas = 64;
-constexpr int kReplicaStride = 32;  // counters per replica: 16 used, 128 B apart
-constexpr int kCountSets = 3;       // a set is read by launch n, cleared by launch n+1, filled by launch n+2
-constexpr int kIdSets = 2;          // a region is read by launch n and filled by launch n+1
-
-struct TileOrder {
-    uint32_t *counts;  // [kCountSets][kReplicas][kReplicaStride]
-    uint32_t *ids;     // [kIdSets][kReplicas][kCostBuckets][cap]
-    int tiles, cap;    // cap = ceil(tiles / 64): the most tiles a replica holds
-    int src, dst, clr;  // counter sets read, filled, cleared by this launch
-    int idSrc, idDst;   // id regions read, filled
-};
-
-__device__ __forceinline__ int cost_bucket(uint32_t c) { return min(kCostBuckets - 1, 31 - __builtin_clz(c | 1u)); }
-
-__device__ __forceinline__ size_t replica_counts(int set, int rep) {
-    return (size_t(set) * kReplicas + rep) * kReplicaStride;
-}
-__device__ __forceinline__ size_t replica_ids(const TileOrder &o, int set, int rep, int bucket) {
-    return ((size_t(set) * kReplicas + rep) * kCostBuckets + bucket) * size_t(o.cap);
-}
-
-// The tile of workgroup b: lane k < 16 holds bucket 15 - k's count of replica
-// b % 64.
-__device__ __forceinline__ int ordered_tile(const TileOrder &o, int b, int lane) {
-    if (o.src < 0) return b;
-    const int rep = b % kReplicas, k = b / kReplicas;
-    const uint32_t c = lane < kCostBuckets ? o.counts[replica_counts(o.src, rep) + (kCostBuckets - 1 - lane)] : 0u;
-    uint32_t incl = c;  // inclusive prefix over lanes 0..lane
-#pragma unroll
-    for (int d = 1; d < kCostBuckets; d <<= 1) {
-        const uint32_t v = uint32_t(__shfl_up(int(incl), d));
-        if (lane >= d) incl += v;
-    }
-    const uint64_t past = __ballot(lane < kCostBuckets && uint32_t(k) < incl);
-    if (!past) return b;  // counts that do not cover the launch (not reached through the plan's checks)
-    const int j = __ffsll(static_cast<unsigned long long>(past)) - 1;  // the first bucket reaching past k
-    const uint32_t before = uint32_t(__shfl(int(incl - c), j));
-    return int(o.ids[replica_ids(o, o.idSrc, rep, kCostBuckets - 1 - j) + (uint32_t(k) - before)]);
-}
-
+// Cost-ordered form of primary_kernel (tile_order.hpp): workgroup b traces the
+// tile the plan's order gives and files it by cost for the next launch.
 template <typename Idx, int Reg>
 __global__ void __launch_bounds__(64) primary_ordered_kernel(const uint32_t *__restrict__ tree, PrimaryParams p,
                                                              int slots, TileOrder o, uint8_t *hit, uint32_t *nrm,
@@ -189,7 +129,6 @@ __global__ void __launch_bounds__(64) primary_ordered_kernel(const uint32_t *__r
     extern __shared__ __align__(16) unsigned char lds[];
     const int lane = threadIdx.x;
     const int tile = ordered_tile(o, int(blockIdx.x), lane);
-    if (blockIdx.x < kReplicas && lane < kCostBuckets) o.counts[replica_counts(o.clr, blockIdx.x) + lane] = 0u;
     const int x = (tile % p.tilesX) * 8 + (lane & 7);
     const int y = (tile / p.tilesX) * 8 + (lane >> 3);
     uint32_t passes = 0;
@@ -209,15 +148,7 @@ __global__ void __launch_bounds__(64) primary_ordered_kernel(const uint32_t *__r
         } while (code == kRunning);
         store_result<Idx>(code, normal, t, leaf, int64_t(y) * p.width + x, hit, nrm, tt, lf);
     }
-#pragma unroll
-    for (int d = 32; d > 0; d >>= 1) passes = max(passes, uint32_t(__shfl_xor(int(passes), d)));
-    if (lane == 0) {
-        const int bucket = cost_bucket(passes);
-        const int rep = int(blockIdx.x) % kReplicas;
-        const uint32_t at = atomicAdd(&o.counts[replica_counts(o.dst, rep) + bucket], 1u);
-        if (at < uint32_t(o.cap))  // a replica files at most cap tiles per launch
-            o.ids[replica_ids(o, o.idDst, rep, bucket) + at] = uint32_t(tile);
-    }
+    file_tile(o, int(blockIdx.x), tile, passes, lane);
 }
 
 // Primary rays and hard shadows of one 8x8 tile in one wave: every lane's
@@ -414,28 +345,13 @@ void launch_primary(const svo_octree &h, const TileLaunch &L, const PrimaryParam
 
 }  // namespace
 
-}  // namespace svo
-
-// Per-tile costs and tile order of one view size, kept on the device between
-// renders (svo_render_primary_planned_device).
-struct svo_tile_plan {
-    int width = 0, height = 0, tiles = 0;
-    int cap = 0;                 // ceil(tiles / 64)
-    int device = -1;
-    uint64_t renders = 0;        // launches made through the plan (0: no filing yet)
-    uint32_t *counts = nullptr;  // 3 sets of 64 replicas of 16 bucket counters
-    uint32_t *ids = nullptr;     // 2 regions of 64 replicas x 16 buckets x cap tile ids
-};
-
-namespace svo {
-
-int tile_plan_create(int width, int height, svo_tile_plan **out) {
+// Tile plans (tile_order.hpp): the cost order of one image size or one batch,
+// kept on the device between launches.
+int tile_plan_alloc(int64_t tiles, int width, int height, svo_tile_plan **out) {
     if (!out) return fail(SVO_ERR_INVALID, "null argument");
     *out = nullptr;
-    if (width <= 0 || height <= 0 || width > 65536 || height > 65536)
-        return fail(SVO_ERR_INVALID, "image size out of range");
-    const int64_t tiles = int64_t((width + 7) / 8) * ((height + 7) / 8);
-    if (tiles > 0x7FFFFFFF) return fail(SVO_ERR_LIMIT, "too many tiles");
+    if (tiles <= 0) return fail(SVO_ERR_INVALID, "a tile plan needs at least one tile");
+    if (tiles > 0x7FFFFFFF) return fail(SVO_ERR_LIMIT, "too many tiles for one launch");
     svo_tile_plan *p = new (std::nothrow) svo_tile_plan();
     if (!p) return fail(SVO_ERR_NOMEM, "out of host memory");
     p->width = width;
@@ -447,13 +363,21 @@ int tile_plan_create(int width, int height, svo_tile_plan **out) {
     if (e == hipSuccess) e = hipMalloc(&p->counts, countBytes);
     if (e == hipSuccess) e = hipMalloc(&p->ids, sizeof(uint32_t) * kIdSets * kReplicas * kCostBuckets * size_t(p->cap));
     if (e == hipSuccess) e = hipMemset(p->counts, 0, countBytes);
-    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);  // cleared before any stream's launch uses it
     if (e != hipSuccess) {
         tile_plan_free(p);
         return fail(SVO_ERR_NOMEM, std::string("tile plan: ") + hipGetErrorString(e));
     }
     *out = p;
     return SVO_OK;
+}
+
+int tile_plan_create(int width, int height, svo_tile_plan **out) {
+    if (!out) return fail(SVO_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (width <= 0 || height <= 0 || width > 65536 || height > 65536)
+        return fail(SVO_ERR_INVALID, "image size out of range");
+    return tile_plan_alloc(int64_t((width + 7) / 8) * ((height + 7) / 8), width, height, out);
 }
 
 void tile_plan_free(svo_tile_plan *p) {
@@ -468,19 +392,7 @@ namespace {
 template <typename Idx, int Reg>
 void launch_ordered(const svo_octree &h, const TileLaunch &L, const PrimaryParams &p, const svo_tile_plan &plan,
                     uint8_t *hit, uint32_t *normal, float *t, uint64_t *leaf, hipStream_t s) {
-    // launch n reads counter set n % 3 and id region n % 2 (filled by launch
-    // n-1), fills set (n+1) % 3 and region (n+1) % 2, clears set (n+2) % 3
-    const uint64_t n = plan.renders;
-    TileOrder o;
-    o.counts = plan.counts;
-    o.ids = plan.ids;
-    o.tiles = plan.tiles;
-    o.cap = plan.cap;
-    o.src = n ? int(n % kCountSets) : -1;
-    o.dst = int((n + 1) % kCountSets);
-    o.clr = int((n + 2) % kCountSets);
-    o.idSrc = int(n % kIdSets);
-    o.idDst = int((n + 1) % kIdSets);
+    const TileOrder o = next_order(plan);
     hipLaunchKernelGGL((primary_ordered_kernel<Idx, Reg>), dim3(unsigned(plan.tiles)), dim3(64), L.lds, s, h.dwords, p,
                        L.slots, o, hit, normal, t, leaf);
 }

@@ -33,6 +33,7 @@
 
 #include "march.hpp"
 #include "svo_internal.hpp"
+#include "tile_order.hpp"
 #include "view_desc.hpp"
 
 namespace svo {
@@ -183,13 +184,16 @@ __global__ void __launch_bounds__(256) batch_kernel(const ViewDesc *__restrict__
 // launches. A tile is 2^twLog pixels wide and 64 / 2^twLog high (8x8 by
 // default; 16x4, 4x16 and 32x2 are the autotuner's other shapes); the views'
 // tilesX / tilesY are counted in that shape.
-template <bool Lod, typename Idx, int Reg>
+//
+// Planned: the batch's tiles in a tile plan's cost order (tile_order.hpp),
+// each filed by cost for the plan's next launch; one launch (first = 0).
+template <bool Lod, typename Idx, int Reg, bool Planned = false>
 __global__ void __launch_bounds__(64) tiles_kernel(const ViewDesc *__restrict__ views,
                                                    const uint64_t *__restrict__ tileBase, int nviews, uint64_t first,
-                                                   int slots, int twLog) {
+                                                   int slots, int twLog, TileOrder o) {
     extern __shared__ __align__(16) unsigned char lds[];
     const int lane = threadIdx.x;
-    const uint64_t tile = first + blockIdx.x;
+    const uint64_t tile = Planned ? uint64_t(ordered_tile(o, int(blockIdx.x), lane)) : first + blockIdx.x;
     const int v = view_of(tileBase, nviews, tile);
     const ViewDesc &vd = views[v];
     const uint64_t local = tile - tileBase[v];
@@ -197,20 +201,26 @@ __global__ void __launch_bounds__(64) tiles_kernel(const ViewDesc *__restrict__ 
     const int ty = int(local / uint64_t(vd.tilesX));
     const int x = (tx << twLog) + (lane & ((1 << twLog) - 1));
     const int y = (ty << (6 - twLog)) + (lane >> twLog);
-    if (tx >= vd.tilesX || ty >= vd.tilesY || x >= vd.width || y >= vd.height) return;
-    float dx, dy, dz;
-    view_ray_dir(vd, x, y, dx, dy, dz);
-    Stack<Idx> stk(lds, 64, slots, lane);
-    Ray<Idx> r;
-    ray_init(r, vd.tree, vd.px, vd.py, vd.pz, dx, dy, dz, vd.rayScale);
-    uint32_t normal = 0;
-    float t = 0.0f;
-    Idx leaf = 0;
-    int code;
-    do {
-        code = ray_step<Lod, true, Idx, true, Reg>(r, vd.tree, stk, normal, t, leaf);
-    } while (code == kRunning);
-    view_store<Idx>(vd, int64_t(y) * vd.width + x, code, normal, t, leaf);
+    const bool inside = !(tx >= vd.tilesX || ty >= vd.tilesY || x >= vd.width || y >= vd.height);
+    if (!Planned && !inside) return;
+    uint32_t passes = 0;
+    if (inside) {
+        float dx, dy, dz;
+        view_ray_dir(vd, x, y, dx, dy, dz);
+        Stack<Idx> stk(lds, 64, slots, lane);
+        Ray<Idx> r;
+        ray_init(r, vd.tree, vd.px, vd.py, vd.pz, dx, dy, dz, vd.rayScale);
+        uint32_t normal = 0;
+        float t = 0.0f;
+        Idx leaf = 0;
+        int code;
+        do {
+            code = ray_step<Lod, true, Idx, true, Reg>(r, vd.tree, stk, normal, t, leaf);
+            if (Planned) passes++;
+        } while (code == kRunning);
+        view_store<Idx>(vd, int64_t(y) * vd.width + x, code, normal, t, leaf);
+    }
+    if (Planned) file_tile(o, int(blockIdx.x), int(tile), passes, lane);
 }
 
 __global__ void zero_counters_kernel(uint32_t *c) {
@@ -416,21 +426,41 @@ void launch_batch(const LaunchPlan &p, const ViewDesc *dViews, const uint64_t *d
 template <int Reg>
 void launch_tiles(const LaunchPlan &p, const ViewDesc *dViews, const uint64_t *dTileBase, hipStream_t s) {
     const uint64_t maxGrid = 0x7FFFFFFFull;
+    const TileOrder none{};
     for (uint64_t first = 0; first < p.tiles; first += maxGrid) {
         const dim3 grid(unsigned(std::min(maxGrid, p.tiles - first)));
         if (!p.wide && !p.lod)
             hipLaunchKernelGGL((tiles_kernel<false, uint32_t, Reg>), grid, dim3(64), p.ldsPerWave, s, dViews,
-                               dTileBase, p.nviews, first, p.slots, p.twLog);
+                               dTileBase, p.nviews, first, p.slots, p.twLog, none);
         else if (!p.wide)
             hipLaunchKernelGGL((tiles_kernel<true, uint32_t, Reg>), grid, dim3(64), p.ldsPerWave, s, dViews,
-                               dTileBase, p.nviews, first, p.slots, p.twLog);
+                               dTileBase, p.nviews, first, p.slots, p.twLog, none);
         else if (!p.lod)
             hipLaunchKernelGGL((tiles_kernel<false, uint64_t, Reg>), grid, dim3(64), p.ldsPerWave, s, dViews,
-                               dTileBase, p.nviews, first, p.slots, p.twLog);
+                               dTileBase, p.nviews, first, p.slots, p.twLog, none);
         else
             hipLaunchKernelGGL((tiles_kernel<true, uint64_t, Reg>), grid, dim3(64), p.ldsPerWave, s, dViews,
-                               dTileBase, p.nviews, first, p.slots, p.twLog);
+                               dTileBase, p.nviews, first, p.slots, p.twLog, none);
     }
+}
+
+// Grid form in a tile plan's order: one launch (the plan holds at most 2^31 - 1 tiles).
+template <int Reg>
+void launch_tiles_planned(const LaunchPlan &p, const ViewDesc *dViews, const uint64_t *dTileBase, const TileOrder &o,
+                          hipStream_t s) {
+    const dim3 grid(unsigned(p.tiles));
+    if (!p.wide && !p.lod)
+        hipLaunchKernelGGL((tiles_kernel<false, uint32_t, Reg, true>), grid, dim3(64), p.ldsPerWave, s, dViews,
+                           dTileBase, p.nviews, uint64_t(0), p.slots, p.twLog, o);
+    else if (!p.wide)
+        hipLaunchKernelGGL((tiles_kernel<true, uint32_t, Reg, true>), grid, dim3(64), p.ldsPerWave, s, dViews,
+                           dTileBase, p.nviews, uint64_t(0), p.slots, p.twLog, o);
+    else if (!p.lod)
+        hipLaunchKernelGGL((tiles_kernel<false, uint64_t, Reg, true>), grid, dim3(64), p.ldsPerWave, s, dViews,
+                           dTileBase, p.nviews, uint64_t(0), p.slots, p.twLog, o);
+    else
+        hipLaunchKernelGGL((tiles_kernel<true, uint64_t, Reg, true>), grid, dim3(64), p.ldsPerWave, s, dViews,
+                           dTileBase, p.nviews, uint64_t(0), p.slots, p.twLog, o);
 }
 
 }  // namespace
@@ -467,6 +497,53 @@ int render_batch(svo_renderer *rd, const svo_view *views, int nviews, const svo_
     launch_plan(plan, rd->dViews, rd->dTileBase, rd->dCounters, s);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(rd->done, s));
+    return SVO_OK;
+}
+
+// The grid form's tile count of a batch (the views' sizes and the tile shape).
+int tile_plan_create_batch(const svo_view *views, int nviews, const svo_batch_opts *opts, svo_tile_plan **out) {
+    if (!out) return fail(SVO_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (!views || nviews <= 0) return fail(SVO_ERR_INVALID, "no views");
+    if (opts && opts->form == SVO_BATCH_PERSISTENT) return fail(SVO_ERR_INVALID, "tile plans order the grid form");
+    const int twLog = tile_width_log(opts ? opts->tile_w : 0);
+    if (twLog < 0) return fail(SVO_ERR_INVALID, "tile_w must be 4, 8, 16 or 32");
+    const int tw = 1 << twLog, th = 64 >> twLog;
+    int64_t tiles = 0;
+    for (int i = 0; i < nviews; ++i) {
+        if (views[i].width <= 0 || views[i].height <= 0 || views[i].width > 65536 || views[i].height > 65536)
+            return fail(SVO_ERR_INVALID, "image size out of range");
+        tiles += int64_t((views[i].width + tw - 1) / tw) * int64_t((views[i].height + th - 1) / th);
+    }
+    return tile_plan_alloc(tiles, 0, 0, out);
+}
+
+// render_batch in the grid form with the tiles in a tile plan's order.
+int render_batch_planned(svo_renderer *rd, const svo_view *views, int nviews, const svo_batch_opts *opts,
+                         svo_tile_plan *tp, void *stream) {
+    if (!rd || !views || !tp) return fail(SVO_ERR_INVALID, "null argument");
+    if (nviews <= 0 || nviews > rd->maxViews) return fail(SVO_ERR_LIMIT, "view count exceeds the renderer's max_views");
+    if (opts && opts->form == SVO_BATCH_PERSISTENT) return fail(SVO_ERR_INVALID, "tile plans order the grid form");
+    int dev = -1;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev != tp->device) return fail(SVO_ERR_INVALID, "the tile plan lives on another device");
+    std::lock_guard<std::mutex> lock(rd->mu);
+    HIP_TRY(hipEventSynchronize(rd->staged));
+    LaunchPlan plan;
+    int r = plan_views(views, nviews, opts, rd->cus, rd->hViews, rd->hTileBase, plan, true);
+    if (r) return r;
+    if (plan.tiles != uint64_t(tp->tiles)) return fail(SVO_ERR_INVALID, "the tile plan was made for another batch");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    HIP_TRY(hipStreamWaitEvent(s, rd->done, 0));
+    HIP_TRY(hipMemcpyAsync(rd->dViews, rd->hViews, sizeof(ViewDesc) * nviews, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(rd->dTileBase, rd->hTileBase, sizeof(uint64_t) * (nviews + 1), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipEventRecord(rd->staged, s));
+    const TileOrder o = next_order(*tp);
+    if (plan.reg == 3) launch_tiles_planned<3>(plan, rd->dViews, rd->dTileBase, o, s);
+    else launch_tiles_planned<1>(plan, rd->dViews, rd->dTileBase, o, s);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(rd->done, s));
+    tp->renders++;
     return SVO_OK;
 }
 

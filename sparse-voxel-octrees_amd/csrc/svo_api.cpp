@@ -376,6 +376,13 @@ int svo_tile_plan_create(int width, int height, svo_tile_plan **out) {
     });
 }
 
+int svo_tile_plan_create_batch(const svo_view *views, int nviews, const svo_batch_opts *opts, svo_tile_plan **out) {
+    return guarded([&]() -> int {
+        clear_error();
+        return tile_plan_create_batch(views, nviews, opts, out);
+    });
+}
+
 void svo_tile_plan_free(svo_tile_plan *p) { tile_plan_free(p); }
 
 int svo_render_primary_planned_device(svo_octree *h, svo_tile_plan *plan, const svo_camera *cam, int width,
@@ -512,6 +519,14 @@ int svo_render_batch_device(svo_renderer *r, const svo_view *views, int nviews, 
     return guarded([&]() -> int {
         clear_error();
         return render_batch(r, views, nviews, opts, stream);
+    });
+}
+
+int svo_render_batch_planned_device(svo_renderer *r, const svo_view *views, int nviews, const svo_batch_opts *opts,
+                                    svo_tile_plan *plan, void *stream) {
+    return guarded([&]() -> int {
+        clear_error();
+        return render_batch_planned(r, views, nviews, opts, plan, stream);
     });
 }
 

@@ -143,6 +143,7 @@ int raymarch_device(svo_octree &h, int64_t n, const float *o, const float *d, co
                     float rs0, uint8_t *hit, uint32_t *normal, float *t, uint64_t *leaf, void *stream);
 int render_primary_device(svo_octree &h, const svo_camera &cam, int width, int height, uint8_t *hit,
                           uint32_t *normal, float *t, uint64_t *leaf, void *stream);
+int tile_plan_alloc(int64_t tiles, int width, int height, svo_tile_plan **out);
 int tile_plan_create(int width, int height, svo_tile_plan **out);
 void tile_plan_free(svo_tile_plan *p);
 int render_primary_planned_device(svo_octree &h, svo_tile_plan *plan, const svo_camera &cam, int width, int height,
@@ -165,6 +166,9 @@ int64_t frame_depth_count(int width, int height, int stripes);
 int renderer_create(int maxViews, svo_renderer **out);
 void renderer_free(svo_renderer *r);
 int render_batch(svo_renderer *r, const svo_view *views, int nviews, const svo_batch_opts *opts, void *stream);
+int tile_plan_create_batch(const svo_view *views, int nviews, const svo_batch_opts *opts, svo_tile_plan **out);
+int render_batch_planned(svo_renderer *r, const svo_view *views, int nviews, const svo_batch_opts *opts,
+                         svo_tile_plan *plan, void *stream);
 int render_batch_noalloc(const svo_view *views, int nviews, const svo_batch_opts *opts, void *stream);
 int renderer_autotune(svo_renderer *r, const svo_view *views, int nviews, void *stream, svo_batch_opts *best,
                       float *times);

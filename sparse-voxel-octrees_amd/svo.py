@@ -38,7 +38,8 @@ EXPORTS = [
     "svo_server_create", "svo_server_submit", "svo_server_poll", "svo_server_wait", "svo_server_outputs",
     "svo_server_copy_outputs", "svo_server_release", "svo_server_stats_get", "svo_server_run_schedule",
     "svo_server_destroy", "svo_tile_plan_create", "svo_tile_plan_free", "svo_render_primary_planned_device",
-    "svo_render_batch_noalloc", "svo_load_device_code",
+    "svo_render_batch_noalloc", "svo_load_device_code", "svo_tile_plan_create_batch",
+    "svo_render_batch_planned_device",
     "svo_build_stats_get",
 ]
 
@@ -331,6 +332,8 @@ def _load():
         "svo_tile_plan_create": [ctypes.c_int, ctypes.c_int, P], "svo_tile_plan_free": [P],
         "svo_render_primary_planned_device": [P, P, P, ctypes.c_int, ctypes.c_int, P, P, P, P, P],
         "svo_render_batch_noalloc": [P, ctypes.c_int, P, P],
+        "svo_tile_plan_create_batch": [P, ctypes.c_int, P, P],
+        "svo_render_batch_planned_device": [P, P, ctypes.c_int, P, P, P],
         "svo_load_device_code": [],
     }
     for name, args in sig.items():
@@ -662,13 +665,24 @@ def render_batch_noalloc(views, stream=None, waves_per_cu=0, refill_threshold=0,
 
 
 class TilePlan:
-    """Per-tile costs and tile order of one image size on the device
+    """The tile cost order of one image size, or of one batch, on the device
     (svo_tile_plan): renders through it start the slowest tiles first."""
 
-    def __init__(self, width, height):
+    def __init__(self, width, height, _batch=None):
         self.width, self.height = int(width), int(height)
         self._p = ctypes.c_void_p()
-        _check(lib.svo_tile_plan_create(self.width, self.height, ctypes.byref(self._p)))
+        if _batch is None:
+            _check(lib.svo_tile_plan_create(self.width, self.height, ctypes.byref(self._p)))
+        else:
+            views, opts = _batch
+            arr = (View * len(views))(*views)
+            _check(lib.svo_tile_plan_create_batch(arr, len(views), ctypes.byref(opts), ctypes.byref(self._p)))
+
+    @classmethod
+    def for_batch(cls, views, tile_w=0):
+        """A plan for Renderer.render_device(..., plan=...) over views of these
+        sizes in the grid form with this tile width."""
+        return cls(0, 0, _batch=(views, _batch_opts(0, 0, None, tile_w)))
 
     def render_device(self, tree, cam, hit_ptr=None, normal_ptr=None, t_ptr=None, leaf_ptr=None, stream=None):
         _check(lib.svo_render_primary_planned_device(tree.handle, self._p, ctypes.byref(cam), self.width, self.height,
@@ -714,10 +728,15 @@ class Renderer:
         v.hit, v.normal, v.t, v.leaf = hit, normal, t, leaf
         return v
 
-    def render_device(self, views, stream=None, waves_per_cu=0, refill_threshold=0, form=None, tile_w=0):
+    def render_device(self, views, stream=None, waves_per_cu=0, refill_threshold=0, form=None, tile_w=0, plan=None):
+        """One batched launch; with `plan` (TilePlan.for_batch) the grid form's
+        tiles run in the plan's cost order."""
         arr = (View * len(views))(*views)
         o = _batch_opts(waves_per_cu, refill_threshold, form, tile_w)
-        _check(lib.svo_render_batch_device(self._r, arr, len(views), ctypes.byref(o), stream))
+        if plan is None:
+            _check(lib.svo_render_batch_device(self._r, arr, len(views), ctypes.byref(o), stream))
+        else:
+            _check(lib.svo_render_batch_planned_device(self._r, arr, len(views), ctypes.byref(o), plan._p, stream))
 
     def autotune(self, views, stream=None):
         """((form, waves_per_cu, refill, tile_w), {"grid8x8": ms, ..., "WxR": ms})

@@ -566,3 +566,58 @@ def test_cost_ordered_renders_match_plain_kernel(svo, trees):
     assert e.value.code == svo.svo.ERR_INVALID
     plan.close()
     other.close()
+
+
+def test_cost_ordered_batches_match_unordered(svo, trees):
+    """svo_render_batch_planned_device: five launches of one batch (views of
+    two octrees, edge tiles, a view below one tile, a LOD view) through one
+    batch plan, in the 8x8 and 16x4 tile shapes, equal the unordered grid
+    launch on every ray (outputs start from a fill no result has); the plan
+    refuses another batch, the persistent form and a plain image plan of
+    another tile count."""
+    import torch
+    dev = torch.device("cuda", 0)
+    specs = [("dragon", 320, 180, 20.0, 35.0, 0.0), ("sdf128", 257, 131, -30.0, 200.0, 0.0),
+             ("dragon", 7, 5, 10.0, 10.0, 0.0), ("dragon", 200, 100, -60.0, 90.0, 0.02)]
+    ren = svo.svo.Renderer(8)
+
+    def batch():
+        outs, views = [], []
+        for name, w, h, p, y, rs in specs:
+            t = trees[name]
+            o = {"hit": torch.full((w * h,), 9, dtype=torch.uint8, device=dev),
+                 "normal": torch.zeros(w * h, dtype=torch.int32, device=dev),
+                 "t": torch.zeros(w * h, dtype=torch.float32, device=dev),
+                 "leaf": torch.full((w * h,), -2, dtype=torch.int64, device=dev)}
+            outs.append(o)
+            views.append(ren.view(t, t.orbit_camera(p, y, 1.0), w, h, o["hit"].data_ptr(), o["normal"].data_ptr(),
+                                  o["t"].data_ptr(), o["leaf"].data_ptr(), ray_scale=rs))
+        return outs, views
+
+    for tile_w in (8, 16):
+        ref, rviews = batch()
+        ren.render_device(rviews, None, tile_w=tile_w)
+        torch.cuda.synchronize()
+        outs, views = batch()
+        plan = svo.svo.TilePlan.for_batch(views, tile_w=tile_w)
+        for k in range(5):
+            for o in outs:
+                o["hit"].fill_(9)
+                o["leaf"].fill_(-2)
+            print(f"[gpu] planned batch tile_w={tile_w} launch {k}", flush=True)
+            ren.render_device(views, None, tile_w=tile_w, plan=plan)
+            torch.cuda.synchronize()
+            for a, b in zip(outs, ref):
+                assert all(bool(torch.equal(a[f], b[f])) for f in ("hit", "normal", "t", "leaf")), (tile_w, k)
+        with pytest.raises(svo.SvoError) as e:
+            ren.render_device(views[:2], None, tile_w=tile_w, plan=plan)
+        assert e.value.code == svo.svo.ERR_INVALID
+        with pytest.raises(svo.SvoError) as e:
+            ren.render_device(views, None, form=svo.svo.BATCH_PERSISTENT, plan=plan)
+        assert e.value.code == svo.svo.ERR_INVALID
+        plan.close()
+    image_plan = svo.svo.TilePlan(64, 64)
+    with pytest.raises(svo.SvoError) as e:
+        ren.render_device(views, None, plan=image_plan)
+    assert e.value.code == svo.svo.ERR_INVALID
+    image_plan.close()

@@ -72,10 +72,11 @@ def ico2048(svo, tmp_path_factory):
 def test_ico2048_eight_views_match_oracle(svo, ico2048):
     """The headline workload's render half: 8 seeded 1920x1080 orbit views of
     the GPU-built 2048^3 tree, plain kernel vs the oracle on all 16,588,800
-    rays; the batched grid form, a graph replay and the cost-ordered renders
-    of the bench's headline (one tile plan per view, three renders each: raster
-    order, then the tiles filed by the previous render's costs) vs the plain
-    kernel."""
+    rays; the batched grid form, a graph replay, the cost-ordered renders of
+    the bench's headline (one tile plan per view, three renders each: raster
+    order, then the tiles filed by the previous render's costs) and three
+    cost-ordered batched launches (one plan over the 8 views' tiles) vs the
+    plain kernel."""
     import torch
     dev = torch.device("cuda", 0)
     w, h = 1920, 1080
@@ -126,6 +127,17 @@ def test_ico2048_eight_views_match_oracle(svo, ico2048):
         assert all(_equal_on_device(torch, a, b) for a, b in zip(outs, plain))
     for plan in plans:
         plan.close()
+    bplan = svo.svo.TilePlan.for_batch(views)
+    for rep in range(3):
+        for o in outs:
+            o["hit"].fill_(9)
+            o["leaf"].fill_(-2)
+        torch.cuda.synchronize()
+        print(f"[gpu] ico2048 cost-ordered batch, launch {rep}", flush=True)
+        ren.render_device(views, None, 0, 0, svo.svo.BATCH_GRID, plan=bplan)
+        torch.cuda.synchronize()
+        assert all(_equal_on_device(torch, a, b) for a, b in zip(outs, plain))
+    bplan.close()
 
 
 @pytest.fixture(scope="module")
