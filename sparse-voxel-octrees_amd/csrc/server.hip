@@ -242,8 +242,11 @@ __global__ void __launch_bounds__(256) serve_kernel(ServeCtrl *ctrl, const Serve
         unsigned long long g = 0;
         if (lane == 0) g = atomicAdd(&st->cursor[cls].next, 1ull) * kCursors + uint64_t(cls);
         g = __shfl(g, 0);
-        // hold the tile index until it is published (or the launch ends)
-        while (g >= (known & kTileMask)) {
+        // hold the tile index until it is published (or the launch ends);
+        // each poll is a device-scope load of one word by every waiting wave,
+        // so a wave that keeps finding nothing polls less often (up to ~8 K
+        // cycles apart)
+        for (uint32_t naps = 0; g >= (known & kTileMask);) {
             uint64_t p = 0;
             if (lane == 0) p = ld_dev(&st->published);
             known = __shfl(p, 0);
@@ -251,7 +254,8 @@ __global__ void __launch_bounds__(256) serve_kernel(ServeCtrl *ctrl, const Serve
             uint32_t stop = 0;
             if (lane == 0) stop = ld_dev(&st->stop);
             if (__shfl(stop, 0)) return;
-            __builtin_amdgcn_s_sleep(2);
+            for (uint32_t k = 0; k < (1u << naps); ++k) __builtin_amdgcn_s_sleep(2);
+            if (naps < 6) ++naps;
         }
         // the request holding tile g: dbase[q] <= g < dbase[q + 1] (g only grows per wave)
         const uint32_t count = uint32_t(known >> kCountShift);
