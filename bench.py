@@ -125,6 +125,45 @@ def make_workload(pkg, name, cache_dir, no_cpu=False):
     raise SystemExit(f"unknown workload {name}")
 
 
+def run_mesh_big(pkg, cache_dir):
+    """The headline mesh built at 4096^3 and 8192^3 (8 threads' partition,
+    1 GiB budget): end-to-end seconds of a second build in the process, the
+    device phase, the construction roofline and the .oct md5 against the
+    reference builder's (tests/golden/builds.json)."""
+    import hashlib
+    with open(os.path.join(ROOT, "tests", "golden", "builds.json")) as fp:
+        golden = json.load(fp)
+    ply = os.path.join(cache_dir, "ico316_seed2048.ply")
+    if not os.path.exists(ply):
+        os.makedirs(cache_dir, exist_ok=True)
+        v, f = pkg.scenes.icosphere(316, seed=2048, displacement=0.05)
+        pkg.scenes.write_ply(ply, v, f)
+    pkg.svo.load_device_code()
+    out = {}
+    for key in ("ico4096", "ico8192"):
+        g = golden.get(key)
+        if g is None:
+            continue
+        res = g["resolution"]
+        t0 = time.time()
+        tree = pkg.VoxelOctree.build_from_ply(ply, res, threads=8, memory_budget=1 << 30)
+        first_s = time.time() - t0
+        md5 = hashlib.md5(tree.encode()).hexdigest()
+        del tree
+        t0 = time.time()
+        warm = pkg.VoxelOctree.build_from_ply(ply, res, threads=8, memory_budget=1 << 30)
+        warm_s = time.time() - t0
+        st = warm.build_stats()
+        del warm
+        out[key] = {"resolution": res, "build_seconds": round(warm_s, 3), "build_seconds_first_build": round(first_s, 3),
+                    "leaves": st["leaves"], "words": st["words"], "candidates": st["candidates"],
+                    "peak_bytes": st["peak_bytes"], "build_roofline": build_roofline(st),
+                    "leaf_voxels_per_s": round(st["leaves"] / warm_s, 1),
+                    "oct_md5_matches_reference": md5 == g["md5"]}
+        progress(f"mesh build {res}^3 done")
+    return out
+
+
 def host_cpu():
     """(model name, os.cpu_count()) of this host."""
     model = "unknown"
@@ -535,7 +574,7 @@ def reference_frame(tree, angle, width, height, threads, cache_dir):
     return None
 
 
-def run_served(pkg, trees, rates, seconds, seed=2000, full=False):
+def run_served(pkg, trees, rates, seconds, seed=2000, full=False, waves_per_cu=0):
     """Served stream over the 64 resident octrees: seeded Poisson request
     streams, each request (octree U{0..63}, seeded orbit camera, resolution as
     the mixed views). For each mean rate the live server (svo_server_*: one
@@ -561,7 +600,7 @@ def run_served(pkg, trees, rates, seconds, seed=2000, full=False):
                 "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
                 "grays_s": round(rays / mk / 1e9, 4), "requests": len(reqs)}
 
-    srv = pkg.svo.Server(trees, output_budget=4 << 30)
+    srv = pkg.svo.Server(trees, output_budget=4 << 30, waves_per_cu=waves_per_cu)
     # first use of the serving kernel, outside the measurements
     warm = make(8)
     srv.run_schedule(warm, np.zeros(len(warm)))
@@ -587,6 +626,7 @@ def run_served(pkg, trees, rates, seconds, seed=2000, full=False):
     out["burst"] = {"live_server": s_, "one_per_launch": p_, "relaunch_batched_4_streams": r_,
                     "live_vs_one_per_launch": round(s_["grays_s"] / p_["grays_s"], 3)}
     out["server_launches"] = st["launches"]
+    out["server_waves_per_cu"] = waves_per_cu or 24
     return out
 
 
@@ -614,11 +654,13 @@ def main():
     ap.add_argument("--serve-rates", default="100,200,500,1000,2000", help="mean request rates (req/s) to replay")
     ap.add_argument("--serve-seconds", type=float, default=4.0, help="schedule length per rate (at most 5000 requests)")
     ap.add_argument("--serve-full", action="store_true", help="5000 requests at every rate")
+    ap.add_argument("--serve-waves-per-cu", type=int, default=0,
+                    help="live server: wave slots per CU its launch holds (0 = the library's default, 24)")
     ap.add_argument("--no-terrain", action="store_true", help="skip the 8192^3 terrain + shadow-ray workload")
     ap.add_argument("--no-frames", action="store_true", help="skip the viewer-frame workload")
-    ap.add_argument("--only", choices=["terrain", "frames", "build"], default=None,
+    ap.add_argument("--only", choices=["terrain", "frames", "build", "meshbig"], default=None,
                     help="run just that workload and print its JSON (profiling runs); build: the headline "
-                         "octree's two GPU builds from its PLY file")
+                         "octree's two GPU builds from its PLY file; meshbig: the same mesh at 4096^3 and 8192^3")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (no multi-GPU path); run with --gpus 1",
@@ -631,6 +673,9 @@ def main():
     if pkg.svo.device_count() < 1 or not torch.cuda.is_available():
         raise SystemExit("bench.py: no GPU visible")
     torch.cuda.set_device(0)
+    if args.only == "meshbig":
+        print(json.dumps(run_mesh_big(pkg, args.cache)), flush=True)
+        return
     if args.only == "terrain":
         print(json.dumps(run_terrain(pkg, torch, torch.cuda.current_stream(), args.steps, args.warmup,
                                      args.cpu_threads, args.cache)), flush=True)
@@ -788,7 +833,7 @@ def main():
     served = None
     if not args.no_mixed and not args.no_serve:
         served = run_served(pkg, run_mixed.trees, [float(x) for x in args.serve_rates.split(",")], args.serve_seconds,
-                            full=args.serve_full)
+                            full=args.serve_full, waves_per_cu=args.serve_waves_per_cu)
         progress("served stream done")
 
     frames = None

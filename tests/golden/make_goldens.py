@@ -30,6 +30,8 @@ Fixtures written (all small):
   builds.json["ico2048"]  the full-size build: the reference's .oct (md5, size,
                          node counts) for a 1,997,120-triangle displaced
                          icosphere at 2048^3, 8 threads, 1 GiB
+  builds.json["ico4096"], ["ico8192"]
+                         the same mesh built by the reference at 4096^3 and 8192^3
 
   build_terrain_*.oct + builds.json["terrain"]
                          the reference's file-source build of the fBm terrain
@@ -40,7 +42,7 @@ Fixtures written (all small):
                           .voxel files of the mesh fixtures (single and multi-slab)
                          edge volumes through the reference's file-source builder
 
-`--only frames|ico2048|terrain|edges|extremes|convert` regenerates just that part;
+`--only frames|ico2048|big|terrain|edges|extremes|convert` regenerates just that part;
 other entries of builds.json are kept.
 """
 import argparse
@@ -257,14 +259,15 @@ def terrain_builds(tmp):
     return out
 
 
-def full_size_build(tmp):
-    """Reference builder on the full-size mesh (about 10 s of reference build)."""
-    e = {"mesh": "icosphere", "frequency": 316, "seed": 2048, "displacement": 0.05, "resolution": 2048,
+def full_size_build(tmp, resolution=2048):
+    """Reference builder on the full-size mesh (about 10 s of reference build
+    at 2048^3, 25 s at 4096^3, minutes at 8192^3)."""
+    e = {"mesh": "icosphere", "frequency": 316, "seed": 2048, "displacement": 0.05, "resolution": resolution,
          "threads": 8, "mem": 1 << 30}
     v, f = scenes.icosphere(e["frequency"], seed=e["seed"], displacement=e["displacement"])
     ply = os.path.join(tmp, "ico2048.ply")
     scenes.write_ply(ply, v, f)
-    op = os.path.join(tmp, "ico2048.oct")
+    op = os.path.join(tmp, f"ico{resolution}.oct")
     run_ref("build-ply", ply, e["resolution"], e["threads"], e["mem"], op, timeout=900)
     _, words = oracle.read_oct(op)
     st = oracle.octree_stats(words)
@@ -388,7 +391,8 @@ GRID_CASES = [
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--reference", default="/root/reference")
-    ap.add_argument("--only", choices=["frames", "ico2048", "terrain", "edges", "extremes", "convert"], default=None)
+    ap.add_argument("--only", choices=["frames", "ico2048", "big", "terrain", "edges", "extremes", "convert"],
+                    default=None)
     args = ap.parse_args()
     if not os.path.exists(REF_BIN):
         subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "ref", f"REF={args.reference}"], check=True)
@@ -403,11 +407,14 @@ def main():
             return
 
         bpath = os.path.join(HERE, "builds.json")
-        if args.only in ("ico2048", "terrain", "edges", "convert"):
+        if args.only in ("ico2048", "big", "terrain", "edges", "convert"):
             with open(bpath) as fp:
                 builds = json.load(fp)
             if args.only == "ico2048":
                 builds["ico2048"] = full_size_build(tmp)
+            elif args.only == "big":
+                builds["ico4096"] = full_size_build(tmp, 4096)
+                builds["ico8192"] = full_size_build(tmp, 8192)
             elif args.only == "terrain":
                 builds["terrain"] = terrain_builds(tmp)
             elif args.only == "convert":

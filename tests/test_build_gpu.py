@@ -176,6 +176,29 @@ def test_ico2048_full_size_build_matches_reference(svo, golden, tmp_path):
     assert hashlib.md5(data).hexdigest() == b["md5"]
 
 
+@pytest.mark.parametrize("key", ["ico4096", "ico8192"])
+def test_large_mesh_builds_match_reference(svo, golden, tmp_path, key):
+    """The same 1,997,120-triangle mesh built at 4096^3 and 8192^3 (8 threads'
+    partition, 1 GiB budget; at 8192^3 about 16x the 2048^3 build's 39 M
+    (voxel, triangle) candidates, in one global sort): the .oct must hash to
+    the reference builder's output, and the builder's depth must be the
+    host walk's."""
+    import hashlib
+    b = _builds(golden)[key]
+    v, f = svo.scenes.icosphere(b["frequency"], seed=b["seed"], displacement=b["displacement"])
+    ply = str(tmp_path / "ico.ply")
+    svo.scenes.write_ply(ply, v, f)
+    print(f"[gpu] large mesh build {b['resolution']}^3", flush=True)
+    t = svo.VoxelOctree.build_from_ply(ply, b["resolution"], threads=b["threads"], memory_budget=b["mem"])
+    st = t.build_stats()
+    assert st["words"] == b["words"] and st["leaves"] == b["leaves"]
+    data = t.encode()
+    assert len(data) == b["size"]
+    assert hashlib.md5(data).hexdigest() == b["md5"]
+    w = t.words()
+    assert t.depth() == svo.VoxelOctree.from_words(w, t.center()).depth()
+
+
 @pytest.mark.parametrize("n,seed,mem", [(64, 3, 1 << 30), (256, 4, 1 << 30), (256, 6, 1 << 22)])
 def test_sdf_source_matches_oracle_build(svo, n, seed, mem):
     """GPU SDF scene source: its octree equals the oracle's file-source build of
