@@ -663,7 +663,7 @@ def main():
                          "octree's two GPU builds from its PLY file; meshbig: the same mesh at 4096^3 and 8192^3")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU only (no multi-GPU path); run with --gpus 1",
+        print("bench.py: measures one GPU only (no multi-GPU path); run with --gpus 1",
               file=sys.stderr)
         sys.exit(2)
 

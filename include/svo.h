@@ -43,7 +43,7 @@ enum {
     SVO_ERR_FORMAT = 3,      /* corrupt .oct / .voxel / PLY, or malformed octree */
     SVO_ERR_NOMEM = 4,       /* host or device allocation failed */
     SVO_ERR_DEVICE = 5,      /* HIP runtime error (no GPU, launch failure, ...) */
-    SVO_ERR_UNSUPPORTED = 6, /* input outside what this build handles (see message) */
+    SVO_ERR_UNSUPPORTED = 6, /* input outside what the library handles (see message) */
     SVO_ERR_LIMIT = 7        /* request exceeds a stated limit (budget, size) */
 };
 
