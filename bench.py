@@ -578,8 +578,9 @@ def run_served(pkg, trees, rates, seconds, seed=2000, full=False, waves_per_cu=0
     """Served stream over the 64 resident octrees: seeded Poisson request
     streams, each request (octree U{0..63}, seeded orbit camera, resolution as
     the mixed views). For each mean rate the live server (svo_server_*: one
-    running persistent launch, requests folded in as they arrive, outputs
-    left in device memory) and the one-request-per-launch baseline (plain
+    running persistent launch, requests folded in as they arrive; a request
+    arriving with nothing in flight runs as its own launch; outputs left in
+    device memory) and the one-request-per-launch baseline (plain
     kernel, outputs in device memory) replay the same schedule; latency is
     scheduled arrival -> completion seen on the host. Requests per rate:
     min(5000, rate * seconds), or 5000 with `full`. A final burst (every
@@ -626,6 +627,7 @@ def run_served(pkg, trees, rates, seconds, seed=2000, full=False, waves_per_cu=0
     out["burst"] = {"live_server": s_, "one_per_launch": p_, "relaunch_batched_4_streams": r_,
                     "live_vs_one_per_launch": round(s_["grays_s"] / p_["grays_s"], 3)}
     out["server_launches"] = st["launches"]
+    out["server_direct_launches"] = st["direct"]
     out["server_waves_per_cu"] = waves_per_cu or 24
     return out
 

@@ -372,8 +372,11 @@ int svo_serve_replay(svo_octree *const *trees, int ntrees, const svo_request *re
  * any time, from any thread, in one persistent launch that runs while work is
  * in flight: a leader wave copies newly published requests from a ring in
  * pinned host memory into device tables, the other waves claim 8x8 tiles of
- * every published request from one queue, and a request that arrives while
- * others are being traced is picked up by the next waves that free up.
+ * every published request from 8 queues, and a request that arrives while
+ * others are being traced is picked up by the next waves that free up. While
+ * no persistent launch runs and fewer than four requests are in flight, a
+ * request runs as its own ordinary launch instead (lower latency on a lightly
+ * loaded GPU; persistent_only turns that off).
  * Requests are admitted in submission order while their outputs (16 B per
  * ray: normal u32, t f32, leaf u64) fit the output arena and a ring slot is
  * free; the others wait in the server. Results equal svo_render_primary's for
@@ -394,7 +397,11 @@ typedef struct {
     int host_outputs;       /* 0: outputs stay in device memory (svo_server_outputs gives
                                device pointers); 1: pinned host memory written by the kernel */
     int idle_stop_us;       /* the launch ends after this long with nothing in flight (0 = 2000) */
-    int reserved[6];
+    int persistent_only;    /* 0: a request admitted while no persistent launch runs and fewer
+                               than 4 requests are in flight runs as its own ordinary launch
+                               (svo_render_primary_device's kernel); 1: every request goes
+                               through the persistent launch */
+    int reserved[5];
 } svo_server_config;
 
 enum { SVO_REQ_WAITING = 0, SVO_REQ_IN_FLIGHT = 1, SVO_REQ_DONE = 2 };
@@ -405,7 +412,9 @@ typedef struct {
     uint64_t in_flight; /* published, not yet complete */
     uint64_t waiting;   /* submitted, not yet admitted (arena or ring full) */
     int live;           /* a launch is running */
-    int reserved[7];
+    int reserved0;
+    uint64_t direct;    /* requests run as their own launch (few others in flight) */
+    int reserved[4];
 } svo_server_stats;
 
 /* Makes the octrees resident on the current device and starts the request

@@ -22,6 +22,13 @@
 //    writes the request's sequence number into a completion word in host
 //    memory.
 //
+// A request admitted while no persistent launch runs and fewer than four
+// requests are in flight is traced by an ordinary primary launch on a stream
+// of its own instead, and its completion is an event: on a lightly loaded GPU
+// such launches end sooner than the persistent one would (all wave slots,
+// hardware dispatch, no claims or counters); beyond four requests in flight
+// the persistent launch takes over.
+//
 // The host side is a request loop on its own thread: it records completions
 // (latency = submit -> completion seen on the host), admits waiting requests
 // as memory and ring slots free up, keeps a heartbeat the kernel checks, and
@@ -358,7 +365,9 @@ struct svo_server {
     uint8_t *arena = nullptr;         // device or pinned
     hipStream_t stream = nullptr;
     hipStream_t copyStream = nullptr;  // copies of completed outputs to the host
+    hipStream_t directStreams[4] = {};   // requests run as their own launch, in turn
     hipEvent_t ended = nullptr;
+    bool persistentOnly = false;
     svo::RangeAlloc alloc;  // first fit over the output arena (offsets in bytes, kAlign units)
     // request state
     std::mutex mu;
@@ -366,7 +375,11 @@ struct svo_server {
     std::condition_variable loopCv;  // wakes the request loop when a launch starts
     std::unordered_map<uint64_t, svo::Ticket> tickets;
     std::deque<uint64_t> waiting;        // submitted, not yet admitted
-    std::deque<uint64_t> inflight;       // admitted, in sequence order
+    std::deque<uint64_t> inflight;       // admitted to the persistent launch, in sequence order
+    std::deque<std::pair<uint64_t, hipEvent_t>> direct;  // running as their own launch
+    std::vector<hipEvent_t> events;      // free completion events of direct launches
+    uint64_t directLaunches = 0;
+    static constexpr size_t kMaxDirect = 4;  // direct launches in flight at most
     std::vector<uint64_t> slotTicket;    // ring slot -> ticket
     uint64_t nextTicket = 1, nextSeq = 0;
     bool live = false;
@@ -393,9 +406,22 @@ void set_error(svo_server &s, const std::string &msg) {
     }
 }
 
+// Makes the server's device current for the scope (admission runs on the
+// submitting thread).
+struct OnDevice {
+    int prev = -1, dev;
+    explicit OnDevice(int d) : dev(d) {
+        if (hipGetDevice(&prev) == hipSuccess && prev != dev) (void)hipSetDevice(dev);
+    }
+    ~OnDevice() {
+        if (prev >= 0 && prev != dev) (void)hipSetDevice(prev);
+    }
+};
+
 // Starts a launch whose first request is sequence number s.nextSeq (called
 // with the lock held, no launch running).
 bool start_launch(svo_server &s) {
+    OnDevice on(s.device);
     s.seq0 = s.nextSeq;
     __atomic_store_n(&s.ctrl->stop, 0ull, __ATOMIC_RELEASE);
     __atomic_store_n(&s.ctrl->mirrored, s.seq0, __ATOMIC_RELEASE);
@@ -440,12 +466,73 @@ void stop_launch(svo_server &s) {
     s.live = false;
 }
 
+// Output pointers of a request placed at arena offset `off`.
+void request_outputs(const svo_server &s, const Ticket &tk, uint32_t *&normal, float *&t, uint64_t *&leaf) {
+    const uint64_t n = uint64_t(tk.req.width) * uint64_t(tk.req.height);
+    normal = reinterpret_cast<uint32_t *>(s.arena + tk.off);
+    t = reinterpret_cast<float *>(s.arena + tk.off + round_up(n * 4));
+    leaf = reinterpret_cast<uint64_t *>(s.arena + tk.off + 2 * round_up(n * 4));
+}
+
+// While no persistent launch runs and fewer than kMaxDirect requests are in
+// flight, a request runs as an ordinary primary launch (the kernel
+// svo_render_primary_device uses) on one of kMaxDirect streams in turn: on a
+// lightly loaded GPU that ends sooner than folding it into the persistent
+// launch, which pays off once more requests overlap. Returns false when the
+// arena cannot hold it yet or on an error (lock held).
+bool launch_direct(svo_server &s, uint64_t id) {
+    Ticket &tk = s.tickets[id];
+    uint64_t off = 0;
+    if (!s.alloc.alloc(tk.bytes, off)) return false;
+    tk.off = off;
+    hipEvent_t ev = nullptr;
+    if (!s.events.empty()) {
+        ev = s.events.back();
+        s.events.pop_back();
+    } else if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+        s.alloc.release(off, tk.bytes);
+        set_error(s, "server: event for a direct launch");
+        return false;
+    }
+    uint32_t *normal;
+    float *t;
+    uint64_t *leaf;
+    request_outputs(s, tk, normal, t, leaf);
+    OnDevice on(s.device);
+    hipStream_t st = s.directStreams[s.directLaunches % svo_server::kMaxDirect];
+    int r = render_primary_device(*s.trees[size_t(tk.req.tree)], tk.req.camera, tk.req.width, tk.req.height, nullptr,
+                                  normal, t, leaf, st);
+    hipError_t e = r ? hipSuccess : hipEventRecord(ev, st);
+    if (r || e != hipSuccess) {
+        s.alloc.release(off, tk.bytes);
+        s.events.push_back(ev);
+        set_error(s, r ? std::string("server: direct launch: ") + svo_last_error()
+                       : std::string("server: direct launch: ") + hipGetErrorString(e));
+        return false;
+    }
+    tk.state = 1;
+    s.direct.emplace_back(id, ev);
+    s.published++;
+    s.directLaunches++;
+    s.lastBusy = Clock::now();
+    s.loopCv.notify_one();  // the loop watches the launch
+    return true;
+}
+
 // Admits waiting requests in arrival order while arena memory, a ring slot
-// and room in the launch's request table are free, and publishes them.
+// and room in the launch's request table are free, and publishes them (to a
+// direct launch when nothing is in flight).
 void admit(svo_server &s) {
     if (s.error) return;
     while (!s.waiting.empty()) {
         Ticket &tk = s.tickets[s.waiting.front()];
+        if (!s.persistentOnly && !s.live && s.inflight.empty() && s.direct.size() < svo_server::kMaxDirect) {
+            // no persistent launch and few requests in flight: the request
+            // runs as its own launch
+            if (!launch_direct(s, s.waiting.front())) return;
+            s.waiting.pop_front();
+            continue;
+        }
         // ring window: the oldest request in flight holds its slot
         const uint64_t oldest = s.inflight.empty() ? s.nextSeq : s.tickets[s.inflight.front()].seq;
         if (s.nextSeq - oldest >= uint64_t(s.ring)) return;
@@ -469,15 +556,12 @@ void admit(svo_server &s) {
         s.slotTicket[size_t(slot)] = id;
         ServeSlot &hs = s.hslots[slot];
         const svo_request &r = tk.req;
-        const uint64_t n = uint64_t(r.width) * uint64_t(r.height);
         svo_view v{};
         v.tree = s.trees[size_t(r.tree)];
         v.camera = r.camera;
         v.width = r.width;
         v.height = r.height;
-        v.normal = reinterpret_cast<uint32_t *>(s.arena + off);
-        v.t = reinterpret_cast<float *>(s.arena + off + round_up(n * 4));
-        v.leaf = reinterpret_cast<uint64_t *>(s.arena + off + 2 * round_up(n * 4));
+        request_outputs(s, tk, v.normal, v.t, v.leaf);
         ViewDesc d;
         (void)fill_view_desc(v, d);  // checked at submit
         std::memcpy(&hs.desc, &d, sizeof(d));
@@ -494,6 +578,20 @@ void admit(svo_server &s) {
 // Records completions of requests in flight (lock held).
 bool reap(svo_server &s) {
     bool any = false;
+    for (auto it = s.direct.begin(); it != s.direct.end();) {
+        const hipError_t e = hipEventQuery(it->second);
+        if (e == hipErrorNotReady) {
+            ++it;
+            continue;
+        }
+        if (e != hipSuccess) set_error(s, std::string("server: direct launch: ") + hipGetErrorString(e));
+        Ticket &tk = s.tickets[it->first];
+        tk.state = 2;
+        tk.completed = now_s(s);
+        s.events.push_back(it->second);
+        it = s.direct.erase(it);
+        any = true;
+    }
     for (auto it = s.inflight.begin(); it != s.inflight.end();) {
         Ticket &tk = s.tickets[*it];
         const int slot = int(tk.seq % uint64_t(s.ring));
@@ -529,13 +627,13 @@ void loop_main(svo_server *sp) {
     std::unique_lock<std::mutex> lock(s.mu);
     while (!s.quit) {
         __atomic_fetch_add(&s.ctrl->heartbeat, 1ull, __ATOMIC_RELEASE);
+        if (reap(s)) {
+            s.lastBusy = Clock::now();
+            admit(s);
+            s.cv.notify_all();
+        }
         if (s.live) {
-            if (reap(s)) {
-                s.lastBusy = Clock::now();
-                admit(s);
-                s.cv.notify_all();
-            }
-            if (s.live && hipEventQuery(s.ended) == hipSuccess) {
+            if (hipEventQuery(s.ended) == hipSuccess) {
                 // the launch ended by itself (heartbeat lost): requests it did
                 // not take over are published again in a new launch
                 s.live = false;
@@ -573,14 +671,14 @@ void loop_main(svo_server *sp) {
             admit(s);
         }
         if (s.error) s.cv.notify_all();
-        if (s.live) {
+        if (s.live || !s.direct.empty()) {
             // spin briefly without the lock, so submitters and pollers get in
             lock.unlock();
             for (int k = 0; k < 64; ++k) __builtin_ia32_pause();
             lock.lock();
         } else {
-            // no launch: sleep until one starts (start_launch signals) or a
-            // waiting request may be admitted
+            // nothing running: sleep until a launch starts (start_launch and
+            // launch_direct signal) or a waiting request may be admitted
             s.loopCv.wait_for(lock, std::chrono::microseconds(200));
         }
     }
@@ -606,6 +704,12 @@ void free_server(svo_server *s) {
         s->loop.join();
     }
     if (s->stream) (void)hipStreamSynchronize(s->stream);
+    for (hipStream_t ds : s->directStreams)
+        if (ds) (void)hipStreamSynchronize(ds);
+    for (auto &d : s->direct) s->events.push_back(d.second);
+    for (hipEvent_t ev : s->events) (void)hipEventDestroy(ev);
+    for (hipStream_t ds : s->directStreams)
+        if (ds) (void)hipStreamDestroy(ds);
     if (s->ended) (void)hipEventDestroy(s->ended);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     if (s->copyStream) (void)hipStreamDestroy(s->copyStream);
@@ -647,6 +751,7 @@ int server_create(svo_octree *const *trees, int ntrees, const svo_server_config 
     s->budget = budget / kAlign * kAlign;
     s->idleStop = idle;
     s->wavesPerCU = waves;
+    s->persistentOnly = cfg && cfg->persistent_only;
     int maxDepth = 1;
     for (svo_octree *t : s->trees) {
         int r = ensure_device(*t);  // octrees stay resident while the server lives
@@ -680,6 +785,8 @@ int server_create(svo_octree *const *trees, int ntrees, const svo_server_config 
         }
         if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
         if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->copyStream, hipStreamNonBlocking);
+        for (hipStream_t &ds : s->directStreams)
+            if (e == hipSuccess) e = hipStreamCreateWithFlags(&ds, hipStreamNonBlocking);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&s->ended, hipEventDisableTiming);
     }
     if (e != hipSuccess) {
@@ -815,9 +922,10 @@ int server_stats(svo_server *s, svo_server_stats *out) {
     std::memset(out, 0, sizeof(*out));
     out->launches = s->launches;
     out->published = s->published;
-    out->in_flight = s->inflight.size();
+    out->in_flight = s->inflight.size() + s->direct.size();
     out->waiting = s->waiting.size();
     out->live = s->live ? 1 : 0;
+    out->direct = s->directLaunches;
     return SVO_OK;
 }
 

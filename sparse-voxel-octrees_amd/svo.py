@@ -175,12 +175,14 @@ def serve_replay(trees, requests, arrivals, streams=4, output_budget=1 << 30, ma
 
 class ServerConfig(ctypes.Structure):
     _fields_ = [("output_budget", ctypes.c_uint64), ("ring_slots", ctypes.c_int), ("waves_per_cu", ctypes.c_int),
-                ("host_outputs", ctypes.c_int), ("idle_stop_us", ctypes.c_int), ("reserved", ctypes.c_int * 6)]
+                ("host_outputs", ctypes.c_int), ("idle_stop_us", ctypes.c_int), ("persistent_only", ctypes.c_int),
+                ("reserved", ctypes.c_int * 5)]
 
 
 class ServerStats(ctypes.Structure):
     _fields_ = [("launches", ctypes.c_uint64), ("published", ctypes.c_uint64), ("in_flight", ctypes.c_uint64),
-                ("waiting", ctypes.c_uint64), ("live", ctypes.c_int), ("reserved", ctypes.c_int * 7)]
+                ("waiting", ctypes.c_uint64), ("live", ctypes.c_int), ("reserved0", ctypes.c_int),
+                ("direct", ctypes.c_uint64), ("reserved", ctypes.c_int * 4)]
 
 
 REQ_WAITING, REQ_IN_FLIGHT, REQ_DONE = 0, 1, 2
@@ -195,15 +197,18 @@ def _requests(requests):
 
 class Server:
     """Live request server (svo_server): requests submitted at any time are
-    folded into one running persistent launch. requests are (tree index,
-    Camera, width, height) over the octrees given here."""
+    folded into one running persistent launch (a request arriving with nothing
+    in flight runs as its own launch unless persistent_only). requests are
+    (tree index, Camera, width, height) over the octrees given here."""
 
-    def __init__(self, trees, output_budget=0, ring_slots=0, waves_per_cu=0, host_outputs=False, idle_stop_us=0):
+    def __init__(self, trees, output_budget=0, ring_slots=0, waves_per_cu=0, host_outputs=False, idle_stop_us=0,
+                 persistent_only=False):
         self._trees = list(trees)
         tarr = (ctypes.c_void_p * len(self._trees))(*[t.handle.value for t in self._trees])
         c = ServerConfig()
         c.output_budget, c.ring_slots, c.waves_per_cu = int(output_budget), int(ring_slots), int(waves_per_cu)
         c.host_outputs, c.idle_stop_us = int(bool(host_outputs)), int(idle_stop_us)
+        c.persistent_only = int(bool(persistent_only))
         self._s = ctypes.c_void_p()
         _check(lib.svo_server_create(tarr, len(self._trees), ctypes.byref(c), ctypes.byref(self._s)))
 
@@ -244,7 +249,7 @@ class Server:
     def stats(self):
         st = ServerStats()
         _check(lib.svo_server_stats_get(self._s, ctypes.byref(st)))
-        return {k: getattr(st, k) for k in ("launches", "published", "in_flight", "waiting", "live")}
+        return {k: getattr(st, k) for k in ("launches", "published", "in_flight", "waiting", "live", "direct")}
 
     def run_schedule(self, requests, arrivals, keep=False):
         """Submits request i at arrivals[i] seconds; returns (latencies,

@@ -108,27 +108,51 @@ def test_admission_by_output_footprint(svo, trees):
 
 
 def test_launch_ends_when_idle_and_restarts(svo, trees):
-    """The launch ends after idle_stop_us with nothing in flight; the next
-    submission starts a new one and is traced correctly."""
+    """Requests beyond the four that run as their own launches go to the
+    persistent launch; it ends after idle_stop_us with nothing in flight, and
+    the next burst starts a new one; every result is traced correctly."""
     srv = svo.svo.Server(trees, idle_stop_us=1000)
-    reqs = _requests(trees, 6, seed=23)
+    reqs = _requests(trees, 16, seed=23)
     print("[gpu] live server idle stop / restart", flush=True)
-    for r in reqs[:3]:
-        tk = srv.submit(*r)
-        srv.wait(tk, timeout=60.0)
-        _check(trees, r, srv.copy_outputs(tk, r[2], r[3]))
-        srv.release(tk)
+
+    def burst(rs):
+        tks = [srv.submit(*r) for r in rs]
+        for r, tk in zip(rs, tks):
+            srv.wait(tk, timeout=60.0)
+            _check(trees, r, srv.copy_outputs(tk, r[2], r[3]))
+            srv.release(tk)
+
+    burst(reqs[:8])
     deadline = time.time() + 5.0
     while srv.stats()["live"] and time.time() < deadline:
         time.sleep(0.01)
     assert srv.stats()["live"] == 0
     first = srv.stats()["launches"]
-    for r in reqs[3:]:
+    assert first >= 1
+    burst(reqs[8:])
+    assert srv.stats()["launches"] >= first + 1
+    srv.close()
+
+
+@pytest.mark.parametrize("persistent_only", [False, True])
+def test_lone_requests_run_as_their_own_launch(svo, trees, persistent_only):
+    """A request submitted with nothing in flight runs as an ordinary launch
+    (stats: direct) unless persistent_only; either way its result equals the
+    request rendered alone and the oracle on every 7th pixel."""
+    srv = svo.svo.Server(trees, idle_stop_us=1000, persistent_only=persistent_only)
+    reqs = _requests(trees, 4, seed=41)
+    print(f"[gpu] live server lone requests, persistent_only={persistent_only}", flush=True)
+    for r in reqs:
         tk = srv.submit(*r)
         srv.wait(tk, timeout=60.0)
-        _check(trees, r, srv.copy_outputs(tk, r[2], r[3]))
+        _check(trees, r, srv.copy_outputs(tk, r[2], r[3]), oracle_every=7)
         srv.release(tk)
-    assert srv.stats()["launches"] >= first + 1
+    st = srv.stats()
+    assert st["published"] == len(reqs) and st["in_flight"] == 0
+    if persistent_only:
+        assert st["direct"] == 0 and st["launches"] >= 1
+    else:
+        assert st["direct"] == len(reqs) and st["launches"] == 0
     srv.close()
 
 
