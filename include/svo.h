@@ -443,7 +443,9 @@ int svo_server_stats_get(svo_server *s, svo_server_stats *out);
  * With tickets non-NULL the requests are kept (tickets[i] is request i's; the
  * caller reads and releases them, and the arena must hold every output);
  * otherwise each request is released once complete. makespan_s (may be NULL)
- * = first arrival -> last completion. */
+ * = first arrival -> last completion. The calling thread polls from 200 us
+ * before each arrival (a timed sleep can wake late), so it keeps a core busy
+ * while arrivals are close together. */
 int svo_server_run_schedule(svo_server *s, const svo_request *requests, const double *arrival_s, int n,
                             uint64_t *tickets, double *latency_s, double *makespan_s);
 /* Ends the launch (finishing requests in flight) and frees the server. */

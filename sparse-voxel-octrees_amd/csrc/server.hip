@@ -982,14 +982,25 @@ int server_run_schedule(svo_server *s, const svo_request *reqs, const double *ar
                 return fail(SVO_ERR_LIMIT, "schedule: the kept outputs fill the arena; raise output_budget");
             } else if (completed < n) {
                 // sleep on the request loop's completion signal (which releases
-                // the lock to it) until the next arrival is due, at most 50 us
-                auto until = Clock::now() + std::chrono::microseconds(50);
+                // the lock to it), at most 50 us; within 200 us of the next
+                // arrival poll instead (a timed sleep can wake tens of
+                // microseconds late, and latency counts from the arrival)
+                const auto nowc = Clock::now();
+                auto until = nowc + std::chrono::microseconds(50);
+                bool poll = false;
                 if (next < n) {
                     const auto due = t0 + std::chrono::duration_cast<Clock::duration>(
                                               std::chrono::duration<double>(arrival[next]));
-                    if (due < until) until = due;
+                    poll = due - nowc < std::chrono::microseconds(200);
+                    if (due - std::chrono::microseconds(100) < until) until = due - std::chrono::microseconds(100);
                 }
-                s->cv.wait_until(lock, until);
+                if (poll) {
+                    lock.unlock();
+                    for (int k = 0; k < 64; ++k) __builtin_ia32_pause();
+                    lock.lock();
+                } else {
+                    s->cv.wait_until(lock, until);
+                }
             }
         }
     }
