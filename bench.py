@@ -521,6 +521,34 @@ def run_frames(pkg, torch, stream, tree, cams, width, height, steps, warmup, cpu
     out["frames_call"] = {"ms_per_frame": round(ms2 / len(cams), 4), "frames_per_s": round(len(cams) / (ms2 / 1e3), 1),
                           "vs_one_frame_per_call": round(ms / ms2, 3), "mismatched_vs_one_frame_per_call": diff,
                           "form": f"one svo_render_frames_device call for the {len(cams)} frames"}
+    # One frame per call through a tile plan per camera (a viewer's next frame
+    # of the same view): the fine pass starts the tiles the previous frame
+    # found slowest first. Same images and depth tables.
+    plans = [pkg.svo.TilePlan.for_frame(width, height, 16) for _ in cams]
+    rgba3 = [torch.empty_like(x) for x in rgba]
+    depth3 = [torch.empty_like(x) for x in depth]
+
+    def step3():
+        for k, c in enumerate(cams):
+            tree.render_frame_device(c, width, height, rgba3[k].data_ptr(), depth3[k].data_ptr(), 16, 1, sptr,
+                                     plan=plans[k])
+
+    for _ in range(max(warmup, 1)):
+        step3()
+    torch.cuda.synchronize()
+    a.record(stream)
+    for _ in range(steps):
+        step3()
+    b.record(stream)
+    torch.cuda.synchronize()
+    ms3 = a.elapsed_time(b) / steps
+    diff3 = sum(int(torch.count_nonzero(rgba3[k] != rgba[k])) + int(torch.count_nonzero(
+        depth3[k].view(torch.int32) != depth[k].view(torch.int32))) for k in range(len(cams)))
+    out["planned"] = {"ms_per_frame": round(ms3 / len(cams), 4), "frames_per_s": round(len(cams) / (ms3 / 1e3), 1),
+                      "vs_one_frame_per_call": round(ms / ms3, 3), "mismatched_vs_one_frame_per_call": diff3,
+                      "form": "one frame per call, the fine pass in a tile plan's cost order (one plan per camera)"}
+    for pl in plans:
+        pl.close()
     if cpu_check:
         words = tree.words()
         tf = np.frombuffer(cams[0].tform, np.float32)
@@ -959,6 +987,8 @@ def main():
     if mismatches or batched_mismatch or config.get("build_oct_md5_matches_reference") is False or \
             (mixed and mixed["mismatched_fields_vs_plain"]) or (terrain and terrain["parity"]["mismatched_fields"]) or \
             (frames and (frames["parity"]["mismatched_pixels"] or frames["parity"]["mismatched_depths"]
+                         or frames["frames_call"]["mismatched_vs_one_frame_per_call"]
+                         or frames["planned"]["mismatched_vs_one_frame_per_call"]
                          or any(frames["parity"].get("vs_reference_classes", {}).values()))):
         sys.exit(3)
 

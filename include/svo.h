@@ -160,7 +160,8 @@ int svo_render_primary_device(svo_octree *h, const svo_camera *cam, int width, i
  * leaving them to run alone at the end of the launch. The first render of a
  * plan runs in raster order. A plan holds 128 bytes of device memory per tile
  * (4 MiB at 1920x1080). One plan per image size and stream: renders on one
- * plan must be ordered (one stream). */
+ * plan must be ordered (one stream). Each render entry takes only plans made
+ * by its own create call (SVO_ERR_INVALID otherwise). */
 typedef struct svo_tile_plan svo_tile_plan;
 int svo_tile_plan_create(int width, int height, svo_tile_plan **out);
 void svo_tile_plan_free(svo_tile_plan *p);
@@ -204,6 +205,15 @@ int svo_render_frame_device(svo_octree *h, const svo_camera *cam, int width, int
  * Every camera and image pointer is checked before anything is enqueued. */
 int svo_render_frames_device(svo_octree *h, const svo_camera *cams, int nframes, int width, int height, int stripes,
                              int stride, uint32_t *const *rgba, float *const *depth, void *stream);
+/* svo_render_frame_device with the fine pass's tiles in a tile plan's cost
+ * order (no counterpart in the reference): a viewer's next frame starts the
+ * tiles its previous frame found slowest first; the same pixels and depths.
+ * The plan is made for the frame's size and stripe split
+ * (svo_tile_plan_create_frame; SVO_ERR_INVALID otherwise, before anything is
+ * enqueued); frames through one plan must be ordered (one stream). */
+int svo_tile_plan_create_frame(int width, int height, int stripes, svo_tile_plan **out);
+int svo_render_frame_planned_device(svo_octree *h, svo_tile_plan *plan, const svo_camera *cam, int width, int height,
+                                    int stripes, int stride, uint32_t *rgba, float *depth, void *stream);
 /* Coarse depth entries of a frame, or -1 (message in svo_last_error) when the
  * size or stripe split is refused. */
 int64_t svo_frame_depth_count(int width, int height, int stripes);

@@ -23,6 +23,7 @@
 
 #include "build_common.hpp"
 #include "march.hpp"
+#include "tile_order.hpp"
 #include "svo_internal.hpp"
 
 namespace svo {
@@ -71,16 +72,19 @@ __device__ __forceinline__ int stripe_y1(const FrameParams &p, int s) { return m
 
 // One ray to completion; the leaf word (normal) is read only when Normal is set.
 // The frame kernels keep the whole restart stack in LDS (the fine pass measured
-// slower with the root's entry in registers).
-template <bool Lod, bool Normal, typename Idx>
+// slower with the root's entry in registers). With Count, *passes counts the
+// loop passes.
+template <bool Lod, bool Normal, typename Idx, bool Count = false>
 __device__ __forceinline__ int trace(const uint32_t *__restrict__ tree, float ox, float oy, float oz, float dx, float dy,
-                                     float dz, float rayScale, const Stack<Idx> &stk, uint32_t &normal, float &t) {
+                                     float dz, float rayScale, const Stack<Idx> &stk, uint32_t &normal, float &t,
+                                     uint32_t *passes = nullptr) {
     Ray<Idx> r;
     ray_init(r, tree, ox, oy, oz, dx, dy, dz, rayScale);
     Idx leaf = 0;
     int code;
     do {
         code = ray_step<Lod, true, Idx, Normal, 0>(r, tree, stk, normal, t, leaf);  // camera rays are finite
+        if (Count) ++*passes;
     } while (code == kRunning);
     return code;
 }
@@ -122,12 +126,16 @@ __global__ void __launch_bounds__(64) coarse_kernel(const uint32_t *__restrict__
     depth[i] = code != march::kMiss ? t : kTreeMiss;
 }
 
-template <typename Idx, int N>
+//
+// Planned (one frame): the tiles in a tile plan's cost order (tile_order.hpp),
+// each filed by cost for the plan's next frame.
+template <typename Idx, int N, bool Planned = false>
 __global__ void __launch_bounds__(64) fine_kernel(const uint32_t *__restrict__ tree, const FrameSet<N> set,
-                                                  unsigned tilesPerFrame, int slots) {
+                                                  unsigned tilesPerFrame, int slots, TileOrder o) {
     extern __shared__ __align__(16) unsigned char lds[];
-    const unsigned f = N == 1 ? 0u : blockIdx.x / tilesPerFrame;
-    const int64_t tile = blockIdx.x - f * tilesPerFrame;
+    const int64_t g = Planned ? int64_t(ordered_tile(o, int(blockIdx.x), int(threadIdx.x))) : int64_t(blockIdx.x);
+    const unsigned f = N == 1 ? 0u : unsigned(g / tilesPerFrame);
+    const int64_t tile = g - int64_t(f) * tilesPerFrame;
     const FrameParams &p = set.p[f];
     const float *__restrict__ depth = set.depth[f];
     uint32_t *__restrict__ rgba = set.rgba[f];
@@ -145,7 +153,7 @@ __global__ void __launch_bounds__(64) fine_kernel(const uint32_t *__restrict__ t
     const int px = lane & 7, py = lane >> 3;
     const int x = x0 + px, y = y0 + py;
     const bool valid = x < x1 && y < y1;
-    uint32_t color = 0;
+    uint32_t color = 0, passes = 0;
     if (minT != kTreeMiss) {
         const bool corner = (px % p.stride) == 0 && (py % p.stride) == 0;
         if (valid && corner) {
@@ -160,8 +168,8 @@ __global__ void __launch_bounds__(64) fine_kernel(const uint32_t *__restrict__ t
             uint32_t normal = 0;
             float t = 0.0f;
             float v = 0.0f;
-            const int code = trace<false, true, Idx>(tree, p.px + d[0] * m, p.py + d[1] * m, p.pz + d[2] * m, d[0], d[1], d[2],
-                                               0.0f, stk, normal, t);
+            const int code = trace<false, true, Idx, Planned>(tree, p.px + d[0] * m, p.py + d[1] * m, p.pz + d[2] * m,
+                                                              d[0], d[1], d[2], 0.0f, stk, normal, t, &passes);
             if (code != march::kMiss) {
                 // shade, src/Main.cpp:81-90
                 float n[3], c;
@@ -172,14 +180,15 @@ __global__ void __launch_bounds__(64) fine_kernel(const uint32_t *__restrict__ t
                 const float specular = dd * dd;
                 v = c * 0.9f * fabsf(p.lx * n[0] + p.ly * n[1] + p.lz * n[2]) + specular * 0.2f;
             }
-            const uint32_t g = uint32_t(double(dev::fmin_ref(v, 1.0f)) * 255.0);
-            color = g | (g << 8) | (g << 16) | 0xFF000000u;
+            const uint32_t grey = uint32_t(double(dev::fmin_ref(v, 1.0f)) * 255.0);
+            color = grey | (grey << 8) | (grey << 16) | 0xFF000000u;
         }
         // pixels between stride corners copy their corner (renderTile's half-size mode)
         const int src = (py - py % p.stride) * 8 + (px - px % p.stride);
         color = __shfl(color, src);
     }
     if (valid) rgba[int64_t(y) * p.width + x] = color;
+    if (Planned) file_tile(o, int(blockIdx.x), int(g), passes, lane);
 }
 
 }  // namespace
@@ -249,17 +258,27 @@ constexpr int kFramesPerLaunch = 16;
 // launch over every frame's corners, then one fine launch over every frame's
 // tiles. Frames with a null depth table get one from a stream-ordered scratch
 // allocation.
-template <int N>
-int launch_frames(svo_octree &h, FrameSet<N> &set, int n, hipStream_t s) {
-    const FrameParams &p = set.p[0];
-    // corner and fine-tile counts over the stripes (the last may be shorter)
-    int64_t corners = 0, tiles = 0;
+// Corner and fine-tile counts of a frame over its stripes (the last may be shorter).
+void frame_counts(const FrameParams &p, int64_t &corners, int64_t &tiles) {
+    corners = 0;
+    tiles = 0;
     for (int i = 0; i < p.stripes; ++i) {
         const int y0 = i * p.sh, y1 = std::min(y0 + p.sh, p.height);
         const int tilesY = (y1 - y0 - 1) / 8 + 2;
         corners += int64_t(tilesY) * p.tilesX;
         tiles += int64_t(tilesY - 1) * (p.tilesX - 1);
     }
+}
+
+// With `plan` (n == 1), the fine pass runs in the plan's cost order.
+template <int N>
+int launch_frames(svo_octree &h, FrameSet<N> &set, int n, hipStream_t s, svo_tile_plan *plan = nullptr) {
+    const FrameParams &p = set.p[0];
+    int64_t corners = 0, tiles = 0;
+    frame_counts(p, corners, tiles);
+    if (plan && (n != 1 || plan->kind != svo_tile_plan::kFrame || plan->width != p.width || plan->height != p.height ||
+                 plan->stripes != p.stripes || int64_t(plan->tiles) != tiles))
+        return fail(SVO_ERR_INVALID, "the tile plan was made for another frame size or stripe split");
     const int64_t cblocks = (corners + 63) / 64;
     if (tiles * n > 0x7FFFFFFF || cblocks * n > 0x7FFFFFFF) return fail(SVO_ERR_LIMIT, "image too large");
     int nulls = 0;
@@ -272,27 +291,59 @@ int launch_frames(svo_octree &h, FrameSet<N> &set, int n, hipStream_t s) {
     const bool wide = h.words.size() >= (uint64_t(1) << 32);
     const size_t lds = size_t(slots) * 64 * ((wide ? 8 : 4) + sizeof(float));
     const dim3 cgrid(static_cast<unsigned>(cblocks * n)), fgrid(static_cast<unsigned>(tiles * n)), block(64);
+    const TileOrder none{};
     if (!wide) {
         hipLaunchKernelGGL((coarse_kernel<uint32_t, N>), cgrid, block, lds, s, h.dwords, set, corners, unsigned(cblocks),
                            slots);
-        if (tiles)
-            hipLaunchKernelGGL((fine_kernel<uint32_t, N>), fgrid, block, lds, s, h.dwords, set, unsigned(tiles), slots);
+        if (tiles && plan)
+            hipLaunchKernelGGL((fine_kernel<uint32_t, N, true>), fgrid, block, lds, s, h.dwords, set, unsigned(tiles),
+                               slots, next_order(*plan));
+        else if (tiles)
+            hipLaunchKernelGGL((fine_kernel<uint32_t, N>), fgrid, block, lds, s, h.dwords, set, unsigned(tiles), slots,
+                               none);
     } else {
         hipLaunchKernelGGL((coarse_kernel<uint64_t, N>), cgrid, block, lds, s, h.dwords, set, corners, unsigned(cblocks),
                            slots);
-        if (tiles)
-            hipLaunchKernelGGL((fine_kernel<uint64_t, N>), fgrid, block, lds, s, h.dwords, set, unsigned(tiles), slots);
+        if (tiles && plan)
+            hipLaunchKernelGGL((fine_kernel<uint64_t, N, true>), fgrid, block, lds, s, h.dwords, set, unsigned(tiles),
+                               slots, next_order(*plan));
+        else if (tiles)
+            hipLaunchKernelGGL((fine_kernel<uint64_t, N>), fgrid, block, lds, s, h.dwords, set, unsigned(tiles), slots,
+                               none);
     }
     hipError_t e = hipGetLastError();
     if (scratch) (void)hipFreeAsync(scratch, s);
     if (e != hipSuccess) return fail(SVO_ERR_DEVICE, std::string("frame: ") + hipGetErrorString(e));
+    if (plan && tiles) plan->renders++;
     return SVO_OK;
 }
 
 }  // namespace
 
+int tile_plan_create_frame(int width, int height, int stripes, svo_tile_plan **out) {
+    if (!out) return fail(SVO_ERR_INVALID, "null argument");
+    *out = nullptr;
+    int r = frame_layout(width, height, stripes);
+    if (r) return r;
+    svo_camera cam{};
+    const FrameParams p = frame_params(cam, width, height, stripes, 1);
+    int64_t corners = 0, tiles = 0;
+    frame_counts(p, corners, tiles);
+    r = tile_plan_alloc(tiles, width, height, out);
+    if (r == SVO_OK) {
+        (*out)->kind = svo_tile_plan::kFrame;
+        (*out)->stripes = stripes;
+    }
+    return r;
+}
+
 int render_frame_device(svo_octree &h, const svo_camera &cam, int width, int height, int stripes, int stride,
-                        uint32_t *rgba, float *depthOut, void *stream) {
+                        uint32_t *rgba, float *depthOut, void *stream, svo_tile_plan *plan) {
+    if (plan) {
+        int dev = -1;
+        HIP_TRY(hipGetDevice(&dev));
+        if (dev != plan->device) return fail(SVO_ERR_INVALID, "the tile plan lives on another device");
+    }
     int r = frame_layout(width, height, stripes);
     if (r) return r;
     if (stride <= 0 || stride > 8) return fail(SVO_ERR_INVALID, "stride must be in [1, 8]");
@@ -304,7 +355,7 @@ int render_frame_device(svo_octree &h, const svo_camera &cam, int width, int hei
     set.p[0] = frame_params(cam, width, height, stripes, stride);
     set.rgba[0] = rgba;
     set.depth[0] = depthOut;
-    return launch_frames(h, set, 1, static_cast<hipStream_t>(stream));
+    return launch_frames(h, set, 1, static_cast<hipStream_t>(stream), plan);
 }
 
 // Several frames of one size in one call: each group of up to 16 frames is one

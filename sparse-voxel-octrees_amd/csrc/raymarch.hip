@@ -402,8 +402,8 @@ void launch_ordered(const svo_octree &h, const TileLaunch &L, const PrimaryParam
 int render_primary_planned_device(svo_octree &h, svo_tile_plan *plan, const svo_camera &cam, int width, int height,
                                   uint8_t *hit, uint32_t *normal, float *t, uint64_t *leaf, void *stream) {
     if (!plan) return fail(SVO_ERR_INVALID, "null tile plan");
-    if (width != plan->width || height != plan->height)
-        return fail(SVO_ERR_INVALID, "the tile plan was made for another image size");
+    if (plan->kind != svo_tile_plan::kPrimary || width != plan->width || height != plan->height)
+        return fail(SVO_ERR_INVALID, "the tile plan was made for another image size or render entry");
     if (!camera_ok(cam)) return fail(SVO_ERR_INVALID, "camera entries must be finite and within +-1e6");
     int dev = -1;
     HIP_TRY(hipGetDevice(&dev));

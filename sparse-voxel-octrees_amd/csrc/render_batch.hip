@@ -515,7 +515,9 @@ int tile_plan_create_batch(const svo_view *views, int nviews, const svo_batch_op
             return fail(SVO_ERR_INVALID, "image size out of range");
         tiles += int64_t((views[i].width + tw - 1) / tw) * int64_t((views[i].height + th - 1) / th);
     }
-    return tile_plan_alloc(tiles, 0, 0, out);
+    const int r = tile_plan_alloc(tiles, 0, 0, out);
+    if (r == SVO_OK) (*out)->kind = svo_tile_plan::kBatch;
+    return r;
 }
 
 // render_batch in the grid form with the tiles in a tile plan's order.
@@ -527,6 +529,7 @@ int render_batch_planned(svo_renderer *rd, const svo_view *views, int nviews, co
     int dev = -1;
     HIP_TRY(hipGetDevice(&dev));
     if (dev != tp->device) return fail(SVO_ERR_INVALID, "the tile plan lives on another device");
+    if (tp->kind != svo_tile_plan::kBatch) return fail(SVO_ERR_INVALID, "the tile plan was not made for a batch");
     std::lock_guard<std::mutex> lock(rd->mu);
     HIP_TRY(hipEventSynchronize(rd->staged));
     LaunchPlan plan;

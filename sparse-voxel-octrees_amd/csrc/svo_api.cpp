@@ -465,6 +465,22 @@ int svo_render_frame_device(svo_octree *h, const svo_camera *cam, int width, int
     });
 }
 
+int svo_tile_plan_create_frame(int width, int height, int stripes, svo_tile_plan **out) {
+    return guarded([&]() -> int {
+        clear_error();
+        return tile_plan_create_frame(width, height, stripes, out);
+    });
+}
+
+int svo_render_frame_planned_device(svo_octree *h, svo_tile_plan *plan, const svo_camera *cam, int width, int height,
+                                    int stripes, int stride, uint32_t *rgba, float *depth, void *stream) {
+    return guarded([&]() -> int {
+        clear_error();
+        if (!h || !cam || !plan) return fail(SVO_ERR_INVALID, "null argument");
+        return render_frame_device(*h, *cam, width, height, stripes, stride, rgba, depth, stream, plan);
+    });
+}
+
 int svo_render_frames_device(svo_octree *h, const svo_camera *cams, int nframes, int width, int height, int stripes,
                              int stride, uint32_t *const *rgba, float *const *depth, void *stream) {
     return guarded([&]() -> int {

@@ -29,7 +29,10 @@
 
 // A tile plan: counter sets and id regions on one device (opaque in the C ABI).
 struct svo_tile_plan {
-    int width = 0, height = 0;   // plain renders: the image size (0 x 0 for a batch plan)
+    enum Kind : int { kPrimary, kBatch, kFrame };
+    Kind kind = kPrimary;        // the render entry the plan was made for
+    int width = 0, height = 0;   // primary renders and frames: the image size (0 x 0 for a batch plan)
+    int stripes = 0;             // frames: the stripe split
     int tiles = 0;               // tiles of every launch made through the plan
     int cap = 0;                 // ceil(tiles / 64): the most tiles a replica holds
     int device = -1;

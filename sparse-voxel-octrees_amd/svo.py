@@ -39,7 +39,7 @@ EXPORTS = [
     "svo_server_copy_outputs", "svo_server_release", "svo_server_stats_get", "svo_server_run_schedule",
     "svo_server_destroy", "svo_tile_plan_create", "svo_tile_plan_free", "svo_render_primary_planned_device",
     "svo_render_batch_noalloc", "svo_load_device_code", "svo_tile_plan_create_batch",
-    "svo_render_batch_planned_device",
+    "svo_render_batch_planned_device", "svo_tile_plan_create_frame", "svo_render_frame_planned_device",
     "svo_build_stats_get",
 ]
 
@@ -315,6 +315,8 @@ def _load():
         "svo_serve_replay": [P, ctypes.c_int, P, P, ctypes.c_int, P, P, P, P, P, P],
         "svo_render_frame": [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P],
         "svo_render_frame_device": [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, P],
+        "svo_tile_plan_create_frame": [ctypes.c_int, ctypes.c_int, ctypes.c_int, P],
+        "svo_render_frame_planned_device": [P, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, P],
         "svo_render_frames_device": [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, P],
         "svo_frame_depth_count": [ctypes.c_int, ctypes.c_int, ctypes.c_int],
         "svo_build_from_terrain": [ctypes.c_int, ctypes.c_uint32, P, P],
@@ -629,9 +631,16 @@ class VoxelOctree:
                                     _p(rgba), _p(depth)))
         return rgba.reshape(height, width), depth
 
-    def render_frame_device(self, cam, width, height, rgba_ptr, depth_ptr=None, stripes=16, stride=1, stream=None):
-        _check(lib.svo_render_frame_device(self._h, ctypes.byref(cam), int(width), int(height), int(stripes),
-                                           int(stride), rgba_ptr, depth_ptr, stream))
+    def render_frame_device(self, cam, width, height, rgba_ptr, depth_ptr=None, stripes=16, stride=1, stream=None,
+                            plan=None):
+        """One viewer frame on the device; with `plan` (TilePlan.for_frame) the
+        fine pass runs in the cost order the previous frame through it filed."""
+        if plan is None:
+            _check(lib.svo_render_frame_device(self._h, ctypes.byref(cam), int(width), int(height), int(stripes),
+                                               int(stride), rgba_ptr, depth_ptr, stream))
+        else:
+            _check(lib.svo_render_frame_planned_device(self._h, plan._p, ctypes.byref(cam), int(width), int(height),
+                                                       int(stripes), int(stride), rgba_ptr, depth_ptr, stream))
 
     def render_frames_device(self, cams, width, height, rgba_ptrs, depth_ptrs=None, stripes=16, stride=1,
                              stream=None):
@@ -673,15 +682,23 @@ class TilePlan:
     """The tile cost order of one image size, or of one batch, on the device
     (svo_tile_plan): renders through it start the slowest tiles first."""
 
-    def __init__(self, width, height, _batch=None):
+    def __init__(self, width, height, _batch=None, _frame=None):
         self.width, self.height = int(width), int(height)
         self._p = ctypes.c_void_p()
-        if _batch is None:
+        if _frame is not None:
+            _check(lib.svo_tile_plan_create_frame(*_frame, ctypes.byref(self._p)))
+        elif _batch is None:
             _check(lib.svo_tile_plan_create(self.width, self.height, ctypes.byref(self._p)))
         else:
             views, opts = _batch
             arr = (View * len(views))(*views)
             _check(lib.svo_tile_plan_create_batch(arr, len(views), ctypes.byref(opts), ctypes.byref(self._p)))
+
+    @classmethod
+    def for_frame(cls, width, height, stripes=16):
+        """A plan for VoxelOctree.render_frame_device(..., plan=...) at this
+        frame size and stripe split."""
+        return cls(0, 0, _frame=(int(width), int(height), int(stripes)))
 
     @classmethod
     def for_batch(cls, views, tile_w=0):

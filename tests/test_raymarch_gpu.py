@@ -531,6 +531,45 @@ def test_frames_in_one_call_equal_single_frames(svo, trees):
     assert all(bool((x == 7).all()) for x in out)  # nothing was enqueued
 
 
+def test_planned_frames_equal_unplanned_frames(svo, trees):
+    """svo_render_frame_planned_device: a viewer's frames through one tile plan
+    (the first in raster order, the next ones with the fine pass in the cost
+    order the previous frame filed, over seven camera moves that take every
+    counter set and id region through filling, reading and clearing twice)
+    give RGBA and depth tables bit-equal to unplanned frames, at stride 1 and
+    3 and with a NULL depth table; a plan made for another size or stripe
+    split is refused before anything is enqueued."""
+    import torch
+    t = trees["dragon"]
+    w, h, stripes = 1280, 720, 16
+    nd = svo.svo.frame_depth_count(w, h, stripes)
+    stream = torch.cuda.Stream()
+    for stride in (1, 3):
+        plan = svo.svo.TilePlan.for_frame(w, h, stripes)
+        for k in range(7):
+            cam = t.orbit_camera(20.0 - 6.0 * k, 30.0 + 15.0 * k, 1.0)
+            rgba = torch.full((h * w,), 0x12345678, dtype=torch.int32, device="cuda")
+            depth = torch.full((nd,), -1.0, dtype=torch.float32, device="cuda")
+            dptr = depth.data_ptr() if k != 4 else None
+            print(f"[gpu] planned frame {k} stride {stride} {w}x{h}", flush=True)
+            t.render_frame_device(cam, w, h, rgba.data_ptr(), dptr, stripes, stride, stream.cuda_stream, plan=plan)
+            stream.synchronize()
+            ref_rgba, ref_depth = t.render_frame(cam, w, h, stripes, stride)
+            assert np.array_equal(rgba.cpu().numpy().view(np.uint32), ref_rgba.reshape(-1)), (stride, k)
+            if dptr is not None:
+                assert np.array_equal(depth.cpu().numpy().view(np.uint32), ref_depth.view(np.uint32)), (stride, k)
+    cam = t.orbit_camera(0.0, 0.0, 1.0)
+    out = torch.full((h * w,), 7, dtype=torch.int32, device="cuda")
+    for pw, ph, ps in ((w, h + 8, stripes), (w, h, 8)):
+        other = svo.svo.TilePlan.for_frame(pw, ph, ps)
+        print(f"[gpu] planned frame refused: plan {pw}x{ph}/{ps}", flush=True)
+        with pytest.raises(svo.SvoError) as e:
+            t.render_frame_device(cam, w, h, out.data_ptr(), None, stripes, 1, stream.cuda_stream, plan=other)
+        assert e.value.code == svo.svo.ERR_INVALID
+    stream.synchronize()
+    assert bool((out == 7).all())
+
+
 def test_cost_ordered_renders_match_plain_kernel(svo, trees):
     """svo_render_primary_planned_device: the first render of a plan (raster
     order), the next ones (tiles in the order the previous render filed them
