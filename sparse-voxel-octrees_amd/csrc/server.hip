@@ -145,6 +145,12 @@ __device__ __forceinline__ void st_dev(T *p, T v) { __hip_atomic_store(p, v, __A
 // s_waitcnt vmcnt(0): this wave's vector memory operations are performed
 __device__ __forceinline__ void wait_stores() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
+// Lane 0's value in a wave-uniform (scalar) register.
+__device__ __forceinline__ uint64_t lane0(uint64_t v) {
+    const uint32_t lo = __builtin_amdgcn_readlane(uint32_t(v), 0), hi = __builtin_amdgcn_readlane(uint32_t(v >> 32), 0);
+    return (uint64_t(hi) << 32) | lo;
+}
+
 // A request's view descriptor, read coherently: lane k < 32 reads word k,
 // every field is then taken from its lane into a wave-uniform register.
 __device__ __forceinline__ ViewDesc load_desc(const ViewDesc *p, int lane) {
@@ -242,13 +248,16 @@ __global__ void __launch_bounds__(256) serve_kernel(ServeCtrl *ctrl, const Serve
 
     // ---- tracing waves ----
     Stack<Idx> stk(lds + size_t(wave) * slots * 64 * (sizeof(Idx) + sizeof(float)), 64, slots, lane);
-    uint64_t known = 0;  // last published pair seen
-    uint32_t q = 0;      // launch-relative request of the last tile
+    uint64_t known = 0;          // last published pair seen
+    uint32_t q = 0;              // launch-relative request of the last tile
+    uint64_t qa = 0, qb = 0;     // its tiles [qa, qb); qb = 0 until the first tile
+    int s = 0;                   // its ring slot
+    ViewDesc vd{};               // and view, kept while the wave's tiles stay in it
     const int cls = int(blockIdx.x % kCursors);
+    unsigned long long g = 0;
+    if (lane == 0) g = atomicAdd(&st->cursor[cls].next, 1ull) * kCursors + uint64_t(cls);
+    g = lane0(g);
     for (;;) {
-        unsigned long long g = 0;
-        if (lane == 0) g = atomicAdd(&st->cursor[cls].next, 1ull) * kCursors + uint64_t(cls);
-        g = __shfl(g, 0);
         // hold the tile index until it is published (or the launch ends);
         // each poll is a device-scope load of one word by every waiting wave,
         // so a wave that keeps finding nothing polls less often (up to ~8 K
@@ -256,28 +265,29 @@ __global__ void __launch_bounds__(256) serve_kernel(ServeCtrl *ctrl, const Serve
         for (uint32_t naps = 0; g >= (known & kTileMask);) {
             uint64_t p = 0;
             if (lane == 0) p = ld_dev(&st->published);
-            known = __shfl(p, 0);
+            known = lane0(p);
             if (g < (known & kTileMask)) break;
             uint32_t stop = 0;
             if (lane == 0) stop = ld_dev(&st->stop);
-            if (__shfl(stop, 0)) return;
+            if (__builtin_amdgcn_readlane(stop, 0)) return;
             for (uint32_t k = 0; k < (1u << naps); ++k) __builtin_amdgcn_s_sleep(2);
             if (naps < 6) ++naps;
         }
-        // the request holding tile g: dbase[q] <= g < dbase[q + 1] (g only grows per wave)
-        const uint32_t count = uint32_t(known >> kCountShift);
-        if (ld_dev(&dbase[q + 1]) <= g) {
-            uint32_t lo = q + 1, hi = count;  // dbase[lo] <= g < dbase[hi]
+        if (g >= qb) {
+            // a later request: dbase[lo] <= g < dbase[hi] (g only grows per
+            // wave; the tables of published requests do not change)
+            uint32_t lo = qb ? q + 1 : 0, hi = uint32_t(known >> kCountShift);
             while (hi - lo > 1) {
                 const uint32_t mid = (lo + hi) >> 1;
                 if (ld_dev(&dbase[mid]) <= g) lo = mid; else hi = mid;
             }
             q = lo;
+            s = int((seq0 + q) % uint64_t(ring));
+            vd = load_desc(&dslots[s], lane);
+            qa = ld_dev(&dbase[q]);
+            qb = ld_dev(&dbase[q + 1]);
         }
-        const uint64_t seq = seq0 + q;
-        const int s = int(seq % uint64_t(ring));
-        const ViewDesc vd = load_desc(&dslots[s], lane);
-        const uint64_t local = g - ld_dev(&dbase[q]);
+        const uint64_t local = g - qa;
         const int tx = int(local % uint64_t(vd.tilesX));
         const int ty = int(local / uint64_t(vd.tilesX));
         const int x = tx * 8 + (lane & 7);
@@ -296,29 +306,32 @@ __global__ void __launch_bounds__(256) serve_kernel(ServeCtrl *ctrl, const Serve
             } while (code == kRunning);
             store_sys<Idx>(vd, int64_t(y) * vd.width + x, code, normal, t, leaf);
         }
-        // this tile's outputs performed, then counted in its class
+        // this tile's outputs performed, then counted in its class; the next
+        // claim travels with the count (one round trip for both)
         wait_stores();
         __builtin_amdgcn_wave_barrier();
+        unsigned long long next = 0;
         if (lane == 0) {
-            const uint64_t a = ld_dev(&dbase[q]), b = ld_dev(&dbase[q + 1]);  // the request's tiles
             uint32_t *cnt = done_word(done, s, cls);
             const uint32_t prior = __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (prior + 1 == class_tiles(a, b, cls)) {
+            next = atomicAdd(&st->cursor[cls].next, 1ull) * kCursors + uint64_t(cls);
+            if (prior + 1 == class_tiles(qa, qb, cls)) {
                 // the class's last tile: reset its counter for the slot's next
                 // request (performed before the class is counted), count the class
                 st_dev(cnt, 0u);
                 wait_stores();
                 uint32_t *all = done_word(done, s, kCursors);
-                const uint32_t classes = uint32_t(b - a < uint64_t(kCursors) ? b - a : uint64_t(kCursors));
+                const uint32_t classes = uint32_t(qb - qa < uint64_t(kCursors) ? qb - qa : uint64_t(kCursors));
                 if (__hip_atomic_fetch_add(all, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1 == classes) {
                     // the request's last class: every tile's outputs were
                     // performed before its count; reset, then publish completion
                     st_dev(all, 0u);
                     wait_stores();
-                    st_sys(&complete[s], seq + 1);
+                    st_sys(&complete[s], seq0 + q + 1);
                 }
             }
         }
+        g = lane0(next);
     }
 }
 
