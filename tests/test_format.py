@@ -405,6 +405,31 @@ def test_frame_depth_count_matches_oracle_and_refuses_empty_stripes(svo):
         assert e.value.code == svo.svo.ERR_INVALID
 
 
+def test_tile_plans_refuse_bad_arguments_before_device_use(svo, golden):
+    """svo_tile_plan_create / _create_frame / _create_batch check the image
+    size, the frame's stripe split and the batch's views and tile shape before
+    any device call; the planned render entries refuse a null plan first."""
+    m = svo.svo
+    for w, h in [(0, 8), (8, 0), (65537, 8)]:
+        with pytest.raises(svo.SvoError) as e:
+            m.TilePlan(w, h)
+        assert e.value.code == m.ERR_INVALID, (w, h)
+    for w, h, stripes in [(320, 100, 16), (0, 10, 1), (8, 8, 9)]:
+        with pytest.raises(svo.SvoError) as e:
+            m.TilePlan.for_frame(w, h, stripes)
+        assert e.value.code == m.ERR_INVALID, (w, h, stripes)
+    t = svo.VoxelOctree(os.path.join(golden, "dragon.oct"))
+    cam = t.orbit_camera(30.0, 10.0, 1.0)
+    for views, tile_w in [([], 0), ([m.Renderer.view(t, cam, 64, 64)], 5), ([m.Renderer.view(t, cam, 0, 64)], 0)]:
+        with pytest.raises(svo.SvoError) as e:
+            m.TilePlan.for_batch(views, tile_w)
+        assert e.value.code == m.ERR_INVALID, (len(views), tile_w)
+    assert m.lib.svo_render_primary_planned_device(t.handle, None, ctypes.byref(cam), 64, 64, None, None, None, None,
+                                                   None) == m.ERR_INVALID
+    assert m.lib.svo_render_frame_planned_device(t.handle, None, ctypes.byref(cam), 64, 64, 16, 1, None, None,
+                                                 None) == m.ERR_INVALID
+
+
 def test_frames_call_refuses_bad_arguments_before_device_use(svo, golden):
     """svo_render_frames_device checks the frame count, the layout, the stride,
     every image pointer and every camera before anything touches a device."""
