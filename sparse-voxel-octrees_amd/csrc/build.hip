@@ -1293,7 +1293,9 @@ __global__ void terrain_emit_kernel(int n, const int32_t *__restrict__ H, const 
 constexpr int kScanItems = 8;
 constexpr uint64_t kScanTile = uint64_t(kThreads) * kScanItems;
 
-// Exclusive sum of `v` over the 256 threads of the workgroup; *all = total.
+// Exclusive sum of `v` over the Threads threads of the workgroup (lds holds
+// Threads / 64 words); *all = total.
+template <int Threads = kThreads>
 __device__ __forceinline__ uint64_t block_exclusive(uint64_t v, uint64_t *lds, uint64_t *all) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     uint64_t x = v;
@@ -1306,7 +1308,7 @@ __device__ __forceinline__ uint64_t block_exclusive(uint64_t v, uint64_t *lds, u
     __syncthreads();
     uint64_t before = 0, total = 0;
 #pragma unroll
-    for (int w = 0; w < kThreads / 64; ++w) {
+    for (int w = 0; w < Threads / 64; ++w) {
         if (w < wave) before += lds[w];
         total += lds[w];
     }
@@ -1335,17 +1337,29 @@ __global__ void __launch_bounds__(kThreads) scan_sums_kernel(const In *__restric
     if (threadIdx.x == 0) sums[blockIdx.x] = total;
 }
 
-// One workgroup: exclusive scan of nb block sums in place; *total = their sum.
-__global__ void __launch_bounds__(kThreads) scan_block_sums_kernel(uint64_t *__restrict__ sums, uint64_t nb,
-                                                                   uint64_t *__restrict__ total) {
-    __shared__ uint64_t lds[kThreads / 64];
+// One workgroup of 1024 threads, 4 sums each per round: exclusive scan of nb
+// block sums in place; *total = their sum.
+constexpr int kSumThreads = 1024, kSumItems = 4;
+__global__ void __launch_bounds__(kSumThreads) scan_block_sums_kernel(uint64_t *__restrict__ sums, uint64_t nb,
+                                                                      uint64_t *__restrict__ total) {
+    __shared__ uint64_t lds[kSumThreads / 64];
     uint64_t carry = 0;
-    for (uint64_t base = 0; base < nb; base += kThreads) {
-        const uint64_t i = base + threadIdx.x;
-        const uint64_t v = i < nb ? sums[i] : 0;
+    for (uint64_t base = 0; base < nb; base += uint64_t(kSumThreads) * kSumItems) {
+        const uint64_t first = base + uint64_t(threadIdx.x) * kSumItems;
+        uint64_t v[kSumItems];
+        uint64_t sum = 0;
+#pragma unroll
+        for (int k = 0; k < kSumItems; ++k) {
+            v[k] = first + k < nb ? sums[first + k] : 0;
+            sum += v[k];
+        }
         uint64_t all;
-        const uint64_t ex = block_exclusive(v, lds, &all);
-        if (i < nb) sums[i] = carry + ex;
+        uint64_t run = carry + block_exclusive<kSumThreads>(sum, lds, &all);
+#pragma unroll
+        for (int k = 0; k < kSumItems; ++k) {
+            if (first + k < nb) sums[first + k] = run;
+            run += v[k];
+        }
         carry += all;
     }
     if (threadIdx.x == 0) *total = carry;
@@ -1410,7 +1424,7 @@ int exclusive_scan_dev(Arena &ar, const In *in, uint64_t *out, const uint64_t *n
     if (r) return r;
     if (!tot && (r = ar.alloc(tot, 1))) return r;
     hipLaunchKernelGGL(scan_sums_kernel<In>, dim3(unsigned(nb)), dim3(kThreads), 0, s, in, nptr, sums);
-    hipLaunchKernelGGL(scan_block_sums_kernel, dim3(1), dim3(kThreads), 0, s, sums, nb, tot);
+    hipLaunchKernelGGL(scan_block_sums_kernel, dim3(1), dim3(kSumThreads), 0, s, sums, nb, tot);
     hipLaunchKernelGGL(scan_down_kernel<In>, dim3(unsigned(nb)), dim3(kThreads), 0, s, in, nptr, sums, out);
     HIP_TRY(hipGetLastError());
     ar.release(sums);
