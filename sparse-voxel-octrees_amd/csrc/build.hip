@@ -642,15 +642,26 @@ __device__ __forceinline__ uint64_t find_pair(const uint64_t *__restrict__ rowSt
     return lo;
 }
 
+// A row's voxels for the emit pass, recorded by the count pass: the first
+// voxel's x, the row's y and z, the triangle, and which of the first 64
+// voxels of the row overlap it (rows longer than 64 voxels are marked
+// x0 = kLongRow and tested again by the emit pass).
+struct RowHits {
+    uint64_t mask;
+    uint32_t x0, y, z, t;
+};
+constexpr uint32_t kLongRow = 0xFFFFFFFFu;
+
 // The voxel loop of PlyLoader::triangleToVolume (src/PlyLoader.cpp:353-373)
 // for one (y, z) row: centres start at (l - 0.5)*hx and step by +hx, exactly as
-// the reference accumulates them. Count pass (keys == nullptr) or emit pass.
-__global__ void row_voxels_kernel(const uint32_t *__restrict__ cells, const uint32_t *__restrict__ tris, uint64_t P,
-                                  const uint64_t *__restrict__ rowStart, uint64_t R, const float *__restrict__ pos,
-                                  MeshParams mp, int tribits, uint32_t *__restrict__ hitCount,
-                                  const uint64_t *__restrict__ hitStart, uint64_t *__restrict__ keys) {
-    uint64_t r = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (r >= R) return;
+// the reference accumulates them. Count pass (keys == nullptr: the row's hit
+// count and its RowHits) or emit pass (keys at hitStart).
+__device__ __forceinline__ void row_voxels(uint64_t r, const uint32_t *__restrict__ cells,
+                                           const uint32_t *__restrict__ tris, uint64_t P,
+                                           const uint64_t *__restrict__ rowStart, const float *__restrict__ pos,
+                                           const MeshParams &mp, int tribits, uint32_t *__restrict__ hitCount,
+                                           RowHits *__restrict__ hits, const uint64_t *__restrict__ hitStart,
+                                           uint64_t *__restrict__ keys) {
     const uint64_t i = find_pair(rowStart, P, r);
     const uint32_t t = tris[i];
     float tv[9];
@@ -670,14 +681,50 @@ __global__ void row_voxels_kernel(const uint32_t *__restrict__ cells, const uint
     for (int k = l[1]; k < y; ++k) ctr[1] += hx;
     ctr[0] = (float(l[0]) - 0.5f) * hx;
     uint32_t n = 0;
+    uint64_t mask = 0;
     uint64_t at = keys ? hitStart[r] : 0;
     for (int x = l[0]; x <= u[0]; ++x, ctr[0] += hx) {
         if (tri_box(ctr, half, tv)) {
             if (keys) keys[at + n] = (morton3(uint32_t(x), uint32_t(y), uint32_t(z)) << tribits) | uint64_t(t);
+            else if (x - l[0] < 64) mask |= uint64_t(1) << (x - l[0]);
             n++;
         }
     }
-    if (!keys) hitCount[r] = n;
+    if (!keys) {
+        hitCount[r] = n;
+        const bool longRow = u[0] - l[0] >= 64;
+        hits[r] = RowHits{mask, longRow ? kLongRow : uint32_t(l[0]), uint32_t(y), uint32_t(z), t};
+    }
+}
+
+__global__ void row_count_kernel(const uint32_t *__restrict__ cells, const uint32_t *__restrict__ tris, uint64_t P,
+                                 const uint64_t *__restrict__ rowStart, uint64_t R, const float *__restrict__ pos,
+                                 MeshParams mp, int tribits, uint32_t *__restrict__ hitCount,
+                                 RowHits *__restrict__ hits) {
+    const uint64_t r = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r >= R) return;
+    row_voxels(r, cells, tris, P, rowStart, pos, mp, tribits, hitCount, hits, nullptr, nullptr);
+}
+
+// Emit pass: a row of at most 64 voxels writes its keys from the count pass's
+// record (the same keys, in the same ascending x order); a longer row runs
+// the voxel loop again.
+__global__ void row_emit_kernel(const uint32_t *__restrict__ cells, const uint32_t *__restrict__ tris, uint64_t P,
+                                const uint64_t *__restrict__ rowStart, uint64_t R, const float *__restrict__ pos,
+                                MeshParams mp, int tribits, const RowHits *__restrict__ hits,
+                                const uint64_t *__restrict__ hitStart, uint64_t *__restrict__ keys) {
+    const uint64_t r = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r >= R) return;
+    const RowHits h = hits[r];
+    if (h.x0 == kLongRow) {
+        row_voxels(r, cells, tris, P, rowStart, pos, mp, tribits, nullptr, nullptr, hitStart, keys);
+        return;
+    }
+    uint64_t at = hitStart[r];
+    for (uint64_t m = h.mask; m; m &= m - 1) {
+        const uint32_t x = h.x0 + uint32_t(__builtin_ctzll(m));
+        keys[at++] = (morton3(x, h.y, h.z) << tribits) | uint64_t(h.t);
+    }
 }
 
 __global__ void seg_heads_kernel(const uint64_t *__restrict__ keys, uint64_t K, int shift, uint32_t *__restrict__ head) {
@@ -914,19 +961,26 @@ __global__ void parent_heads_kernel(const uint64_t *__restrict__ child, const ui
     head[i] = (i == 0 || (child[i] >> 3) != (child[i - 1] >> 3)) ? 1u : 0u;
 }
 
-// parent key / child start / mask from sorted child keys (no extra parents)
+// parent key / child start / mask from sorted child keys (no extra parents):
+// the first child of each parent writes all three; its siblings (at most 7,
+// keys are unique) follow it in the sorted keys
 __global__ void parents_kernel(const uint64_t *__restrict__ child, const uint64_t *__restrict__ nptr,
-                               const uint32_t *__restrict__ head, const uint64_t *__restrict__ incl,
+                               const uint32_t *__restrict__ head, const uint64_t *__restrict__ ex,
                                uint64_t *__restrict__ pkey, uint64_t *__restrict__ pstart, uint32_t *__restrict__ pmask) {
-    uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= *nptr) return;
-    const uint64_t p = incl[i];  // exclusive scan of heads: parent index of child i is p + head - 1
-    const uint64_t pi = p + head[i] - 1;
-    if (head[i]) {
-        pkey[pi] = child[i] >> 3;
-        pstart[pi] = i;
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const uint64_t n = *nptr;
+    if (i >= n || !head[i]) return;
+    const uint64_t pi = ex[i];  // exclusive scan of heads: the parent's index
+    const uint64_t parent = child[i] >> 3;
+    uint32_t mask = 1u << uint32_t(child[i] & 7);
+    for (uint64_t j = i + 1; j < n && j < i + 8; ++j) {
+        const uint64_t k = child[j];
+        if ((k >> 3) != parent) break;
+        mask |= 1u << uint32_t(k & 7);
     }
-    atomicOr(&pmask[pi], 1u << uint32_t(child[i] & 7));
+    pkey[pi] = parent;
+    pstart[pi] = i;
+    pmask[pi] = mask;
 }
 
 // Level that also takes the cache blocks the occupancy table marks: parents
@@ -1662,7 +1716,7 @@ int emit_tree(Arena &ar, const Geometry &g, uint64_t *leafKey, uint32_t *leafWor
             if ((r = exclusive_scan_dev(ar, head, ex, c.cnt, c.bound, p.cnt, st))) return r;
             if ((r = ar.alloc(p.key, p.bound)) || (r = ar.alloc(p.start, p.bound)) || (r = ar.alloc(p.mask, p.bound)))
                 return r;
-            if ((r = fill32(p.mask, p.bound, 0, st))) return r;
+            // every parent below the count is written once (by its first child)
             hipLaunchKernelGGL(parents_kernel, dim3(grid_for(c.bound)), dim3(kThreads), 0, st, c.key, c.cnt, head, ex,
                                p.key, p.start, p.mask);
             ar.release(head);
@@ -2162,24 +2216,27 @@ static int mesh_core(Arena &ar, float *dv, int64_t nverts, int has_normals, int3
     // pass writes the keys at the rows' exclusive offsets (a single pass that
     // reserved space per wave with device-scope atomics on one counter
     // measured 5.7 ms against 1.8 ms for the pair: the ~300 K reservations
-    // serialise)
+    // serialise); the count pass also records each row's hits, so the emit
+    // pass repeats the SAT only for rows longer than 64 voxels
     uint32_t *hitCount;
     uint64_t *hitStart;
-    if ((r = ar.alloc(hitCount, R)) || (r = ar.alloc(hitStart, R))) return r;
+    RowHits *hits;
+    if ((r = ar.alloc(hitCount, R)) || (r = ar.alloc(hitStart, R)) || (r = ar.alloc(hits, R))) return r;
     if (R)
-        hipLaunchKernelGGL(row_voxels_kernel, dim3(grid_for(R)), dim3(kThreads), 0, st, cells, ptri, P, rowStart, R, pos,
-                           mp, tribits, hitCount, nullptr, nullptr);
+        hipLaunchKernelGGL(row_count_kernel, dim3(grid_for(R)), dim3(kThreads), 0, st, cells, ptri, P, rowStart, R, pos,
+                           mp, tribits, hitCount, hits);
     HIP_TRY(hipGetLastError());
     uint64_t K;
     if ((r = scan_count(ar, hitCount, hitStart, R, st, K))) return r;
     uint64_t *keys, *keys2;
     if ((r = ar.alloc(keys, K)) || (r = ar.alloc(keys2, K))) return r;
     if (R)
-        hipLaunchKernelGGL(row_voxels_kernel, dim3(grid_for(R)), dim3(kThreads), 0, st, cells, ptri, P, rowStart, R, pos,
-                           mp, tribits, hitCount, hitStart, keys);
+        hipLaunchKernelGGL(row_emit_kernel, dim3(grid_for(R)), dim3(kThreads), 0, st, cells, ptri, P, rowStart, R, pos,
+                           mp, tribits, hits, hitStart, keys);
     HIP_TRY(hipGetLastError());
     ar.release(hitCount);
     ar.release(hitStart);
+    ar.release(hits);
     out.build.triangles = uint64_t(T);
     out.build.candidates = K;
     ar.release(cells);
