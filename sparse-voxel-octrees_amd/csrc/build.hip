@@ -1371,6 +1371,24 @@ __device__ __forceinline__ uint64_t block_exclusive(uint64_t v, uint64_t *lds, u
     return before + x - v;
 }
 
+// Thread items [first, first + kScanItems) of in[0, n) (0 past n); a full
+// run of 32-bit items is two 16-byte loads (the arrays are 16-byte aligned).
+template <typename In>
+__device__ __forceinline__ void load_items(const In *__restrict__ in, uint64_t first, uint64_t n,
+                                           uint64_t v[kScanItems]) {
+    if constexpr (sizeof(In) == 4 && kScanItems == 8) {
+        if (first + kScanItems <= n) {
+            const uint4 a = *reinterpret_cast<const uint4 *>(in + first);
+            const uint4 b = *reinterpret_cast<const uint4 *>(in + first + 4);
+            v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+            v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+            return;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) v[k] = first + k < n ? uint64_t(in[first + k]) : 0;
+}
+
 template <typename In>
 __global__ void __launch_bounds__(kThreads) scan_sums_kernel(const In *__restrict__ in, const uint64_t *__restrict__ nptr,
                                                              uint64_t *__restrict__ sums) {
@@ -1381,13 +1399,13 @@ __global__ void __launch_bounds__(kThreads) scan_sums_kernel(const In *__restric
         if (threadIdx.x == 0) sums[blockIdx.x] = 0;
         return;
     }
-    uint64_t v = 0;
-    const uint64_t first = base + uint64_t(threadIdx.x) * kScanItems;
+    uint64_t v[kScanItems];
+    load_items(in, base + uint64_t(threadIdx.x) * kScanItems, n, v);
+    uint64_t sum = 0;
 #pragma unroll
-    for (int k = 0; k < kScanItems; ++k)
-        if (first + k < n) v += uint64_t(in[first + k]);
+    for (int k = 0; k < kScanItems; ++k) sum += v[k];
     uint64_t total;
-    block_exclusive(v, lds, &total);
+    block_exclusive(sum, lds, &total);
     if (threadIdx.x == 0) sums[blockIdx.x] = total;
 }
 
@@ -1428,14 +1446,25 @@ __global__ void __launch_bounds__(kThreads) scan_down_kernel(const In *__restric
     if (base >= n) return;
     const uint64_t first = base + uint64_t(threadIdx.x) * kScanItems;
     uint64_t v[kScanItems];
+    load_items(in, first, n, v);
     uint64_t sum = 0;
 #pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        v[k] = first + k < n ? uint64_t(in[first + k]) : 0;
-        sum += v[k];
-    }
+    for (int k = 0; k < kScanItems; ++k) sum += v[k];
     uint64_t total;
     uint64_t run = offs[blockIdx.x] + block_exclusive(sum, lds, &total);
+    if (first + kScanItems <= n) {
+        // a full run: four 16-byte stores
+        uint64_t o[kScanItems];
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            o[k] = run;
+            run += v[k];
+        }
+#pragma unroll
+        for (int k = 0; k < kScanItems; k += 2)
+            *reinterpret_cast<ulonglong2 *>(out + first + k) = make_ulonglong2(o[k], o[k + 1]);
+        return;
+    }
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
         if (first + k < n) out[first + k] = run;
@@ -1473,6 +1502,8 @@ int exclusive_scan_dev(Arena &ar, const In *in, uint64_t *out, const uint64_t *n
                        hipStream_t s) {
     const uint64_t nb = std::max<uint64_t>(1, (bound + kScanTile - 1) / kScanTile);
     if (nb > 0x7FFFFFFFull) return fail(SVO_ERR_LIMIT, "builder: scan too large");
+    if (reinterpret_cast<uintptr_t>(in) % 16 || reinterpret_cast<uintptr_t>(out) % 16)
+        return fail(SVO_ERR_DEVICE, "builder: scan arrays must be 16-byte aligned (internal error)");
     uint64_t *sums, *tot = total;
     int r = ar.alloc(sums, nb);
     if (r) return r;
