@@ -17,7 +17,8 @@
 //    (the wave polls the published pair), so a request that arrives while
 //    the kernel is tracing others is traced by the next waves that free up;
 //  - a wave traces its 8x8 tile (march.hpp, the reference's raymarch per
-//    lane: src/VoxelOctree.cpp:207-346), releases its outputs at system scope
+//    lane: src/VoxelOctree.cpp:207-346), stores its outputs coherently (agent
+//    scope in device memory, system scope in host memory), waits for them
 //    and counts the tile done; the wave that completes a request's last tile
 //    writes the request's sequence number into a completion word in host
 //    memory.
@@ -164,15 +165,23 @@ __device__ __forceinline__ ViewDesc load_desc(const ViewDesc *p, int lane) {
     return d;
 }
 
-// One ray's outputs as system-scope stores (performed at memory, visible to
-// the host and to later kernels once the wave has waited for them). Requests
-// carry no hit array.
+// One ray's outputs as scoped stores, visible once the wave has waited for
+// them: system scope for outputs in host memory (read by the host), agent
+// scope for outputs in device memory (read by later kernels and copies, which
+// start after the host has seen the request complete). Requests carry no hit
+// array.
+template <typename T>
+__device__ __forceinline__ void st_out(T *p, T v, bool host) {
+    if (host) st_sys(p, v);
+    else st_dev(p, v);
+}
 template <typename Idx>
-__device__ __forceinline__ void store_sys(const ViewDesc &vd, int64_t i, int code, uint32_t normal, float t, Idx leaf) {
-    if (vd.normal) st_sys(vd.normal + i, code == kLeaf ? normal : 0u);
-    if (vd.t) st_sys(reinterpret_cast<uint32_t *>(vd.t) + i, code == kMiss ? 0u : __float_as_uint(t));
-    if (vd.leaf) st_sys(reinterpret_cast<unsigned long long *>(vd.leaf) + i,
-                        (unsigned long long)(code == kLeaf ? uint64_t(leaf) : SVO_LEAF_NONE));
+__device__ __forceinline__ void store_out(const ViewDesc &vd, int64_t i, int code, uint32_t normal, float t, Idx leaf,
+                                          bool host) {
+    if (vd.normal) st_out(vd.normal + i, code == kLeaf ? normal : 0u, host);
+    if (vd.t) st_out(reinterpret_cast<uint32_t *>(vd.t) + i, code == kMiss ? 0u : __float_as_uint(t), host);
+    if (vd.leaf) st_out(reinterpret_cast<unsigned long long *>(vd.leaf) + i,
+                        (unsigned long long)(code == kLeaf ? uint64_t(leaf) : SVO_LEAF_NONE), host);
 }
 
 // Persistent serving kernel. Workgroups of 256 threads (4 waves); wave 0 of
@@ -181,7 +190,7 @@ template <typename Idx, int Reg>
 __global__ void __launch_bounds__(256) serve_kernel(ServeCtrl *ctrl, const ServeSlot *hslots, ServeState *st,
                                                     ViewDesc *dslots, uint64_t *dbase,
                                                     uint32_t *done, uint64_t *complete, int ring, uint64_t seq0,
-                                                    uint32_t maxReqs, int slots) {
+                                                    uint32_t maxReqs, int slots, int hostOut) {
     extern __shared__ __align__(16) unsigned char lds[];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -304,7 +313,7 @@ __global__ void __launch_bounds__(256) serve_kernel(ServeCtrl *ctrl, const Serve
             do {
                 code = ray_step<false, true, Idx, true, Reg>(r, vd.tree, stk, normal, t, leaf);
             } while (code == kRunning);
-            store_sys<Idx>(vd, int64_t(y) * vd.width + x, code, normal, t, leaf);
+            store_out<Idx>(vd, int64_t(y) * vd.width + x, code, normal, t, leaf, hostOut != 0);
         }
         // this tile's outputs performed, then counted in its class; the next
         // claim travels with the count (one round trip for both)
@@ -448,16 +457,16 @@ bool start_launch(svo_server &s) {
     const dim3 grid(unsigned(s.cus * blocksPerCU)), block(256);
     if (!s.wide && s.reg == 1)
         hipLaunchKernelGGL((serve_kernel<uint32_t, 1>), grid, block, lds, s.stream, s.ctrl, s.hslots, s.st, s.dslots,
-                           s.dbase, s.done, s.complete, s.ring, s.seq0, s.maxReqs, s.stackSlots);
+                           s.dbase, s.done, s.complete, s.ring, s.seq0, s.maxReqs, s.stackSlots, int(s.hostOutputs));
     else if (!s.wide)
         hipLaunchKernelGGL((serve_kernel<uint32_t, 3>), grid, block, lds, s.stream, s.ctrl, s.hslots, s.st, s.dslots,
-                           s.dbase, s.done, s.complete, s.ring, s.seq0, s.maxReqs, s.stackSlots);
+                           s.dbase, s.done, s.complete, s.ring, s.seq0, s.maxReqs, s.stackSlots, int(s.hostOutputs));
     else if (s.reg == 1)
         hipLaunchKernelGGL((serve_kernel<uint64_t, 1>), grid, block, lds, s.stream, s.ctrl, s.hslots, s.st, s.dslots,
-                           s.dbase, s.done, s.complete, s.ring, s.seq0, s.maxReqs, s.stackSlots);
+                           s.dbase, s.done, s.complete, s.ring, s.seq0, s.maxReqs, s.stackSlots, int(s.hostOutputs));
     else
         hipLaunchKernelGGL((serve_kernel<uint64_t, 3>), grid, block, lds, s.stream, s.ctrl, s.hslots, s.st, s.dslots,
-                           s.dbase, s.done, s.complete, s.ring, s.seq0, s.maxReqs, s.stackSlots);
+                           s.dbase, s.done, s.complete, s.ring, s.seq0, s.maxReqs, s.stackSlots, int(s.hostOutputs));
     e = hipGetLastError();
     if (e == hipSuccess) e = hipEventRecord(s.ended, s.stream);
     if (e != hipSuccess) {
