@@ -745,16 +745,12 @@ __global__ void seg_starts_kernel(const uint32_t *__restrict__ head, const uint6
 // (word, count) the voxel's cell holds when its block is processed (0 for a
 // fresh cell). The voxel centre each triangle used is rebuilt with the same
 // +hx steps from that triangle's clipped start.
-__device__ __forceinline__ void fold_voxel(const uint64_t *__restrict__ keys, uint64_t b, uint64_t e, int tribits,
-                                           const float *__restrict__ pos, const float *__restrict__ nrm,
-                                           const float *__restrict__ col, const MeshParams &mp, uint32_t &word,
-                                           uint32_t &count) {
-    const uint64_t m = keys[b] >> tribits;
-    const int vx[3] = {int(compact3(m)), int(compact3(m >> 1)), int(compact3(m >> 2))};
-    const uint64_t triMask = (uint64_t(1) << tribits) - 1;
+// One triangle t of voxel vx folded into (word, count).
+__device__ __forceinline__ void fold_step(int64_t t, const int vx[3], const float *__restrict__ pos,
+                                          const float *__restrict__ nrm, const float *__restrict__ col,
+                                          const MeshParams &mp, uint32_t &word, uint32_t &count) {
     const float hx = mp.hx;
-    for (uint64_t k = b; k < e; ++k) {
-        const int64_t t = int64_t(keys[k] & triMask);
+    {
         float tv[9];
         load_tri(pos, t, tv);
         int l[3], u[3];
@@ -826,17 +822,67 @@ __device__ __forceinline__ void fold_voxel(const uint64_t *__restrict__ keys, ui
     }
 }
 
-__global__ void fold_kernel(const uint64_t *__restrict__ keys, const uint64_t *__restrict__ segStart, uint64_t S,
+__device__ __forceinline__ void fold_voxel(const uint64_t *__restrict__ keys, uint64_t b, uint64_t e, int tribits,
+                                           const float *__restrict__ pos, const float *__restrict__ nrm,
+                                           const float *__restrict__ col, const MeshParams &mp, uint32_t &word,
+                                           uint32_t &count) {
+    const uint64_t m = keys[b] >> tribits;
+    const int vx[3] = {int(compact3(m)), int(compact3(m >> 1)), int(compact3(m >> 2))};
+    const uint64_t triMask = (uint64_t(1) << tribits) - 1;
+    for (uint64_t k = b; k < e; ++k) fold_step(int64_t(keys[k] & triMask), vx, pos, nrm, col, mp, word, count);
+}
+
+// Voxels per wave in fold_kernel: a lane that has folded its voxel takes the
+// chunk's next one, so a wave's lanes stay busy while their voxels' triangle
+// counts differ (one lane per voxel would run every lane as long as the
+// voxel with the most triangles).
+constexpr int kFoldChunk = 512;
+constexpr uint64_t kFoldVoxelsPerBlock = uint64_t(kFoldChunk) * (kThreads / 64);
+
+__global__ void __launch_bounds__(kThreads) fold_kernel(const uint64_t *__restrict__ keys, const uint64_t *__restrict__ segStart, uint64_t S,
                             int tribits, const float *__restrict__ pos, const float *__restrict__ nrm,
                             const float *__restrict__ col, MeshParams mp, uint64_t *__restrict__ vkey,
                             uint32_t *__restrict__ vword, uint8_t *__restrict__ vcount) {
-    uint64_t s = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (s >= S) return;
+    const int lane = threadIdx.x & 63;
+    const uint64_t base = (uint64_t(blockIdx.x) * (kThreads / 64) + (threadIdx.x >> 6)) * kFoldChunk;
+    if (base >= S) return;
+    const uint64_t end = min(base + uint64_t(kFoldChunk), S);
+    const uint64_t triMask = (uint64_t(1) << tribits) - 1;
+    uint64_t next = base;  // the chunk's next unassigned voxel (wave-uniform)
+    uint64_t v = 0, k = 0, e = 0;
+    bool busy = false;
+    int vx[3] = {0, 0, 0};
     uint32_t word = 0, count = 0;
-    fold_voxel(keys, segStart[s], segStart[s + 1], tribits, pos, nrm, col, mp, word, count);
-    vkey[s] = keys[segStart[s]] >> tribits;
-    vword[s] = word;
-    vcount[s] = uint8_t(count);
+    for (;;) {
+        // lanes without a voxel take the next ones, in lane order
+        const uint64_t idle = __ballot(!busy);
+        if (!busy) {
+            const uint64_t w = next + uint64_t(__popcll(idle & ((uint64_t(1) << lane) - 1)));
+            if (w < end) {
+                v = w;
+                k = segStart[w];
+                e = segStart[w + 1];
+                const uint64_t m = keys[k] >> tribits;
+                vx[0] = int(compact3(m));
+                vx[1] = int(compact3(m >> 1));
+                vx[2] = int(compact3(m >> 2));
+                word = 0;
+                count = 0;
+                busy = true;
+            }
+        }
+        next = min(next + uint64_t(__popcll(idle)), end);
+        if (!__ballot(busy)) break;
+        if (busy) {
+            fold_step(int64_t(keys[k] & triMask), vx, pos, nrm, col, mp, word, count);
+            if (++k == e) {
+                vkey[v] = keys[e - 1] >> tribits;
+                vword[v] = word;
+                vcount[v] = uint8_t(count);
+                busy = false;
+            }
+        }
+    }
 }
 
 // One slab of PlyLoader::convertToVolume (src/PlyLoader.cpp:521-530): every
@@ -2306,7 +2352,8 @@ static int mesh_core(Arena &ar, float *dv, int64_t nverts, int has_normals, int3
     }
     if ((r = ar.alloc(vkey, V)) || (r = ar.alloc(vword, V)) || (r = ar.alloc(vcount, V))) return r;
     if (V)
-        hipLaunchKernelGGL(fold_kernel, dim3(grid_for(V)), dim3(kThreads), 0, st, keys2, segStart, V, tribits, pos, nrm,
+        hipLaunchKernelGGL(fold_kernel, dim3(unsigned((V + kFoldVoxelsPerBlock - 1) / kFoldVoxelsPerBlock)), dim3(kThreads),
+                           0, st, keys2, segStart, V, tribits, pos, nrm,
                            col, mp, vkey, vword, vcount);
     HIP_TRY(hipGetLastError());
     ar.release(head);
