@@ -606,14 +606,39 @@ __device__ __forceinline__ bool pair_box(uint32_t f, const float tv[9], const Me
     return true;
 }
 
+// A pair's clipped voxel box, kept for its rows.
+struct PairBox {
+    int l[3], u[3];
+};
+
 __global__ void pair_rows_kernel(const uint32_t *__restrict__ cells, const uint32_t *__restrict__ tris, uint64_t P,
-                                 const float *__restrict__ pos, MeshParams mp, uint32_t *__restrict__ rows) {
+                                 const float *__restrict__ pos, MeshParams mp, uint32_t *__restrict__ rows,
+                                 PairBox *__restrict__ boxes) {
     uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= P) return;
     float tv[9];
     load_tri(pos, tris[i], tv);
-    int l[3], u[3];
-    rows[i] = pair_box(cells[i], tv, mp, l, u) ? uint32_t((u[1] - l[1] + 1) * (u[2] - l[2] + 1)) : 0u;
+    PairBox b;
+    const bool any = pair_box(cells[i], tv, mp, b.l, b.u);
+    rows[i] = any ? uint32_t((b.u[1] - b.l[1] + 1) * (b.u[2] - b.l[2] + 1)) : 0u;
+    if (any) boxes[i] = b;
+}
+
+// rowPair[r] = the pair of row r (each pair's rows are contiguous from
+// rowStart[i]). A wave takes 64 pairs and writes their rows one pair after
+// another with all lanes (coalesced stores, no lane waiting on a long pair).
+__global__ void __launch_bounds__(kThreads) row_pairs_kernel(const uint64_t *__restrict__ rowStart, uint64_t P,
+                                                             uint32_t *__restrict__ rowPair) {
+    const int lane = threadIdx.x & 63;
+    const uint64_t base = (uint64_t(blockIdx.x) * (kThreads / 64) + (threadIdx.x >> 6)) * 64;
+    if (base >= P) return;
+    const uint64_t i = base + uint64_t(lane);
+    const uint64_t start = i < P ? rowStart[i] : 0, end = i < P ? rowStart[i + 1] : 0;
+    const int n = int(min(uint64_t(64), P - base));
+    for (int j = 0; j < n; ++j) {
+        const uint64_t s = __shfl(start, j), e = __shfl(end, j);
+        for (uint64_t r = s + uint64_t(lane); r < e; r += 64) rowPair[r] = uint32_t(base + uint64_t(j));
+    }
 }
 
 // Cache blocks with a non-empty cell list (PlyLoader::isBlockEmpty,
@@ -632,16 +657,6 @@ __global__ void block_flags_kernel(const uint32_t *__restrict__ cells, uint64_t 
     flags[morton3(b[0], b[1], b[2])] = 1u;
 }
 
-// Row r of the pair found by binary search over the exclusive row offsets.
-__device__ __forceinline__ uint64_t find_pair(const uint64_t *__restrict__ rowStart, uint64_t P, uint64_t r) {
-    uint64_t lo = 0, hi = P;  // rowStart[lo] <= r < rowStart[hi]
-    while (hi - lo > 1) {
-        const uint64_t mid = (lo + hi) >> 1;
-        if (rowStart[mid] <= r) lo = mid; else hi = mid;
-    }
-    return lo;
-}
-
 // A row's voxels for the emit pass, recorded by the count pass: the first
 // voxel's x, the row's y and z, the triangle, and which of the first 64
 // voxels of the row overlap it (rows longer than 64 voxels are marked
@@ -656,22 +671,22 @@ constexpr uint32_t kLongRow = 0xFFFFFFFFu;
 // for one (y, z) row: centres start at (l - 0.5)*hx and step by +hx, exactly as
 // the reference accumulates them. Count pass (keys == nullptr: the row's hit
 // count and its RowHits) or emit pass (keys at hitStart).
-__device__ __forceinline__ void row_voxels(uint64_t r, const uint32_t *__restrict__ cells,
-                                           const uint32_t *__restrict__ tris, uint64_t P,
+__device__ __forceinline__ void row_voxels(uint64_t r, const uint32_t *__restrict__ rowPair,
+                                           const PairBox *__restrict__ boxes, const uint32_t *__restrict__ tris,
                                            const uint64_t *__restrict__ rowStart, const float *__restrict__ pos,
                                            const MeshParams &mp, int tribits, uint32_t *__restrict__ hitCount,
                                            RowHits *__restrict__ hits, const uint64_t *__restrict__ hitStart,
                                            uint64_t *__restrict__ keys) {
-    const uint64_t i = find_pair(rowStart, P, r);
+    const uint32_t i = rowPair[r];
     const uint32_t t = tris[i];
     float tv[9];
     load_tri(pos, t, tv);
-    int l[3], u[3];
-    pair_box(cells[i], tv, mp, l, u);
-    const uint64_t local = r - rowStart[i];
-    const int ny = u[1] - l[1] + 1;
-    const int y = l[1] + int(local % uint64_t(ny));
-    const int z = l[2] + int(local / uint64_t(ny));
+    const PairBox b = boxes[i];
+    const int *l = b.l, *u = b.u;
+    const uint32_t local = uint32_t(r - rowStart[i]);  // below the pair's row count (32 bits)
+    const uint32_t ny = uint32_t(u[1] - l[1] + 1);
+    const int y = l[1] + int(local % ny);
+    const int z = l[2] + int(local / ny);
     const float hx = mp.hx;
     const float half[3] = {0.5f * hx, 0.5f * hx, 0.5f * hx};
     float ctr[3];
@@ -697,27 +712,28 @@ __device__ __forceinline__ void row_voxels(uint64_t r, const uint32_t *__restric
     }
 }
 
-__global__ void row_count_kernel(const uint32_t *__restrict__ cells, const uint32_t *__restrict__ tris, uint64_t P,
-                                 const uint64_t *__restrict__ rowStart, uint64_t R, const float *__restrict__ pos,
-                                 MeshParams mp, int tribits, uint32_t *__restrict__ hitCount,
-                                 RowHits *__restrict__ hits) {
+__global__ void row_count_kernel(const uint32_t *__restrict__ rowPair, const PairBox *__restrict__ boxes,
+                                 const uint32_t *__restrict__ tris, const uint64_t *__restrict__ rowStart, uint64_t R,
+                                 const float *__restrict__ pos, MeshParams mp, int tribits,
+                                 uint32_t *__restrict__ hitCount, RowHits *__restrict__ hits) {
     const uint64_t r = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (r >= R) return;
-    row_voxels(r, cells, tris, P, rowStart, pos, mp, tribits, hitCount, hits, nullptr, nullptr);
+    row_voxels(r, rowPair, boxes, tris, rowStart, pos, mp, tribits, hitCount, hits, nullptr, nullptr);
 }
 
 // Emit pass: a row of at most 64 voxels writes its keys from the count pass's
 // record (the same keys, in the same ascending x order); a longer row runs
 // the voxel loop again.
-__global__ void row_emit_kernel(const uint32_t *__restrict__ cells, const uint32_t *__restrict__ tris, uint64_t P,
-                                const uint64_t *__restrict__ rowStart, uint64_t R, const float *__restrict__ pos,
-                                MeshParams mp, int tribits, const RowHits *__restrict__ hits,
-                                const uint64_t *__restrict__ hitStart, uint64_t *__restrict__ keys) {
+__global__ void row_emit_kernel(const uint32_t *__restrict__ rowPair, const PairBox *__restrict__ boxes,
+                                const uint32_t *__restrict__ tris, const uint64_t *__restrict__ rowStart, uint64_t R,
+                                const float *__restrict__ pos, MeshParams mp, int tribits,
+                                const RowHits *__restrict__ hits, const uint64_t *__restrict__ hitStart,
+                                uint64_t *__restrict__ keys) {
     const uint64_t r = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (r >= R) return;
     const RowHits h = hits[r];
     if (h.x0 == kLongRow) {
-        row_voxels(r, cells, tris, P, rowStart, pos, mp, tribits, nullptr, nullptr, hitStart, keys);
+        row_voxels(r, rowPair, boxes, tris, rowStart, pos, mp, tribits, nullptr, nullptr, hitStart, keys);
         return;
     }
     uint64_t at = hitStart[r];
@@ -2280,15 +2296,22 @@ static int mesh_core(Arena &ar, float *dv, int64_t nverts, int has_normals, int3
     }
 
     // voxel rows
-    uint32_t *rows;
+    if (P > 0xFFFFFFFFull) return fail(SVO_ERR_LIMIT, "builder: more than 2^32 (cell, triangle) pairs");
+    uint32_t *rows, *rowPair;
     uint64_t *rowStart;
-    if ((r = ar.alloc(rows, P)) || (r = ar.alloc(rowStart, P + 1))) return r;
-    if (P) hipLaunchKernelGGL(pair_rows_kernel, dim3(grid_for(P)), dim3(kThreads), 0, st, cells, ptri, P, pos, mp, rows);
+    PairBox *boxes;
+    if ((r = ar.alloc(rows, P)) || (r = ar.alloc(rowStart, P + 1)) || (r = ar.alloc(boxes, P))) return r;
+    if (P)
+        hipLaunchKernelGGL(pair_rows_kernel, dim3(grid_for(P)), dim3(kThreads), 0, st, cells, ptri, P, pos, mp, rows,
+                           boxes);
     HIP_TRY(hipGetLastError());
     uint64_t R;
     if ((r = scan_count(ar, rows, rowStart, P, st, R))) return r;
     hipLaunchKernelGGL(set_u64_kernel, dim3(1), dim3(1), 0, st, rowStart + P, R);
     ar.release(rows);
+    // each row's pair (the row passes read it instead of searching rowStart)
+    if ((r = ar.alloc(rowPair, R))) return r;
+    if (P) hipLaunchKernelGGL(row_pairs_kernel, dim3(grid_for(P)), dim3(kThreads), 0, st, rowStart, P, rowPair);  // 64 pairs per wave
     // (voxel, triangle) keys: a count pass sizes each row's output, an emit
     // pass writes the keys at the rows' exclusive offsets (a single pass that
     // reserved space per wave with device-scope atomics on one counter
@@ -2300,20 +2323,22 @@ static int mesh_core(Arena &ar, float *dv, int64_t nverts, int has_normals, int3
     RowHits *hits;
     if ((r = ar.alloc(hitCount, R)) || (r = ar.alloc(hitStart, R)) || (r = ar.alloc(hits, R))) return r;
     if (R)
-        hipLaunchKernelGGL(row_count_kernel, dim3(grid_for(R)), dim3(kThreads), 0, st, cells, ptri, P, rowStart, R, pos,
-                           mp, tribits, hitCount, hits);
+        hipLaunchKernelGGL(row_count_kernel, dim3(grid_for(R)), dim3(kThreads), 0, st, rowPair, boxes, ptri, rowStart, R,
+                           pos, mp, tribits, hitCount, hits);
     HIP_TRY(hipGetLastError());
     uint64_t K;
     if ((r = scan_count(ar, hitCount, hitStart, R, st, K))) return r;
     uint64_t *keys, *keys2;
     if ((r = ar.alloc(keys, K)) || (r = ar.alloc(keys2, K))) return r;
     if (R)
-        hipLaunchKernelGGL(row_emit_kernel, dim3(grid_for(R)), dim3(kThreads), 0, st, cells, ptri, P, rowStart, R, pos,
-                           mp, tribits, hits, hitStart, keys);
+        hipLaunchKernelGGL(row_emit_kernel, dim3(grid_for(R)), dim3(kThreads), 0, st, rowPair, boxes, ptri, rowStart, R,
+                           pos, mp, tribits, hits, hitStart, keys);
     HIP_TRY(hipGetLastError());
     ar.release(hitCount);
     ar.release(hitStart);
     ar.release(hits);
+    ar.release(rowPair);
+    ar.release(boxes);
     out.build.triangles = uint64_t(T);
     out.build.candidates = K;
     ar.release(cells);
