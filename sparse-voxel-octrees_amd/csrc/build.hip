@@ -2345,14 +2345,19 @@ static int mesh_core(Arena &ar, float *dv, int64_t nverts, int has_normals, int3
     ar.release(ptri);
     ar.release(rowStart);
 
-    // sort (voxel, triangle)
+    // sort (voxel, triangle). The keys are written in ascending triangle order
+    // (pairs in triangle order from the binning, rows in pair order, each
+    // row's keys at its scanned offset), and rocPRIM's radix sort is stable,
+    // so sorting the voxel bits alone leaves each voxel's triangles ascending:
+    // the same array as sorting the whole key, in 5 passes instead of 7 at
+    // 2048^3 (33 of 54 bits)
     if (K) {
         size_t bytes = 0;
         const int endBit = 3 * g.depth + tribits;
-        HIP_TRY(rocprim::radix_sort_keys(nullptr, bytes, keys, keys2, K, 0, endBit, st));
+        HIP_TRY(rocprim::radix_sort_keys(nullptr, bytes, keys, keys2, K, tribits, endBit, st));
         uint8_t *tmp;
         if ((r = ar.alloc(tmp, bytes))) return r;
-        HIP_TRY(rocprim::radix_sort_keys(tmp, bytes, keys, keys2, K, 0, endBit, st));
+        HIP_TRY(rocprim::radix_sort_keys(tmp, bytes, keys, keys2, K, tribits, endBit, st));
         ar.release(tmp);
     }
     ar.release(keys);
