@@ -743,19 +743,6 @@ __global__ void row_emit_kernel(const uint32_t *__restrict__ rowPair, const Pair
     }
 }
 
-__global__ void seg_heads_kernel(const uint64_t *__restrict__ keys, uint64_t K, int shift, uint32_t *__restrict__ head) {
-    uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= K) return;
-    head[i] = (i == 0 || (keys[i] >> shift) != (keys[i - 1] >> shift)) ? 1u : 0u;
-}
-
-__global__ void seg_starts_kernel(const uint32_t *__restrict__ head, const uint64_t *__restrict__ segIdx, uint64_t K,
-                                  uint64_t *__restrict__ segStart) {
-    uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= K) return;
-    if (head[i]) segStart[segIdx[i]] = i;
-}
-
 // PlyLoader::writeTriangleCell, src/PlyLoader.cpp:296-336, folded over one
 // voxel's triangles in ascending index order, starting from the buffer state
 // (word, count) the voxel's cell holds when its block is processed (0 for a
@@ -1014,36 +1001,8 @@ __global__ void compact_kernel(const uint64_t *__restrict__ key, const uint32_t 
 // ---------------------------------------------------------------------------
 
 // Level kernels read the level's node count from a device word (nptr): the
-// count comes from the previous level's scan, so no host round trip sizes
+// count comes from the previous level's pass, so no host round trip sizes
 // them; grids are sized by a host upper bound.
-__global__ void parent_heads_kernel(const uint64_t *__restrict__ child, const uint64_t *__restrict__ nptr,
-                                    uint32_t *__restrict__ head) {
-    uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= *nptr) return;
-    head[i] = (i == 0 || (child[i] >> 3) != (child[i - 1] >> 3)) ? 1u : 0u;
-}
-
-// parent key / child start / mask from sorted child keys (no extra parents):
-// the first child of each parent writes all three; its siblings (at most 7,
-// keys are unique) follow it in the sorted keys
-__global__ void parents_kernel(const uint64_t *__restrict__ child, const uint64_t *__restrict__ nptr,
-                               const uint32_t *__restrict__ head, const uint64_t *__restrict__ ex,
-                               uint64_t *__restrict__ pkey, uint64_t *__restrict__ pstart, uint32_t *__restrict__ pmask) {
-    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    const uint64_t n = *nptr;
-    if (i >= n || !head[i]) return;
-    const uint64_t pi = ex[i];  // exclusive scan of heads: the parent's index
-    const uint64_t parent = child[i] >> 3;
-    uint32_t mask = 1u << uint32_t(child[i] & 7);
-    for (uint64_t j = i + 1; j < n && j < i + 8; ++j) {
-        const uint64_t k = child[j];
-        if ((k >> 3) != parent) break;
-        mask |= 1u << uint32_t(k & 7);
-    }
-    pkey[pi] = parent;
-    pstart[pi] = i;
-    pmask[pi] = mask;
-}
 
 // Level that also takes the cache blocks the occupancy table marks: parents
 // are the flagged cells of a dense Morton-indexed table.
@@ -1534,6 +1493,94 @@ __global__ void __launch_bounds__(kThreads) scan_down_kernel(const In *__restric
     }
 }
 
+// Heads of runs of equal keys >> shift in sorted keys[0, n): i is a head when
+// i == 0 or its key differs from key i - 1. A compaction in the scan's three
+// passes with the head flags computed from the keys in the first and third
+// (no flag or index arrays): the first counts each tile's heads, the second
+// scans the tile counts, the third gives each head its index j among the
+// heads and calls emit(j, i, n).
+__device__ __forceinline__ uint32_t run_head_flags(const uint64_t *__restrict__ keys, uint64_t first, uint64_t n,
+                                                   int shift) {
+    uint64_t k[kScanItems];
+    if (first + kScanItems <= n) {
+#pragma unroll
+        for (int q = 0; q < kScanItems; q += 2) {
+            const ulonglong2 v = *reinterpret_cast<const ulonglong2 *>(keys + first + q);
+            k[q] = v.x;
+            k[q + 1] = v.y;
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < kScanItems; ++q) k[q] = first + q < n ? keys[first + q] : 0;
+    }
+    uint64_t prev = (first && first <= n) ? keys[first - 1] >> shift : 0;
+    uint32_t flags = 0;
+#pragma unroll
+    for (int q = 0; q < kScanItems; ++q) {
+        const uint64_t cur = k[q] >> shift;
+        if (first + q < n && (first + q == 0 || cur != prev)) flags |= 1u << q;
+        prev = cur;
+    }
+    return flags;
+}
+
+__global__ void __launch_bounds__(kThreads) run_sums_kernel(const uint64_t *__restrict__ keys,
+                                                            const uint64_t *__restrict__ nptr, int shift,
+                                                            uint64_t *__restrict__ sums) {
+    __shared__ uint64_t lds[kThreads / 64];
+    const uint64_t n = *nptr;
+    const uint64_t base = uint64_t(blockIdx.x) * kScanTile;
+    if (base >= n) {
+        if (threadIdx.x == 0) sums[blockIdx.x] = 0;
+        return;
+    }
+    const uint32_t flags = run_head_flags(keys, base + uint64_t(threadIdx.x) * kScanItems, n, shift);
+    uint64_t total;
+    block_exclusive(uint64_t(__builtin_popcount(flags)), lds, &total);
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+template <typename Emit>
+__global__ void __launch_bounds__(kThreads) run_emit_kernel(const uint64_t *__restrict__ keys,
+                                                            const uint64_t *__restrict__ nptr, int shift,
+                                                            const uint64_t *__restrict__ offs, Emit emit) {
+    __shared__ uint64_t lds[kThreads / 64];
+    const uint64_t n = *nptr;
+    const uint64_t base = uint64_t(blockIdx.x) * kScanTile;
+    if (base >= n) return;
+    const uint64_t first = base + uint64_t(threadIdx.x) * kScanItems;
+    uint32_t flags = run_head_flags(keys, first, n, shift);
+    uint64_t total;
+    uint64_t j = offs[blockIdx.x] + block_exclusive(uint64_t(__builtin_popcount(flags)), lds, &total);
+    for (; flags; flags &= flags - 1, ++j) emit(j, first + uint64_t(__builtin_ctz(flags)), n);
+}
+
+// The fold's segments: segStart[j] = the first candidate of voxel j.
+struct SegmentEmit {
+    uint64_t *segStart;
+    __device__ void operator()(uint64_t j, uint64_t i, uint64_t) const { segStart[j] = i; }
+};
+
+// A level's parents from the sorted child keys: parent j's key, first child
+// and child mask (its children, at most 8 since keys are unique, follow i).
+struct ParentEmit {
+    const uint64_t *child;
+    uint64_t *pkey, *pstart;
+    uint32_t *pmask;
+    __device__ void operator()(uint64_t j, uint64_t i, uint64_t n) const {
+        const uint64_t parent = child[i] >> 3;
+        uint32_t mask = 1u << uint32_t(child[i] & 7);
+        for (uint64_t q = i + 1; q < n && q < i + 8; ++q) {
+            const uint64_t k = child[q];
+            if ((k >> 3) != parent) break;
+            mask |= 1u << uint32_t(k & 7);
+        }
+        pkey[j] = parent;
+        pstart[j] = i;
+        pmask[j] = mask;
+    }
+};
+
 __global__ void set_u64_kernel(uint64_t *p, uint64_t v) { *p = v; }
 
 // *out = 1 when a cache block other than the last flagged one (the last
@@ -1576,6 +1623,27 @@ int exclusive_scan_dev(Arena &ar, const In *in, uint64_t *out, const uint64_t *n
     HIP_TRY(hipGetLastError());
     ar.release(sums);
     if (!total) ar.release(tot);
+    return SVO_OK;
+}
+
+// Run heads of keys[0, *nptr) (sorted; runs of equal keys >> shift): emit(j,
+// i, n) for each head i, j its index among the heads; *total (device) = the
+// head count. `bound` (host) bounds *nptr and sizes the grids.
+template <typename Emit>
+int run_heads(Arena &ar, const uint64_t *keys, const uint64_t *nptr, uint64_t bound, int shift, uint64_t *total,
+              const Emit &emit, hipStream_t s) {
+    const uint64_t nb = std::max<uint64_t>(1, (bound + kScanTile - 1) / kScanTile);
+    if (nb > 0x7FFFFFFFull) return fail(SVO_ERR_LIMIT, "builder: scan too large");
+    if (reinterpret_cast<uintptr_t>(keys) % 16)
+        return fail(SVO_ERR_DEVICE, "builder: key arrays must be 16-byte aligned (internal error)");
+    uint64_t *sums;
+    int r = ar.alloc(sums, nb);
+    if (r) return r;
+    hipLaunchKernelGGL(run_sums_kernel, dim3(unsigned(nb)), dim3(kThreads), 0, s, keys, nptr, shift, sums);
+    hipLaunchKernelGGL(scan_block_sums_kernel, dim3(1), dim3(kSumThreads), 0, s, sums, nb, total);
+    hipLaunchKernelGGL(run_emit_kernel<Emit>, dim3(unsigned(nb)), dim3(kThreads), 0, s, keys, nptr, shift, sums, emit);
+    HIP_TRY(hipGetLastError());
+    ar.release(sums);
     return SVO_OK;
 }
 
@@ -1802,18 +1870,11 @@ int emit_tree(Arena &ar, const Geometry &g, uint64_t *leafKey, uint32_t *leafWor
                 continue;
             }
             p.bound = d < 21 ? std::min(c.bound, uint64_t(1) << (3 * d)) : c.bound;
-            uint32_t *head;
-            uint64_t *ex;
-            if ((r = ar.alloc(head, c.bound)) || (r = ar.alloc(ex, c.bound))) return r;
-            hipLaunchKernelGGL(parent_heads_kernel, dim3(grid_for(c.bound)), dim3(kThreads), 0, st, c.key, c.cnt, head);
-            if ((r = exclusive_scan_dev(ar, head, ex, c.cnt, c.bound, p.cnt, st))) return r;
             if ((r = ar.alloc(p.key, p.bound)) || (r = ar.alloc(p.start, p.bound)) || (r = ar.alloc(p.mask, p.bound)))
                 return r;
             // every parent below the count is written once (by its first child)
-            hipLaunchKernelGGL(parents_kernel, dim3(grid_for(c.bound)), dim3(kThreads), 0, st, c.key, c.cnt, head, ex,
-                               p.key, p.start, p.mask);
-            ar.release(head);
-            ar.release(ex);
+            if ((r = run_heads(ar, c.key, c.cnt, c.bound, 3, p.cnt, ParentEmit{c.key, p.key, p.start, p.mask}, st)))
+                return r;
         }
         HIP_TRY(hipGetLastError());
     }
@@ -2363,31 +2424,29 @@ static int mesh_core(Arena &ar, float *dv, int64_t nverts, int has_normals, int3
     ar.release(keys);
 
     // per-voxel fold
-    uint32_t *head;
-    uint64_t *segIdx;
-    if ((r = ar.alloc(head, K)) || (r = ar.alloc(segIdx, K))) return r;
-    if (K) hipLaunchKernelGGL(seg_heads_kernel, dim3(grid_for(K)), dim3(kThreads), 0, st, keys2, K, tribits, head);
-    uint64_t V;
-    if ((r = scan_count(ar, head, segIdx, K, st, V))) return r;
-    uint64_t *segStart, *vkey;
+    // each voxel's first candidate (segStart sized by the candidate count, the
+    // voxel count's bound), then the voxel count in one read
+    uint64_t V = 0;
+    uint64_t *segStart, *vkey, *segInfo;
     uint32_t *vword;
     uint8_t *vcount;
-    if ((r = ar.alloc(segStart, V + 1))) return r;
-    if (K) hipLaunchKernelGGL(seg_starts_kernel, dim3(grid_for(K)), dim3(kThreads), 0, st, head, segIdx, K, segStart);
-    hipLaunchKernelGGL(set_u64_kernel, dim3(1), dim3(1), 0, st, segStart + V, K);
-    if (sink) {
-        ar.release(head);
-        ar.release(segIdx);
-        return write_volume(ar, g, mp, tribits, keys2, segStart, V, pos, nrm, col, sliceZ, sink->fp);
+    if ((r = ar.alloc(segStart, K + 1)) || (r = ar.alloc(segInfo, 2))) return r;
+    if (K) {
+        hipLaunchKernelGGL(set_u64_kernel, dim3(1), dim3(1), 0, st, segInfo, K);
+        if ((r = run_heads(ar, keys2, segInfo, K, tribits, segInfo + 1, SegmentEmit{segStart}, st))) return r;
+        uint64_t info[2];
+        if ((r = read_words(segInfo, info, 2, st))) return r;
+        V = info[1];
     }
+    ar.release(segInfo);
+    hipLaunchKernelGGL(set_u64_kernel, dim3(1), dim3(1), 0, st, segStart + V, K);
+    if (sink) return write_volume(ar, g, mp, tribits, keys2, segStart, V, pos, nrm, col, sliceZ, sink->fp);
     if ((r = ar.alloc(vkey, V)) || (r = ar.alloc(vword, V)) || (r = ar.alloc(vcount, V))) return r;
     if (V)
         hipLaunchKernelGGL(fold_kernel, dim3(unsigned((V + kFoldVoxelsPerBlock - 1) / kFoldVoxelsPerBlock)), dim3(kThreads),
                            0, st, keys2, segStart, V, tribits, pos, nrm,
                            col, mp, vkey, vword, vcount);
     HIP_TRY(hipGetLastError());
-    ar.release(head);
-    ar.release(segIdx);
 
     uint64_t *lkey, L;
     uint32_t *lword;
