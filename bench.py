@@ -544,6 +544,28 @@ def run_frames(pkg, torch, stream, tree, cams, width, height, steps, warmup, cpu
     ms3 = a.elapsed_time(b) / steps
     diff3 = sum(int(torch.count_nonzero(rgba3[k] != rgba[k])) + int(torch.count_nonzero(
         depth3[k].view(torch.int32) != depth[k].view(torch.int32))) for k in range(len(cams)))
+    # the frames call through a plan made for its frame count (the fine pass
+    # over all frames' tiles in the previous call's cost order)
+    fplan = pkg.svo.TilePlan.for_frames(width, height, len(cams), 16) if len(cams) <= 16 else None
+    if fplan is not None:
+        rgba4 = [torch.empty_like(x) for x in rgba]
+        depth4 = [torch.empty_like(x) for x in depth]
+        rptrs4, dptrs4 = [x.data_ptr() for x in rgba4], [x.data_ptr() for x in depth4]
+        for _ in range(max(warmup, 1)):
+            tree.render_frames_device(cams, width, height, rptrs4, dptrs4, 16, 1, sptr, plan=fplan)
+        torch.cuda.synchronize()
+        a.record(stream)
+        for _ in range(steps):
+            tree.render_frames_device(cams, width, height, rptrs4, dptrs4, 16, 1, sptr, plan=fplan)
+        b.record(stream)
+        torch.cuda.synchronize()
+        ms4 = a.elapsed_time(b) / steps
+        diff4 = sum(int(torch.count_nonzero(rgba4[k] != rgba[k])) + int(torch.count_nonzero(
+            depth4[k].view(torch.int32) != depth[k].view(torch.int32))) for k in range(len(cams)))
+        out["frames_call"]["planned"] = {"ms_per_frame": round(ms4 / len(cams), 4),
+                                         "frames_per_s": round(len(cams) / (ms4 / 1e3), 1),
+                                         "mismatched_vs_one_frame_per_call": diff4}
+        fplan.close()
     out["planned"] = {"ms_per_frame": round(ms3 / len(cams), 4), "frames_per_s": round(len(cams) / (ms3 / 1e3), 1),
                       "vs_one_frame_per_call": round(ms / ms3, 3), "mismatched_vs_one_frame_per_call": diff3,
                       "form": "one frame per call, the fine pass in a tile plan's cost order (one plan per camera)"}
@@ -988,6 +1010,7 @@ def main():
             (mixed and mixed["mismatched_fields_vs_plain"]) or (terrain and terrain["parity"]["mismatched_fields"]) or \
             (frames and (frames["parity"]["mismatched_pixels"] or frames["parity"]["mismatched_depths"]
                          or frames["frames_call"]["mismatched_vs_one_frame_per_call"]
+                         or frames["frames_call"].get("planned", {}).get("mismatched_vs_one_frame_per_call")
                          or frames["planned"]["mismatched_vs_one_frame_per_call"]
                          or any(frames["parity"].get("vs_reference_classes", {}).values()))):
         sys.exit(3)

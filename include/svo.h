@@ -214,6 +214,16 @@ int svo_render_frames_device(svo_octree *h, const svo_camera *cams, int nframes,
 int svo_tile_plan_create_frame(int width, int height, int stripes, svo_tile_plan **out);
 int svo_render_frame_planned_device(svo_octree *h, svo_tile_plan *plan, const svo_camera *cam, int width, int height,
                                     int stripes, int stride, uint32_t *rgba, float *depth, void *stream);
+/* svo_render_frames_device through a tile plan (no counterpart in the
+ * reference): the fine pass over all nframes frames' tiles runs in the cost
+ * order the plan's previous call filed. The plan is made for the frame size,
+ * stripe split and frame count (svo_tile_plan_create_frames, 1 to 16 frames:
+ * one launch group; SVO_ERR_INVALID otherwise, before anything is enqueued).
+ * The same pixels and depths as svo_render_frames_device. */
+int svo_tile_plan_create_frames(int width, int height, int stripes, int nframes, svo_tile_plan **out);
+int svo_render_frames_planned_device(svo_octree *h, svo_tile_plan *plan, const svo_camera *cams, int nframes,
+                                     int width, int height, int stripes, int stride, uint32_t *const *rgba,
+                                     float *const *depth, void *stream);
 /* Coarse depth entries of a frame, or -1 (message in svo_last_error) when the
  * size or stripe split is refused. */
 int64_t svo_frame_depth_count(int width, int height, int stripes);

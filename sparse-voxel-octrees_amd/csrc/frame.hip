@@ -270,15 +270,15 @@ void frame_counts(const FrameParams &p, int64_t &corners, int64_t &tiles) {
     }
 }
 
-// With `plan` (n == 1), the fine pass runs in the plan's cost order.
+// With `plan` (made for n frames), the fine pass runs in the plan's cost order.
 template <int N>
 int launch_frames(svo_octree &h, FrameSet<N> &set, int n, hipStream_t s, svo_tile_plan *plan = nullptr) {
     const FrameParams &p = set.p[0];
     int64_t corners = 0, tiles = 0;
     frame_counts(p, corners, tiles);
-    if (plan && (n != 1 || plan->kind != svo_tile_plan::kFrame || plan->width != p.width || plan->height != p.height ||
-                 plan->stripes != p.stripes || int64_t(plan->tiles) != tiles))
-        return fail(SVO_ERR_INVALID, "the tile plan was made for another frame size or stripe split");
+    if (plan && (n != plan->frames || plan->kind != svo_tile_plan::kFrame || plan->width != p.width ||
+                 plan->height != p.height || plan->stripes != p.stripes || int64_t(plan->tiles) != tiles * n))
+        return fail(SVO_ERR_INVALID, "the tile plan was made for another frame size, stripe split or frame count");
     const int64_t cblocks = (corners + 63) / 64;
     if (tiles * n > 0x7FFFFFFF || cblocks * n > 0x7FFFFFFF) return fail(SVO_ERR_LIMIT, "image too large");
     int nulls = 0;
@@ -320,19 +320,22 @@ int launch_frames(svo_octree &h, FrameSet<N> &set, int n, hipStream_t s, svo_til
 
 }  // namespace
 
-int tile_plan_create_frame(int width, int height, int stripes, svo_tile_plan **out) {
+int tile_plan_create_frames(int width, int height, int stripes, int nframes, svo_tile_plan **out) {
     if (!out) return fail(SVO_ERR_INVALID, "null argument");
     *out = nullptr;
+    if (nframes < 1 || nframes > kFramesPerLaunch)
+        return fail(SVO_ERR_INVALID, "a frame tile plan covers 1 to 16 frames (one launch)");
     int r = frame_layout(width, height, stripes);
     if (r) return r;
     svo_camera cam{};
     const FrameParams p = frame_params(cam, width, height, stripes, 1);
     int64_t corners = 0, tiles = 0;
     frame_counts(p, corners, tiles);
-    r = tile_plan_alloc(tiles, width, height, out);
+    r = tile_plan_alloc(tiles * nframes, width, height, out);
     if (r == SVO_OK) {
         (*out)->kind = svo_tile_plan::kFrame;
         (*out)->stripes = stripes;
+        (*out)->frames = nframes;
     }
     return r;
 }
@@ -364,8 +367,14 @@ int render_frame_device(svo_octree &h, const svo_camera &cam, int width, int hei
 // GPU idle) overlap the other frames' fine work. Each frame equals
 // render_frame_device's.
 int render_frames_device(svo_octree &h, const svo_camera *cams, int n, int width, int height, int stripes, int stride,
-                         uint32_t *const *rgba, float *const *depth, void *stream) {
+                         uint32_t *const *rgba, float *const *depth, void *stream, svo_tile_plan *plan) {
     if (!cams || !rgba || n <= 0) return fail(SVO_ERR_INVALID, "null camera or image array, or no frames");
+    if (plan) {
+        if (n != plan->frames) return fail(SVO_ERR_INVALID, "the tile plan was made for another frame count");
+        int dev = -1;
+        HIP_TRY(hipGetDevice(&dev));
+        if (dev != plan->device) return fail(SVO_ERR_INVALID, "the tile plan lives on another device");
+    }
     int r = frame_layout(width, height, stripes);
     if (r) return r;
     if (stride <= 0 || stride > 8) return fail(SVO_ERR_INVALID, "stride must be in [1, 8]");
@@ -385,7 +394,7 @@ int render_frames_device(svo_octree &h, const svo_camera *cams, int n, int width
             set.rgba[k] = rgba[f];
             set.depth[k] = depth ? depth[f] : nullptr;
         }
-        r = launch_frames(h, set, m, s);
+        r = launch_frames(h, set, m, s, plan);  // a plan covers one launch group (n <= 16)
     }
     return r;
 }

@@ -468,7 +468,14 @@ int svo_render_frame_device(svo_octree *h, const svo_camera *cam, int width, int
 int svo_tile_plan_create_frame(int width, int height, int stripes, svo_tile_plan **out) {
     return guarded([&]() -> int {
         clear_error();
-        return tile_plan_create_frame(width, height, stripes, out);
+        return tile_plan_create_frames(width, height, stripes, 1, out);
+    });
+}
+
+int svo_tile_plan_create_frames(int width, int height, int stripes, int nframes, svo_tile_plan **out) {
+    return guarded([&]() -> int {
+        clear_error();
+        return tile_plan_create_frames(width, height, stripes, nframes, out);
     });
 }
 
@@ -487,6 +494,16 @@ int svo_render_frames_device(svo_octree *h, const svo_camera *cams, int nframes,
         clear_error();
         if (!h) return fail(SVO_ERR_INVALID, "null handle");
         return render_frames_device(*h, cams, nframes, width, height, stripes, stride, rgba, depth, stream);
+    });
+}
+
+int svo_render_frames_planned_device(svo_octree *h, svo_tile_plan *plan, const svo_camera *cams, int nframes,
+                                     int width, int height, int stripes, int stride, uint32_t *const *rgba,
+                                     float *const *depth, void *stream) {
+    return guarded([&]() -> int {
+        clear_error();
+        if (!h || !plan) return fail(SVO_ERR_INVALID, "null argument");
+        return render_frames_device(*h, cams, nframes, width, height, stripes, stride, rgba, depth, stream, plan);
     });
 }
 

@@ -158,9 +158,9 @@ int render_shadow_device(svo_octree &h, const svo_camera &cam, int width, int he
 // ---- viewer frame (frame.hip) ----
 int render_frame_device(svo_octree &h, const svo_camera &cam, int width, int height, int stripes, int stride,
                         uint32_t *rgba, float *depth, void *stream, svo_tile_plan *plan = nullptr);
-int tile_plan_create_frame(int width, int height, int stripes, svo_tile_plan **out);
+int tile_plan_create_frames(int width, int height, int stripes, int nframes, svo_tile_plan **out);
 int render_frames_device(svo_octree &h, const svo_camera *cams, int n, int width, int height, int stripes, int stride,
-                         uint32_t *const *rgba, float *const *depth, void *stream);
+                         uint32_t *const *rgba, float *const *depth, void *stream, svo_tile_plan *plan = nullptr);
 int64_t frame_depth_count(int width, int height, int stripes);
 
 // ---- batched persistent renderer (render_batch.hip) ----

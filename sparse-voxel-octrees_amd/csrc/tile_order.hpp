@@ -33,6 +33,7 @@ struct svo_tile_plan {
     Kind kind = kPrimary;        // the render entry the plan was made for
     int width = 0, height = 0;   // primary renders and frames: the image size (0 x 0 for a batch plan)
     int stripes = 0;             // frames: the stripe split
+    int frames = 0;              // frames: the frames of each launch made through the plan
     int tiles = 0;               // tiles of every launch made through the plan
     int cap = 0;                 // ceil(tiles / 64): the most tiles a replica holds
     int device = -1;

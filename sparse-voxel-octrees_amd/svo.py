@@ -40,6 +40,7 @@ EXPORTS = [
     "svo_server_destroy", "svo_tile_plan_create", "svo_tile_plan_free", "svo_render_primary_planned_device",
     "svo_render_batch_noalloc", "svo_load_device_code", "svo_tile_plan_create_batch",
     "svo_render_batch_planned_device", "svo_tile_plan_create_frame", "svo_render_frame_planned_device",
+    "svo_tile_plan_create_frames", "svo_render_frames_planned_device",
     "svo_build_stats_get",
 ]
 
@@ -316,6 +317,9 @@ def _load():
         "svo_render_frame": [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P],
         "svo_render_frame_device": [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, P],
         "svo_tile_plan_create_frame": [ctypes.c_int, ctypes.c_int, ctypes.c_int, P],
+        "svo_tile_plan_create_frames": [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, P],
+        "svo_render_frames_planned_device": [P, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                             ctypes.c_int, P, P, P],
         "svo_render_frame_planned_device": [P, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, P],
         "svo_render_frames_device": [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, P],
         "svo_frame_depth_count": [ctypes.c_int, ctypes.c_int, ctypes.c_int],
@@ -643,15 +647,21 @@ class VoxelOctree:
                                                        int(stripes), int(stride), rgba_ptr, depth_ptr, stream))
 
     def render_frames_device(self, cams, width, height, rgba_ptrs, depth_ptrs=None, stripes=16, stride=1,
-                             stream=None):
+                             stream=None, plan=None):
         """Several frames in one call (svo_render_frames_device): device
-        pointers per frame; depth_ptrs may be None or hold None entries."""
+        pointers per frame; depth_ptrs may be None or hold None entries. With
+        `plan` (TilePlan.for_frames, same frame count) the fine pass runs in
+        the plan's cost order."""
         n = len(cams)
         carr = (Camera * n)(*cams)
         rarr = (ctypes.c_void_p * n)(*rgba_ptrs)
         darr = None if depth_ptrs is None else (ctypes.c_void_p * n)(*depth_ptrs)
-        _check(lib.svo_render_frames_device(self._h, carr, n, int(width), int(height), int(stripes), int(stride),
-                                            rarr, darr, stream))
+        if plan is None:
+            _check(lib.svo_render_frames_device(self._h, carr, n, int(width), int(height), int(stripes), int(stride),
+                                                rarr, darr, stream))
+        else:
+            _check(lib.svo_render_frames_planned_device(self._h, plan._p, carr, n, int(width), int(height),
+                                                        int(stripes), int(stride), rarr, darr, stream))
 
 
 def convert_to_volume(ply_path, resolution, voxel_path, threads=0, memory_budget=0):
@@ -686,7 +696,7 @@ class TilePlan:
         self.width, self.height = int(width), int(height)
         self._p = ctypes.c_void_p()
         if _frame is not None:
-            _check(lib.svo_tile_plan_create_frame(*_frame, ctypes.byref(self._p)))
+            _check(lib.svo_tile_plan_create_frames(*_frame, ctypes.byref(self._p)))
         elif _batch is None:
             _check(lib.svo_tile_plan_create(self.width, self.height, ctypes.byref(self._p)))
         else:
@@ -698,7 +708,13 @@ class TilePlan:
     def for_frame(cls, width, height, stripes=16):
         """A plan for VoxelOctree.render_frame_device(..., plan=...) at this
         frame size and stripe split."""
-        return cls(0, 0, _frame=(int(width), int(height), int(stripes)))
+        return cls(0, 0, _frame=(int(width), int(height), int(stripes), 1))
+
+    @classmethod
+    def for_frames(cls, width, height, nframes, stripes=16):
+        """A plan for VoxelOctree.render_frames_device(..., plan=...) over
+        nframes (1 to 16) frames of this size and stripe split per call."""
+        return cls(0, 0, _frame=(int(width), int(height), int(stripes), int(nframes)))
 
     @classmethod
     def for_batch(cls, views, tile_w=0):

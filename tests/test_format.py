@@ -418,6 +418,10 @@ def test_tile_plans_refuse_bad_arguments_before_device_use(svo, golden):
         with pytest.raises(svo.SvoError) as e:
             m.TilePlan.for_frame(w, h, stripes)
         assert e.value.code == m.ERR_INVALID, (w, h, stripes)
+    for nframes in (0, 17):  # a multi-frame plan covers one launch group of 1 to 16 frames
+        with pytest.raises(svo.SvoError) as e:
+            m.TilePlan.for_frames(64, 64, nframes)
+        assert e.value.code == m.ERR_INVALID, nframes
     t = svo.VoxelOctree(os.path.join(golden, "dragon.oct"))
     cam = t.orbit_camera(30.0, 10.0, 1.0)
     for views, tile_w in [([], 0), ([m.Renderer.view(t, cam, 64, 64)], 5), ([m.Renderer.view(t, cam, 0, 64)], 0)]:

@@ -570,6 +570,46 @@ def test_planned_frames_equal_unplanned_frames(svo, trees):
     assert bool((out == 7).all())
 
 
+def test_planned_frame_calls_equal_unplanned_calls(svo, trees):
+    """svo_render_frames_planned_device: 8 frames per call through one plan
+    (the fine pass over all 8 frames' tiles in the cost order the previous
+    call filed), over five calls of moving cameras at stride 1 and 3, give
+    the same RGBA and depth tables as svo_render_frames_device; a plan made
+    for another frame count is refused before anything is enqueued."""
+    import torch
+    t = trees["dragon"]
+    w, h, stripes, nf = 640, 360, 16, 8
+    nd = svo.svo.frame_depth_count(w, h, stripes)
+    stream = torch.cuda.Stream()
+    for stride in (1, 3):
+        plan = svo.svo.TilePlan.for_frames(w, h, nf, stripes)
+        for call in range(5):
+            cams = [t.orbit_camera(15.0 - 4.0 * call + 2.0 * k, 20.0 * call + 40.0 * k, 1.0) for k in range(nf)]
+            rgba = [torch.full((h * w,), 0x12345678, dtype=torch.int32, device="cuda") for _ in cams]
+            depth = [torch.full((nd,), -1.0, dtype=torch.float32, device="cuda") for _ in cams]
+            ref_rgba = [torch.zeros_like(x) for x in rgba]
+            ref_depth = [torch.zeros_like(x) for x in depth]
+            print(f"[gpu] planned frames call {call} stride {stride} {nf}x{w}x{h}", flush=True)
+            t.render_frames_device(cams, w, h, [x.data_ptr() for x in rgba], [x.data_ptr() for x in depth], stripes,
+                                   stride, stream.cuda_stream, plan=plan)
+            t.render_frames_device(cams, w, h, [x.data_ptr() for x in ref_rgba], [x.data_ptr() for x in ref_depth],
+                                   stripes, stride, stream.cuda_stream)
+            stream.synchronize()
+            for k in range(nf):
+                assert torch.equal(rgba[k], ref_rgba[k]), (stride, call, k)
+                assert torch.equal(depth[k].view(torch.int32), ref_depth[k].view(torch.int32)), (stride, call, k)
+    cams = [t.orbit_camera(0.0, 30.0 * k, 1.0) for k in range(nf)]
+    out = [torch.full((h * w,), 7, dtype=torch.int32, device="cuda") for _ in cams]
+    other = svo.svo.TilePlan.for_frames(w, h, nf - 1, stripes)
+    print("[gpu] planned frames call refused: plan for 7 frames", flush=True)
+    with pytest.raises(svo.SvoError) as e:
+        t.render_frames_device(cams, w, h, [x.data_ptr() for x in out], None, stripes, 1, stream.cuda_stream,
+                               plan=other)
+    assert e.value.code == svo.svo.ERR_INVALID
+    stream.synchronize()
+    assert all(bool((x == 7).all()) for x in out)
+
+
 def test_cost_ordered_renders_match_plain_kernel(svo, trees):
     """svo_render_primary_planned_device: the first render of a plan (raster
     order), the next ones (tiles in the order the previous render filed them
