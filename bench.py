@@ -393,7 +393,22 @@ def run_terrain(pkg, torch, stream, steps, warmup, cpu_threads, cache_dir, n=819
         return a.elapsed_time(b) / steps
 
     prim_ms = timed(primary_only)
+    # the same step through a tile plan (each launch starts the tiles the
+    # previous one found slowest, primary plus shadow passes); its outputs
+    # must equal the unplanned step's
+    plan = pkg.svo.TilePlan(width, height)
+    planned_out = [torch.empty_like(x) for x in (hit, nrm, tt, lf, sh)]
+
+    def planned_step():
+        ph, pn, pt, pl, ps = planned_out
+        tree.render_shadow_device(cam, width, height, light, eps, ph.data_ptr(), pt.data_ptr(), ps.data_ptr(),
+                                  pn.data_ptr(), pl.data_ptr(), sptr, plan=plan)
+
+    planned_ms = timed(planned_step)
     ms = timed(step)
+    planned_mism = sum(int(torch.count_nonzero(a.view(torch.uint8) != b.view(torch.uint8)))
+                       for a, b in zip(planned_out, (hit, nrm, tt, lf, sh)))
+    plan.close()
     g_hit = hit.cpu().numpy()
     g_sh = sh.cpu().numpy()
     shadow_rays = int((g_hit == 1).sum())
@@ -451,6 +466,9 @@ def run_terrain(pkg, torch, stream, steps, warmup, cpu_threads, cache_dir, n=819
             "grays_s": round(rays / (ms / 1e3) / 1e9, 4), "ms_per_step": round(ms, 3),
             "primary_only_grays_s": round(m / (prim_ms / 1e3) / 1e9, 4), "primary_only_ms": round(prim_ms, 3),
             "shadow_pass_ms": round(ms - prim_ms, 3),
+            "planned": {"grays_s": round(rays / (planned_ms / 1e3) / 1e9, 4), "ms_per_step": round(planned_ms, 3),
+                        "mismatched_bytes_vs_unplanned": planned_mism,
+                        "form": "svo_render_shadow_planned_device: the same launch through a tile plan"},
             "bytes_per_ray": round(b_ray, 2),
             "hbm_frac": round(b_ray * rays / (ms / 1e3) / 1e9 / HBM_PEAK_GBS, 5),
             "cpu_baseline": cpu_baseline,
@@ -1007,7 +1025,7 @@ def main():
     }
     print(json.dumps(out), flush=True)
     if mismatches or batched_mismatch or config.get("build_oct_md5_matches_reference") is False or \
-            (mixed and mixed["mismatched_fields_vs_plain"]) or (terrain and terrain["parity"]["mismatched_fields"]) or \
+            (mixed and mixed["mismatched_fields_vs_plain"]) or (terrain and (terrain["parity"]["mismatched_fields"] or terrain["planned"]["mismatched_bytes_vs_unplanned"])) or \
             (frames and (frames["parity"]["mismatched_pixels"] or frames["parity"]["mismatched_depths"]
                          or frames["frames_call"]["mismatched_vs_one_frame_per_call"]
                          or frames["frames_call"].get("planned", {}).get("mismatched_vs_one_frame_per_call")

@@ -181,6 +181,14 @@ int svo_render_shadow(svo_octree *h, const svo_camera *cam, int width, int heigh
 int svo_render_shadow_device(svo_octree *h, const svo_camera *cam, int width, int height, const float light[3],
                              float eps, uint8_t *hit, uint32_t *normal, float *t, uint64_t *leaf, uint8_t *shadow,
                              void *stream);
+/* svo_render_shadow_device with the tiles in a tile plan's cost order (no
+ * counterpart in the reference): the plan comes from svo_tile_plan_create for
+ * the image size (SVO_ERR_INVALID otherwise) and each launch files a tile by
+ * its primary plus shadow passes, so keep a plan for shadow renders apart
+ * from one used by svo_render_primary_planned_device. Same outputs. */
+int svo_render_shadow_planned_device(svo_octree *h, svo_tile_plan *plan, const svo_camera *cam, int width, int height,
+                                     const float light[3], float eps, uint8_t *hit, uint32_t *normal, float *t,
+                                     uint64_t *leaf, uint8_t *shadow, void *stream);
 
 /* The viewer's frame (src/Main.cpp renderBatch/renderTile/shade, :81-202),
  * bit-identical: the image is split into `stripes` horizontal stripes as the

@@ -35,16 +35,18 @@ using namespace march;
 
 // Runs one ray to completion. Returns 0 miss, 1 leaf hit, 2 LOD hit. With
 // Normal set, a leaf hit also gives the leaf's word index and word.
-template <bool Lod, bool Finite, typename Idx, bool Normal = true, int Reg = 1>
+// With Count, *passes counts the loop passes.
+template <bool Lod, bool Finite, typename Idx, bool Normal = true, int Reg = 1, bool Count = false>
 __device__ __forceinline__ int march_ray(const uint32_t *__restrict__ tree, float ox, float oy, float oz, float dx,
                                          float dy, float dz, float rayScale, const Stack<Idx> &stk, uint32_t &normal,
-                                         float &tOut, Idx &leaf) {
+                                         float &tOut, Idx &leaf, uint32_t *passes = nullptr) {
     Ray<Idx> r;
     ray_init(r, tree, ox, oy, oz, dx, dy, dz, rayScale);
     if (!Finite && !ray_defined(r)) return kMiss;  // the reference never ends such a ray
     int code;
     do {
         code = ray_step<Lod, Finite, Idx, Normal, Reg>(r, tree, stk, normal, tOut, leaf);
+        if (Count) ++*passes;
     } while (code == kRunning);
     return code;
 }
@@ -161,38 +163,45 @@ __global__ void __launch_bounds__(64) primary_ordered_kernel(const uint32_t *__r
 // tail instead of two (8192^3 terrain at 3840x2160: 1.644 ms against 1.779 ms
 // for a primary launch then a shadow launch, each split in two halves over two
 // streams).
-template <typename Idx, int Reg>
+//
+// Planned: the tiles in a tile plan's cost order (tile_order.hpp), each filed
+// by its primary plus shadow passes for the plan's next launch.
+template <typename Idx, int Reg, bool Planned = false>
 __global__ void __launch_bounds__(64) primary_shadow_kernel(const uint32_t *__restrict__ tree, PrimaryParams p,
                                                             float lx, float ly, float lz, float eps, int slots,
                                                             uint8_t *hit, uint32_t *nrm, float *tt, uint64_t *lf,
-                                                            uint8_t *__restrict__ shadow) {
+                                                            uint8_t *__restrict__ shadow, TileOrder o) {
     extern __shared__ __align__(16) unsigned char lds[];
-    const int tile = int(blockIdx.x);
     const int lane = threadIdx.x;
+    const int tile = Planned ? ordered_tile(o, int(blockIdx.x), lane) : int(blockIdx.x);
     const int x = (tile % p.tilesX) * 8 + (lane & 7);
     const int y = (tile / p.tilesX) * 8 + (lane >> 3);
-    if (x >= p.width || y >= p.height) return;
-    float d[3];
-    primary_dir(p, x, y, d);
-    Stack<Idx> stk(lds, 64, slots, lane);
-    uint32_t normal = 0;
-    float t = 0.0f;
-    Idx leaf = 0;
-    const int code = march_ray<false, true, Idx, true, Reg>(tree, p.px, p.py, p.pz, d[0], d[1], d[2], 0.0f, stk, normal,
-                                                            t, leaf);
-    const int64_t i = int64_t(y) * p.width + x;
-    store_result<Idx>(code, normal, t, leaf, i, hit, nrm, tt, lf);
-    uint8_t sh = 0;
-    if (code == 1) {
-        const float back = t - eps;
-        uint32_t n2 = 0;
-        float t2 = 0.0f;
-        Idx leaf2 = 0;
-        const int c2 = march_ray<false, true, Idx, false, Reg>(tree, p.px + d[0] * back, p.py + d[1] * back,
-                                                               p.pz + d[2] * back, lx, ly, lz, 0.0f, stk, n2, t2, leaf2);
-        sh = (c2 == 1 || c2 == 2) ? 1 : 0;
+    uint32_t passes = 0;
+    if (x < p.width && y < p.height) {
+        float d[3];
+        primary_dir(p, x, y, d);
+        Stack<Idx> stk(lds, 64, slots, lane);
+        uint32_t normal = 0;
+        float t = 0.0f;
+        Idx leaf = 0;
+        const int code = march_ray<false, true, Idx, true, Reg, Planned>(tree, p.px, p.py, p.pz, d[0], d[1], d[2], 0.0f,
+                                                                         stk, normal, t, leaf, &passes);
+        const int64_t i = int64_t(y) * p.width + x;
+        store_result<Idx>(code, normal, t, leaf, i, hit, nrm, tt, lf);
+        uint8_t sh = 0;
+        if (code == 1) {
+            const float back = t - eps;
+            uint32_t n2 = 0;
+            float t2 = 0.0f;
+            Idx leaf2 = 0;
+            const int c2 = march_ray<false, true, Idx, false, Reg, Planned>(tree, p.px + d[0] * back, p.py + d[1] * back,
+                                                                            p.pz + d[2] * back, lx, ly, lz, 0.0f, stk,
+                                                                            n2, t2, leaf2, &passes);
+            sh = (c2 == 1 || c2 == 2) ? 1 : 0;
+        }
+        shadow[i] = sh;
     }
-    shadow[i] = sh;
+    if (Planned) file_tile(o, int(blockIdx.x), tile, passes, lane);
 }
 
 int stack_slots(const svo_octree &h, int reg) { return lds_slots(h.depth, reg); }
@@ -436,9 +445,11 @@ int render_primary_device(svo_octree &h, const svo_camera &cam, int width, int h
     return SVO_OK;
 }
 
-// Primary pass and one shadow ray per leaf hit, both in primary_shadow_kernel.
+// Primary pass and one shadow ray per leaf hit, both in primary_shadow_kernel;
+// with `plan` (made by tile_plan_create for the image size) in its cost order.
 int render_shadow_device(svo_octree &h, const svo_camera &cam, int width, int height, const float light[3], float eps,
-                         uint8_t *hit, uint32_t *normal, float *t, uint64_t *leaf, uint8_t *shadow, void *stream) {
+                         uint8_t *hit, uint32_t *normal, float *t, uint64_t *leaf, uint8_t *shadow, void *stream,
+                         svo_tile_plan *plan) {
     if (!hit || !t || !shadow || !light) return fail(SVO_ERR_INVALID, "shadow pass needs hit, t, shadow and light");
     for (int k = 0; k < 3; ++k)
         if (!(std::fabs(light[k]) <= 1e6f)) return fail(SVO_ERR_INVALID, "light direction must be finite");
@@ -449,6 +460,13 @@ int render_shadow_device(svo_octree &h, const svo_camera &cam, int width, int he
     if (width <= 0 || height <= 0 || width > 65536 || height > 65536)
         return fail(SVO_ERR_INVALID, "image size out of range");
     if (!camera_ok(cam)) return fail(SVO_ERR_INVALID, "camera entries must be finite and within +-1e6");
+    if (plan) {
+        if (plan->kind != svo_tile_plan::kPrimary || width != plan->width || height != plan->height)
+            return fail(SVO_ERR_INVALID, "the tile plan was made for another image size or render entry");
+        int dev = -1;
+        HIP_TRY(hipGetDevice(&dev));
+        if (dev != plan->device) return fail(SVO_ERR_INVALID, "the tile plan lives on another device");
+    }
     int r = ensure_device(h);
     if (r) return r;
     const PrimaryParams p = primary_params(cam, width, height);
@@ -456,19 +474,35 @@ int render_shadow_device(svo_octree &h, const svo_camera &cam, int width, int he
     const TileLaunch L = tile_launch(h);
     hipStream_t s = static_cast<hipStream_t>(stream);
     const dim3 grid(static_cast<unsigned>(tiles)), block(64);
-    if (!L.wide && L.reg == 1)
+    const TileOrder o = plan ? next_order(*plan) : TileOrder{};
+    if (plan) {
+        if (!L.wide && L.reg == 1)
+            hipLaunchKernelGGL((primary_shadow_kernel<uint32_t, 1, true>), grid, block, L.lds, s, h.dwords, p, light[0],
+                               light[1], light[2], eps, L.slots, hit, normal, t, leaf, shadow, o);
+        else if (!L.wide)
+            hipLaunchKernelGGL((primary_shadow_kernel<uint32_t, 3, true>), grid, block, L.lds, s, h.dwords, p, light[0],
+                               light[1], light[2], eps, L.slots, hit, normal, t, leaf, shadow, o);
+        else if (L.reg == 1)
+            hipLaunchKernelGGL((primary_shadow_kernel<uint64_t, 1, true>), grid, block, L.lds, s, h.dwords, p, light[0],
+                               light[1], light[2], eps, L.slots, hit, normal, t, leaf, shadow, o);
+        else
+            hipLaunchKernelGGL((primary_shadow_kernel<uint64_t, 3, true>), grid, block, L.lds, s, h.dwords, p, light[0],
+                               light[1], light[2], eps, L.slots, hit, normal, t, leaf, shadow, o);
+    } else if (!L.wide && L.reg == 1) {
         hipLaunchKernelGGL((primary_shadow_kernel<uint32_t, 1>), grid, block, L.lds, s, h.dwords, p, light[0], light[1],
-                           light[2], eps, L.slots, hit, normal, t, leaf, shadow);
-    else if (!L.wide)
+                           light[2], eps, L.slots, hit, normal, t, leaf, shadow, o);
+    } else if (!L.wide) {
         hipLaunchKernelGGL((primary_shadow_kernel<uint32_t, 3>), grid, block, L.lds, s, h.dwords, p, light[0], light[1],
-                           light[2], eps, L.slots, hit, normal, t, leaf, shadow);
-    else if (L.reg == 1)
+                           light[2], eps, L.slots, hit, normal, t, leaf, shadow, o);
+    } else if (L.reg == 1) {
         hipLaunchKernelGGL((primary_shadow_kernel<uint64_t, 1>), grid, block, L.lds, s, h.dwords, p, light[0], light[1],
-                           light[2], eps, L.slots, hit, normal, t, leaf, shadow);
-    else
+                           light[2], eps, L.slots, hit, normal, t, leaf, shadow, o);
+    } else {
         hipLaunchKernelGGL((primary_shadow_kernel<uint64_t, 3>), grid, block, L.lds, s, h.dwords, p, light[0], light[1],
-                           light[2], eps, L.slots, hit, normal, t, leaf, shadow);
+                           light[2], eps, L.slots, hit, normal, t, leaf, shadow, o);
+    }
     HIP_TRY(hipGetLastError());
+    if (plan) plan->renders++;
     return SVO_OK;
 }
 

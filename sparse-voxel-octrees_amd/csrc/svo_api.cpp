@@ -430,6 +430,16 @@ int svo_render_shadow_device(svo_octree *h, const svo_camera *cam, int width, in
     });
 }
 
+int svo_render_shadow_planned_device(svo_octree *h, svo_tile_plan *plan, const svo_camera *cam, int width, int height,
+                                     const float light[3], float eps, uint8_t *hit, uint32_t *normal, float *t,
+                                     uint64_t *leaf, uint8_t *shadow, void *stream) {
+    return guarded([&]() -> int {
+        clear_error();
+        if (!h || !cam || !plan) return fail(SVO_ERR_INVALID, "null argument");
+        return render_shadow_device(*h, *cam, width, height, light, eps, hit, normal, t, leaf, shadow, stream, plan);
+    });
+}
+
 int svo_render_shadow(svo_octree *h, const svo_camera *cam, int width, int height, const float light[3], float eps,
                       uint8_t *hit, uint32_t *normal, float *t, uint64_t *leaf, uint8_t *shadow) {
     return guarded([&]() -> int {

@@ -153,7 +153,8 @@ int raymarch_one(const svo_octree &h, const float o[3], const float d[3], float 
                  uint64_t &leaf);
 
 int render_shadow_device(svo_octree &h, const svo_camera &cam, int width, int height, const float light[3], float eps,
-                         uint8_t *hit, uint32_t *normal, float *t, uint64_t *leaf, uint8_t *shadow, void *stream);
+                         uint8_t *hit, uint32_t *normal, float *t, uint64_t *leaf, uint8_t *shadow, void *stream,
+                         svo_tile_plan *plan = nullptr);
 
 // ---- viewer frame (frame.hip) ----
 int render_frame_device(svo_octree &h, const svo_camera &cam, int width, int height, int stripes, int stride,

@@ -33,7 +33,7 @@ EXPORTS = [
     "svo_build_from_sdf", "svo_sdf_volume", "svo_serve_replay",
     "svo_render_frame", "svo_render_frame_device", "svo_render_frames_device", "svo_frame_depth_count",
     "svo_build_from_terrain", "svo_terrain_volume", "svo_terrain_heights",
-    "svo_render_shadow", "svo_render_shadow_device", "svo_decode_words",
+    "svo_render_shadow", "svo_render_shadow_device", "svo_render_shadow_planned_device", "svo_decode_words",
     "svo_renderer_autotune", "svo_batch_graph_create", "svo_batch_graph_launch", "svo_batch_graph_free",
     "svo_server_create", "svo_server_submit", "svo_server_poll", "svo_server_wait", "svo_server_outputs",
     "svo_server_copy_outputs", "svo_server_release", "svo_server_stats_get", "svo_server_run_schedule",
@@ -334,6 +334,7 @@ def _load():
         "svo_batch_graph_free": [P],
         "svo_render_shadow": [P, P, ctypes.c_int, ctypes.c_int, P, F, P, P, P, P, P],
         "svo_render_shadow_device": [P, P, ctypes.c_int, ctypes.c_int, P, F, P, P, P, P, P, P],
+        "svo_render_shadow_planned_device": [P, P, P, ctypes.c_int, ctypes.c_int, P, F, P, P, P, P, P, P],
         "svo_server_create": [P, ctypes.c_int, P, P], "svo_server_submit": [P, P, P],
         "svo_server_poll": [P, U64, P, P], "svo_server_wait": [P, U64, ctypes.c_double, P],
         "svo_server_outputs": [P, U64, P, P, P], "svo_server_release": [P, U64],
@@ -621,10 +622,17 @@ class VoxelOctree:
         return out
 
     def render_shadow_device(self, cam, width, height, light, eps, hit_ptr, t_ptr, shadow_ptr, normal_ptr=None,
-                             leaf_ptr=None, stream=None):
+                             leaf_ptr=None, stream=None, plan=None):
+        """Primary + shadow rays on the device; with `plan` (a TilePlan of the
+        image size, kept for shadow renders) in the plan's cost order."""
         lt = np.ascontiguousarray(light, np.float32)
-        _check(lib.svo_render_shadow_device(self._h, ctypes.byref(cam), int(width), int(height), _p(lt), float(eps),
-                                            hit_ptr, normal_ptr, t_ptr, leaf_ptr, shadow_ptr, stream))
+        if plan is None:
+            _check(lib.svo_render_shadow_device(self._h, ctypes.byref(cam), int(width), int(height), _p(lt),
+                                                float(eps), hit_ptr, normal_ptr, t_ptr, leaf_ptr, shadow_ptr, stream))
+        else:
+            _check(lib.svo_render_shadow_planned_device(self._h, plan._p, ctypes.byref(cam), int(width), int(height),
+                                                        _p(lt), float(eps), hit_ptr, normal_ptr, t_ptr, leaf_ptr,
+                                                        shadow_ptr, stream))
 
     def render_frame(self, cam, width, height, stripes=16, stride=1):
         """The viewer's shaded frame (src/Main.cpp:81-202): (rgba uint32

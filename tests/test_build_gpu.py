@@ -307,6 +307,41 @@ def test_shadow_rays_match_oracle(svo, n, seed, w, h):
         assert e.value.code == svo.svo.ERR_INVALID
 
 
+def test_planned_shadow_renders_equal_unplanned(svo):
+    """svo_render_shadow_planned_device: primary + shadow renders through one
+    tile plan (the first in raster order, the next ones in the cost order the
+    previous launch filed from its primary plus shadow passes), over five
+    camera moves, give hit, normal, t, leaf and shadow arrays equal to the
+    unplanned renders; a plan for another size or render entry is refused."""
+    import torch
+    n, seed, w, h = 1024, 33, 1283, 723
+    t = svo.VoxelOctree.build_from_terrain(n, seed)
+    pitch, yaw, light = svo.scenes.terrain_view(seed)
+    eps = 0.25 / n
+    m = w * h
+    plan = svo.svo.TilePlan(w, h)
+    stream = torch.cuda.Stream()
+    bufs = [[torch.full((m,), fill, dtype=dt, device="cuda") for dt, fill in
+             ((torch.uint8, 7), (torch.int32, 7), (torch.float32, -1.0), (torch.int64, 7), (torch.uint8, 7))]
+            for _ in range(2)]
+    for k in range(5):
+        cam = t.orbit_camera(pitch + 2.0 * k, yaw + 9.0 * k, 1.0)
+        print(f"[gpu] planned shadow render {k} {w}x{h}", flush=True)
+        for j, pl in enumerate((plan, None)):
+            hit, nrm, tt, lf, sh = bufs[j]
+            t.render_shadow_device(cam, w, h, light, eps, hit.data_ptr(), tt.data_ptr(), sh.data_ptr(), nrm.data_ptr(),
+                                   lf.data_ptr(), stream.cuda_stream, plan=pl)
+        stream.synchronize()
+        for a, b in zip(bufs[0], bufs[1]):
+            assert torch.equal(a.view(torch.uint8), b.view(torch.uint8)), k
+    cam = t.orbit_camera(pitch, yaw, 1.0)
+    for other in (svo.svo.TilePlan(w + 8, h), svo.svo.TilePlan.for_frame(w, h, 16)):
+        with pytest.raises(svo.SvoError) as e:
+            t.render_shadow_device(cam, w, h, light, eps, bufs[0][0].data_ptr(), bufs[0][2].data_ptr(),
+                                   bufs[0][4].data_ptr(), stream=stream.cuda_stream, plan=other)
+        assert e.value.code == svo.svo.ERR_INVALID
+
+
 def test_terrain_full_size(svo):
     """8192^3 terrain (exposed shell of a seeded fBm heightfield, 512^3 cache
     blocks under a 1 GiB budget), built on the GPU; 3840x2160 primary rays plus
