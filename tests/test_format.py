@@ -422,6 +422,8 @@ def test_tile_plans_refuse_bad_arguments_before_device_use(svo, golden):
         with pytest.raises(svo.SvoError) as e:
             m.TilePlan.for_frames(64, 64, nframes)
         assert e.value.code == m.ERR_INVALID, nframes
+    out = ctypes.c_void_p()
+    assert m.lib.svo_tile_plan_create_frame(320, 100, 16, ctypes.byref(out)) == m.ERR_INVALID and not out.value
     t = svo.VoxelOctree(os.path.join(golden, "dragon.oct"))
     cam = t.orbit_camera(30.0, 10.0, 1.0)
     for views, tile_w in [([], 0), ([m.Renderer.view(t, cam, 64, 64)], 5), ([m.Renderer.view(t, cam, 0, 64)], 0)]:
@@ -432,6 +434,13 @@ def test_tile_plans_refuse_bad_arguments_before_device_use(svo, golden):
                                                    None) == m.ERR_INVALID
     assert m.lib.svo_render_frame_planned_device(t.handle, None, ctypes.byref(cam), 64, 64, 16, 1, None, None,
                                                  None) == m.ERR_INVALID
+    cams = (m.Camera * 1)(cam)
+    assert m.lib.svo_render_frames_planned_device(t.handle, None, cams, 1, 64, 64, 16, 1, None, None,
+                                                  None) == m.ERR_INVALID
+    light = np.array([0.0, 1.0, 0.0], np.float32)
+    assert m.lib.svo_render_shadow_planned_device(t.handle, None, ctypes.byref(cam), 64, 64,
+                                                  light.ctypes.data_as(ctypes.c_void_p), 0.001, None, None, None,
+                                                  None, None, None) == m.ERR_INVALID
 
 
 def test_frames_call_refuses_bad_arguments_before_device_use(svo, golden):
