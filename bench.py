@@ -645,10 +645,11 @@ def reference_frame(tree, angle, width, height, threads, cache_dir):
 def run_served(pkg, trees, rates, seconds, seed=2000, full=False, waves_per_cu=0):
     """Served stream over the 64 resident octrees: seeded Poisson request
     streams, each request (octree U{0..63}, seeded orbit camera, resolution as
-    the mixed views). For each mean rate the live server (svo_server_*: one
-    running persistent launch, requests folded in as they arrive; a request
-    arriving with nothing in flight runs as its own launch; outputs left in
-    device memory) and the one-request-per-launch baseline (plain
+    the mixed views). For each mean rate the live server (svo_server_*:
+    requests run as their own launches while few are in flight, the others
+    together in batched launches; outputs left in device memory), the same
+    server with its persistent launch (requests beyond four in flight folded
+    into one running launch) and the one-request-per-launch baseline (plain
     kernel, outputs in device memory) replay the same schedule; latency is
     scheduled arrival -> completion seen on the host. Requests per rate:
     min(5000, rate * seconds), or 5000 with `full`. A final burst (every
@@ -669,10 +670,13 @@ def run_served(pkg, trees, rates, seconds, seed=2000, full=False, waves_per_cu=0
                 "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
                 "grays_s": round(rays / mk / 1e9, 4), "requests": len(reqs)}
 
-    srv = pkg.svo.Server(trees, output_budget=4 << 30, waves_per_cu=waves_per_cu)
-    # first use of the serving kernel, outside the measurements
+    srv = pkg.svo.Server(trees, output_budget=4 << 30)
+    # the same server with the persistent launch instead of batched launches
+    srvp = pkg.svo.Server(trees, output_budget=4 << 30, waves_per_cu=waves_per_cu, persistent_launch=True)
+    # first use of the serving kernels, outside the measurements
     warm = make(8)
     srv.run_schedule(warm, np.zeros(len(warm)))
+    srvp.run_schedule(warm, np.zeros(len(warm)))
     for lam in rates:
         n = 5000 if full else int(max(200, min(5000, lam * seconds)))
         reqs = make(n)
@@ -680,23 +684,31 @@ def run_served(pkg, trees, rates, seconds, seed=2000, full=False, waves_per_cu=0
         arr -= arr[0]
         lat_s, mk_s, _ = srv.run_schedule(reqs, arr)
         progress(f"served {lam:g} req/s: live server, {n} requests")
+        lat_b, mk_b, _ = srvp.run_schedule(reqs, arr)
+        progress(f"served {lam:g} req/s: live server with the persistent launch")
         lat_p, mk_p = pkg.svo.serve_replay(trees, reqs, arr, streams=1, one_per_launch=True, device_outputs=True)
         progress(f"served {lam:g} req/s: one per launch")
         out["rates"].append({"req_per_s": lam, "live_server": stats(lat_s, mk_s, reqs),
+                             "live_server_persistent_launch": stats(lat_b, mk_b, reqs),
                              "one_per_launch": stats(lat_p, mk_p, reqs)})
     reqs = make(1000)
     arr = np.zeros(len(reqs))
     lat_s, mk_s, _ = srv.run_schedule(reqs, arr)
     st = srv.stats()
     srv.close()
+    lat_b, mk_b, _ = srvp.run_schedule(reqs, arr)
+    stp = srvp.stats()
+    srvp.close()
     lat_p, mk_p = pkg.svo.serve_replay(trees, reqs, arr, streams=1, one_per_launch=True, device_outputs=True)
     lat_r, mk_r = pkg.svo.serve_replay(trees, reqs, arr, streams=4, device_outputs=True)
     s_, p_, r_ = stats(lat_s, mk_s, reqs), stats(lat_p, mk_p, reqs), stats(lat_r, mk_r, reqs)
-    out["burst"] = {"live_server": s_, "one_per_launch": p_, "relaunch_batched_4_streams": r_,
+    out["burst"] = {"live_server": s_, "live_server_persistent_launch": stats(lat_b, mk_b, reqs),
+                    "one_per_launch": p_, "relaunch_batched_4_streams": r_,
                     "live_vs_one_per_launch": round(s_["grays_s"] / p_["grays_s"], 3)}
-    out["server_launches"] = st["launches"]
-    out["server_direct_launches"] = st["direct"]
-    out["server_waves_per_cu"] = waves_per_cu or 24
+    out["server"] = {"direct_requests": st["direct"], "batched_requests": st["batched"],
+                     "persistent_launches": st["launches"]}
+    out["persistent_server"] = {"direct_requests": stp["direct"], "persistent_launches": stp["launches"],
+                                "waves_per_cu": waves_per_cu or 24}
     return out
 
 
@@ -725,7 +737,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=4.0, help="schedule length per rate (at most 5000 requests)")
     ap.add_argument("--serve-full", action="store_true", help="5000 requests at every rate")
     ap.add_argument("--serve-waves-per-cu", type=int, default=0,
-                    help="live server: wave slots per CU its launch holds (0 = the library's default, 24)")
+                    help="live server with the persistent launch: wave slots per CU it holds (0 = the library's "
+                         "default, 24)")
     ap.add_argument("--no-terrain", action="store_true", help="skip the 8192^3 terrain + shadow-ray workload")
     ap.add_argument("--no-frames", action="store_true", help="skip the viewer-frame workload")
     ap.add_argument("--only", choices=["terrain", "frames", "build", "meshbig"], default=None,

@@ -1,12 +1,18 @@
-// Live request server: continuous batching of render requests into one
-// running persistent launch.
+// Live request server: continuous batching of render requests.
 //
 // A caller submits render requests (octree id, camera, resolution) at any
-// time from any thread. The server keeps the octrees resident, admits
+// time from any thread. The server keeps the octrees resident and admits
 // requests in arrival order while their outputs (normal 4 B, t 4 B, leaf 8 B
-// per ray) fit the output arena and a ring slot is free (admission by HBM
-// footprint), and publishes each admitted request to a ring in pinned,
-// coherent host memory. One persistent kernel runs while there is work:
+// per ray) fit the output arena (admission by HBM footprint). A request
+// admitted while fewer than four launches are in flight runs as an ordinary
+// primary launch on a stream of its own (one of four); requests arriving
+// while four are in flight wait, and the next stream that frees up takes all
+// waiting requests (up to 256, while their outputs fit) in one batched
+// grid-form launch (render_batch). A launch's completion is an event.
+//
+// With persistent_launch, the requests beyond four in flight go instead to
+// one running persistent launch, published through a ring in pinned,
+// coherent host memory:
 //
 //  - a leader wave (wave 0 of workgroup 0) polls the ring's tail, copies each
 //    new request's view descriptor into device memory, extends the launch's
@@ -23,18 +29,17 @@
 //    writes the request's sequence number into a completion word in host
 //    memory.
 //
-// A request admitted while no persistent launch runs and fewer than four
-// requests are in flight is traced by an ordinary primary launch on a stream
-// of its own instead, and its completion is an event: on a lightly loaded GPU
-// such launches end sooner than the persistent one would (all wave slots,
-// hardware dispatch, no claims or counters); beyond four requests in flight
-// the persistent launch takes over.
+// Measured on the 64-octree request stream, the batched launches beat the
+// persistent one (14.8 against 12.1 Grays/s under a 1000-request burst, p99
+// 0.38 against 0.58 ms at 2000 req/s): hardware dispatch of the tiles, all
+// wave slots, no claims or done counters, and outputs in the caches until the
+// launch ends.
 //
 // The host side is a request loop on its own thread: it records completions
 // (latency = submit -> completion seen on the host), admits waiting requests
-// as memory and ring slots free up, keeps a heartbeat the kernel checks, and
-// ends the launch when no request has been in flight for `idle_stop_us`
-// (the next admission starts a new one). Work the caller synchronises with
+// as memory, streams and ring slots free up, keeps a heartbeat the persistent
+// kernel checks, and ends a persistent launch when no request has been in
+// flight for `idle_stop_us` (the next admission starts a new one). Work the caller synchronises with
 // device-wide (hipDeviceSynchronize, hipFree) therefore waits at most until
 // the requests in flight are done and the launch has ended.
 //
@@ -399,9 +404,13 @@ struct svo_server {
     std::deque<uint64_t> waiting;        // submitted, not yet admitted
     std::deque<uint64_t> inflight;       // admitted to the persistent launch, in sequence order
     std::deque<std::pair<uint64_t, hipEvent_t>> direct;  // running as their own launch
-    std::vector<hipEvent_t> events;      // free completion events of direct launches
-    uint64_t directLaunches = 0;
-    static constexpr size_t kMaxDirect = 4;  // direct launches in flight at most
+    std::deque<std::pair<std::vector<uint64_t>, hipEvent_t>> groups;  // running in one batched launch each
+    std::vector<hipEvent_t> events;      // free completion events of direct and batched launches
+    uint64_t directLaunches = 0, groupLaunches = 0, grouped = 0;
+    static constexpr size_t kMaxDirect = 4;  // direct and batched launches in flight at most
+    static constexpr int kMaxGroup = 256;    // requests per batched launch
+    bool batchedLaunches = false;
+    svo_renderer *renderers[4] = {};     // batched launches, one per stream
     std::vector<uint64_t> slotTicket;    // ring slot -> ticket
     uint64_t nextTicket = 1, nextSeq = 0;
     bool live = false;
@@ -521,7 +530,7 @@ bool launch_direct(svo_server &s, uint64_t id) {
     uint64_t *leaf;
     request_outputs(s, tk, normal, t, leaf);
     OnDevice on(s.device);
-    hipStream_t st = s.directStreams[s.directLaunches % svo_server::kMaxDirect];
+    hipStream_t st = s.directStreams[(s.directLaunches + s.groupLaunches) % svo_server::kMaxDirect];
     int r = render_primary_device(*s.trees[size_t(tk.req.tree)], tk.req.camera, tk.req.width, tk.req.height, nullptr,
                                   normal, t, leaf, st);
     hipError_t e = r ? hipSuccess : hipEventRecord(ev, st);
@@ -541,20 +550,95 @@ bool launch_direct(svo_server &s, uint64_t id) {
     return true;
 }
 
+// With batched launches: the waiting requests at the front of the queue
+// (up to kMaxGroup, while their outputs fit the arena) run in one batched
+// grid-form launch (render_batch) on the next stream; its completion is an
+// event. Returns false when nothing could be admitted or on an error (lock
+// held).
+bool launch_group(svo_server &s) {
+    std::vector<uint64_t> ids;
+    std::vector<svo_view> views;
+    const size_t most = std::min<size_t>(s.waiting.size(), size_t(svo_server::kMaxGroup));
+    for (size_t k = 0; k < most; ++k) {
+        Ticket &tk = s.tickets[s.waiting[k]];
+        uint64_t off = 0;
+        if (!s.alloc.alloc(tk.bytes, off)) break;  // admission by output footprint, in arrival order
+        tk.off = off;
+        ids.push_back(s.waiting[k]);
+        svo_view v{};
+        v.tree = s.trees[size_t(tk.req.tree)];
+        v.camera = tk.req.camera;
+        v.width = tk.req.width;
+        v.height = tk.req.height;
+        request_outputs(s, tk, v.normal, v.t, v.leaf);
+        views.push_back(v);
+    }
+    if (ids.empty()) return false;
+    auto release = [&]() {
+        for (uint64_t id : ids) s.alloc.release(s.tickets[id].off, s.tickets[id].bytes);
+    };
+    if (ids.size() == 1) {  // one request: the plain kernel
+        release();
+        if (!launch_direct(s, ids[0])) return false;
+        s.waiting.pop_front();
+        return true;
+    }
+    hipEvent_t ev = nullptr;
+    if (!s.events.empty()) {
+        ev = s.events.back();
+        s.events.pop_back();
+    } else if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+        release();
+        set_error(s, "server: event for a batched launch");
+        return false;
+    }
+    OnDevice on(s.device);
+    const size_t k = (s.directLaunches + s.groupLaunches) % svo_server::kMaxDirect;
+    hipStream_t st = s.directStreams[k];
+    int r = render_batch(s.renderers[k], views.data(), int(views.size()), nullptr, st);
+    hipError_t e = r ? hipSuccess : hipEventRecord(ev, st);
+    if (r || e != hipSuccess) {
+        release();
+        s.events.push_back(ev);
+        set_error(s, r ? std::string("server: batched launch: ") + svo_last_error()
+                       : std::string("server: batched launch: ") + hipGetErrorString(e));
+        return false;
+    }
+    for (uint64_t id : ids) {
+        s.tickets[id].state = 1;
+        s.waiting.pop_front();  // ids are the queue's front, in order
+    }
+    s.published += ids.size();
+    s.grouped += ids.size();
+    s.groupLaunches++;
+    s.groups.emplace_back(std::move(ids), ev);
+    s.lastBusy = Clock::now();
+    s.loopCv.notify_one();  // the loop watches the launch
+    return true;
+}
+
 // Admits waiting requests in arrival order while arena memory, a ring slot
 // and room in the launch's request table are free, and publishes them (to a
-// direct launch when nothing is in flight).
+// direct launch when nothing is in flight; with batched launches, to direct
+// and batched launches only).
 void admit(svo_server &s) {
     if (s.error) return;
     while (!s.waiting.empty()) {
         Ticket &tk = s.tickets[s.waiting.front()];
-        if (!s.persistentOnly && !s.live && s.inflight.empty() && s.direct.size() < svo_server::kMaxDirect) {
-            // no persistent launch and few requests in flight: the request
-            // runs as its own launch
+        if (!s.persistentOnly && !s.live && s.inflight.empty() &&
+            s.direct.size() + s.groups.size() < svo_server::kMaxDirect) {
+            // no persistent launch and few launches in flight: the request
+            // runs as its own launch (with batched launches, the waiting
+            // requests together)
+            if (s.batchedLaunches && s.waiting.size() > 1) {
+                if (!launch_group(s)) return;
+                continue;
+            }
             if (!launch_direct(s, s.waiting.front())) return;
             s.waiting.pop_front();
             continue;
         }
+        if (s.batchedLaunches && !s.persistentOnly) return;  // the next free stream takes them
         // ring window: the oldest request in flight holds its slot
         const uint64_t oldest = s.inflight.empty() ? s.nextSeq : s.tickets[s.inflight.front()].seq;
         if (s.nextSeq - oldest >= uint64_t(s.ring)) return;
@@ -612,6 +696,23 @@ bool reap(svo_server &s) {
         tk.completed = now_s(s);
         s.events.push_back(it->second);
         it = s.direct.erase(it);
+        any = true;
+    }
+    for (auto it = s.groups.begin(); it != s.groups.end();) {
+        const hipError_t e = hipEventQuery(it->second);
+        if (e == hipErrorNotReady) {
+            ++it;
+            continue;
+        }
+        if (e != hipSuccess) set_error(s, std::string("server: batched launch: ") + hipGetErrorString(e));
+        const double now = now_s(s);
+        for (uint64_t id : it->first) {
+            Ticket &tk = s.tickets[id];
+            tk.state = 2;
+            tk.completed = now;
+        }
+        s.events.push_back(it->second);
+        it = s.groups.erase(it);
         any = true;
     }
     for (auto it = s.inflight.begin(); it != s.inflight.end();) {
@@ -693,7 +794,7 @@ void loop_main(svo_server *sp) {
             admit(s);
         }
         if (s.error) s.cv.notify_all();
-        if (s.live || !s.direct.empty()) {
+        if (s.live || !s.direct.empty() || !s.groups.empty()) {
             // spin briefly without the lock, so submitters and pollers get in
             lock.unlock();
             for (int k = 0; k < 64; ++k) __builtin_ia32_pause();
@@ -729,6 +830,9 @@ void free_server(svo_server *s) {
     for (hipStream_t ds : s->directStreams)
         if (ds) (void)hipStreamSynchronize(ds);
     for (auto &d : s->direct) s->events.push_back(d.second);
+    for (auto &g : s->groups) s->events.push_back(g.second);
+    for (svo_renderer *rd : s->renderers)
+        if (rd) renderer_free(rd);
     for (hipEvent_t ev : s->events) (void)hipEventDestroy(ev);
     for (hipStream_t ds : s->directStreams)
         if (ds) (void)hipStreamDestroy(ds);
@@ -774,6 +878,7 @@ int server_create(svo_octree *const *trees, int ntrees, const svo_server_config 
     s->idleStop = idle;
     s->wavesPerCU = waves;
     s->persistentOnly = cfg && cfg->persistent_only;
+    s->batchedLaunches = !(cfg && (cfg->persistent_launch || cfg->persistent_only));
     int maxDepth = 1;
     for (svo_octree *t : s->trees) {
         int r = ensure_device(*t);  // octrees stay resident while the server lives
@@ -809,6 +914,9 @@ int server_create(svo_octree *const *trees, int ntrees, const svo_server_config 
         if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->copyStream, hipStreamNonBlocking);
         for (hipStream_t &ds : s->directStreams)
             if (e == hipSuccess) e = hipStreamCreateWithFlags(&ds, hipStreamNonBlocking);
+        if (s->batchedLaunches)
+            for (svo_renderer *&rd : s->renderers)
+                if (e == hipSuccess && renderer_create(svo_server::kMaxGroup, &rd) != SVO_OK) e = hipErrorOutOfMemory;
         if (e == hipSuccess) e = hipEventCreateWithFlags(&s->ended, hipEventDisableTiming);
     }
     if (e != hipSuccess) {
@@ -944,10 +1052,13 @@ int server_stats(svo_server *s, svo_server_stats *out) {
     std::memset(out, 0, sizeof(*out));
     out->launches = s->launches;
     out->published = s->published;
-    out->in_flight = s->inflight.size() + s->direct.size();
+    uint64_t grouped = 0;
+    for (auto &g : s->groups) grouped += g.first.size();
+    out->in_flight = s->inflight.size() + s->direct.size() + grouped;
     out->waiting = s->waiting.size();
     out->live = s->live ? 1 : 0;
     out->direct = s->directLaunches;
+    out->batched = s->grouped;
     return SVO_OK;
 }
 

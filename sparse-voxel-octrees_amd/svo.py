@@ -177,13 +177,13 @@ def serve_replay(trees, requests, arrivals, streams=4, output_budget=1 << 30, ma
 class ServerConfig(ctypes.Structure):
     _fields_ = [("output_budget", ctypes.c_uint64), ("ring_slots", ctypes.c_int), ("waves_per_cu", ctypes.c_int),
                 ("host_outputs", ctypes.c_int), ("idle_stop_us", ctypes.c_int), ("persistent_only", ctypes.c_int),
-                ("reserved", ctypes.c_int * 5)]
+                ("persistent_launch", ctypes.c_int), ("reserved", ctypes.c_int * 4)]
 
 
 class ServerStats(ctypes.Structure):
     _fields_ = [("launches", ctypes.c_uint64), ("published", ctypes.c_uint64), ("in_flight", ctypes.c_uint64),
                 ("waiting", ctypes.c_uint64), ("live", ctypes.c_int), ("reserved0", ctypes.c_int),
-                ("direct", ctypes.c_uint64), ("reserved", ctypes.c_int * 4)]
+                ("direct", ctypes.c_uint64), ("batched", ctypes.c_uint64), ("reserved", ctypes.c_int * 2)]
 
 
 REQ_WAITING, REQ_IN_FLIGHT, REQ_DONE = 0, 1, 2
@@ -197,19 +197,21 @@ def _requests(requests):
 
 
 class Server:
-    """Live request server (svo_server): requests submitted at any time are
-    folded into one running persistent launch (a request arriving with nothing
-    in flight runs as its own launch unless persistent_only). requests are
-    (tree index, Camera, width, height) over the octrees given here."""
+    """Live request server (svo_server): requests submitted at any time run
+    as their own launches while few are in flight, the others together in
+    batched launches (persistent_launch: in one running persistent launch
+    instead; persistent_only: every request there). requests are (tree
+    index, Camera, width, height) over the octrees given here."""
 
     def __init__(self, trees, output_budget=0, ring_slots=0, waves_per_cu=0, host_outputs=False, idle_stop_us=0,
-                 persistent_only=False):
+                 persistent_only=False, persistent_launch=False):
         self._trees = list(trees)
         tarr = (ctypes.c_void_p * len(self._trees))(*[t.handle.value for t in self._trees])
         c = ServerConfig()
         c.output_budget, c.ring_slots, c.waves_per_cu = int(output_budget), int(ring_slots), int(waves_per_cu)
         c.host_outputs, c.idle_stop_us = int(bool(host_outputs)), int(idle_stop_us)
         c.persistent_only = int(bool(persistent_only))
+        c.persistent_launch = int(bool(persistent_launch))
         self._s = ctypes.c_void_p()
         _check(lib.svo_server_create(tarr, len(self._trees), ctypes.byref(c), ctypes.byref(self._s)))
 
@@ -250,7 +252,8 @@ class Server:
     def stats(self):
         st = ServerStats()
         _check(lib.svo_server_stats_get(self._s, ctypes.byref(st)))
-        return {k: getattr(st, k) for k in ("launches", "published", "in_flight", "waiting", "live", "direct")}
+        return {k: getattr(st, k) for k in ("launches", "published", "in_flight", "waiting", "live", "direct",
+                                            "batched")}
 
     def run_schedule(self, requests, arrivals, keep=False):
         """Submits request i at arrivals[i] seconds; returns (latencies,

@@ -1,7 +1,8 @@
 """GPU tests of the live request server (svo_server_*): requests submitted
-at any time, from several host threads, are folded into one running
-persistent launch; each result must equal the same request rendered alone by
-the plain kernel, bit for bit, and the oracle on sampled pixels.
+at any time, from several host threads, run as their own launches or
+together in batched launches (or, with persistent_launch, folded into one
+running persistent launch); each result must equal the same request rendered
+alone by the plain kernel, bit for bit, and the oracle on sampled pixels.
 """
 import os
 import threading
@@ -47,14 +48,17 @@ def _check(trees, req, out, oracle_every=0):
         assert np.array_equal(out["leaf"][sample], o["leaf"])
 
 
-@pytest.mark.parametrize("host_outputs", [False, True])
-def test_results_equal_alone_and_in_the_mix(svo, trees, host_outputs):
+@pytest.mark.parametrize("host_outputs,persistent", [(False, False), (True, False), (False, True), (True, True)])
+def test_results_equal_alone_and_in_the_mix(svo, trees, host_outputs, persistent):
     """240 requests submitted from 4 host threads at once (sizes 1x1 to
     400x300 over two octrees): every result equals the request rendered alone
-    and the oracle on every 13th pixel."""
+    and the oracle on every 13th pixel; by default no persistent launch runs
+    and the requests that queue up run in batched launches, with
+    persistent_launch they go to the persistent launch."""
     reqs = _requests(trees, 240, seed=5)
-    print(f"[gpu] live server, 4 submitting threads, host_outputs={host_outputs}", flush=True)
-    srv = svo.svo.Server(trees, output_budget=1 << 30, host_outputs=host_outputs)
+    print(f"[gpu] live server, 4 submitting threads, host_outputs={host_outputs}, persistent={persistent}",
+          flush=True)
+    srv = svo.svo.Server(trees, output_budget=1 << 30, host_outputs=host_outputs, persistent_launch=persistent)
     tickets = [None] * len(reqs)
     errors = []
 
@@ -80,18 +84,23 @@ def test_results_equal_alone_and_in_the_mix(svo, trees, host_outputs):
         srv.release(tk)
     st = srv.stats()
     assert st["published"] == len(reqs) and st["in_flight"] == 0
+    if persistent:
+        assert st["launches"] >= 1 and st["batched"] == 0
+    else:
+        assert st["launches"] == 0 and st["batched"] > 0 and st["direct"] + st["batched"] == len(reqs)
     srv.close()
 
 
-def test_admission_by_output_footprint(svo, trees):
+@pytest.mark.parametrize("persistent", [False, True])
+def test_admission_by_output_footprint(svo, trees, persistent):
     """An arena that holds about two 320x240 requests: 30 such requests wait
     for memory, are admitted as finished ones are released and all complete
     correctly; a request larger than the arena is refused."""
     budget = 2 * 320 * 240 * 16 + 4096
-    srv = svo.svo.Server(trees, output_budget=budget)
+    srv = svo.svo.Server(trees, output_budget=budget, persistent_launch=persistent)
     reqs = _requests(trees, 30, seed=11, wmax=2, hmax=2)
     reqs = [(ti, cam, 320, 240) for ti, cam, _, _ in reqs]
-    print("[gpu] live server admission by footprint", flush=True)
+    print(f"[gpu] live server admission by footprint, persistent={persistent}", flush=True)
     tickets = [srv.submit(*r) for r in reqs]
     assert srv.stats()["waiting"] > 0
     for r, tk in zip(reqs, tickets):
@@ -108,10 +117,11 @@ def test_admission_by_output_footprint(svo, trees):
 
 
 def test_launch_ends_when_idle_and_restarts(svo, trees):
-    """Requests beyond the four that run as their own launches go to the
-    persistent launch; it ends after idle_stop_us with nothing in flight, and
-    the next burst starts a new one; every result is traced correctly."""
-    srv = svo.svo.Server(trees, idle_stop_us=1000)
+    """With persistent_launch, requests beyond the four that run as their own
+    launches go to the persistent launch; it ends after idle_stop_us with
+    nothing in flight, and the next burst starts a new one; every result is
+    traced correctly."""
+    srv = svo.svo.Server(trees, idle_stop_us=1000, persistent_launch=True)
     reqs = _requests(trees, 16, seed=23)
     print("[gpu] live server idle stop / restart", flush=True)
 

@@ -201,9 +201,11 @@ def test_mixed_batch_matches_plain_and_oracle(svo, mixed):
     g.close()
 
 
-def test_served_stream_over_the_mixed_octrees(svo, mixed):
-    """The served workload through the live server: a seeded Poisson stream
-    at 2000 req/s of 500 requests (octree U{0..63}, seeded orbit camera,
+@pytest.mark.parametrize("persistent", [False, True])
+def test_served_stream_over_the_mixed_octrees(svo, mixed, persistent):
+    """The served workload through the live server (its default direct and
+    batched launches, or with persistent_launch): a seeded Poisson stream at
+    2000 req/s of 500 requests (octree U{0..63}, seeded orbit camera,
     resolution as the mixed views) plus a burst of 100 requests at once.
     Every served result equals the request rendered alone by the plain kernel
     on every ray and the oracle on every 97th pixel."""
@@ -216,8 +218,8 @@ def test_served_stream_over_the_mixed_octrees(svo, mixed):
     arr -= arr[0]
     arr = np.concatenate([arr, np.full(100, arr[-1] + 0.01)])
     budget = sum(w * h for _, _, w, h in reqs) * 16 + (1 << 26)
-    srv = svo.svo.Server(trees, output_budget=budget)
-    print(f"[gpu] served stream: 600 requests, arena {budget >> 20} MiB", flush=True)
+    srv = svo.svo.Server(trees, output_budget=budget, persistent_launch=persistent)
+    print(f"[gpu] served stream: 600 requests, arena {budget >> 20} MiB, persistent={persistent}", flush=True)
     lat, mk, tickets = srv.run_schedule(reqs, arr, keep=True)
     assert np.all(lat > 0) and mk > 0
     words = {}
