@@ -399,8 +399,8 @@ int svo_serve_replay(svo_octree *const *trees, int ntrees, const svo_request *re
 /* A request server keeps octrees resident and traces requests submitted at
  * any time, from any thread. By default a request runs as its own ordinary
  * launch while fewer than four launches are in flight (one of four streams);
- * requests that arrive meanwhile wait, and the next free stream takes all of
- * them (up to 256, while their outputs fit the arena) in one batched
+ * requests that arrive meanwhile wait, and the next free stream takes them
+ * (up to max_batch, while their outputs fit the arena) in one batched
  * grid-form launch. With persistent_launch, the requests beyond four in
  * flight go instead to one persistent launch that runs while work is in
  * flight: a leader wave copies newly published requests from a ring in
@@ -434,7 +434,8 @@ typedef struct {
     int persistent_only;    /* 1: every request goes through the persistent launch */
     int persistent_launch;  /* 1: requests beyond four in flight go to the persistent launch
                                instead of batched launches */
-    int reserved[4];
+    int max_batch;          /* requests per batched launch at most, 1..256 (0 = 32) */
+    int reserved[3];
 } svo_server_config;
 
 enum { SVO_REQ_WAITING = 0, SVO_REQ_IN_FLIGHT = 1, SVO_REQ_DONE = 2 };

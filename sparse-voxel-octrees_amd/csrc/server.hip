@@ -6,9 +6,10 @@
 // per ray) fit the output arena (admission by HBM footprint). A request
 // admitted while fewer than four launches are in flight runs as an ordinary
 // primary launch on a stream of its own (one of four); requests arriving
-// while four are in flight wait, and the next stream that frees up takes all
-// waiting requests (up to 256, while their outputs fit) in one batched
-// grid-form launch (render_batch). A launch's completion is an event.
+// while four are in flight wait, and the next stream that frees up takes the
+// waiting requests (up to max_batch, 32 by default, while their outputs fit)
+// in one batched grid-form launch (render_batch). A launch's completion is an
+// event.
 //
 // With persistent_launch, the requests beyond four in flight go instead to
 // one running persistent launch, published through a ring in pinned,
@@ -408,8 +409,11 @@ struct svo_server {
     std::vector<hipEvent_t> events;      // free completion events of direct and batched launches
     uint64_t directLaunches = 0, groupLaunches = 0, grouped = 0;
     static constexpr size_t kMaxDirect = 4;  // direct and batched launches in flight at most
-    static constexpr int kMaxGroup = 256;    // requests per batched launch
+    static constexpr int kMaxGroup = 256;    // the largest max_batch (the renderers' view tables)
     bool batchedLaunches = false;
+    int maxGroup = 32;                   // requests per batched launch at most (burst of 1000 over
+                                         // the 64 octrees: p50 22.8 ms at 32, 24-30 ms at 64-256,
+                                         // the same 14.5-14.7 Grays/s)
     svo_renderer *renderers[4] = {};     // batched launches, one per stream
     std::vector<uint64_t> slotTicket;    // ring slot -> ticket
     uint64_t nextTicket = 1, nextSeq = 0;
@@ -551,14 +555,14 @@ bool launch_direct(svo_server &s, uint64_t id) {
 }
 
 // With batched launches: the waiting requests at the front of the queue
-// (up to kMaxGroup, while their outputs fit the arena) run in one batched
+// (up to maxGroup, while their outputs fit the arena) run in one batched
 // grid-form launch (render_batch) on the next stream; its completion is an
 // event. Returns false when nothing could be admitted or on an error (lock
 // held).
 bool launch_group(svo_server &s) {
     std::vector<uint64_t> ids;
     std::vector<svo_view> views;
-    const size_t most = std::min<size_t>(s.waiting.size(), size_t(svo_server::kMaxGroup));
+    const size_t most = std::min<size_t>(s.waiting.size(), size_t(s.maxGroup));
     for (size_t k = 0; k < most; ++k) {
         Ticket &tk = s.tickets[s.waiting[k]];
         uint64_t off = 0;
@@ -866,6 +870,8 @@ int server_create(svo_octree *const *trees, int ntrees, const svo_server_config 
     const uint64_t budget = (cfg && cfg->output_budget) ? cfg->output_budget : (uint64_t(1) << 30);
     const int waves = (cfg && cfg->waves_per_cu > 0) ? cfg->waves_per_cu : 24;
     if (waves < 4 || waves > 32) return fail(SVO_ERR_INVALID, "server: waves_per_cu must be 4..32");
+    if (cfg && (cfg->max_batch < 0 || cfg->max_batch > svo_server::kMaxGroup))
+        return fail(SVO_ERR_INVALID, "server: max_batch must be 0..256");
     const double idle = (cfg && cfg->idle_stop_us > 0) ? cfg->idle_stop_us * 1e-6 : 2e-3;
     if (device_count() < 1) return fail(SVO_ERR_DEVICE, "server: no HIP device");
     svo_server *s = new (std::nothrow) svo_server();
@@ -879,6 +885,7 @@ int server_create(svo_octree *const *trees, int ntrees, const svo_server_config 
     s->wavesPerCU = waves;
     s->persistentOnly = cfg && cfg->persistent_only;
     s->batchedLaunches = !(cfg && (cfg->persistent_launch || cfg->persistent_only));
+    if (cfg && cfg->max_batch > 0) s->maxGroup = cfg->max_batch;
     int maxDepth = 1;
     for (svo_octree *t : s->trees) {
         int r = ensure_device(*t);  // octrees stay resident while the server lives

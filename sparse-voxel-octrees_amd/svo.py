@@ -177,7 +177,7 @@ def serve_replay(trees, requests, arrivals, streams=4, output_budget=1 << 30, ma
 class ServerConfig(ctypes.Structure):
     _fields_ = [("output_budget", ctypes.c_uint64), ("ring_slots", ctypes.c_int), ("waves_per_cu", ctypes.c_int),
                 ("host_outputs", ctypes.c_int), ("idle_stop_us", ctypes.c_int), ("persistent_only", ctypes.c_int),
-                ("persistent_launch", ctypes.c_int), ("reserved", ctypes.c_int * 4)]
+                ("persistent_launch", ctypes.c_int), ("max_batch", ctypes.c_int), ("reserved", ctypes.c_int * 3)]
 
 
 class ServerStats(ctypes.Structure):
@@ -204,7 +204,7 @@ class Server:
     index, Camera, width, height) over the octrees given here."""
 
     def __init__(self, trees, output_budget=0, ring_slots=0, waves_per_cu=0, host_outputs=False, idle_stop_us=0,
-                 persistent_only=False, persistent_launch=False):
+                 persistent_only=False, persistent_launch=False, max_batch=0):
         self._trees = list(trees)
         tarr = (ctypes.c_void_p * len(self._trees))(*[t.handle.value for t in self._trees])
         c = ServerConfig()
@@ -212,6 +212,7 @@ class Server:
         c.host_outputs, c.idle_stop_us = int(bool(host_outputs)), int(idle_stop_us)
         c.persistent_only = int(bool(persistent_only))
         c.persistent_launch = int(bool(persistent_launch))
+        c.max_batch = int(max_batch)
         self._s = ctypes.c_void_p()
         _check(lib.svo_server_create(tarr, len(self._trees), ctypes.byref(c), ctypes.byref(self._s)))
 
