@@ -443,6 +443,17 @@ def test_tile_plans_refuse_bad_arguments_before_device_use(svo, golden):
                                                   None, None, None) == m.ERR_INVALID
 
 
+def test_server_refuses_bad_configuration_before_device_use(svo, golden):
+    """svo_server_create checks waves_per_cu, ring_slots and max_batch before
+    it looks for a device or allocates anything."""
+    t = svo.VoxelOctree(os.path.join(golden, "dragon.oct"))
+    for kw in ({"waves_per_cu": 3}, {"waves_per_cu": 33}, {"ring_slots": (1 << 20) + 1}, {"max_batch": 257},
+               {"max_batch": -1}):
+        with pytest.raises(svo.SvoError) as e:
+            svo.svo.Server([t], **kw)
+        assert e.value.code == svo.svo.ERR_INVALID, kw
+
+
 def test_frames_call_refuses_bad_arguments_before_device_use(svo, golden):
     """svo_render_frames_device checks the frame count, the layout, the stride,
     every image pointer and every camera before anything touches a device."""
