@@ -610,6 +610,34 @@ def test_planned_frame_calls_equal_unplanned_calls(svo, trees):
     assert all(bool((x == 7).all()) for x in out)
 
 
+@pytest.mark.parametrize("name,w,h", [("dragon", 1, 1), ("dragon", 9, 7), ("empty", 64, 48), ("one", 33, 17),
+                                       ("sdf128", 520, 8)])
+def test_tile_plans_on_small_and_empty_images(svo, trees, golden, name, w, h):
+    """Tile plans at the edges: a 1x1 image (one tile: a plan of one
+    replica), images of fewer tiles than the 64 replicas, an octree without
+    leaves and a one-voxel octree, and a one-tile-high strip: three planned
+    primary renders (raster, then the filed order) equal the plain kernel."""
+    import torch
+    dev = torch.device("cuda", 0)
+    if name in trees:
+        t = trees[name]
+    else:
+        t = svo.VoxelOctree(os.path.join(golden, "build_grid_empty16.oct" if name == "empty" else "build_grid_one16.oct"))
+    plan = svo.svo.TilePlan(w, h)
+    for k, (pitch, yaw) in enumerate([(20.0, 35.0), (20.0, 35.0), (-30.0, 200.0)]):
+        cam = t.orbit_camera(pitch, yaw, 1.0)
+        o = {"hit": torch.full((w * h,), 9, dtype=torch.uint8, device=dev),
+             "normal": torch.zeros(w * h, dtype=torch.int32, device=dev),
+             "t": torch.zeros(w * h, dtype=torch.float32, device=dev),
+             "leaf": torch.zeros(w * h, dtype=torch.int64, device=dev)}
+        print(f"[gpu] planned render {name} {w}x{h} #{k}", flush=True)
+        plan.render_device(t, cam, o["hit"].data_ptr(), o["normal"].data_ptr(), o["t"].data_ptr(), o["leaf"].data_ptr())
+        torch.cuda.synchronize()
+        mine = {"hit": o["hit"].cpu().numpy(), "normal": o["normal"].cpu().numpy().view(np.uint32),
+                "t": o["t"].cpu().numpy(), "leaf": o["leaf"].cpu().numpy().view(np.uint64)}
+        _same(mine, t.render_primary(cam, w, h), f"planned render {name} {k}")
+
+
 def test_cost_ordered_renders_match_plain_kernel(svo, trees):
     """svo_render_primary_planned_device: the first render of a plan (raster
     order), the next ones (tiles in the order the previous render filed them
